@@ -1,0 +1,889 @@
+/*
+ * torch_oracle.c -- CPU FP64 restatement of TORCH's hydro + thermodynamics
+ * step.  TEST INFRASTRUCTURE ONLY (see torch_oracle.h): the HIP product path
+ * never links or calls this file.
+ *
+ * Build: gcc -O2 -ffp-contract=off -fopenmp (no -march, no -ffast-math), so
+ * that every + - * / is an IEEE double operation in source order, as in the
+ * reference's g++ x86-64 build.
+ *
+ * Each function names the reference file:line it restates.  Expressions keep
+ * the reference's operand order and parenthesisation; where the reference
+ * uses Eigen 3.2.1 fixed-size algebra, the order Eigen's unrollers produce is
+ * written out explicitly:
+ *   cross  (a1b2-a2b1, a2b0-a0b2, a0b1-a1b0)   lib/Eigen/src/Geometry/OrthoMethods.h:34-38
+ *   dot / squaredNorm   x0 + (x1 + x2)          lib/Eigen/src/Core/Redux.h:77-106
+ *   mat*vec, mat*mat    (l0r0 + l1r1) + l2r2     lib/Eigen/src/Core/products/CoeffBasedProduct.h:239-257
+ *   normalize           x_i * (1 / norm)        lib/Eigen/src/Core/Dot.h:154-157 (operator/= by a
+ *                       scalar multiplies by its reciprocal in Eigen 3.2)
+ */
+#include "torch_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define DEN ORC_DEN
+#define PRE ORC_PRE
+#define HII ORC_HII
+#define ADV ORC_ADV
+#define VEL ORC_VEL
+#define NV ORC_NV
+
+/* std::min / std::max semantics (NaN handling follows the comparison). */
+static inline double smin(double a, double b) { return (b < a) ? b : a; }
+static inline double smax(double a, double b) { return (a < b) ? b : a; }
+
+/* ------------------------------------------------------------------------ */
+/* Slope limiters: reference src/Integrators/SlopeLimiter.cpp:263-311        */
+/* ------------------------------------------------------------------------ */
+static inline double minmod(double a, double b) {
+    if (a * b > 0) return fabs(a) < fabs(b) ? a : b;
+    return 0;
+}
+static inline double maxmod(double a, double b) {
+    if (a * b > 0) return fabs(a) > fabs(b) ? a : b;
+    return 0;
+}
+
+double orc_limiter(int limiter, double a, double b) {
+    switch (limiter) {
+    case ORC_LIM_MC:       return minmod((a + b) / 2.0, minmod(2.0 * a, 2.0 * b));
+    case ORC_LIM_SUPERBEE: return maxmod(minmod(b, 2.0 * a), minmod(2.0 * b, a));
+    case ORC_LIM_MINMOD:   return minmod(a, b);
+    case ORC_LIM_MAXMOD:   return maxmod(a, b);
+    case ORC_LIM_LEER:
+        if (b == 0) return 0;
+        else { double r = fabs(a / b); return (a + b * r) / (1.0 + r); }
+    case ORC_LIM_OSPRE:
+        if (a * b <= 1.0e-30) return 0;
+        return 1.5 * b * (a * a + a * b) / (a * a + a * b + b * b);
+    default: /* ORC_LIM_ALBADA, SlopeLimiter.cpp:309-311 */
+        return a * b <= 1.0e-30 ? 0 : a * b * (a + b) / (a * a + b * b);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Conversions: reference src/Fluid/GridCell.cpp:163-213 (nd = 3)           */
+/* ------------------------------------------------------------------------ */
+void orc_UfromQ(double u[7], const double q[7], double gamma) {          /* :163-173 */
+    double ke = 0;
+    u[DEN] = q[DEN];
+    for (int dim = 0; dim < 3; ++dim) {
+        u[VEL + dim] = q[VEL + dim] * q[DEN];
+        ke += 0.5 * q[DEN] * q[VEL + dim] * q[VEL + dim];
+    }
+    u[PRE] = q[PRE] / (gamma - 1.0) + ke;
+    u[HII] = q[HII] * q[DEN];
+    u[ADV] = q[ADV] * q[DEN];
+}
+
+void orc_QfromU(double q[7], const double u[7], double gamma) {          /* :175-185 */
+    q[DEN] = u[DEN];
+    double ke = 0;
+    for (int dim = 0; dim < 3; ++dim) {
+        q[VEL + dim] = u[VEL + dim] / u[DEN];
+        ke += 0.5 * u[VEL + dim] * u[VEL + dim] / u[DEN];
+    }
+    q[PRE] = (u[PRE] - ke) * (gamma - 1.0);
+    q[HII] = u[HII] / u[DEN];
+    q[ADV] = u[ADV] / u[DEN];
+}
+
+void orc_FfromQ(double f[7], const double q[7], double gamma, int dim) { /* :200-213 */
+    f[DEN] = q[DEN] * q[VEL + dim];
+    double ke = 0;
+    for (int id = 0; id < 3; ++id) {
+        ke += q[VEL + id] * q[VEL + id];
+        f[VEL + (dim + id) % 3] = q[DEN] * q[VEL + (dim + id) % 3] * q[VEL + dim];
+    }
+    ke *= 0.5 * q[DEN];
+    f[VEL + dim] += q[PRE];
+    double g2 = gamma / (gamma - 1.0);
+    f[PRE] = q[VEL + dim] * (g2 * q[PRE] + ke);
+    f[HII] = q[DEN] * q[VEL + dim] * q[HII];
+    f[ADV] = q[DEN] * q[VEL + dim] * q[ADV];
+}
+
+/* ------------------------------------------------------------------------ */
+/* Riemann solvers: reference src/Integrators/Riemann.cpp                    */
+/* ------------------------------------------------------------------------ */
+static void wave_speeds(double a_l2, double a_r2, const double* Ql, const double* Qr, int dim,
+                        double* S_l, double* S_r) {                        /* :27-42 */
+    double sqrtrho_l = sqrt(Ql[DEN]);
+    double sqrtrho_r = sqrt(Qr[DEN]);
+    double u_tilde = (sqrtrho_l * Ql[VEL + dim] + sqrtrho_r * Qr[VEL + dim]) / (sqrtrho_l + sqrtrho_r);
+    double nu2 = 0.5 * sqrtrho_r * sqrtrho_l / ((sqrtrho_l + sqrtrho_r) * (sqrtrho_l + sqrtrho_r));
+    double dsqrd = (sqrtrho_l * a_l2 + sqrtrho_r * a_r2) / (sqrtrho_l + sqrtrho_r);
+    dsqrd += nu2 * (Qr[VEL + dim] - Ql[VEL + dim]) * (Qr[VEL + dim] - Ql[VEL + dim]);
+    double d = sqrt(dsqrd);
+    *S_l = u_tilde - d;
+    *S_r = u_tilde + d;
+}
+
+static int hllc(double F[7], const double* Ql, const double* Qr, double a_l2, double a_r2,
+                double gamma, int dim) {                                    /* :52-104 */
+    double S_l, S_r;
+    wave_speeds(a_l2, a_r2, Ql, Qr, dim, &S_l, &S_r);
+    if (S_l >= 0) {
+        orc_FfromQ(F, Ql, gamma, dim);
+    } else if (S_r <= 0) {
+        orc_FfromQ(F, Qr, gamma, dim);
+    } else {
+        double S_c = Qr[PRE] - Ql[PRE];
+        S_c += Ql[DEN] * Ql[VEL + dim] * (S_l - Ql[VEL + dim]);
+        S_c -= Qr[DEN] * Qr[VEL + dim] * (S_r - Qr[VEL + dim]);
+        S_c /= (Ql[DEN] * (S_l - Ql[VEL + dim]) - Qr[DEN] * (S_r - Qr[VEL + dim]));
+
+        const double* Q = (S_c >= 0) ? Ql : Qr;
+        double S = (S_c >= 0) ? S_l : S_r;
+        double U[7], Fk[7], Uc[7];
+        orc_UfromQ(U, Q, gamma);
+        orc_FfromQ(Fk, Q, gamma, dim);
+        double A = Q[DEN] * (S - Q[VEL + dim]) / (S - S_c);
+        Uc[DEN] = A;
+        for (int id = 0; id < 3; ++id) Uc[VEL + id] = A * Q[VEL + id];
+        Uc[VEL + dim] = A * S_c;
+        Uc[PRE] = A * ((U[PRE] / Q[DEN]) + (S_c - Q[VEL + dim]) * (S_c + Q[PRE] / (Q[DEN] * (S - Q[VEL + dim]))));
+        Uc[HII] = A * Q[HII];
+        Uc[ADV] = A * Q[ADV];
+        for (int i = 0; i < NV; ++i) F[i] = Fk[i] + S * (Uc[i] - U[i]);
+    }
+    return F[HII] != F[HII];
+}
+
+static int hll(double F[7], const double* Ql, const double* Qr, double a_l2, double a_r2,
+               double gamma, int dim) {                                     /* :108-140 */
+    double S_l, S_r;
+    wave_speeds(a_l2, a_r2, Ql, Qr, dim, &S_l, &S_r);
+    if (S_r <= 0) {
+        orc_FfromQ(F, Qr, gamma, dim);
+    } else if (S_l >= 0) {
+        orc_FfromQ(F, Ql, gamma, dim);
+    } else {
+        double U_l[7], U_r[7], F_l[7], F_r[7];
+        orc_UfromQ(U_l, Ql, gamma);
+        orc_UfromQ(U_r, Qr, gamma);
+        orc_FfromQ(F_l, Ql, gamma, dim);
+        orc_FfromQ(F_r, Qr, gamma, dim);
+        for (int i = 0; i < NV; ++i)
+            F[i] = (S_r * F_l[i] - S_l * F_r[i] + S_l * S_r * (U_r[i] - U_l[i])) / (S_r - S_l);
+    }
+    return F[HII] != F[HII];
+}
+
+/* Eigen-ordered 3-vector helpers. */
+static inline void cross3(const double a[3], const double b[3], double r[3]) {
+    r[0] = a[1] * b[2] - a[2] * b[1];
+    r[1] = a[2] * b[0] - a[0] * b[2];
+    r[2] = a[0] * b[1] - a[1] * b[0];
+}
+static inline double dot3(const double a[3], const double b[3]) {
+    return a[0] * b[0] + (a[1] * b[1] + a[2] * b[2]);
+}
+static inline double norm3(const double a[3]) { return sqrt(dot3(a, a)); }
+
+/* getRotationFromA2B: Riemann.cpp:161-171. R = (I + vx) + (k*vx)*vx. */
+static void rot_a2b(const double A[3], const double B[3], double R[3][3]) {
+    double v[3];
+    cross3(A, B, v);
+    double s = norm3(v);
+    double c = dot3(A, B);
+    double vx[3][3] = {{0, -v[2], v[1]}, {v[2], 0, -v[0]}, {-v[1], v[0], 0}};
+    double k = (1.0 - c) / (s * s);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double p = (k * vx[i][0]) * vx[0][j];
+            p += (k * vx[i][1]) * vx[1][j];
+            p += (k * vx[i][2]) * vx[2][j];
+            R[i][j] = ((i == j ? 1.0 : 0.0) + vx[i][j]) + p;
+        }
+}
+
+/* rotate: Riemann.cpp:154-159, vel <- R*vel. */
+static void rotate(const double R[3][3], double* v) {
+    double a = v[VEL + 0], b = v[VEL + 1], c = v[VEL + 2];
+    for (int i = 0; i < 3; ++i) {
+        double r = R[i][0] * a;
+        r += R[i][1] * b;
+        r += R[i][2] * c;
+        v[VEL + i] = r;
+    }
+}
+
+static int rotated_hllc(double F[7], const double* Ql, const double* Qr, double a_l2, double a_r2,
+                        double gamma, int dim) {                            /* :181-244 */
+    int pureHLLC = 0, pureHLL = 0;
+    double d[3] = {dim == 0 ? 1.0 : 0.0, dim == 1 ? 1.0 : 0.0, dim == 2 ? 1.0 : 0.0};
+    double n1[3] = {Qr[VEL + 0] - Ql[VEL + 0], Qr[VEL + 1] - Ql[VEL + 1], Qr[VEL + 2] - Ql[VEL + 2]};
+    double n2[3] = {0, 0, 0};
+    double t[3];
+
+    if (n1[dim] < 0)
+        for (int i = 0; i < 3; ++i) n1[i] = -1.0 * n1[i];
+    if (norm3(n1) < 1.0e-6) {
+        pureHLLC = 1;
+    } else {
+        double inv = 1.0 / norm3(n1);
+        for (int i = 0; i < 3; ++i) n1[i] *= inv;
+        cross3(n1, d, t);
+        if (norm3(t) == 0) {
+            pureHLL = 1;
+        } else {
+            cross3(t, n1, n2);
+            if (norm3(n2) == 0) {
+                pureHLL = 1;
+            } else {
+                if (n2[dim] < 0)
+                    for (int i = 0; i < 3; ++i) n2[i] = -1.0 * n2[i];
+                double inv2 = 1.0 / norm3(n2);
+                for (int i = 0; i < 3; ++i) n2[i] *= inv2;
+                cross3(n2, d, t);
+                if (norm3(t) == 0) pureHLLC = 1;
+            }
+        }
+    }
+
+    int bad = 0;
+    if (!pureHLLC && !pureHLL) {
+        double F1[7], F2[7], QlR[7], QrR[7], R[3][3];
+        memcpy(QlR, Ql, sizeof QlR);
+        memcpy(QrR, Qr, sizeof QrR);
+        rot_a2b(n1, d, R);
+        rotate(R, QlR);
+        rotate(R, QrR);
+        bad |= hll(F1, QlR, QrR, a_l2, a_r2, gamma, dim);
+        rot_a2b(d, n1, R);
+        rotate(R, QlR);
+        rotate(R, QrR);
+        rotate(R, F1);
+        rot_a2b(n2, d, R);
+        rotate(R, QlR);
+        rotate(R, QrR);
+        bad |= hllc(F2, QlR, QrR, a_l2, a_r2, gamma, dim);
+        rot_a2b(d, n2, R);
+        rotate(R, F2);
+        double w1 = fabs(dot3(d, n1)), w2 = fabs(dot3(d, n2));
+        for (int iu = 0; iu < NV; ++iu) F[iu] = w1 * F1[iu] + w2 * F2[iu];
+    } else if (pureHLL) {
+        bad |= hll(F, Ql, Qr, a_l2, a_r2, gamma, dim);
+    } else {
+        bad |= hllc(F, Ql, Qr, a_l2, a_r2, gamma, dim);
+    }
+    return bad;
+}
+
+int orc_riemann(int solver, double F[7], const double Ql[7], const double Qr[7],
+                double a_l2, double a_r2, double gamma, int dim) {
+    switch (solver) {
+    case ORC_HLL:  return hll(F, Ql, Qr, a_l2, a_r2, gamma, dim);
+    case ORC_HLLC: return hllc(F, Ql, Qr, a_l2, a_r2, gamma, dim);
+    default:       return rotated_hllc(F, Ql, Qr, a_l2, a_r2, gamma, dim);
+    }
+}
+
+void orc_riemann_batch(int solver, int n, const double* Ql, const double* Qr, const int* dims,
+                       double gamma, double* F, int* status) {
+    for (int i = 0; i < n; ++i) {
+        const double* l = Ql + 7 * i;
+        const double* r = Qr + 7 * i;
+        /* sound speeds as Hydro.cpp:69-71,101-102 */
+        double a_l2 = gamma * l[PRE] / l[DEN];
+        double a_r2 = gamma * r[PRE] / r[DEN];
+        status[i] = orc_riemann(solver, F + 7 * i, l, r, a_l2, a_r2, gamma, dims[i]);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Whole-grid operations: reference src/Fluid/Fluid.cpp                      */
+/* ------------------------------------------------------------------------ */
+static inline int64_t ncells(const orc_hydro_params* p) {
+    return (int64_t)p->nx * p->ny * p->nz;
+}
+
+static void set_threads(const orc_hydro_params* p) {
+#ifdef _OPENMP
+    omp_set_num_threads(p->nthreads > 0 ? p->nthreads : 1);
+#else
+    (void)p;
+#endif
+}
+
+void orc_globalQfromU(const orc_hydro_params* p, const double* U, double* Q) {  /* :162-165 */
+    const int64_t n = ncells(p);
+    set_threads(p);
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < n; ++c) {
+        double u[7], q[7];
+        for (int v = 0; v < NV; ++v) u[v] = U[v * n + c];
+        orc_QfromU(q, u, p->gamma);
+        for (int v = 0; v < NV; ++v) Q[v * n + c] = q[v];
+    }
+}
+
+void orc_fixPrimitives(const orc_hydro_params* p, double* Q, const double* U) {  /* :138-150 */
+    const int64_t n = ncells(p);
+    const double X = p->mass_frac_h;
+    set_threads(p);
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < n; ++c) {
+        Q[HII * n + c] = smax(smin(Q[HII * n + c], 1.0), 0.0);
+        Q[ADV * n + c] = smax(smin(Q[ADV * n + c], 1.0), 0.0);
+        Q[DEN * n + c] = smax(Q[DEN * n + c], p->dfloor);
+        Q[PRE * n + c] = smax(Q[PRE * n + c], p->pfloor);
+        double mu_inv = X * (Q[HII * n + c] + 1.0) + (1.0 - X) * 0.25;
+        double temperature = Q[PRE * n + c] / (mu_inv * p->spec_gas_const * Q[DEN * n + c]);
+        if (temperature < p->tfloor)
+            Q[PRE * n + c] = mu_inv * p->spec_gas_const * U[DEN * n + c] * p->tfloor;
+    }
+}
+
+/* Fluid::fixSolution, Fluid.cpp:89-136.  Returns the number of cells that
+ * make the reference throw (non-finite input, zero or non-finite output). */
+int orc_fixSolution(const orc_hydro_params* p, double* U) {
+    const int64_t n = ncells(p);
+    const double X = p->mass_frac_h, g = p->gamma;
+    int bad = 0;
+    set_threads(p);
+#pragma omp parallel for schedule(static) reduction(+ : bad)
+    for (int64_t c = 0; c < n; ++c) {
+        double u[7];
+        for (int v = 0; v < NV; ++v) u[v] = U[v * n + c];
+        if (!isfinite(u[DEN]) || !isfinite(u[PRE])) { bad += 1; continue; }
+        double hii = smax(smin(u[HII] / u[DEN], 1.0), 0.0);
+        double adv = smax(smin(u[ADV] / u[DEN], 1.0), 0.0);
+        double v3[3];
+        double den = smax(u[DEN], p->dfloor);
+        for (int dim = 0; dim < 3; ++dim) v3[dim] = u[VEL + dim] / u[DEN];
+        double ke = 0.0;
+        for (int dim = 0; dim < 3; ++dim) ke += v3[dim] * v3[dim];
+        ke *= 0.5 * u[DEN];
+        double pre = (u[PRE] - ke) * (g - 1.0);
+        ke *= den / u[DEN];
+        if (pre < p->pfloor) pre = p->pfloor;
+        double mu_inv = X * (hii + 1.0) + (1.0 - X) * 0.25;
+        double temperature = pre / (mu_inv * p->spec_gas_const * den);
+        if (temperature < p->tfloor) pre = mu_inv * p->spec_gas_const * den * p->tfloor;
+        u[DEN] = den;
+        u[PRE] = pre / (g - 1.0) + ke;
+        u[HII] = hii * den;
+        u[ADV] = adv * den;
+        for (int dim = 0; dim < 3; ++dim) u[VEL + dim] = den * v3[dim];
+        int fail = (u[DEN] == 0 || u[PRE] == 0);
+        for (int v = 0; v < NV; ++v)
+            if (u[v] != u[v] || isinf(u[v])) fail = 1;
+        bad += fail;
+        for (int v = 0; v < NV; ++v) U[v * n + c] = u[v];
+    }
+    return bad;
+}
+
+/* Hydrodynamics::calculateTimeStep, Hydro.cpp:77-89 (CFL literal 0.5). */
+double orc_hydro_timestep(const orc_hydro_params* p, const double* Q) {
+    const int64_t n = ncells(p);
+    double tmin = p->dt_max;
+    set_threads(p);
+#pragma omp parallel
+    {
+        double local = p->dt_max;
+#pragma omp for schedule(static)
+        for (int64_t c = 0; c < n; ++c) {
+            double inv_t = 0;
+            double ss = sqrt(p->gamma * Q[PRE * n + c] / Q[DEN * n + c]);
+            for (int dim = 0; dim < 3; ++dim)
+                inv_t += (fabs(Q[(VEL + dim) * n + c]) + ss) / p->dx;
+            if (inv_t == 0) inv_t = 1.0 / p->dt_max;
+            local = smin(local, 0.5 / inv_t);
+        }
+#pragma omp critical
+        tmin = smin(tmin, local);
+    }
+    return smin(tmin, p->dt_max);
+}
+
+/* Ghost fill for one line along a dimension: Grid::applyBCs, Grid.cpp:649-802.
+ * line[g + i][v] holds cell i for i in [-ng, n+ng). */
+static void fill_ghosts(double (*line)[7], int n, int ng, int dim, int bcl, int bcr) {
+    const int o = ng; /* offset of cell 0 */
+    for (int L = 1; L <= ng; ++L) {
+        double* gh = line[o - L];
+        switch (bcl) {
+        case ORC_BC_PERIODIC:  memcpy(gh, line[o + n - L], sizeof(double) * 7); break;
+        case ORC_BC_REFLECTING:
+            memcpy(gh, line[o + L], sizeof(double) * 7);   /* Grid.cpp:765-779 */
+            gh[VEL + dim] = -line[o + L][VEL + dim];
+            break;
+        default:
+            memcpy(gh, line[o], sizeof(double) * 7);
+            if (bcl == ORC_BC_OUTFLOW && line[o][VEL + dim] > 0) gh[VEL + dim] = -1.0 * line[o][VEL + dim];
+            if (bcl == ORC_BC_INFLOW && line[o][VEL + dim] < 0) gh[VEL + dim] = -1.0 * line[o][VEL + dim];
+        }
+    }
+    for (int L = 1; L <= ng; ++L) {
+        double* gh = line[o + n - 1 + L];
+        switch (bcr) {
+        case ORC_BC_PERIODIC:  memcpy(gh, line[o + L - 1], sizeof(double) * 7); break;
+        case ORC_BC_REFLECTING:
+            memcpy(gh, line[o + n - 1 - L], sizeof(double) * 7);
+            gh[VEL + dim] = -line[o + n - 1 - L][VEL + dim];
+            break;
+        default:
+            memcpy(gh, line[o + n - 1], sizeof(double) * 7);
+            if (bcr == ORC_BC_OUTFLOW && line[o + n - 1][VEL + dim] < 0) gh[VEL + dim] = -1.0 * line[o + n - 1][VEL + dim];
+            if (bcr == ORC_BC_INFLOW && line[o + n - 1][VEL + dim] > 0) gh[VEL + dim] = -1.0 * line[o + n - 1][VEL + dim];
+        }
+    }
+}
+
+/* Hydrodynamics::integrate + updateSourceTerms (Hydro.cpp:31-34,44-67,91-156)
+ * for one state Q.  UDOT is overwritten (the reference's UDOT is zero on
+ * entry: advSolution zeroes it, Fluid.cpp:80-87). */
+int orc_hydro_rhs(const orc_hydro_params* p, const double* Q, double* UDOT) {
+    const int64_t n = ncells(p);
+    const int N[3] = {p->nx, p->ny, p->nz};
+    const int64_t stride[3] = {1, p->nx, (int64_t)p->nx * p->ny};
+    const int ng = p->spatial_order + 1;
+    const double g = p->gamma;
+    /* Cartesian geometry, Grid.cpp:846-907: area = 1*dx*dx, vol = 1*dx*dx*dx */
+    const double area = 1.0 * p->dx * p->dx;
+    const double vol = 1.0 * p->dx * p->dx * p->dx;
+    int bad = 0;
+
+    memset(UDOT, 0, sizeof(double) * 7 * n);
+    set_threads(p);
+    for (int dim = 0; dim < 3; ++dim) {
+        const int nd = N[dim];
+        const int o1 = (dim + 1) % 3, o2 = (dim + 2) % 3;
+        const int64_t nlines = (int64_t)N[o1] * N[o2];
+#pragma omp parallel reduction(| : bad)
+        {
+            double (*line)[7] = malloc(sizeof(double) * 7 * (nd + 2 * ng));
+            double (*QL)[7] = malloc(sizeof(double) * 7 * (nd + 2));
+            double (*QR)[7] = malloc(sizeof(double) * 7 * (nd + 2));
+            double (*F)[7] = malloc(sizeof(double) * 7 * (nd + 1));
+#pragma omp for schedule(static)
+            for (int64_t li = 0; li < nlines; ++li) {
+                int a = (int)(li % N[o1]), b = (int)(li / N[o1]);
+                int64_t base = a * stride[o1] + b * stride[o2];
+                for (int i = 0; i < nd; ++i)
+                    for (int v = 0; v < NV; ++v) line[ng + i][v] = Q[v * n + base + i * stride[dim]];
+                fill_ghosts(line, nd, ng, dim, p->bc_left[dim], p->bc_right[dim]);
+                /* reconstruct cells -1..nd (ghost layer 1 included), Hydro.cpp:52-67 */
+                for (int i = -1; i <= nd; ++i) {
+                    const double* qc = line[ng + i];
+                    for (int v = 0; v < NV; ++v) {
+                        if (p->spatial_order == 1) {
+                            const double* ql = line[ng + i - 1];
+                            const double* qr = line[ng + i + 1];
+                            double dQdr = orc_limiter(p->limiter, qc[v] - ql[v], qr[v] - qc[v]);
+                            QL[i + 1][v] = qc[v] - 0.5 * dQdr;
+                            QR[i + 1][v] = qc[v] + 0.5 * dQdr;
+                        } else {
+                            QL[i + 1][v] = qc[v];
+                            QR[i + 1][v] = qc[v];
+                        }
+                    }
+                }
+                /* face f between cell f-1 and cell f, Hydro.cpp:96-113 */
+                for (int f = 0; f <= nd; ++f) {
+                    const double* l = QR[f];      /* cell f-1 */
+                    const double* r = QL[f + 1];  /* cell f   */
+                    double a_l2 = g * l[PRE] / l[DEN];
+                    double a_r2 = g * r[PRE] / r[DEN];
+                    bad |= orc_riemann(p->solver, F[f], l, r, a_l2, a_r2, g, dim);
+                }
+                /* flux divergence, Hydro.cpp:125-131 */
+                for (int i = 0; i < nd; ++i) {
+                    int64_t c = base + i * stride[dim];
+                    for (int v = 0; v < NV; ++v) {
+                        double u = UDOT[v * n + c];
+                        u += (area / vol) * F[i][v];
+                        u -= (area / vol) * F[i + 1][v];
+                        UDOT[v * n + c] = u;
+                    }
+                }
+            }
+            free(line); free(QL); free(QR); free(F);
+        }
+    }
+    /* gravity source with GRAV = 0, Hydro.cpp:134-137 */
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < n; ++c) {
+        for (int i = 0; i < 3; ++i) {
+            const double grav = 0.0;
+            UDOT[(VEL + i) * n + c] += grav;
+            UDOT[PRE * n + c] += Q[(VEL + i) * n + c] * grav;
+        }
+    }
+    return bad;
+}
+
+static void adv_solution(const orc_hydro_params* p, double* U, double* UDOT, double dt) {
+    const int64_t n = ncells(p);                                            /* Fluid.cpp:80-87 */
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < n; ++c)
+        for (int v = 0; v < NV; ++v) {
+            U[v * n + c] += dt * UDOT[v * n + c];
+            UDOT[v * n + c] = 0;
+        }
+}
+
+/* Torch::hydroStep(dt, true), Torch.cpp:366-387. */
+int orc_hydro_step(const orc_hydro_params* p, double* U, double* Q, double* W, double* UDOT, double dt) {
+    const int64_t n = ncells(p);
+    int err = 0;
+    memcpy(W, U, sizeof(double) * 7 * n);                      /* globalWfromU */
+    if (orc_hydro_rhs(p, Q, UDOT)) err = 1;                      /* integrate + updateSourceTerms */
+    adv_solution(p, U, UDOT, dt / 2.0);
+    if (orc_fixSolution(p, U) && !err) err = 2;
+    orc_globalQfromU(p, U, Q);                                   /* corrector */
+    memcpy(U, W, sizeof(double) * 7 * n);                        /* globalUfromW */
+    if (orc_hydro_rhs(p, Q, UDOT) && !err) err = 1;
+    adv_solution(p, U, UDOT, dt);
+    if (orc_fixSolution(p, U) && !err) err = 2;
+    return err;
+}
+
+/* Checkpointer::update, src/IO/Checkpointer.cpp:14-26. */
+static double checkpoint_dt(double t, double tmax, int ncheck, int* count) {
+    double delta = 100.0 / ncheck;
+    double pct = smin(100.0, 100.0 * t / tmax);
+    while (pct - *count * delta > delta - 1.0e-8) *count += 1;
+    return ((*count + 1) * delta - pct) * tmax / 100.0;
+}
+
+/* Torch::run / fullStep / calculateTimeStep for a hydro-only run,
+ * Torch.cpp:240-302, 304-339, 389-404. */
+int orc_run_hydro(const orc_hydro_params* p, double* U, int nsteps, double tmax, int ncheckpoints,
+                  double* dts, double* time_out) {
+    const int64_t n = ncells(p);
+    double* Q = malloc(sizeof(double) * 7 * n);
+    double* W = malloc(sizeof(double) * 7 * n);
+    double* UDOT = calloc(7 * n, sizeof(double));
+    double t = 0;
+    int count = 0, err = 0;
+    (void)checkpoint_dt(t, tmax, ncheckpoints, &count);
+    for (int s = 0; s < nsteps && t < tmax; ++s) {
+        double dt_next = checkpoint_dt(t, tmax, ncheckpoints, &count);
+        orc_globalQfromU(p, U, Q);
+        orc_fixPrimitives(p, Q, U);
+        /* Torch::checkValues("hydro before"), Torch.cpp:420-443 */
+        int bad = 0;
+        for (int64_t c = 0; c < n && !bad; ++c) {
+            if (Q[DEN * n + c] == 0 || Q[PRE * n + c] == 0) bad = 1;
+            for (int v = 0; v < NV; ++v)
+                if (U[v * n + c] != U[v * n + c] || isinf(U[v * n + c])) bad = 1;
+        }
+        if (bad) { err = 3; break; }
+        double dt = (s == 0) ? p->dt_max * 1.0e-20 : orc_hydro_timestep(p, Q);
+        dt = smin(dt_next, dt);
+        err = orc_hydro_step(p, U, Q, W, UDOT, dt);
+        if (dts) dts[s] = dt;
+        t += dt;
+        if (err) break;
+    }
+    if (time_out) *time_out = t;
+    free(Q); free(W); free(UDOT);
+    return err;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Units: reference src/Torch/Converter.cpp:17-63                            */
+/* ------------------------------------------------------------------------ */
+void orc_converter_set_rho_pressure_time(orc_converter* c, double rho, double pressure, double time) {
+    /* defaults from the Converter constructor, Converter.cpp:7-9 */
+    c->M = 1.0 * 2.0e33;
+    c->L = 14.6 * 3.09e18;
+    c->T = 10000.0 * 3.15569e7;
+    double rho_0 = (rho > 0) ? rho : c->M / (c->L * c->L * c->L);
+    double pressure_0 = (pressure > 0) ? pressure : rho_0 * c->L * c->L / (c->T * c->T);
+    double time_0 = (time > 0) ? time : c->T;
+    c->T = time_0;
+    c->M = sqrt((pressure_0 / rho_0) * pressure_0 * pressure_0) * c->T * c->T * c->T;
+    c->L = sqrt(pressure_0 / rho_0) * c->T;
+}
+
+static double convert_units(const orc_converter* c, double val, double mi, double li, double ti, int to_code) {
+    double MM, LL, TT;
+    MM = mi >= 0 ? c->M : 1.0 / c->M;
+    MM = to_code ? 1.0 / MM : MM;
+    LL = li >= 0 ? c->L : 1.0 / c->L;
+    LL = to_code ? 1.0 / LL : LL;
+    TT = ti >= 0 ? c->T : 1.0 / c->T;
+    TT = to_code ? 1.0 / TT : TT;
+    int im = (int)fabs(mi), il = (int)fabs(li), it = (int)fabs(ti);
+    double result = val;
+    while (im >= 1 || il >= 1 || it >= 1) {
+        if (im-- >= 1) result *= MM;
+        if (il-- >= 1) result *= LL;
+        if (it-- >= 1) result *= TT;
+    }
+    double left = fabs(mi) - (int)fabs(mi);
+    if (left != 0) result *= pow(MM, left);
+    left = fabs(li) - (int)fabs(li);
+    if (left != 0) result *= pow(LL, left);
+    left = fabs(ti) - (int)fabs(ti);
+    if (left != 0) result *= pow(TT, left);
+    return result;
+}
+double orc_to_code_units(const orc_converter* c, double v, double mi, double li, double ti) {
+    return convert_units(c, v, mi, li, ti, 1);
+}
+double orc_from_code_units(const orc_converter* c, double v, double mi, double li, double ti) {
+    return convert_units(c, v, mi, li, ti, 0);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Thermodynamics: reference src/Integrators/Thermodynamics.cpp              */
+/* ------------------------------------------------------------------------ */
+
+/* Natural cubic spline second derivatives, SplineData.cpp:15-55. */
+static void spline_y2(const double* x, const double* f, int n, double* y2) {
+    double u[64];
+    y2[0] = 0.0;
+    u[0] = 0.0;
+    for (int i = 1; i < n - 1; i++) {
+        double sig = (x[i] - x[i - 1]) / (x[i + 1] - x[i - 1]);
+        double p = sig * y2[i - 1] + 2.0;
+        y2[i] = (sig - 1.0) / p;
+        u[i] = (f[i + 1] - f[i]) / (x[i + 1] - x[i]) - (f[i] - f[i - 1]) / (x[i] - x[i - 1]);
+        u[i] = (6.0 * u[i] / (x[i + 1] - x[i - 1]) - sig * u[i - 1]) / p;
+    }
+    double qn = 0.0, un = 0.0;
+    y2[n - 1] = (un - qn * u[n - 2]) / (qn * y2[n - 2] + 1.0);
+    for (int k = n - 2; k >= 0; k--) y2[k] = y2[k] * y2[k + 1] + u[k];
+}
+
+/* SplineData::splint, SplineData.cpp:57-81. */
+static double splint(const double* x, const double* f, const double* f2, int n, double x2) {
+    int klo = 0, khi = n - 1;
+    while (khi - klo > 1) {
+        int k = (khi + klo) >> 1;
+        if (x[k] > x2) khi = k;
+        else klo = k;
+    }
+    double h = x[khi] - x[klo];
+    double a = (x[khi] - x2) / h;
+    double b = (x2 - x[klo]) / h;
+    return a * f[klo] + b * f[khi] + ((a * a * a - a) * f2[klo] + (b * b * b - b) * f2[khi]) * (h * h) / 6.0;
+}
+
+/* LogSplineData::interpolate, SplineData.cpp:142-151. */
+double orc_log_spline(const orc_thermo_params* t, double x) {
+    if (x > t->spl_x[25]) return t->spl_y[25] + t->spl_max_slope * (x - t->spl_x[25]);
+    if (x < t->spl_x[0]) return t->spl_y[0] + t->spl_min_slope * (x - t->spl_x[0]);
+    return splint(t->spl_x, t->spl_y, t->spl_y2, 26, x);
+}
+
+/* Thermodynamics::initialise + initCollisionalExcitationHI, :24-80;
+ * Constants::initialise, Constants.cpp:27-40. */
+void orc_thermo_init(orc_thermo_params* t, const orc_converter* c, double thermo_hii_switch,
+                     double heating_amplification, double mass_frac_h, double gamma, double pfloor,
+                     int subcycling) {
+    memset(t, 0, sizeof *t);
+    t->hydrogen_mass = orc_to_code_units(c, 1.674e-24, 1, 0, 0);
+    t->spec_gas_const = orc_to_code_units(c, 8.314462e7, 0, 2, -2);
+    t->boltzmann = orc_to_code_units(c, 1.3806488e-16, 1, 2, -2);
+    t->dust_xsec = orc_to_code_units(c, 5.0e-22, 0, 2, 0);
+    t->pi = 3.14159265359;
+    t->z0 = 5.0e-4;
+    t->T1 = 33610; t->T2 = 2180; t->T3 = 28390; t->T4 = 1780;
+    t->imlc = orc_to_code_units(c, 2.905e-19, 1.0, 5.0, -3.0);
+    t->nmlc = orc_to_code_units(c, 4.477e-20, 1.0, 5.0, -3.0);
+    t->ciec_minT = 5.0e4;
+    t->ciec = orc_to_code_units(c, 3.485e-15, 1.0, 5.0, -3.0);
+    t->cxhi_damp = 5.0e5;
+    t->n0 = orc_to_code_units(c, 1.0e6, 0.0, -3.0, 0.0);
+    t->nmc = orc_to_code_units(c, 3.981e-27, 1.0, 3.8, -3.0);
+    t->fuvh_a = orc_to_code_units(c, 1.9e-26, 1.0, 4.0, -2.0);
+    t->fuvh_b = orc_to_code_units(c, 1.00000, 0.0, 0.0, -1.0);
+    t->fuvh_c = orc_to_code_units(c, 6.40000, 0.0, -1.0, 0.0);
+    t->irh_a = orc_to_code_units(c, 7.7e-32, 1.0, 4.0, -2.0);
+    t->irh_b = orc_to_code_units(c, 3.0e4, 0.0, -3.0, 0.0);
+    t->crh = orc_to_code_units(c, 5.0e-27, 1.0, 2.0, -3.0);
+    t->T_min_default = 100;
+
+    static const double T[26] = {3162.2776602, 3981.0717055, 5011.8723363, 6309.5734448, 7943.2823472,
+        10000.0000000, 12589.2541179, 15848.9319246, 19952.6231497, 25118.8643151,
+        31622.7766017, 39810.7170553, 50118.7233627, 63095.7344480, 79432.8234724,
+        100000.0000000, 125892.5411794, 158489.3192461, 199526.2314969, 251188.6431510,
+        316227.7660168, 398107.1705535, 501187.2336273, 630957.3444802, 794328.2347243,
+        1000000.0000000};
+    static const double R[26] = {1.150800e-34, 2.312065e-31, 9.571941e-29, 1.132400e-26, 4.954502e-25,
+        9.794900e-24, 1.035142e-22, 6.652732e-22, 2.870781e-21, 9.036495e-21, 2.218196e-20,
+        4.456562e-20, 7.655966e-20, 1.158777e-19, 1.588547e-19, 2.013724e-19, 2.393316e-19,
+        2.710192e-19, 2.944422e-19, 3.104560e-19, 3.191538e-19, 3.213661e-19, 3.191538e-19,
+        3.126079e-19, 3.033891e-19, 2.917427e-19};
+    for (int i = 0; i < 26; ++i) {
+        t->spl_x[i] = log10(T[i]);
+        t->spl_y[i] = log10(orc_to_code_units(c, R[i], 1, 5, -3));
+    }
+    spline_y2(t->spl_x, t->spl_y, 26, t->spl_y2);
+    t->spl_min_slope = (t->spl_y[1] - t->spl_y[0]) / (t->spl_x[1] - t->spl_x[0]);
+    t->spl_max_slope = (t->spl_y[25] - t->spl_y[24]) / (t->spl_x[25] - t->spl_x[24]);
+
+    t->thermo_hii_switch = thermo_hii_switch;
+    t->heating_amplification = heating_amplification;
+    t->mass_frac_h = mass_frac_h;
+    t->gamma = gamma;
+    t->pfloor = pfloor;
+    t->subcycling = subcycling;
+}
+
+/* Rate functions, Thermodynamics.cpp:113-203. */
+static inline double imlc(const orc_thermo_params* t, double ne, double T) {
+    return t->imlc * t->z0 * ne * ne * exp(-t->T1 / T - (t->T2 / T) * (t->T2 / T));
+}
+static inline double nmlc(const orc_thermo_params* t, double ne, double nn, double T) {
+    return t->nmlc * t->z0 * ne * nn * exp(-t->T3 / T - (t->T4 / T) * (t->T4 / T));
+}
+static inline double ciec(const orc_thermo_params* t, double ne, double T) {
+    if (T > t->ciec_minT) {
+        double cie_rate = t->ciec * ne * ne * t->z0 * exp(-0.63 * log(T)) * (1.0 - exp(-pow(1.0e-5 * T, 1.63)));
+        double smoothing = smin(1.0, (T - 5.0e4) / (2.0e4));
+        return cie_rate * smoothing;
+    }
+    return 0;
+}
+static inline double nmc(const orc_thermo_params* t, double nH, double x, double T) {
+    double T0 = 70.0 + 220.0 * pow(nH / t->n0, 0.2);
+    return t->nmc * (1.0 - x) * (1.0 - x) * pow(nH, 1.6) * sqrt(T) * exp(-T0 / T);
+}
+static inline double cehi(const orc_thermo_params* t, double nH, double x, double T) {
+    double rate = orc_log_spline(t, log10(T));
+    return x * (1.0 - x) * nH * nH * exp((2.302585093 * rate) - ((T / t->cxhi_damp) * (T / t->cxhi_damp)));
+}
+static inline double fuvh(const orc_thermo_params* t, double nH, double Av, double F) {
+    return t->fuvh_a * nH * F * exp(-1.9 * Av) / (t->fuvh_b + t->fuvh_c * F * exp(-1.9 * Av) / nH);
+}
+static inline double irh(const orc_thermo_params* t, double nH, double Av, double F) {
+    return t->irh_a * nH * F * exp(-0.05 * Av) * exp(-2.0 * log(1.0 + t->irh_b / nH));
+}
+static inline double crh(const orc_thermo_params* t, double nH) { return t->crh * nH; }
+static inline double soft_landing(double rate, double T, double T_min) {
+    double result = rate;
+    if (rate < 0.0) {
+        if (T <= T_min) result = 0;
+        else if (T <= T_min + 200) result = rate * (T - T_min) / 200;
+    }
+    return result;
+}
+
+void orc_thermo_rates(const orc_thermo_params* t, double nH, double x, double T, double Av, double F,
+                      double out[9]) {
+    double ne = nH * (x), nn = nH * (1.0 - x);
+    out[0] = fuvh(t, nH, Av, F);
+    out[1] = irh(t, nH, Av, F);
+    out[2] = crh(t, nH);
+    out[3] = imlc(t, ne, T);
+    out[4] = nmlc(t, ne, nn, T);
+    out[5] = cehi(t, nH, x, T);
+    out[6] = ciec(t, ne, T);
+    out[7] = nmc(t, nH, x, T);
+    out[8] = orc_log_spline(t, log10(T));
+}
+
+/* preTimeStepCalculations (:218-264) followed by integrate (:266-337) for one
+ * non-wind cell. */
+void orc_thermo_cell(const orc_thermo_params* t, double den, double pre, double hii, double adv,
+                     double E, double T_min, double col_den, double F_FUV, double dt, double* heat_out,
+                     double* rate_pre_out, double* rate_out, int64_t* nsub_out, double* p_sub_out) {
+    const double X = t->mass_frac_h;
+    *nsub_out = 0;
+    *p_sub_out = pre;
+    *heat_out = 0;
+    if (adv < t->thermo_hii_switch) {
+        *rate_pre_out = 0;
+        *rate_out = 0;
+        return;
+    }
+    double nH = X * den / t->hydrogen_mass;
+    double HIIFRAC = hii;
+    double ne = nH * (HIIFRAC);
+    double nn = nH * (1.0 - HIIFRAC);
+    double mu_inv0 = X * (hii + 1.0) + (1.0 - X) * 0.25;              /* Fluid.cpp:71-74 */
+    double T = (pre / den) * (1.0 / mu_inv0) / t->spec_gas_const;
+    double Av = 1.086 * t->dust_xsec * col_den;
+
+    double rate = 0;
+    rate += fuvh(t, nH, Av, F_FUV);
+    rate += irh(t, nH, Av, F_FUV);
+    rate += crh(t, nH);
+    double heat = rate;
+    rate -= imlc(t, ne, T);
+    rate -= nmlc(t, ne, nn, T);
+    rate -= cehi(t, nH, HIIFRAC, T);
+    rate -= ciec(t, ne, T);
+    rate -= nmc(t, nH, HIIFRAC, T);
+    rate = soft_landing(rate, T, T_min);
+    double RATE = t->heating_amplification * rate;
+    *heat_out = heat;
+    *rate_pre_out = RATE;
+    if (!t->subcycling) {
+        *rate_out = RATE;
+        return;
+    }
+
+    double dti = fabs(0.10 * E / RATE);
+    double mu_inv = X * (hii + 1.0) + (1.0 - X) * 0.25;
+    double pre2temp = 1.0 / (mu_inv * t->spec_gas_const * den);
+    double temp2pre = (mu_inv * t->spec_gas_const * den);
+    double rate2dpre = smin(dt, dti) * (t->gamma - 1.0);
+    double dpre2rate = 1.0 / rate2dpre;
+
+    double pressure = pre + RATE * rate2dpre;
+    double subcycleT = pressure * pre2temp;
+    int64_t nsub = 0;
+    if (pressure < t->pfloor || subcycleT < T_min) {
+        double pf = smax(T_min * temp2pre, t->pfloor);
+        subcycleT = pf * pre2temp;
+        pressure = pf;
+    }
+    if (dt > dti) {
+        double dtdti = dt / dti;
+        /* (int) conversions as in :307; dtdti beyond INT_MAX is rejected by
+         * callers (the reference's int conversion is undefined there). */
+        int nsteps = dtdti - (int)dtdti > 0 ? (int)(dtdti + 1.0) : (int)(dtdti + 0.5);
+        --nsteps;
+        for (int i = 0; i < nsteps; ++i) {
+            double subcycleRate = heat;
+            subcycleRate -= imlc(t, ne, subcycleT);
+            subcycleRate -= nmlc(t, ne, nn, subcycleT);
+            subcycleRate -= cehi(t, nH, HIIFRAC, subcycleT);
+            subcycleRate -= ciec(t, ne, subcycleT);
+            subcycleRate -= nmc(t, nH, HIIFRAC, subcycleT);
+            subcycleRate = t->heating_amplification * soft_landing(subcycleRate, subcycleT, T_min);
+            pressure += subcycleRate * rate2dpre;
+            subcycleT = pressure * pre2temp;
+            if (pressure < t->pfloor || subcycleT < T_min) {
+                double pf = smax(T_min * temp2pre, t->pfloor);
+                subcycleT = pf * pre2temp;
+                pressure = pf;
+            }
+        }
+        nsub = nsteps;
+    }
+    *rate_out = (pressure - pre) * dpre2rate;
+    *nsub_out = nsub;
+    *p_sub_out = pressure;
+}
+
+void orc_thermo_batch(const orc_thermo_params* t, int64_t n, const double* in, double* out, int nthreads) {
+#ifdef _OPENMP
+    omp_set_num_threads(nthreads > 0 ? nthreads : 1);
+#else
+    (void)nthreads;
+#endif
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < n; ++i) {
+        const double* r = in + 9 * i;
+        double heat, rp, rate, ps;
+        int64_t nsub;
+        orc_thermo_cell(t, r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8], &heat, &rp, &rate, &nsub, &ps);
+        double* o = out + 5 * i;
+        o[0] = heat; o[1] = rp; o[2] = rate; o[3] = (double)nsub; o[4] = ps;
+    }
+}

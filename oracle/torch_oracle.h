@@ -1,0 +1,136 @@
+/*
+ * torch_oracle.h -- CPU FP64 restatement of TORCH's hydro + thermodynamics step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the correctness oracle that the
+ * HIP path is checked against.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  Nothing in torch_amd/ links or
+ * calls it; the product path never falls back to it.
+ *
+ * Every function follows the reference (hgsteggles/TORCH, src/...) arithmetic
+ * operation by operation, in the same order, so that results are bitwise
+ * equal to the reference's own FP64 build (g++ -O3, x86-64, no FMA
+ * contraction).  Citations are given per function in torch_oracle.c.
+ *
+ * Parity pinning (what checks each part):
+ *   - hydro (limiter, UfromQ/QfromU/FfromQ, HLL, HLLC, rotated solver, BCs,
+ *     fixPrimitives/fixSolution, CFL, predictor-corrector, driver): bitwise
+ *     against fixtures written by the reference's own translation units
+ *     (oracle/_ref harness) in tests/golden/, plus closed forms and
+ *     invariants in tests/test_oracle_*.py.
+ *   - thermodynamics (units, rate functions, spline, preTimeStep, subcycle):
+ *     against reference-harness fixtures (bitwise where only + - * / sqrt are
+ *     involved; libm exp/log/pow calls are glibc's on both sides).
+ *   - MLP reference (forward/backward): finite differences + torch autograd.
+ *
+ * Layout: structure-of-arrays, U[v*ncell + i + nx*(j + ny*k)], variable order
+ * DEN, PRE, HII, ADV, VEL0, VEL1, VEL2 (reference src/Torch/Common.hpp:23-25).
+ */
+#ifndef TORCH_ORACLE_H_
+#define TORCH_ORACLE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_DEN = 0, ORC_PRE = 1, ORC_HII = 2, ORC_ADV = 3, ORC_VEL = 4, ORC_NV = 7 };
+
+/* Boundary conditions (reference src/Torch/Common.hpp:27 Condition order). */
+enum { ORC_BC_FREE = 0, ORC_BC_REFLECTING = 1, ORC_BC_OUTFLOW = 2, ORC_BC_INFLOW = 3, ORC_BC_PERIODIC = 4 };
+/* Riemann solvers (reference Riemann.cpp:246-257). */
+enum { ORC_HLL = 0, ORC_HLLC = 1, ORC_ROTATED_HLLC = 2 };
+/* Slope limiters (reference SlopeLimiter.cpp:313-332). */
+enum { ORC_LIM_MC = 0, ORC_LIM_SUPERBEE = 1, ORC_LIM_MINMOD = 2, ORC_LIM_MAXMOD = 3,
+       ORC_LIM_LEER = 4, ORC_LIM_OSPRE = 5, ORC_LIM_ALBADA = 6 };
+
+typedef struct {
+    int nx, ny, nz;
+    int spatial_order;        /* 0 or 1 (1 = linear reconstruction)          */
+    int solver;               /* ORC_HLL / ORC_HLLC / ORC_ROTATED_HLLC        */
+    int limiter;              /* ORC_LIM_*                                    */
+    int bc_left[3], bc_right[3];
+    double dx;                /* cell width, code units                       */
+    double gamma;
+    double dfloor, pfloor, tfloor;
+    double mass_frac_h;       /* X                                            */
+    double spec_gas_const;    /* R in code units                              */
+    double dt_max;
+    int nthreads;             /* OpenMP threads for the cell loops (>=1)      */
+} orc_hydro_params;
+
+/* ---- per-face / per-cell primitives (exposed for unit tests) ---------- */
+double orc_limiter(int limiter, double a, double b);
+void orc_UfromQ(double u[7], const double q[7], double gamma);
+void orc_QfromU(double q[7], const double u[7], double gamma);
+void orc_FfromQ(double f[7], const double q[7], double gamma, int dim);
+/* One face flux: solver(F, Ql, Qr, a_l2, a_r2, gamma, dim).  Returns 0, or 1
+ * if the HII flux is NaN (the reference throws there). */
+int orc_riemann(int solver, double F[7], const double Ql[7], const double Qr[7],
+                double a_l2, double a_r2, double gamma, int dim);
+void orc_riemann_batch(int solver, int n, const double* Ql, const double* Qr,
+                       const int* dims, double gamma, double* F, int* status);
+
+/* ---- whole-grid operations (interior cells only, SoA) ------------------ */
+void orc_globalQfromU(const orc_hydro_params* p, const double* U, double* Q);
+void orc_fixPrimitives(const orc_hydro_params* p, double* Q, const double* U);
+/* Returns number of cells that would make the reference throw. */
+int orc_fixSolution(const orc_hydro_params* p, double* U);
+double orc_hydro_timestep(const orc_hydro_params* p, const double* Q);
+/* UDOT = flux divergence of Q (BCs + reconstruction + Riemann + sources).
+ * Returns nonzero if a Riemann solve produced a NaN HII flux. */
+int orc_hydro_rhs(const orc_hydro_params* p, const double* Q, double* UDOT);
+
+/* Torch::hydroStep(dt, true) on U (Q must be fixPrimitives(QfromU(U))).
+ * W, Q2, UDOT: caller-provided work arrays of 7*ncell doubles. */
+int orc_hydro_step(const orc_hydro_params* p, double* U, double* Q, double* W,
+                   double* UDOT, double dt);
+
+/* Torch::run for a hydro-only problem, capped at nsteps.  tmax is code time,
+ * ncheckpoints as in the reference.  dts (nsteps) receives each dt.
+ * Returns 0 or an error code (1 NaN flux, 2 fixSolution failure, 3 checkValues). */
+int orc_run_hydro(const orc_hydro_params* p, double* U, int nsteps, double tmax,
+                  int ncheckpoints, double* dts, double* time_out);
+
+/* ---- units (reference Converter.cpp / Constants.cpp) ------------------- */
+typedef struct { double M, L, T; } orc_converter;
+void orc_converter_set_rho_pressure_time(orc_converter* c, double rho, double pre, double t);
+double orc_to_code_units(const orc_converter* c, double val, double mi, double li, double ti);
+double orc_from_code_units(const orc_converter* c, double val, double mi, double li, double ti);
+
+/* ---- thermodynamics (reference Thermodynamics.cpp) --------------------- */
+typedef struct {
+    /* constants in code units */
+    double hydrogen_mass, spec_gas_const, boltzmann, dust_xsec, pi;
+    double z0, T1, T2, T3, T4, imlc, nmlc, ciec_minT, ciec, cxhi_damp, n0, nmc;
+    double fuvh_a, fuvh_b, fuvh_c, irh_a, irh_b, crh, T_min_default;
+    /* log-log natural spline of the HI collisional-excitation rate */
+    double spl_x[26], spl_y[26], spl_y2[26], spl_min_slope, spl_max_slope;
+    /* switches */
+    double thermo_hii_switch, heating_amplification, mass_frac_h, gamma, pfloor;
+    int subcycling;
+} orc_thermo_params;
+
+void orc_thermo_init(orc_thermo_params* t, const orc_converter* c, double thermo_hii_switch,
+                     double heating_amplification, double mass_frac_h, double gamma,
+                     double pfloor, int subcycling);
+double orc_log_spline(const orc_thermo_params* t, double x);
+/* Individual rates (for unit tests); T in K, densities in code units. */
+void orc_thermo_rates(const orc_thermo_params* t, double nH, double hii, double T,
+                      double Av, double F_FUV, double out[9]);
+/* Per-cell preTimeStepCalculations + integrate.  Inputs are the cell's Q
+ * (den, pre, hii, adv), U[PRE] (total energy), T_min, COL_DEN and F_FUV.
+ * Outputs: heat (T[HEAT]), rate_pre (T[RATE] after preTimeStep), rate
+ * (T[RATE] after integrate), nsub (number of subcycle rate evaluations),
+ * p_sub (subcycled pressure). */
+void orc_thermo_cell(const orc_thermo_params* t, double den, double pre, double hii,
+                     double adv, double E, double T_min, double col_den, double F_FUV,
+                     double dt, double* heat, double* rate_pre, double* rate,
+                     int64_t* nsub, double* p_sub);
+void orc_thermo_batch(const orc_thermo_params* t, int64_t n, const double* in /* n x 9 */,
+                      double* out /* n x 5 */, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,198 @@
+"""Writes the golden fixtures of tests/golden/ by running the reference's own
+FP64 translation units (hgsteggles/TORCH src/Integrators, src/Fluid,
+src/Torch, compiled unmodified into oracle/_ref/ref_harness by
+``make -C oracle ref``; the harness's driver restates Torch::run /
+fullStep / hydroStep / calculateTimeStep, src/Torch/Torch.cpp:240-418).
+
+Run in the build container (where /root/reference is mounted):
+    make -C oracle ref && python tests/golden/make_golden.py
+The fixtures are inputs and the reference's outputs only; no reference source
+is stored.  Sizes are kept small (a few MB in total).
+"""
+from __future__ import annotations
+
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+from torch_amd import cooling_inputs, setups  # noqa: E402
+
+HARNESS = oracle.REF_HARNESS
+R_CGS = 8.314462e7
+
+
+def run_hydro_case(Q: np.ndarray, nsteps: int, dumps, tfloor=0.0, extra=None):
+    nz, ny, nx = Q.shape[1:]
+    with tempfile.TemporaryDirectory() as d:
+        Q.astype(np.float64).tofile(f"{d}/q.bin")
+        kv = {"nx": nx, "ny": ny, "nz": nz, "side_length": 1.0, "gamma": 1.67, "density_floor": 1e-30,
+              "temperature_floor": tfloor, "simulation_time": 1000, "ncheckpoints": 1, "nsteps": nsteps,
+              "input_q": f"{d}/q.bin", "dump_steps": " ".join(str(s) for s in dumps), "output_prefix": f"{d}/ref"}
+        kv.update(extra or {})
+        with open(f"{d}/case.txt", "w") as f:
+            for k, v in kv.items():
+                f.write(f"{k} {v}\n")
+        r = subprocess.run([HARNESS, "hydro", f"{d}/case.txt"], capture_output=True, text=True, check=True)
+        out = {s: np.fromfile(f"{d}/ref_U_{s:04d}.bin").reshape(7, nz, ny, nx) for s in dumps}
+        dts = np.fromfile(f"{d}/ref_dt.bin")
+        return out, dts, json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def sod():
+    n = 64
+    Q = setups.sod(n)
+    out, dts, info = run_hydro_case(Q, 200, [1, 2, 10, 200])
+    lines = {}
+    for s, U in out.items():
+        # planar symmetry: every (j, k) line is bitwise identical
+        assert (U == U[:, :1, :1, :]).all(), f"Sod step {s}: lines differ"
+        lines[f"U_{s}"] = U[:, 0, 0, :].copy()
+    np.savez_compressed(f"{HERE}/sod64.npz", dts=dts, time=info["time"], **lines)
+    print("sod64", info)
+
+
+def turbulent():
+    cases = {
+        "turb16_s1_m2": dict(n=16, sigma=1.0, mach=2.0, steps=20),
+        "turb24_s15_m4": dict(n=24, sigma=1.5, mach=4.0, steps=20),
+    }
+    for name, c in cases.items():
+        Q = setups.turbulent(c["n"], c["sigma"], c["mach"])
+        out, dts, info = run_hydro_case(Q, c["steps"], [0, 2, c["steps"]], tfloor=1e-12)
+        np.savez_compressed(f"{HERE}/{name}.npz", Q=Q, U0=out[0], U2=out[2], Uend=out[c["steps"]], dts=dts,
+                            steps=c["steps"], tfloor=1e-12)
+        print(name, info)
+
+
+def variants():
+    """Ragged 12x10x9 grid, mixed boundaries, other solvers and limiters."""
+    Q = setups.turbulent(16, 1.0, 1.5)[:, :9, :10, :12].copy()
+    bcs = {"left_boundary_condition_x": "outflow", "right_boundary_condition_x": "inflow",
+           "left_boundary_condition_y": "periodic", "right_boundary_condition_y": "periodic",
+           "left_boundary_condition_z": "reflecting", "right_boundary_condition_z": "free"}
+    res = {"Q": Q}
+    for solver, limiter, order in [("HLL", "minmod", 1), ("HLLC", "superbee", 1),
+                                   ("RotatedHLLC", "monotonised_central", 1), ("RotatedHLLC", "leer", 1),
+                                   ("HLLC", "ospre", 1), ("RotatedHLLC", "albada", 0)]:
+        extra = dict(bcs, riemann_solver=solver, slope_limiter=limiter, spatial_order=order)
+        out, dts, info = run_hydro_case(Q, 6, [6], tfloor=1e-12, extra=extra)
+        key = f"{solver}_{limiter}_{order}"
+        res[f"U_{key}"] = out[6]
+        res[f"dts_{key}"] = dts
+        print(key, info)
+    np.savez_compressed(f"{HERE}/variants.npz", **res)
+
+
+def riemann():
+    rng = np.random.Generator(np.random.PCG64(7))
+    n = 3000
+    Ql = np.zeros((n, 7))
+    Qr = np.zeros((n, 7))
+    for Q in (Ql, Qr):
+        Q[:, 0] = np.exp(rng.normal(0, 1, n))
+        Q[:, 1] = np.exp(rng.normal(0, 1, n))
+        Q[:, 2] = rng.uniform(0, 1, n)
+        Q[:, 3] = rng.uniform(0, 1, n)
+        Q[:, 4:7] = rng.normal(0, 2, (n, 3))
+    dims = rng.integers(0, 3, n)
+    # branch coverage: equal velocities (pure HLLC), velocity jump along the
+    # normal only (pure HLL), supersonic left/right states
+    k = n // 6
+    Qr[:k, 4:7] = Ql[:k, 4:7]
+    for i in range(k, 2 * k):
+        Qr[i, 4:7] = Ql[i, 4:7]
+        Qr[i, 4 + dims[i]] += rng.normal(0, 1)
+    Ql[2 * k:3 * k, 4:7] += 10.0
+    Qr[2 * k:3 * k, 4:7] += 10.0
+    Ql[3 * k:4 * k, 4:7] -= 10.0
+    Qr[3 * k:4 * k, 4:7] -= 10.0
+    Qr[4 * k:4 * k + 50, 4:7] = Ql[4 * k:4 * k + 50, 4:7] + 1e-8  # |dv| < 1e-6 threshold
+    res = {"Ql": Ql, "Qr": Qr, "dims": dims.astype(np.int32), "gamma": 1.67}
+    inp = np.concatenate([Ql, Qr, dims[:, None].astype(float)], axis=1)
+    with tempfile.TemporaryDirectory() as d:
+        inp.tofile(f"{d}/in.bin")
+        for solver in ("HLL", "HLLC", "RotatedHLLC"):
+            subprocess.run([HARNESS, "riemann", solver, "1.67", str(n), f"{d}/in.bin", f"{d}/out.bin"], check=True)
+            out = np.fromfile(f"{d}/out.bin").reshape(n, 8)
+            res[f"F_{solver}"] = out[:, :7]
+            res[f"status_{solver}"] = out[:, 7]
+    np.savez_compressed(f"{HERE}/riemann.npz", **res)
+    print("riemann", n)
+
+
+def consts():
+    res = {}
+    for name, scales in {"unit": (1.0, 1.0, 1.0), "cooling": (1e-20, 1e-28, 1e11),
+                         "cloud": (1e-20, 1e-28, 1e11)}.items():
+        r = subprocess.run([HARNESS, "consts", *map(str, scales), "0.1", "1e-30"], capture_output=True, text=True,
+                           check=True)
+        res[name] = json.loads(r.stdout)
+        res[name]["scales"] = scales
+    with open(f"{HERE}/consts.json", "w") as f:
+        json.dump(res, f, indent=1)
+    print("consts")
+
+
+def thermo():
+    """Per-cell Thermodynamics::preTimeStepCalculations + integrate on cells
+    drawn from the cooling-workload distribution (code units of the
+    1e-20/1e-28/1e11 scales).  Cells whose subcycle count is large are
+    thinned so that the reference finishes in about a minute."""
+    c = oracle.converter(1e-20, 1e-28, 1e11)
+    tc = lambda v, m, l, t: oracle.to_code(c, v, m, l, t)  # noqa: E731
+    s = cooling_inputs.sample_cgs(6000, seed=11)
+    inp9 = cooling_inputs.to_code_inputs(s, tc)
+    # estimate subcycle counts with the oracle (subcycling off) and keep
+    # cells with at most 2e5 substeps
+    dfloor = tc(1e-30, 1, -3, 0)
+    R = tc(R_CGS, 0, 2, -2)
+    pfloor = 0.1 * R * dfloor * 0.1
+    t = oracle.thermo_params(c, pfloor=pfloor, subcycling=False)
+    est = oracle.thermo_batch(t, inp9, 8)
+    ratio = inp9[:, 8] / np.abs(0.1 * inp9[:, 4] / est[:, 1])
+    keep = np.where(~(ratio > 2e5))[0][:4000]
+    inp9 = inp9[keep]
+    n = inp9.shape[0]
+    # gate cases: ADV below thermo_hii_switch -> exact no-op
+    inp9[:100, 3] = 0.005
+    # harness input: den, pre, hii, adv, T_min, col_den, photonRate, dt, E.
+    # photonRate S such that fluxFUV(0.5 S, r^2) = F_FUV with the harness's
+    # r^2 = dx^2 (dx = side_length / 2 in code units).
+    side = 1.0
+    dx = tc(side, 0, 1, 0) / 2.0
+    rsq = ((1.0 * 1.0) * dx) * dx
+    S = inp9[:, 7] * (1.2e7 * 4 * 3.14159265359 * rsq) / 0.5
+    hin = np.stack([inp9[:, 0], inp9[:, 1], inp9[:, 2], inp9[:, 3], inp9[:, 5], inp9[:, 6], S, inp9[:, 8],
+                    inp9[:, 4]], axis=1)
+    with tempfile.TemporaryDirectory() as d:
+        hin.tofile(f"{d}/in.bin")
+        with open(f"{d}/case.txt", "w") as f:
+            f.write("density_scale 1e-20\npressure_scale 1e-28\ntime_scale 1e11\n"
+                    f"side_length {side}\ngamma 1.67\ndensity_floor 1e-30\ntemperature_floor 0.1\n"
+                    "mass_fraction_hydrogen 1.0\nthermo_hii_switch 1e-2\nthermo_subcycling 1\n"
+                    f"ncells_in {n}\ninput {d}/in.bin\noutput {d}/out.bin\n")
+        subprocess.run([HARNESS, "thermo", f"{d}/case.txt"], check=True)
+        out = np.fromfile(f"{d}/out.bin").reshape(n, 10)
+    # the harness reports the F_FUV and COL_DEN the reference actually used
+    inp9[:, 7] = out[:, 5]
+    inp9[:, 6] = out[:, 4]
+    np.savez_compressed(f"{HERE}/thermo.npz", inputs=inp9, heat=out[:, 0], rate_pre=out[:, 1], rate=out[:, 2],
+                        h_tot=out[:, 3], temperature=out[:, 7], pfloor=pfloor)
+    print("thermo", n)
+
+
+if __name__ == "__main__":
+    if not os.path.exists(HARNESS):
+        sys.exit("build the reference harness first: make -C oracle ref")
+    which = sys.argv[1:] or ["consts", "riemann", "thermo", "variants", "turbulent", "sod"]
+    for w in which:
+        globals()[w]()

@@ -1,0 +1,147 @@
+"""GPU parity of the hydro step against the FP64 oracle (bitwise for FP64).
+
+The oracle (oracle/torch_oracle.c) is itself pinned bitwise to the
+reference's own translation units by tests/test_oracle_golden.py.
+"""
+import numpy as np
+import pytest
+
+from torch_amd import setups
+
+pytestmark = pytest.mark.gpu
+
+
+def _grid(gpu, shape, **kw):
+    return gpu.Grid(gpu.make_params(shape, side_length=kw.pop("side_length", 1.0), **kw))
+
+
+def _oparams(orc, shape, **kw):
+    return orc.make_params(shape, kw.pop("dx"), nthreads=8, **kw)
+
+
+def _u0(orc, p, Q):
+    return orc.UfromQ_grid(orc.fix_primitives(p, Q, np.zeros_like(Q)), p.gamma)
+
+
+def _run_both(gpu, orc, Q, nsteps, gkw=None, okw=None):
+    gkw = gkw or {}
+    okw = okw or {}
+    shape = Q.shape[1:]
+    g = _grid(gpu, shape, **gkw)
+    g.set_primitive(np.ascontiguousarray(Q))
+    p = _oparams(orc, shape, dx=gkw.get("side_length", 1.0) / shape[2], **okw)
+    U0 = _u0(orc, p, Q)
+    np.testing.assert_array_equal(g.get_conserved(), U0)
+    g.run(nsteps)
+    try:
+        Uo, dts, t = orc.run_hydro(p, U0, nsteps)
+    except RuntimeError as e:
+        # the reference throws here; the GPU path must report the same error
+        code = {"1": 3, "2": 4, "3": 5}[str(e).split("code ")[1].rstrip(")")]
+        with pytest.raises(gpu.TorchGpuError) as ge:
+            g.synchronize()
+        assert ge.value.code == code, (ge.value, e)
+        return None, None, None, None
+    g.synchronize()
+    Ug = g.get_conserved()
+    tg, dtg, n = g.clock()
+    return Ug, Uo, (tg, dtg, n), (t, dts[-1], nsteps)
+
+
+@pytest.mark.parametrize("n,steps", [(16, 12), (32, 5)])
+def test_sod_bitwise(gpu, orc, n, steps):
+    Q = setups.sod(n)
+    Ug, Uo, cg, co = _run_both(gpu, orc, Q, steps)
+    np.testing.assert_array_equal(Ug, Uo)
+    assert cg == co
+
+
+@pytest.mark.parametrize("n,sigma,mach,steps", [(16, 1.0, 2.0, 20), (24, 1.5, 4.0, 10)])
+def test_turbulent_bitwise(gpu, orc, n, sigma, mach, steps):
+    Q = setups.turbulent(n, sigma, mach)
+    kw = dict(tfloor=1e-12, pfloor=0.1 * 8.314462e7 * 1e-30 * 1e-12)
+    Ug, Uo, cg, co = _run_both(gpu, orc, Q, steps, gkw=dict(kw), okw=dict(kw))
+    np.testing.assert_array_equal(Ug, Uo)
+    assert cg == co
+
+
+@pytest.mark.parametrize("solver,limiter,order", [
+    ("HLL", "minmod", 1), ("HLLC", "superbee", 1), ("RotatedHLLC", "monotonised_central", 1),
+    ("RotatedHLLC", "leer", 1), ("HLLC", "ospre", 1), ("RotatedHLLC", "maxmod", 1), ("RotatedHLLC", "albada", 0)])
+def test_solver_limiter_variants_ragged_grid(gpu, orc, solver, limiter, order):
+    Q = setups.turbulent(16, 1.0, 1.5)[:, :9, :10, :12].copy()  # ragged 12 x 10 x 9
+    bcs = ["outflow", "inflow", "periodic", "periodic", "reflecting", "free"]
+    kw = dict(tfloor=1e-12, pfloor=0.1 * 8.314462e7 * 1e-30 * 1e-12, solver=solver, limiter=limiter,
+              spatial_order=order, bc=bcs)
+    Ug, Uo, cg, co = _run_both(gpu, orc, Q, 6, gkw=dict(kw), okw=dict(kw))
+    np.testing.assert_array_equal(Ug, Uo)
+
+
+def test_explicit_step_and_timestep(gpu, orc):
+    Q = setups.turbulent(16, 1.0, 2.0)
+    kw = dict(tfloor=1e-12, pfloor=0.1 * 8.314462e7 * 1e-30 * 1e-12)
+    g = _grid(gpu, Q.shape[1:], **kw)
+    g.set_primitive(Q)
+    p = _oparams(orc, Q.shape[1:], dx=1.0 / 16, **kw)
+    U = _u0(orc, p, Q)
+    for _ in range(3):
+        dt_o = orc.hydro_timestep(p, orc.fix_primitives(p, orc.q_from_u(p, U), U))
+        dt_g = g.hydro_timestep()
+        assert dt_g == dt_o
+        g.hydro_step(dt_g)
+        U = orc.hydro_step(p, U, dt_o)
+        g.synchronize()
+        np.testing.assert_array_equal(g.get_conserved(), U)
+    Qg = g.get_primitive(fix=True)
+    np.testing.assert_array_equal(Qg, orc.fix_primitives(p, orc.q_from_u(p, U), U))
+
+
+def test_deterministic_rerun(gpu):
+    Q = setups.turbulent(32, 1.0, 2.0)
+    kw = dict(tfloor=1e-12, pfloor=0.1 * 8.314462e7 * 1e-30 * 1e-12)
+    out = []
+    for _ in range(2):
+        g = _grid(gpu, Q.shape[1:], **kw)
+        g.set_primitive(Q)
+        g.run(8)
+        g.synchronize()
+        out.append(g.get_conserved())
+        g.close()
+    np.testing.assert_array_equal(out[0], out[1])
+
+
+def test_nan_state_raises(gpu):
+    """A non-finite conserved value makes the reference's checkValues throw
+    (Torch.cpp:420-443); the GPU path reports it at the next sync."""
+    Q = setups.sod(16)
+    g = _grid(gpu, Q.shape[1:])
+    g.set_primitive(Q)
+    U = g.get_conserved()
+    U[1, 3, 4, 5] = np.nan
+    g.set_conserved(U)
+    g.run(2)
+    with pytest.raises(gpu.TorchGpuError) as e:
+        g.synchronize()
+    assert e.value.code in (4, 5)
+
+
+def test_invalid_order_rejected(gpu):
+    with pytest.raises(gpu.TorchGpuError) as e:
+        gpu.Grid(gpu.make_params((8, 8, 8), spatial_order=2))
+    assert e.value.code == 6
+
+
+def test_mixed_precision_close_to_fp64(gpu):
+    Q = setups.turbulent(32, 1.0, 2.0)
+    kw = dict(tfloor=1e-12, pfloor=0.1 * 8.314462e7 * 1e-30 * 1e-12)
+    res = []
+    for prec in (gpu.FP64, gpu.MIXED):
+        g = _grid(gpu, Q.shape[1:], precision=prec, **kw)
+        g.set_primitive(Q)
+        g.run(20)
+        g.synchronize()
+        res.append(g.get_primitive(fix=False))
+        g.close()
+    d64, dmx = res[0][0], res[1][0]
+    rel_l1 = np.abs(dmx - d64).sum() / np.abs(d64).sum()
+    assert rel_l1 < 1e-5, rel_l1

@@ -1,0 +1,191 @@
+"""torch_amd -- MI355X (gfx950) hydro + thermodynamics step of TORCH.
+
+Python binding of the C ABI in include/torchgpu.h (libtorchgpu.so, built
+in-tree by torch_amd/build.py).  There is no CPU fallback: if the HIP
+library is missing, importing the binding raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtorchgpu.so")
+
+OK = 0
+ERRORS = {
+    1: "invalid argument", 2: "HIP error", 3: "RiemannSolver::solve: HII flux is NaN",
+    4: "Fluid::fixSolution: invalid value", 5: "hydro before produced an error",
+    6: "Hydrodynamics::calcFluxes: invalid order", 7: "RiemannSolverFactory::create: unknown type",
+    8: "SlopeLimiterFactory::create: unknown type", 9: "subcycle count out of range",
+    10: "surrogate not trained",
+}
+BC = {"free": 0, "reflecting": 1, "outflow": 2, "inflow": 3, "periodic": 4, "default": 0}
+SOLVER = {"HLL": 0, "HLLC": 1, "RotatedHLLC": 2, "default": 2}
+LIMITER = {"monotonised_central": 0, "superbee": 1, "minmod": 2, "maxmod": 3,
+           "leer": 4, "ospre": 5, "albada": 6, "default": 6}
+FP64, MIXED = 0, 1
+
+
+class TorchGpuError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("spatial_order", ctypes.c_int),
+        ("bc_left", ctypes.c_int * 3), ("bc_right", ctypes.c_int * 3),
+        ("riemann_solver", ctypes.c_int), ("slope_limiter", ctypes.c_int),
+        ("precision", ctypes.c_int), ("ncheckpoints", ctypes.c_int),
+        ("side_length", ctypes.c_double), ("gamma", ctypes.c_double),
+        ("dfloor", ctypes.c_double), ("pfloor", ctypes.c_double), ("tfloor", ctypes.c_double),
+        ("mass_fraction_h", ctypes.c_double), ("spec_gas_const", ctypes.c_double),
+        ("tmax", ctypes.c_double), ("dt_max", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run `python -m torch_amd.build` (there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        d = ctypes.c_double
+        i = ctypes.c_int
+        L.torchgpu_create.argtypes = [ctypes.POINTER(Params), vp, ctypes.POINTER(vp)]
+        L.torchgpu_destroy.argtypes = [vp]
+        L.torchgpu_destroy.restype = None
+        L.torchgpu_last_error.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
+        L.torchgpu_state_ptr.argtypes = [vp, ctypes.POINTER(vp)]
+        for name in ("torchgpu_set_primitive", "torchgpu_set_conserved", "torchgpu_get_conserved"):
+            getattr(L, name).argtypes = [vp, vp, i]
+        L.torchgpu_get_primitive.argtypes = [vp, vp, i, i]
+        L.torchgpu_hydro_timestep.argtypes = [vp, d, ctypes.POINTER(d)]
+        L.torchgpu_hydro_step.argtypes = [vp, d]
+        L.torchgpu_run.argtypes = [vp, i]
+        L.torchgpu_reset_clock.argtypes = [vp]
+        L.torchgpu_synchronize.argtypes = [vp]
+        L.torchgpu_get_clock.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(i)]
+        _lib = L
+    return _lib
+
+
+def make_params(shape, side_length=1.0, gamma=1.67, dfloor=1e-30, pfloor=0.0, tfloor=0.0,
+                mass_fraction_h=1.0, spec_gas_const=8.314462e7, tmax=1e3, dt_max=None,
+                ncheckpoints=1, solver="RotatedHLLC", limiter="albada", spatial_order=1,
+                bc="outflow", precision=FP64) -> Params:
+    """Parameters in code units.  shape = (nz, ny, nx)."""
+    p = Params()
+    p.nz, p.ny, p.nx = shape
+    p.spatial_order = spatial_order
+    bcs = bc if isinstance(bc, (list, tuple)) else [bc] * 6
+    for k in range(3):
+        p.bc_left[k] = BC[bcs[2 * k]]
+        p.bc_right[k] = BC[bcs[2 * k + 1]]
+    p.riemann_solver = SOLVER[solver] if isinstance(solver, str) else solver
+    p.slope_limiter = LIMITER[limiter] if isinstance(limiter, str) else limiter
+    p.precision = precision
+    p.ncheckpoints = ncheckpoints
+    p.side_length, p.gamma = side_length, gamma
+    p.dfloor, p.pfloor, p.tfloor = dfloor, pfloor, tfloor
+    p.mass_fraction_h, p.spec_gas_const = mass_fraction_h, spec_gas_const
+    p.tmax = tmax
+    p.dt_max = tmax / 10.0 if dt_max is None else dt_max
+    return p
+
+
+def _ptr(a) -> int:
+    """Address of a contiguous float64 numpy array or torch tensor."""
+    if isinstance(a, np.ndarray):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+        return a.ctypes.data
+    assert a.dtype.__str__() == "torch.float64" and a.is_contiguous()
+    return a.data_ptr()
+
+
+class Grid:
+    """One whole grid on the current GPU (a torchgpu_ctx)."""
+
+    def __init__(self, params: Params, stream: int | None = None):
+        self.params = params
+        self.shape = (params.nz, params.ny, params.nx)
+        self._h = ctypes.c_void_p()
+        rc = lib().torchgpu_create(ctypes.byref(params), ctypes.c_void_p(stream or 0), ctypes.byref(self._h))
+        if rc:
+            raise TorchGpuError(rc, ERRORS.get(rc, "create failed"))
+
+    def _check(self, rc: int):
+        if rc:
+            buf = ctypes.create_string_buffer(4096)
+            lib().torchgpu_last_error(self._h, buf, 4096)
+            raise TorchGpuError(rc, buf.value.decode() or ERRORS.get(rc, "error"))
+
+    def close(self):
+        if self._h:
+            lib().torchgpu_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state -------------------------------------------------------------
+    def _is_dev(self, a) -> int:
+        return 0 if isinstance(a, np.ndarray) else int(a.is_cuda)
+
+    def set_primitive(self, Q):
+        self._check(lib().torchgpu_set_primitive(self._h, ctypes.c_void_p(_ptr(Q)), self._is_dev(Q)))
+
+    def set_conserved(self, U):
+        self._check(lib().torchgpu_set_conserved(self._h, ctypes.c_void_p(_ptr(U)), self._is_dev(U)))
+
+    def get_conserved(self, out=None):
+        out = np.empty((7, *self.shape)) if out is None else out
+        self._check(lib().torchgpu_get_conserved(self._h, ctypes.c_void_p(_ptr(out)), self._is_dev(out)))
+        return out
+
+    def get_primitive(self, fix: bool = True, out=None):
+        out = np.empty((7, *self.shape)) if out is None else out
+        self._check(lib().torchgpu_get_primitive(self._h, ctypes.c_void_p(_ptr(out)), int(fix), self._is_dev(out)))
+        return out
+
+    def state_ptr(self) -> int:
+        p = ctypes.c_void_p()
+        self._check(lib().torchgpu_state_ptr(self._h, ctypes.byref(p)))
+        return p.value
+
+    # -- operators ---------------------------------------------------------
+    def hydro_timestep(self, dt_max: float | None = None) -> float:
+        dt = ctypes.c_double()
+        self._check(lib().torchgpu_hydro_timestep(self._h, self.params.dt_max if dt_max is None else dt_max,
+                                                  ctypes.byref(dt)))
+        return dt.value
+
+    def hydro_step(self, dt: float):
+        self._check(lib().torchgpu_hydro_step(self._h, dt))
+
+    def run(self, nsteps: int):
+        self._check(lib().torchgpu_run(self._h, nsteps))
+
+    def reset_clock(self):
+        self._check(lib().torchgpu_reset_clock(self._h))
+
+    def synchronize(self):
+        self._check(lib().torchgpu_synchronize(self._h))
+
+    def clock(self):
+        t, dt, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        self._check(lib().torchgpu_get_clock(self._h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n)))
+        return t.value, dt.value, n.value

@@ -1,0 +1,275 @@
+// api.hip -- C ABI over the HIP kernels (see include/torchgpu.h).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/torchgpu.h"
+#include "torchgpu_internal.hpp"
+
+struct torchgpu_ctx {
+    torchgpu_params p;
+    tg::HydroGeom g;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    double* U = nullptr;      // conserved state
+    double* Ustar = nullptr;  // predictor state (also scratch for Q transfers)
+    tg::StepState* st = nullptr;
+    std::string err;
+};
+
+namespace {
+
+int fail(torchgpu_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+int hip_fail(torchgpu_ctx* c, hipError_t e, const char* where) {
+    return fail(c, TORCHGPU_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(ctx, call)                                   \
+    do {                                                     \
+        hipError_t e_ = (call);                              \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, #call); \
+    } while (0)
+
+size_t state_bytes(const torchgpu_ctx* c) { return sizeof(double) * 7 * (size_t)c->g.ncell; }
+
+// Maps the device error bits of the first failing step to the exception the
+// reference throws first inside that step.
+int check_device_errors(torchgpu_ctx* c) {
+    tg::StepState st;
+    HIP_TRY(c, hipMemcpyAsync(&st, c->st, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (!st.err) return TORCHGPU_OK;
+    const std::string at = " (step " + std::to_string(st.err_step1 - 1) + ")";
+    const unsigned int first = st.err & (0u - st.err);
+    switch (first) {
+    case tg::ERR_CHECK_VALUES:
+        return fail(c, TORCHGPU_ERR_CHECK_VALUES, "hydro before produced an error." + at);
+    case tg::ERR_RIEMANN_NAN_1:
+    case tg::ERR_RIEMANN_NAN_2:
+        return fail(c, TORCHGPU_ERR_RIEMANN_NAN, "RiemannSolver::solve: HII flux is NaN" + at);
+    case tg::ERR_THERMO_NSTEPS:
+        return fail(c, TORCHGPU_ERR_NSTEPS_RANGE, "Thermodynamics::integrate: subcycle count out of int range" + at);
+    default:
+        return fail(c, TORCHGPU_ERR_FIX_SOLUTION, "Fluid::fixSolution: invalid value." + at);
+    }
+}
+
+int reset_clock(torchgpu_ctx* c) {
+    tg::StepState st;
+    std::memset(&st, 0, sizeof st);
+    st.dt_max = c->p.dt_max;
+    st.tmax = c->p.tmax;
+    st.first = 1;
+    st.ncheckpoints = c->p.ncheckpoints > 0 ? c->p.ncheckpoints : 1;
+    HIP_TRY(c, hipMemcpyAsync(c->st, &st, sizeof st, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return TORCHGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int torchgpu_create(const torchgpu_params* p, void* stream, torchgpu_ctx** out) {
+    if (!p || !out) return TORCHGPU_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (p->nx < 1 || p->ny < 1 || p->nz < 1) return TORCHGPU_ERR_INVALID_ARG;
+    if (p->spatial_order != 0 && p->spatial_order != 1) return TORCHGPU_ERR_INVALID_ORDER;
+    if (p->riemann_solver < 0 || p->riemann_solver > 2) return TORCHGPU_ERR_UNKNOWN_SOLVER;
+    if (p->slope_limiter < 0 || p->slope_limiter > 6) return TORCHGPU_ERR_UNKNOWN_LIMITER;
+    for (int d = 0; d < 3; ++d)
+        if (p->bc_left[d] < 0 || p->bc_left[d] > 4 || p->bc_right[d] < 0 || p->bc_right[d] > 4)
+            return TORCHGPU_ERR_INVALID_ARG;
+    if (p->precision != TORCHGPU_FP64 && p->precision != TORCHGPU_MIXED) return TORCHGPU_ERR_INVALID_ARG;
+
+    torchgpu_ctx* c = new torchgpu_ctx();
+    c->p = *p;
+    tg::HydroGeom& g = c->g;
+    std::memset(&g, 0, sizeof g);
+    g.nx = p->nx; g.ny = p->ny; g.nz = p->nz;
+    g.ntx = (p->nx + 7) / 8; g.nty = (p->ny + 7) / 8; g.ntz = (p->nz + 7) / 8;
+    g.ntiles = g.ntx * g.nty * g.ntz;
+    for (int d = 0; d < 3; ++d) { g.bc_l[d] = p->bc_left[d]; g.bc_r[d] = p->bc_right[d]; }
+    g.limiter = p->slope_limiter;
+    g.spatial_order = p->spatial_order;
+    g.ncell = (long long)p->nx * p->ny * p->nz;
+    // Grid::initialise: dx[i] = sideLength / ncells[0] (Grid.cpp:471-472)
+    g.dx = p->side_length / (double)p->nx;
+    // Cartesian area and volume (Grid.cpp:846-907), A/V as Hydro.cpp:129-130
+    const double area = 1.0 * g.dx * g.dx;
+    const double vol = 1.0 * g.dx * g.dx * g.dx;
+    g.kflux = area / vol;
+    g.dfloor = p->dfloor; g.pfloor = p->pfloor; g.tfloor = p->tfloor;
+    g.X = p->mass_fraction_h;
+    g.mu_c = (1.0 - p->mass_fraction_h) * 0.25;
+    g.Rgas = p->spec_gas_const;
+    g.dt_max = p->dt_max;
+    g.g = p->gamma;
+    g.gm1 = p->gamma - 1.0;
+    g.g2 = p->gamma / (p->gamma - 1.0);
+    g.gf = (float)p->gamma;
+    g.gm1f = (float)p->gamma - 1.0f;
+    g.g2f = (float)p->gamma / ((float)p->gamma - 1.0f);
+
+    if (stream) {
+        c->stream = (hipStream_t)stream;
+    } else {
+        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) { delete c; return TORCHGPU_ERR_HIP; }
+        c->own_stream = true;
+    }
+    const size_t bytes = state_bytes(c);
+    if (hipMalloc(&c->U, bytes) != hipSuccess || hipMalloc(&c->Ustar, bytes) != hipSuccess ||
+        hipMalloc(&c->st, sizeof(tg::StepState)) != hipSuccess) {
+        torchgpu_destroy(c);
+        return TORCHGPU_ERR_HIP;
+    }
+    if (reset_clock(c) != TORCHGPU_OK) {
+        torchgpu_destroy(c);
+        return TORCHGPU_ERR_HIP;
+    }
+    *out = c;
+    return TORCHGPU_OK;
+}
+
+void torchgpu_destroy(torchgpu_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->U) (void)hipFree(c->U);
+    if (c->Ustar) (void)hipFree(c->Ustar);
+    if (c->st) (void)hipFree(c->st);
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int torchgpu_last_error(const torchgpu_ctx* c, char* buf, size_t len) {
+    if (!c || !buf || len == 0) return TORCHGPU_ERR_INVALID_ARG;
+    std::snprintf(buf, len, "%s", c->err.c_str());
+    return TORCHGPU_OK;
+}
+
+int torchgpu_state_ptr(torchgpu_ctx* c, double** U) {
+    if (!c || !U) return TORCHGPU_ERR_INVALID_ARG;
+    *U = c->U;
+    return TORCHGPU_OK;
+}
+
+int torchgpu_set_primitive(torchgpu_ctx* c, const double* Q, int on_device) {
+    if (!c || !Q) return TORCHGPU_ERR_INVALID_ARG;
+    HIP_TRY(c, hipMemcpyAsync(c->Ustar, Q, state_bytes(c), on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                              c->stream));
+    HIP_TRY(c, tg::launch_u_from_q(c->g, c->Ustar, c->U, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return TORCHGPU_OK;
+}
+
+int torchgpu_set_conserved(torchgpu_ctx* c, const double* U, int on_device) {
+    if (!c || !U) return TORCHGPU_ERR_INVALID_ARG;
+    HIP_TRY(c, hipMemcpyAsync(c->U, U, state_bytes(c), on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return TORCHGPU_OK;
+}
+
+int torchgpu_get_conserved(torchgpu_ctx* c, double* U, int on_device) {
+    if (!c || !U) return TORCHGPU_ERR_INVALID_ARG;
+    HIP_TRY(c, hipMemcpyAsync(U, c->U, state_bytes(c), on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return TORCHGPU_OK;
+}
+
+int torchgpu_get_primitive(torchgpu_ctx* c, double* Q, int fix, int on_device) {
+    if (!c || !Q) return TORCHGPU_ERR_INVALID_ARG;
+    HIP_TRY(c, tg::launch_q_from_u(c->g, c->U, c->Ustar, fix, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(Q, c->Ustar, state_bytes(c), on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return TORCHGPU_OK;
+}
+
+int torchgpu_hydro_timestep(torchgpu_ctx* c, double dt_max, double* dt) {
+    if (!c || !dt) return TORCHGPU_ERR_INVALID_ARG;
+    // a private StepState slot: reuse the context's maxinv after saving it
+    tg::StepState saved;
+    HIP_TRY(c, hipMemcpyAsync(&saved, c->st, sizeof saved, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    tg::StepState tmp = saved;
+    tmp.maxinv = 0ull;
+    HIP_TRY(c, hipMemcpyAsync(c->st, &tmp, sizeof tmp, hipMemcpyHostToDevice, c->stream));
+    tg::HydroGeom g = c->g;
+    g.dt_max = dt_max;
+    HIP_TRY(c, tg::launch_cfl(g, c->U, c->st, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(&tmp, c->st, sizeof tmp, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->st, &saved, sizeof saved, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    // Hydro.cpp:78-88: tmin = min(dt_max, min_cells 0.5/inv_t) = min(dt_max, 0.5/max inv_t)
+    double maxinv;
+    std::memcpy(&maxinv, &tmp.maxinv, sizeof maxinv);
+    double tmin = dt_max;
+    if (maxinv > 0.0) { double t = 0.5 / maxinv; tmin = (t < tmin) ? t : tmin; }
+    *dt = (dt_max < tmin) ? dt_max : tmin;
+    return TORCHGPU_OK;
+}
+
+static int hydro_step_device(torchgpu_ctx* c) {
+    // Torch::hydroStep: predictor U* = fix(U + dt/2 L(Q(U))), corrector
+    // U = fix(U + dt L(Q(U*))); U is updated in place by the corrector.
+    HIP_TRY(c, tg::launch_hydro_stage(c->g, c->p.riemann_solver, c->p.precision, 1, c->U, c->U, c->Ustar, c->st,
+                                      c->stream));
+    HIP_TRY(c, tg::launch_hydro_stage(c->g, c->p.riemann_solver, c->p.precision, 2, c->Ustar, c->U, c->U, c->st,
+                                      c->stream));
+    return TORCHGPU_OK;
+}
+
+int torchgpu_hydro_step(torchgpu_ctx* c, double dt) {
+    if (!c) return TORCHGPU_ERR_INVALID_ARG;
+    // an explicit dt: step_begin with use_cfl = 0 still honours tmax halting
+    // bookkeeping but takes dt as given
+    HIP_TRY(c, tg::launch_step_begin(c->st, 0, dt, c->stream));
+    int rc = hydro_step_device(c);
+    if (rc) return rc;
+    HIP_TRY(c, tg::launch_step_end(c->st, c->stream));
+    return TORCHGPU_OK;
+}
+
+int torchgpu_run(torchgpu_ctx* c, int nsteps) {
+    if (!c || nsteps < 0) return TORCHGPU_ERR_INVALID_ARG;
+    for (int s = 0; s < nsteps; ++s) {
+        HIP_TRY(c, tg::launch_step_begin(c->st, 1, 0.0, c->stream));
+        int rc = hydro_step_device(c);
+        if (rc) return rc;
+        HIP_TRY(c, tg::launch_step_end(c->st, c->stream));
+    }
+    return TORCHGPU_OK;
+}
+
+int torchgpu_reset_clock(torchgpu_ctx* c) {
+    if (!c) return TORCHGPU_ERR_INVALID_ARG;
+    return reset_clock(c);
+}
+
+int torchgpu_synchronize(torchgpu_ctx* c) {
+    if (!c) return TORCHGPU_ERR_INVALID_ARG;
+    return check_device_errors(c);
+}
+
+int torchgpu_get_clock(torchgpu_ctx* c, double* time, double* last_dt, int* steps) {
+    if (!c) return TORCHGPU_ERR_INVALID_ARG;
+    tg::StepState st;
+    HIP_TRY(c, hipMemcpyAsync(&st, c->st, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (time) *time = st.time;
+    if (last_dt) *last_dt = st.dt;
+    if (steps) *steps = st.step;
+    return TORCHGPU_OK;
+}
+
+}  // extern "C"

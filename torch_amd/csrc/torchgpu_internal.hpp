@@ -1,0 +1,72 @@
+// torchgpu_internal.hpp -- device-side parameter blocks, the per-run step
+// state and the launcher entry points shared by the HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace tg {
+
+// Grid + hydro parameters, passed by value to every hydro kernel.
+struct HydroGeom {
+    int nx, ny, nz;
+    int ntx, nty, ntz;      // 8^3 tiles per dimension
+    int ntiles;
+    int bc_l[3], bc_r[3];
+    int limiter;
+    int spatial_order;
+    long long ncell;        // nx*ny*nz, SoA stride
+    double dx;
+    double kflux;           // (area / vol) of a Cartesian face, Hydro.cpp:129-130
+    double dfloor, pfloor, tfloor;
+    double X;               // mass fraction of hydrogen
+    double mu_c;            // (1.0 - X) * 0.25
+    double Rgas;            // specific gas constant (code units)
+    double dt_max;
+    double g, gm1, g2;      // gamma, gamma - 1.0, gamma / (gamma - 1.0)
+    float gf, gm1f, g2f;    // the same rounded to float (mixed path)
+};
+
+// Device-resident state of a time-stepping run (one per context).
+struct StepState {
+    double dt;              // dt of the step in flight
+    double time;            // simulation time (code units)
+    double dt_max;
+    double tmax;
+    unsigned long long maxinv;  // bit pattern of max_cells inv_t (>= 0)
+    int first;              // next step is the reference's first step
+    int ckpt_count;
+    int ncheckpoints;
+    int halted;             // time >= tmax: further steps are no-ops
+    unsigned int err;       // error bits (ERR_*) raised during step err_step
+    int err_step1;          // 1 + step index of the first error (0: none)
+    int step;               // steps completed
+    int pad;
+};
+
+// Error bits in the order the reference would meet them inside one step;
+// the lowest set bit of the first failing step is the error reported.
+enum : unsigned int {
+    ERR_CHECK_VALUES = 1u,    // Torch::checkValues("hydro before") throw (Torch.cpp:420-443)
+    ERR_RIEMANN_NAN_1 = 2u,   // RiemannSolver::solve: HII flux is NaN, predictor (Riemann.cpp:95-103)
+    ERR_FIX_SOLUTION_1 = 4u,  // Fluid::fixSolution throw, predictor (Fluid.cpp:91-92,128-134)
+    ERR_RIEMANN_NAN_2 = 8u,   // ... corrector
+    ERR_FIX_SOLUTION_2 = 16u, // ... corrector
+    ERR_THERMO_NSTEPS = 32u,  // subcycle count beyond the int range of Thermodynamics.cpp:307
+    ERR_THERMO_FIX = 64u,     // fixSolution throw after a thermo sub-step
+};
+
+// Launchers (hydro.hip).  All enqueue on `stream` and return hipError_t.
+// precision: 0 = FP64 everywhere, 1 = FP32 reconstruction + Riemann flux
+// with FP64 accumulation, pressure recovery and CFL.
+hipError_t launch_hydro_stage(const HydroGeom& g, int solver, int precision, int stage,
+                              const double* Usrc, const double* Ubase, double* Udst,
+                              StepState* st, hipStream_t stream);
+hipError_t launch_cfl(const HydroGeom& g, const double* U, StepState* st, hipStream_t stream);
+hipError_t launch_step_begin(StepState* st, int use_cfl, double dt_override, hipStream_t stream);
+hipError_t launch_step_end(StepState* st, hipStream_t stream);
+hipError_t launch_q_from_u(const HydroGeom& g, const double* U, double* Q, int fix, hipStream_t stream);
+hipError_t launch_u_from_q(const HydroGeom& g, double* Q, double* U, hipStream_t stream);
+
+}  // namespace tg
