@@ -24,8 +24,8 @@
  *     + updateSourceTerms on independent cells
  *     (src/Integrators/Thermodynamics.cpp:218-337,478-485)
  *   Thermodynamics operator on the grid state             torchgpu_thermo_substep
- *   MLP surrogate of the subcycled update (no reference   torchgpu_mlp_*
- *     counterpart; FP64 integrate remains the fallback)
+ *   MLP surrogate of the subcycled update (no reference   torchgpu_mlp_*,
+ *     counterpart; FP64 integrate remains the fallback)     torchgpu_thermo_cells_surrogate
  *
  * Conventions
  *   - State arrays are structure-of-arrays float64 [7][nz][ny][nx], x fastest,
@@ -123,6 +123,17 @@ int torchgpu_hydro_timestep(torchgpu_ctx* ctx, double dt_max, double* dt);
 /* Torch::hydroStep(dt, true).  Asynchronous; errors surface at the next
  * synchronizing call. */
 int torchgpu_hydro_step(torchgpu_ctx* ctx, double dt);
+/* One stage of Torch::hydroStep: Hydrodynamics::integrate + updateSourceTerms
+ * on Q = QfromU(U) (fixPrimitives'd when stage == 1), then
+ * Fluid::advSolution(h * dt) + fixSolution applied to U, written to the
+ * predictor buffer (stage 1, h = 1/2) or in place to U (stage 2, h = 1;
+ * Q from the predictor buffer, CFL reduction of the next step fused).
+ * Exposed for per-kernel timing and for drivers that sequence the stages
+ * themselves (Torch.cpp:374-386).  dt is the last dt set by
+ * torchgpu_hydro_step / torchgpu_run / torchgpu_set_dt. */
+int torchgpu_hydro_stage(torchgpu_ctx* ctx, int stage);
+/* Sets the dt used by torchgpu_hydro_stage. */
+int torchgpu_set_dt(torchgpu_ctx* ctx, double dt);
 /* Up to `nsteps` iterations of the Torch::run loop for a hydro-only run,
  * with dt chosen on the device (first step dt_max * 1e-20, then the CFL
  * min-reduction fused into the corrector stage, clipped to the next
@@ -134,6 +145,105 @@ int torchgpu_reset_clock(torchgpu_ctx* ctx);
 int torchgpu_synchronize(torchgpu_ctx* ctx);
 /* Simulation time, dt of the last step and number of completed steps. */
 int torchgpu_get_clock(torchgpu_ctx* ctx, double* time, double* last_dt, int* steps);
+
+
+/* ------------------------------------------------------------------------ */
+/* Thermodynamics (cooling) operator                                         */
+/* ------------------------------------------------------------------------ */
+
+/* Constants in code units and the ThermoParameters switches, as
+ * Thermodynamics::initialise holds them (Thermodynamics.cpp:24-80) after
+ * Constants::initialise (Constants.cpp:27-40). */
+typedef struct torchgpu_thermo_params {
+    double thermo_hii_switch, heating_amplification, mass_fraction_h, gamma, pfloor;
+    double hydrogen_mass, spec_gas_const, dust_xsec, pi;
+    double z0, T1, T2, T3, T4, imlc, nmlc, ciec_minT, ciec, cxhi_damp, n0, nmc;
+    double fuvh_a, fuvh_b, fuvh_c, irh_a, irh_b, crh;
+    double spl_min_slope, spl_max_slope;
+    double spl_x[26], spl_y[26], spl_y2[26];
+    int subcycling;
+    int pad;
+} torchgpu_thermo_params;
+
+/* Converter::toCodeUnits / fromCodeUnits (Converter.cpp:27-63) for the
+ * scales of Integration.{density,pressure,time}_scale. */
+int torchgpu_units(double density_scale, double pressure_scale, double time_scale, double value,
+                   double mass_index, double length_index, double time_index, int to_code, double* out);
+/* Fills `t` from the Lua-level parameters (cgs floors, code scales). */
+int torchgpu_thermo_params_init(torchgpu_thermo_params* t, double density_scale, double pressure_scale,
+                                double time_scale, double thermo_hii_switch, double heating_amplification,
+                                double mass_fraction_h, double gamma, double density_floor_cgs,
+                                double temperature_floor, int subcycling);
+
+typedef struct torchgpu_cooling torchgpu_cooling;
+
+/* A cooling operator on the current device (spline tables, work counters and
+ * the surrogate network live here). */
+int torchgpu_cooling_create(const torchgpu_thermo_params* t, void* stream, torchgpu_cooling** out);
+void torchgpu_cooling_destroy(torchgpu_cooling* c);
+int torchgpu_cooling_last_error(const torchgpu_cooling* c, char* buf, size_t len);
+
+/* Thermodynamics::preTimeStepCalculations + integrate on n independent
+ * non-wind cells (Thermodynamics.cpp:218-337), FP64, subcycled.
+ * in  (device, float64 [9][n]): den, pre, hii, adv, E (= U[PRE]), T_min,
+ *      COL_DEN, F_FUV, dt -- code units.
+ * out (device, float64 [5][n]): T[HEAT], T[RATE] after preTimeStep, T[RATE]
+ *      after integrate, number of subcycle rate evaluations, subcycled
+ *      pressure.
+ * total_substeps (host, may be NULL) receives the sum of subcycle counts
+ * (this synchronizes). */
+int torchgpu_thermo_cells(torchgpu_cooling* c, int64_t n, const double* in, double* out, int64_t* total_substeps);
+
+/* ------------------------------------------------------------------------ */
+/* Cooling surrogate (no reference counterpart: a learned replacement of the  */
+/* subcycled integrate, with the FP64 integrator as gate and fallback)       */
+/* ------------------------------------------------------------------------ */
+
+/* MLP 8 -> 128 -> 128 -> 128 -> 128 -> 3 (SiLU), bf16 weights, fp32
+ * accumulation.  Parameter blobs (fp32, host): W = W1[128][8] | W2 | W3 |
+ * W4 [128][128] | W5[3][128]; b = b1 | b2 | b3 | b4 [128] | b5[3]. */
+enum { TORCHGPU_MLP_NW = 128 * 8 + 3 * 128 * 128 + 3 * 128, TORCHGPU_MLP_NB = 4 * 128 + 3 };
+typedef struct torchgpu_mlp torchgpu_mlp;
+int torchgpu_mlp_create(int max_batch, void* stream, torchgpu_mlp** out);
+void torchgpu_mlp_destroy(torchgpu_mlp* m);
+int torchgpu_mlp_set_params(torchgpu_mlp* m, const float* W, const float* b);
+int torchgpu_mlp_get_params(torchgpu_mlp* m, float* W, float* b);
+/* y[B][3] = MLP(feat[B][8]) with the fused inference kernel (device ptrs). */
+int torchgpu_mlp_forward(torchgpu_mlp* m, const float* feat, int64_t B, float* y);
+/* Loss 0.5/B sum_o w_o (y_o - t_o)^2 (output 2's target is the log10
+ * relative error of output 0 when err_head != 0) and its gradients, without
+ * an update.  feat [B][8], target [B][3] device; gW/gb host (may be NULL). */
+int torchgpu_mlp_loss_grad(torchgpu_mlp* m, const float* feat, const float* target, int B, const float* w3,
+                           int err_head, float* loss, float* gW, float* gb);
+/* One Adam step on a batch (loss_grad + update of the master weights). */
+int torchgpu_mlp_train_step(torchgpu_mlp* m, const float* feat, const float* target, int B, const float* w3,
+                            int err_head, float lr, float weight_decay, float* loss);
+
+/* Surrogate configuration of a cooling operator: network, feature
+ * normalisation f = (u - mu) * isd of the raw features u = (log10 rho,
+ * log10 T, x, log10 F_FUV, Av, log10 dt) in code units, the training
+ * envelope [lo, hi] of u, the error gate on the estimated relative T error
+ * and the subcycle count up to which cells are integrated exactly. */
+typedef struct torchgpu_surrogate_cfg {
+    float mu[8], isd[8];
+    double lo[6], hi[6];
+    double gate;
+    int nexact;
+    int pad;
+} torchgpu_surrogate_cfg;
+int torchgpu_cooling_set_surrogate(torchgpu_cooling* c, torchgpu_mlp* m, const torchgpu_surrogate_cfg* cfg);
+/* The surrogate step: out as torchgpu_thermo_cells; path (device, uint8
+ * [n], may be NULL): 0 exact FP64, 1 surrogate, 2 FP64 fallback.  counts
+ * (host, may be NULL, synchronizes): cells on the surrogate and fallback
+ * paths. */
+int torchgpu_thermo_cells_surrogate(torchgpu_cooling* c, int64_t n, const double* in, double* out, uint8_t* path,
+                                    int64_t* counts);
+/* Training data: for the cells with more than nexact subcycles, writes their
+ * indices (list, device int64 [n]), features (feat [n][8], normalised with
+ * the configured mu/isd) and, from the FP64 integrator's output `out`, the
+ * targets (target [n][3]: d log10 T_sub, 0, 0).  Returns the count. */
+int torchgpu_surrogate_training_data(torchgpu_cooling* c, int64_t n, const double* in, const double* out,
+                                     int64_t* list, float* feat, float* target, float* raw, int64_t* count);
 
 #ifdef __cplusplus
 }

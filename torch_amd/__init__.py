@@ -73,6 +73,8 @@ def lib() -> ctypes.CDLL:
         L.torchgpu_hydro_timestep.argtypes = [vp, d, ctypes.POINTER(d)]
         L.torchgpu_hydro_step.argtypes = [vp, d]
         L.torchgpu_run.argtypes = [vp, i]
+        L.torchgpu_hydro_stage.argtypes = [vp, i]
+        L.torchgpu_set_dt.argtypes = [vp, d]
         L.torchgpu_reset_clock.argtypes = [vp]
         L.torchgpu_synchronize.argtypes = [vp]
         L.torchgpu_get_clock.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(i)]
@@ -179,6 +181,12 @@ class Grid:
     def run(self, nsteps: int):
         self._check(lib().torchgpu_run(self._h, nsteps))
 
+    def set_dt(self, dt: float):
+        self._check(lib().torchgpu_set_dt(self._h, dt))
+
+    def stage(self, stage: int):
+        self._check(lib().torchgpu_hydro_stage(self._h, stage))
+
     def reset_clock(self):
         self._check(lib().torchgpu_reset_clock(self._h))
 
@@ -189,3 +197,99 @@ class Grid:
         t, dt, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
         self._check(lib().torchgpu_get_clock(self._h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n)))
         return t.value, dt.value, n.value
+
+
+# -- cooling operator -------------------------------------------------------
+class ThermoParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "thermo_hii_switch", "heating_amplification", "mass_fraction_h", "gamma", "pfloor",
+        "hydrogen_mass", "spec_gas_const", "dust_xsec", "pi",
+        "z0", "T1", "T2", "T3", "T4", "imlc", "nmlc", "ciec_minT", "ciec", "cxhi_damp", "n0", "nmc",
+        "fuvh_a", "fuvh_b", "fuvh_c", "irh_a", "irh_b", "crh", "spl_min_slope", "spl_max_slope")] + [
+        ("spl_x", ctypes.c_double * 26), ("spl_y", ctypes.c_double * 26), ("spl_y2", ctypes.c_double * 26),
+        ("subcycling", ctypes.c_int), ("pad", ctypes.c_int)]
+
+
+def _bind_cooling(L):
+    vp, d, i = ctypes.c_void_p, ctypes.c_double, ctypes.c_int
+    L.torchgpu_units.argtypes = [d, d, d, d, d, d, d, i, ctypes.POINTER(d)]
+    L.torchgpu_thermo_params_init.argtypes = [ctypes.POINTER(ThermoParams), d, d, d, d, d, d, d, d, d, i]
+    L.torchgpu_cooling_create.argtypes = [ctypes.POINTER(ThermoParams), vp, ctypes.POINTER(vp)]
+    L.torchgpu_cooling_destroy.argtypes = [vp]
+    L.torchgpu_cooling_destroy.restype = None
+    L.torchgpu_cooling_last_error.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
+    L.torchgpu_thermo_cells.argtypes = [vp, ctypes.c_int64, vp, vp, ctypes.POINTER(ctypes.c_int64)]
+
+
+def units(scales, value, m, l, t, to_code=True) -> float:
+    L = lib()
+    if not hasattr(L, "_cooling_bound"):
+        _bind_cooling(L)
+        L._cooling_bound = True
+    out = ctypes.c_double()
+    rc = L.torchgpu_units(scales[0], scales[1], scales[2], value, m, l, t, int(to_code), ctypes.byref(out))
+    if rc:
+        raise TorchGpuError(rc, "units")
+    return out.value
+
+
+def thermo_params(scales=(1e-20, 1e-28, 1e11), thermo_hii_switch=1e-2, heating_amplification=1.0,
+                  mass_fraction_h=1.0, gamma=1.67, density_floor=1e-30, temperature_floor=0.1,
+                  subcycling=True) -> ThermoParams:
+    L = lib()
+    if not hasattr(L, "_cooling_bound"):
+        _bind_cooling(L)
+        L._cooling_bound = True
+    p = ThermoParams()
+    rc = L.torchgpu_thermo_params_init(ctypes.byref(p), scales[0], scales[1], scales[2], thermo_hii_switch,
+                                       heating_amplification, mass_fraction_h, gamma, density_floor,
+                                       temperature_floor, int(subcycling))
+    if rc:
+        raise TorchGpuError(rc, "thermo_params_init")
+    return p
+
+
+class Cooling:
+    """The cooling operator on the current GPU (torchgpu_cooling)."""
+
+    def __init__(self, params: ThermoParams, stream: int | None = None):
+        L = lib()
+        if not hasattr(L, "_cooling_bound"):
+            _bind_cooling(L)
+            L._cooling_bound = True
+        self.params = params
+        self._h = ctypes.c_void_p()
+        rc = L.torchgpu_cooling_create(ctypes.byref(params), ctypes.c_void_p(stream or 0), ctypes.byref(self._h))
+        if rc:
+            raise TorchGpuError(rc, ERRORS.get(rc, "cooling create failed"))
+
+    def _check(self, rc):
+        if rc:
+            buf = ctypes.create_string_buffer(1024)
+            lib().torchgpu_cooling_last_error(self._h, buf, 1024)
+            raise TorchGpuError(rc, buf.value.decode() or ERRORS.get(rc, "error"))
+
+    def close(self):
+        if self._h:
+            lib().torchgpu_cooling_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def thermo_cells(self, inp, out=None, count=True):
+        """inp: device float64 tensor [9][n]; returns out [5][n] (device) and
+        the total number of subcycles (if count)."""
+        import torch
+
+        n = inp.shape[1]
+        if out is None:
+            out = torch.empty((5, n), dtype=torch.float64, device=inp.device)
+        tot = ctypes.c_int64()
+        self._check(lib().torchgpu_thermo_cells(self._h, n, ctypes.c_void_p(inp.data_ptr()),
+                                                ctypes.c_void_p(out.data_ptr()),
+                                                ctypes.byref(tot) if count else None))
+        return out, tot.value

@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtorchgpu.so")
-SOURCES = ["hydro.hip", "thermo.hip", "mlp.hip", "api.hip"]
+SOURCES = ["hydro.hip", "thermo.hip", "mlp.hip", "api.hip", "units.cpp", "mlp_api.cpp"]
 ARCH = os.environ.get("TORCHGPU_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -38,7 +38,7 @@ def build(verbose: bool = False) -> str:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue
-        obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+        obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
         cmd = [HIPCC, *CFLAGS, "-c", path, "-o", obj]
         if verbose:

@@ -240,6 +240,23 @@ int torchgpu_hydro_step(torchgpu_ctx* c, double dt) {
     return TORCHGPU_OK;
 }
 
+int torchgpu_set_dt(torchgpu_ctx* c, double dt) {
+    if (!c) return TORCHGPU_ERR_INVALID_ARG;
+    HIP_TRY(c, tg::launch_step_begin(c->st, 0, dt, c->stream));
+    return TORCHGPU_OK;
+}
+
+int torchgpu_hydro_stage(torchgpu_ctx* c, int stage) {
+    if (!c || (stage != 1 && stage != 2)) return TORCHGPU_ERR_INVALID_ARG;
+    if (stage == 1)
+        HIP_TRY(c, tg::launch_hydro_stage(c->g, c->p.riemann_solver, c->p.precision, 1, c->U, c->U, c->Ustar, c->st,
+                                          c->stream));
+    else
+        HIP_TRY(c, tg::launch_hydro_stage(c->g, c->p.riemann_solver, c->p.precision, 2, c->Ustar, c->U, c->U, c->st,
+                                          c->stream));
+    return TORCHGPU_OK;
+}
+
 int torchgpu_run(torchgpu_ctx* c, int nsteps) {
     if (!c || nsteps < 0) return TORCHGPU_ERR_INVALID_ARG;
     for (int s = 0; s < nsteps; ++s) {

@@ -1,0 +1,631 @@
+// thermo.hip -- subcycled Henney heating/cooling source step on independent
+// cells, FP64 (reference Thermodynamics::preTimeStepCalculations +
+// integrate, src/Integrators/Thermodynamics.cpp:218-337), and the surrogate
+// path that replaces the subcycle by the MLP of mlp.hip.
+//
+// The per-cell work is an explicit subcycle of nsteps = ceil(dt/dti) rate
+// evaluations (:304-331); nsteps spans 1 .. ~1e8 across the cooling
+// workload, so a lane-per-cell launch would leave most lanes idle behind the
+// slowest lane of their wave.  The FP64 kernel is persistent instead: every
+// lane owns one cell at a time and advances it by one subcycle per loop
+// iteration; a lane whose cell is finished writes it out and takes the next
+// work item from a global counter (one wave-aggregated atomic per refill).
+// Work items may be an index list (the surrogate's fallback cells).
+//
+// Surrogate path (torchgpu_thermo_cells_surrogate), all on the device with
+// no host round trip:
+//   prepass   FP64 pre-step rates and subcycle count of every cell; cells
+//             with at most `nexact` subcycles are integrated exactly here,
+//             the rest are queued for the MLP with their normalised
+//             features (cells outside the training envelope go to the
+//             fallback queue)
+//   MLP       fused BF16-MFMA inference over the queue (mlp.hip)
+//   finish    T_sub = T0 10^y0 -> subcycled pressure -> RATE exactly as
+//             :334; cells whose estimated relative T error exceeds the gate
+//             are queued for the fallback
+//   fallback  the persistent FP64 kernel over the fallback queue
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+
+#include "../../include/torchgpu.h"
+#include "mlp.hpp"
+#include "thermo_math.hpp"
+#include "torchgpu_internal.hpp"
+
+namespace tg {
+
+__device__ __forceinline__ double smin_d(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double smax_d(double a, double b) { return (a < b) ? b : a; }
+
+struct CellState {
+    double pre, heat, rate_pre, pressure, subT;
+    double pre2temp, temp2pre, rate2dpre, dpre2rate, T_min, pf, T0;
+    CoolingCell cc;
+    long long remaining, nsub;
+};
+
+// Thermodynamics::preTimeStepCalculations for one non-wind cell followed by
+// the set-up of integrate (:266-311).  Returns false when the cell is done
+// (gate, no subcycling, or a single step).
+__device__ __forceinline__ bool thermo_init(const ThermoConsts& t, const double* sx, const double* sy,
+                                            const double* sy2, const double* in, long long stride, long long cell,
+                                            CellState& s, unsigned int* err) {
+    const double den = in[0 * stride + cell], pre = in[1 * stride + cell], hii = in[2 * stride + cell];
+    const double adv = in[3 * stride + cell], E = in[4 * stride + cell], T_min = in[5 * stride + cell];
+    const double col_den = in[6 * stride + cell], F = in[7 * stride + cell], dt = in[8 * stride + cell];
+    s.pre = pre;
+    s.pressure = pre;
+    s.heat = 0;
+    s.rate_pre = 0;
+    s.dpre2rate = 0;
+    s.remaining = 0;
+    s.nsub = 0;
+    s.T0 = 0;
+    if (adv < t.thermo_hii_switch) return false;  // :227-230, :275-280
+    const double X = t.X;
+    const double nH = X * den / t.hydrogen_mass;
+    const double x = hii;
+    const double mu_inv = X * (hii + 1.0) + (1.0 - X) * 0.25;
+    const double T = (pre / den) * (1.0 / mu_inv) / t.spec_gas_const;  // Fluid::calcTemperature
+    const double Av = 1.086 * t.dust_xsec * col_den;
+    s.T0 = T;
+    s.cc = cooling_cell(t, nH, x);
+    double rate = heating(t, nH, Av, F);
+    s.heat = rate;
+    rate = subtract_cooling(t, s.cc, sx, sy, sy2, rate, T);
+    rate = soft_landing(rate, T, T_min);
+    const double RATE = t.heating_amplification * rate;
+    s.rate_pre = RATE;
+    if (!t.subcycling) return false;
+    double dti = fabs(0.10 * E / RATE);
+    s.pre2temp = 1.0 / (mu_inv * t.spec_gas_const * den);
+    s.temp2pre = (mu_inv * t.spec_gas_const * den);
+    s.rate2dpre = smin_d(dt, dti) * (t.gamma - 1.0);
+    s.dpre2rate = 1.0 / s.rate2dpre;
+    s.T_min = T_min;
+    s.pf = smax_d(T_min * s.temp2pre, t.pfloor);
+    double pressure = pre + RATE * s.rate2dpre;
+    double subT = pressure * s.pre2temp;
+    if (pressure < t.pfloor || subT < T_min) {
+        subT = s.pf * s.pre2temp;
+        pressure = s.pf;
+    }
+    s.pressure = pressure;
+    s.subT = subT;
+    if (dt > dti) {
+        const double dtdti = dt / dti;
+        if (!(dtdti < 2147483647.0)) {
+            // the reference's int conversion (:307) is undefined here
+            atomicOr(err, ERR_THERMO_NSTEPS);
+            return false;
+        }
+        int nsteps = dtdti - (int)dtdti > 0 ? (int)(dtdti + 1.0) : (int)(dtdti + 0.5);
+        --nsteps;
+        s.remaining = nsteps;
+    }
+    return s.remaining > 0;
+}
+
+// One subcycle, :314-330.
+__device__ __forceinline__ void thermo_substep(const ThermoConsts& t, const double* sx, const double* sy,
+                                               const double* sy2, CellState& s) {
+    double r = subtract_cooling(t, s.cc, sx, sy, sy2, s.heat, s.subT);
+    r = t.heating_amplification * soft_landing(r, s.subT, s.T_min);
+    double pressure = s.pressure + r * s.rate2dpre;
+    double subT = pressure * s.pre2temp;
+    if (pressure < t.pfloor || subT < s.T_min) {
+        subT = s.pf * s.pre2temp;
+        pressure = s.pf;
+    }
+    s.pressure = pressure;
+    s.subT = subT;
+    s.nsub += 1;
+    s.remaining -= 1;
+}
+
+__device__ __forceinline__ void thermo_finish(const CellState& s, double* out, long long stride, long long cell) {
+    out[0 * stride + cell] = s.heat;
+    out[1 * stride + cell] = s.rate_pre;
+    // :334 (gate or no subcycling: RATE as preTimeStep left it)
+    out[2 * stride + cell] = s.dpre2rate == 0 ? s.rate_pre : (s.pressure - s.pre) * s.dpre2rate;
+    out[3 * stride + cell] = (double)s.nsub;
+    out[4 * stride + cell] = s.pressure;
+}
+
+__device__ __forceinline__ long long shfl_ll(long long v, int src) {
+    int lo = __shfl((int)(v & 0xffffffffll), src, 64);
+    int hi = __shfl((int)(v >> 32), src, 64);
+    return ((long long)hi << 32) | (unsigned int)lo;
+}
+
+// Wave-aggregated append: returns this lane's slot in list (only for lanes
+// with pred true).
+__device__ __forceinline__ long long wave_append(bool pred, unsigned long long* counter) {
+    const unsigned long long mask = __ballot(pred);
+    if (!mask) return -1;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)mask) - 1;
+    long long base = 0;
+    if (lane == leader) base = (long long)atomicAdd(counter, (unsigned long long)__popcll(mask));
+    base = shfl_ll(base, leader);
+    return pred ? base + __popcll(mask & ((1ull << lane) - 1ull)) : -1;
+}
+
+__device__ __forceinline__ void load_tables(const double* tables, double* sx, double* sy, double* sy2) {
+    if (threadIdx.x < 26) {
+        sx[threadIdx.x] = tables[threadIdx.x];
+        sy[threadIdx.x] = tables[26 + threadIdx.x];
+        sy2[threadIdx.x] = tables[52 + threadIdx.x];
+    }
+    __syncthreads();
+}
+
+// counters[0]: work counter; counters[1]: total subcycles.  Work items are
+// 0..count-1 (count = *count_dev when given), cell = order ? order[i] : i.
+__global__ __launch_bounds__(256) void thermo_cells_kernel(ThermoConsts t, const double* __restrict__ tables,
+                                                          long long stride, long long count,
+                                                          const unsigned long long* __restrict__ count_dev,
+                                                          const long long* __restrict__ order,
+                                                          const double* __restrict__ in, double* __restrict__ out,
+                                                          unsigned long long* __restrict__ counters,
+                                                          unsigned int* __restrict__ err, unsigned char* path) {
+    __shared__ double sx[26], sy[26], sy2[26];
+    load_tables(tables, sx, sy, sy2);
+    if (count_dev) count = (long long)*count_dev;
+    const int lane = threadIdx.x & 63;
+    bool active = false, exhausted = false;
+    long long cell = 0;
+    unsigned long long nsub_total = 0;
+    CellState s;
+    while (true) {
+        const unsigned long long need = __ballot(!active && !exhausted);
+        if (need) {
+            const int leader = __ffsll((long long)need) - 1;
+            long long base = 0;
+            if (lane == leader) base = (long long)atomicAdd(&counters[0], (unsigned long long)__popcll(need));
+            base = shfl_ll(base, leader);
+            if (!active && !exhausted) {
+                const long long idx = base + __popcll(need & ((1ull << lane) - 1ull));
+                if (idx < count) {
+                    cell = order ? order[idx] : idx;
+                    active = thermo_init(t, sx, sy, sy2, in, stride, cell, s, err);
+                    if (path) path[cell] = order ? 2 : 0;
+                    if (!active) thermo_finish(s, out, stride, cell);
+                } else {
+                    exhausted = true;
+                }
+            }
+        }
+        if (!__any(active) && !__any(!exhausted)) break;
+        if (active) {
+            thermo_substep(t, sx, sy, sy2, s);
+            if (s.remaining <= 0) {
+                thermo_finish(s, out, stride, cell);
+                nsub_total += (unsigned long long)s.nsub;
+                active = false;
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned int lo = __shfl_xor((unsigned int)(nsub_total & 0xffffffffull), off, 64);
+        unsigned int hi = __shfl_xor((unsigned int)(nsub_total >> 32), off, 64);
+        nsub_total += ((unsigned long long)hi << 32) | lo;
+    }
+    if (lane == 0) atomicAdd(&counters[1], nsub_total);
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
+// ---------------------------------------------------------------------------
+// surrogate
+// ---------------------------------------------------------------------------
+struct SurrogateConsts {
+    float mu[8], isd[8];
+    double lo[6], hi[6];
+    double gate;
+    int nexact;
+    int train;  // 1: queue every cell past nexact (no envelope), for training data
+};
+
+// Per-queued-cell data the finish step needs (FP64).
+struct SurrCell {
+    double T0, pre, temp2pre, pf, dpre2rate, T_min, heat, rate_pre;
+};
+
+// Raw features u (code units): log10 rho, log10 T, x, log10 F_FUV, Av, log10 dt.
+__device__ __forceinline__ void raw_features(const ThermoConsts& t, const double* in, long long stride, long long cell,
+                                             double T0, double u[6]) {
+    u[0] = log10(in[0 * stride + cell]);
+    u[1] = log10(T0);
+    u[2] = in[2 * stride + cell];
+    u[3] = log10(in[7 * stride + cell]);
+    u[4] = 1.086 * t.dust_xsec * in[6 * stride + cell];
+    u[5] = log10(in[8 * stride + cell]);
+}
+
+// counters: [2] surrogate queue length, [3] fallback queue length
+__global__ __launch_bounds__(256) void surrogate_prepass_kernel(ThermoConsts t, const double* __restrict__ tables,
+                                                               SurrogateConsts sc, long long n,
+                                                               const double* __restrict__ in, double* __restrict__ out,
+                                                               unsigned char* path, long long* __restrict__ mlp_list,
+                                                               float* __restrict__ feat, SurrCell* __restrict__ sdata,
+                                                               float* __restrict__ raw_out,
+                                                               long long* __restrict__ fb_list,
+                                                               unsigned long long* __restrict__ counters,
+                                                               unsigned int* __restrict__ err) {
+    __shared__ double sx[26], sy[26], sy2[26];
+    load_tables(tables, sx, sy, sy2);
+    const long long nloop = ((n + blockDim.x - 1) / blockDim.x) * blockDim.x;  // uniform trip count per wave
+    for (long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x; cell - threadIdx.x < nloop;
+         cell += (long long)gridDim.x * blockDim.x) {
+        const bool valid = cell < n;
+        bool to_mlp = false, to_fb = false;
+        CellState s;
+        double u[6];
+        if (valid) {
+            bool active = thermo_init(t, sx, sy, sy2, in, n, cell, s, err);
+            if (active && s.remaining < sc.nexact) {
+                while (s.remaining > 0) thermo_substep(t, sx, sy, sy2, s);
+                active = false;
+            }
+            if (!active) {
+                thermo_finish(s, out, n, cell);
+                if (path) path[cell] = 0;
+            } else {
+                raw_features(t, in, n, cell, s.T0, u);
+                bool inside = true;
+#pragma unroll
+                for (int i = 0; i < 6; ++i) inside &= (u[i] >= sc.lo[i]) && (u[i] <= sc.hi[i]);
+                to_mlp = inside || sc.train;
+                to_fb = !to_mlp;
+            }
+        }
+        const long long slot = wave_append(to_mlp, &counters[2]);
+        const long long fslot = wave_append(to_fb, &counters[3]);
+        if (to_mlp) {
+            mlp_list[slot] = cell;
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                feat[slot * 8 + i] = i < 6 ? (float)(u[i] - (double)sc.mu[i]) * sc.isd[i] : 0.f;
+            if (raw_out) {
+#pragma unroll
+                for (int i = 0; i < 6; ++i) raw_out[slot * 8 + i] = (float)u[i];
+                raw_out[slot * 8 + 6] = (float)s.nsub;
+                raw_out[slot * 8 + 7] = (float)s.remaining;
+            }
+            SurrCell d;
+            d.T0 = s.T0;
+            d.pre = s.pre;
+            d.temp2pre = s.temp2pre;
+            d.pf = s.pf;
+            d.dpre2rate = s.dpre2rate;
+            d.T_min = s.T_min;
+            d.heat = s.heat;
+            d.rate_pre = s.rate_pre;
+            sdata[slot] = d;
+            if (path) path[cell] = 1;
+        }
+        if (to_fb) {
+            fb_list[fslot] = cell;
+            if (path) path[cell] = 2;
+        }
+    }
+}
+
+// y (fp32 [count][3]) -> subcycled pressure and RATE; gated cells -> fallback.
+__global__ void surrogate_finish_kernel(ThermoConsts t, SurrogateConsts sc, long long stride,
+                                        const unsigned long long* __restrict__ counters_in,
+                                        const long long* __restrict__ mlp_list, const SurrCell* __restrict__ sdata,
+                                        const float* __restrict__ y, double* __restrict__ out, unsigned char* path,
+                                        long long* __restrict__ fb_list, unsigned long long* __restrict__ counters) {
+    const long long count = (long long)counters_in[2];
+    const long long nloop = ((count + blockDim.x - 1) / blockDim.x) * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i - threadIdx.x < nloop;
+         i += (long long)gridDim.x * blockDim.x) {
+        bool to_fb = false;
+        long long cell = 0;
+        if (i < count) {
+            cell = mlp_list[i];
+            const SurrCell d = sdata[i];
+            const double y0 = (double)y[i * 3 + 0];
+            const double est = exp10((double)y[i * 3 + 2]);
+            if (!(est <= sc.gate) || !isfinite(y0)) {
+                to_fb = true;
+            } else {
+                const double Tsub = d.T0 * exp10(y0);
+                double p = Tsub * d.temp2pre;
+                if (p < t.pfloor || Tsub < d.T_min) p = d.pf;
+                out[0 * stride + cell] = d.heat;
+                out[1 * stride + cell] = d.rate_pre;
+                out[2 * stride + cell] = (p - d.pre) * d.dpre2rate;
+                out[3 * stride + cell] = 0.0;
+                out[4 * stride + cell] = p;
+            }
+        }
+        const long long fslot = wave_append(to_fb, &counters[3]);
+        if (to_fb) {
+            fb_list[fslot] = cell;
+            if (path) path[cell] = 2;
+        }
+    }
+}
+
+// Training targets of the queued cells from the FP64 integrator's output:
+// t0 = log10(p_sub * pre2temp) - log10(T0) with pre2temp = 1/temp2pre.
+__global__ void surrogate_targets_kernel(const unsigned long long* __restrict__ counters, long long stride,
+                                         const long long* __restrict__ mlp_list, const SurrCell* __restrict__ sdata,
+                                         const double* __restrict__ out, float* __restrict__ target) {
+    const long long count = (long long)counters[2];
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (long long)gridDim.x * blockDim.x) {
+        const SurrCell d = sdata[i];
+        const double p = out[4 * stride + mlp_list[i]];
+        const double Tsub = p * (1.0 / d.temp2pre);
+        target[i * 3 + 0] = (float)(log10(Tsub) - log10(d.T0));
+        target[i * 3 + 1] = 0.f;
+        target[i * 3 + 2] = 0.f;
+    }
+}
+
+}  // namespace tg
+
+struct torchgpu_cooling {
+    torchgpu_thermo_params p;
+    tg::ThermoConsts t;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    double* tables = nullptr;                 // spline x, y, y2 (78 doubles)
+    unsigned long long* counters = nullptr;   // [0] work, [1] subcycles, [2] surrogate queue, [3] fallback queue
+    unsigned int* err = nullptr;
+    char msg[256] = {0};
+    int nsm = 256;
+    // surrogate
+    torchgpu_mlp* mlp = nullptr;
+    tg::SurrogateConsts sc{};
+    bool has_surrogate = false;
+    long long cap = 0;                        // capacity of the queues below
+    long long* mlp_list = nullptr;
+    long long* fb_list = nullptr;
+    float* feat = nullptr;
+    float* y = nullptr;
+    tg::SurrCell* sdata = nullptr;
+};
+
+// the MLP handle is defined in mlp_api.cpp; the cooling operator only needs
+// its device weights
+extern "C" int torchgpu_mlp_device_weights(torchgpu_mlp* m, const uint16_t** w16, const float** b);
+
+namespace {
+
+#define HIP_OK(call)                                 \
+    do {                                             \
+        hipError_t e_ = (call);                      \
+        if (e_ != hipSuccess) {                      \
+            if (c) snprintf(c->msg, sizeof c->msg, "%s: %s", #call, hipGetErrorString(e_)); \
+            return TORCHGPU_ERR_HIP;                 \
+        }                                            \
+    } while (0)
+
+tg::ThermoConsts to_consts(const torchgpu_thermo_params& p) {
+    tg::ThermoConsts t{};
+    t.thermo_hii_switch = p.thermo_hii_switch;
+    t.heating_amplification = p.heating_amplification;
+    t.X = p.mass_fraction_h;
+    t.gamma = p.gamma;
+    t.pfloor = p.pfloor;
+    t.hydrogen_mass = p.hydrogen_mass;
+    t.spec_gas_const = p.spec_gas_const;
+    t.dust_xsec = p.dust_xsec;
+    t.pi = p.pi;
+    t.z0 = p.z0; t.T1 = p.T1; t.T2 = p.T2; t.T3 = p.T3; t.T4 = p.T4;
+    t.imlc = p.imlc; t.nmlc = p.nmlc; t.ciec_minT = p.ciec_minT; t.ciec = p.ciec; t.cxhi_damp = p.cxhi_damp;
+    t.n0 = p.n0; t.nmc = p.nmc;
+    t.fuvh_a = p.fuvh_a; t.fuvh_b = p.fuvh_b; t.fuvh_c = p.fuvh_c; t.irh_a = p.irh_a; t.irh_b = p.irh_b;
+    t.crh = p.crh;
+    t.spl_min_slope = p.spl_min_slope;
+    t.spl_max_slope = p.spl_max_slope;
+    t.subcycling = p.subcycling;
+    return t;
+}
+
+int zero_counters(torchgpu_cooling* c) {
+    hipLaunchKernelGGL(tg::zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, c->counters, 4);
+    HIP_OK(hipGetLastError());
+    return TORCHGPU_OK;
+}
+
+int launch_fp64(torchgpu_cooling* c, long long stride, long long count, const unsigned long long* count_dev,
+                const long long* order, const double* in, double* out, unsigned char* path) {
+    long long blocks = (long long)c->nsm * 8;  // 32 waves per CU
+    if (!count_dev && blocks * 256 > count) blocks = (count + 255) / 256;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(tg::thermo_cells_kernel, dim3((unsigned)blocks), dim3(256), 0, c->stream, c->t, c->tables,
+                       stride, count, count_dev, order, in, out, c->counters, c->err, path);
+    HIP_OK(hipGetLastError());
+    return TORCHGPU_OK;
+}
+
+int ensure_queues(torchgpu_cooling* c, long long n) {
+    if (n <= c->cap) return TORCHGPU_OK;
+    (void)hipStreamSynchronize(c->stream);
+    if (c->mlp_list) (void)hipFree(c->mlp_list);
+    if (c->fb_list) (void)hipFree(c->fb_list);
+    if (c->feat) (void)hipFree(c->feat);
+    if (c->y) (void)hipFree(c->y);
+    if (c->sdata) (void)hipFree(c->sdata);
+    c->mlp_list = c->fb_list = nullptr;
+    c->feat = c->y = nullptr;
+    c->sdata = nullptr;
+    c->cap = 0;
+    HIP_OK(hipMalloc(&c->mlp_list, n * sizeof(long long)));
+    HIP_OK(hipMalloc(&c->fb_list, n * sizeof(long long)));
+    HIP_OK(hipMalloc(&c->feat, n * 8 * sizeof(float)));
+    HIP_OK(hipMalloc(&c->y, n * 3 * sizeof(float)));
+    HIP_OK(hipMalloc(&c->sdata, n * sizeof(tg::SurrCell)));
+    c->cap = n;
+    return TORCHGPU_OK;
+}
+
+int check_err(torchgpu_cooling* c) {
+    unsigned int e = 0;
+    HIP_OK(hipMemcpyAsync(&e, c->err, sizeof e, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    if (e & tg::ERR_THERMO_NSTEPS) {
+        snprintf(c->msg, sizeof c->msg, "Thermodynamics::integrate: subcycle count out of int range");
+        return TORCHGPU_ERR_NSTEPS_RANGE;
+    }
+    return TORCHGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int torchgpu_cooling_create(const torchgpu_thermo_params* p, void* stream, torchgpu_cooling** out) {
+    if (!p || !out) return TORCHGPU_ERR_INVALID_ARG;
+    *out = nullptr;
+    torchgpu_cooling* c = new torchgpu_cooling();
+    c->p = *p;
+    c->t = to_consts(*p);
+    if (stream) {
+        c->stream = (hipStream_t)stream;
+    } else {
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return TORCHGPU_ERR_HIP; }
+        c->own_stream = true;
+    }
+    double tab[78];
+    for (int i = 0; i < 26; ++i) { tab[i] = p->spl_x[i]; tab[26 + i] = p->spl_y[i]; tab[52 + i] = p->spl_y2[i]; }
+    hipDeviceProp_t prop;
+    int dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+        c->nsm = prop.multiProcessorCount;
+    unsigned int zero = 0;
+    if (hipMalloc(&c->tables, sizeof tab) != hipSuccess || hipMalloc(&c->counters, 4 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&c->err, sizeof(unsigned int)) != hipSuccess ||
+        hipMemcpy(c->tables, tab, sizeof tab, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->err, &zero, sizeof zero, hipMemcpyHostToDevice) != hipSuccess) {
+        torchgpu_cooling_destroy(c);
+        return TORCHGPU_ERR_HIP;
+    }
+    *out = c;
+    return TORCHGPU_OK;
+}
+
+void torchgpu_cooling_destroy(torchgpu_cooling* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void* ptr : {(void*)c->tables, (void*)c->counters, (void*)c->err, (void*)c->mlp_list, (void*)c->fb_list,
+                      (void*)c->feat, (void*)c->y, (void*)c->sdata})
+        if (ptr) (void)hipFree(ptr);
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int torchgpu_cooling_last_error(const torchgpu_cooling* c, char* buf, size_t len) {
+    if (!c || !buf || !len) return TORCHGPU_ERR_INVALID_ARG;
+    snprintf(buf, len, "%s", c->msg);
+    return TORCHGPU_OK;
+}
+
+int torchgpu_thermo_cells(torchgpu_cooling* c, int64_t n, const double* in, double* out, int64_t* total_substeps) {
+    if (!c || n < 0 || (n > 0 && (!in || !out))) return TORCHGPU_ERR_INVALID_ARG;
+    if (n == 0) {
+        if (total_substeps) *total_substeps = 0;
+        return TORCHGPU_OK;
+    }
+    int rc = zero_counters(c);
+    if (rc) return rc;
+    rc = launch_fp64(c, n, n, nullptr, nullptr, in, out, nullptr);
+    if (rc) return rc;
+    if (total_substeps) {
+        unsigned long long tot = 0;
+        HIP_OK(hipMemcpyAsync(&tot, c->counters + 1, sizeof tot, hipMemcpyDeviceToHost, c->stream));
+        rc = check_err(c);
+        *total_substeps = (int64_t)tot;
+        if (rc) return rc;
+    }
+    return TORCHGPU_OK;
+}
+
+int torchgpu_cooling_set_surrogate(torchgpu_cooling* c, torchgpu_mlp* m, const torchgpu_surrogate_cfg* cfg) {
+    if (!c || !cfg) return TORCHGPU_ERR_INVALID_ARG;
+    c->mlp = m;
+    for (int i = 0; i < 8; ++i) { c->sc.mu[i] = cfg->mu[i]; c->sc.isd[i] = cfg->isd[i]; }
+    for (int i = 0; i < 6; ++i) { c->sc.lo[i] = cfg->lo[i]; c->sc.hi[i] = cfg->hi[i]; }
+    c->sc.gate = cfg->gate;
+    c->sc.nexact = cfg->nexact;
+    c->sc.train = 0;
+    c->has_surrogate = (m != nullptr);
+    return TORCHGPU_OK;
+}
+
+int torchgpu_thermo_cells_surrogate(torchgpu_cooling* c, int64_t n, const double* in, double* out, uint8_t* path,
+                                    int64_t* counts) {
+    if (!c || n < 0 || (n > 0 && (!in || !out))) return TORCHGPU_ERR_INVALID_ARG;
+    if (!c->has_surrogate) return TORCHGPU_ERR_NOT_TRAINED;
+    if (n == 0) return TORCHGPU_OK;
+    int rc = ensure_queues(c, n);
+    if (rc) return rc;
+    if ((rc = zero_counters(c))) return rc;
+    const uint16_t* w16 = nullptr;
+    const float* b = nullptr;
+    if ((rc = torchgpu_mlp_device_weights(c->mlp, &w16, &b))) return rc;
+    tg::SurrogateConsts sc = c->sc;
+    sc.train = 0;
+    long long nb = (n + 255) / 256;
+    if (nb > (long long)c->nsm * 16) nb = (long long)c->nsm * 16;
+    hipLaunchKernelGGL(tg::surrogate_prepass_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t, c->tables, sc,
+                       (long long)n, in, out, path, c->mlp_list, c->feat, c->sdata, (float*)nullptr, c->fb_list,
+                       c->counters, c->err);
+    HIP_OK(hipGetLastError());
+    HIP_OK(tg::launch_mlp_infer_dev(w16, b, c->feat, c->counters + 2, n, c->y, c->nsm, c->stream));
+    hipLaunchKernelGGL(tg::surrogate_finish_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t, sc,
+                       (long long)n, c->counters, c->mlp_list, c->sdata, c->y, out, path, c->fb_list, c->counters);
+    HIP_OK(hipGetLastError());
+    // fallback queue through the FP64 integrator (work counter reset first)
+    hipLaunchKernelGGL(tg::zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, c->counters, 1);
+    HIP_OK(hipGetLastError());
+    if ((rc = launch_fp64(c, n, n, c->counters + 3, c->fb_list, in, out, path))) return rc;
+    if (counts) {
+        unsigned long long h[4];
+        HIP_OK(hipMemcpyAsync(h, c->counters, sizeof h, hipMemcpyDeviceToHost, c->stream));
+        if ((rc = check_err(c))) return rc;
+        counts[0] = (int64_t)h[2];
+        counts[1] = (int64_t)h[3];
+    }
+    return TORCHGPU_OK;
+}
+
+int torchgpu_surrogate_training_data(torchgpu_cooling* c, int64_t n, const double* in, const double* out,
+                                     int64_t* list, float* feat, float* target, float* raw, int64_t* count) {
+    if (!c || n <= 0 || !in || !out || !list || !feat || !target || !count) return TORCHGPU_ERR_INVALID_ARG;
+    int rc = ensure_queues(c, n);
+    if (rc) return rc;
+    if ((rc = zero_counters(c))) return rc;
+    tg::SurrogateConsts sc = c->sc;
+    sc.train = 1;
+    // the prepass rewrites `out` for the exactly-integrated cells with the
+    // same FP64 values the integrator produced; a scratch copy keeps `out`
+    // untouched
+    double* scratch = nullptr;
+    HIP_OK(hipMalloc(&scratch, 5 * n * sizeof(double)));
+    long long nb = (n + 255) / 256;
+    if (nb > (long long)c->nsm * 16) nb = (long long)c->nsm * 16;
+    hipLaunchKernelGGL(tg::surrogate_prepass_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t, c->tables, sc,
+                       (long long)n, in, scratch, (unsigned char*)nullptr, (long long*)list, feat, c->sdata, raw,
+                       c->fb_list, c->counters, c->err);
+    HIP_OK(hipGetLastError());
+    hipLaunchKernelGGL(tg::surrogate_targets_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, c->counters,
+                       (long long)n, (const long long*)list, c->sdata, out, target);
+    HIP_OK(hipGetLastError());
+    unsigned long long h[4];
+    HIP_OK(hipMemcpyAsync(h, c->counters, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    (void)hipFree(scratch);
+    *count = (int64_t)h[2];
+    return check_err(c);
+}
+
+}  // extern "C"
