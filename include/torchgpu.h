@@ -134,6 +134,12 @@ int torchgpu_hydro_step(torchgpu_ctx* ctx, double dt);
 int torchgpu_hydro_stage(torchgpu_ctx* ctx, int stage);
 /* Sets the dt used by torchgpu_hydro_stage. */
 int torchgpu_set_dt(torchgpu_ctx* ctx, double dt);
+/* As torchgpu_hydro_stage with the scaled dt given (dtf = dt/2.0 for the
+ * predictor, dt for the corrector); stage 0 is the single-stage sub-step of
+ * an operator-split step (Torch::subStep with Hydrodynamics,
+ * Torch.cpp:352-364): U <- fixSolution(U + dtf * L(fixPrimitives(QfromU(U)))).
+ * The state pointer of torchgpu_state_ptr may change after stage 0. */
+int torchgpu_hydro_stage_dt(torchgpu_ctx* ctx, int stage, double dtf);
 /* Up to `nsteps` iterations of the Torch::run loop for a hydro-only run,
  * with dt chosen on the device (first step dt_max * 1e-20, then the CFL
  * min-reduction fused into the corrector stage, clipped to the next
@@ -193,6 +199,21 @@ int torchgpu_cooling_last_error(const torchgpu_cooling* c, char* buf, size_t len
  * total_substeps (host, may be NULL) receives the sum of subcycle counts
  * (this synchronizes). */
 int torchgpu_thermo_cells(torchgpu_cooling* c, int64_t n, const double* in, double* out, int64_t* total_substeps);
+
+/* Thermodynamics on the grid state of ctx (operator-split step).  Per-cell
+ * inputs come from U (Q = fixPrimitives(QfromU(U)), E = U[PRE]); T_min,
+ * col_den (T[COL_DEN]) and F_FUV are device arrays [n] or NULL (100 K, 0,
+ * 0).  `excluded` is the flat index of the star's cell (CausalWind,
+ * Fluid.cpp:41-54) or -1. */
+/* Thermodynamics::calculateTimeStep (Thermodynamics.cpp:463-476) after
+ * preTimeStepCalculations on the current state. */
+int torchgpu_thermo_grid_timestep(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* T_min, const double* col_den,
+                                  const double* F_FUV, int64_t excluded, double dt_max, double* dt);
+/* The thermo part of Torch::subStep (Torch.cpp:352-364): preTimeStep +
+ * integrate(dt) (subcycled FP64, or the surrogate when `surrogate`) +
+ * updateSourceTerms + Fluid::advSolution(dt) + fixSolution. */
+int torchgpu_thermo_grid_substep(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* T_min, const double* col_den,
+                                 const double* F_FUV, int64_t excluded, double dt, int surrogate);
 
 /* ------------------------------------------------------------------------ */
 /* Cooling surrogate (no reference counterpart: a learned replacement of the  */

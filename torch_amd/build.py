@@ -58,7 +58,26 @@ def build(verbose: bool = False) -> str:
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
+
+
+HOST_SOURCES = ["lua_config.cpp", "torch_gpu.cpp", "torch_gpu_main.cpp"]
+HOST_EXE = os.path.join(HERE, "torch_gpu")
+
+
+def build_host(verbose: bool = False) -> str:
+    """The C++ host layer and command-line driver, linked against the C ABI."""
+    srcs = [os.path.join(HERE, "host", s) for s in HOST_SOURCES]
+    rocm = os.path.dirname(os.path.dirname(HIPCC))
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__", f"-I{os.path.dirname(HERE)}/include",
+           *srcs, "-o", HOST_EXE + ".tmp", f"-L{HERE}", "-ltorchgpu", f"-Wl,-rpath,$ORIGIN",
+           f"-Wl,-rpath,{rocm}/lib", f"-L{rocm}/lib", "-lamdhip64"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(HOST_EXE + ".tmp", HOST_EXE)
+    return HOST_EXE
 
 
 if __name__ == "__main__":

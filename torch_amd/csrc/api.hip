@@ -4,6 +4,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <utility>
 
 #include "../../include/torchgpu.h"
 #include "torchgpu_internal.hpp"
@@ -73,6 +74,8 @@ int reset_clock(torchgpu_ctx* c) {
 }
 
 }  // namespace
+
+tg::CtxView tg::ctx_view(torchgpu_ctx* c) { return tg::CtxView{c->U, c->Ustar, c->st, c->stream, &c->g}; }
 
 extern "C" {
 
@@ -254,6 +257,22 @@ int torchgpu_hydro_stage(torchgpu_ctx* c, int stage) {
     else
         HIP_TRY(c, tg::launch_hydro_stage(c->g, c->p.riemann_solver, c->p.precision, 2, c->Ustar, c->U, c->U, c->st,
                                           c->stream));
+    return TORCHGPU_OK;
+}
+
+int torchgpu_hydro_stage_dt(torchgpu_ctx* c, int stage, double dtf) {
+    if (!c || stage < 0 || stage > 2 || !(dtf >= 0.0)) return TORCHGPU_ERR_INVALID_ARG;
+    const int solver = c->p.riemann_solver, prec = c->p.precision;
+    if (stage == 1) {
+        HIP_TRY(c, tg::launch_hydro_stage(c->g, solver, prec, 1, c->U, c->U, c->Ustar, c->st, c->stream, dtf));
+    } else if (stage == 2) {
+        HIP_TRY(c, tg::launch_hydro_stage(c->g, solver, prec, 2, c->Ustar, c->U, c->U, c->st, c->stream, dtf));
+    } else {
+        // single-stage sub-step (Torch::subStep with the hydro component):
+        // written out of place, then the two state buffers swap roles
+        HIP_TRY(c, tg::launch_hydro_stage(c->g, solver, prec, 1, c->U, c->U, c->Ustar, c->st, c->stream, dtf));
+        std::swap(c->U, c->Ustar);
+    }
     return TORCHGPU_OK;
 }
 

@@ -322,9 +322,12 @@ __device__ __forceinline__ void sweep(const HydroGeom& g, TileLDS<R>& s, const d
     }
 }
 
+// dtf_host >= 0 overrides the stage's dt (st->dt / 2.0 for the predictor,
+// st->dt for the corrector) with a host-given, already scaled dt.
 template <int SOLVER, typename R, int STAGE>
 __global__ __launch_bounds__(NTHREADS) void hydro_stage_kernel(HydroGeom g, const double* __restrict__ Usrc,
-                                                              const double* Ubase, double* Udst, StepState* st) {
+                                                              const double* Ubase, double* Udst, StepState* st,
+                                                              double dtf_host) {
     __shared__ TileLDS<R> s;
     if (st->halted) return;
     const int tile = tile_of_block(blockIdx.x, g.ntiles);
@@ -360,7 +363,7 @@ __global__ __launch_bounds__(NTHREADS) void hydro_stage_kernel(HydroGeom g, cons
             udot[PRE] += (double)s.qc[VEL + i][tid] * grav;
         }
         const long long c = (long long)gx + (long long)g.nx * ((long long)gy + (long long)g.ny * gz);
-        const double dtf = STAGE == 1 ? st->dt / 2.0 : st->dt;
+        const double dtf = dtf_host >= 0.0 ? dtf_host : (STAGE == 1 ? st->dt / 2.0 : st->dt);
         double u[7];
 #pragma unroll
         for (int v = 0; v < 7; ++v) u[v] = Ubase[v * g.ncell + c];
@@ -475,30 +478,57 @@ __global__ void u_from_q_kernel(HydroGeom g, double* __restrict__ Q, double* __r
     }
 }
 
+// Thermodynamics::updateSourceTerms + Fluid::advSolution + fixSolution on
+// the grid: UDOT[PRE] = rate (other UDOT zero), U += dtf * UDOT, then
+// fixSolution (Thermodynamics.cpp:478-485, Fluid.cpp:80-136).
+__global__ void apply_rate_fix_kernel(HydroGeom g, double* __restrict__ U, const double* __restrict__ rate, double dtf,
+                                      StepState* st) {
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < g.ncell; c += (long long)gridDim.x * blockDim.x) {
+        double u[7];
+#pragma unroll
+        for (int v = 0; v < 7; ++v) u[v] = U[v * g.ncell + c];
+        double udot[7] = {0, 0, 0, 0, 0, 0, 0};
+        udot[PRE] += rate ? rate[c] : 0.0;
+#pragma unroll
+        for (int v = 0; v < 7; ++v) u[v] += dtf * udot[v];
+        if (fix_solution(g, u)) set_error(st, ERR_THERMO_FIX);
+#pragma unroll
+        for (int v = 0; v < 7; ++v) U[v * g.ncell + c] = u[v];
+    }
+}
+
+hipError_t launch_fix_solution(const HydroGeom& g, double* U, const double* rate, double dtf, StepState* st,
+                               hipStream_t stream) {
+    long long nb = (g.ncell + 255) / 256;
+    if (nb > 65536) nb = 65536;
+    hipLaunchKernelGGL(apply_rate_fix_kernel, dim3((unsigned)nb), dim3(256), 0, stream, g, U, rate, dtf, st);
+    return hipGetLastError();
+}
+
 template <int SOLVER, typename R>
 static hipError_t launch_stage_t(const HydroGeom& g, int stage, const double* Usrc, const double* Ubase, double* Udst,
-                                 StepState* st, hipStream_t stream) {
+                                 StepState* st, double dtf, hipStream_t stream) {
     dim3 grid(g.ntiles), block(NTHREADS);
     if (stage == 1)
-        hipLaunchKernelGGL((hydro_stage_kernel<SOLVER, R, 1>), grid, block, 0, stream, g, Usrc, Ubase, Udst, st);
+        hipLaunchKernelGGL((hydro_stage_kernel<SOLVER, R, 1>), grid, block, 0, stream, g, Usrc, Ubase, Udst, st, dtf);
     else
-        hipLaunchKernelGGL((hydro_stage_kernel<SOLVER, R, 2>), grid, block, 0, stream, g, Usrc, Ubase, Udst, st);
+        hipLaunchKernelGGL((hydro_stage_kernel<SOLVER, R, 2>), grid, block, 0, stream, g, Usrc, Ubase, Udst, st, dtf);
     return hipGetLastError();
 }
 
 hipError_t launch_hydro_stage(const HydroGeom& g, int solver, int precision, int stage, const double* Usrc,
-                              const double* Ubase, double* Udst, StepState* st, hipStream_t stream) {
+                              const double* Ubase, double* Udst, StepState* st, hipStream_t stream, double dtf) {
     if (precision == 0) {
         switch (solver) {
-        case SOLVER_HLL:  return launch_stage_t<SOLVER_HLL, double>(g, stage, Usrc, Ubase, Udst, st, stream);
-        case SOLVER_HLLC: return launch_stage_t<SOLVER_HLLC, double>(g, stage, Usrc, Ubase, Udst, st, stream);
-        default:          return launch_stage_t<SOLVER_ROTATED, double>(g, stage, Usrc, Ubase, Udst, st, stream);
+        case SOLVER_HLL:  return launch_stage_t<SOLVER_HLL, double>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
+        case SOLVER_HLLC: return launch_stage_t<SOLVER_HLLC, double>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
+        default:          return launch_stage_t<SOLVER_ROTATED, double>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
         }
     }
     switch (solver) {
-    case SOLVER_HLL:  return launch_stage_t<SOLVER_HLL, float>(g, stage, Usrc, Ubase, Udst, st, stream);
-    case SOLVER_HLLC: return launch_stage_t<SOLVER_HLLC, float>(g, stage, Usrc, Ubase, Udst, st, stream);
-    default:          return launch_stage_t<SOLVER_ROTATED, float>(g, stage, Usrc, Ubase, Udst, st, stream);
+    case SOLVER_HLL:  return launch_stage_t<SOLVER_HLL, float>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
+    case SOLVER_HLLC: return launch_stage_t<SOLVER_HLLC, float>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
+    default:          return launch_stage_t<SOLVER_ROTATED, float>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
     }
 }
 

@@ -27,8 +27,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstring>
 
 #include "../../include/torchgpu.h"
+#include "hydro_math.hpp"
 #include "mlp.hpp"
 #include "thermo_math.hpp"
 #include "torchgpu_internal.hpp"
@@ -369,6 +371,77 @@ __global__ void surrogate_targets_kernel(const unsigned long long* __restrict__ 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Thermodynamics on the grid state (operator-split step)
+// ---------------------------------------------------------------------------
+// Per-cell inputs of the cooling kernels from the grid: Q =
+// fixPrimitives(QfromU(U)) (Fluid.cpp:138-165), E = U[PRE], T_min, COL_DEN,
+// F_FUV (arrays or defaults) and dt.  The excluded cell (the star's cell,
+// which Fluid::initialiseGrid puts in CausalWind, Fluid.cpp:41-54) gets
+// ADV = -1 so that the gate makes it a no-op.
+__global__ void thermo_gather_kernel(HydroGeom g, const double* __restrict__ U, const double* __restrict__ T_min,
+                                     double T_min_default, const double* __restrict__ col_den,
+                                     const double* __restrict__ F, double dt, long long excluded,
+                                     double* __restrict__ in9) {
+    const long long n = g.ncell;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+        double u[7], q[7];
+#pragma unroll
+        for (int v = 0; v < 7; ++v) u[v] = U[v * n + c];
+        QfromU(q, u, g.gm1);
+        q[HII] = smax_d(smin_d(q[HII], 1.0), 0.0);
+        q[ADV] = smax_d(smin_d(q[ADV], 1.0), 0.0);
+        q[DEN] = smax_d(q[DEN], g.dfloor);
+        q[PRE] = smax_d(q[PRE], g.pfloor);
+        const double mu_inv = g.X * (q[HII] + 1.0) + g.mu_c;
+        const double temperature = q[PRE] / (mu_inv * g.Rgas * q[DEN]);
+        if (temperature < g.tfloor) q[PRE] = mu_inv * g.Rgas * u[DEN] * g.tfloor;
+        in9[0 * n + c] = q[DEN];
+        in9[1 * n + c] = q[PRE];
+        in9[2 * n + c] = q[HII];
+        in9[3 * n + c] = (c == excluded) ? -1.0 : q[ADV];
+        in9[4 * n + c] = u[PRE];
+        in9[5 * n + c] = T_min ? T_min[c] : T_min_default;
+        in9[6 * n + c] = col_den ? col_den[c] : 0.0;
+        in9[7 * n + c] = F ? F[c] : 0.0;
+        in9[8 * n + c] = dt;
+    }
+}
+
+// Thermodynamics::calculateTimeStep (Thermodynamics.cpp:463-476): min over
+// cells with RATE != 0 of |f U[PRE] / RATE|, f = 1 when subcycling else 0.1,
+// from the pre-step RATE; reduced as the bit pattern of a non-negative double.
+__global__ __launch_bounds__(256) void thermo_timestep_kernel(ThermoConsts t, const double* __restrict__ tables,
+                                                             long long n, const double* __restrict__ in9,
+                                                             unsigned long long* __restrict__ dtmin_bits,
+                                                             unsigned int* __restrict__ err) {
+    __shared__ double sx[26], sy[26], sy2[26];
+    __shared__ double red[4];
+    load_tables(tables, sx, sy, sy2);
+    double m = __longlong_as_double(0x7fefffffffffffffll);
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+        CellState s;
+        ThermoConsts t1 = t;
+        t1.subcycling = 0;  // pre-step rate only
+        (void)thermo_init(t1, sx, sy, sy2, in9, n, c, s, err);
+        if (s.rate_pre != 0) {
+            const double frac = t.subcycling ? 1.0 : 0.1;
+            const double dti = fabs(frac * in9[4 * n + c] / s.rate_pre);
+            if (dti < m) m = dti;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = fmin(fmin(red[0], red[1]), fmin(red[2], red[3]));
+        if (b >= 0.0) atomicMin(dtmin_bits, (unsigned long long)__double_as_longlong(b));
+    }
+}
+
+__global__ void set_u64_kernel(unsigned long long* p, unsigned long long v) { *p = v; }
+
 }  // namespace tg
 
 struct torchgpu_cooling {
@@ -391,6 +464,11 @@ struct torchgpu_cooling {
     float* feat = nullptr;
     float* y = nullptr;
     tg::SurrCell* sdata = nullptr;
+    // grid-operator scratch
+    long long grid_cap = 0;
+    double* in9 = nullptr;
+    double* out5 = nullptr;
+    unsigned long long* dtbits = nullptr;
 };
 
 // the MLP handle is defined in mlp_api.cpp; the cooling operator only needs
@@ -517,7 +595,7 @@ void torchgpu_cooling_destroy(torchgpu_cooling* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (void* ptr : {(void*)c->tables, (void*)c->counters, (void*)c->err, (void*)c->mlp_list, (void*)c->fb_list,
-                      (void*)c->feat, (void*)c->y, (void*)c->sdata})
+                      (void*)c->feat, (void*)c->y, (void*)c->sdata, (void*)c->in9, (void*)c->out5, (void*)c->dtbits})
         if (ptr) (void)hipFree(ptr);
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -626,6 +704,72 @@ int torchgpu_surrogate_training_data(torchgpu_cooling* c, int64_t n, const doubl
     (void)hipFree(scratch);
     *count = (int64_t)h[2];
     return check_err(c);
+}
+
+static int ensure_grid(torchgpu_cooling* c, long long n) {
+    if (n <= c->grid_cap) return TORCHGPU_OK;
+    (void)hipStreamSynchronize(c->stream);
+    if (c->in9) (void)hipFree(c->in9);
+    if (c->out5) (void)hipFree(c->out5);
+    c->in9 = c->out5 = nullptr;
+    c->grid_cap = 0;
+    HIP_OK(hipMalloc(&c->in9, 9 * n * sizeof(double)));
+    HIP_OK(hipMalloc(&c->out5, 5 * n * sizeof(double)));
+    if (!c->dtbits) HIP_OK(hipMalloc(&c->dtbits, sizeof(unsigned long long)));
+    c->grid_cap = n;
+    return TORCHGPU_OK;
+}
+
+static int gather(torchgpu_cooling* c, torchgpu_ctx* ctx, const double* T_min, const double* col_den, const double* F,
+                  long long excluded, double dt) {
+    tg::CtxView v = tg::ctx_view(ctx);
+    const long long n = v.g->ncell;
+    int rc = ensure_grid(c, n);
+    if (rc) return rc;
+    long long nb = (n + 255) / 256;
+    if (nb > 65536) nb = 65536;
+    hipLaunchKernelGGL(tg::thermo_gather_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, *v.g, v.U, T_min,
+                       100.0, col_den, F, dt, excluded, c->in9);
+    HIP_OK(hipGetLastError());
+    return TORCHGPU_OK;
+}
+
+int torchgpu_thermo_grid_timestep(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* T_min, const double* col_den,
+                                  const double* F, int64_t excluded, double dt_max, double* dt) {
+    if (!ctx || !c || !dt) return TORCHGPU_ERR_INVALID_ARG;
+    int rc = gather(c, ctx, T_min, col_den, F, excluded, 0.0);
+    if (rc) return rc;
+    const long long n = tg::ctx_view(ctx).g->ncell;
+    hipLaunchKernelGGL(tg::set_u64_kernel, dim3(1), dim3(1), 0, c->stream, c->dtbits, 0x7fefffffffffffffull);
+    long long nb = (n + 255) / 256;
+    if (nb > (long long)c->nsm * 16) nb = (long long)c->nsm * 16;
+    hipLaunchKernelGGL(tg::thermo_timestep_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t, c->tables, n,
+                       c->in9, c->dtbits, c->err);
+    HIP_OK(hipGetLastError());
+    unsigned long long bits = 0;
+    HIP_OK(hipMemcpyAsync(&bits, c->dtbits, sizeof bits, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    double m;
+    std::memcpy(&m, &bits, sizeof m);
+    // Thermodynamics.cpp:465-474: dt = dt_max, lowered by every smaller dti
+    *dt = (m < dt_max) ? m : dt_max;
+    return TORCHGPU_OK;
+}
+
+int torchgpu_thermo_grid_substep(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* T_min, const double* col_den,
+                                 const double* F, int64_t excluded, double dt, int surrogate) {
+    if (!ctx || !c || !(dt >= 0.0)) return TORCHGPU_ERR_INVALID_ARG;
+    if (surrogate && !c->has_surrogate) return TORCHGPU_ERR_NOT_TRAINED;
+    int rc = gather(c, ctx, T_min, col_den, F, excluded, dt);
+    if (rc) return rc;
+    tg::CtxView v = tg::ctx_view(ctx);
+    const long long n = v.g->ncell;
+    if (surrogate) rc = torchgpu_thermo_cells_surrogate(c, n, c->in9, c->out5, nullptr, nullptr);
+    else rc = torchgpu_thermo_cells(c, n, c->in9, c->out5, nullptr);
+    if (rc) return rc;
+    // Thermodynamics::updateSourceTerms + advSolution(dt) + fixSolution
+    HIP_OK(tg::launch_fix_solution(*v.g, v.U, c->out5 + 2 * n, dt, v.st, c->stream));
+    return TORCHGPU_OK;
 }
 
 }  // extern "C"

@@ -57,12 +57,30 @@ enum : unsigned int {
     ERR_THERMO_FIX = 64u,     // fixSolution throw after a thermo sub-step
 };
 
+// Device view of a grid context (api.hip), for operators implemented in
+// other translation units.
+struct CtxView {
+    double* U;
+    double* Ustar;
+    StepState* st;
+    hipStream_t stream;
+    const HydroGeom* g;
+};
+
+}  // namespace tg
+typedef struct torchgpu_ctx torchgpu_ctx;
+namespace tg {
+CtxView ctx_view(torchgpu_ctx* c);
+
 // Launchers (hydro.hip).  All enqueue on `stream` and return hipError_t.
+hipError_t launch_fix_solution(const HydroGeom& g, double* U, const double* dpre, double dtf, StepState* st,
+                               hipStream_t stream);
 // precision: 0 = FP64 everywhere, 1 = FP32 reconstruction + Riemann flux
 // with FP64 accumulation, pressure recovery and CFL.
+// dtf < 0: the stage takes its dt from st (dt/2 predictor, dt corrector)
 hipError_t launch_hydro_stage(const HydroGeom& g, int solver, int precision, int stage,
                               const double* Usrc, const double* Ubase, double* Udst,
-                              StepState* st, hipStream_t stream);
+                              StepState* st, hipStream_t stream, double dtf = -1.0);
 hipError_t launch_cfl(const HydroGeom& g, const double* U, StepState* st, hipStream_t stream);
 hipError_t launch_step_begin(StepState* st, int use_cfl, double dt_override, hipStream_t stream);
 hipError_t launch_step_end(StepState* st, hipStream_t stream);

@@ -1,0 +1,394 @@
+// torch_gpu.cpp -- host layer over the C ABI (see torch_gpu.hpp).
+#include "torch_gpu.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <sstream>
+#include <vector>
+
+namespace tg {
+
+namespace {
+
+void check_rc(int rc, const char* what, torchgpu_ctx* ctx = nullptr) {
+    if (rc == TORCHGPU_OK) return;
+    char buf[1024] = {0};
+    if (ctx) torchgpu_last_error(ctx, buf, sizeof buf);
+    throw std::runtime_error(std::string(what) + ": " + (buf[0] ? buf : ("error " + std::to_string(rc))));
+}
+
+// Converter (src/Torch/Converter.cpp:17-63) through the library.
+double convert(const TorchParameters& p, double v, double m, double l, double t, bool to_code) {
+    double out = 0;
+    check_rc(torchgpu_units(p.dscale, p.pscale, p.tscale, v, m, l, t, to_code ? 1 : 0, &out), "units");
+    return out;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// parameters
+// ---------------------------------------------------------------------------
+int riemannSolverId(const std::string& type) {  // RiemannSolverFactory::create, Riemann.cpp:246-257
+    if (type == "HLLC") return TORCHGPU_SOLVER_HLLC;
+    if (type == "RotatedHLLC" || type == "default") return TORCHGPU_SOLVER_ROTATED_HLLC;
+    if (type == "HLL") return TORCHGPU_SOLVER_HLL;
+    throw std::runtime_error("RiemannSolverFactory::create: unknown type: " + type + "\n");
+}
+
+int slopeLimiterId(const std::string& type) {  // SlopeLimiterFactory::create, SlopeLimiter.cpp:313-332
+    if (type == "monotonised_central") return TORCHGPU_LIM_MC;
+    if (type == "superbee") return TORCHGPU_LIM_SUPERBEE;
+    if (type == "minmod") return TORCHGPU_LIM_MINMOD;
+    if (type == "maxmod") return TORCHGPU_LIM_MAXMOD;
+    if (type == "leer") return TORCHGPU_LIM_LEER;
+    if (type == "ospre") return TORCHGPU_LIM_OSPRE;
+    if (type == "albada" || type == "default") return TORCHGPU_LIM_ALBADA;
+    throw std::runtime_error("SlopeLimiterFactory::create: unknown type: " + type + "\n");
+}
+
+int conditionId(const std::string& c) {  // Constants::Constants conditionParser, Constants.cpp:9-14
+    if (c == "free" || c == "default") return TORCHGPU_BC_FREE;
+    if (c == "reflecting") return TORCHGPU_BC_REFLECTING;
+    if (c == "outflow") return TORCHGPU_BC_OUTFLOW;
+    if (c == "inflow") return TORCHGPU_BC_INFLOW;
+    if (c == "periodic") return TORCHGPU_BC_PERIODIC;
+    // EnumParser falls back to "default" with a warning (Constants.hpp:28-45)
+    return TORCHGPU_BC_FREE;
+}
+
+double TorchParameters::toCodeUnits(double v, double m, double l, double t) const { return convert(*this, v, m, l, t, true); }
+
+void TorchParameters::initialise() {
+    // Constants::initialise_DPT + TorchParameters::initialise (Parameters.cpp:9-27)
+    tmax = toCodeUnits(tmax, 0, 0, 1);
+    dt_max = tmax / 10.0;
+    sideLength = toCodeUnits(sideLength, 0, 1, 0);
+    if (geometry == "spherical" || geometry == "cylindrical") leftBC[0] = "reflecting";
+    dfloor = toCodeUnits(dfloor, 1, -3, 0);
+    specificGasConstant = toCodeUnits(8.314462e7, 0, 2, -2);
+    pfloor = 0.1 * specificGasConstant * dfloor * tfloor;
+    photonRate = toCodeUnits(photonRate, 0, 0, -1);
+}
+
+torchgpu_params TorchParameters::gridParams() const {
+    if (geometry != "cartesian") throw std::runtime_error("only Cartesian geometry runs on the GPU path");
+    if (nd != 3) throw std::runtime_error("the GPU path runs 3-D grids (Grid.no_dimensions = 3)");
+    if (spatialOrder != 0 && spatialOrder != 1)
+        throw std::runtime_error("Hydrodynamics::calcFluxes: invalid order(=" + std::to_string(spatialOrder) +
+                                 "). Valid orders = {0, 1}.");
+    torchgpu_params g;
+    std::memset(&g, 0, sizeof g);
+    g.nx = ncells[0]; g.ny = ncells[1]; g.nz = ncells[2];
+    g.spatial_order = spatialOrder;
+    for (int d = 0; d < 3; ++d) {
+        g.bc_left[d] = conditionId(leftBC[d]);
+        g.bc_right[d] = conditionId(rightBC[d]);
+    }
+    // Torch::initialise keeps the default when the factory rejects the
+    // string (Torch.cpp:65-77)
+    try { g.riemann_solver = riemannSolverId(riemannSolver); } catch (std::exception&) { g.riemann_solver = TORCHGPU_SOLVER_ROTATED_HLLC; }
+    try { g.slope_limiter = slopeLimiterId(slopeLimiter); } catch (std::exception&) { g.slope_limiter = TORCHGPU_LIM_ALBADA; }
+    g.precision = (precision == "mixed") ? TORCHGPU_MIXED : TORCHGPU_FP64;
+    g.ncheckpoints = ncheckpoints;
+    g.side_length = sideLength;
+    g.gamma = heatCapacityRatio;
+    g.dfloor = dfloor;
+    g.pfloor = pfloor;
+    g.tfloor = tfloor;
+    g.mass_fraction_h = massFractionH;
+    g.spec_gas_const = specificGasConstant;
+    g.tmax = tmax;
+    g.dt_max = dt_max;
+    return g;
+}
+
+namespace {
+template <typename T>
+void take(const LuaValue& v, T& var, const std::string& name, std::string* warn) {
+    if (!v.exists()) {
+        if (warn) *warn += "parseLuaVariable: <" + name + "> does not exist, using default\n";
+        return;
+    }
+    if constexpr (std::is_same<T, std::string>::value) {
+        var = v.kind == LuaValue::STRING ? v.str : var;
+    } else if constexpr (std::is_same<T, bool>::value) {
+        var = v.kind == LuaValue::BOOLEAN ? v.boolean : (v.kind == LuaValue::NUMBER && v.num != 0);
+    } else {
+        var = (T)(v.kind == LuaValue::BOOLEAN ? (v.boolean ? 1 : 0) : v.num);
+    }
+}
+}  // namespace
+
+void parseParameters(const LuaConfig& cfg, TorchParameters& p, std::string* w) {
+    const LuaValue& P = cfg["Parameters"];
+    const LuaValue& I = P["Integration"];
+    const LuaValue& G = P["Grid"];
+    const LuaValue& H = P["Hydrodynamics"];
+    const LuaValue& R = P["Radiation"];
+    const LuaValue& T = P["Thermodynamics"];
+    const LuaValue& S = P["Star"];
+    take(I["density_scale"], p.dscale, "density_scale", w);
+    take(I["pressure_scale"], p.pscale, "pressure_scale", w);
+    take(I["time_scale"], p.tscale, "time_scale", w);
+    take(I["spatial_order"], p.spatialOrder, "spatial_order", w);
+    take(I["temporal_order"], p.temporalOrder, "temporal_order", w);
+    take(I["simulation_time"], p.tmax, "simulation_time", w);
+    take(I["radiation_on"], p.radiation_on, "radiation_on", w);
+    take(I["cooling_on"], p.cooling_on, "cooling_on", w);
+    take(I["debug"], p.debug, "debug", w);
+    take(I["initial_conditions"], p.initialConditions, "initial_conditions", w);
+    take(I["ncheckpoints"], p.ncheckpoints, "ncheckpoints", w);
+    take(I["output_directory"], p.outputDirectory, "output_directory", w);
+    take(G["no_dimensions"], p.nd, "no_dimensions", w);
+    take(G["no_cells_x"], p.ncells[0], "no_cells_x", w);
+    take(G["no_cells_y"], p.ncells[1], "no_cells_y", w);
+    take(G["no_cells_z"], p.ncells[2], "no_cells_z", w);
+    take(G["side_length"], p.sideLength, "side_length", w);
+    take(G["geometry"], p.geometry, "geometry", w);
+    const char* xyz[3] = {"x", "y", "z"};
+    for (int d = 0; d < 3; ++d) {
+        take(G[std::string("left_boundary_condition_") + xyz[d]], p.leftBC[d], "left_boundary_condition", w);
+        take(G[std::string("right_boundary_condition_") + xyz[d]], p.rightBC[d], "right_boundary_condition", w);
+    }
+    take(H["gamma"], p.heatCapacityRatio, "gamma", w);
+    take(H["density_floor"], p.dfloor, "density_floor", w);
+    take(H["pressure_floor"], p.pfloor, "pressure_floor", w);
+    take(H["temperature_floor"], p.tfloor, "temperature_floor", w);
+    take(H["riemann_solver"], p.riemannSolver, "riemann_solver", w);
+    take(H["slope_limiter"], p.slopeLimiter, "slope_limiter", w);
+    if (H["precision"].exists()) take(H["precision"], p.precision, "precision", w);
+    take(R["mass_fraction_hydrogen"], p.massFractionH, "mass_fraction_hydrogen", w);
+    take(T["thermo_hii_switch"], p.thermoHII_Switch, "thermo_hii_switch", w);
+    take(T["heating_amplification"], p.heatingAmplification, "heating_amplification", w);
+    take(T["thermo_subcycling"], p.thermoSubcycling, "thermo_subcycling", w);
+    take(T["min_temp_initial_state"], p.minTempInitialState, "min_temp_initial_state", w);
+    take(S["on"], p.star_on, "on", w);
+    take(S["cell_position_x"], p.star_position[0], "cell_position_x", w);
+    take(S["cell_position_y"], p.star_position[1], "cell_position_y", w);
+    take(S["cell_position_z"], p.star_position[2], "cell_position_z", w);
+    take(S["photon_rate"], p.photonRate, "photon_rate", w);
+}
+
+// ---------------------------------------------------------------------------
+// Fluid
+// ---------------------------------------------------------------------------
+Fluid::~Fluid() {
+    if (ctx_) torchgpu_destroy(ctx_);
+}
+
+void Fluid::initialise(const TorchParameters& p, void* stream) {
+    torchgpu_params g = p.gridParams();
+    heatCapacityRatio = p.heatCapacityRatio;
+    massFractionH = p.massFractionH;
+    check_rc(torchgpu_create(&g, stream, &ctx_), "torchgpu_create");
+}
+
+void Fluid::setPrimitives(const double* Q) { check_rc(torchgpu_set_primitive(ctx_, Q, 0), "set_primitive", ctx_); }
+void Fluid::getPrimitives(double* Q, bool fix) const {
+    check_rc(torchgpu_get_primitive(ctx_, Q, fix ? 1 : 0, 0), "get_primitive", ctx_);
+}
+void Fluid::getConserved(double* U) const { check_rc(torchgpu_get_conserved(ctx_, U, 0), "get_conserved", ctx_); }
+void Fluid::check() const { check_rc(torchgpu_synchronize(ctx_), "device", ctx_); }
+
+// Q is never materialised: these record which state the next flux
+// evaluation derives Q from.
+void Fluid::globalQfromU() { qsrc_ = u_is_w_ ? QSource::U : (w_saved_ ? QSource::PREDICTOR : QSource::U); }
+void Fluid::fixPrimitives() {}
+void Fluid::globalWfromU() {
+    w_saved_ = true;
+    u_is_w_ = false;
+    qsrc_ = QSource::U;
+}
+void Fluid::globalUfromW() { u_is_w_ = true; }
+
+void Fluid::advSolution(double dt) {
+    if (pending == Pending::HYDRO) {
+        int stage;
+        if (w_saved_ && !u_is_w_) stage = 1;                       // predictor of hydroStep
+        else if (w_saved_ && u_is_w_ && qsrc_ == QSource::PREDICTOR) stage = 2;  // corrector
+        else stage = 0;                                             // single-stage sub-step
+        check_rc(torchgpu_hydro_stage_dt(ctx_, stage, dt), "hydro stage", ctx_);
+        if (stage == 2) w_saved_ = u_is_w_ = false;
+    } else if (pending == Pending::THERMO) {
+        check_rc(torchgpu_thermo_grid_substep(ctx_, cooling, nullptr, nullptr, nullptr, 0, dt, 0), "thermo substep",
+                 ctx_);
+    }
+    pending = Pending::NONE;
+}
+
+void Fluid::fixSolution() {}  // fused into the launch of advSolution
+
+// ---------------------------------------------------------------------------
+// integrators
+// ---------------------------------------------------------------------------
+void Hydrodynamics::preTimeStepCalculations(Fluid&) const {}  // sound speeds are recomputed in-kernel
+
+double Hydrodynamics::calculateTimeStep(double dt_max, Fluid& fluid) const {
+    double dt = 0;
+    check_rc(torchgpu_hydro_timestep(fluid.ctx(), dt_max, &dt), "hydro timestep", fluid.ctx());
+    return dt;
+}
+
+void Hydrodynamics::integrate(double, Fluid& fluid) const { fluid.pending = Fluid::Pending::HYDRO; }
+void Hydrodynamics::updateSourceTerms(double, Fluid&) const {}
+
+Thermodynamics::~Thermodynamics() {
+    if (cool_) torchgpu_cooling_destroy(cool_);
+}
+
+void Thermodynamics::initialise(const TorchParameters& p, void* stream) {
+    torchgpu_thermo_params t;
+    // the Lua-level floors (cgs density floor): TorchParameters keeps dfloor
+    // in code units after initialise, so convert back for the thermo set-up
+    const double dfloor_cgs = convert(p, p.dfloor, 1, -3, 0, false);
+    check_rc(torchgpu_thermo_params_init(&t, p.dscale, p.pscale, p.tscale, p.thermoHII_Switch, p.heatingAmplification,
+                                         p.massFractionH, p.heatCapacityRatio, dfloor_cgs, p.tfloor,
+                                         p.thermoSubcycling ? 1 : 0),
+             "thermo params");
+    t.pfloor = p.pfloor;
+    check_rc(torchgpu_cooling_create(&t, stream, &cool_), "cooling create");
+}
+
+void Thermodynamics::preTimeStepCalculations(Fluid&) const {}  // rates are evaluated with integrate
+
+double Thermodynamics::calculateTimeStep(double dt_max, Fluid& fluid) const {
+    double dt = 0;
+    check_rc(torchgpu_thermo_grid_timestep(fluid.ctx(), cool_, nullptr, nullptr, nullptr, 0, dt_max, &dt),
+             "thermo timestep", fluid.ctx());
+    return dt;
+}
+
+void Thermodynamics::integrate(double, Fluid& fluid) const {
+    fluid.cooling = cool_;
+    fluid.pending = Fluid::Pending::THERMO;
+}
+void Thermodynamics::updateSourceTerms(double, Fluid&) const {}
+
+// ---------------------------------------------------------------------------
+// driver (Torch.cpp)
+// ---------------------------------------------------------------------------
+void Torch::initialise(const TorchParameters& p_in, void* stream) {
+    params = p_in;
+    params.initialise();
+    fluid.initialise(params, stream);
+    if (params.cooling_on) thermodynamics.initialise(params, stream);
+    currentTime = 0;
+    steps = 0;
+    first_time_ = true;
+    stepCounter_ = 0;
+}
+
+void Torch::setPrimitivesCgs(const double* Qcgs) {
+    // Torch::toCodeUnits (Torch.cpp:146-155), then fixPrimitives + globalUfromQ
+    const torchgpu_params g = params.gridParams();
+    const size_t n = (size_t)g.nx * g.ny * g.nz;
+    std::vector<double> Q(Qcgs, Qcgs + 7 * n);
+    const double kd = params.toCodeUnits(1.0, 1, -3, 0), kp = params.toCodeUnits(1.0, 1, -1, -2),
+                 kv = params.toCodeUnits(1.0, 0, 1, -1);
+    const bool unit = (kd == 1.0 && kp == 1.0 && kv == 1.0);
+    for (size_t c = 0; c < n && !unit; ++c) {
+        Q[0 * n + c] = params.toCodeUnits(Q[0 * n + c], 1, -3, 0);
+        Q[1 * n + c] = params.toCodeUnits(Q[1 * n + c], 1, -1, -2);
+        for (int d = 0; d < 3; ++d) Q[(4 + d) * n + c] = params.toCodeUnits(Q[(4 + d) * n + c], 0, 1, -1);
+    }
+    fluid.setPrimitives(Q.data());
+}
+
+double Torch::calculateTimeStep() {  // Torch.cpp:304-339 (single process: the MPI min is the identity)
+    double dt;
+    if (first_time_) {
+        dt = params.dt_max * 1.0e-20;
+        first_time_ = false;
+    } else {
+        double dt_hydro = hydrodynamics.calculateTimeStep(params.dt_max, fluid);
+        double dt_rad = dt_hydro, dt_thermo = dt_hydro;
+        if (params.cooling_on) dt_thermo = thermodynamics.calculateTimeStep(params.dt_max, fluid);
+        dt = std::min(std::min(dt_hydro, dt_rad), dt_thermo);
+    }
+    return dt;
+}
+
+void Torch::subStep(double dt, bool hasCalculatedHeatFlux, Integrator& comp) {  // Torch.cpp:352-364
+    if (!hasCalculatedHeatFlux) {
+        fluid.globalQfromU();
+        fluid.fixPrimitives();
+        comp.preTimeStepCalculations(fluid);
+    }
+    comp.integrate(dt, fluid);
+    comp.updateSourceTerms(dt, fluid);
+    fluid.advSolution(dt);
+    fluid.fixSolution();
+}
+
+void Torch::hydroStep(double dt, bool hasCalculatedHeatFlux) {  // Torch.cpp:366-387
+    fluid.globalWfromU();
+    if (!hasCalculatedHeatFlux) {
+        fluid.globalQfromU();
+        fluid.fixPrimitives();
+        hydrodynamics.preTimeStepCalculations(fluid);
+    }
+    hydrodynamics.integrate(dt, fluid);
+    hydrodynamics.updateSourceTerms(dt, fluid);
+    fluid.advSolution(dt / 2.0);
+    fluid.fixSolution();
+    fluid.globalQfromU();
+    fluid.globalUfromW();
+    hydrodynamics.integrate(dt, fluid);
+    hydrodynamics.updateSourceTerms(dt, fluid);
+    fluid.advSolution(dt);
+    fluid.fixSolution();
+}
+
+double Torch::fullStep(double dt_nextCheckPoint) {  // Torch.cpp:389-418
+    fluid.globalQfromU();
+    fluid.fixPrimitives();
+    if (params.cooling_on) thermodynamics.preTimeStepCalculations(fluid);
+    double dt = std::min(dt_nextCheckPoint, calculateTimeStep());
+    std::vector<Integrator*> comps{&hydrodynamics};
+    if (params.cooling_on) comps.push_back(&thermodynamics);
+    const int ncomps = (int)comps.size();
+    if (ncomps == 1) {
+        hydroStep(dt, true);
+        return dt;
+    }
+    stepCounter_ = (stepCounter_ + 1) % ncomps;
+    for (int i = 0; i < ncomps; ++i) {
+        double h = (i == ncomps - 1) ? 1.0 : 0.5;
+        subStep(h * dt, i == 0, *comps[(i + stepCounter_) % ncomps]);
+    }
+    for (int i = ncomps - 2; i >= 0; --i) subStep(dt / 2.0, false, *comps[(i + stepCounter_) % ncomps]);
+    return dt;
+}
+
+void Torch::run(int max_steps) {  // Torch.cpp:240-302 (printing omitted), capped at max_steps
+    if (!params.cooling_on) {
+        // hydro-only: the whole loop stays on the device (dt chosen there)
+        check_rc(torchgpu_run(fluid.ctx(), max_steps), "run", fluid.ctx());
+        fluid.check();
+        int n = 0;
+        double t = 0, dt = 0;
+        check_rc(torchgpu_get_clock(fluid.ctx(), &t, &dt, &n), "clock", fluid.ctx());
+        currentTime = t;
+        steps = n;
+        return;
+    }
+    int count = 0;
+    const double delta = 100.0 / params.ncheckpoints;
+    auto checkpoint = [&](double t) {  // Checkpointer::update, Checkpointer.cpp:14-26
+        double pct = std::min(100.0, 100.0 * t / params.tmax);
+        while (pct - count * delta > delta - 1.0e-8) count += 1;
+        return ((count + 1) * delta - pct) * params.tmax / 100.0;
+    };
+    checkpoint(currentTime);
+    while (currentTime < params.tmax && steps < max_steps) {
+        double dt_next = checkpoint(currentTime);
+        double dt = fullStep(dt_next);
+        currentTime += dt;
+        ++steps;
+    }
+    fluid.check();
+}
+
+}  // namespace tg

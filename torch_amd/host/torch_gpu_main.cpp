@@ -1,0 +1,102 @@
+// torch_gpu_main.cpp -- command-line driver: reads a TORCH Lua parameter
+// file, sets the initial state and runs Torch::run on the GPU.
+//
+//   torch_gpu --paramfile=FILE [--setup=sod|FILE.bin] [--steps=N]
+//             [--output=FILE.bin] [--dump-params]
+//
+// --setup=sod builds the Sod shock tube along x; FILE.bin holds primitive
+// variables in cgs, float64 [7][nz][ny][nx] (UID order), the values a Lua
+// set-up function would return (the reference's Torch::setUpLua,
+// Torch.cpp:192-228, needs a Lua runtime).
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "torch_gpu.hpp"
+
+namespace {
+
+std::string arg(int argc, char** argv, const std::string& key, const std::string& def) {
+    for (int i = 1; i < argc; ++i) {
+        std::string a = argv[i];
+        if (a.rfind(key + "=", 0) == 0) return a.substr(key.size() + 1);
+        if (a == key) return "1";
+    }
+    return def;
+}
+
+void dump(const tg::TorchParameters& p) {
+    std::printf("{\"nd\": %d, \"ncells\": [%d, %d, %d], \"side_length\": %.17g, \"spatial_order\": %d,\n", p.nd,
+                p.ncells[0], p.ncells[1], p.ncells[2], p.sideLength, p.spatialOrder);
+    std::printf(" \"riemann_solver\": \"%s\", \"slope_limiter\": \"%s\", \"gamma\": %.17g, \"tmax\": %.17g,\n",
+                p.riemannSolver.c_str(), p.slopeLimiter.c_str(), p.heatCapacityRatio, p.tmax);
+    std::printf(" \"dfloor\": %.17g, \"tfloor\": %.17g, \"pfloor\": %.17g, \"scales\": [%.17g, %.17g, %.17g],\n",
+                p.dfloor, p.tfloor, p.pfloor, p.dscale, p.pscale, p.tscale);
+    std::printf(" \"bc\": [\"%s\", \"%s\", \"%s\", \"%s\", \"%s\", \"%s\"], \"cooling_on\": %s, \"ncheckpoints\": %d,\n",
+                p.leftBC[0].c_str(), p.rightBC[0].c_str(), p.leftBC[1].c_str(), p.rightBC[1].c_str(),
+                p.leftBC[2].c_str(), p.rightBC[2].c_str(), p.cooling_on ? "true" : "false", p.ncheckpoints);
+    std::printf(" \"mass_fraction_h\": %.17g, \"thermo_hii_switch\": %.17g, \"precision\": \"%s\"}\n", p.massFractionH,
+                p.thermoHII_Switch, p.precision.c_str());
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    const std::string paramfile = arg(argc, argv, "--paramfile", "");
+    if (paramfile.empty()) {
+        std::fprintf(stderr, "usage: torch_gpu --paramfile=FILE [--setup=sod|FILE.bin] [--steps=N] [--output=FILE.bin] [--dump-params]\n");
+        return 2;
+    }
+    try {
+        tg::TorchParameters p;
+        std::string warnings;
+        tg::parseParameters(tg::LuaConfig::parse_file(paramfile), p, &warnings);
+        if (arg(argc, argv, "--dump-params", "") == "1") {
+            dump(p);
+            return 0;
+        }
+        tg::Torch torch;
+        torch.initialise(p);
+        const int nx = p.ncells[0], ny = p.ncells[1], nz = p.ncells[2];
+        const size_t n = (size_t)nx * ny * nz;
+        std::vector<double> Q(7 * n, 0.0);
+        const std::string setup = arg(argc, argv, "--setup", "sod");
+        if (setup == "sod") {
+            for (int k = 0; k < nz; ++k)
+                for (int j = 0; j < ny; ++j)
+                    for (int i = 0; i < nx; ++i) {
+                        const size_t c = i + (size_t)nx * (j + (size_t)ny * k);
+                        const bool left = (i + 0.5) / nx < 0.5;
+                        Q[0 * n + c] = left ? 1.0 : 0.125;
+                        Q[1 * n + c] = left ? 1.0 : 0.1;
+                    }
+        } else {
+            std::ifstream f(setup, std::ios::binary);
+            if (!f.read(reinterpret_cast<char*>(Q.data()), (std::streamsize)(Q.size() * sizeof(double))))
+                throw std::runtime_error("cannot read " + setup);
+        }
+        torch.setPrimitivesCgs(Q.data());
+        const int steps = std::stoi(arg(argc, argv, "--steps", "100"));
+        auto t0 = std::chrono::steady_clock::now();
+        torch.run(steps);
+        auto t1 = std::chrono::steady_clock::now();
+        const double secs = std::chrono::duration<double>(t1 - t0).count();
+        std::printf("{\"steps\": %d, \"time\": %.17g, \"seconds\": %.6f, \"mcell_updates_per_s\": %.3f}\n", torch.steps,
+                    torch.currentTime, secs, (double)n * torch.steps / secs / 1e6);
+        const std::string output = arg(argc, argv, "--output", "");
+        if (!output.empty()) {
+            std::vector<double> U(7 * n);
+            torch.fluid.getConserved(U.data());
+            std::ofstream o(output, std::ios::binary);
+            o.write(reinterpret_cast<const char*>(U.data()), (std::streamsize)(U.size() * sizeof(double)));
+        }
+    } catch (std::exception& e) {
+        std::fprintf(stderr, "torch_gpu: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}
