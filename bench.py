@@ -90,6 +90,19 @@ def cpu_baseline(n_sample: int, steps: int, threads: int) -> dict:
             "kind": "port", "sample": f"{n_sample}^3 turbulent box, {steps} steps, FP64 oracle, {threads} OpenMP threads"}
 
 
+def measured_traffic(n: int, kernel: str):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, scaled by a
+    device copy of known size) from the committed rocprofv3 summary of the
+    same workload; None when no summary covers this grid size."""
+    path = os.path.join(ROOT, "profiles", f"hydro_stage_turb{n}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        d = json.load(fh)
+    k = d.get("kernels", {}).get(kernel)
+    return None if k is None else round(k["read_bytes"] + k["write_bytes"])
+
+
 def hydro_rates(n: int, steps: int, warmup: int, err_steps: int):
     import numpy as np
     import torch
@@ -189,6 +202,7 @@ def main():
     mx, f64 = res["mixed"], res["fp64"]
     ncell = args.n ** 3
     achieved = BYTES_STAGE2 * ncell / (mx["stage2_ms"] * 1e-3) / 1e9
+    traffic = measured_traffic(args.n, "void tg::hydro_stage_kernel<2, float, 2>")
     out = {
         "metric": "Mcell-updates/s: FP64 vs mixed hydro; surrogate vs subcycled cooling",
         "value": round(mx["mcells"], 2),
@@ -211,7 +225,7 @@ def main():
         "mixed_vs_fp64_error": errs,
         "roofline": {"bound": "hbm", "kernel": "hydro_stage_kernel<rotated,float,corrector>",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_cell": BYTES_STAGE2, "launch_ms": round(mx["stage2_ms"], 4)},
         "cpu_baseline": cpu,
     }

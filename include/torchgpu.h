@@ -102,8 +102,8 @@ typedef struct torchgpu_params {
 typedef struct torchgpu_ctx torchgpu_ctx;
 
 /* Context: owns the device state (U, the predictor buffer, step state) on
- * the current HIP device.  `stream` is a hipStream_t (NULL: the library
- * creates one). */
+ * the current HIP device.  `stream` is the hipStream_t all of its work is
+ * enqueued on (NULL: the null stream). */
 int torchgpu_create(const torchgpu_params* params, void* stream, torchgpu_ctx** out);
 void torchgpu_destroy(torchgpu_ctx* ctx);
 int torchgpu_last_error(const torchgpu_ctx* ctx, char* buf, size_t len);
@@ -229,7 +229,10 @@ int torchgpu_mlp_create(int max_batch, void* stream, torchgpu_mlp** out);
 void torchgpu_mlp_destroy(torchgpu_mlp* m);
 int torchgpu_mlp_set_params(torchgpu_mlp* m, const float* W, const float* b);
 int torchgpu_mlp_get_params(torchgpu_mlp* m, float* W, float* b);
-/* y[B][3] = MLP(feat[B][8]) with the fused inference kernel (device ptrs). */
+/* y[B][3] = MLP(feat[B][8]) with the fused inference kernel (device ptrs).
+ * Asynchronous on the handle's stream: feat must be ready on that stream and
+ * y is valid once it has been synchronized (pass the caller's stream at
+ * create time to order the call with the caller's own work). */
 int torchgpu_mlp_forward(torchgpu_mlp* m, const float* feat, int64_t B, float* y);
 /* Loss 0.5/B sum_o w_o (y_o - t_o)^2 (output 2's target is the log10
  * relative error of output 0 when err_head != 0) and its gradients, without

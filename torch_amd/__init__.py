@@ -82,6 +82,18 @@ def lib() -> ctypes.CDLL:
     return _lib
 
 
+def default_stream(stream: int | None = None) -> int:
+    """The HIP stream a handle enqueues on: the one given, else torch's
+    current stream on the current device, so that library work is ordered
+    with the torch tensors it reads and writes (the library's own stream
+    would race with them)."""
+    if stream is not None:
+        return stream
+    import torch
+
+    return torch.cuda.current_stream().cuda_stream
+
+
 def make_params(shape, side_length=1.0, gamma=1.67, dfloor=1e-30, pfloor=0.0, tfloor=0.0,
                 mass_fraction_h=1.0, spec_gas_const=8.314462e7, tmax=1e3, dt_max=None,
                 ncheckpoints=1, solver="RotatedHLLC", limiter="albada", spatial_order=1,
@@ -122,7 +134,8 @@ class Grid:
         self.params = params
         self.shape = (params.nz, params.ny, params.nx)
         self._h = ctypes.c_void_p()
-        rc = lib().torchgpu_create(ctypes.byref(params), ctypes.c_void_p(stream or 0), ctypes.byref(self._h))
+        rc = lib().torchgpu_create(ctypes.byref(params), ctypes.c_void_p(default_stream(stream)),
+                                   ctypes.byref(self._h))
         if rc:
             raise TorchGpuError(rc, ERRORS.get(rc, "create failed"))
 
@@ -259,7 +272,8 @@ class Cooling:
             L._cooling_bound = True
         self.params = params
         self._h = ctypes.c_void_p()
-        rc = L.torchgpu_cooling_create(ctypes.byref(params), ctypes.c_void_p(stream or 0), ctypes.byref(self._h))
+        rc = L.torchgpu_cooling_create(ctypes.byref(params), ctypes.c_void_p(default_stream(stream)),
+                                       ctypes.byref(self._h))
         if rc:
             raise TorchGpuError(rc, ERRORS.get(rc, "cooling create failed"))
 

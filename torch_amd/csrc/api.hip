@@ -13,7 +13,6 @@ struct torchgpu_ctx {
     torchgpu_params p;
     tg::HydroGeom g;
     hipStream_t stream = nullptr;
-    bool own_stream = false;
     double* U = nullptr;      // conserved state
     double* Ustar = nullptr;  // predictor state (also scratch for Q transfers)
     tg::StepState* st = nullptr;
@@ -119,14 +118,11 @@ int torchgpu_create(const torchgpu_params* p, void* stream, torchgpu_ctx** out) 
     g.gf = (float)p->gamma;
     g.gm1f = (float)p->gamma - 1.0f;
     g.g2f = (float)p->gamma / ((float)p->gamma - 1.0f);
+    g.igm1f = 1.0f / g.gm1f;
+    g.igm1 = 1.0 / g.gm1;
+    g.idx = 1.0 / g.dx;
 
-    if (stream) {
-        c->stream = (hipStream_t)stream;
-    } else {
-        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-        if (e != hipSuccess) { delete c; return TORCHGPU_ERR_HIP; }
-        c->own_stream = true;
-    }
+    c->stream = (hipStream_t)stream;  // NULL: the null (default) stream
     const size_t bytes = state_bytes(c);
     if (hipMalloc(&c->U, bytes) != hipSuccess || hipMalloc(&c->Ustar, bytes) != hipSuccess ||
         hipMalloc(&c->st, sizeof(tg::StepState)) != hipSuccess) {
@@ -147,7 +143,6 @@ void torchgpu_destroy(torchgpu_ctx* c) {
     if (c->U) (void)hipFree(c->U);
     if (c->Ustar) (void)hipFree(c->Ustar);
     if (c->st) (void)hipFree(c->st);
-    if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 

@@ -99,6 +99,63 @@ __device__ __forceinline__ double cfl_inv_t(const HydroGeom& g, const double q[7
     return inv_t;
 }
 
+// Mixed-precision path: the same FP64 floors, primitive recovery and CFL
+// term, with each run of divisions by one quantity replaced by one
+// reciprocal and the temperature tests written as products.
+__device__ __forceinline__ void q_from_u_mixed(double q[7], const double u[7], double gm1) {
+    const double ir = 1.0 / u[DEN];
+    q[DEN] = u[DEN];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) q[VEL + d] = u[VEL + d] * ir;
+    const double ke = 0.5 * (u[VEL + 0] * q[VEL + 0] + u[VEL + 1] * q[VEL + 1] + u[VEL + 2] * q[VEL + 2]);
+    q[PRE] = (u[PRE] - ke) * gm1;
+    q[HII] = u[HII] * ir;
+    q[ADV] = u[ADV] * ir;
+}
+
+__device__ __forceinline__ void fix_primitives_mixed(const HydroGeom& g, double q[7], double u_den) {
+    q[HII] = smax(smin(q[HII], 1.0), 0.0);
+    q[ADV] = smax(smin(q[ADV], 1.0), 0.0);
+    q[DEN] = smax(q[DEN], g.dfloor);
+    q[PRE] = smax(q[PRE], g.pfloor);
+    const double muR = (g.X * (q[HII] + 1.0) + g.mu_c) * g.Rgas;
+    if (q[PRE] < g.tfloor * muR * q[DEN]) q[PRE] = muR * u_den * g.tfloor;
+}
+
+__device__ __forceinline__ bool fix_solution_mixed(const HydroGeom& g, double u[7]) {
+    bool bad = !isfinite(u[DEN]) || !isfinite(u[PRE]);
+    const double ir = 1.0 / u[DEN];
+    const double hii = smax(smin(u[HII] * ir, 1.0), 0.0);
+    const double adv = smax(smin(u[ADV] * ir, 1.0), 0.0);
+    const double den = smax(u[DEN], g.dfloor);
+    double v[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) v[d] = u[VEL + d] * ir;
+    double ke = 0.5 * u[DEN] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    double pre = (u[PRE] - ke) * g.gm1;
+    ke *= den * ir;
+    if (pre < g.pfloor) pre = g.pfloor;
+    const double muR = (g.X * (hii + 1.0) + g.mu_c) * g.Rgas;
+    if (pre < g.tfloor * muR * den) pre = muR * den * g.tfloor;
+    u[DEN] = den;
+    u[PRE] = pre * g.igm1 + ke;
+    u[HII] = hii * den;
+    u[ADV] = adv * den;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) u[VEL + d] = den * v[d];
+    bad |= (u[DEN] == 0.0 || u[PRE] == 0.0);
+#pragma unroll
+    for (int i = 0; i < 7; ++i) bad |= (u[i] != u[i]) || isinf(u[i]);
+    return bad;
+}
+
+__device__ __forceinline__ double cfl_inv_t_mixed(const HydroGeom& g, const double q[7]) {
+    const double ss = sqrt(g.g * q[PRE] / q[DEN]);
+    double inv_t = (fabs(q[VEL + 0]) + ss + (fabs(q[VEL + 1]) + ss) + (fabs(q[VEL + 2]) + ss)) * g.idx;
+    if (inv_t == 0) inv_t = 1.0 / g.dt_max;
+    return inv_t;
+}
+
 // Records an error of the current step unless an earlier step already
 // failed (the reference stops at its first exception).
 __device__ __forceinline__ void set_error(StepState* st, unsigned int bit) {
@@ -180,7 +237,7 @@ __device__ __forceinline__ void apply_ghost_mode(double q[7], int mode, int dim)
 
 // Load Q of the cell at global (x, y, z), which may lie in the 2-deep ghost
 // layer of one dimension (the first out-of-range dimension decides).
-template <int STAGE>
+template <int STAGE, bool MIXED>
 __device__ __forceinline__ void load_q(const HydroGeom& g, const double* __restrict__ U, int x, int y, int z,
                                        double q[7], StepState* st, bool check) {
     int c[3] = {x, y, z};
@@ -201,9 +258,11 @@ __device__ __forceinline__ void load_q(const HydroGeom& g, const double* __restr
     double u[7];
 #pragma unroll
     for (int v = 0; v < 7; ++v) u[v] = U[v * g.ncell + idx];
-    QfromU(q, u, g.gm1);
+    if constexpr (MIXED) q_from_u_mixed(q, u, g.gm1);
+    else QfromU(q, u, g.gm1);
     if (STAGE == 1) {
-        fix_primitives(g, q, u[DEN]);
+        if constexpr (MIXED) fix_primitives_mixed(g, q, u[DEN]);
+        else fix_primitives(g, q, u[DEN]);
         if (check) {
             // Torch::checkValues("hydro before"), Torch.cpp:420-443
             bool bad = (q[DEN] == 0.0) || (q[PRE] == 0.0);
@@ -246,6 +305,53 @@ __device__ __forceinline__ void q_at(const TileLDS<R>& s, int pos, int ta, int t
     }
 }
 
+// Flux through face f (between along-positions f-1 and f) of pencil (ta, tb)
+// of sweep DIM: reconstruction from the tile's LDS Q and the Riemann solve.
+// Returns true if the HII flux is NaN.
+template <int SOLVER, typename R, int DIM>
+__device__ __forceinline__ bool face_flux(const HydroGeom& g, const TileLDS<R>& s, int f, int ta, int tb,
+                                          const GasConsts<R>& gc, R F[7]) {
+    R ql[7], qr[7];
+    if (g.spatial_order == 1) {
+        R a[7], b[7], cc[7], dd[7];
+        q_at<R, DIM>(s, f - 2, ta, tb, a);
+        q_at<R, DIM>(s, f - 1, ta, tb, b);
+        q_at<R, DIM>(s, f, ta, tb, cc);
+        q_at<R, DIM>(s, f + 1, ta, tb, dd);
+        bool fast = false;
+        if constexpr (sizeof(R) == 4) {
+            if (g.limiter == LIM_ALBADA) {
+                fast = true;
+#pragma unroll
+                for (int v = 0; v < 7; ++v) {
+                    ql[v] = b[v] + 0.5f * albada_f(b[v] - a[v], cc[v] - b[v]);
+                    qr[v] = cc[v] - 0.5f * albada_f(cc[v] - b[v], dd[v] - cc[v]);
+                }
+            }
+        }
+        if (!fast) {
+#pragma unroll
+            for (int v = 0; v < 7; ++v) {
+                R dl = limiter<R>(g.limiter, b[v] - a[v], cc[v] - b[v]);
+                ql[v] = b[v] + R(0.5) * dl;   // QR of cell f-1
+                R dr = limiter<R>(g.limiter, cc[v] - b[v], dd[v] - cc[v]);
+                qr[v] = cc[v] - R(0.5) * dr;  // QL of cell f
+            }
+        }
+    } else {
+        q_at<R, DIM>(s, f - 1, ta, tb, ql);
+        q_at<R, DIM>(s, f, ta, tb, qr);
+    }
+    if constexpr (sizeof(R) == 4) {
+        // mixed precision: hardware reciprocals, rotation-free hybrid flux
+        return riemann_f<SOLVER, DIM>(F, ql, qr, gc, g.igm1f);
+    } else {
+        R a_l2 = gc.g * ql[PRE] / ql[DEN];
+        R a_r2 = gc.g * qr[PRE] / qr[DEN];
+        return riemann<SOLVER, R, DIM>(F, ql, qr, a_l2, a_r2, gc);
+    }
+}
+
 template <int SOLVER, typename R, int STAGE, int DIM>
 __device__ __forceinline__ void sweep(const HydroGeom& g, TileLDS<R>& s, const double* __restrict__ Usrc,
                                       const int o[3], double udot[7], StepState* st, const GasConsts<R>& gc) {
@@ -263,7 +369,7 @@ __device__ __forceinline__ void sweep(const HydroGeom& g, TileLDS<R>& s, const d
         c[DA] = o[DA] + ta;
         c[DB] = o[DB] + tb;
         double q[7];
-        load_q<STAGE>(g, Usrc, c[0], c[1], c[2], q, st, false);
+        load_q<STAGE, (sizeof(R) == 4)>(g, Usrc, c[0], c[1], c[2], q, st, false);
 #pragma unroll
         for (int v = 0; v < 7; ++v) s.qh[v][tid] = (R)q[v];
     }
@@ -275,33 +381,13 @@ __device__ __forceinline__ void sweep(const HydroGeom& g, TileLDS<R>& s, const d
         const int ta = rest & 7, tb = rest >> 3;
         const bool valid = (o[DIM] + f <= N[DIM]) && (o[DA] + ta < N[DA]) && (o[DB] + tb < N[DB]);
         R F[7];
-        if (valid) {
-            R ql[7], qr[7];
-            if (g.spatial_order == 1) {
-                R a[7], b[7], cc[7], dd[7];
-                q_at<R, DIM>(s, f - 2, ta, tb, a);
-                q_at<R, DIM>(s, f - 1, ta, tb, b);
-                q_at<R, DIM>(s, f, ta, tb, cc);
-                q_at<R, DIM>(s, f + 1, ta, tb, dd);
-#pragma unroll
-                for (int v = 0; v < 7; ++v) {
-                    R dl = limiter<R>(g.limiter, b[v] - a[v], cc[v] - b[v]);
-                    ql[v] = b[v] + R(0.5) * dl;   // QR of cell f-1
-                    R dr = limiter<R>(g.limiter, cc[v] - b[v], dd[v] - cc[v]);
-                    qr[v] = cc[v] - R(0.5) * dr;  // QL of cell f
-                }
-            } else {
-                q_at<R, DIM>(s, f - 1, ta, tb, ql);
-                q_at<R, DIM>(s, f, ta, tb, qr);
-            }
-            R a_l2 = gc.g * ql[PRE] / ql[DEN];
-            R a_r2 = gc.g * qr[PRE] / qr[DEN];
-            if (riemann<SOLVER, R, DIM>(F, ql, qr, a_l2, a_r2, gc))
-                set_error(st, STAGE == 1 ? ERR_RIEMANN_NAN_1 : ERR_RIEMANN_NAN_2);
-        } else {
+        bool bad = false;
+        if (valid) bad = face_flux<SOLVER, R, DIM>(g, s, f, ta, tb, gc, F);
+        else {
 #pragma unroll
             for (int v = 0; v < 7; ++v) F[v] = R(0);
         }
+        if (bad) set_error(st, STAGE == 1 ? ERR_RIEMANN_NAN_1 : ERR_RIEMANN_NAN_2);
 #pragma unroll
         for (int v = 0; v < 7; ++v) s.fl[v][tid] = F[v];
     }
@@ -343,7 +429,7 @@ __global__ __launch_bounds__(NTHREADS) void hydro_stage_kernel(HydroGeom g, cons
     const bool active = tid < TCELLS && gx < g.nx && gy < g.ny && gz < g.nz;
     if (tid < TCELLS) {
         double q[7];
-        load_q<STAGE>(g, Usrc, gx, gy, gz, q, st, active);
+        load_q<STAGE, (sizeof(R) == 4)>(g, Usrc, gx, gy, gz, q, st, active);
 #pragma unroll
         for (int v = 0; v < 7; ++v) s.qc[v][tid] = (R)q[v];
     }
@@ -369,14 +455,21 @@ __global__ __launch_bounds__(NTHREADS) void hydro_stage_kernel(HydroGeom g, cons
         for (int v = 0; v < 7; ++v) u[v] = Ubase[v * g.ncell + c];
 #pragma unroll
         for (int v = 0; v < 7; ++v) u[v] += dtf * udot[v];
-        if (fix_solution(g, u)) set_error(st, STAGE == 1 ? ERR_FIX_SOLUTION_1 : ERR_FIX_SOLUTION_2);
+        const bool bad = sizeof(R) == 4 ? fix_solution_mixed(g, u) : fix_solution(g, u);
+        if (bad) set_error(st, STAGE == 1 ? ERR_FIX_SOLUTION_1 : ERR_FIX_SOLUTION_2);
 #pragma unroll
         for (int v = 0; v < 7; ++v) Udst[v * g.ncell + c] = u[v];
         if (STAGE == 2) {
             double q[7];
-            QfromU(q, u, g.gm1);
-            fix_primitives(g, q, u[DEN]);
-            inv_t = cfl_inv_t(g, q);
+            if constexpr (sizeof(R) == 4) {
+                q_from_u_mixed(q, u, g.gm1);
+                fix_primitives_mixed(g, q, u[DEN]);
+                inv_t = cfl_inv_t_mixed(g, q);
+            } else {
+                QfromU(q, u, g.gm1);
+                fix_primitives(g, q, u[DEN]);
+                inv_t = cfl_inv_t(g, q);
+            }
         }
     }
     if (STAGE == 2) block_max_to(inv_t, &st->maxinv, s.red);

@@ -306,4 +306,206 @@ __device__ __forceinline__ bool riemann(R F[7], const R* Ql, const R* Qr, R a_l2
     else return rotated_hllc<R, DIM>(F, Ql, Qr, a_l2, a_r2, c);
 }
 
+// ---------------------------------------------------------------------------
+// Mixed-precision face flux (float).
+//
+// The same solvers, reorganised for throughput rather than bitwise parity:
+//  * reciprocals and square roots are the hardware's (v_rcp_f32, v_sqrt_f32,
+//    ~1 ulp), each shared wherever the reference divides twice by the same
+//    quantity;
+//  * the rotated hybrid solver is evaluated without rotation matrices.
+//    Rotating both states so that a unit normal n maps onto the face axis,
+//    solving there and rotating the flux back (Riemann.cpp:154-244) is the
+//    same as solving along n in the original frame, because the Euler flux
+//    is rotation covariant: with u_n = v.n,
+//        F(Q; n) = (rho u_n, u_n (g2 p + ke), rho u_n x, rho u_n a, rho v u_n + p n),
+//    and the HLLC star state is A (1, E/rho + (S_c - u_n)(S_c + p/(rho (S - u_n))),
+//    x, a, v + (S_c - u_n) n).  For the unit vector n1 (velocity jump,
+//    oriented so that n1_d >= 0) and face axis d, the second normal is
+//    n2 = (n1 x d) x n1 / |.| = (d - n1_d n1) / s with s = |n1 - n1_d d|,
+//    so n2_d = s and the hybrid weights |d.n1|, |d.n2| are n1_d and s.
+//  * both solves share the Roe-average factors, which depend only on the
+//    two densities and sound speeds.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+__device__ __forceinline__ float albada_f(float a, float b) {
+    const float ab = a * b;
+    return ab <= 1.0e-30f ? 0.f : ab * (a + b) * frcp(a * a + b * b);
+}
+
+struct SideF {
+    float rho, p, x, a, v[3];
+    float sr;   // sqrt(rho)
+    float a2;   // gamma p / rho
+    float ke;   // rho |v|^2 / 2
+    float E;    // p / (gamma - 1) + ke
+};
+
+__device__ __forceinline__ SideF side_f(const float q[7], float g, float igm1) {
+    SideF s;
+    s.rho = q[DEN];
+    s.p = q[PRE];
+    s.x = q[HII];
+    s.a = q[ADV];
+    s.v[0] = q[VEL + 0];
+    s.v[1] = q[VEL + 1];
+    s.v[2] = q[VEL + 2];
+    s.sr = fsqrt(s.rho);
+    s.a2 = g * s.p * frcp(s.rho);
+    s.ke = 0.5f * s.rho * (s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2]);
+    s.E = s.p * igm1 + s.ke;
+    return s;
+}
+
+__device__ __forceinline__ float dotn(const float v[3], const float n[3]) { return v[0] * n[0] + v[1] * n[1] + v[2] * n[2]; }
+
+// F(Q; n) for a side with normal velocity un.
+__device__ __forceinline__ void flux_n(float F[7], const SideF& s, float un, const float n[3], float g2) {
+    const float m = s.rho * un;
+    F[DEN] = m;
+    F[PRE] = un * (g2 * s.p + s.ke);
+    F[HII] = m * s.x;
+    F[ADV] = m * s.a;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) F[VEL + i] = m * s.v[i] + s.p * n[i];
+}
+
+__device__ __forceinline__ void cons_of(float U[7], const SideF& s) {
+    U[DEN] = s.rho;
+    U[PRE] = s.E;
+    U[HII] = s.rho * s.x;
+    U[ADV] = s.rho * s.a;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) U[VEL + i] = s.rho * s.v[i];
+}
+
+// Roe-average pieces shared by every normal: 1/(sr_l + sr_r), the mixed
+// sound speed term and the velocity-jump weight nu2.
+struct RoeF {
+    float is, base2, nu2;
+};
+
+__device__ __forceinline__ RoeF roe_f(const SideF& L, const SideF& R) {
+    RoeF r;
+    r.is = frcp(L.sr + R.sr);
+    r.base2 = (L.sr * L.a2 + R.sr * R.a2) * r.is;
+    r.nu2 = 0.5f * L.sr * R.sr * r.is * r.is;
+    return r;
+}
+
+__device__ __forceinline__ void speeds_f(const SideF& L, const SideF& R, const RoeF& r, float unl, float unr,
+                                         float& S_l, float& S_r) {
+    const float ut = (L.sr * unl + R.sr * unr) * r.is;
+    const float du = unr - unl;
+    const float d = fsqrt(r.base2 + r.nu2 * du * du);
+    S_l = ut - d;
+    S_r = ut + d;
+}
+
+// HLL flux along unit normal n (HartenLaxLeerSolver::solve).
+__device__ __forceinline__ void hll_f(float F[7], const SideF& L, const SideF& R, const RoeF& r, const float n[3],
+                                      float g2) {
+    const float unl = dotn(L.v, n), unr = dotn(R.v, n);
+    float S_l, S_r;
+    speeds_f(L, R, r, unl, unr, S_l, S_r);
+    float Fl[7], Fr[7];
+    flux_n(Fl, L, unl, n, g2);
+    flux_n(Fr, R, unr, n, g2);
+    if (S_r <= 0.f) {
+#pragma unroll
+        for (int i = 0; i < 7; ++i) F[i] = Fr[i];
+    } else if (S_l >= 0.f) {
+#pragma unroll
+        for (int i = 0; i < 7; ++i) F[i] = Fl[i];
+    } else {
+        float Ul[7], Ur[7];
+        cons_of(Ul, L);
+        cons_of(Ur, R);
+        const float iw = frcp(S_r - S_l), slr = S_l * S_r;
+#pragma unroll
+        for (int i = 0; i < 7; ++i) F[i] = (S_r * Fl[i] - S_l * Fr[i] + slr * (Ur[i] - Ul[i])) * iw;
+    }
+}
+
+// HLLC flux along unit normal n (HartenLaxLeerContactSolver::solve).
+__device__ __forceinline__ void hllc_f(float F[7], const SideF& L, const SideF& R, const RoeF& r, const float n[3],
+                                       float g2) {
+    const float unl = dotn(L.v, n), unr = dotn(R.v, n);
+    float S_l, S_r;
+    speeds_f(L, R, r, unl, unr, S_l, S_r);
+    const float wl = L.rho * (S_l - unl), wr = R.rho * (S_r - unr);
+    const float S_c = (R.p - L.p + wl * unl - wr * unr) * frcp(wl - wr);
+    // supersonic cases take the upwind flux unchanged
+    const bool sub = !(S_l >= 0.f) && !(S_r <= 0.f);
+    const bool left = (S_l >= 0.f) || (sub && S_c >= 0.f);
+    const SideF& K = left ? L : R;
+    const float un = left ? unl : unr;
+    const float S = left ? S_l : S_r;
+    flux_n(F, K, un, n, g2);
+    if (sub) {
+        const float w = S - un;
+        const float A = K.rho * w * frcp(S - S_c);
+        const float irho = frcp(K.rho);
+        const float dsc = S_c - un;
+        float dU[7];
+        dU[DEN] = A - K.rho;
+        dU[PRE] = A * (K.E * irho + dsc * (S_c + K.p * frcp(K.rho * w))) - K.E;
+        dU[HII] = (A - K.rho) * K.x;
+        dU[ADV] = (A - K.rho) * K.a;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) dU[VEL + i] = A * (K.v[i] + dsc * n[i]) - K.rho * K.v[i];
+#pragma unroll
+        for (int i = 0; i < 7; ++i) F[i] += S * dU[i];
+    }
+}
+
+// Mixed-precision face flux for solver SOLVER along axis DIM.  Returns true
+// if the HII flux is NaN.
+template <int SOLVER, int DIM>
+__device__ __forceinline__ bool riemann_f(float F[7], const float* Ql, const float* Qr, const GasConsts<float>& c,
+                                          float igm1) {
+    const SideF L = side_f(Ql, c.g, igm1), R = side_f(Qr, c.g, igm1);
+    const RoeF r = roe_f(L, R);
+    const float d[3] = {DIM == 0 ? 1.f : 0.f, DIM == 1 ? 1.f : 0.f, DIM == 2 ? 1.f : 0.f};
+    if constexpr (SOLVER == SOLVER_HLL) {
+        hll_f(F, L, R, r, d, c.g2);
+    } else if constexpr (SOLVER == SOLVER_HLLC) {
+        hllc_f(F, L, R, r, d, c.g2);
+    } else {
+        constexpr int A = DIM == 0 ? 1 : 0, B = DIM == 2 ? 1 : 2;  // transverse axes
+        float n1[3] = {R.v[0] - L.v[0], R.v[1] - L.v[1], R.v[2] - L.v[2]};
+        if (n1[DIM] < 0.f) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) n1[i] = -n1[i];
+        }
+        const float nn = n1[0] * n1[0] + n1[1] * n1[1] + n1[2] * n1[2];
+        if (nn < 1.0e-12f) {  // |n1| < 1e-6: pure HLLC along the face axis
+            hllc_f(F, L, R, r, d, c.g2);
+        } else {
+            const float in = __builtin_amdgcn_rsqf(nn);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) n1[i] *= in;
+            const float s2 = n1[A] * n1[A] + n1[B] * n1[B];
+            if (s2 == 0.f) {  // n1 along the face axis: pure HLL
+                hll_f(F, L, R, r, d, c.g2);
+            } else {
+                const float s = fsqrt(s2), is = frcp(s);
+                float n2[3];
+                n2[DIM] = s;
+                n2[A] = -n1[A] * n1[DIM] * is;
+                n2[B] = -n1[B] * n1[DIM] * is;
+                float F1[7], F2[7];
+                hll_f(F1, L, R, r, n1, c.g2);
+                hllc_f(F2, L, R, r, n2, c.g2);
+                const float w1 = n1[DIM];
+#pragma unroll
+                for (int i = 0; i < 7; ++i) F[i] = w1 * F1[i] + s * F2[i];
+            }
+        }
+    }
+    return F[HII] != F[HII];
+}
+
 }  // namespace tg

@@ -21,7 +21,6 @@ using tg::MLP_NW;
 
 struct torchgpu_mlp {
     hipStream_t stream = nullptr;
-    bool own_stream = false;
     int max_batch = 0;
     int nsm = 256;
     long long step = 0;
@@ -162,12 +161,7 @@ int torchgpu_mlp_create(int max_batch, void* stream, torchgpu_mlp** out) {
     *out = nullptr;
     torchgpu_mlp* m = new torchgpu_mlp();
     m->max_batch = max_batch;
-    if (stream) {
-        m->stream = (hipStream_t)stream;
-    } else {
-        if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) { delete m; return TORCHGPU_ERR_HIP; }
-        m->own_stream = true;
-    }
+    m->stream = (hipStream_t)stream;  // NULL: the null (default) stream
     int dev = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
@@ -219,7 +213,6 @@ void torchgpu_mlp_destroy(torchgpu_mlp* m) {
     }
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
-    if (m->own_stream) (void)hipStreamDestroy(m->stream);
     delete m;
 }
 

@@ -448,7 +448,6 @@ struct torchgpu_cooling {
     torchgpu_thermo_params p;
     tg::ThermoConsts t;
     hipStream_t stream = nullptr;
-    bool own_stream = false;
     double* tables = nullptr;                 // spline x, y, y2 (78 doubles)
     unsigned long long* counters = nullptr;   // [0] work, [1] subcycles, [2] surrogate queue, [3] fallback queue
     unsigned int* err = nullptr;
@@ -567,12 +566,7 @@ int torchgpu_cooling_create(const torchgpu_thermo_params* p, void* stream, torch
     torchgpu_cooling* c = new torchgpu_cooling();
     c->p = *p;
     c->t = to_consts(*p);
-    if (stream) {
-        c->stream = (hipStream_t)stream;
-    } else {
-        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return TORCHGPU_ERR_HIP; }
-        c->own_stream = true;
-    }
+    c->stream = (hipStream_t)stream;  // NULL: the null (default) stream
     double tab[78];
     for (int i = 0; i < 26; ++i) { tab[i] = p->spl_x[i]; tab[26 + i] = p->spl_y[i]; tab[52 + i] = p->spl_y2[i]; }
     hipDeviceProp_t prop;
@@ -597,7 +591,6 @@ void torchgpu_cooling_destroy(torchgpu_cooling* c) {
     for (void* ptr : {(void*)c->tables, (void*)c->counters, (void*)c->err, (void*)c->mlp_list, (void*)c->fb_list,
                       (void*)c->feat, (void*)c->y, (void*)c->sdata, (void*)c->in9, (void*)c->out5, (void*)c->dtbits})
         if (ptr) (void)hipFree(ptr);
-    if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 

@@ -26,6 +26,8 @@ struct HydroGeom {
     double dt_max;
     double g, gm1, g2;      // gamma, gamma - 1.0, gamma / (gamma - 1.0)
     float gf, gm1f, g2f;    // the same rounded to float (mixed path)
+    float igm1f;            // 1 / gm1f (mixed path)
+    double igm1, idx;       // 1 / gm1, 1 / dx (mixed path)
 };
 
 // Device-resident state of a time-stepping run (one per context).

@@ -15,7 +15,7 @@ import time
 
 import numpy as np
 
-from . import Cooling, TorchGpuError, lib
+from . import Cooling, TorchGpuError, default_stream, lib
 
 NIN, NH, NOUT = 8, 128, 3
 NW = NH * NIN + 3 * NH * NH + NOUT * NH
@@ -91,7 +91,7 @@ class MLP:
     def __init__(self, max_batch: int = 65536, stream: int | None = None):
         _bind()
         self._h = ctypes.c_void_p()
-        _check(lib().torchgpu_mlp_create(max_batch, ctypes.c_void_p(stream or 0), ctypes.byref(self._h)),
+        _check(lib().torchgpu_mlp_create(max_batch, ctypes.c_void_p(default_stream(stream)), ctypes.byref(self._h)),
                "mlp create")
 
     def close(self):
@@ -232,3 +232,68 @@ def train(mlp: MLP, feat, target, epochs: int = 10, batch: int = 65536, lr: floa
         if log:
             log(f"epoch {ep}: loss {hist[-1]:.3e} ({time.perf_counter() - t0:.1f}s)")
     return hist
+
+
+def load(path: str):
+    """(W, b, cfg fields) of a file written by torch_amd.train_surrogate."""
+    f = np.load(path, allow_pickle=False)
+    return {k: f[k] for k in f.files}
+
+
+def attach(cool: Cooling, path: str, gate: float = 0.01, mlp: MLP | None = None, max_batch: int = 65536) -> MLP:
+    """Loads trained parameters into an MLP handle and makes it the cooling
+    operator's surrogate (estimated-error gate `gate`).  Returns the handle,
+    which must outlive its use by `cool`."""
+    d = load(path)
+    if mlp is None:
+        mlp = MLP(max_batch=max_batch)
+    mlp.set_params(d["W"], d["b"])
+    mu = list(d["mu"]) + [0.0, 0.0]
+    isd = list(d["isd"]) + [0.0, 0.0]
+    set_surrogate(cool, mlp, make_cfg(mu, isd, d["lo"], d["hi"], gate=gate, nexact=int(d["nexact"])))
+    return mlp
+
+
+def compare(cool: Cooling, inp):
+    """Runs the FP64 subcycled step and the surrogate step on the same cells
+    ([9][n] device inputs) and returns their event-timed durations and the
+    surrogate's relative temperature error against FP64.  The temperature
+    ratio equals the ratio of the subcycled pressures (same density and
+    ionisation), out[4]."""
+    import torch
+
+    s = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ev[0].record(s)
+    ref, nsub = cool.thermo_cells(inp)
+    ev[1].record(s)
+    ev[2].record(s)
+    out, path, (n_mlp, n_fb) = thermo_cells_surrogate(cool, inp)
+    ev[3].record(s)
+    torch.cuda.synchronize()
+    rel = ((out[4] / ref[4]) - 1.0).abs()
+    p = path
+    acc = p == 1
+    q = torch.tensor([0.5, 0.99], dtype=torch.float64, device=rel.device)
+
+    def quant(x):
+        if x.numel() == 0:
+            return [0.0, 0.0]
+        # torch.quantile is limited to 2^24 elements
+        if x.numel() > (1 << 24):
+            x = x[torch.randperm(x.numel(), device=x.device)[: 1 << 24]]
+        return [float(v) for v in torch.quantile(x, q)]
+
+    all_q, acc_q = quant(rel), quant(rel[acc])
+    n = inp.shape[1]
+    return {
+        "cells": int(n),
+        "fp64_ms": ev[0].elapsed_time(ev[1]),
+        "surrogate_ms": ev[2].elapsed_time(ev[3]),
+        "fp64_subcycles": int(nsub),
+        "exact": int((p == 0).sum()), "mlp_accepted": int(acc.sum()), "fallback": int((p == 2).sum()),
+        "mlp_queued": int(n_mlp),
+        "rel_T_err_all": {"median": all_q[0], "p99": all_q[1], "max": float(rel.max())},
+        "rel_T_err_mlp_cells": {"median": acc_q[0], "p99": acc_q[1],
+                                "max": float(rel[acc].max()) if int(acc.sum()) else 0.0},
+    }

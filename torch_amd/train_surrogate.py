@@ -1,0 +1,131 @@
+"""Trains the cooling surrogate on the GPU and writes its parameters.
+
+    python -m torch_amd.train_surrogate [--samples N] [--chunk C] [--epochs E]
+                                        [--out torch_amd/data/surrogate_cooling.npz]
+
+1. Samples: cells of the cooling workload (torch_amd.cooling_inputs: density
+   log-uniform 1e-24..1e-19 g/cm^3, T 10..1e5 K, x in [0, 1], ionising flux
+   1e6..1e14 cm^-2 s^-1, tau in [0, 10], dt 1e7..1e11 s) drawn with seeds
+   1000 + chunk, disjoint from the evaluation seeds.
+2. Targets: the FP64 subcycled integrator (torchgpu_thermo_cells) on the
+   same GPU; the cells with more than `nexact` subcycles (the ones the
+   network replaces) are extracted on the device with their raw features by
+   torchgpu_surrogate_training_data.
+3. Normalisation: per-feature mean and standard deviation of the training
+   set; the training envelope is the per-feature [min, max].
+4. Training: Adam with a cosine learning-rate decay over shuffled batches,
+   every step a torchgpu_mlp_train_step (the library's BF16-MFMA forward and
+   backward kernels with FP32 accumulation).  Output 2 learns the log10 of
+   the current |relative T error| of output 0 from epoch 1 on; the cooling
+   step gates on it.
+
+The output file holds W, b (fp32), mu, isd, lo, hi, nexact and the training
+record; torch_amd.surrogate.load() reads it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_OUT = os.path.join(HERE, "data", "surrogate_cooling.npz")
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+def workload_inputs(n: int, seed: int, device="cuda"):
+    """[9][n] float64 device tensor of cooling-workload cells in code units."""
+    import torch
+
+    import torch_amd as ta
+    from torch_amd import cooling_inputs as ci
+
+    scales = (ci.DENSITY_SCALE, ci.PRESSURE_SCALE, ci.TIME_SCALE)
+    tc = lambda v, m, l, t: ta.units(scales, v, m, l, t)  # noqa: E731
+    x = ci.to_code_inputs(ci.sample_cgs(n, seed=seed), tc)
+    return torch.from_numpy(np.ascontiguousarray(x.T)).to(device)
+
+
+def generate(cool, mlp, samples: int, chunk: int, nexact: int, seed0: int = 1000):
+    """Raw features [k][6] and targets [k][3] (device fp32) of the cells past
+    nexact subcycles in `samples` workload cells."""
+    import torch
+
+    from torch_amd import surrogate as sg
+
+    big = [1e30] * 6
+    sg.set_surrogate(cool, mlp, sg.make_cfg([0.0] * 8, [1.0] * 8, [-x for x in big], big, gate=0.0, nexact=nexact))
+    raws, tgts = [], []
+    done, i = 0, 0
+    while done < samples:
+        n = min(chunk, samples - done)
+        t0 = time.perf_counter()
+        inp = workload_inputs(n, seed0 + i)
+        out, nsub = cool.thermo_cells(inp)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        _, _, tgt, raw = sg.training_data(cool, inp, out)
+        raws.append(raw[:, :6].clone())
+        tgts.append(tgt.clone())
+        log(f"chunk {i}: {n} cells, {tgt.shape[0]} past {nexact} subcycles, {nsub:.3e} subcycles, "
+            f"integrator {t1 - t0:.1f}s")
+        del inp, out
+        done += n
+        i += 1
+    return torch.cat(raws), torch.cat(tgts)
+
+
+def main():
+    import torch
+
+    import torch_amd as ta
+    from torch_amd import surrogate as sg
+
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--samples", type=int, default=1 << 24)
+    ap.add_argument("--chunk", type=int, default=1 << 24)
+    ap.add_argument("--epochs", type=int, default=12)
+    ap.add_argument("--batch", type=int, default=65536)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--nexact", type=int, default=4)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--out", default=DEFAULT_OUT)
+    a = ap.parse_args()
+
+    t_start = time.perf_counter()
+    cool = ta.Cooling(ta.thermo_params())
+    mlp = sg.MLP(max_batch=a.batch)
+    raw, tgt = generate(cool, mlp, a.samples, a.chunk, a.nexact)
+    k = raw.shape[0]
+    mu = raw.mean(0).double()
+    sd = raw.double().std(0).clamp_min(1e-6)
+    lo = raw.min(0).values.double()
+    hi = raw.max(0).values.double()
+    feat = torch.zeros((k, 8), dtype=torch.float32, device=raw.device)
+    feat[:, :6] = ((raw.double() - mu) / sd).float()
+    del raw
+    log(f"training set: {k} cells; mu {mu.tolist()} sd {sd.tolist()}")
+    W, b = sg.init_params(a.seed)
+    mlp.set_params(W, b)
+    t0 = time.perf_counter()
+    hist = sg.train(mlp, feat, tgt.contiguous(), epochs=a.epochs, batch=a.batch, lr=a.lr, seed=a.seed, log=log)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    W, b = mlp.get_params()
+    os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+    record = {"samples": a.samples, "train_cells": int(k), "epochs": a.epochs, "batch": a.batch, "lr": a.lr,
+              "seed": a.seed, "loss_history": hist, "train_seconds": t_train,
+              "total_seconds": time.perf_counter() - t_start}
+    np.savez(a.out, W=W, b=b, mu=mu.cpu().numpy(), isd=(1.0 / sd).cpu().numpy(), lo=lo.cpu().numpy(),
+             hi=hi.cpu().numpy(), nexact=np.int32(a.nexact), record=np.array(json.dumps(record)))
+    log(json.dumps(record))
+
+
+if __name__ == "__main__":
+    main()
