@@ -177,6 +177,54 @@ def hydro_rates(n: int, steps: int, warmup: int, err_steps: int):
     return res, {"steps": err_steps, **errs}
 
 
+def cooling_rates(log2_cells: int, weights: str, gate: float) -> dict:
+    """The cooling source step on 2^log2_cells independent cells of the
+    cooling workload (evaluation seed 11, disjoint from the training seeds):
+    the FP64 subcycled integrator and the surrogate step (FP64 prepass, BF16
+    MLP, FP64 fallback) on the same inputs, each event-timed on the stream
+    it runs on, with the surrogate's relative temperature error against the
+    FP64 result."""
+    import torch
+
+    import torch_amd as ta
+    from torch_amd import surrogate as sg
+    from torch_amd.train_surrogate import workload_inputs
+
+    cool = ta.Cooling(ta.thermo_params())
+    have = os.path.exists(weights)
+    mlp = sg.attach(cool, weights, gate=gate) if have else None
+    warm = workload_inputs(1 << 12, seed=12)
+    cool.thermo_cells(warm)
+    if have:
+        sg.thermo_cells_surrogate(cool, warm)
+    torch.cuda.synchronize()
+    n = 1 << log2_cells
+    inp = workload_inputs(n, seed=11)
+    if not have:
+        s = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        _, nsub = cool.thermo_cells(inp)
+        e1.record(s)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        return {"cells": n, "fp64_subcycled": {"ms": round(ms, 2), "Mcells_per_s": round(n / ms / 1e3, 4),
+                                               "subcycles": int(nsub)}, "surrogate": None}
+    r = sg.compare(cool, inp)
+    del mlp
+    return {
+        "cells": n,
+        "fp64_subcycled": {"ms": round(r["fp64_ms"], 2), "Mcells_per_s": round(n / r["fp64_ms"] / 1e3, 4),
+                           "subcycles": r["fp64_subcycles"]},
+        "surrogate": {"ms": round(r["surrogate_ms"], 2), "Mcells_per_s": round(n / r["surrogate_ms"] / 1e3, 4),
+                      "gate": gate, "cells_exact": r["exact"], "cells_mlp": r["mlp_accepted"],
+                      "cells_fallback": r["fallback"]},
+        "speedup": round(r["fp64_ms"] / r["surrogate_ms"], 3),
+        "rel_T_err_all_cells": r["rel_T_err_all"],
+        "rel_T_err_mlp_cells": r["rel_T_err_mlp_cells"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,6 +234,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--cpu-sample", type=int, default=96)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cooling-log2", type=int, default=24, help="cooling workload size, 2^k cells")
+    ap.add_argument("--no-cooling", action="store_true")
+    ap.add_argument("--surrogate", default=os.path.join(ROOT, "torch_amd", "data", "surrogate_cooling.npz"))
+    ap.add_argument("--gate", type=float, default=0.01)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}); "
@@ -200,6 +252,11 @@ def main():
 
     res, errs = hydro_rates(args.n, args.steps, args.warmup, args.warmup + args.steps)
     mx, f64 = res["mixed"], res["fp64"]
+    cool = None
+    if not args.no_cooling:
+        t0 = time.perf_counter()
+        cool = cooling_rates(args.cooling_log2, args.surrogate, args.gate)
+        log(f"cooling {cool} ({time.perf_counter() - t0:.1f}s)")
     ncell = args.n ** 3
     achieved = BYTES_STAGE2 * ncell / (mx["stage2_ms"] * 1e-3) / 1e9
     traffic = measured_traffic(args.n, "void tg::hydro_stage_kernel<2, float, 2>")
@@ -228,6 +285,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_cell": BYTES_STAGE2, "launch_ms": round(mx["stage2_ms"], 4)},
         "cpu_baseline": cpu,
+        "cooling": cool,
     }
     print(json.dumps(out), flush=True)
 

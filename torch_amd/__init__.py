@@ -74,6 +74,7 @@ def lib() -> ctypes.CDLL:
         L.torchgpu_hydro_step.argtypes = [vp, d]
         L.torchgpu_run.argtypes = [vp, i]
         L.torchgpu_hydro_stage.argtypes = [vp, i]
+        L.torchgpu_hydro_stage_dt.argtypes = [vp, i, d]
         L.torchgpu_set_dt.argtypes = [vp, d]
         L.torchgpu_reset_clock.argtypes = [vp]
         L.torchgpu_synchronize.argtypes = [vp]
@@ -200,6 +201,11 @@ class Grid:
     def stage(self, stage: int):
         self._check(lib().torchgpu_hydro_stage(self._h, stage))
 
+    def stage_dt(self, stage: int, dtf: float):
+        """One stage with an explicit, already scaled dt; stage 0 is the
+        single-stage hydro sub-step of an operator-split step."""
+        self._check(lib().torchgpu_hydro_stage_dt(self._h, stage, dtf))
+
     def reset_clock(self):
         self._check(lib().torchgpu_reset_clock(self._h))
 
@@ -232,6 +238,8 @@ def _bind_cooling(L):
     L.torchgpu_cooling_destroy.restype = None
     L.torchgpu_cooling_last_error.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
     L.torchgpu_thermo_cells.argtypes = [vp, ctypes.c_int64, vp, vp, ctypes.POINTER(ctypes.c_int64)]
+    L.torchgpu_thermo_grid_timestep.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int64, d, ctypes.POINTER(d)]
+    L.torchgpu_thermo_grid_substep.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int64, d, i]
 
 
 def units(scales, value, m, l, t, to_code=True) -> float:
@@ -293,6 +301,25 @@ class Cooling:
             self.close()
         except Exception:
             pass
+
+    @staticmethod
+    def _dptr(a):
+        return ctypes.c_void_p(0 if a is None else a.data_ptr())
+
+    def grid_timestep(self, grid: "Grid", dt_max: float, T_min=None, col_den=None, F_FUV=None,
+                      excluded: int = -1) -> float:
+        """Thermodynamics::calculateTimeStep on the grid state (device
+        arrays [n] or None for T_min, COL_DEN, F_FUV)."""
+        dt = ctypes.c_double()
+        self._check(lib().torchgpu_thermo_grid_timestep(grid._h, self._h, self._dptr(T_min), self._dptr(col_den),
+                                                        self._dptr(F_FUV), excluded, dt_max, ctypes.byref(dt)))
+        return dt.value
+
+    def grid_substep(self, grid: "Grid", dt: float, T_min=None, col_den=None, F_FUV=None, excluded: int = -1,
+                     surrogate: bool = False):
+        """The thermo sub-step of Torch::subStep on the grid state."""
+        self._check(lib().torchgpu_thermo_grid_substep(grid._h, self._h, self._dptr(T_min), self._dptr(col_den),
+                                                       self._dptr(F_FUV), excluded, dt, int(surrogate)))
 
     def thermo_cells(self, inp, out=None, count=True):
         """inp: device float64 tensor [9][n]; returns out [5][n] (device) and

@@ -139,7 +139,7 @@ int torchgpu_create(const torchgpu_params* p, void* stream, torchgpu_ctx** out) 
 
 void torchgpu_destroy(torchgpu_ctx* c) {
     if (!c) return;
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(c->stream);
     if (c->U) (void)hipFree(c->U);
     if (c->Ustar) (void)hipFree(c->Ustar);
     if (c->st) (void)hipFree(c->st);

@@ -24,261 +24,21 @@
 // workgroup b is given tile (b % 8) * (ntiles / 8) + b / 8) so that each
 // XCD's L2 sees neighbouring tiles' halos.
 //
-// For precision == 1 (mixed) Q is rounded to float after the FP64 pressure
-// recovery, reconstruction and the Riemann solve run in float, and the
-// fluxes are accumulated, applied and floored in FP64.
+// This is the FP64 parity kernel: compiled with -ffp-contract=off it keeps
+// the reference's operand order and is bitwise equal to its x86-64 build.
+// The sweep fluxes of all three directions stay in LDS (140 KB) and are
+// accumulated once per cell after the sweeps, so the accumulators are not
+// live across the face solves.  The mixed-precision kernel is in
+// hydro_mixed.hip.
 
-#include "hydro_math.hpp"
-#include "torchgpu_internal.hpp"
+#include "hydro_device.hpp"
 
 namespace tg {
 
-constexpr int TILE = 8;
-constexpr int TCELLS = TILE * TILE * TILE;        // 512
-constexpr int TFACES = (TILE + 1) * TILE * TILE;  // 576
-constexpr int THALO = 4 * TILE * TILE;            // 256 (2 layers x 2 sides)
-constexpr int NTHREADS = TFACES;
-
-__device__ __forceinline__ int tile_of_block(int b, int ntiles) {
-    const int q = ntiles >> 3, r = ntiles & 7;
-    const int x = b & 7, i = b >> 3;
-    return x * q + (x < r ? x : r) + i;
-}
-
-// fixPrimitives for one cell (Fluid.cpp:138-150).  u_den is U[DEN] of the
-// same cell, used by the reference's temperature floor.
-__device__ __forceinline__ void fix_primitives(const HydroGeom& g, double q[7], double u_den) {
-    q[HII] = smax(smin(q[HII], 1.0), 0.0);
-    q[ADV] = smax(smin(q[ADV], 1.0), 0.0);
-    q[DEN] = smax(q[DEN], g.dfloor);
-    q[PRE] = smax(q[PRE], g.pfloor);
-    double mu_inv = g.X * (q[HII] + 1.0) + g.mu_c;
-    double temperature = q[PRE] / (mu_inv * g.Rgas * q[DEN]);
-    if (temperature < g.tfloor) q[PRE] = mu_inv * g.Rgas * u_den * g.tfloor;
-}
-
-// fixSolution for one cell (Fluid.cpp:89-136).  Returns true if the
-// reference would throw.
-__device__ __forceinline__ bool fix_solution(const HydroGeom& g, double u[7]) {
-    bool bad = !isfinite(u[DEN]) || !isfinite(u[PRE]);
-    double hii = smax(smin(u[HII] / u[DEN], 1.0), 0.0);
-    double adv = smax(smin(u[ADV] / u[DEN], 1.0), 0.0);
-    double den = smax(u[DEN], g.dfloor);
-    double v[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) v[d] = u[VEL + d] / u[DEN];
-    double ke = 0.0;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) ke += v[d] * v[d];
-    ke *= 0.5 * u[DEN];
-    double pre = (u[PRE] - ke) * g.gm1;
-    ke *= den / u[DEN];
-    if (pre < g.pfloor) pre = g.pfloor;
-    double mu_inv = g.X * (hii + 1.0) + g.mu_c;
-    double temperature = pre / (mu_inv * g.Rgas * den);
-    if (temperature < g.tfloor) pre = mu_inv * g.Rgas * den * g.tfloor;
-    u[DEN] = den;
-    u[PRE] = pre / g.gm1 + ke;
-    u[HII] = hii * den;
-    u[ADV] = adv * den;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) u[VEL + d] = den * v[d];
-    bad |= (u[DEN] == 0.0 || u[PRE] == 0.0);
-#pragma unroll
-    for (int i = 0; i < 7; ++i) bad |= (u[i] != u[i]) || isinf(u[i]);
-    return bad;
-}
-
-// CFL term of one cell, Hydro.cpp:80-85 (inv_t == 0 -> 1/dt_max).
-__device__ __forceinline__ double cfl_inv_t(const HydroGeom& g, const double q[7]) {
-    double inv_t = 0;
-    double ss = sqrt(g.g * q[PRE] / q[DEN]);
-#pragma unroll
-    for (int d = 0; d < 3; ++d) inv_t += (fabs(q[VEL + d]) + ss) / g.dx;
-    if (inv_t == 0) inv_t = 1.0 / g.dt_max;
-    return inv_t;
-}
-
-// Mixed-precision path: the same FP64 floors, primitive recovery and CFL
-// term, with each run of divisions by one quantity replaced by one
-// reciprocal and the temperature tests written as products.
-__device__ __forceinline__ void q_from_u_mixed(double q[7], const double u[7], double gm1) {
-    const double ir = 1.0 / u[DEN];
-    q[DEN] = u[DEN];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) q[VEL + d] = u[VEL + d] * ir;
-    const double ke = 0.5 * (u[VEL + 0] * q[VEL + 0] + u[VEL + 1] * q[VEL + 1] + u[VEL + 2] * q[VEL + 2]);
-    q[PRE] = (u[PRE] - ke) * gm1;
-    q[HII] = u[HII] * ir;
-    q[ADV] = u[ADV] * ir;
-}
-
-__device__ __forceinline__ void fix_primitives_mixed(const HydroGeom& g, double q[7], double u_den) {
-    q[HII] = smax(smin(q[HII], 1.0), 0.0);
-    q[ADV] = smax(smin(q[ADV], 1.0), 0.0);
-    q[DEN] = smax(q[DEN], g.dfloor);
-    q[PRE] = smax(q[PRE], g.pfloor);
-    const double muR = (g.X * (q[HII] + 1.0) + g.mu_c) * g.Rgas;
-    if (q[PRE] < g.tfloor * muR * q[DEN]) q[PRE] = muR * u_den * g.tfloor;
-}
-
-__device__ __forceinline__ bool fix_solution_mixed(const HydroGeom& g, double u[7]) {
-    bool bad = !isfinite(u[DEN]) || !isfinite(u[PRE]);
-    const double ir = 1.0 / u[DEN];
-    const double hii = smax(smin(u[HII] * ir, 1.0), 0.0);
-    const double adv = smax(smin(u[ADV] * ir, 1.0), 0.0);
-    const double den = smax(u[DEN], g.dfloor);
-    double v[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) v[d] = u[VEL + d] * ir;
-    double ke = 0.5 * u[DEN] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-    double pre = (u[PRE] - ke) * g.gm1;
-    ke *= den * ir;
-    if (pre < g.pfloor) pre = g.pfloor;
-    const double muR = (g.X * (hii + 1.0) + g.mu_c) * g.Rgas;
-    if (pre < g.tfloor * muR * den) pre = muR * den * g.tfloor;
-    u[DEN] = den;
-    u[PRE] = pre * g.igm1 + ke;
-    u[HII] = hii * den;
-    u[ADV] = adv * den;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) u[VEL + d] = den * v[d];
-    bad |= (u[DEN] == 0.0 || u[PRE] == 0.0);
-#pragma unroll
-    for (int i = 0; i < 7; ++i) bad |= (u[i] != u[i]) || isinf(u[i]);
-    return bad;
-}
-
-__device__ __forceinline__ double cfl_inv_t_mixed(const HydroGeom& g, const double q[7]) {
-    const double ss = sqrt(g.g * q[PRE] / q[DEN]);
-    double inv_t = (fabs(q[VEL + 0]) + ss + (fabs(q[VEL + 1]) + ss) + (fabs(q[VEL + 2]) + ss)) * g.idx;
-    if (inv_t == 0) inv_t = 1.0 / g.dt_max;
-    return inv_t;
-}
-
-// Records an error of the current step unless an earlier step already
-// failed (the reference stops at its first exception).
-__device__ __forceinline__ void set_error(StepState* st, unsigned int bit) {
-    const int s1 = st->step + 1;
-    const int prev = atomicCAS(&st->err_step1, 0, s1);
-    if (prev == 0 || prev == s1) atomicOr(&st->err, bit);
-}
-
-// Block-wide max of non-negative doubles (NaN entries ignored) -> one
-// atomicMax on the bit pattern.  All NTHREADS threads must call it.
-__device__ __forceinline__ void block_max_to(double v, unsigned long long* dst, double* red) {
-    if (!(v >= 0.0)) v = 0.0;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) red[wave] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double m = red[0];
-        for (int w = 1; w < NTHREADS / 64; ++w) m = fmax(m, red[w]);
-        atomicMax(dst, (unsigned long long)__double_as_longlong(m));
-    }
-}
-
-// Source interior coordinate and velocity sign for a ghost position along
-// one dimension (Grid.cpp:661-802).  pos is the global coordinate, -2..N+1.
-// Returns the interior index to read; flip = -1 negates the normal velocity,
-// +1 keeps it, 0 = "flip if the interior velocity points inwards" (outflow)
-// or outwards (inflow) -- resolved by the caller.
-struct GhostMap {
-    int src;
-    int mode;  // 0 copy, 1 negate, 2 outflow-left, 3 outflow-right, 4 inflow-left, 5 inflow-right
-};
-
-__device__ __forceinline__ GhostMap ghost_map(int pos, int N, int bcl, int bcr) {
-    GhostMap m{pos, 0};
-    if (pos < 0) {
-        const int L = -pos;  // layer 1 or 2
-        switch (bcl) {
-        case BC_PERIODIC:   m.src = N - L; break;
-        case BC_REFLECTING: m.src = L; m.mode = 1; break;
-        case BC_OUTFLOW:    m.src = 0; m.mode = 2; break;
-        case BC_INFLOW:     m.src = 0; m.mode = 4; break;
-        default:            m.src = 0; break;
-        }
-    } else if (pos >= N) {
-        const int L = pos - N + 1;
-        switch (bcr) {
-        case BC_PERIODIC:   m.src = L - 1; break;
-        case BC_REFLECTING: m.src = N - 1 - L; m.mode = 1; break;
-        case BC_OUTFLOW:    m.src = N - 1; m.mode = 3; break;
-        case BC_INFLOW:     m.src = N - 1; m.mode = 5; break;
-        default:            m.src = N - 1; break;
-        }
-    }
-    // keep every read in bounds (positions far outside the domain are never
-    // used by a valid face)
-    m.src = m.src < 0 ? 0 : (m.src >= N ? N - 1 : m.src);
-    return m;
-}
-
-__device__ __forceinline__ void flip_normal(double& vn, int mode) {
-    switch (mode) {
-    case 1: vn = -vn; break;
-    case 2: if (vn > 0) vn = -1.0 * vn; break;
-    case 3: if (vn < 0) vn = -1.0 * vn; break;
-    case 4: if (vn < 0) vn = -1.0 * vn; break;
-    case 5: if (vn > 0) vn = -1.0 * vn; break;
-    default: break;
-    }
-}
-
-__device__ __forceinline__ void apply_ghost_mode(double q[7], int mode, int dim) {
-    // static register indices only (no dynamically indexed private array)
-    if (dim == 0) flip_normal(q[VEL + 0], mode);
-    else if (dim == 1) flip_normal(q[VEL + 1], mode);
-    else flip_normal(q[VEL + 2], mode);
-}
-
-// Load Q of the cell at global (x, y, z), which may lie in the 2-deep ghost
-// layer of one dimension (the first out-of-range dimension decides).
-template <int STAGE, bool MIXED>
-__device__ __forceinline__ void load_q(const HydroGeom& g, const double* __restrict__ U, int x, int y, int z,
-                                       double q[7], StepState* st, bool check) {
-    int c[3] = {x, y, z};
-    const int N[3] = {g.nx, g.ny, g.nz};
-    int mode = 0, mdim = 0;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        if ((c[d] < 0 || c[d] >= N[d]) && mode == 0 && mdim == 0) {
-            GhostMap m = ghost_map(c[d], N[d], g.bc_l[d], g.bc_r[d]);
-            c[d] = m.src;
-            mode = m.mode;
-            mdim = d + 1;
-        } else {
-            c[d] = c[d] < 0 ? 0 : (c[d] >= N[d] ? N[d] - 1 : c[d]);
-        }
-    }
-    const long long idx = (long long)c[0] + (long long)g.nx * ((long long)c[1] + (long long)g.ny * c[2]);
-    double u[7];
-#pragma unroll
-    for (int v = 0; v < 7; ++v) u[v] = U[v * g.ncell + idx];
-    if constexpr (MIXED) q_from_u_mixed(q, u, g.gm1);
-    else QfromU(q, u, g.gm1);
-    if (STAGE == 1) {
-        if constexpr (MIXED) fix_primitives_mixed(g, q, u[DEN]);
-        else fix_primitives(g, q, u[DEN]);
-        if (check) {
-            // Torch::checkValues("hydro before"), Torch.cpp:420-443
-            bool bad = (q[DEN] == 0.0) || (q[PRE] == 0.0);
-#pragma unroll
-            for (int v = 0; v < 7; ++v) bad |= (u[v] != u[v]) || isinf(u[v]);
-            if (bad) set_error(st, ERR_CHECK_VALUES);
-        }
-    }
-    if (mdim) apply_ghost_mode(q, mode, mdim - 1);
-}
-
-template <typename R>
 struct TileLDS {
-    R qc[7][TCELLS];
-    R qh[7][THALO];
-    R fl[7][TFACES];
+    double qc[7][TCELLS];
+    double qh[7][THALO];
+    double fl[3][7][TFACES];  // face fluxes of the three sweeps
     double red[NTHREADS / 64];
 };
 
@@ -291,8 +51,8 @@ __device__ __forceinline__ int center_index(int pos, int ta, int tb) {
     return ta + TILE * (tb + TILE * pos);
 }
 
-template <typename R, int DIM>
-__device__ __forceinline__ void q_at(const TileLDS<R>& s, int pos, int ta, int tb, R q[7]) {
+template <int DIM>
+__device__ __forceinline__ void q_at(const TileLDS& s, int pos, int ta, int tb, double q[7]) {
     if (pos >= 0 && pos < TILE) {
         const int i = center_index<DIM>(pos, ta, tb);
 #pragma unroll
@@ -308,59 +68,42 @@ __device__ __forceinline__ void q_at(const TileLDS<R>& s, int pos, int ta, int t
 // Flux through face f (between along-positions f-1 and f) of pencil (ta, tb)
 // of sweep DIM: reconstruction from the tile's LDS Q and the Riemann solve.
 // Returns true if the HII flux is NaN.
-template <int SOLVER, typename R, int DIM>
-__device__ __forceinline__ bool face_flux(const HydroGeom& g, const TileLDS<R>& s, int f, int ta, int tb,
-                                          const GasConsts<R>& gc, R F[7]) {
-    R ql[7], qr[7];
+template <int SOLVER, int DIM>
+__device__ __forceinline__ bool face_flux(const HydroGeom& g, const TileLDS& s, int f, int ta, int tb,
+                                          const GasConsts<double>& gc, double F[7]) {
+    double ql[7], qr[7];
     if (g.spatial_order == 1) {
-        R a[7], b[7], cc[7], dd[7];
-        q_at<R, DIM>(s, f - 2, ta, tb, a);
-        q_at<R, DIM>(s, f - 1, ta, tb, b);
-        q_at<R, DIM>(s, f, ta, tb, cc);
-        q_at<R, DIM>(s, f + 1, ta, tb, dd);
-        bool fast = false;
-        if constexpr (sizeof(R) == 4) {
-            if (g.limiter == LIM_ALBADA) {
-                fast = true;
+        double a[7], b[7], cc[7], dd[7];
+        q_at<DIM>(s, f - 2, ta, tb, a);
+        q_at<DIM>(s, f - 1, ta, tb, b);
+        q_at<DIM>(s, f, ta, tb, cc);
+        q_at<DIM>(s, f + 1, ta, tb, dd);
 #pragma unroll
-                for (int v = 0; v < 7; ++v) {
-                    ql[v] = b[v] + 0.5f * albada_f(b[v] - a[v], cc[v] - b[v]);
-                    qr[v] = cc[v] - 0.5f * albada_f(cc[v] - b[v], dd[v] - cc[v]);
-                }
-            }
-        }
-        if (!fast) {
-#pragma unroll
-            for (int v = 0; v < 7; ++v) {
-                R dl = limiter<R>(g.limiter, b[v] - a[v], cc[v] - b[v]);
-                ql[v] = b[v] + R(0.5) * dl;   // QR of cell f-1
-                R dr = limiter<R>(g.limiter, cc[v] - b[v], dd[v] - cc[v]);
-                qr[v] = cc[v] - R(0.5) * dr;  // QL of cell f
-            }
+        for (int v = 0; v < 7; ++v) {
+            double dl = limiter<double>(g.limiter, b[v] - a[v], cc[v] - b[v]);
+            ql[v] = b[v] + 0.5 * dl;   // QR of cell f-1
+            double dr = limiter<double>(g.limiter, cc[v] - b[v], dd[v] - cc[v]);
+            qr[v] = cc[v] - 0.5 * dr;  // QL of cell f
         }
     } else {
-        q_at<R, DIM>(s, f - 1, ta, tb, ql);
-        q_at<R, DIM>(s, f, ta, tb, qr);
+        q_at<DIM>(s, f - 1, ta, tb, ql);
+        q_at<DIM>(s, f, ta, tb, qr);
     }
-    if constexpr (sizeof(R) == 4) {
-        // mixed precision: hardware reciprocals, rotation-free hybrid flux
-        return riemann_f<SOLVER, DIM>(F, ql, qr, gc, g.igm1f);
-    } else {
-        R a_l2 = gc.g * ql[PRE] / ql[DEN];
-        R a_r2 = gc.g * qr[PRE] / qr[DEN];
-        return riemann<SOLVER, R, DIM>(F, ql, qr, a_l2, a_r2, gc);
-    }
+    double a_l2 = gc.g * ql[PRE] / ql[DEN];
+    double a_r2 = gc.g * qr[PRE] / qr[DEN];
+    return riemann<SOLVER, double, DIM>(F, ql, qr, a_l2, a_r2, gc);
 }
 
-template <int SOLVER, typename R, int STAGE, int DIM>
-__device__ __forceinline__ void sweep(const HydroGeom& g, TileLDS<R>& s, const double* __restrict__ Usrc,
-                                      const int o[3], double udot[7], StepState* st, const GasConsts<R>& gc) {
+template <int SOLVER, int STAGE, int DIM>
+__device__ __forceinline__ void sweep(const HydroGeom& g, TileLDS& s, const double* __restrict__ Usrc,
+                                      const int o[3], StepState* st, const GasConsts<double>& gc) {
     const int tid = threadIdx.x;
     const int N[3] = {g.nx, g.ny, g.nz};
     constexpr int DA = DIM == 0 ? 1 : 0;       // transverse dim "a"
     constexpr int DB = DIM == 2 ? 1 : 2;       // transverse dim "b"
 
-    // 1. halo slabs of this direction
+    // 1. halo slabs of this direction (the previous sweep's face phase, the
+    //    last reader of qh, ended at a barrier)
     if (tid < THALO) {
         const int h = tid >> 6, ta = tid & 7, tb = (tid >> 3) & 7;
         const int pos = h < 2 ? h - 2 : TILE + h - 2;
@@ -369,9 +112,9 @@ __device__ __forceinline__ void sweep(const HydroGeom& g, TileLDS<R>& s, const d
         c[DA] = o[DA] + ta;
         c[DB] = o[DB] + tb;
         double q[7];
-        load_q<STAGE, (sizeof(R) == 4)>(g, Usrc, c[0], c[1], c[2], q, st, false);
+        load_q<STAGE, false>(g, Usrc, c[0], c[1], c[2], q, st, false);
 #pragma unroll
-        for (int v = 0; v < 7; ++v) s.qh[v][tid] = (R)q[v];
+        for (int v = 0; v < 7; ++v) s.qh[v][tid] = q[v];
     }
     __syncthreads();
 
@@ -380,29 +123,37 @@ __device__ __forceinline__ void sweep(const HydroGeom& g, TileLDS<R>& s, const d
         const int f = tid % (TILE + 1), rest = tid / (TILE + 1);
         const int ta = rest & 7, tb = rest >> 3;
         const bool valid = (o[DIM] + f <= N[DIM]) && (o[DA] + ta < N[DA]) && (o[DB] + tb < N[DB]);
-        R F[7];
+        double F[7];
         bool bad = false;
-        if (valid) bad = face_flux<SOLVER, R, DIM>(g, s, f, ta, tb, gc, F);
+        if (valid) bad = face_flux<SOLVER, DIM>(g, s, f, ta, tb, gc, F);
         else {
 #pragma unroll
-            for (int v = 0; v < 7; ++v) F[v] = R(0);
+            for (int v = 0; v < 7; ++v) F[v] = 0.0;
         }
         if (bad) set_error(st, STAGE == 1 ? ERR_RIEMANN_NAN_1 : ERR_RIEMANN_NAN_2);
 #pragma unroll
-        for (int v = 0; v < 7; ++v) s.fl[v][tid] = F[v];
+        for (int v = 0; v < 7; ++v) s.fl[DIM][v][tid] = F[v];
     }
     __syncthreads();
+}
 
-    // 3. flux divergence in the reference order: += (A/V) F_L, -= (A/V) F_R
-    if (tid < TCELLS) {
-        const int lc[3] = {tid & 7, (tid >> 3) & 7, tid >> 6};
-        const int a = lc[DIM], ta = lc[DA], tb = lc[DB];
+// Flux divergence of one cell in the reference order, per dimension
+// += (A/V) F_L, -= (A/V) F_R (Hydro.cpp:117-137).  The accumulators are only
+// live here, not across the face solves.
+__device__ __forceinline__ void flux_divergence(const HydroGeom& g, const TileLDS& s, int tid, double udot[7]) {
+    const int lc[3] = {tid & 7, (tid >> 3) & 7, tid >> 6};
+#pragma unroll
+    for (int v = 0; v < 7; ++v) udot[v] = 0.0;
+#pragma unroll
+    for (int dim = 0; dim < 3; ++dim) {
+        const int a = lc[dim];
+        const int ta = lc[dim == 0 ? 1 : 0], tb = lc[dim == 2 ? 1 : 2];
         const int fL = (tb * TILE + ta) * (TILE + 1) + a;
 #pragma unroll
         for (int v = 0; v < 7; ++v) {
             double u = udot[v];
-            u += g.kflux * (double)s.fl[v][fL];
-            u -= g.kflux * (double)s.fl[v][fL + 1];
+            u += g.kflux * s.fl[dim][v][fL];
+            u -= g.kflux * s.fl[dim][v][fL + 1];
             udot[v] = u;
         }
     }
@@ -410,18 +161,17 @@ __device__ __forceinline__ void sweep(const HydroGeom& g, TileLDS<R>& s, const d
 
 // dtf_host >= 0 overrides the stage's dt (st->dt / 2.0 for the predictor,
 // st->dt for the corrector) with a host-given, already scaled dt.
-template <int SOLVER, typename R, int STAGE>
+template <int SOLVER, int STAGE>
 __global__ __launch_bounds__(NTHREADS) void hydro_stage_kernel(HydroGeom g, const double* __restrict__ Usrc,
                                                               const double* Ubase, double* Udst, StepState* st,
                                                               double dtf_host) {
-    __shared__ TileLDS<R> s;
+    __shared__ TileLDS s;
     if (st->halted) return;
     const int tile = tile_of_block(blockIdx.x, g.ntiles);
     const int tx = tile % g.ntx, ty = (tile / g.ntx) % g.nty, tz = tile / (g.ntx * g.nty);
     const int o[3] = {tx * TILE, ty * TILE, tz * TILE};
     const int tid = threadIdx.x;
-    const GasConsts<R> gc = {(R)(sizeof(R) == 8 ? g.g : g.gf), (R)(sizeof(R) == 8 ? g.gm1 : g.gm1f),
-                             (R)(sizeof(R) == 8 ? g.g2 : g.g2f)};
+    const GasConsts<double> gc = {g.g, g.gm1, g.g2};
 
     // tile cells: Q into LDS (ghost mapping for positions outside the grid)
     const int lx = tid & 7, ly = (tid >> 3) & 7, lz = tid >> 6;
@@ -429,24 +179,24 @@ __global__ __launch_bounds__(NTHREADS) void hydro_stage_kernel(HydroGeom g, cons
     const bool active = tid < TCELLS && gx < g.nx && gy < g.ny && gz < g.nz;
     if (tid < TCELLS) {
         double q[7];
-        load_q<STAGE, (sizeof(R) == 4)>(g, Usrc, gx, gy, gz, q, st, active);
+        load_q<STAGE, false>(g, Usrc, gx, gy, gz, q, st, active);
 #pragma unroll
-        for (int v = 0; v < 7; ++v) s.qc[v][tid] = (R)q[v];
+        for (int v = 0; v < 7; ++v) s.qc[v][tid] = q[v];
     }
-    double udot[7] = {0, 0, 0, 0, 0, 0, 0};
-
-    sweep<SOLVER, R, STAGE, 0>(g, s, Usrc, o, udot, st, gc);
-    sweep<SOLVER, R, STAGE, 1>(g, s, Usrc, o, udot, st, gc);
-    sweep<SOLVER, R, STAGE, 2>(g, s, Usrc, o, udot, st, gc);
+    sweep<SOLVER, STAGE, 0>(g, s, Usrc, o, st, gc);
+    sweep<SOLVER, STAGE, 1>(g, s, Usrc, o, st, gc);
+    sweep<SOLVER, STAGE, 2>(g, s, Usrc, o, st, gc);
 
     double inv_t = 0.0;
     if (active) {
+        double udot[7];
+        flux_divergence(g, s, tid, udot);
         // gravity source with GRAV = 0 (Hydro.cpp:134-137)
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             const double grav = 0.0;
             udot[VEL + i] += grav;
-            udot[PRE] += (double)s.qc[VEL + i][tid] * grav;
+            udot[PRE] += s.qc[VEL + i][tid] * grav;
         }
         const long long c = (long long)gx + (long long)g.nx * ((long long)gy + (long long)g.ny * gz);
         const double dtf = dtf_host >= 0.0 ? dtf_host : (STAGE == 1 ? st->dt / 2.0 : st->dt);
@@ -455,21 +205,14 @@ __global__ __launch_bounds__(NTHREADS) void hydro_stage_kernel(HydroGeom g, cons
         for (int v = 0; v < 7; ++v) u[v] = Ubase[v * g.ncell + c];
 #pragma unroll
         for (int v = 0; v < 7; ++v) u[v] += dtf * udot[v];
-        const bool bad = sizeof(R) == 4 ? fix_solution_mixed(g, u) : fix_solution(g, u);
-        if (bad) set_error(st, STAGE == 1 ? ERR_FIX_SOLUTION_1 : ERR_FIX_SOLUTION_2);
+        if (fix_solution(g, u)) set_error(st, STAGE == 1 ? ERR_FIX_SOLUTION_1 : ERR_FIX_SOLUTION_2);
 #pragma unroll
         for (int v = 0; v < 7; ++v) Udst[v * g.ncell + c] = u[v];
         if (STAGE == 2) {
             double q[7];
-            if constexpr (sizeof(R) == 4) {
-                q_from_u_mixed(q, u, g.gm1);
-                fix_primitives_mixed(g, q, u[DEN]);
-                inv_t = cfl_inv_t_mixed(g, q);
-            } else {
-                QfromU(q, u, g.gm1);
-                fix_primitives(g, q, u[DEN]);
-                inv_t = cfl_inv_t(g, q);
-            }
+            QfromU(q, u, g.gm1);
+            fix_primitives(g, q, u[DEN]);
+            inv_t = cfl_inv_t(g, q);
         }
     }
     if (STAGE == 2) block_max_to(inv_t, &st->maxinv, s.red);
@@ -598,14 +341,14 @@ hipError_t launch_fix_solution(const HydroGeom& g, double* U, const double* rate
     return hipGetLastError();
 }
 
-template <int SOLVER, typename R>
+template <int SOLVER>
 static hipError_t launch_stage_t(const HydroGeom& g, int stage, const double* Usrc, const double* Ubase, double* Udst,
                                  StepState* st, double dtf, hipStream_t stream) {
     dim3 grid(g.ntiles), block(NTHREADS);
     if (stage == 1)
-        hipLaunchKernelGGL((hydro_stage_kernel<SOLVER, R, 1>), grid, block, 0, stream, g, Usrc, Ubase, Udst, st, dtf);
+        hipLaunchKernelGGL((hydro_stage_kernel<SOLVER, 1>), grid, block, 0, stream, g, Usrc, Ubase, Udst, st, dtf);
     else
-        hipLaunchKernelGGL((hydro_stage_kernel<SOLVER, R, 2>), grid, block, 0, stream, g, Usrc, Ubase, Udst, st, dtf);
+        hipLaunchKernelGGL((hydro_stage_kernel<SOLVER, 2>), grid, block, 0, stream, g, Usrc, Ubase, Udst, st, dtf);
     return hipGetLastError();
 }
 
@@ -613,16 +356,12 @@ hipError_t launch_hydro_stage(const HydroGeom& g, int solver, int precision, int
                               const double* Ubase, double* Udst, StepState* st, hipStream_t stream, double dtf) {
     if (precision == 0) {
         switch (solver) {
-        case SOLVER_HLL:  return launch_stage_t<SOLVER_HLL, double>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
-        case SOLVER_HLLC: return launch_stage_t<SOLVER_HLLC, double>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
-        default:          return launch_stage_t<SOLVER_ROTATED, double>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
+        case SOLVER_HLL:  return launch_stage_t<SOLVER_HLL>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
+        case SOLVER_HLLC: return launch_stage_t<SOLVER_HLLC>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
+        default:          return launch_stage_t<SOLVER_ROTATED>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
         }
     }
-    switch (solver) {
-    case SOLVER_HLL:  return launch_stage_t<SOLVER_HLL, float>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
-    case SOLVER_HLLC: return launch_stage_t<SOLVER_HLLC, float>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
-    default:          return launch_stage_t<SOLVER_ROTATED, float>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
-    }
+    return launch_hydro_stage_mixed(g, solver, stage, Usrc, Ubase, Udst, st, stream, dtf);
 }
 
 hipError_t launch_cfl(const HydroGeom& g, const double* U, StepState* st, hipStream_t stream) {

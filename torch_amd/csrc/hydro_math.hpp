@@ -127,60 +127,59 @@ __device__ __forceinline__ void wave_speeds(R a_l2, R a_r2, const R* Ql, const R
 }
 
 // HartenLaxLeerContactSolver::solve.  Returns true when the HII flux is NaN
-// (the reference throws there).
+// (the reference throws there).  Written branch-free: the upwind side is
+// selected first and F(Q_side) is the reference's supersonic flux and the
+// Fk of its subsonic star-state flux alike, so every lane runs one FfromQ;
+// the values are those of the reference's branches.
 template <typename R, int DIM>
 __device__ __forceinline__ bool hllc(R F[7], const R* Ql, const R* Qr, R a_l2, R a_r2, const GasConsts<R>& c) {
     R S_l, S_r;
     wave_speeds<R, DIM>(a_l2, a_r2, Ql, Qr, S_l, S_r);
-    if (S_l >= R(0)) {
-        FfromQ<R, DIM>(F, Ql, c);
-    } else if (S_r <= R(0)) {
-        FfromQ<R, DIM>(F, Qr, c);
-    } else {
-        R S_c = Qr[PRE] - Ql[PRE];
-        S_c += Ql[DEN] * Ql[VEL + DIM] * (S_l - Ql[VEL + DIM]);
-        S_c -= Qr[DEN] * Qr[VEL + DIM] * (S_r - Qr[VEL + DIM]);
-        S_c /= (Ql[DEN] * (S_l - Ql[VEL + DIM]) - Qr[DEN] * (S_r - Qr[VEL + DIM]));
-        const bool left = S_c >= R(0);
-        R Q[7];
+    R S_c = Qr[PRE] - Ql[PRE];
+    S_c += Ql[DEN] * Ql[VEL + DIM] * (S_l - Ql[VEL + DIM]);
+    S_c -= Qr[DEN] * Qr[VEL + DIM] * (S_r - Qr[VEL + DIM]);
+    S_c /= (Ql[DEN] * (S_l - Ql[VEL + DIM]) - Qr[DEN] * (S_r - Qr[VEL + DIM]));
+    const bool sup_l = S_l >= R(0);
+    const bool sup_r = !sup_l && S_r <= R(0);
+    const bool left = sup_l || (!sup_r && S_c >= R(0));
+    R Q[7];
 #pragma unroll
-        for (int i = 0; i < 7; ++i) Q[i] = left ? Ql[i] : Qr[i];
-        const R S = left ? S_l : S_r;
-        R U[7], Fk[7], Uc[7];
-        UfromQ<R>(U, Q, c);
-        FfromQ<R, DIM>(Fk, Q, c);
-        R A = Q[DEN] * (S - Q[VEL + DIM]) / (S - S_c);
-        Uc[DEN] = A;
+    for (int i = 0; i < 7; ++i) Q[i] = left ? Ql[i] : Qr[i];
+    const R S = left ? S_l : S_r;
+    R U[7], Fk[7], Uc[7];
+    UfromQ<R>(U, Q, c);
+    FfromQ<R, DIM>(Fk, Q, c);
+    R A = Q[DEN] * (S - Q[VEL + DIM]) / (S - S_c);
+    Uc[DEN] = A;
 #pragma unroll
-        for (int id = 0; id < 3; ++id) Uc[VEL + id] = A * Q[VEL + id];
-        Uc[VEL + DIM] = A * S_c;
-        Uc[PRE] = A * ((U[PRE] / Q[DEN]) + (S_c - Q[VEL + DIM]) * (S_c + Q[PRE] / (Q[DEN] * (S - Q[VEL + DIM]))));
-        Uc[HII] = A * Q[HII];
-        Uc[ADV] = A * Q[ADV];
+    for (int id = 0; id < 3; ++id) Uc[VEL + id] = A * Q[VEL + id];
+    Uc[VEL + DIM] = A * S_c;
+    Uc[PRE] = A * ((U[PRE] / Q[DEN]) + (S_c - Q[VEL + DIM]) * (S_c + Q[PRE] / (Q[DEN] * (S - Q[VEL + DIM]))));
+    Uc[HII] = A * Q[HII];
+    Uc[ADV] = A * Q[ADV];
+    const bool sub = !sup_l && !sup_r;
 #pragma unroll
-        for (int i = 0; i < 7; ++i) F[i] = Fk[i] + S * (Uc[i] - U[i]);
-    }
+    for (int i = 0; i < 7; ++i) F[i] = sub ? Fk[i] + S * (Uc[i] - U[i]) : Fk[i];
     return F[HII] != F[HII];
 }
 
-// HartenLaxLeerSolver::solve.
+// HartenLaxLeerSolver::solve, branch-free (both one-sided fluxes are needed
+// by the subsonic case; the three candidates are selected per lane).
 template <typename R, int DIM>
 __device__ __forceinline__ bool hll(R F[7], const R* Ql, const R* Qr, R a_l2, R a_r2, const GasConsts<R>& c) {
     R S_l, S_r;
     wave_speeds<R, DIM>(a_l2, a_r2, Ql, Qr, S_l, S_r);
-    if (S_r <= R(0)) {
-        FfromQ<R, DIM>(F, Qr, c);
-    } else if (S_l >= R(0)) {
-        FfromQ<R, DIM>(F, Ql, c);
-    } else {
-        R U_l[7], U_r[7], F_l[7], F_r[7];
-        UfromQ<R>(U_l, Ql, c);
-        UfromQ<R>(U_r, Qr, c);
-        FfromQ<R, DIM>(F_l, Ql, c);
-        FfromQ<R, DIM>(F_r, Qr, c);
+    R U_l[7], U_r[7], F_l[7], F_r[7];
+    UfromQ<R>(U_l, Ql, c);
+    UfromQ<R>(U_r, Qr, c);
+    FfromQ<R, DIM>(F_l, Ql, c);
+    FfromQ<R, DIM>(F_r, Qr, c);
+    const bool right = S_r <= R(0);
+    const bool left = !right && S_l >= R(0);
 #pragma unroll
-        for (int i = 0; i < 7; ++i)
-            F[i] = (S_r * F_l[i] - S_l * F_r[i] + S_l * S_r * (U_r[i] - U_l[i])) / (S_r - S_l);
+    for (int i = 0; i < 7; ++i) {
+        const R hl = (S_r * F_l[i] - S_l * F_r[i] + S_l * S_r * (U_r[i] - U_l[i])) / (S_r - S_l);
+        F[i] = right ? F_r[i] : (left ? F_l[i] : hl);
     }
     return F[HII] != F[HII];
 }
@@ -269,34 +268,49 @@ __device__ __forceinline__ bool rotated_hllc(R F[7], const R* Ql, const R* Qr, R
             }
         }
     }
-    bool bad = false;
-    if (!pureHLLC && !pureHLL) {
-        R F1[7], F2[7], QlR[7], QrR[7], M[3][3];
+    // One HLL and one HLLC solve per face for every lane: on the hybrid path
+    // they take the rotated states, on the pure paths the unrotated ones (the
+    // reference's single solve; the other result is discarded), so a wave
+    // with mixed paths runs the solvers once instead of once per path.
+    const bool hybrid = !pureHLLC && !pureHLL;
+    R F1[7], F2[7], QlR[7], QrR[7], Qa[7], Qb[7], M[3][3];
 #pragma unroll
-        for (int i = 0; i < 7; ++i) { QlR[i] = Ql[i]; QrR[i] = Qr[i]; }
-        rot_a2b(n1, d, M);
-        rotate(M, QlR);
-        rotate(M, QrR);
-        bad |= hll<R, DIM>(F1, QlR, QrR, a_l2, a_r2, c);
-        rot_a2b(d, n1, M);
-        rotate(M, QlR);
-        rotate(M, QrR);
-        rotate(M, F1);
-        rot_a2b(n2, d, M);
-        rotate(M, QlR);
-        rotate(M, QrR);
-        bad |= hllc<R, DIM>(F2, QlR, QrR, a_l2, a_r2, c);
-        rot_a2b(d, n2, M);
-        rotate(M, F2);
-        const R w1 = fabs(dot3(d, n1)), w2 = fabs(dot3(d, n2));
+    for (int i = 0; i < 7; ++i) { QlR[i] = Ql[i]; QrR[i] = Qr[i]; }
+    rot_a2b(n1, d, M);
+    rotate(M, QlR);
+    rotate(M, QrR);
 #pragma unroll
-        for (int i = 0; i < 7; ++i) F[i] = w1 * F1[i] + w2 * F2[i];
-    } else if (pureHLL) {
-        bad |= hll<R, DIM>(F, Ql, Qr, a_l2, a_r2, c);
-    } else {
-        bad |= hllc<R, DIM>(F, Ql, Qr, a_l2, a_r2, c);
+    for (int i = 0; i < 7; ++i) { Qa[i] = hybrid ? QlR[i] : Ql[i]; Qb[i] = hybrid ? QrR[i] : Qr[i]; }
+    const bool bad1 = hll<R, DIM>(F1, Qa, Qb, a_l2, a_r2, c);
+    rot_a2b(d, n1, M);
+    rotate(M, QlR);
+    rotate(M, QrR);
+    // A = w1 * (M F1) on the hybrid path (the first term of the reference's
+    // w1 F1 + w2 F2), F1 itself on the pure-HLL path
+    const R w1 = fabs(dot3(d, n1));
+    {
+        R Fr[7];
+#pragma unroll
+        for (int i = 0; i < 7; ++i) Fr[i] = F1[i];
+        rotate(M, Fr);
+#pragma unroll
+        for (int i = 0; i < 7; ++i) F1[i] = hybrid ? w1 * Fr[i] : F1[i];
     }
-    return bad;
+    rot_a2b(n2, d, M);
+    rotate(M, QlR);
+    rotate(M, QrR);
+#pragma unroll
+    for (int i = 0; i < 7; ++i) { Qa[i] = hybrid ? QlR[i] : Ql[i]; Qb[i] = hybrid ? QrR[i] : Qr[i]; }
+    const bool bad2 = hllc<R, DIM>(F2, Qa, Qb, a_l2, a_r2, c);
+    rot_a2b(d, n2, M);
+    const R w2 = fabs(dot3(d, n2));
+    R Fr[7];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) Fr[i] = F2[i];
+    rotate(M, Fr);
+#pragma unroll
+    for (int i = 0; i < 7; ++i) F[i] = hybrid ? F1[i] + w2 * Fr[i] : (pureHLL ? F1[i] : F2[i]);
+    return hybrid ? (bad1 || bad2) : (pureHLL ? bad1 : bad2);
 }
 
 template <int SOLVER, typename R, int DIM>

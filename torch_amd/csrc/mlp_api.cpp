@@ -203,7 +203,7 @@ int torchgpu_mlp_create(int max_batch, void* stream, torchgpu_mlp** out) {
 
 void torchgpu_mlp_destroy(torchgpu_mlp* m) {
     if (!m) return;
-    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    (void)hipStreamSynchronize(m->stream);
     std::vector<void*> ptrs = {m->W, m->b, m->W16, m->gW, m->gb, m->mW, m->vW, m->mb, m->vb, m->Z[5], m->dy32,
                                m->dy16, m->feat32, m->partial, m->bpartial, m->loss_partial, m->H[0]};
     for (int L = 1; L <= 4; ++L) {

@@ -165,7 +165,7 @@ __device__ __forceinline__ void load_tables(const double* tables, double* sx, do
 
 // counters[0]: work counter; counters[1]: total subcycles.  Work items are
 // 0..count-1 (count = *count_dev when given), cell = order ? order[i] : i.
-__global__ __launch_bounds__(256) void thermo_cells_kernel(ThermoConsts t, const double* __restrict__ tables,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void thermo_cells_kernel(ThermoConsts t, const double* __restrict__ tables,
                                                           long long stride, long long count,
                                                           const unsigned long long* __restrict__ count_dev,
                                                           const long long* __restrict__ order,
@@ -442,6 +442,82 @@ __global__ __launch_bounds__(256) void thermo_timestep_kernel(ThermoConsts t, co
 
 __global__ void set_u64_kernel(unsigned long long* p, unsigned long long v) { *p = v; }
 
+// ---------------------------------------------------------------------------
+// Longest-first work order for the persistent FP64 kernel.  A cell's cost is
+// its subcycle count, known from the pre-step rate (thermo_init), and spans
+// 1 .. ~1e8; handed out in index order, a long cell that starts late ends
+// the launch long after the rest.  Cells are bucketed by floor(log2(count))
+// and the kernel takes the buckets longest first (LPT scheduling, within a
+// factor of two of the exact order), so the launch ends near
+// max(longest cell, total work / throughput).  Results do not depend on the
+// order: every cell is integrated independently.
+// ---------------------------------------------------------------------------
+constexpr int LPT_NB = 64;
+
+__global__ __launch_bounds__(256) void lpt_bucket_kernel(ThermoConsts t, const double* __restrict__ tables,
+                                                        long long stride, long long count,
+                                                        const unsigned long long* __restrict__ count_dev,
+                                                        const long long* __restrict__ list,
+                                                        const double* __restrict__ in,
+                                                        unsigned char* __restrict__ bucket,
+                                                        unsigned long long* __restrict__ hist,
+                                                        unsigned int* __restrict__ err) {
+    __shared__ double sx[26], sy[26], sy2[26];
+    __shared__ unsigned int h[LPT_NB];
+    if (threadIdx.x < LPT_NB) h[threadIdx.x] = 0;
+    load_tables(tables, sx, sy, sy2);
+    if (count_dev) count = (long long)*count_dev;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (long long)gridDim.x * blockDim.x) {
+        const long long cell = list ? list[i] : i;
+        CellState s;
+        thermo_init(t, sx, sy, sy2, in, stride, cell, s, err);
+        const int b = s.remaining > 0 ? 64 - __clzll(s.remaining) : 0;
+        bucket[i] = (unsigned char)b;
+        atomicAdd(&h[b], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < LPT_NB && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], (unsigned long long)h[threadIdx.x]);
+}
+
+// cursor[b] = number of items in buckets above b (longest first)
+__global__ void lpt_offsets_kernel(const unsigned long long* __restrict__ hist, unsigned long long* __restrict__ cursor) {
+    if (threadIdx.x != 0) return;
+    unsigned long long off = 0;
+    for (int b = LPT_NB - 1; b >= 0; --b) {
+        cursor[b] = off;
+        off += hist[b];
+    }
+}
+
+__global__ __launch_bounds__(256) void lpt_scatter_kernel(long long count, const unsigned long long* __restrict__ count_dev,
+                                                         const long long* __restrict__ list,
+                                                         const unsigned char* __restrict__ bucket,
+                                                         unsigned long long* __restrict__ cursor,
+                                                         long long* __restrict__ order) {
+    __shared__ unsigned int h[LPT_NB];
+    __shared__ unsigned long long base[LPT_NB];
+    if (count_dev) count = (long long)*count_dev;
+    const int tid = threadIdx.x;
+    // the trip count is uniform over the block (barriers inside)
+    for (long long i0 = (long long)blockIdx.x * blockDim.x; i0 < count; i0 += (long long)gridDim.x * blockDim.x) {
+        if (tid < LPT_NB) h[tid] = 0;
+        __syncthreads();
+        const long long i = i0 + tid;
+        int b = -1;
+        unsigned int rank = 0;
+        if (i < count) {
+            b = bucket[i];
+            rank = atomicAdd(&h[b], 1u);
+        }
+        __syncthreads();
+        if (tid < LPT_NB && h[tid]) base[tid] = atomicAdd(&cursor[tid], (unsigned long long)h[tid]);
+        __syncthreads();
+        if (b >= 0) order[base[b] + rank] = list ? list[i] : i;
+        __syncthreads();
+    }
+}
+
 }  // namespace tg
 
 struct torchgpu_cooling {
@@ -463,6 +539,11 @@ struct torchgpu_cooling {
     float* feat = nullptr;
     float* y = nullptr;
     tg::SurrCell* sdata = nullptr;
+    // longest-first order of the FP64 kernel's work items
+    long long lpt_cap = 0;
+    long long* lpt_order = nullptr;
+    unsigned char* lpt_bucket = nullptr;
+    unsigned long long* lpt_hist = nullptr;   // [0, 64) histogram, [64, 128) cursors
     // grid-operator scratch
     long long grid_cap = 0;
     double* in9 = nullptr;
@@ -521,6 +602,41 @@ int launch_fp64(torchgpu_cooling* c, long long stride, long long count, const un
     hipLaunchKernelGGL(tg::thermo_cells_kernel, dim3((unsigned)blocks), dim3(256), 0, c->stream, c->t, c->tables,
                        stride, count, count_dev, order, in, out, c->counters, c->err, path);
     HIP_OK(hipGetLastError());
+    return TORCHGPU_OK;
+}
+
+// Work order (longest cells first) of `count` items (count_dev: on the
+// device) that are cells list[i] (list == NULL: i); returns c->lpt_order.
+int lpt_order(torchgpu_cooling* c, long long stride, long long count, const unsigned long long* count_dev,
+              const long long* list, const double* in, const long long** order) {
+    if (count > c->lpt_cap) {
+        (void)hipStreamSynchronize(c->stream);
+        if (c->lpt_order) (void)hipFree(c->lpt_order);
+        if (c->lpt_bucket) (void)hipFree(c->lpt_bucket);
+        c->lpt_order = nullptr;
+        c->lpt_bucket = nullptr;
+        c->lpt_cap = 0;
+        HIP_OK(hipMalloc(&c->lpt_order, count * sizeof(long long)));
+        HIP_OK(hipMalloc(&c->lpt_bucket, count));
+        c->lpt_cap = count;
+    }
+    if (!c->lpt_hist) HIP_OK(hipMalloc(&c->lpt_hist, 2 * tg::LPT_NB * sizeof(unsigned long long)));
+    hipLaunchKernelGGL(tg::zero_u64_kernel, dim3(1), dim3(2 * tg::LPT_NB), 0, c->stream, c->lpt_hist,
+                       2 * tg::LPT_NB);
+    HIP_OK(hipGetLastError());
+    long long blocks = (count + 255) / 256;
+    if (blocks > (long long)c->nsm * 16) blocks = (long long)c->nsm * 16;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(tg::lpt_bucket_kernel, dim3((unsigned)blocks), dim3(256), 0, c->stream, c->t, c->tables,
+                       stride, count, count_dev, list, in, c->lpt_bucket, c->lpt_hist, c->err);
+    HIP_OK(hipGetLastError());
+    hipLaunchKernelGGL(tg::lpt_offsets_kernel, dim3(1), dim3(64), 0, c->stream, c->lpt_hist,
+                       c->lpt_hist + tg::LPT_NB);
+    HIP_OK(hipGetLastError());
+    hipLaunchKernelGGL(tg::lpt_scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, c->stream, count, count_dev,
+                       list, c->lpt_bucket, c->lpt_hist + tg::LPT_NB, c->lpt_order);
+    HIP_OK(hipGetLastError());
+    *order = c->lpt_order;
     return TORCHGPU_OK;
 }
 
@@ -587,9 +703,10 @@ int torchgpu_cooling_create(const torchgpu_thermo_params* p, void* stream, torch
 
 void torchgpu_cooling_destroy(torchgpu_cooling* c) {
     if (!c) return;
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(c->stream);
     for (void* ptr : {(void*)c->tables, (void*)c->counters, (void*)c->err, (void*)c->mlp_list, (void*)c->fb_list,
-                      (void*)c->feat, (void*)c->y, (void*)c->sdata, (void*)c->in9, (void*)c->out5, (void*)c->dtbits})
+                      (void*)c->feat, (void*)c->y, (void*)c->sdata, (void*)c->in9, (void*)c->out5, (void*)c->dtbits,
+                      (void*)c->lpt_order, (void*)c->lpt_bucket, (void*)c->lpt_hist})
         if (ptr) (void)hipFree(ptr);
     delete c;
 }
@@ -608,7 +725,9 @@ int torchgpu_thermo_cells(torchgpu_cooling* c, int64_t n, const double* in, doub
     }
     int rc = zero_counters(c);
     if (rc) return rc;
-    rc = launch_fp64(c, n, n, nullptr, nullptr, in, out, nullptr);
+    const long long* order = nullptr;
+    if ((rc = lpt_order(c, n, n, nullptr, nullptr, in, &order))) return rc;
+    rc = launch_fp64(c, n, n, nullptr, order, in, out, nullptr);
     if (rc) return rc;
     if (total_substeps) {
         unsigned long long tot = 0;
@@ -658,7 +777,9 @@ int torchgpu_thermo_cells_surrogate(torchgpu_cooling* c, int64_t n, const double
     // fallback queue through the FP64 integrator (work counter reset first)
     hipLaunchKernelGGL(tg::zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, c->counters, 1);
     HIP_OK(hipGetLastError());
-    if ((rc = launch_fp64(c, n, n, c->counters + 3, c->fb_list, in, out, path))) return rc;
+    const long long* order = nullptr;
+    if ((rc = lpt_order(c, n, n, c->counters + 3, c->fb_list, in, &order))) return rc;
+    if ((rc = launch_fp64(c, n, n, c->counters + 3, order, in, out, path))) return rc;
     if (counts) {
         unsigned long long h[4];
         HIP_OK(hipMemcpyAsync(h, c->counters, sizeof h, hipMemcpyDeviceToHost, c->stream));

@@ -1,9 +1,10 @@
 // thermo_math.hpp -- Henney et al. (2009) heating/cooling functions and the
 // subcycled explicit source step of the reference's Thermodynamics operator
 // (hgsteggles/TORCH src/Integrators/Thermodynamics.cpp), FP64, for the HIP
-// kernels.  Operand order follows the reference; exp/log/log10/pow/sqrt are
-// the device library's (ocml), so agreement with the x86 build is to within
-// the libm ulp differences rather than bitwise.
+// kernels.  Operand order follows the reference except inside the cooling
+// sum, whose exponentials share one reciprocal and one log of T (see
+// subtract_cooling); exp/log/pow are the device library's (ocml), so
+// agreement with the x86 build is to within rounding rather than bitwise.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -24,17 +25,19 @@ struct ThermoConsts {
 
 // Natural cubic spline in log10 space: LogSplineData::interpolate +
 // SplineData::splint (SplineData.cpp:57-81,142-151).  x, y, y2 are 26-entry
-// tables (LDS).
+// tables (LDS) whose abscissae are log10 T = 3.5, 3.6, ..., 6.0 up to
+// rounding: the bracketing interval of the reference's bisection (largest
+// klo <= 24 with x[klo] <= x) is found from the spacing and corrected by one
+// step against the stored abscissae, which gives the same klo.
 __device__ __forceinline__ double log_spline(const ThermoConsts& t, const double* sx, const double* sy,
                                              const double* sy2, double x) {
     if (x > sx[25]) return sy[25] + t.spl_max_slope * (x - sx[25]);
     if (x < sx[0]) return sy[0] + t.spl_min_slope * (x - sx[0]);
-    int klo = 0, khi = 25;
-    while (khi - klo > 1) {
-        int k = (khi + klo) >> 1;
-        if (sx[k] > x) khi = k;
-        else klo = k;
-    }
+    int klo = (int)((x - sx[0]) * 10.0);
+    klo = klo < 0 ? 0 : (klo > 24 ? 24 : klo);
+    if (sx[klo] > x) --klo;
+    else if (klo < 24 && sx[klo + 1] <= x) ++klo;
+    const int khi = klo + 1;
     double h = sx[khi] - sx[klo];
     double a = (sx[khi] - x) / h;
     double b = (x - sx[klo]) / h;
@@ -75,24 +78,40 @@ __device__ __forceinline__ CoolingCell cooling_cell(const ThermoConsts& t, doubl
 }
 
 // Sum of the five cooling terms, subtracted in the reference's order from
-// `rate` (Thermodynamics.cpp:255-259 / 315-319).
+// `rate` (Thermodynamics.cpp:255-259 / 315-319).  The transcendental work is
+// shared between the terms: one reciprocal and one natural log of T feed
+// every exponent (x/T = x * (1/T); log10 T = ln T / ln 10; sqrt(T) e^-a =
+// e^(ln T / 2 - a); pow(1e-5 T, 1.63) = e^(1.63 (ln T + ln 1e-5))), leaving
+// six exponentials per evaluation instead of ten libm calls.  The values
+// differ from the reference's by rounding of the exponents only (relative
+// ~1e-15 on each term).
 __device__ __forceinline__ double subtract_cooling(const ThermoConsts& t, const CoolingCell& c, const double* sx,
                                                    const double* sy, const double* sy2, double rate, double T) {
-    rate -= c.imlc_k * exp(-t.T1 / T - (t.T2 / T) * (t.T2 / T));
-    rate -= c.nmlc_k * exp(-t.T3 / T - (t.T4 / T) * (t.T4 / T));
+    const double iT = 1.0 / T;
+    const double lnT = log(T);
     {
-        double r = log_spline(t, sx, sy, sy2, log10(T));
-        rate -= c.cehi_k * exp((2.302585093 * r) - ((T / t.cxhi_damp) * (T / t.cxhi_damp)));
+        const double a = t.T2 * iT;
+        rate -= c.imlc_k * exp(-t.T1 * iT - a * a);
+    }
+    {
+        const double a = t.T4 * iT;
+        rate -= c.nmlc_k * exp(-t.T3 * iT - a * a);
+    }
+    {
+        const double r = log_spline(t, sx, sy, sy2, lnT * 0.43429448190325182765);
+        const double a = T * (1.0 / t.cxhi_damp);
+        rate -= c.cehi_k * exp((2.302585093 * r) - a * a);
     }
     if (T > t.ciec_minT) {
-        double cie_rate = c.ciec_k * exp(-0.63 * log(T)) * (1.0 - exp(-pow(1.0e-5 * T, 1.63)));
+        const double p163 = exp(1.63 * (lnT - 11.512925464970228420));
+        double cie_rate = c.ciec_k * exp(-0.63 * lnT) * (1.0 - exp(-p163));
         double smoothing = (T - 5.0e4) / (2.0e4);
         smoothing = (smoothing < 1.0) ? smoothing : 1.0;
         rate -= cie_rate * smoothing;
     } else {
         rate -= 0;
     }
-    rate -= c.nmc_k * sqrt(T) * exp(-c.nmc_T0 / T);
+    rate -= c.nmc_k * exp(0.5 * lnT - c.nmc_T0 * iT);
     return rate;
 }
 

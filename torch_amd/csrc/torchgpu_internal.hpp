@@ -80,6 +80,11 @@ hipError_t launch_fix_solution(const HydroGeom& g, double* U, const double* dpre
 // precision: 0 = FP64 everywhere, 1 = FP32 reconstruction + Riemann flux
 // with FP64 accumulation, pressure recovery and CFL.
 // dtf < 0: the stage takes its dt from st (dt/2 predictor, dt corrector)
+// Mixed-precision stage kernel (hydro_mixed.hip); launch_hydro_stage
+// dispatches to it for precision 1.
+hipError_t launch_hydro_stage_mixed(const HydroGeom& g, int solver, int stage, const double* Usrc,
+                                    const double* Ubase, double* Udst, StepState* st, hipStream_t stream,
+                                    double dtf);
 hipError_t launch_hydro_stage(const HydroGeom& g, int solver, int precision, int stage,
                               const double* Usrc, const double* Ubase, double* Udst,
                               StepState* st, hipStream_t stream, double dtf = -1.0);

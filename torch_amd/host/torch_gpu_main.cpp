@@ -39,8 +39,13 @@ void dump(const tg::TorchParameters& p) {
     std::printf(" \"bc\": [\"%s\", \"%s\", \"%s\", \"%s\", \"%s\", \"%s\"], \"cooling_on\": %s, \"ncheckpoints\": %d,\n",
                 p.leftBC[0].c_str(), p.rightBC[0].c_str(), p.leftBC[1].c_str(), p.rightBC[1].c_str(),
                 p.leftBC[2].c_str(), p.rightBC[2].c_str(), p.cooling_on ? "true" : "false", p.ncheckpoints);
-    std::printf(" \"mass_fraction_h\": %.17g, \"thermo_hii_switch\": %.17g, \"precision\": \"%s\"}\n", p.massFractionH,
+    std::printf(" \"mass_fraction_h\": %.17g, \"thermo_hii_switch\": %.17g, \"precision\": \"%s\",\n", p.massFractionH,
                 p.thermoHII_Switch, p.precision.c_str());
+    // the same after TorchParameters::initialise (code units)
+    tg::TorchParameters c = p;
+    c.initialise();
+    std::printf(" \"code_units\": {\"side_length\": %.17g, \"tmax\": %.17g, \"dt_max\": %.17g, \"dfloor\": %.17g, "
+                "\"pfloor\": %.17g}}\n", c.sideLength, c.tmax, c.dt_max, c.dfloor, c.pfloor);
 }
 
 }  // namespace
