@@ -259,7 +259,7 @@ def main():
         log(f"cooling {cool} ({time.perf_counter() - t0:.1f}s)")
     ncell = args.n ** 3
     achieved = BYTES_STAGE2 * ncell / (mx["stage2_ms"] * 1e-3) / 1e9
-    traffic = measured_traffic(args.n, "void tg::hydro_stage_kernel<2, float, 2>")
+    traffic = measured_traffic(args.n, "void tg::hydro_stage_mixed_kernel<2, 2>")
     out = {
         "metric": "Mcell-updates/s: FP64 vs mixed hydro; surrogate vs subcycled cooling",
         "value": round(mx["mcells"], 2),
@@ -280,7 +280,7 @@ def main():
                  "stage1_ms": round(f64["stage1_ms"], 4), "stage2_ms": round(f64["stage2_ms"], 4)},
         "mixed_over_fp64": round(mx["mcells"] / f64["mcells"], 3),
         "mixed_vs_fp64_error": errs,
-        "roofline": {"bound": "hbm", "kernel": "hydro_stage_kernel<rotated,float,corrector>",
+        "roofline": {"bound": "hbm", "kernel": "hydro_stage_mixed_kernel<rotated,corrector>",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_cell": BYTES_STAGE2, "launch_ms": round(mx["stage2_ms"], 4)},

@@ -245,9 +245,11 @@ int torchgpu_mlp_train_step(torchgpu_mlp* m, const float* feat, const float* tar
 
 /* Surrogate configuration of a cooling operator: network, feature
  * normalisation f = (u - mu) * isd of the raw features u = (log10 rho,
- * log10 T, x, log10 F_FUV, Av, log10 dt) in code units, the training
- * envelope [lo, hi] of u, the error gate on the estimated relative T error
- * and the subcycle count up to which cells are integrated exactly. */
+ * log10 T, x, log10 F_FUV, Av, log10 dt, log10 subcycle count, sign of the
+ * pre-step rate) in code units (the last two from the FP64 prepass), the
+ * training envelope [lo, hi] of the first six, the error gate on the
+ * estimated relative T error and the subcycle count up to which cells are
+ * integrated exactly. */
 typedef struct torchgpu_surrogate_cfg {
     float mu[8], isd[8];
     double lo[6], hi[6];
@@ -264,8 +266,9 @@ int torchgpu_thermo_cells_surrogate(torchgpu_cooling* c, int64_t n, const double
                                     int64_t* counts);
 /* Training data: for the cells with more than nexact subcycles, writes their
  * indices (list, device int64 [n]), features (feat [n][8], normalised with
- * the configured mu/isd) and, from the FP64 integrator's output `out`, the
- * targets (target [n][3]: d log10 T_sub, 0, 0).  Returns the count. */
+ * the configured mu/isd), raw features (raw [n][8], may be NULL) and, from
+ * the FP64 integrator's output `out`, the targets (target [n][3]:
+ * d log10 T_sub, 0, 0).  Returns the count. */
 int torchgpu_surrogate_training_data(torchgpu_cooling* c, int64_t n, const double* in, const double* out,
                                      int64_t* list, float* feat, float* target, float* raw, int64_t* count);
 

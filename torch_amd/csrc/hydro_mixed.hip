@@ -6,30 +6,40 @@
 // floors and the CFL reduction in FP64.
 //
 // Layout for throughput: a workgroup (576 threads, 9 waves) owns an 8x8x8
-// tile.  One LDS array q[12][12][12][8] (float, x fastest, 7 variables + a
-// pad word so that a cell is two 16-byte LDS reads) holds the tile and its
-// 2-deep halo slabs of all three directions, loaded once at the start by all
-// threads together, so a face reads its four stencil cells with plain
-// address arithmetic.  Per direction every thread solves one face into an
-// LDS flux array and the 512 cell threads accumulate their two faces into
-// FP64 registers in the reference's order.  For the y and z sweeps
-// consecutive lanes take consecutive x (16-byte stride in LDS); for the x
-// sweep consecutive lanes walk along x.  LDS 71.5 KB: two workgroups per CU.
+// tile.  One LDS array q[7][1280] (float, one plane per variable) holds the
+// tile's 512 cells and the 2-deep halo slabs of all three directions
+// (3 x 4 x 64 cells), loaded once at the start by all threads together; a
+// face finds each stencil cell with a select between its tile and halo
+// index (no branches) and reads the 7 variables with one address and
+// immediate plane offsets.  Per direction every thread solves one face into
+// an LDS flux array and the 512 cell threads accumulate their two faces
+// into FP64 registers in the reference's order.  For the y and z sweeps
+// consecutive lanes take consecutive x; for the x sweep consecutive lanes
+// walk along x.  LDS 52 KB: three workgroups per CU.
 #include "hydro_device.hpp"
 
 namespace tg {
 
-constexpr int XW = TILE + 4;                     // stencil extent per axis (-2..9)
 constexpr int NLOAD = TCELLS + 3 * THALO;        // tile cells + halo slabs: 1280
 
 struct MixedLDS {
-    float q[XW][XW][XW][8];
+    float q[7][NLOAD];   // [0, 512): tile cells x + 8 (y + 8 z); [512, 1280): halo slabs
     float fl[7][TFACES];
     double red[NTHREADS / 64];
 };
 
-__device__ __forceinline__ const float* cell_q(const MixedLDS& s, int x, int y, int z) {
-    return &s.q[z + 2][y + 2][x + 2][0];
+// LDS slot of along-position pos (-2..9) of pencil (ta, tb) in sweep DIM.
+// Tile cells: x + 8 (y + 8 z); halo cell of direction DIM at layer h
+// (pos -2, -1, 8, 9 -> h 0..3): 512 + 256 DIM + 64 h + 8 tb + ta.
+template <int DIM>
+__device__ __forceinline__ int slot(int pos, int ta, int tb) {
+    int c;
+    if (DIM == 0) c = pos + TILE * (ta + TILE * tb);
+    else if (DIM == 1) c = ta + TILE * (pos + TILE * tb);
+    else c = ta + TILE * (tb + TILE * pos);
+    const int h = pos < 0 ? pos + 2 : pos - (TILE - 2);
+    const int hs = TCELLS + DIM * THALO + h * (TILE * TILE) + tb * TILE + ta;
+    return (unsigned)pos < (unsigned)TILE ? c : hs;
 }
 
 // Face index of along-position f of pencil (ta, tb) in sweep DIM.
@@ -41,14 +51,9 @@ __device__ __forceinline__ int face_id(int f, int ta, int tb) {
 
 template <int DIM>
 __device__ __forceinline__ void read_q(const MixedLDS& s, int pos, int ta, int tb, float q[7]) {
-    int c[3];
-    c[DIM] = pos;
-    c[DIM == 0 ? 1 : 0] = ta;
-    c[DIM == 2 ? 1 : 2] = tb;
-    const float4* p = reinterpret_cast<const float4*>(cell_q(s, c[0], c[1], c[2]));
-    const float4 a = p[0], b = p[1];
-    q[0] = a.x; q[1] = a.y; q[2] = a.z; q[3] = a.w;
-    q[4] = b.x; q[5] = b.y; q[6] = b.z;
+    const int i = slot<DIM>(pos, ta, tb);
+#pragma unroll
+    for (int v = 0; v < 7; ++v) q[v] = s.q[v][i];
 }
 
 template <int SOLVER, int DIM>
@@ -164,9 +169,9 @@ __global__ __launch_bounds__(NTHREADS) void hydro_stage_mixed_kernel(HydroGeom g
         }
         double q[7];
         load_q<STAGE, true>(g, Usrc, o[0] + c[0], o[1] + c[1], o[2] + c[2], q, st, check);
-        float4* p = reinterpret_cast<float4*>(&s.q[c[2] + 2][c[1] + 2][c[0] + 2][0]);
-        p[0] = make_float4((float)q[0], (float)q[1], (float)q[2], (float)q[3]);
-        p[1] = make_float4((float)q[4], (float)q[5], (float)q[6], 0.f);
+        // slot idx: the tile cell x + 8 (y + 8 z), or halo slot 512 + 256 dim + 64 h + 8 tb + ta
+#pragma unroll
+        for (int v = 0; v < 7; ++v) s.q[v][idx] = (float)q[v];
     }
     __syncthreads();
 

@@ -15,14 +15,19 @@
 // Surrogate path (torchgpu_thermo_cells_surrogate), all on the device with
 // no host round trip:
 //   prepass   FP64 pre-step rates and subcycle count of every cell; cells
-//             with at most `nexact` subcycles are integrated exactly here,
-//             the rest are queued for the MLP with their normalised
-//             features (cells outside the training envelope go to the
-//             fallback queue)
-//   MLP       fused BF16-MFMA inference over the queue (mlp.hip)
-//   finish    T_sub = T0 10^y0 -> subcycled pressure -> RATE exactly as
-//             :334; cells whose estimated relative T error exceeds the gate
-//             are queued for the fallback
+//             with at most `nexact` subcycles are integrated exactly here;
+//             for the rest a coarse FP64 estimate Tc of the end temperature
+//             (exponential-Euler steps of <= 2 % change) and the normalised
+//             features are queued for the MLP (cells outside the training
+//             envelope go to the fallback queue)
+//   MLP       fused BF16-MFMA inference over the queue (mlp.hip); output 0
+//             is log10(T_sub / Tc) / sigma(n_sub), a residual in units of
+//             the coarse estimate's expected error, so the BF16 rounding of
+//             the hidden activations scales with the residual; output 2 is
+//             log10 of the estimated |error| of output 0 in the same units
+//   finish    T_sub = Tc 10^(sigma y0) -> subcycled pressure -> RATE exactly
+//             as :334; cells whose estimated relative T error sigma 10^y2
+//             exceeds the gate are queued for the fallback
 //   fallback  the persistent FP64 kernel over the fallback queue
 #include <hip/hip_runtime.h>
 
@@ -235,21 +240,73 @@ struct SurrogateConsts {
 // Per-queued-cell data the finish step needs (FP64).
 struct SurrCell {
     double T0, pre, temp2pre, pf, dpre2rate, T_min, heat, rate_pre;
+    double Tc;     // coarse estimate of the subcycled end temperature
+    double sigma;  // residual scale: the network predicts log10(T_sub / Tc) / sigma
 };
 
-// Raw features u (code units): log10 rho, log10 T, x, log10 F_FUV, Av, log10 dt.
+// Scale (log10 units) of the coarse estimate's error for a subcycle of n
+// steps: the explicit steps of short subcycles are coarse (large residuals),
+// long ones resolve the evolution the coarse integration follows.  Dividing
+// the residual by it keeps the network's output O(1), so the BF16 rounding
+// of its hidden activations shrinks with the residual.
+__device__ __forceinline__ double residual_scale(long long nsub) { return 0.01 / sqrt((double)nsub) + 2.0e-4; }
+
+// Rate of one subcycle step at temperature T (Thermodynamics.cpp:314-320),
+// as dp per subcycle.
+__device__ __forceinline__ double substep_dp(const ThermoConsts& t, const double* sx, const double* sy,
+                                             const double* sy2, const CellState& s, double T) {
+    const double r = subtract_cooling(t, s.cc, sx, sy, sy2, s.heat, T);
+    return t.heating_amplification * soft_landing(r, T, s.T_min) * s.rate2dpre;
+}
+
+// Coarse estimate of the temperature at the end of the subcycle: the
+// remaining s.remaining explicit steps, each dp = rate(T) * rate2dpre, seen
+// as the ODE dp/dk = rate(T(p)) * rate2dpre over k in [0, remaining] and
+// integrated with exponential-Euler steps (linearised rate, Jacobian by a
+// forward difference) that change p by at most ~2 % or span the rest, with
+// the subcycle's floor after each step; at most COARSE_STEPS steps.  For
+// long subcycles (the explicit steps resolve the evolution) this is close to
+// the subcycle's own result; what remains is learned by the network.
+constexpr int COARSE_STEPS = 512;
+__device__ __forceinline__ double coarse_end_temperature(const ThermoConsts& t, const double* sx, const double* sy,
+                                                         const double* sy2, const CellState& s) {
+    double p = s.pressure;
+    double kleft = (double)s.remaining;
+    for (int it = 0; it < COARSE_STEPS && kleft > 0.0; ++it) {
+        const double r0 = substep_dp(t, sx, sy, sy2, s, p * s.pre2temp);
+        if (r0 == 0.0) break;
+        const double eps = 1.0e-4 * p;
+        const double r1 = substep_dp(t, sx, sy, sy2, s, (p + eps) * s.pre2temp);
+        const double J = (r1 - r0) / eps;
+        double dk = fmin(kleft, fmax(1.0, 0.02 * p / fabs(r0)));
+        if (J > 0.0) dk = fmin(dk, fmax(1.0, 1.0 / J));
+        const double x = J * dk;
+        const double phi = fabs(x) < 1.0e-8 ? dk : expm1(x) / J;
+        p += r0 * phi;
+        if (p < t.pfloor || p * s.pre2temp < s.T_min) p = s.pf;
+        kleft -= dk;
+    }
+    return p * s.pre2temp;
+}
+
+// Raw features u (code units): log10 rho, log10 T, x, log10 F_FUV, Av,
+// log10 dt, and from the FP64 prepass log10 of the subcycle count and
+// log10(Tc / T0) of the coarse end-temperature estimate.  The training
+// envelope applies to the first six.
 __device__ __forceinline__ void raw_features(const ThermoConsts& t, const double* in, long long stride, long long cell,
-                                             double T0, double u[6]) {
+                                             const CellState& s, double Tc, double u[8]) {
     u[0] = log10(in[0 * stride + cell]);
-    u[1] = log10(T0);
+    u[1] = log10(s.T0);
     u[2] = in[2 * stride + cell];
     u[3] = log10(in[7 * stride + cell]);
     u[4] = 1.086 * t.dust_xsec * in[6 * stride + cell];
     u[5] = log10(in[8 * stride + cell]);
+    u[6] = log10((double)(s.remaining + 1));
+    u[7] = log10(Tc / s.T0);
 }
 
 // counters: [2] surrogate queue length, [3] fallback queue length
-__global__ __launch_bounds__(256) void surrogate_prepass_kernel(ThermoConsts t, const double* __restrict__ tables,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void surrogate_prepass_kernel(ThermoConsts t, const double* __restrict__ tables,
                                                                SurrogateConsts sc, long long n,
                                                                const double* __restrict__ in, double* __restrict__ out,
                                                                unsigned char* path, long long* __restrict__ mlp_list,
@@ -266,7 +323,8 @@ __global__ __launch_bounds__(256) void surrogate_prepass_kernel(ThermoConsts t, 
         const bool valid = cell < n;
         bool to_mlp = false, to_fb = false;
         CellState s;
-        double u[6];
+        double u[8];
+        double Tc = 0.0;
         if (valid) {
             bool active = thermo_init(t, sx, sy, sy2, in, n, cell, s, err);
             if (active && s.remaining < sc.nexact) {
@@ -277,7 +335,8 @@ __global__ __launch_bounds__(256) void surrogate_prepass_kernel(ThermoConsts t, 
                 thermo_finish(s, out, n, cell);
                 if (path) path[cell] = 0;
             } else {
-                raw_features(t, in, n, cell, s.T0, u);
+                Tc = coarse_end_temperature(t, sx, sy, sy2, s);
+                raw_features(t, in, n, cell, s, Tc, u);
                 bool inside = true;
 #pragma unroll
                 for (int i = 0; i < 6; ++i) inside &= (u[i] >= sc.lo[i]) && (u[i] <= sc.hi[i]);
@@ -290,13 +349,10 @@ __global__ __launch_bounds__(256) void surrogate_prepass_kernel(ThermoConsts t, 
         if (to_mlp) {
             mlp_list[slot] = cell;
 #pragma unroll
-            for (int i = 0; i < 8; ++i)
-                feat[slot * 8 + i] = i < 6 ? (float)(u[i] - (double)sc.mu[i]) * sc.isd[i] : 0.f;
+            for (int i = 0; i < 8; ++i) feat[slot * 8 + i] = (float)(u[i] - (double)sc.mu[i]) * sc.isd[i];
             if (raw_out) {
 #pragma unroll
-                for (int i = 0; i < 6; ++i) raw_out[slot * 8 + i] = (float)u[i];
-                raw_out[slot * 8 + 6] = (float)s.nsub;
-                raw_out[slot * 8 + 7] = (float)s.remaining;
+                for (int i = 0; i < 8; ++i) raw_out[slot * 8 + i] = (float)u[i];
             }
             SurrCell d;
             d.T0 = s.T0;
@@ -307,6 +363,8 @@ __global__ __launch_bounds__(256) void surrogate_prepass_kernel(ThermoConsts t, 
             d.T_min = s.T_min;
             d.heat = s.heat;
             d.rate_pre = s.rate_pre;
+            d.Tc = Tc;
+            d.sigma = residual_scale(s.remaining + 1);
             sdata[slot] = d;
             if (path) path[cell] = 1;
         }
@@ -333,11 +391,12 @@ __global__ void surrogate_finish_kernel(ThermoConsts t, SurrogateConsts sc, long
             cell = mlp_list[i];
             const SurrCell d = sdata[i];
             const double y0 = (double)y[i * 3 + 0];
-            const double est = exp10((double)y[i * 3 + 2]);
+            // output 2 estimates log10 |error of output 0| (scaled units, ln 10 included)
+            const double est = d.sigma * exp10((double)y[i * 3 + 2]);
             if (!(est <= sc.gate) || !isfinite(y0)) {
                 to_fb = true;
             } else {
-                const double Tsub = d.T0 * exp10(y0);
+                const double Tsub = d.Tc * exp10(d.sigma * y0);
                 double p = Tsub * d.temp2pre;
                 if (p < t.pfloor || Tsub < d.T_min) p = d.pf;
                 out[0 * stride + cell] = d.heat;
@@ -356,7 +415,8 @@ __global__ void surrogate_finish_kernel(ThermoConsts t, SurrogateConsts sc, long
 }
 
 // Training targets of the queued cells from the FP64 integrator's output:
-// t0 = log10(p_sub * pre2temp) - log10(T0) with pre2temp = 1/temp2pre.
+// t0 = (log10(p_sub * pre2temp) - log10(Tc)) / sigma with pre2temp =
+// 1/temp2pre.
 __global__ void surrogate_targets_kernel(const unsigned long long* __restrict__ counters, long long stride,
                                          const long long* __restrict__ mlp_list, const SurrCell* __restrict__ sdata,
                                          const double* __restrict__ out, float* __restrict__ target) {
@@ -365,7 +425,7 @@ __global__ void surrogate_targets_kernel(const unsigned long long* __restrict__ 
         const SurrCell d = sdata[i];
         const double p = out[4 * stride + mlp_list[i]];
         const double Tsub = p * (1.0 / d.temp2pre);
-        target[i * 3 + 0] = (float)(log10(Tsub) - log10(d.T0));
+        target[i * 3 + 0] = (float)((log10(Tsub) - log10(d.Tc)) / d.sigma);
         target[i * 3 + 1] = 0.f;
         target[i * 3 + 2] = 0.f;
     }

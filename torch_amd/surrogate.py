@@ -56,15 +56,13 @@ def _check(rc, what):
 
 
 def init_params(seed: int = 0):
-    """He-normal weights (W1 columns 6, 7 zero), zero biases."""
+    """He-normal weights, zero biases."""
     rng = np.random.Generator(np.random.PCG64(seed))
     dims = [NIN, NH, NH, NH, NH, NOUT]
     Ws, bs = [], []
     for L in range(1, 6):
-        fan_in = 6 if L == 1 else dims[L - 1]
+        fan_in = dims[L - 1]
         W = rng.normal(0.0, math.sqrt(2.0 / fan_in), (dims[L], dims[L - 1]))
-        if L == 1:
-            W[:, 6:] = 0.0
         Ws.append(W)
         bs.append(np.zeros(dims[L]))
     return pack(Ws, bs)
@@ -200,22 +198,58 @@ def training_data(cool: Cooling, inp, out_fp64):
     return lst[:k], feat[:k], tgt[:k], raw[:k]
 
 
+def residual_scale(log10_nsub):
+    """Scale of the network's output 0 (log10 units) for subcycle counts
+    10^log10_nsub: 0.01 / sqrt(n) + 2e-4, as the cooling kernels use it."""
+    import torch
+
+    return 0.01 * torch.pow(10.0, -0.5 * log10_nsub.double()) + 2.0e-4
+
+
+def validate(mlp: MLP, feat, target, sigma, gate: float = 0.01) -> dict:
+    """Relative T error |10^(sigma (y0 - t0)) - 1| of the network on held-out
+    cells (sigma: their residual scales), and the fraction its error head
+    accepts at `gate` (estimated error sigma 10^y2)."""
+    import torch
+
+    y = mlp.forward(feat)
+    torch.cuda.synchronize()
+    err = (torch.pow(10.0, sigma * (y[:, 0] - target[:, 0]).double()) - 1.0).abs()
+    acc = sigma * torch.pow(10.0, y[:, 2].double()) <= gate
+    q = torch.quantile(err, torch.tensor([0.5, 0.9, 0.99], dtype=torch.float64, device=err.device))
+    out = {"median": float(q[0]), "p90": float(q[1]), "p99": float(q[2]), "accepted": float(acc.double().mean())}
+    if int(acc.sum()):
+        out["accepted_p99"] = float(torch.quantile(err[acc], 0.99))
+    return out
+
+
 def train(mlp: MLP, feat, target, epochs: int = 10, batch: int = 65536, lr: float = 2e-3, seed: int = 0,
-          err_head_from: int = 1, weights=(1.0, 1.0, 0.05), log=None):
-    """Adam over shuffled mini-batches with a cosine learning-rate decay.  The
-    error head is trained from epoch `err_head_from` on."""
+          err_head_from: int = 1, weights=(1.0, 1.0, 0.05), log=None, sample_weight=None, val=None,
+          val_every: int = 10):
+    """Adam over mini-batches with a cosine learning-rate decay.  Each epoch
+    draws n samples: a permutation, or, with `sample_weight`, samples drawn
+    with replacement in proportion to it (inverse-CDF).  The error head is
+    trained from epoch `err_head_from` on."""
     import torch
 
     n = feat.shape[0]
     g = torch.Generator(device=feat.device)
     g.manual_seed(seed)
+    cdf = None
+    if sample_weight is not None:
+        cdf = torch.cumsum(sample_weight.double(), 0)
+        cdf = cdf / cdf[-1]
     steps_per_epoch = max(1, n // batch)
     total = epochs * steps_per_epoch
     step = 0
     t0 = time.perf_counter()
     hist = []
     for ep in range(epochs):
-        perm = torch.randperm(n, generator=g, device=feat.device)
+        if cdf is None:
+            perm = torch.randperm(n, generator=g, device=feat.device)
+        else:
+            u = torch.rand(n, generator=g, device=feat.device, dtype=torch.float64)
+            perm = torch.searchsorted(cdf, u).clamp_max(n - 1)
         losses = []
         for s in range(steps_per_epoch):
             idx = perm[s * batch:(s + 1) * batch]
@@ -230,7 +264,10 @@ def train(mlp: MLP, feat, target, epochs: int = 10, batch: int = 65536, lr: floa
             step += 1
         hist.append(losses[-1] if losses else float("nan"))
         if log:
-            log(f"epoch {ep}: loss {hist[-1]:.3e} ({time.perf_counter() - t0:.1f}s)")
+            msg = f"epoch {ep}: loss {hist[-1]:.3e} ({time.perf_counter() - t0:.1f}s)"
+            if val is not None and (ep % val_every == val_every - 1 or ep == epochs - 1):
+                msg += " val " + " ".join(f"{k} {v:.4g}" for k, v in validate(mlp, *val).items())
+            log(msg)
     return hist
 
 
@@ -248,16 +285,17 @@ def attach(cool: Cooling, path: str, gate: float = 0.01, mlp: MLP | None = None,
     if mlp is None:
         mlp = MLP(max_batch=max_batch)
     mlp.set_params(d["W"], d["b"])
-    mu = list(d["mu"]) + [0.0, 0.0]
-    isd = list(d["isd"]) + [0.0, 0.0]
+    mu = list(d["mu"]) + [0.0] * (8 - len(d["mu"]))
+    isd = list(d["isd"]) + [0.0] * (8 - len(d["isd"]))
     set_surrogate(cool, mlp, make_cfg(mu, isd, d["lo"], d["hi"], gate=gate, nexact=int(d["nexact"])))
     return mlp
 
 
-def compare(cool: Cooling, inp):
+def compare(cool: Cooling, inp, return_fp64: bool = False):
     """Runs the FP64 subcycled step and the surrogate step on the same cells
     ([9][n] device inputs) and returns their event-timed durations and the
-    surrogate's relative temperature error against FP64.  The temperature
+    surrogate's relative temperature error against FP64 (and the FP64 output
+    when `return_fp64`).  The temperature
     ratio equals the ratio of the subcycled pressures (same density and
     ionisation), out[4]."""
     import torch
@@ -286,7 +324,7 @@ def compare(cool: Cooling, inp):
 
     all_q, acc_q = quant(rel), quant(rel[acc])
     n = inp.shape[1]
-    return {
+    res = {
         "cells": int(n),
         "fp64_ms": ev[0].elapsed_time(ev[1]),
         "surrogate_ms": ev[2].elapsed_time(ev[3]),
@@ -297,3 +335,4 @@ def compare(cool: Cooling, inp):
         "rel_T_err_mlp_cells": {"median": acc_q[0], "p99": acc_q[1],
                                 "max": float(rel[acc].max()) if int(acc.sum()) else 0.0},
     }
+    return (res, ref) if return_fp64 else res

@@ -10,13 +10,20 @@
 2. Targets: the FP64 subcycled integrator (torchgpu_thermo_cells) on the
    same GPU; the cells with more than `nexact` subcycles (the ones the
    network replaces) are extracted on the device with their raw features by
-   torchgpu_surrogate_training_data.
-3. Normalisation: per-feature mean and standard deviation of the training
-   set; the training envelope is the per-feature [min, max].
+   torchgpu_surrogate_training_data.  Output 0's target is the residual
+   log10(T_sub / Tc) of the prepass's coarse end-temperature estimate Tc,
+   divided by the residual scale sigma = 0.01 / sqrt(subcycles) + 2e-4.
+3. Features: log10 rho, log10 T, x, log10 F_FUV, A_V, log10 dt and, from
+   the FP64 prepass, log10 of the subcycle count and log10(Tc / T0);
+   normalisation: per-feature mean and standard deviation of
+   the training set; the training envelope is the [min, max] of the first
+   six.  Batches oversample cells in proportion to 1 + w sqrt(subcycles)
+   (an FP64 fallback of a long cell costs its whole subcycle); the last
+   2^18 queued cells are held out for validation.
 4. Training: Adam with a cosine learning-rate decay over shuffled batches,
    every step a torchgpu_mlp_train_step (the library's BF16-MFMA forward and
    backward kernels with FP32 accumulation).  Output 2 learns the log10 of
-   the current |relative T error| of output 0 from epoch 1 on; the cooling
+   the current |relative T error| of output 0 from epoch --err-head-from on; the cooling
    step gates on it.
 
 The output file holds W, b (fp32), mu, isd, lo, hi, nexact and the training
@@ -53,7 +60,7 @@ def workload_inputs(n: int, seed: int, device="cuda"):
 
 
 def generate(cool, mlp, samples: int, chunk: int, nexact: int, seed0: int = 1000):
-    """Raw features [k][6] and targets [k][3] (device fp32) of the cells past
+    """Raw features [k][8] and targets [k][3] (device fp32) of the cells past
     nexact subcycles in `samples` workload cells."""
     import torch
 
@@ -71,7 +78,7 @@ def generate(cool, mlp, samples: int, chunk: int, nexact: int, seed0: int = 1000
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         _, _, tgt, raw = sg.training_data(cool, inp, out)
-        raws.append(raw[:, :6].clone())
+        raws.append(raw.clone())
         tgts.append(tgt.clone())
         log(f"chunk {i}: {n} cells, {tgt.shape[0]} past {nexact} subcycles, {nsub:.3e} subcycles, "
             f"integrator {t1 - t0:.1f}s")
@@ -90,11 +97,14 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--samples", type=int, default=1 << 24)
     ap.add_argument("--chunk", type=int, default=1 << 24)
-    ap.add_argument("--epochs", type=int, default=12)
+    ap.add_argument("--epochs", type=int, default=100)
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--nexact", type=int, default=4)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--err-head-from", type=int, default=5)
+    ap.add_argument("--long-weight", type=float, default=0.05,
+                    help="sampling weight 1 + w sqrt(subcycles): oversamples long subcycles")
     ap.add_argument("--out", default=DEFAULT_OUT)
     a = ap.parse_args()
 
@@ -104,23 +114,39 @@ def main():
     raw, tgt = generate(cool, mlp, a.samples, a.chunk, a.nexact)
     k = raw.shape[0]
     mu = raw.mean(0).double()
-    sd = raw.double().std(0).clamp_min(1e-6)
-    lo = raw.min(0).values.double()
-    hi = raw.max(0).values.double()
-    feat = torch.zeros((k, 8), dtype=torch.float32, device=raw.device)
-    feat[:, :6] = ((raw.double() - mu) / sd).float()
+    sd = raw.double().std(0).clamp_min(1e-3)
+    lo = raw[:, :6].min(0).values.double()
+    hi = raw[:, :6].max(0).values.double()
+    feat = ((raw.double() - mu) / sd).float().contiguous()
+    # long subcycles weigh more: they are the cells the surrogate saves time on
+    # (an FP64 fallback of one of them costs its whole subcycle)
+    sample_weight = 1.0 + a.long_weight * torch.pow(10.0, 0.5 * raw[:, 6].double())
+    # held-out cells: the last chunk's first 2^18 queued cells, long ones
+    # (>= 1e6 subcycles) reported separately
+    nv = min(1 << 18, k // 8)
+    vidx = torch.arange(k - nv, k, device=feat.device)
+    long_v = vidx[raw[k - nv:, 6] >= 6.0]
+    sample_weight[k - nv:] = 0.0
+    sigma = sg.residual_scale(raw[:, 6])
+    val_sets = (feat[vidx].contiguous(), tgt[vidx].contiguous(), sigma[vidx])
+    val_long = (feat[long_v].contiguous(), tgt[long_v].contiguous(), sigma[long_v]) if long_v.numel() else None
     del raw
     log(f"training set: {k} cells; mu {mu.tolist()} sd {sd.tolist()}")
     W, b = sg.init_params(a.seed)
     mlp.set_params(W, b)
     t0 = time.perf_counter()
-    hist = sg.train(mlp, feat, tgt.contiguous(), epochs=a.epochs, batch=a.batch, lr=a.lr, seed=a.seed, log=log)
+    hist = sg.train(mlp, feat, tgt.contiguous(), epochs=a.epochs, batch=a.batch, lr=a.lr, seed=a.seed, log=log,
+                    sample_weight=sample_weight, err_head_from=a.err_head_from, val=val_sets)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
+    val_all = sg.validate(mlp, *val_sets)
+    val_l = sg.validate(mlp, *val_long) if val_long is not None else None
+    log(f"held-out: all {val_all}; >=1e6 subcycles ({0 if val_long is None else val_long[0].shape[0]}) {val_l}")
     W, b = mlp.get_params()
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
     record = {"samples": a.samples, "train_cells": int(k), "epochs": a.epochs, "batch": a.batch, "lr": a.lr,
-              "seed": a.seed, "loss_history": hist, "train_seconds": t_train,
+              "seed": a.seed, "long_weight": a.long_weight, "err_head_from": a.err_head_from,
+              "held_out": val_all, "held_out_long": val_l, "loss_history": hist, "train_seconds": t_train,
               "total_seconds": time.perf_counter() - t_start}
     np.savez(a.out, W=W, b=b, mu=mu.cpu().numpy(), isd=(1.0 / sd).cpu().numpy(), lo=lo.cpu().numpy(),
              hi=hi.cpu().numpy(), nexact=np.int32(a.nexact), record=np.array(json.dumps(record)))
