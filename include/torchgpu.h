@@ -107,7 +107,12 @@ typedef struct torchgpu_ctx torchgpu_ctx;
 int torchgpu_create(const torchgpu_params* params, void* stream, torchgpu_ctx** out);
 void torchgpu_destroy(torchgpu_ctx* ctx);
 int torchgpu_last_error(const torchgpu_ctx* ctx, char* buf, size_t len);
-/* Device address of the conserved state U ([7][nz][ny][nx] float64). */
+/* Device address of the conserved state U ([7][nz][ny][nx] float64).
+ * Mixed precision: the float state is scaled by powers of two taken from
+ * the largest density and pressure, which set_primitive, set_conserved and
+ * reset_clock measure and every predictor launch records for the next one;
+ * after writing U through this pointer by more than ~1e14x, call
+ * torchgpu_reset_clock (or set_conserved) to re-measure. */
 int torchgpu_state_ptr(torchgpu_ctx* ctx, double** U);
 
 /* Torch::initialise tail (Torch.cpp:117-119): Q is fixed by fixPrimitives
@@ -145,7 +150,8 @@ int torchgpu_hydro_stage_dt(torchgpu_ctx* ctx, int stage, double dtf);
  * min-reduction fused into the corrector stage, clipped to the next
  * checkpoint).  Asynchronous. */
 int torchgpu_run(torchgpu_ctx* ctx, int nsteps);
-/* Restart the Torch::run bookkeeping (time 0, first-step rule, checkpoints). */
+/* Restart the Torch::run bookkeeping (time 0, first-step rule, checkpoints);
+ * mixed precision: also re-measures the state scale.  Synchronises. */
 int torchgpu_reset_clock(torchgpu_ctx* ctx);
 /* Waits for the stream and reports the first device-side error. */
 int torchgpu_synchronize(torchgpu_ctx* ctx);

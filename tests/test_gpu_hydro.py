@@ -171,3 +171,53 @@ def test_mixed_kernel_variants_ragged_grid(gpu, orc, solver, limiter, order):
         assert rel < 1e-5, (v, rel)
     tg, dtg, n = g.clock()
     assert n == 6 and abs(tg - t) / t < 1e-5
+
+
+@pytest.mark.parametrize("solver,limiter", [("RotatedHLLC", "albada"), ("HLL", "ospre"), ("HLLC", "leer")])
+def test_mixed_kernel_large_code_values(gpu, orc, solver, limiter):
+    """Code units with pressures of 1e24 and velocities of 1e12 (the
+    turbulent field with p x 1e24, v x 1e12, the gas constant x 1e24 so that
+    temperatures are unchanged): the squares of the FP32 limiter and the HLL
+    S_l S_r dU term would overflow FP32 there.  The mixed kernel must match
+    the FP64 oracle on the scaled problem to the same 1e-5 relative L1 as on
+    the unscaled one."""
+    Q = setups.turbulent(16, 1.0, 1.5)[:, :9, :10, :12].copy()
+    Q[1] *= 1e24
+    Q[4:7] *= 1e12
+    R = 8.314462e7 * 1e24
+    kw = dict(tfloor=1e-12, pfloor=0.1 * R * 1e-30 * 1e-12, spec_gas_const=R, solver=solver, limiter=limiter,
+              bc=["outflow", "inflow", "periodic", "periodic", "reflecting", "free"])
+    g = _grid(gpu, Q.shape[1:], precision=gpu.MIXED, **kw)
+    g.set_primitive(np.ascontiguousarray(Q))
+    p = _oparams(orc, Q.shape[1:], dx=1.0 / Q.shape[3], **kw)
+    Uo, dts, t = orc.run_hydro(p, _u0(orc, p, Q), 6)
+    assert np.isfinite(Uo).all()
+    g.run(6)
+    g.synchronize()
+    Ug = g.get_conserved()
+    for v in (0, 1):
+        rel = np.abs(Ug[v] - Uo[v]).sum() / np.abs(Uo[v]).sum()
+        assert rel < 1e-5, (v, rel)
+
+
+def test_sod64_200_steps_bitwise_vs_reference_fixture(gpu):
+    """The 64^3 Sod configuration through 200 steps on the GPU FP64 path
+    against tests/golden/sod64.npz, which the reference's own translation
+    units wrote: one x-line at steps 1, 2, 10 and 200 (all lines of the
+    planar problem must be identical) and every dt."""
+    import os
+
+    f = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sod64.npz"))
+    g = _grid(gpu, (64, 64, 64))
+    g.set_primitive(np.ascontiguousarray(setups.sod(64)))
+    dts = []
+    for s in range(1, 201):
+        g.run(1)
+        _, dt, n = g.clock()
+        assert n == s
+        dts.append(dt)
+        if s in (1, 2, 10, 200):
+            U = g.get_conserved()
+            np.testing.assert_array_equal(U[:, 0, 0, :], f[f"U_{s}"], err_msg=f"step {s}")
+            assert (U == U[:, :1, :1, :]).all(), s
+    np.testing.assert_array_equal(np.array(dts), f["dts"])

@@ -72,6 +72,30 @@ def setup(n: int, to_code):
     return Q, col, ffuv, excluded, side * to_code(1.0, 0, 1, 0)
 
 
+def cooling_inputs(g, p, col_den, F_FUV, excluded, dt):
+    """[9][n] device inputs of the cooling step for the grid's current state,
+    formed as the library's grid gather forms them (Q = fixPrimitives(QfromU(U))
+    with the HII/ADV clamps and floors, E = U[PRE], T_min 100 K default, the
+    excluded cell's ADV = -1)."""
+    import torch
+
+    n = p.nx * p.ny * p.nz
+    U = torch.empty((7, n), dtype=torch.float64, device="cuda")
+    g.get_conserved(out=U)
+    den = U[0]
+    ke = 0.5 * (U[4:7] * U[4:7]).sum(0) / den
+    pre = ((U[1] - ke) * (p.gamma - 1.0)).clamp_min(p.pfloor)
+    hii = (U[2] / den).clamp(0.0, 1.0)
+    adv = (U[3] / den).clamp(0.0, 1.0)
+    adv[excluded] = -1.0
+    dc = den.clamp_min(p.dfloor)
+    mu_inv = p.mass_fraction_h * (hii + 1.0) + (1.0 - p.mass_fraction_h) * 0.25
+    T = pre / (mu_inv * p.spec_gas_const * dc)
+    pre = torch.where(T < p.tfloor, mu_inv * p.spec_gas_const * den * p.tfloor, pre)
+    tmin = torch.full_like(den, 100.0)
+    return torch.stack([dc, pre, hii, adv, U[1], tmin, col_den, F_FUV, torch.full_like(den, dt)]).contiguous()
+
+
 def main():
     import torch
 
@@ -87,6 +111,9 @@ def main():
                                                         "surrogate_cooling.npz"))
     ap.add_argument("--gate", type=float, default=0.01)
     ap.add_argument("--tmax", type=float, default=1.0e13, help="simulation time, s (dt_max = tmax / 10)")
+    ap.add_argument("--every", type=int, default=10, help="report (and check for step errors) every N steps")
+    ap.add_argument("--check", default="", help="comma-separated steps after which the surrogate and the FP64 "
+                    "integrator are compared on the grid's cells (untimed)")
     a = ap.parse_args()
     n = a.n
     tc = lambda v, m, l, t: ta.units(SCALES, v, m, l, t)  # noqa: E731
@@ -115,7 +142,9 @@ def main():
     first = True
     counter = 0
     sim_t = 0.0
+    checks = {int(x) for x in a.check.split(",") if x}
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ms = 0.0
     ev0.record()
     t_wall = time.perf_counter()
     for step in range(1, a.steps + 1):
@@ -136,13 +165,24 @@ def main():
             else:
                 cool.grid_substep(g, h, surrogate=use_sur, **thermo)
         sim_t += dt
-        if step % 10 == 0 or step == a.steps:
+        if step % a.every == 0 or step == a.steps:
             g.synchronize()
             print(json.dumps({"step": step, "dt": dt, "time": sim_t,
                               "elapsed_s": round(time.perf_counter() - t_wall, 2)}), flush=True)
+        if step in checks and use_sur:
+            ev1.record()
+            torch.cuda.synchronize()
+            ms += ev0.elapsed_time(ev1)
+            inp = cooling_inputs(g, p, col_d, f_d, excluded, dt)
+            r = sg.compare(cool, inp)
+            r["out_of_envelope"] = r["cells"] - r["exact"] - r["mlp_queued"]
+            print(json.dumps({"step": step, "cooling_check": r}), flush=True)
+            del inp
+            torch.cuda.empty_cache()
+            ev0.record()
     ev1.record()
     torch.cuda.synchronize()
-    ms = ev0.elapsed_time(ev1)
+    ms += ev0.elapsed_time(ev1)
     print(json.dumps({"workload": f"cloud {n}^3, {a.steps} operator-split steps, {a.precision} hydro, "
                                   f"{a.cooling} cooling", "ms_per_step": round(ms / a.steps, 3),
                       "Mcell_updates_per_s": round(n ** 3 * a.steps / (ms * 1e-3) / 1e6, 2)}), flush=True)

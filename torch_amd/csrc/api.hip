@@ -60,6 +60,14 @@ int check_device_errors(torchgpu_ctx* c) {
     }
 }
 
+// Mixed path: the power-of-two state scale from the current state (the
+// predictor kernels keep it up to date from then on).  Synchronises.
+int seed_scale(torchgpu_ctx* c) {
+    if (c->p.precision == TORCHGPU_MIXED) HIP_TRY(c, tg::launch_mixed_scale_seed(c->g, c->U, c->st, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return TORCHGPU_OK;
+}
+
 int reset_clock(torchgpu_ctx* c) {
     tg::StepState st;
     std::memset(&st, 0, sizeof st);
@@ -163,16 +171,14 @@ int torchgpu_set_primitive(torchgpu_ctx* c, const double* Q, int on_device) {
     HIP_TRY(c, hipMemcpyAsync(c->Ustar, Q, state_bytes(c), on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                               c->stream));
     HIP_TRY(c, tg::launch_u_from_q(c->g, c->Ustar, c->U, c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    return TORCHGPU_OK;
+    return seed_scale(c);
 }
 
 int torchgpu_set_conserved(torchgpu_ctx* c, const double* U, int on_device) {
     if (!c || !U) return TORCHGPU_ERR_INVALID_ARG;
     HIP_TRY(c, hipMemcpyAsync(c->U, U, state_bytes(c), on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                               c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    return TORCHGPU_OK;
+    return seed_scale(c);
 }
 
 int torchgpu_get_conserved(torchgpu_ctx* c, double* U, int on_device) {
@@ -284,7 +290,9 @@ int torchgpu_run(torchgpu_ctx* c, int nsteps) {
 
 int torchgpu_reset_clock(torchgpu_ctx* c) {
     if (!c) return TORCHGPU_ERR_INVALID_ARG;
-    return reset_clock(c);
+    int rc = reset_clock(c);
+    if (rc) return rc;
+    return seed_scale(c);
 }
 
 int torchgpu_synchronize(torchgpu_ctx* c) {

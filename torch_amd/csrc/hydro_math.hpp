@@ -344,9 +344,22 @@ __device__ __forceinline__ bool riemann(R F[7], const R* Ql, const R* Qr, R a_l2
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
-__device__ __forceinline__ float albada_f(float a, float b) {
+// FP32 slope limiters of the mixed path.  The mixed kernel works on a state
+// scaled by powers of two (largest density and pressure near 2^16), so the
+// squares below stay in FP32 range; the reference's a b <= 1e-30 test is
+// applied with the threshold scaled alike (thr).
+__device__ __forceinline__ float albada_f(float a, float b, float thr) {
     const float ab = a * b;
-    return ab <= 1.0e-30f ? 0.f : ab * (a + b) * frcp(a * a + b * b);
+    return ab <= thr ? 0.f : ab * (a + b) * frcp(a * a + b * b);
+}
+
+__device__ __forceinline__ float limiter_f(int lim, float a, float b, float thr) {
+    if (lim == LIM_ALBADA) return albada_f(a, b, thr);
+    if (lim == LIM_OSPRE) {
+        if (a * b <= thr) return 0.f;
+        return 1.5f * b * (a * a + a * b) / (a * a + a * b + b * b);
+    }
+    return limiter<float>(lim, a, b);  // minmod, maxmod, MC, superbee, van Leer: no threshold
 }
 
 struct SideF {
@@ -366,8 +379,10 @@ __device__ __forceinline__ SideF side_f(const float q[7], float g, float igm1) {
     s.v[0] = q[VEL + 0];
     s.v[1] = q[VEL + 1];
     s.v[2] = q[VEL + 2];
-    s.sr = fsqrt(s.rho);
-    s.a2 = g * s.p * frcp(s.rho);
+    // one reciprocal square root gives both sqrt(rho) and 1/rho
+    const float rs = __builtin_amdgcn_rsqf(s.rho);
+    s.sr = s.rho * rs;
+    s.a2 = g * s.p * (rs * rs);
     s.ke = 0.5f * s.rho * (s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2]);
     s.E = s.p * igm1 + s.ke;
     return s;
@@ -437,9 +452,13 @@ __device__ __forceinline__ void hll_f(float F[7], const SideF& L, const SideF& R
         float Ul[7], Ur[7];
         cons_of(Ul, L);
         cons_of(Ur, R);
-        const float iw = frcp(S_r - S_l), slr = S_l * S_r;
+        // weights S_r / (S_r - S_l) in (0, 1) and S_l / (S_r - S_l) in
+        // (-1, 0) first: no product is larger than a flux or a speed times a
+        // conserved jump (S_l S_r dE alone overflows FP32 for energies of
+        // 1e22 code units)
+        const float iw = frcp(S_r - S_l), wr = S_r * iw, wl = S_l * iw, slr = S_l * wr;
 #pragma unroll
-        for (int i = 0; i < 7; ++i) F[i] = (S_r * Fl[i] - S_l * Fr[i] + slr * (Ur[i] - Ul[i])) * iw;
+        for (int i = 0; i < 7; ++i) F[i] = wr * Fl[i] - wl * Fr[i] + slr * (Ur[i] - Ul[i]);
     }
 }
 
@@ -459,13 +478,15 @@ __device__ __forceinline__ void hllc_f(float F[7], const SideF& L, const SideF& 
     const float S = left ? S_l : S_r;
     flux_n(F, K, un, n, g2);
     if (sub) {
+        // A = rho w / (S - S_c); A E / rho = w E / (S - S_c) and
+        // A p / (rho w) = p / (S - S_c): one reciprocal for the star state
         const float w = S - un;
-        const float A = K.rho * w * frcp(S - S_c);
-        const float irho = frcp(K.rho);
+        const float iss = frcp(S - S_c);
+        const float A = K.rho * w * iss;
         const float dsc = S_c - un;
         float dU[7];
         dU[DEN] = A - K.rho;
-        dU[PRE] = A * (K.E * irho + dsc * (S_c + K.p * frcp(K.rho * w))) - K.E;
+        dU[PRE] = w * iss * K.E + dsc * (A * S_c + K.p * iss) - K.E;
         dU[HII] = (A - K.rho) * K.x;
         dU[ADV] = (A - K.rho) * K.a;
 #pragma unroll
@@ -479,7 +500,7 @@ __device__ __forceinline__ void hllc_f(float F[7], const SideF& L, const SideF& 
 // if the HII flux is NaN.
 template <int SOLVER, int DIM>
 __device__ __forceinline__ bool riemann_f(float F[7], const float* Ql, const float* Qr, const GasConsts<float>& c,
-                                          float igm1) {
+                                          float igm1, float nn_thr) {
     const SideF L = side_f(Ql, c.g, igm1), R = side_f(Qr, c.g, igm1);
     const RoeF r = roe_f(L, R);
     const float d[3] = {DIM == 0 ? 1.f : 0.f, DIM == 1 ? 1.f : 0.f, DIM == 2 ? 1.f : 0.f};
@@ -495,7 +516,7 @@ __device__ __forceinline__ bool riemann_f(float F[7], const float* Ql, const flo
             for (int i = 0; i < 3; ++i) n1[i] = -n1[i];
         }
         const float nn = n1[0] * n1[0] + n1[1] * n1[1] + n1[2] * n1[2];
-        if (nn < 1.0e-12f) {  // |n1| < 1e-6: pure HLLC along the face axis
+        if (nn < nn_thr) {  // |n1| < 1e-6 (code units): pure HLLC along the face axis
             hllc_f(F, L, R, r, d, c.g2);
         } else {
             const float in = __builtin_amdgcn_rsqf(nn);
@@ -505,7 +526,7 @@ __device__ __forceinline__ bool riemann_f(float F[7], const float* Ql, const flo
             if (s2 == 0.f) {  // n1 along the face axis: pure HLL
                 hll_f(F, L, R, r, d, c.g2);
             } else {
-                const float s = fsqrt(s2), is = frcp(s);
+                const float is = __builtin_amdgcn_rsqf(s2), s = s2 * is;
                 float n2[3];
                 n2[DIM] = s;
                 n2[A] = -n1[A] * n1[DIM] * is;

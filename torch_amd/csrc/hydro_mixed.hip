@@ -15,7 +15,24 @@
 // an LDS flux array and the 512 cell threads accumulate their two faces
 // into FP64 registers in the reference's order.  For the y and z sweeps
 // consecutive lanes take consecutive x; for the x sweep consecutive lanes
-// walk along x.  LDS 52 KB: three workgroups per CU.
+// walk along x.  LDS 52 KB, 74 VGPRs (six waves per SIMD: two workgroups
+// per CU; capping the registers at the 72 that three workgroups need
+// spills and measured 6 % slower).
+//
+// FP32 range: the float state is the FP64 state scaled by powers of two,
+// density by 2^kr and pressure by 2^kp with the grid's largest density and
+// pressure brought near 2^16, velocities by 2^kv with 2 kv = kp - kr (so
+// that the Euler relations hold in the scaled units), HII and ADV fractions
+// unscaled.  Slope squares, S_l S_r dU and the energy fluxes then stay in
+// FP32 range whatever the code units: with the reference's own scalings
+// (1e-20 g/cm^3, 1e-28 dyn/cm^2, 1e11 s) pressures are ~1e22 and their
+// squares would overflow.  Scaling by powers of two is exact, the fluxes are
+// unscaled exactly in the FP64 accumulation (A/V 2^-k per variable), and
+// the reference's thresholds (limiter a b <= 1e-30, |velocity jump| < 1e-6)
+// are scaled alike.  Every predictor launch records its input's largest
+// density and pressure (float bit patterns, atomicMax) for the next launch's
+// scale; set_primitive / set_conserved / reset_clock seed it with a
+// reduction over the grid.
 #include "hydro_device.hpp"
 
 namespace tg {
@@ -26,7 +43,55 @@ struct MixedLDS {
     float q[7][NLOAD];   // [0, 512): tile cells x + 8 (y + 8 z); [512, 1280): halo slabs
     float fl[7][TFACES];
     double red[NTHREADS / 64];
+    unsigned int qmw[NTHREADS / 64][2];  // per-wave largest density / pressure (float bits)
 };
+
+// Power-of-two state scale of a launch (see the header) and the reference's
+// thresholds in the scaled units.  Everything here is uniform over the
+// launch and derived with integer arithmetic, so it stays in scalar registers.
+struct MixedScale {
+    int kr, kp, kv;
+    float thr_r, thr_p, thr_v;  // limiter threshold 1e-30 for density, pressure, velocity
+    float nn_thr;               // |velocity jump|^2 threshold 1e-12
+};
+
+// 2^k brings the largest value (float bits) into [2^15, 2^16); no value
+// recorded (0) or a non-finite one leaves the variable unscaled.
+__device__ __forceinline__ int scale_shift(unsigned int bits) {
+    const int e = (int)(bits >> 23) - 127;
+    if (bits == 0 || e > 127) return 0;
+    const int k = 15 - e;
+    return k < -100 ? -100 : (k > 100 ? 100 : k);
+}
+
+// x 2^k2 for a positive normal float x given by its bits, the exponent
+// clamped to the normal range (integer min/max: scalar instructions).
+__device__ __forceinline__ float scaled_threshold(unsigned int bits, int k2) {
+    const int e = min(max((int)(bits >> 23) + k2, 1), 254);
+    return __uint_as_float((bits & 0x007fffffu) | ((unsigned int)e << 23));
+}
+
+__device__ __forceinline__ MixedScale mixed_scale(const StepState* st) {
+    MixedScale m;
+    m.kr = scale_shift(st->qmax[0]);
+    m.kp = scale_shift(st->qmax[1]);
+    if ((m.kp - m.kr) & 1) m.kp -= 1;
+    m.kv = (m.kp - m.kr) / 2;
+    const unsigned int t30 = __float_as_uint(1.0e-30f), t12 = __float_as_uint(1.0e-12f);
+    m.thr_r = scaled_threshold(t30, 2 * m.kr);
+    m.thr_p = scaled_threshold(t30, 2 * m.kp);
+    m.thr_v = scaled_threshold(t30, 2 * m.kv);
+    m.nn_thr = scaled_threshold(t12, 2 * m.kv);
+    return m;
+}
+
+__device__ __forceinline__ float thr_of(const MixedScale& m, int v) {
+    return v == DEN ? m.thr_r : (v == PRE ? m.thr_p : (v >= VEL ? m.thr_v : 1.0e-30f));
+}
+
+__device__ __forceinline__ unsigned int max_bits(double v) {
+    return v > 0.0 ? __float_as_uint((float)fmin(v, 3.0e38)) : 0u;
+}
 
 // LDS slot of along-position pos (-2..9) of pencil (ta, tb) in sweep DIM.
 // Tile cells: x + 8 (y + 8 z); halo cell of direction DIM at layer h
@@ -57,8 +122,8 @@ __device__ __forceinline__ void read_q(const MixedLDS& s, int pos, int ta, int t
 }
 
 template <int SOLVER, int DIM>
-__device__ __forceinline__ bool face_flux_mixed(const HydroGeom& g, const MixedLDS& s, int f, int ta, int tb,
-                                                const GasConsts<float>& gc, float F[7]) {
+__device__ __forceinline__ bool face_flux_mixed(const HydroGeom& g, const MixedLDS& s, const MixedScale& m, int f,
+                                                int ta, int tb, const GasConsts<float>& gc, float F[7]) {
     float ql[7], qr[7];
     if (g.spatial_order == 1) {
         float a[7], b[7], cc[7], dd[7];
@@ -69,26 +134,26 @@ __device__ __forceinline__ bool face_flux_mixed(const HydroGeom& g, const MixedL
         if (g.limiter == LIM_ALBADA) {
 #pragma unroll
             for (int v = 0; v < 7; ++v) {
-                ql[v] = b[v] + 0.5f * albada_f(b[v] - a[v], cc[v] - b[v]);
-                qr[v] = cc[v] - 0.5f * albada_f(cc[v] - b[v], dd[v] - cc[v]);
+                ql[v] = b[v] + 0.5f * albada_f(b[v] - a[v], cc[v] - b[v], thr_of(m, v));
+                qr[v] = cc[v] - 0.5f * albada_f(cc[v] - b[v], dd[v] - cc[v], thr_of(m, v));
             }
         } else {
 #pragma unroll
             for (int v = 0; v < 7; ++v) {
-                ql[v] = b[v] + 0.5f * limiter<float>(g.limiter, b[v] - a[v], cc[v] - b[v]);
-                qr[v] = cc[v] - 0.5f * limiter<float>(g.limiter, cc[v] - b[v], dd[v] - cc[v]);
+                ql[v] = b[v] + 0.5f * limiter_f(g.limiter, b[v] - a[v], cc[v] - b[v], thr_of(m, v));
+                qr[v] = cc[v] - 0.5f * limiter_f(g.limiter, cc[v] - b[v], dd[v] - cc[v], thr_of(m, v));
             }
         }
     } else {
         read_q<DIM>(s, f - 1, ta, tb, ql);
         read_q<DIM>(s, f, ta, tb, qr);
     }
-    return riemann_f<SOLVER, DIM>(F, ql, qr, gc, g.igm1f);
+    return riemann_f<SOLVER, DIM>(F, ql, qr, gc, g.igm1f, m.nn_thr);
 }
 
 template <int SOLVER, int STAGE, int DIM>
-__device__ __forceinline__ void sweep_mixed(const HydroGeom& g, MixedLDS& s, const int o[3], StepState* st,
-                                            const GasConsts<float>& gc, double udot[7]) {
+__device__ __forceinline__ void sweep_mixed(const HydroGeom& g, MixedLDS& s, const MixedScale& m, const int o[3],
+                                            StepState* st, const GasConsts<float>& gc, double udot[7]) {
     const int tid = threadIdx.x;
     const int N[3] = {g.nx, g.ny, g.nz};
     constexpr int DA = DIM == 0 ? 1 : 0;
@@ -109,7 +174,7 @@ __device__ __forceinline__ void sweep_mixed(const HydroGeom& g, MixedLDS& s, con
         float F[7];
         bool bad = false;
         if (valid) {
-            bad = face_flux_mixed<SOLVER, DIM>(g, s, f, ta, tb, gc, F);
+            bad = face_flux_mixed<SOLVER, DIM>(g, s, m, f, ta, tb, gc, F);
         } else {
 #pragma unroll
             for (int v = 0; v < 7; ++v) F[v] = 0.f;
@@ -147,9 +212,12 @@ __global__ __launch_bounds__(NTHREADS) void hydro_stage_mixed_kernel(HydroGeom g
     const int o[3] = {tx * TILE, ty * TILE, tz * TILE};
     const int tid = threadIdx.x;
     const GasConsts<float> gc = {g.gf, g.gm1f, g.g2f};
+    const MixedScale m = mixed_scale(st);
+    const int ks[7] = {m.kr, m.kp, 0, 0, m.kv, m.kv, m.kv};
 
     // tile cells and the halo slabs of all three directions: Q (FP64
-    // recovery, floors in stage 1) rounded to float into LDS
+    // recovery, floors in stage 1) scaled and rounded to float into LDS
+    unsigned int mr = 0, mp = 0;
     for (int idx = tid; idx < NLOAD; idx += NTHREADS) {
         int c[3];
         bool check = false;
@@ -169,16 +237,47 @@ __global__ __launch_bounds__(NTHREADS) void hydro_stage_mixed_kernel(HydroGeom g
         }
         double q[7];
         load_q<STAGE, true>(g, Usrc, o[0] + c[0], o[1] + c[1], o[2] + c[2], q, st, check);
+        if (STAGE == 1 && check) {
+            mr = max_bits(q[DEN]);
+            mp = max_bits(q[PRE]);
+        }
         // slot idx: the tile cell x + 8 (y + 8 z), or halo slot 512 + 256 dim + 64 h + 8 tb + ta
 #pragma unroll
-        for (int v = 0; v < 7; ++v) s.q[v][idx] = (float)q[v];
+        for (int v = 0; v < 7; ++v) s.q[v][idx] = (float)ldexp(q[v], ks[v]);
+    }
+    if (STAGE == 1) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            mr = max(mr, (unsigned int)__shfl_xor((int)mr, off, 64));
+            mp = max(mp, (unsigned int)__shfl_xor((int)mp, off, 64));
+        }
+        if ((tid & 63) == 0) {
+            s.qmw[tid >> 6][0] = mr;
+            s.qmw[tid >> 6][1] = mp;
+        }
     }
     __syncthreads();
+    if (STAGE == 1 && tid == 0) {
+        unsigned int a = 0, b = 0;
+        for (int w = 0; w < NTHREADS / 64; ++w) {
+            a = max(a, s.qmw[w][0]);
+            b = max(b, s.qmw[w][1]);
+        }
+        if (a) atomicMax(&st->qmax_next[0], a);
+        if (b) atomicMax(&st->qmax_next[1], b);
+    }
 
     double udot[7] = {0, 0, 0, 0, 0, 0, 0};
-    sweep_mixed<SOLVER, STAGE, 0>(g, s, o, st, gc, udot);
-    sweep_mixed<SOLVER, STAGE, 1>(g, s, o, st, gc, udot);
-    sweep_mixed<SOLVER, STAGE, 2>(g, s, o, st, gc, udot);
+    sweep_mixed<SOLVER, STAGE, 0>(g, s, m, o, st, gc, udot);
+    sweep_mixed<SOLVER, STAGE, 1>(g, s, m, o, st, gc, udot);
+    sweep_mixed<SOLVER, STAGE, 2>(g, s, m, o, st, gc, udot);
+    {
+        // the sums are in flux units scaled by 2^(kr+kv) (mass, HII, ADV),
+        // 2^(kp+kv) (energy) and 2^kp (momentum): unscaled exactly
+        const int ke[7] = {m.kr + m.kv, m.kp + m.kv, m.kr + m.kv, m.kr + m.kv, m.kp, m.kp, m.kp};
+#pragma unroll
+        for (int v = 0; v < 7; ++v) udot[v] = ldexp(udot[v], -ke[v]);
+    }
 
     const int lx = tid & 7, ly = (tid >> 3) & 7, lz = tid >> 6;
     const int gx = o[0] + lx, gy = o[1] + ly, gz = o[2] + lz;
@@ -219,9 +318,57 @@ static hipError_t launch_mixed_t(const HydroGeom& g, int stage, const double* Us
     return hipGetLastError();
 }
 
+// Before a predictor launch: the largest density and pressure the previous
+// predictor (or the seeding reduction) recorded become the scale; the record
+// starts afresh.  No record keeps the scale in use.
+__global__ void mixed_scale_rotate_kernel(StepState* st) {
+    if (st->qmax_next[0] | st->qmax_next[1]) {
+        st->qmax[0] = st->qmax_next[0];
+        st->qmax[1] = st->qmax_next[1];
+    }
+    st->qmax_next[0] = 0;
+    st->qmax_next[1] = 0;
+}
+
+// Largest density and pressure (after the floors of fixPrimitives) over the
+// grid into qmax_next, one atomicMax per wave.
+__global__ __launch_bounds__(256) void state_max_kernel(HydroGeom g, const double* __restrict__ U, StepState* st) {
+    unsigned int mr = 0, mp = 0;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < g.ncell;
+         c += (long long)gridDim.x * blockDim.x) {
+        double u[7], q[7];
+#pragma unroll
+        for (int v = 0; v < 7; ++v) u[v] = U[v * g.ncell + c];
+        q_from_u_mixed(q, u, g.gm1);
+        fix_primitives_mixed(g, q, u[DEN]);
+        mr = max(mr, max_bits(q[DEN]));
+        mp = max(mp, max_bits(q[PRE]));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        mr = max(mr, (unsigned int)__shfl_xor((int)mr, off, 64));
+        mp = max(mp, (unsigned int)__shfl_xor((int)mp, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (mr) atomicMax(&st->qmax_next[0], mr);
+        if (mp) atomicMax(&st->qmax_next[1], mp);
+    }
+}
+
+hipError_t launch_mixed_scale_seed(const HydroGeom& g, const double* U, StepState* st, hipStream_t stream) {
+    // clear the record, reduce the grid into it, make it the scale
+    hipLaunchKernelGGL(mixed_scale_rotate_kernel, dim3(1), dim3(1), 0, stream, st);
+    long long nb = (g.ncell + 255) / 256;
+    if (nb > 4096) nb = 4096;
+    hipLaunchKernelGGL(state_max_kernel, dim3((unsigned)nb), dim3(256), 0, stream, g, U, st);
+    hipLaunchKernelGGL(mixed_scale_rotate_kernel, dim3(1), dim3(1), 0, stream, st);
+    return hipGetLastError();
+}
+
 hipError_t launch_hydro_stage_mixed(const HydroGeom& g, int solver, int stage, const double* Usrc,
                                     const double* Ubase, double* Udst, StepState* st, hipStream_t stream,
                                     double dtf) {
+    if (stage == 1) hipLaunchKernelGGL(mixed_scale_rotate_kernel, dim3(1), dim3(1), 0, stream, st);
     switch (solver) {
     case SOLVER_HLL:  return launch_mixed_t<SOLVER_HLL>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
     case SOLVER_HLLC: return launch_mixed_t<SOLVER_HLLC>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);

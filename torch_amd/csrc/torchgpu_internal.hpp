@@ -45,6 +45,11 @@ struct StepState {
     int err_step1;          // 1 + step index of the first error (0: none)
     int step;               // steps completed
     int pad;
+    // mixed path: float bit patterns of the largest density and pressure
+    // the state scale is taken from, and of those the running predictor
+    // records for the next one (hydro_mixed.hip)
+    unsigned int qmax[2];
+    unsigned int qmax_next[2];
 };
 
 // Error bits in the order the reference would meet them inside one step;
@@ -82,6 +87,7 @@ hipError_t launch_fix_solution(const HydroGeom& g, double* U, const double* dpre
 // dtf < 0: the stage takes its dt from st (dt/2 predictor, dt corrector)
 // Mixed-precision stage kernel (hydro_mixed.hip); launch_hydro_stage
 // dispatches to it for precision 1.
+hipError_t launch_mixed_scale_seed(const HydroGeom& g, const double* U, StepState* st, hipStream_t stream);
 hipError_t launch_hydro_stage_mixed(const HydroGeom& g, int solver, int stage, const double* Usrc,
                                     const double* Ubase, double* Udst, StepState* st, hipStream_t stream,
                                     double dtf);
