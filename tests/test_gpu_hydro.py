@@ -110,19 +110,23 @@ def test_deterministic_rerun(gpu):
     np.testing.assert_array_equal(out[0], out[1])
 
 
-def test_nan_state_raises(gpu):
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_nan_state_raises(gpu, precision):
     """A non-finite conserved value makes the reference's checkValues throw
-    (Torch.cpp:420-443); the GPU path reports it at the next sync."""
+    (Torch.cpp:420-443); the GPU path reports it at the next sync, and the
+    run ends at the failing step as the reference's does."""
     Q = setups.sod(16)
-    g = _grid(gpu, Q.shape[1:])
+    g = _grid(gpu, Q.shape[1:], precision=gpu.FP64 if precision == "fp64" else gpu.MIXED)
     g.set_primitive(Q)
     U = g.get_conserved()
     U[1, 3, 4, 5] = np.nan
     g.set_conserved(U)
-    g.run(2)
+    g.run(5)
     with pytest.raises(gpu.TorchGpuError) as e:
         g.synchronize()
     assert e.value.code in (4, 5)
+    assert "(step 0)" in str(e.value)
+    assert g.clock()[2] == 1
 
 
 def test_invalid_order_rejected(gpu):

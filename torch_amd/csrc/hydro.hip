@@ -253,7 +253,9 @@ __global__ void step_begin_kernel(StepState* st, int use_cfl, double dt_override
         return;
     }
     if (st->halted) return;
-    if (!(st->time < st->tmax)) {
+    // a step that raised an error ends the run as the reference's exception
+    // does; the error is reported by the next synchronising call
+    if (!(st->time < st->tmax) || st->err) {
         st->halted = 1;
         return;
     }

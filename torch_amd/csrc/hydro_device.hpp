@@ -134,6 +134,10 @@ __device__ __forceinline__ double cfl_inv_t_mixed(const HydroGeom& g, const doub
 
 // Records an error of the current step unless an earlier step already
 // failed (the reference stops at its first exception).
+// Records error `bit` for the current step unless an earlier step failed
+// first.  (Lane aggregation with a ballot and coherent pre-check loads here
+// slowed the mixed stage kernel by half even with no error raised: a failing
+// state instead ends the run at the next step, see step_begin_kernel.)
 __device__ __forceinline__ void set_error(StepState* st, unsigned int bit) {
     const int s1 = st->step + 1;
     const int prev = atomicCAS(&st->err_step1, 0, s1);
@@ -211,11 +215,15 @@ __device__ __forceinline__ void apply_ghost_mode(double q[7], int mode, int dim)
     else flip_normal(q[VEL + 2], mode);
 }
 
-// Load Q of the cell at global (x, y, z), which may lie in the 2-deep ghost
-// layer of one dimension (the first out-of-range dimension decides).
-template <int STAGE, bool MIXED>
-__device__ __forceinline__ void load_q(const HydroGeom& g, const double* __restrict__ U, int x, int y, int z,
-                                       double q[7], StepState* st, bool check) {
+// Interior cell a stencil position at global (x, y, z) reads, which may lie
+// in the 2-deep ghost layer of one dimension (the first out-of-range
+// dimension decides), and the velocity rule of that ghost layer.
+struct CellSrc {
+    long long idx;
+    int mode, mdim;  // mdim = 1 + ghost dimension, 0 for an interior cell
+};
+
+__device__ __forceinline__ CellSrc cell_source(const HydroGeom& g, int x, int y, int z) {
     int c[3] = {x, y, z};
     const int N[3] = {g.nx, g.ny, g.nz};
     int mode = 0, mdim = 0;
@@ -230,10 +238,14 @@ __device__ __forceinline__ void load_q(const HydroGeom& g, const double* __restr
             c[d] = c[d] < 0 ? 0 : (c[d] >= N[d] ? N[d] - 1 : c[d]);
         }
     }
-    const long long idx = (long long)c[0] + (long long)g.nx * ((long long)c[1] + (long long)g.ny * c[2]);
-    double u[7];
-#pragma unroll
-    for (int v = 0; v < 7; ++v) u[v] = U[v * g.ncell + idx];
+    return CellSrc{(long long)c[0] + (long long)g.nx * ((long long)c[1] + (long long)g.ny * c[2]), mode, mdim};
+}
+
+// Q of a stencil cell from its conserved values u: QfromU, in stage 1
+// fixPrimitives and the "hydro before" check, then the ghost velocity rule.
+template <int STAGE, bool MIXED>
+__device__ __forceinline__ void q_of_cell(const HydroGeom& g, const double u[7], int mode, int mdim, double q[7],
+                                          StepState* st, bool check) {
     if constexpr (MIXED) q_from_u_mixed(q, u, g.gm1);
     else QfromU(q, u, g.gm1);
     if (STAGE == 1) {
@@ -248,6 +260,16 @@ __device__ __forceinline__ void load_q(const HydroGeom& g, const double* __restr
         }
     }
     if (mdim) apply_ghost_mode(q, mode, mdim - 1);
+}
+
+template <int STAGE, bool MIXED>
+__device__ __forceinline__ void load_q(const HydroGeom& g, const double* __restrict__ U, int x, int y, int z,
+                                       double q[7], StepState* st, bool check) {
+    const CellSrc src = cell_source(g, x, y, z);
+    double u[7];
+#pragma unroll
+    for (int v = 0; v < 7; ++v) u[v] = U[v * g.ncell + src.idx];
+    q_of_cell<STAGE, MIXED>(g, u, src.mode, src.mdim, q, st, check);
 }
 
 }  // namespace tg

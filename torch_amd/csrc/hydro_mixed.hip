@@ -121,7 +121,7 @@ __device__ __forceinline__ void read_q(const MixedLDS& s, int pos, int ta, int t
     for (int v = 0; v < 7; ++v) q[v] = s.q[v][i];
 }
 
-template <int SOLVER, int DIM>
+template <int SOLVER, bool ALB, int DIM>
 __device__ __forceinline__ bool face_flux_mixed(const HydroGeom& g, const MixedLDS& s, const MixedScale& m, int f,
                                                 int ta, int tb, const GasConsts<float>& gc, float F[7]) {
     float ql[7], qr[7];
@@ -131,7 +131,7 @@ __device__ __forceinline__ bool face_flux_mixed(const HydroGeom& g, const MixedL
         read_q<DIM>(s, f - 1, ta, tb, b);
         read_q<DIM>(s, f, ta, tb, cc);
         read_q<DIM>(s, f + 1, ta, tb, dd);
-        if (g.limiter == LIM_ALBADA) {
+        if constexpr (ALB) {
 #pragma unroll
             for (int v = 0; v < 7; ++v) {
                 ql[v] = b[v] + 0.5f * albada_f(b[v] - a[v], cc[v] - b[v], thr_of(m, v));
@@ -151,7 +151,7 @@ __device__ __forceinline__ bool face_flux_mixed(const HydroGeom& g, const MixedL
     return riemann_f<SOLVER, DIM>(F, ql, qr, gc, g.igm1f, m.nn_thr);
 }
 
-template <int SOLVER, int STAGE, int DIM>
+template <int SOLVER, bool ALB, int STAGE, int DIM>
 __device__ __forceinline__ void sweep_mixed(const HydroGeom& g, MixedLDS& s, const MixedScale& m, const int o[3],
                                             StepState* st, const GasConsts<float>& gc, double udot[7]) {
     const int tid = threadIdx.x;
@@ -174,7 +174,7 @@ __device__ __forceinline__ void sweep_mixed(const HydroGeom& g, MixedLDS& s, con
         float F[7];
         bool bad = false;
         if (valid) {
-            bad = face_flux_mixed<SOLVER, DIM>(g, s, m, f, ta, tb, gc, F);
+            bad = face_flux_mixed<SOLVER, ALB, DIM>(g, s, m, f, ta, tb, gc, F);
         } else {
 #pragma unroll
             for (int v = 0; v < 7; ++v) F[v] = 0.f;
@@ -201,8 +201,78 @@ __device__ __forceinline__ void sweep_mixed(const HydroGeom& g, MixedLDS& s, con
     __syncthreads();
 }
 
-template <int SOLVER, int STAGE>
-__global__ __launch_bounds__(NTHREADS) void hydro_stage_mixed_kernel(HydroGeom g, const double* __restrict__ Usrc,
+// Tile-relative coordinates of load slot idx: the tile cell x + 8 (y + 8 z),
+// or halo slot 512 + 256 dim + 64 h + 8 tb + ta.
+__device__ __forceinline__ void slot_coords(int idx, int c[3]) {
+    if (idx < TCELLS) {
+        c[0] = idx & 7;
+        c[1] = (idx >> 3) & 7;
+        c[2] = idx >> 6;
+    } else {
+        const int k = idx - TCELLS;
+        const int dim = k >> 8, r = k & 255;
+        const int h = r >> 6, ta = r & 7, tb = (r >> 3) & 7;
+        const int pos = h < 2 ? h - 2 : TILE + h - 2;
+        if (dim == 0) { c[0] = pos; c[1] = ta; c[2] = tb; }
+        else if (dim == 1) { c[0] = ta; c[1] = pos; c[2] = tb; }
+        else { c[0] = ta; c[1] = tb; c[2] = pos; }
+    }
+}
+
+// The tile cells and the halo slabs of all three directions: Q (FP64
+// recovery, floors in stage 1) scaled and rounded to float into LDS, three
+// rounds of 576 slots.  INNER: the tile and its 2-deep halos lie inside the
+// grid (no ghost cells, no clamping).  (Issuing all three rounds' loads
+// before using any, or keeping the stage-1 cell's own conserved values for
+// the update instead of reading them again, measured 4 % slower: the extra
+// registers cost more than they save.)
+template <int STAGE, bool INNER>
+__device__ __forceinline__ void load_tile(const HydroGeom& g, const double* __restrict__ Usrc, const int o[3],
+                                          MixedLDS& s, const int ks[7], StepState* st, unsigned int& mr,
+                                          unsigned int& mp) {
+    constexpr int NR = (NLOAD + NTHREADS - 1) / NTHREADS;
+    const int tid = threadIdx.x;
+    for (int r = 0; r < NR; ++r) {
+        const int idx = tid + r * NTHREADS;
+        if (idx < NLOAD) {
+            int c[3];
+            slot_coords(idx, c);
+            long long src;
+            int mode = 0, mdim = 0;
+            if (INNER) {
+                src = (long long)(o[0] + c[0]) +
+                      (long long)g.nx * ((long long)(o[1] + c[1]) + (long long)g.ny * (o[2] + c[2]));
+            } else {
+                const CellSrc cs = cell_source(g, o[0] + c[0], o[1] + c[1], o[2] + c[2]);
+                src = cs.idx;
+                mode = cs.mode;
+                mdim = cs.mdim;
+            }
+            double u[7];
+#pragma unroll
+            for (int v = 0; v < 7; ++v) u[v] = Usrc[v * g.ncell + src];
+            bool check = false;
+            if (r == 0 && idx < TCELLS) {
+                check = INNER || ((o[0] + (idx & 7) < g.nx) && (o[1] + ((idx >> 3) & 7) < g.ny) &&
+                                  (o[2] + (idx >> 6) < g.nz));
+            }
+            double q[7];
+            q_of_cell<STAGE, true>(g, u, mode, mdim, q, st, check);
+            if (STAGE == 1 && check) {
+                mr = max_bits(q[DEN]);
+                mp = max_bits(q[PRE]);
+            }
+#pragma unroll
+            for (int v = 0; v < 7; ++v) s.q[v][idx] = (float)ldexp(q[v], ks[v]);
+        }
+    }
+}
+
+// Six waves per SIMD (at most 80 VGPRs): two 9-wave workgroups per CU.
+// ALB: van Albada limiter (the other limiters in the ALB = false instance).
+template <int SOLVER, bool ALB, int STAGE>
+__global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(6))) void hydro_stage_mixed_kernel(
+    HydroGeom g, const double* __restrict__ Usrc,
                                                                     const double* Ubase, double* Udst, StepState* st,
                                                                     double dtf_host) {
     __shared__ MixedLDS s;
@@ -215,36 +285,16 @@ __global__ __launch_bounds__(NTHREADS) void hydro_stage_mixed_kernel(HydroGeom g
     const MixedScale m = mixed_scale(st);
     const int ks[7] = {m.kr, m.kp, 0, 0, m.kv, m.kv, m.kv};
 
-    // tile cells and the halo slabs of all three directions: Q (FP64
-    // recovery, floors in stage 1) scaled and rounded to float into LDS
+    const int lx = tid & 7, ly = (tid >> 3) & 7, lz = tid >> 6;
+    const int gx = o[0] + lx, gy = o[1] + ly, gz = o[2] + lz;
+    const bool active = tid < TCELLS && gx < g.nx && gy < g.ny && gz < g.nz;
+    const long long cell = (long long)gx + (long long)g.nx * ((long long)gy + (long long)g.ny * gz);
+
     unsigned int mr = 0, mp = 0;
-    for (int idx = tid; idx < NLOAD; idx += NTHREADS) {
-        int c[3];
-        bool check = false;
-        if (idx < TCELLS) {
-            c[0] = idx & 7;
-            c[1] = (idx >> 3) & 7;
-            c[2] = idx >> 6;
-            check = (o[0] + c[0] < g.nx) && (o[1] + c[1] < g.ny) && (o[2] + c[2] < g.nz);
-        } else {
-            const int k = idx - TCELLS;
-            const int dim = k >> 8, r = k & 255;
-            const int h = r >> 6, ta = r & 7, tb = (r >> 3) & 7;
-            const int pos = h < 2 ? h - 2 : TILE + h - 2;
-            if (dim == 0) { c[0] = pos; c[1] = ta; c[2] = tb; }
-            else if (dim == 1) { c[0] = ta; c[1] = pos; c[2] = tb; }
-            else { c[0] = ta; c[1] = tb; c[2] = pos; }
-        }
-        double q[7];
-        load_q<STAGE, true>(g, Usrc, o[0] + c[0], o[1] + c[1], o[2] + c[2], q, st, check);
-        if (STAGE == 1 && check) {
-            mr = max_bits(q[DEN]);
-            mp = max_bits(q[PRE]);
-        }
-        // slot idx: the tile cell x + 8 (y + 8 z), or halo slot 512 + 256 dim + 64 h + 8 tb + ta
-#pragma unroll
-        for (int v = 0; v < 7; ++v) s.q[v][idx] = (float)ldexp(q[v], ks[v]);
-    }
+    const bool inner = o[0] >= 2 && o[1] >= 2 && o[2] >= 2 && o[0] + TILE + 2 <= g.nx &&
+                       o[1] + TILE + 2 <= g.ny && o[2] + TILE + 2 <= g.nz;
+    if (inner) load_tile<STAGE, true>(g, Usrc, o, s, ks, st, mr, mp);
+    else load_tile<STAGE, false>(g, Usrc, o, s, ks, st, mr, mp);
     if (STAGE == 1) {
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
@@ -268,9 +318,9 @@ __global__ __launch_bounds__(NTHREADS) void hydro_stage_mixed_kernel(HydroGeom g
     }
 
     double udot[7] = {0, 0, 0, 0, 0, 0, 0};
-    sweep_mixed<SOLVER, STAGE, 0>(g, s, m, o, st, gc, udot);
-    sweep_mixed<SOLVER, STAGE, 1>(g, s, m, o, st, gc, udot);
-    sweep_mixed<SOLVER, STAGE, 2>(g, s, m, o, st, gc, udot);
+    sweep_mixed<SOLVER, ALB, STAGE, 0>(g, s, m, o, st, gc, udot);
+    sweep_mixed<SOLVER, ALB, STAGE, 1>(g, s, m, o, st, gc, udot);
+    sweep_mixed<SOLVER, ALB, STAGE, 2>(g, s, m, o, st, gc, udot);
     {
         // the sums are in flux units scaled by 2^(kr+kv) (mass, HII, ADV),
         // 2^(kp+kv) (energy) and 2^kp (momentum): unscaled exactly
@@ -279,13 +329,10 @@ __global__ __launch_bounds__(NTHREADS) void hydro_stage_mixed_kernel(HydroGeom g
         for (int v = 0; v < 7; ++v) udot[v] = ldexp(udot[v], -ke[v]);
     }
 
-    const int lx = tid & 7, ly = (tid >> 3) & 7, lz = tid >> 6;
-    const int gx = o[0] + lx, gy = o[1] + ly, gz = o[2] + lz;
-    const bool active = tid < TCELLS && gx < g.nx && gy < g.ny && gz < g.nz;
     double inv_t = 0.0;
     if (active) {
         // gravity source with GRAV = 0 (Hydro.cpp:134-137) adds nothing
-        const long long c = (long long)gx + (long long)g.nx * ((long long)gy + (long long)g.ny * gz);
+        const long long c = cell;
         const double dtf = dtf_host >= 0.0 ? dtf_host : (STAGE == 1 ? st->dt / 2.0 : st->dt);
         double u[7];
 #pragma unroll
@@ -305,17 +352,28 @@ __global__ __launch_bounds__(NTHREADS) void hydro_stage_mixed_kernel(HydroGeom g
     if (STAGE == 2) block_max_to(inv_t, &st->maxinv, s.red);
 }
 
-template <int SOLVER>
+template <int SOLVER, bool ALB>
 static hipError_t launch_mixed_t(const HydroGeom& g, int stage, const double* Usrc, const double* Ubase,
                                  double* Udst, StepState* st, double dtf, hipStream_t stream) {
     dim3 grid(g.ntiles), block(NTHREADS);
     if (stage == 1)
-        hipLaunchKernelGGL((hydro_stage_mixed_kernel<SOLVER, 1>), grid, block, 0, stream, g, Usrc, Ubase, Udst, st,
-                           dtf);
+        hipLaunchKernelGGL((hydro_stage_mixed_kernel<SOLVER, ALB, 1>), grid, block, 0, stream, g, Usrc, Ubase, Udst,
+                           st, dtf);
     else
-        hipLaunchKernelGGL((hydro_stage_mixed_kernel<SOLVER, 2>), grid, block, 0, stream, g, Usrc, Ubase, Udst, st,
-                           dtf);
+        hipLaunchKernelGGL((hydro_stage_mixed_kernel<SOLVER, ALB, 2>), grid, block, 0, stream, g, Usrc, Ubase, Udst,
+                           st, dtf);
     return hipGetLastError();
+}
+
+// The van Albada instance carries no code of the other limiters: the
+// generic limiter switch at the 42 reconstruction sites (14 per sweep)
+// triples the kernel's code (57 KB against 19 KB), close to the 64 KB
+// instruction cache two CUs share.
+template <int SOLVER>
+static hipError_t launch_mixed_l(const HydroGeom& g, int stage, const double* Usrc, const double* Ubase,
+                                 double* Udst, StepState* st, double dtf, hipStream_t stream) {
+    if (g.limiter == LIM_ALBADA) return launch_mixed_t<SOLVER, true>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
+    return launch_mixed_t<SOLVER, false>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
 }
 
 // Before a predictor launch: the largest density and pressure the previous
@@ -370,9 +428,9 @@ hipError_t launch_hydro_stage_mixed(const HydroGeom& g, int solver, int stage, c
                                     double dtf) {
     if (stage == 1) hipLaunchKernelGGL(mixed_scale_rotate_kernel, dim3(1), dim3(1), 0, stream, st);
     switch (solver) {
-    case SOLVER_HLL:  return launch_mixed_t<SOLVER_HLL>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
-    case SOLVER_HLLC: return launch_mixed_t<SOLVER_HLLC>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
-    default:          return launch_mixed_t<SOLVER_ROTATED>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
+    case SOLVER_HLL:  return launch_mixed_l<SOLVER_HLL>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
+    case SOLVER_HLLC: return launch_mixed_l<SOLVER_HLLC>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
+    default:          return launch_mixed_l<SOLVER_ROTATED>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
     }
 }
 

@@ -40,7 +40,7 @@ struct StepState {
     int first;              // next step is the reference's first step
     int ckpt_count;
     int ncheckpoints;
-    int halted;             // time >= tmax: further steps are no-ops
+    int halted;             // time >= tmax, or a step raised an error: further steps are no-ops
     unsigned int err;       // error bits (ERR_*) raised during step err_step
     int err_step1;          // 1 + step index of the first error (0: none)
     int step;               // steps completed
