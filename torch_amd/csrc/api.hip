@@ -16,6 +16,12 @@ struct torchgpu_ctx {
     double* U = nullptr;      // conserved state
     double* Ustar = nullptr;  // predictor state (also scratch for Q transfers)
     tg::StepState* st = nullptr;
+    // mixed path: float primitives, qf[qcur] = fixPrimitives(QfromU(U)) with
+    // the "hydro before" marks when qvalid, qf[1 - qcur] = QfromU(U*) when
+    // qstar_valid; the stage kernels write the next stage's
+    float* qf[2] = {nullptr, nullptr};
+    int qcur = 0;
+    bool qvalid = false, qstar_valid = false;
     std::string err;
 };
 
@@ -84,6 +90,47 @@ int reset_clock(torchgpu_ctx* c) {
 
 tg::CtxView tg::ctx_view(torchgpu_ctx* c) { return tg::CtxView{c->U, c->Ustar, c->st, c->stream, &c->g}; }
 
+void tg::ctx_state_changed(torchgpu_ctx* c) { c->qvalid = false; }
+
+namespace {
+
+// One hydro stage on the context's buffers.  mode 1: predictor U* from U;
+// 2: corrector U from U* and U; 0: single-stage sub-step, U* from U, then
+// the two buffers swap roles.  dtf < 0: dt from the step state.
+int run_stage(torchgpu_ctx* c, int mode, double dtf) {
+    const int solver = c->p.riemann_solver;
+    if (c->p.precision == TORCHGPU_FP64) {
+        if (mode == 2) HIP_TRY(c, tg::launch_hydro_stage(c->g, solver, 2, c->Ustar, c->U, c->U, c->st, c->stream, dtf));
+        else HIP_TRY(c, tg::launch_hydro_stage(c->g, solver, 1, c->U, c->U, c->Ustar, c->st, c->stream, dtf));
+        if (mode == 0) std::swap(c->U, c->Ustar);
+        return TORCHGPU_OK;
+    }
+    float* q0 = c->qf[c->qcur];
+    float* q1 = c->qf[1 - c->qcur];
+    if (mode == 2) {
+        if (!c->qstar_valid) HIP_TRY(c, tg::launch_q_float(c->g, c->Ustar, q1, 0, c->stream));
+        HIP_TRY(c, tg::launch_hydro_stage_mixed(c->g, solver, 2, q1, c->U, c->U, q0, 1, c->st, c->stream, dtf));
+        c->qvalid = true;
+        c->qstar_valid = true;
+        return TORCHGPU_OK;
+    }
+    if (!c->qvalid) HIP_TRY(c, tg::launch_q_float(c->g, c->U, q0, 1, c->stream));
+    HIP_TRY(c, tg::launch_hydro_stage_mixed(c->g, solver, 1, q0, c->U, c->Ustar, q1, mode == 0, c->st, c->stream,
+                                            dtf));
+    if (mode == 1) {
+        c->qvalid = true;
+        c->qstar_valid = true;
+    } else {
+        std::swap(c->U, c->Ustar);
+        c->qcur = 1 - c->qcur;
+        c->qvalid = true;
+        c->qstar_valid = false;
+    }
+    return TORCHGPU_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int torchgpu_create(const torchgpu_params* p, void* stream, torchgpu_ctx** out) {
@@ -137,6 +184,11 @@ int torchgpu_create(const torchgpu_params* p, void* stream, torchgpu_ctx** out) 
         torchgpu_destroy(c);
         return TORCHGPU_ERR_HIP;
     }
+    if (p->precision == TORCHGPU_MIXED &&
+        (hipMalloc(&c->qf[0], bytes / 2) != hipSuccess || hipMalloc(&c->qf[1], bytes / 2) != hipSuccess)) {
+        torchgpu_destroy(c);
+        return TORCHGPU_ERR_HIP;
+    }
     if (reset_clock(c) != TORCHGPU_OK) {
         torchgpu_destroy(c);
         return TORCHGPU_ERR_HIP;
@@ -151,6 +203,8 @@ void torchgpu_destroy(torchgpu_ctx* c) {
     if (c->U) (void)hipFree(c->U);
     if (c->Ustar) (void)hipFree(c->Ustar);
     if (c->st) (void)hipFree(c->st);
+    if (c->qf[0]) (void)hipFree(c->qf[0]);
+    if (c->qf[1]) (void)hipFree(c->qf[1]);
     delete c;
 }
 
@@ -162,6 +216,7 @@ int torchgpu_last_error(const torchgpu_ctx* c, char* buf, size_t len) {
 
 int torchgpu_state_ptr(torchgpu_ctx* c, double** U) {
     if (!c || !U) return TORCHGPU_ERR_INVALID_ARG;
+    c->qvalid = false;  // the caller may write U through it
     *U = c->U;
     return TORCHGPU_OK;
 }
@@ -171,6 +226,7 @@ int torchgpu_set_primitive(torchgpu_ctx* c, const double* Q, int on_device) {
     HIP_TRY(c, hipMemcpyAsync(c->Ustar, Q, state_bytes(c), on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                               c->stream));
     HIP_TRY(c, tg::launch_u_from_q(c->g, c->Ustar, c->U, c->stream));
+    c->qvalid = c->qstar_valid = false;
     return seed_scale(c);
 }
 
@@ -178,6 +234,7 @@ int torchgpu_set_conserved(torchgpu_ctx* c, const double* U, int on_device) {
     if (!c || !U) return TORCHGPU_ERR_INVALID_ARG;
     HIP_TRY(c, hipMemcpyAsync(c->U, U, state_bytes(c), on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                               c->stream));
+    c->qvalid = false;
     return seed_scale(c);
 }
 
@@ -192,6 +249,7 @@ int torchgpu_get_conserved(torchgpu_ctx* c, double* U, int on_device) {
 int torchgpu_get_primitive(torchgpu_ctx* c, double* Q, int fix, int on_device) {
     if (!c || !Q) return TORCHGPU_ERR_INVALID_ARG;
     HIP_TRY(c, tg::launch_q_from_u(c->g, c->U, c->Ustar, fix, c->stream));
+    c->qstar_valid = false;  // U* holds Q now
     HIP_TRY(c, hipMemcpyAsync(Q, c->Ustar, state_bytes(c), on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                               c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -226,11 +284,9 @@ int torchgpu_hydro_timestep(torchgpu_ctx* c, double dt_max, double* dt) {
 static int hydro_step_device(torchgpu_ctx* c) {
     // Torch::hydroStep: predictor U* = fix(U + dt/2 L(Q(U))), corrector
     // U = fix(U + dt L(Q(U*))); U is updated in place by the corrector.
-    HIP_TRY(c, tg::launch_hydro_stage(c->g, c->p.riemann_solver, c->p.precision, 1, c->U, c->U, c->Ustar, c->st,
-                                      c->stream));
-    HIP_TRY(c, tg::launch_hydro_stage(c->g, c->p.riemann_solver, c->p.precision, 2, c->Ustar, c->U, c->U, c->st,
-                                      c->stream));
-    return TORCHGPU_OK;
+    int rc = run_stage(c, 1, -1.0);
+    if (rc) return rc;
+    return run_stage(c, 2, -1.0);
 }
 
 int torchgpu_hydro_step(torchgpu_ctx* c, double dt) {
@@ -252,29 +308,14 @@ int torchgpu_set_dt(torchgpu_ctx* c, double dt) {
 
 int torchgpu_hydro_stage(torchgpu_ctx* c, int stage) {
     if (!c || (stage != 1 && stage != 2)) return TORCHGPU_ERR_INVALID_ARG;
-    if (stage == 1)
-        HIP_TRY(c, tg::launch_hydro_stage(c->g, c->p.riemann_solver, c->p.precision, 1, c->U, c->U, c->Ustar, c->st,
-                                          c->stream));
-    else
-        HIP_TRY(c, tg::launch_hydro_stage(c->g, c->p.riemann_solver, c->p.precision, 2, c->Ustar, c->U, c->U, c->st,
-                                          c->stream));
-    return TORCHGPU_OK;
+    return run_stage(c, stage, -1.0);
 }
 
 int torchgpu_hydro_stage_dt(torchgpu_ctx* c, int stage, double dtf) {
     if (!c || stage < 0 || stage > 2 || !(dtf >= 0.0)) return TORCHGPU_ERR_INVALID_ARG;
-    const int solver = c->p.riemann_solver, prec = c->p.precision;
-    if (stage == 1) {
-        HIP_TRY(c, tg::launch_hydro_stage(c->g, solver, prec, 1, c->U, c->U, c->Ustar, c->st, c->stream, dtf));
-    } else if (stage == 2) {
-        HIP_TRY(c, tg::launch_hydro_stage(c->g, solver, prec, 2, c->Ustar, c->U, c->U, c->st, c->stream, dtf));
-    } else {
-        // single-stage sub-step (Torch::subStep with the hydro component):
-        // written out of place, then the two state buffers swap roles
-        HIP_TRY(c, tg::launch_hydro_stage(c->g, solver, prec, 1, c->U, c->U, c->Ustar, c->st, c->stream, dtf));
-        std::swap(c->U, c->Ustar);
-    }
-    return TORCHGPU_OK;
+    // stage 0: the single-stage sub-step (Torch::subStep with the hydro
+    // component), written out of place, then the two state buffers swap roles
+    return run_stage(c, stage, dtf);
 }
 
 int torchgpu_run(torchgpu_ctx* c, int nsteps) {

@@ -354,16 +354,13 @@ static hipError_t launch_stage_t(const HydroGeom& g, int stage, const double* Us
     return hipGetLastError();
 }
 
-hipError_t launch_hydro_stage(const HydroGeom& g, int solver, int precision, int stage, const double* Usrc,
-                              const double* Ubase, double* Udst, StepState* st, hipStream_t stream, double dtf) {
-    if (precision == 0) {
-        switch (solver) {
-        case SOLVER_HLL:  return launch_stage_t<SOLVER_HLL>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
-        case SOLVER_HLLC: return launch_stage_t<SOLVER_HLLC>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
-        default:          return launch_stage_t<SOLVER_ROTATED>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
-        }
+hipError_t launch_hydro_stage(const HydroGeom& g, int solver, int stage, const double* Usrc, const double* Ubase,
+                              double* Udst, StepState* st, hipStream_t stream, double dtf) {
+    switch (solver) {
+    case SOLVER_HLL:  return launch_stage_t<SOLVER_HLL>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
+    case SOLVER_HLLC: return launch_stage_t<SOLVER_HLLC>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
+    default:          return launch_stage_t<SOLVER_ROTATED>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
     }
-    return launch_hydro_stage_mixed(g, solver, stage, Usrc, Ubase, Udst, st, stream, dtf);
 }
 
 hipError_t launch_cfl(const HydroGeom& g, const double* U, StepState* st, hipStream_t stream) {

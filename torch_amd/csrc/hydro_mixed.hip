@@ -219,15 +219,53 @@ __device__ __forceinline__ void slot_coords(int idx, int c[3]) {
     }
 }
 
-// The tile cells and the halo slabs of all three directions: Q (FP64
-// recovery, floors in stage 1) scaled and rounded to float into LDS, three
-// rounds of 576 slots.  INNER: the tile and its 2-deep halos lie inside the
-// grid (no ghost cells, no clamping).  (Issuing all three rounds' loads
-// before using any, or keeping the stage-1 cell's own conserved values for
-// the update instead of reading them again, measured 4 % slower: the extra
-// registers cost more than they save.)
+// Ghost-layer velocity rule (see flip_normal) on float primitives.
+__device__ __forceinline__ void flip_normal_f(float& vn, int mode) {
+    switch (mode) {
+    case 1: vn = -vn; break;
+    case 2: if (vn > 0.f) vn = -vn; break;
+    case 3: if (vn < 0.f) vn = -vn; break;
+    case 4: if (vn < 0.f) vn = -vn; break;
+    case 5: if (vn > 0.f) vn = -vn; break;
+    default: break;
+    }
+}
+
+__device__ __forceinline__ void apply_ghost_mode_f(float q[7], int mode, int dim) {
+    if (dim == 0) flip_normal_f(q[VEL + 0], mode);
+    else if (dim == 1) flip_normal_f(q[VEL + 1], mode);
+    else flip_normal_f(q[VEL + 2], mode);
+}
+
+__device__ __forceinline__ unsigned int max_bits_f(float v) { return v > 0.f ? __float_as_uint(v) : 0u; }
+
+// The float primitives a stage reads, written per cell by the stage (or
+// pass) that produced U: Q = QfromU(U) in FP64, rounded to float (not
+// scaled).  In fix mode (the input of a predictor or single-stage sub-step)
+// fixPrimitives is applied and Torch::checkValues("hydro before")
+// (Torch.cpp:420-443) is evaluated on U: a failing cell is stored with a NaN
+// density, which the consuming predictor reports at its step.
+__device__ __forceinline__ void store_qf(const HydroGeom& g, double q[7], const double u[7], float* __restrict__ qout,
+                                         long long c, int fix, bool fixed_already) {
+    if (fix) {
+        if (!fixed_already) fix_primitives_mixed(g, q, u[DEN]);
+        bool bad = (q[DEN] == 0.0) || (q[PRE] == 0.0);
+#pragma unroll
+        for (int v = 0; v < 7; ++v) bad |= (u[v] != u[v]) || isinf(u[v]);
+        if (bad) q[DEN] = __builtin_nan("");
+    }
+#pragma unroll
+    for (int v = 0; v < 7; ++v) qout[v * g.ncell + c] = (float)q[v];
+}
+
+// The tile cells and the halo slabs of all three directions, three rounds of
+// 576 slots: float Q of the source cell, ghost velocity rule, power-of-two
+// scale, into LDS.  INNER: the tile and its 2-deep halos lie inside the grid
+// (no ghost cells, no clamping).  (Issuing all three rounds' loads before
+// using any measured slower: the extra registers cost more than the overlap
+// gains.)
 template <int STAGE, bool INNER>
-__device__ __forceinline__ void load_tile(const HydroGeom& g, const double* __restrict__ Usrc, const int o[3],
+__device__ __forceinline__ void load_tile(const HydroGeom& g, const float* __restrict__ qin, const int o[3],
                                           MixedLDS& s, const int ks[7], StepState* st, unsigned int& mr,
                                           unsigned int& mp) {
     constexpr int NR = (NLOAD + NTHREADS - 1) / NTHREADS;
@@ -248,33 +286,33 @@ __device__ __forceinline__ void load_tile(const HydroGeom& g, const double* __re
                 mode = cs.mode;
                 mdim = cs.mdim;
             }
-            double u[7];
+            float q[7];
 #pragma unroll
-            for (int v = 0; v < 7; ++v) u[v] = Usrc[v * g.ncell + src];
-            bool check = false;
-            if (r == 0 && idx < TCELLS) {
-                check = INNER || ((o[0] + (idx & 7) < g.nx) && (o[1] + ((idx >> 3) & 7) < g.ny) &&
-                                  (o[2] + (idx >> 6) < g.nz));
-            }
-            double q[7];
-            q_of_cell<STAGE, true>(g, u, mode, mdim, q, st, check);
-            if (STAGE == 1 && check) {
-                mr = max_bits(q[DEN]);
-                mp = max_bits(q[PRE]);
+            for (int v = 0; v < 7; ++v) q[v] = qin[v * g.ncell + src];
+            if (mdim) apply_ghost_mode_f(q, mode, mdim - 1);
+            if (STAGE == 1 && r == 0 && idx < TCELLS) {
+                const bool check = INNER || ((o[0] + (idx & 7) < g.nx) && (o[1] + ((idx >> 3) & 7) < g.ny) &&
+                                             (o[2] + (idx >> 6) < g.nz));
+                if (check) {
+                    if (q[DEN] != q[DEN]) set_error(st, ERR_CHECK_VALUES);
+                    mr = max_bits_f(q[DEN]);
+                    mp = max_bits_f(q[PRE]);
+                }
             }
 #pragma unroll
-            for (int v = 0; v < 7; ++v) s.q[v][idx] = (float)ldexp(q[v], ks[v]);
+            for (int v = 0; v < 7; ++v) s.q[v][idx] = __builtin_ldexpf(q[v], ks[v]);
         }
     }
 }
 
 // Six waves per SIMD (at most 80 VGPRs): two 9-wave workgroups per CU.
 // ALB: van Albada limiter (the other limiters in the ALB = false instance).
+// qin: float Q of the input state (fixPrimitives applied for stage 1);
+// qout: float Q of the output state for the next stage, fixed if qout_fix.
 template <int SOLVER, bool ALB, int STAGE>
 __global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(6))) void hydro_stage_mixed_kernel(
-    HydroGeom g, const double* __restrict__ Usrc,
-                                                                    const double* Ubase, double* Udst, StepState* st,
-                                                                    double dtf_host) {
+    HydroGeom g, const float* __restrict__ qin, const double* Ubase, double* Udst, float* __restrict__ qout,
+    int qout_fix, StepState* st, double dtf_host) {
     __shared__ MixedLDS s;
     if (st->halted) return;
     const int tile = tile_of_block(blockIdx.x, g.ntiles);
@@ -293,8 +331,8 @@ __global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(6))) v
     unsigned int mr = 0, mp = 0;
     const bool inner = o[0] >= 2 && o[1] >= 2 && o[2] >= 2 && o[0] + TILE + 2 <= g.nx &&
                        o[1] + TILE + 2 <= g.ny && o[2] + TILE + 2 <= g.nz;
-    if (inner) load_tile<STAGE, true>(g, Usrc, o, s, ks, st, mr, mp);
-    else load_tile<STAGE, false>(g, Usrc, o, s, ks, st, mr, mp);
+    if (inner) load_tile<STAGE, true>(g, qin, o, s, ks, st, mr, mp);
+    else load_tile<STAGE, false>(g, qin, o, s, ks, st, mr, mp);
     if (STAGE == 1) {
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
@@ -342,26 +380,27 @@ __global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(6))) v
         if (fix_solution_mixed(g, u)) set_error(st, STAGE == 1 ? ERR_FIX_SOLUTION_1 : ERR_FIX_SOLUTION_2);
 #pragma unroll
         for (int v = 0; v < 7; ++v) Udst[v * g.ncell + c] = u[v];
+        double q[7];
+        q_from_u_mixed(q, u, g.gm1);
         if (STAGE == 2) {
-            double q[7];
-            q_from_u_mixed(q, u, g.gm1);
             fix_primitives_mixed(g, q, u[DEN]);
             inv_t = cfl_inv_t_mixed(g, q);
         }
+        store_qf(g, q, u, qout, c, qout_fix, STAGE == 2);
     }
     if (STAGE == 2) block_max_to(inv_t, &st->maxinv, s.red);
 }
 
 template <int SOLVER, bool ALB>
-static hipError_t launch_mixed_t(const HydroGeom& g, int stage, const double* Usrc, const double* Ubase,
-                                 double* Udst, StepState* st, double dtf, hipStream_t stream) {
+static hipError_t launch_mixed_t(const HydroGeom& g, int stage, const float* qin, const double* Ubase, double* Udst,
+                                 float* qout, int qout_fix, StepState* st, double dtf, hipStream_t stream) {
     dim3 grid(g.ntiles), block(NTHREADS);
     if (stage == 1)
-        hipLaunchKernelGGL((hydro_stage_mixed_kernel<SOLVER, ALB, 1>), grid, block, 0, stream, g, Usrc, Ubase, Udst,
-                           st, dtf);
+        hipLaunchKernelGGL((hydro_stage_mixed_kernel<SOLVER, ALB, 1>), grid, block, 0, stream, g, qin, Ubase, Udst,
+                           qout, qout_fix, st, dtf);
     else
-        hipLaunchKernelGGL((hydro_stage_mixed_kernel<SOLVER, ALB, 2>), grid, block, 0, stream, g, Usrc, Ubase, Udst,
-                           st, dtf);
+        hipLaunchKernelGGL((hydro_stage_mixed_kernel<SOLVER, ALB, 2>), grid, block, 0, stream, g, qin, Ubase, Udst,
+                           qout, qout_fix, st, dtf);
     return hipGetLastError();
 }
 
@@ -370,10 +409,32 @@ static hipError_t launch_mixed_t(const HydroGeom& g, int stage, const double* Us
 // triples the kernel's code (57 KB against 19 KB), close to the 64 KB
 // instruction cache two CUs share.
 template <int SOLVER>
-static hipError_t launch_mixed_l(const HydroGeom& g, int stage, const double* Usrc, const double* Ubase,
-                                 double* Udst, StepState* st, double dtf, hipStream_t stream) {
-    if (g.limiter == LIM_ALBADA) return launch_mixed_t<SOLVER, true>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
-    return launch_mixed_t<SOLVER, false>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
+static hipError_t launch_mixed_l(const HydroGeom& g, int stage, const float* qin, const double* Ubase, double* Udst,
+                                 float* qout, int qout_fix, StepState* st, double dtf, hipStream_t stream) {
+    if (g.limiter == LIM_ALBADA)
+        return launch_mixed_t<SOLVER, true>(g, stage, qin, Ubase, Udst, qout, qout_fix, st, dtf, stream);
+    return launch_mixed_t<SOLVER, false>(g, stage, qin, Ubase, Udst, qout, qout_fix, st, dtf, stream);
+}
+
+// Float Q of a state for the next mixed stage (see store_qf), for a state
+// that no stage produced (set by the caller, or changed by another operator).
+__global__ __launch_bounds__(256) void q_float_kernel(HydroGeom g, const double* __restrict__ U, float* __restrict__ q,
+                                                      int fix) {
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < g.ncell;
+         c += (long long)gridDim.x * blockDim.x) {
+        double u[7], qd[7];
+#pragma unroll
+        for (int v = 0; v < 7; ++v) u[v] = U[v * g.ncell + c];
+        q_from_u_mixed(qd, u, g.gm1);
+        store_qf(g, qd, u, q, c, fix, false);
+    }
+}
+
+hipError_t launch_q_float(const HydroGeom& g, const double* U, float* q, int fix, hipStream_t stream) {
+    long long nb = (g.ncell + 255) / 256;
+    if (nb > 65536) nb = 65536;
+    hipLaunchKernelGGL(q_float_kernel, dim3((unsigned)nb), dim3(256), 0, stream, g, U, q, fix);
+    return hipGetLastError();
 }
 
 // Before a predictor launch: the largest density and pressure the previous
@@ -423,14 +484,15 @@ hipError_t launch_mixed_scale_seed(const HydroGeom& g, const double* U, StepStat
     return hipGetLastError();
 }
 
-hipError_t launch_hydro_stage_mixed(const HydroGeom& g, int solver, int stage, const double* Usrc,
-                                    const double* Ubase, double* Udst, StepState* st, hipStream_t stream,
+hipError_t launch_hydro_stage_mixed(const HydroGeom& g, int solver, int stage, const float* qin, const double* Ubase,
+                                    double* Udst, float* qout, int qout_fix, StepState* st, hipStream_t stream,
                                     double dtf) {
     if (stage == 1) hipLaunchKernelGGL(mixed_scale_rotate_kernel, dim3(1), dim3(1), 0, stream, st);
     switch (solver) {
-    case SOLVER_HLL:  return launch_mixed_l<SOLVER_HLL>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
-    case SOLVER_HLLC: return launch_mixed_l<SOLVER_HLLC>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
-    default:          return launch_mixed_l<SOLVER_ROTATED>(g, stage, Usrc, Ubase, Udst, st, dtf, stream);
+    case SOLVER_HLL:  return launch_mixed_l<SOLVER_HLL>(g, stage, qin, Ubase, Udst, qout, qout_fix, st, dtf, stream);
+    case SOLVER_HLLC: return launch_mixed_l<SOLVER_HLLC>(g, stage, qin, Ubase, Udst, qout, qout_fix, st, dtf, stream);
+    default:
+        return launch_mixed_l<SOLVER_ROTATED>(g, stage, qin, Ubase, Udst, qout, qout_fix, st, dtf, stream);
     }
 }
 

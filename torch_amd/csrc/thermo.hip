@@ -943,6 +943,7 @@ int torchgpu_thermo_grid_substep(torchgpu_ctx* ctx, torchgpu_cooling* c, const d
     if (rc) return rc;
     // Thermodynamics::updateSourceTerms + advSolution(dt) + fixSolution
     HIP_OK(tg::launch_fix_solution(*v.g, v.U, c->out5 + 2 * n, dt, v.st, c->stream));
+    tg::ctx_state_changed(ctx);
     return TORCHGPU_OK;
 }
 

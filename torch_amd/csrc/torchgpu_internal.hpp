@@ -82,18 +82,23 @@ CtxView ctx_view(torchgpu_ctx* c);
 // Launchers (hydro.hip).  All enqueue on `stream` and return hipError_t.
 hipError_t launch_fix_solution(const HydroGeom& g, double* U, const double* dpre, double dtf, StepState* st,
                                hipStream_t stream);
-// precision: 0 = FP64 everywhere, 1 = FP32 reconstruction + Riemann flux
-// with FP64 accumulation, pressure recovery and CFL.
-// dtf < 0: the stage takes its dt from st (dt/2 predictor, dt corrector)
-// Mixed-precision stage kernel (hydro_mixed.hip); launch_hydro_stage
-// dispatches to it for precision 1.
+// FP64 stage kernel (hydro.hip).  dtf < 0: the stage takes its dt from st
+// (dt/2 predictor, dt corrector).
+hipError_t launch_hydro_stage(const HydroGeom& g, int solver, int stage, const double* Usrc, const double* Ubase,
+                              double* Udst, StepState* st, hipStream_t stream, double dtf = -1.0);
+// Mixed-precision stage kernel (hydro_mixed.hip): FP32 reconstruction and
+// Riemann flux, FP64 accumulation, pressure recovery and CFL.  It reads the
+// float primitives qin of its input state and writes those of its output
+// (qout, with fixPrimitives and the "hydro before" check if qout_fix) for the
+// next stage; launch_q_float produces them for any other state.
 hipError_t launch_mixed_scale_seed(const HydroGeom& g, const double* U, StepState* st, hipStream_t stream);
-hipError_t launch_hydro_stage_mixed(const HydroGeom& g, int solver, int stage, const double* Usrc,
-                                    const double* Ubase, double* Udst, StepState* st, hipStream_t stream,
-                                    double dtf);
-hipError_t launch_hydro_stage(const HydroGeom& g, int solver, int precision, int stage,
-                              const double* Usrc, const double* Ubase, double* Udst,
-                              StepState* st, hipStream_t stream, double dtf = -1.0);
+hipError_t launch_q_float(const HydroGeom& g, const double* U, float* q, int fix, hipStream_t stream);
+hipError_t launch_hydro_stage_mixed(const HydroGeom& g, int solver, int stage, const float* qin, const double* Ubase,
+                                    double* Udst, float* qout, int qout_fix, StepState* st, hipStream_t stream,
+                                    double dtf = -1.0);
+// The grid state was changed by another operator (the float primitives of
+// the mixed path are recomputed before the next stage).
+void ctx_state_changed(torchgpu_ctx* c);
 hipError_t launch_cfl(const HydroGeom& g, const double* U, StepState* st, hipStream_t stream);
 hipError_t launch_step_begin(StepState* st, int use_cfl, double dt_override, hipStream_t stream);
 hipError_t launch_step_end(StepState* st, hipStream_t stream);
