@@ -33,13 +33,13 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 SIGMA, MACH, GAMMA = 1.0, 2.0, 1.67
-TFLOOR = 1e-12  # see DESIGN.md: with the C++ default 0 the reference aborts on zero pressure
+TFLOOR = 1e-12  # with the C++ default 0 the reference's step aborts on a zero pressure
 R_CGS = 8.314462e7
 PFLOOR = 0.1 * R_CGS * 1e-30 * TFLOOR
-# algorithmic HBM bytes per cell: 7 FP64 conserved variables
-BYTES_STAGE1 = 2 * 56   # read U^n, write U*
-BYTES_STAGE2 = 3 * 56   # read U*, read U^n, write U^n+1
-BYTES_STEP = BYTES_STAGE1 + BYTES_STAGE2
+# algorithmic HBM bytes per cell of a mixed-precision stage: 7 FP64
+# conserved variables (56 B) and their 7 float primitives (28 B); the
+# corrector reads Q(U*) and U^n and writes U^n+1 and Q(U^n+1)
+BYTES_STAGE2 = 28 + 56 + 56 + 28
 
 
 def log(*a):
@@ -241,7 +241,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}); "
-              "see DESIGN.md for how the grid would shard.", file=sys.stderr)
+              "the grid would shard as 3-D blocks with 2-deep halo exchanges per stage.", file=sys.stderr)
         sys.exit(2)
 
     cpu = None
@@ -259,7 +259,7 @@ def main():
         log(f"cooling {cool} ({time.perf_counter() - t0:.1f}s)")
     ncell = args.n ** 3
     achieved = BYTES_STAGE2 * ncell / (mx["stage2_ms"] * 1e-3) / 1e9
-    traffic = measured_traffic(args.n, "void tg::hydro_stage_mixed_kernel<2, 2>")
+    traffic = measured_traffic(args.n, "void tg::hydro_stage_mixed_kernel<2, true, 2>")
     out = {
         "metric": "Mcell-updates/s: FP64 vs mixed hydro; surrogate vs subcycled cooling",
         "value": round(mx["mcells"], 2),
