@@ -23,7 +23,11 @@
  *   Thermodynamics::preTimeStepCalculations + integrate   torchgpu_thermo_cells
  *     + updateSourceTerms on independent cells
  *     (src/Integrators/Thermodynamics.cpp:218-337,478-485)
- *   Thermodynamics operator on the grid state             torchgpu_thermo_substep
+ *   Thermodynamics::calculateTimeStep / the thermo part   torchgpu_thermo_grid_timestep /
+ *     of Torch::subStep on the grid state                   torchgpu_thermo_grid_substep
+ *     (Thermodynamics.cpp:463-485, Torch.cpp:352-364)
+ *   Torch::run / fullStep / calculateTimeStep with        torchgpu_run_split
+ *     hydrodynamics and thermodynamics (Torch.cpp:240-418)
  *   MLP surrogate of the subcycled update (no reference   torchgpu_mlp_*,
  *     counterpart; FP64 integrate remains the fallback)     torchgpu_thermo_cells_surrogate
  *
@@ -220,6 +224,17 @@ int torchgpu_thermo_grid_timestep(torchgpu_ctx* ctx, torchgpu_cooling* c, const 
  * updateSourceTerms + Fluid::advSolution(dt) + fixSolution. */
 int torchgpu_thermo_grid_substep(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* T_min, const double* col_den,
                                  const double* F_FUV, int64_t excluded, double dt, int surrogate);
+/* Up to `nsteps` iterations of the Torch::run loop for a run with
+ * hydrodynamics and thermodynamics (Torch.cpp:240-302): each a
+ * Torch::fullStep (Torch.cpp:389-418) with dt = min(dt_nextCheckpoint,
+ * calculateTimeStep()) -- dt_max * 1e-20 first, then min(dt_max, hydro CFL,
+ * thermo min |E/RATE|) (Torch.cpp:304-339) -- chosen on the device, and the
+ * Strang alternation of single-stage hydro sub-steps (Torch::subStep) and
+ * thermo sub-steps (as torchgpu_thermo_grid_substep).  Both handles must use
+ * the same stream.  Asynchronous: no host round trip inside the loop; errors
+ * end the run and surface at the next synchronizing call. */
+int torchgpu_run_split(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* T_min, const double* col_den,
+                       const double* F_FUV, int64_t excluded, int nsteps, int surrogate);
 
 /* ------------------------------------------------------------------------ */
 /* Cooling surrogate (no reference counterpart: a learned replacement of the  */

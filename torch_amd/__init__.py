@@ -240,6 +240,7 @@ def _bind_cooling(L):
     L.torchgpu_thermo_cells.argtypes = [vp, ctypes.c_int64, vp, vp, ctypes.POINTER(ctypes.c_int64)]
     L.torchgpu_thermo_grid_timestep.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int64, d, ctypes.POINTER(d)]
     L.torchgpu_thermo_grid_substep.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int64, d, i]
+    L.torchgpu_run_split.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int64, i, i]
 
 
 def units(scales, value, m, l, t, to_code=True) -> float:
@@ -320,6 +321,13 @@ class Cooling:
         """The thermo sub-step of Torch::subStep on the grid state."""
         self._check(lib().torchgpu_thermo_grid_substep(grid._h, self._h, self._dptr(T_min), self._dptr(col_den),
                                                        self._dptr(F_FUV), excluded, dt, int(surrogate)))
+
+    def run_split(self, grid: "Grid", nsteps: int, T_min=None, col_den=None, F_FUV=None, excluded: int = -1,
+                  surrogate: bool = False):
+        """`nsteps` Torch::fullStep iterations of hydro + thermo with dt and
+        the Strang alternation on the device (no host round trip)."""
+        self._check(lib().torchgpu_run_split(grid._h, self._h, self._dptr(T_min), self._dptr(col_den),
+                                             self._dptr(F_FUV), excluded, nsteps, int(surrogate)))
 
     def thermo_cells(self, inp, out=None, count=True):
         """inp: device float64 tensor [9][n]; returns out [5][n] (device) and

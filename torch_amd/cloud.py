@@ -1,7 +1,7 @@
 """Operator-split hydro + cooling on the cloud set-up (the 384^3 workload).
 
     python -m torch_amd.cloud [--n 384] [--steps 100] [--precision mixed]
-                              [--cooling surrogate|fp64] [--surrogate PATH]
+                              [--cooling surrogate|fp64] [--surrogate PATH] [--host-loop]
 
 Set-up (cgs, converted with the scales 1e-20 g/cm^3, 1e-28 dyn/cm^2, 1e11 s):
 a cube of side 0.5 pc with an ionising source at its centre; hydrogen number
@@ -22,7 +22,10 @@ dt = min(hydro CFL, thermo dt) (first step dt_max 1e-20), then the Strang
 sequence alternating T(dt/2) H(dt) T(dt/2) and H(dt/2) T(dt) H(dt/2) with
 single-stage hydro sub-steps; the thermo sub-steps use the surrogate (FP64
 prepass, BF16 MLP, FP64 fallback gated at 1% estimated error) or the FP64
-subcycled integrator.
+subcycled integrator.  By default the whole loop runs on the device
+(torchgpu_run_split: dt reductions and the sequence enqueued without host
+round trips); --host-loop drives each step from the host (HostLoop), which
+reads dt back as the reference's driver does.
 """
 from __future__ import annotations
 
@@ -96,6 +99,57 @@ def cooling_inputs(g, p, col_den, F_FUV, excluded, dt):
     return torch.stack([dc, pre, hii, adv, U[1], tmin, col_den, F_FUV, torch.full_like(den, dt)]).contiguous()
 
 
+def build(n: int, precision: str = "mixed", tmax_s: float = 1.0e13):
+    """The cloud set-up on the GPU: (params, Grid, Cooling, thermo keyword
+    arguments {col_den, F_FUV, excluded} as device tensors)."""
+    import torch
+
+    import torch_amd as ta
+
+    tc = lambda v, m, l, t: ta.units(SCALES, v, m, l, t)  # noqa: E731
+    Q, col, ffuv, excluded, side = setup(n, tc)
+    dfloor = tc(1.0e-30, 1, -3, 0)
+    R = tc(R_CGS, 0, 2, -2)
+    tfloor = 0.1
+    p = ta.make_params((n, n, n), side_length=side, gamma=1.67, dfloor=dfloor, pfloor=0.1 * R * dfloor * tfloor,
+                       tfloor=tfloor, spec_gas_const=R, tmax=tc(tmax_s, 0, 0, 1),
+                       precision=ta.MIXED if precision == "mixed" else ta.FP64)
+    g = ta.Grid(p)
+    g.set_primitive(np.ascontiguousarray(Q))
+    cool = ta.Cooling(ta.thermo_params(scales=SCALES))
+    thermo = dict(col_den=torch.from_numpy(col).cuda(), F_FUV=torch.from_numpy(ffuv).cuda(), excluded=excluded)
+    return p, g, cool, thermo
+
+
+class HostLoop:
+    """Torch::fullStep driven from the host: dt from the two operators' time
+    steps (host round trips), the Strang sequence as separate calls."""
+
+    def __init__(self, g, cool, thermo, dt_max, surrogate):
+        self.g, self.cool, self.thermo, self.dt_max, self.surrogate = g, cool, thermo, dt_max, surrogate
+        self.first = True
+        self.counter = 0
+
+    def step(self) -> float:
+        # Torch::calculateTimeStep
+        if self.first:
+            dt = self.dt_max * 1.0e-20
+            self.first = False
+        else:
+            dt = min(self.g.hydro_timestep(self.dt_max), self.cool.grid_timestep(self.g, self.dt_max, **self.thermo))
+        # Strang alternation of Torch::fullStep over (hydro, thermo)
+        self.counter = (self.counter + 1) % 2
+        comps = ["hydro", "thermo"]
+        seq = [(comps[(0 + self.counter) % 2], 0.5 * dt), (comps[(1 + self.counter) % 2], dt),
+               (comps[(0 + self.counter) % 2], dt / 2.0)]
+        for comp, h in seq:
+            if comp == "hydro":
+                self.g.stage_dt(0, h)
+            else:
+                self.cool.grid_substep(self.g, h, surrogate=self.surrogate, **self.thermo)
+        return dt
+
+
 def main():
     import torch
 
@@ -114,77 +168,53 @@ def main():
     ap.add_argument("--every", type=int, default=10, help="report (and check for step errors) every N steps")
     ap.add_argument("--check", default="", help="comma-separated steps after which the surrogate and the FP64 "
                     "integrator are compared on the grid's cells (untimed)")
+    ap.add_argument("--host-loop", action="store_true",
+                    help="drive each step from the host (dt read back) instead of torchgpu_run_split")
     a = ap.parse_args()
     n = a.n
-    tc = lambda v, m, l, t: ta.units(SCALES, v, m, l, t)  # noqa: E731
     t0 = time.perf_counter()
-    Q, col, ffuv, excluded, side = setup(n, tc)
-    dfloor = tc(1.0e-30, 1, -3, 0)
-    R = tc(R_CGS, 0, 2, -2)
-    tfloor = 0.1
-    p = ta.make_params((n, n, n), side_length=side, gamma=1.67, dfloor=dfloor, pfloor=0.1 * R * dfloor * tfloor,
-                       tfloor=tfloor, spec_gas_const=R, tmax=tc(a.tmax, 0, 0, 1),
-                       precision=ta.MIXED if a.precision == "mixed" else ta.FP64)
-    g = ta.Grid(p)
-    g.set_primitive(np.ascontiguousarray(Q))
-    del Q
-    col_d = torch.from_numpy(col).cuda()
-    f_d = torch.from_numpy(ffuv).cuda()
-    cool = ta.Cooling(ta.thermo_params(scales=SCALES))
+    p, g, cool, thermo = build(n, a.precision, a.tmax)
     mlp = None
     use_sur = a.cooling == "surrogate"
     if use_sur:
         mlp = sg.attach(cool, a.surrogate, gate=a.gate)
     torch.cuda.synchronize()
     print(json.dumps({"setup": f"cloud {n}^3", "seconds": round(time.perf_counter() - t0, 1)}), flush=True)
-    thermo = dict(col_den=col_d, F_FUV=f_d, excluded=excluded)
-    dt_max = p.dt_max
-    first = True
-    counter = 0
-    sim_t = 0.0
+    loop = HostLoop(g, cool, thermo, p.dt_max, use_sur)
     checks = {int(x) for x in a.check.split(",") if x}
+    stops = sorted({s for s in range(a.every, a.steps + 1, a.every)} | {a.steps} | {c for c in checks if c <= a.steps})
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     ms = 0.0
-    ev0.record()
     t_wall = time.perf_counter()
-    for step in range(1, a.steps + 1):
-        # Torch::calculateTimeStep
-        if first:
-            dt = dt_max * 1.0e-20
-            first = False
+    done = 0
+    sim_t = 0.0
+    for stop in stops:
+        ev0.record()
+        if a.host_loop:
+            for _ in range(stop - done):
+                last_dt = loop.step()
+                sim_t += last_dt
         else:
-            dt = min(g.hydro_timestep(dt_max), cool.grid_timestep(g, dt_max, **thermo))
-        # Strang alternation of Torch::fullStep over (hydro, thermo)
-        counter = (counter + 1) % 2
-        comps = ["hydro", "thermo"]
-        seq = [(comps[(0 + counter) % 2], 0.5 * dt), (comps[(1 + counter) % 2], dt),
-               (comps[(0 + counter) % 2], 0.5 * dt)]
-        for comp, h in seq:
-            if comp == "hydro":
-                g.stage_dt(0, h)
-            else:
-                cool.grid_substep(g, h, surrogate=use_sur, **thermo)
-        sim_t += dt
-        if step % a.every == 0 or step == a.steps:
-            g.synchronize()
-            print(json.dumps({"step": step, "dt": dt, "time": sim_t,
-                              "elapsed_s": round(time.perf_counter() - t_wall, 2)}), flush=True)
-        if step in checks and use_sur:
-            ev1.record()
-            torch.cuda.synchronize()
-            ms += ev0.elapsed_time(ev1)
-            inp = cooling_inputs(g, p, col_d, f_d, excluded, dt)
+            cool.run_split(g, stop - done, surrogate=use_sur, **thermo)
+        ev1.record()
+        g.synchronize()
+        ms += ev0.elapsed_time(ev1)
+        done = stop
+        t_dev, dt, nst = g.clock()
+        if a.host_loop:
+            t_dev, dt = sim_t, last_dt
+        print(json.dumps({"step": stop, "dt": dt, "time": t_dev,
+                          "elapsed_s": round(time.perf_counter() - t_wall, 2)}), flush=True)
+        if stop in checks and use_sur:
+            inp = cooling_inputs(g, p, thermo["col_den"], thermo["F_FUV"], thermo["excluded"], dt)
             r = sg.compare(cool, inp)
             r["out_of_envelope"] = r["cells"] - r["exact"] - r["mlp_queued"]
-            print(json.dumps({"step": step, "cooling_check": r}), flush=True)
+            print(json.dumps({"step": stop, "cooling_check": r}), flush=True)
             del inp
             torch.cuda.empty_cache()
-            ev0.record()
-    ev1.record()
-    torch.cuda.synchronize()
-    ms += ev0.elapsed_time(ev1)
     print(json.dumps({"workload": f"cloud {n}^3, {a.steps} operator-split steps, {a.precision} hydro, "
-                                  f"{a.cooling} cooling", "ms_per_step": round(ms / a.steps, 3),
+                                  f"{a.cooling} cooling, {'host' if a.host_loop else 'device'} loop",
+                      "ms_per_step": round(ms / a.steps, 3),
                       "Mcell_updates_per_s": round(n ** 3 * a.steps / (ms * 1e-3) / 1e6, 2)}), flush=True)
     del mlp
 

@@ -22,6 +22,7 @@ struct torchgpu_ctx {
     float* qf[2] = {nullptr, nullptr};
     int qcur = 0;
     bool qvalid = false, qstar_valid = false;
+    int step_counter = 0;  // Torch::stepCounter of the operator-split loop
     std::string err;
 };
 
@@ -81,6 +82,7 @@ int reset_clock(torchgpu_ctx* c) {
     st.tmax = c->p.tmax;
     st.first = 1;
     st.ncheckpoints = c->p.ncheckpoints > 0 ? c->p.ncheckpoints : 1;
+    c->step_counter = 0;
     HIP_TRY(c, hipMemcpyAsync(c->st, &st, sizeof st, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return TORCHGPU_OK;
@@ -130,6 +132,9 @@ int run_stage(torchgpu_ctx* c, int mode, double dtf) {
 }
 
 }  // namespace
+
+int tg::ctx_run_stage(torchgpu_ctx* c, int mode, double dtf) { return run_stage(c, mode, dtf); }
+int& tg::ctx_step_counter(torchgpu_ctx* c) { return c->step_counter; }
 
 extern "C" {
 

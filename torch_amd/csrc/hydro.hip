@@ -199,7 +199,7 @@ __global__ __launch_bounds__(NTHREADS) void hydro_stage_kernel(HydroGeom g, cons
             udot[PRE] += s.qc[VEL + i][tid] * grav;
         }
         const long long c = (long long)gx + (long long)g.nx * ((long long)gy + (long long)g.ny * gz);
-        const double dtf = dtf_host >= 0.0 ? dtf_host : (STAGE == 1 ? st->dt / 2.0 : st->dt);
+        const double dtf = stage_dtf(dtf_host, st, STAGE);
         double u[7];
 #pragma unroll
         for (int v = 0; v < 7; ++v) u[v] = Ubase[v * g.ncell + c];
@@ -246,7 +246,8 @@ __global__ __launch_bounds__(256) void cfl_kernel(HydroGeom g, const double* __r
 // Torch::fullStep + Torch::calculateTimeStep (Torch.cpp:266-270,304-339,397),
 // with the checkpoint arithmetic of Checkpointer::update (Checkpointer.cpp:14-26).
 // use_cfl = 0 takes dt_override instead of the CFL/first-step value.
-__global__ void step_begin_kernel(StepState* st, int use_cfl, double dt_override) {
+__global__ void step_begin_kernel(StepState* st, int use_cfl, double dt_override,
+                                  const unsigned long long* thermo_dt_bits) {
     if (!use_cfl) {  // an explicit Torch::hydroStep(dt): no clock rules apply
         st->halted = 0;
         st->dt = dt_override;
@@ -274,6 +275,12 @@ __global__ void step_begin_kernel(StepState* st, int use_cfl, double dt_override
         double tmin = st->dt_max;
         if (maxinv > 0.0) tmin = smin(tmin, 0.5 / maxinv);
         dt = smin(tmin, st->dt_max);
+        if (thermo_dt_bits) {
+            // Thermodynamics::calculateTimeStep: dt_max lowered by every
+            // smaller |f E/RATE| (Thermodynamics.cpp:465-474)
+            const double m = __longlong_as_double((long long)*thermo_dt_bits);
+            dt = smin(dt, smin(m, st->dt_max));
+        }
     }
     st->dt = smin(dt_next, dt);
     st->maxinv = 0ull;
@@ -321,6 +328,8 @@ __global__ void u_from_q_kernel(HydroGeom g, double* __restrict__ Q, double* __r
 // fixSolution (Thermodynamics.cpp:478-485, Fluid.cpp:80-136).
 __global__ void apply_rate_fix_kernel(HydroGeom g, double* __restrict__ U, const double* __restrict__ rate, double dtf,
                                       StepState* st) {
+    if (dtf < 0.0 && st->halted) return;
+    dtf = stage_dtf(dtf, st, 0);
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < g.ncell; c += (long long)gridDim.x * blockDim.x) {
         double u[7];
 #pragma unroll
@@ -370,8 +379,9 @@ hipError_t launch_cfl(const HydroGeom& g, const double* U, StepState* st, hipStr
     return hipGetLastError();
 }
 
-hipError_t launch_step_begin(StepState* st, int use_cfl, double dt_override, hipStream_t stream) {
-    hipLaunchKernelGGL(step_begin_kernel, dim3(1), dim3(1), 0, stream, st, use_cfl, dt_override);
+hipError_t launch_step_begin(StepState* st, int use_cfl, double dt_override, hipStream_t stream,
+                             const unsigned long long* thermo_dt_bits) {
+    hipLaunchKernelGGL(step_begin_kernel, dim3(1), dim3(1), 0, stream, st, use_cfl, dt_override, thermo_dt_bits);
     return hipGetLastError();
 }
 

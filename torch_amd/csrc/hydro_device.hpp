@@ -134,6 +134,15 @@ __device__ __forceinline__ double cfl_inv_t_mixed(const HydroGeom& g, const doub
 
 // Records an error of the current step unless an earlier step already
 // failed (the reference stops at its first exception).
+// dt of a stage or sub-step: explicit (>= 0) or from the step in flight
+// (the DT_* codes).
+__device__ __forceinline__ double stage_dtf(double dtf, const StepState* st, int stage) {
+    if (dtf >= 0.0) return dtf;
+    if (dtf == DT_HALF) return 0.5 * st->dt;
+    if (dtf == DT_FULL) return st->dt;
+    return stage == 1 ? st->dt / 2.0 : st->dt;
+}
+
 // Records error `bit` for the current step unless an earlier step failed
 // first.  (Lane aggregation with a ballot and coherent pre-check loads here
 // slowed the mixed stage kernel by half even with no error raised: a failing

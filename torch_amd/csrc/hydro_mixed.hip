@@ -371,7 +371,7 @@ __global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(6))) v
     if (active) {
         // gravity source with GRAV = 0 (Hydro.cpp:134-137) adds nothing
         const long long c = cell;
-        const double dtf = dtf_host >= 0.0 ? dtf_host : (STAGE == 1 ? st->dt / 2.0 : st->dt);
+        const double dtf = stage_dtf(dtf_host, st, STAGE);
         double u[7];
 #pragma unroll
         for (int v = 0; v < 7; ++v) u[v] = Ubase[v * g.ncell + c];

@@ -441,9 +441,11 @@ __global__ void surrogate_targets_kernel(const unsigned long long* __restrict__ 
 // ADV = -1 so that the gate makes it a no-op.
 __global__ void thermo_gather_kernel(HydroGeom g, const double* __restrict__ U, const double* __restrict__ T_min,
                                      double T_min_default, const double* __restrict__ col_den,
-                                     const double* __restrict__ F, double dt, long long excluded,
-                                     double* __restrict__ in9) {
+                                     const double* __restrict__ F, double dt, const StepState* st,
+                                     long long excluded, double* __restrict__ in9) {
     const long long n = g.ncell;
+    // DT_HALF / DT_FULL: the sub-step of the step in flight
+    if (dt < 0.0) dt = (dt == DT_HALF) ? 0.5 * st->dt : st->dt;
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
         double u[7], q[7];
 #pragma unroll
@@ -903,14 +905,15 @@ static int gather(torchgpu_cooling* c, torchgpu_ctx* ctx, const double* T_min, c
     long long nb = (n + 255) / 256;
     if (nb > 65536) nb = 65536;
     hipLaunchKernelGGL(tg::thermo_gather_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, *v.g, v.U, T_min,
-                       100.0, col_den, F, dt, excluded, c->in9);
+                       100.0, col_den, F, dt, v.st, excluded, c->in9);
     HIP_OK(hipGetLastError());
     return TORCHGPU_OK;
 }
 
-int torchgpu_thermo_grid_timestep(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* T_min, const double* col_den,
-                                  const double* F, int64_t excluded, double dt_max, double* dt) {
-    if (!ctx || !c || !dt) return TORCHGPU_ERR_INVALID_ARG;
+// Thermodynamics::calculateTimeStep on the grid into c->dtbits (bit pattern
+// of min |f E/RATE| over the gated cells, 0x7fef... when none), no host sync.
+static int grid_timestep_dev(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* T_min, const double* col_den,
+                             const double* F, int64_t excluded) {
     int rc = gather(c, ctx, T_min, col_den, F, excluded, 0.0);
     if (rc) return rc;
     const long long n = tg::ctx_view(ctx).g->ncell;
@@ -920,6 +923,30 @@ int torchgpu_thermo_grid_timestep(torchgpu_ctx* ctx, torchgpu_cooling* c, const 
     hipLaunchKernelGGL(tg::thermo_timestep_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t, c->tables, n,
                        c->in9, c->dtbits, c->err);
     HIP_OK(hipGetLastError());
+    return TORCHGPU_OK;
+}
+
+// One thermo sub-step on the grid (dt explicit or a DT_* code), no host sync.
+static int grid_substep_dev(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* T_min, const double* col_den,
+                            const double* F, int64_t excluded, double dt, int surrogate) {
+    int rc = gather(c, ctx, T_min, col_den, F, excluded, dt);
+    if (rc) return rc;
+    tg::CtxView v = tg::ctx_view(ctx);
+    const long long n = v.g->ncell;
+    if (surrogate) rc = torchgpu_thermo_cells_surrogate(c, n, c->in9, c->out5, nullptr, nullptr);
+    else rc = torchgpu_thermo_cells(c, n, c->in9, c->out5, nullptr);
+    if (rc) return rc;
+    // Thermodynamics::updateSourceTerms + advSolution(dt) + fixSolution
+    HIP_OK(tg::launch_fix_solution(*v.g, v.U, c->out5 + 2 * n, dt, v.st, c->stream));
+    tg::ctx_state_changed(ctx);
+    return TORCHGPU_OK;
+}
+
+int torchgpu_thermo_grid_timestep(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* T_min, const double* col_den,
+                                  const double* F, int64_t excluded, double dt_max, double* dt) {
+    if (!ctx || !c || !dt) return TORCHGPU_ERR_INVALID_ARG;
+    int rc = grid_timestep_dev(ctx, c, T_min, col_den, F, excluded);
+    if (rc) return rc;
     unsigned long long bits = 0;
     HIP_OK(hipMemcpyAsync(&bits, c->dtbits, sizeof bits, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
@@ -934,16 +961,38 @@ int torchgpu_thermo_grid_substep(torchgpu_ctx* ctx, torchgpu_cooling* c, const d
                                  const double* F, int64_t excluded, double dt, int surrogate) {
     if (!ctx || !c || !(dt >= 0.0)) return TORCHGPU_ERR_INVALID_ARG;
     if (surrogate && !c->has_surrogate) return TORCHGPU_ERR_NOT_TRAINED;
-    int rc = gather(c, ctx, T_min, col_den, F, excluded, dt);
-    if (rc) return rc;
+    return grid_substep_dev(ctx, c, T_min, col_den, F, excluded, dt, surrogate);
+}
+
+int torchgpu_run_split(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* T_min, const double* col_den,
+                       const double* F, int64_t excluded, int nsteps, int surrogate) {
+    if (!ctx || !c || nsteps < 0) return TORCHGPU_ERR_INVALID_ARG;
+    if (surrogate && !c->has_surrogate) return TORCHGPU_ERR_NOT_TRAINED;
     tg::CtxView v = tg::ctx_view(ctx);
-    const long long n = v.g->ncell;
-    if (surrogate) rc = torchgpu_thermo_cells_surrogate(c, n, c->in9, c->out5, nullptr, nullptr);
-    else rc = torchgpu_thermo_cells(c, n, c->in9, c->out5, nullptr);
-    if (rc) return rc;
-    // Thermodynamics::updateSourceTerms + advSolution(dt) + fixSolution
-    HIP_OK(tg::launch_fix_solution(*v.g, v.U, c->out5 + 2 * n, dt, v.st, c->stream));
-    tg::ctx_state_changed(ctx);
+    if (v.stream != c->stream) return TORCHGPU_ERR_INVALID_ARG;  // one stream orders the two handles' work
+    int rc;
+    for (int s = 0; s < nsteps; ++s) {
+        v = tg::ctx_view(ctx);  // U and U* swap roles after every hydro sub-step
+        // Torch::calculateTimeStep (Torch.cpp:304-339) on the device: the
+        // hydro CFL maximum, the thermo minimum, then dt in step_begin
+        HIP_OK(tg::launch_cfl(*v.g, v.U, v.st, v.stream));
+        if ((rc = grid_timestep_dev(ctx, c, T_min, col_den, F, excluded))) return rc;
+        HIP_OK(tg::launch_step_begin(v.st, 1, 0.0, v.stream, c->dtbits));
+        // Torch::fullStep's Strang alternation over (hydro, thermo),
+        // Torch.cpp:406-415: T(dt/2) H(dt) T(dt/2), then H(dt/2) T(dt) H(dt/2)
+        int& counter = tg::ctx_step_counter(ctx);
+        counter = (counter + 1) % 2;
+        for (int k = 0; k < 3; ++k) {
+            const bool hydro = ((k == 1 ? 1 : 0) + counter) % 2 == 0;
+            const double code = (k == 1) ? tg::DT_FULL : tg::DT_HALF;
+            if (hydro) {
+                if ((rc = tg::ctx_run_stage(ctx, 0, code))) return rc;
+            } else if ((rc = grid_substep_dev(ctx, c, T_min, col_den, F, excluded, code, surrogate))) {
+                return rc;
+            }
+        }
+        HIP_OK(tg::launch_step_end(tg::ctx_view(ctx).st, v.stream));
+    }
     return TORCHGPU_OK;
 }
 

@@ -52,6 +52,14 @@ struct StepState {
     unsigned int qmax_next[2];
 };
 
+// Negative dt arguments of the stage and cooling kernels take the dt of the
+// step in flight from the device (StepState::dt), so that a time loop runs
+// without host round trips: DT_STEP_RULE = dt/2 for a predictor, dt for a
+// corrector (Torch::hydroStep); DT_HALF / DT_FULL = 0.5 dt / dt (the
+// sub-steps of Torch::fullStep).  The kernels given a code skip their work
+// once the run has halted.
+constexpr double DT_STEP_RULE = -1.0, DT_HALF = -2.0, DT_FULL = -3.0;
+
 // Error bits in the order the reference would meet them inside one step;
 // the lowest set bit of the first failing step is the error reported.
 enum : unsigned int {
@@ -99,8 +107,16 @@ hipError_t launch_hydro_stage_mixed(const HydroGeom& g, int solver, int stage, c
 // The grid state was changed by another operator (the float primitives of
 // the mixed path are recomputed before the next stage).
 void ctx_state_changed(torchgpu_ctx* c);
+// One hydro stage on the context's buffers (api.hip): mode 1 predictor,
+// 2 corrector, 0 single-stage sub-step; dtf as above.
+int ctx_run_stage(torchgpu_ctx* c, int mode, double dtf);
+// The context's Strang alternation counter (Torch::stepCounter).
+int& ctx_step_counter(torchgpu_ctx* c);
 hipError_t launch_cfl(const HydroGeom& g, const double* U, StepState* st, hipStream_t stream);
-hipError_t launch_step_begin(StepState* st, int use_cfl, double dt_override, hipStream_t stream);
+// thermo_dt_bits: the bit pattern of the cooling operator's minimum
+// |E/RATE| (a Torch::fullStep with thermodynamics), or null (hydro only).
+hipError_t launch_step_begin(StepState* st, int use_cfl, double dt_override, hipStream_t stream,
+                             const unsigned long long* thermo_dt_bits = nullptr);
 hipError_t launch_step_end(StepState* st, hipStream_t stream);
 hipError_t launch_q_from_u(const HydroGeom& g, const double* U, double* Q, int fix, hipStream_t stream);
 hipError_t launch_u_from_q(const HydroGeom& g, double* Q, double* U, hipStream_t stream);
