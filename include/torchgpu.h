@@ -28,6 +28,12 @@
  *     (Thermodynamics.cpp:463-485, Torch.cpp:352-364)
  *   Torch::run / fullStep / calculateTimeStep with        torchgpu_run_split
  *     hydrodynamics and thermodynamics (Torch.cpp:240-418)
+ *   Torch::run's Checkpointer test and the state it       torchgpu_pause_at_checkpoints /
+ *     prints (Torch.cpp:268-277, Checkpointer.cpp:14-40)     torchgpu_checkpoint /
+ *                                                            torchgpu_get_print_state
+ *   Torch::initialise start time from initial conditions  torchgpu_set_time
+ *     (Torch.cpp:100-103)
+ *   DataPrinter::print2D, DataReader (src/IO/)            torchgpu_snapshot_*
  *   MLP surrogate of the subcycled update (no reference   torchgpu_mlp_*,
  *     counterpart; FP64 integrate remains the fallback)     torchgpu_thermo_cells_surrogate
  *
@@ -161,7 +167,49 @@ int torchgpu_reset_clock(torchgpu_ctx* ctx);
 int torchgpu_synchronize(torchgpu_ctx* ctx);
 /* Simulation time, dt of the last step and number of completed steps. */
 int torchgpu_get_clock(torchgpu_ctx* ctx, double* time, double* last_dt, int* steps);
+/* Torch::initialise's start time from an initial-conditions snapshot
+ * (src/Torch/Torch.cpp:100-103, code units; t >= 0).  Synchronises. */
+int torchgpu_set_time(torchgpu_ctx* ctx, double t);
+/* With `on`, torchgpu_run and torchgpu_run_split stop at the end of a step
+ * that reaches a checkpoint (the Checkpointer test of Torch::run,
+ * src/Torch/Torch.cpp:268-277, src/IO/Checkpointer.cpp:14-40): later steps
+ * of the same call do nothing until torchgpu_checkpoint resumes the run.
+ * Also keeps the state Torch::run prints at that point.  Synchronises. */
+int torchgpu_pause_at_checkpoints(torchgpu_ctx* ctx, int on);
+/* The Checkpointer's count and whether the run is paused at a checkpoint
+ * (`due`); a paused run is resumed.  Synchronises. */
+int torchgpu_checkpoint(torchgpu_ctx* ctx, int* count, int* due);
+/* Fluid::Q as Torch::run prints it (Torch.cpp:271-276, DataPrinter::print2D):
+ * after a hydro-only step Q(U*) without fixPrimitives (Torch.cpp:366-387);
+ * after a full step with thermodynamics fixPrimitives(Q(input of the last
+ * substep)) (Torch.cpp:341-364), which needs pause_at_checkpoints on; before
+ * any step fixPrimitives(Q(U)).  Q is [7][nz][ny][nx] code units, host
+ * unless `on_device`.  Synchronises. */
+int torchgpu_get_print_state(torchgpu_ctx* ctx, double* Q, int on_device);
 
+
+/* ------------------------------------------------------------------------ */
+/* Snapshots in the reference's data2D text format (host-only)                */
+/* ------------------------------------------------------------------------ */
+/* DataPrinter::print2D (src/IO/DataPrinter.cpp:139-174): time t and the
+ * primitive state Q (host [7][nz][ny][nx], code units) written as text --
+ * t (s), nx, ny, nz, then per cell (x fastest) the nd cell-centre
+ * coordinates (cm), density, pressure, HII fraction and nd velocities (cgs),
+ * tab separated, "%.10e" -- gzip-compressed when `gz` (as the reference's
+ * .txt.gz).  nd = 1 if ny == 1, 2 if nz == 1, else 3; dx = side_length / nx
+ * (code units); scales as torchgpu_units. */
+int torchgpu_snapshot_write(const char* path, int gz, double t, const double* Q, int nx, int ny, int nz,
+                            double side_length, double density_scale, double pressure_scale, double time_scale);
+/* DataReader::readDataParameters (src/IO/DataReader.cpp:17-68): time (s),
+ * cell counts, nd and side length (cm: dx * nx, dx from the first two
+ * cells' coordinates) of a snapshot, plain or gzip text. */
+int torchgpu_snapshot_header(const char* path, double* t, int* nx, int* ny, int* nz, int* nd, double* side_length);
+/* DataReader::readGrid (src/IO/DataReader.cpp:70-125): density, pressure,
+ * HII fraction and the nd velocities (cgs, as stored) of the first
+ * nx*ny*nz cells listed, each placed by its coordinates / dx into Q (host
+ * [7][nz][ny][nx]); cells outside the grid are skipped, other entries of Q
+ * are left as they are. */
+int torchgpu_snapshot_read(const char* path, double* Q, int nx, int ny, int nz);
 
 /* ------------------------------------------------------------------------ */
 /* Thermodynamics (cooling) operator                                         */

@@ -89,6 +89,10 @@ def lib() -> ctypes.CDLL:
         L.orc_hydro_step.restype = ctypes.c_int
         L.orc_run_hydro.argtypes = [P(HydroParams), dp, ctypes.c_int, d, ctypes.c_int, dp, dp]
         L.orc_run_hydro.restype = ctypes.c_int
+        L.orc_run_hydro_print.argtypes = [P(HydroParams), dp, ctypes.c_int, d, d, ctypes.c_int, dp, dp, dp,
+                                          ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_int), dp, ctypes.POINTER(ctypes.c_int)]
+        L.orc_run_hydro_print.restype = ctypes.c_int
         L.orc_UfromQ.argtypes = [dp, dp, d]
         L.orc_QfromU.argtypes = [dp, dp, d]
         L.orc_FfromQ.argtypes = [dp, dp, d, ctypes.c_int]
@@ -192,6 +196,25 @@ def run_hydro(p: HydroParams, U: np.ndarray, nsteps: int, tmax: float = 1e3, nch
     if err:
         raise RuntimeError(f"hydro run failed (code {err})")
     return U, dts, t.value
+
+
+def run_hydro_print(p: HydroParams, U: np.ndarray, nsteps: int, tmax: float, ncheckpoints: int, t0: float = 0.0):
+    """run_hydro from time t0 with the snapshots Torch::run prints
+    (Torch.cpp:240-302): returns (U, dts, time, [(count, time, Q), ...])."""
+    U = np.ascontiguousarray(U, dtype=np.float64).copy()
+    dts = np.zeros(nsteps)
+    t = ctypes.c_double(0)
+    maxp = ncheckpoints + 2
+    qp = np.zeros((maxp,) + U.shape)
+    pc = (ctypes.c_int * maxp)()
+    pt = np.zeros(maxp)
+    npr = ctypes.c_int(0)
+    err = lib().orc_run_hydro_print(ctypes.byref(p), _dp(U), nsteps, t0, tmax, ncheckpoints, _dp(dts), ctypes.byref(t),
+                                    _dp(qp), maxp, pc, _dp(pt), ctypes.byref(npr))
+    if err:
+        raise RuntimeError(f"hydro run failed (code {err})")
+    prints = [(pc[i], pt[i], qp[i]) for i in range(npr.value)]
+    return U, dts, t.value, prints
 
 
 def riemann(solver: str, Ql: np.ndarray, Qr: np.ndarray, dims: np.ndarray, gamma: float):

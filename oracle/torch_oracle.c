@@ -556,18 +556,44 @@ static double checkpoint_dt(double t, double tmax, int ncheck, int* count) {
 }
 
 /* Torch::run / fullStep / calculateTimeStep for a hydro-only run,
- * Torch.cpp:240-302, 304-339, 389-404. */
-int orc_run_hydro(const orc_hydro_params* p, double* U, int nsteps, double tmax, int ncheckpoints,
-                  double* dts, double* time_out) {
+ * Torch.cpp:240-302, 304-339, 389-404, with the snapshots Torch::run prints
+ * (DataPrinter::print2D of Fluid's Q, Torch.cpp:254, 270-281, 295-297) when
+ * qprint is given: the initial fixPrimitives(QfromU(U)); at the start of
+ * every step for which Checkpointer::update reports a new checkpoint, the Q
+ * the previous step left (QfromU(U*) of its corrector); after the loop, if
+ * the last checkpoint print was not the final one and tmax was reached.
+ * Each print stores Q (7n), the checkpoint count and the time. */
+static int run_hydro_impl(const orc_hydro_params* p, double* U, int nsteps, double t0, double tmax,
+                          int ncheckpoints, double* dts, double* time_out, double* qprint, int max_prints,
+                          int* pcount, double* ptime, int* nprint) {
     const int64_t n = ncells(p);
     double* Q = malloc(sizeof(double) * 7 * n);
     double* W = malloc(sizeof(double) * 7 * n);
     double* UDOT = calloc(7 * n, sizeof(double));
-    double t = 0;
-    int count = 0, err = 0;
+    double t = t0;
+    int count = 0, err = 0, np = 0, final_print = 0;
     (void)checkpoint_dt(t, tmax, ncheckpoints, &count);
+#define PRINT_Q(src, cnt)                                                       \
+    do {                                                                      \
+        if (qprint && np < max_prints) {                                      \
+            memcpy(qprint + (size_t)np * 7 * n, (src), sizeof(double) * 7 * n); \
+            pcount[np] = (cnt);                                               \
+            ptime[np] = t;                                                    \
+            ++np;                                                             \
+        }                                                                     \
+    } while (0)
+    if (qprint) {
+        orc_globalQfromU(p, U, Q);
+        orc_fixPrimitives(p, Q, U);
+        PRINT_Q(Q, count);
+    }
     for (int s = 0; s < nsteps && t < tmax; ++s) {
+        const int before = count;
         double dt_next = checkpoint_dt(t, tmax, ncheckpoints, &count);
+        if (count != before) {
+            PRINT_Q(Q, count);
+            final_print = (count != ncheckpoints);
+        }
         orc_globalQfromU(p, U, Q);
         orc_fixPrimitives(p, Q, U);
         /* Torch::checkValues("hydro before"), Torch.cpp:420-443 */
@@ -585,9 +611,24 @@ int orc_run_hydro(const orc_hydro_params* p, double* U, int nsteps, double tmax,
         t += dt;
         if (err) break;
     }
+    if (final_print && !(t < tmax)) PRINT_Q(Q, ncheckpoints);
+#undef PRINT_Q
     if (time_out) *time_out = t;
+    if (nprint) *nprint = np;
     free(Q); free(W); free(UDOT);
     return err;
+}
+
+int orc_run_hydro(const orc_hydro_params* p, double* U, int nsteps, double tmax, int ncheckpoints,
+                  double* dts, double* time_out) {
+    return run_hydro_impl(p, U, nsteps, 0.0, tmax, ncheckpoints, dts, time_out, NULL, 0, NULL, NULL, NULL);
+}
+
+int orc_run_hydro_print(const orc_hydro_params* p, double* U, int nsteps, double t0, double tmax,
+                        int ncheckpoints, double* dts, double* time_out, double* qprint, int max_prints, int* pcount,
+                        double* ptime, int* nprint) {
+    return run_hydro_impl(p, U, nsteps, t0, tmax, ncheckpoints, dts, time_out, qprint, max_prints, pcount, ptime,
+                          nprint);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -91,6 +91,13 @@ int orc_hydro_step(const orc_hydro_params* p, double* U, double* Q, double* W,
  * Returns 0 or an error code (1 NaN flux, 2 fixSolution failure, 3 checkValues). */
 int orc_run_hydro(const orc_hydro_params* p, double* U, int nsteps, double tmax,
                   int ncheckpoints, double* dts, double* time_out);
+/* The same from the start time t0 (the time of an initial-conditions
+ * snapshot, Torch.cpp:58), with the snapshots Torch::run prints
+ * (Torch.cpp:254,270-281,295-297): up to max_prints states Q (7n each) with
+ * their checkpoint count and time; *nprint receives how many. */
+int orc_run_hydro_print(const orc_hydro_params* p, double* U, int nsteps, double t0, double tmax,
+                        int ncheckpoints, double* dts, double* time_out, double* qprint, int max_prints,
+                        int* pcount, double* ptime, int* nprint);
 
 /* ---- units (reference Converter.cpp / Constants.cpp) ------------------- */
 typedef struct { double M, L, T; } orc_converter;

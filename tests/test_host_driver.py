@@ -104,3 +104,160 @@ def test_driver_sod_bitwise(tmp_path, orc):
     Uo, dts, t = orc.run_hydro(p, U0, 6)
     np.testing.assert_array_equal(Ug, Uo)
     assert info["time"] == t
+
+
+def _sod16_lua(tmp_path, name, tmax, nck, outdir):
+    cfg = open(os.path.join(ROOT, "torch_amd", "configs", "sod64.lua")).read()
+    for k in ("x", "y", "z"):
+        cfg = cfg.replace(f"no_cells_{k} = 64", f"no_cells_{k} = 16")
+    cfg = cfg.replace("simulation_time = 1.0e3,", f"simulation_time = {tmax!r}, output_directory = \"{outdir}/\",")
+    cfg = cfg.replace("ncheckpoints =    1,", f"ncheckpoints = {nck},")
+    f = tmp_path / name
+    f.write_text(cfg)
+    return f
+
+
+def _gunzip(path):
+    import gzip
+
+    with gzip.open(path, "rb") as fh:
+        return fh.read()
+
+
+@pytest.mark.gpu
+def test_driver_snapshots_and_restart(tmp_path, orc, gpu):
+    """--snapshots: the data2D files Torch::run prints (Torch.cpp:240-302) for
+    16^3 Sod, tmax 0.1, 4 checkpoints, against the oracle's loop written by
+    the same text writer (identical decompressed text).  Then a run started
+    from the checkpoint-2 file as Integration.initial_conditions (grid shape,
+    side length, state and time from the file, Torch.cpp:38-58,100-103)
+    against the oracle started from the same values: same files, same final
+    conserved state (bitwise)."""
+    _exe_or_skip()
+    from torch_amd import setups
+
+    n, tmax, nck = 16, 0.1, 4
+    f = _sod16_lua(tmp_path, "sod16.lua", tmax, nck, tmp_path / "out")
+    r = subprocess.run([EXE, f"--paramfile={f}", "--setup=sod", "--steps=1000", "--snapshots"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    Q = setups.sod(n)
+    po = orc.make_params(Q.shape[1:], 1.0 / n, dt_max=tmax / 10.0, nthreads=4)
+    U0 = orc.UfromQ_grid(orc.fix_primitives(po, Q, np.zeros_like(Q)), po.gamma)
+    Uo, dts, t, prints = orc.run_hydro_print(po, U0, 1000, tmax, nck)
+    names = sorted(os.listdir(tmp_path / "out"))
+    assert names == [f"data2D_{c:06d}.txt.gz" for c, _, _ in prints] and len(names) == 5
+    for c, tp, Qp in prints:
+        ref = str(tmp_path / f"oracle_{c}.txt.gz")
+        gpu.snapshot_write(ref, tp, Qp, 1.0)
+        assert _gunzip(tmp_path / "out" / f"data2D_{c:06d}.txt.gz") == _gunzip(ref), c
+
+    # restart from checkpoint 2
+    start = str(tmp_path / "out" / "data2D_000002.txt.gz")
+    f2 = _sod16_lua(tmp_path, "restart.lua", tmax, nck, tmp_path / "out2")
+    out = tmp_path / "U.bin"
+    r = subprocess.run([EXE, f"--paramfile={f2}", f"--setup={start}", "--steps=1000", "--snapshots",
+                        f"--output={out}"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    Qs, h = gpu.snapshot_read(start)
+    assert h["side_length"] == 1.0 and h["shape"] == (n, n, n)
+    U1 = orc.UfromQ_grid(orc.fix_primitives(po, Qs, np.zeros_like(Qs)), po.gamma)
+    Uo2, dts2, t2, prints2 = orc.run_hydro_print(po, U1, 1000, tmax, nck, t0=h["time"])
+    assert [c for c, _, _ in prints2] == [2, 3, 4]
+    assert sorted(os.listdir(tmp_path / "out2")) == [f"data2D_{c:06d}.txt.gz" for c, _, _ in prints2]
+    for c, tp, Qp in prints2:
+        ref = str(tmp_path / f"oracle2_{c}.txt.gz")
+        gpu.snapshot_write(ref, tp, Qp, 1.0)
+        assert _gunzip(tmp_path / "out2" / f"data2D_{c:06d}.txt.gz") == _gunzip(ref), c
+    Ug = np.fromfile(out, dtype=np.float64).reshape(7, n, n, n)
+    np.testing.assert_array_equal(Ug, Uo2)
+    assert info["time"] == t2 and info["steps"] == int((dts2 > 0).sum())
+
+
+COOL_LUA = """
+Parameters = {
+    Integration = {
+        density_scale = 1.0e-20, pressure_scale = 1.0e-28, time_scale = 1.0e11,
+        spatial_order = 1, temporal_order = 2, simulation_time = 1.0e13,
+        radiation_on = false, cooling_on = true, ncheckpoints = 10,
+    },
+    Grid = {
+        no_dimensions = 3, no_cells_x = 16, no_cells_y = 16, no_cells_z = 16,
+        geometry = "cartesian", side_length = 1.54285e18,
+        left_boundary_condition_x = "outflow", right_boundary_condition_x = "outflow",
+        left_boundary_condition_y = "periodic", right_boundary_condition_y = "periodic",
+        left_boundary_condition_z = "reflecting", right_boundary_condition_z = "reflecting",
+    },
+    Hydrodynamics = {
+        gamma = 1.67, density_floor = 1.0e-30, temperature_floor = 0.1,
+        riemann_solver = "RotatedHLLC", slope_limiter = "albada",
+    },
+    Radiation = { mass_fraction_hydrogen = 1.0 },
+    Thermodynamics = { thermo_hii_switch = 1.0e-2, heating_amplification = 1.0 },
+}
+"""
+
+
+@pytest.mark.gpu
+def test_driver_cooling_matches_device_loop(tmp_path, gpu):
+    """A 16^3 run with cooling_on through the driver's Torch::run (the
+    host-driven Torch::fullStep sequence of the integrators) and, with
+    --snapshots, through the device loop, against the library's device loop
+    (torchgpu_run_split) driven from Python with the parameters the driver
+    reports: identical conserved states.  Set-up: a warm (3000 K), partly
+    ionised, turbulent medium of 1e-22 g/cm^3 with ADV = 1, so that both
+    operators act."""
+    _exe_or_skip()
+    from torch_amd import setups
+
+    n, steps = 16, 6
+    f = tmp_path / "cool.lua"
+    f.write_text(COOL_LUA)
+    rng = np.random.default_rng(7)
+    Qt = setups.turbulent(n, 0.5, 1.0)
+    rho = 1.0e-22 * Qt[0]
+    hii = rng.uniform(0.0, 1.0, rho.shape)
+    Q = np.zeros((7, n, n, n))
+    Q[0] = rho
+    Q[1] = rho * 8.314462e7 * (1.0 + hii) * 3000.0
+    Q[2] = hii
+    Q[3] = 1.0
+    Q[4:7] = Qt[4:7] / np.abs(Qt[4:7]).max() * 3.0e5
+    setup = tmp_path / "Q.bin"
+    Q.tofile(setup)
+    d = json.loads(subprocess.run([EXE, f"--paramfile={f}", "--dump-params"], capture_output=True, text=True,
+                                  timeout=60).stdout)
+    res = {}
+    for mode in ("host", "device"):
+        out = tmp_path / f"U_{mode}.bin"
+        cmd = [EXE, f"--paramfile={f}", f"--setup={setup}", f"--steps={steps}", f"--output={out}"]
+        if mode == "device":
+            cmd.append("--snapshots")
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=120, cwd=tmp_path)
+        assert r.returncode == 0, r.stderr
+        res[mode] = np.fromfile(out, dtype=np.float64).reshape(7, n, n, n)
+        assert json.loads(r.stdout.strip().splitlines()[-1])["steps"] == steps
+    U0out = tmp_path / "U0.bin"
+    r = subprocess.run([EXE, f"--paramfile={f}", f"--setup={setup}", "--steps=0", f"--output={U0out}"],
+                       capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    U0 = np.fromfile(U0out, dtype=np.float64).reshape(7, n, n, n)
+    cu = d["code_units"]
+    p = gpu.make_params((n, n, n), side_length=cu["side_length"], gamma=d["gamma"], dfloor=cu["dfloor"],
+                        pfloor=cu["pfloor"], tfloor=d["tfloor"], spec_gas_const=cu["spec_gas_const"],
+                        tmax=cu["tmax"], ncheckpoints=d["ncheckpoints"],
+                        bc=["outflow", "outflow", "periodic", "periodic", "reflecting", "reflecting"])
+    g = gpu.Grid(p)
+    g.set_conserved(U0)
+    tp = gpu.thermo_params(scales=tuple(d["scales"]), thermo_hii_switch=d["thermo_hii_switch"],
+                           mass_fraction_h=d["mass_fraction_h"], gamma=d["gamma"], density_floor=d["dfloor"],
+                           temperature_floor=d["tfloor"])
+    tp.pfloor = cu["pfloor"]  # the driver hands the grid's pressure floor to the thermo operator
+    cool = gpu.Cooling(tp)
+    cool.run_split(g, steps)
+    g.synchronize()
+    Upy = g.get_conserved()
+    assert not np.array_equal(Upy[1], U0[1])
+    np.testing.assert_array_equal(res["device"], Upy)
+    np.testing.assert_array_equal(res["host"], Upy)

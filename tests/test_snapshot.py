@@ -1,0 +1,80 @@
+"""The data2D snapshot format (DataPrinter::print2D, DataReader) against the
+reference's own snapshot config/data2D_025.txt (tests/golden/data2D_025.txt.gz,
+written by a 2-D run of the reference: 150 x 200 cells).  Host code only."""
+import gzip
+import os
+
+import numpy as np
+import pytest
+
+import torch_amd as ta
+
+FIX = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "data2D_025.txt.gz")
+SCALES = (1.0e-20, 1.0e-28, 1.0e11)
+
+
+def lines(path):
+    opener = gzip.open if path.endswith(".gz") else open
+    with opener(path, "rt") as fh:
+        return fh.read().split("\n")
+
+
+def test_header():
+    h = ta.snapshot_header(FIX)
+    assert h["time"] == 1.577845e12 and h["shape"] == (1, 200, 150) and h["nd"] == 2
+    assert h["side_length"] == pytest.approx(150 * 1.03e16, rel=1e-12)
+
+
+def test_reference_snapshot_round_trip_is_byte_exact(tmp_path):
+    """Read in cgs, written with unit scales 1: every line of the reference's
+    file comes back byte for byte (the file lists cells rank by rank of an
+    x-slab MPI run, the writer x fastest over the whole grid, so the lines
+    are compared as a set; the header in order)."""
+    Q, h = ta.snapshot_read(FIX)
+    out = str(tmp_path / "out.txt")
+    ta.snapshot_write(out, h["time"], Q, h["side_length"])
+    a, b = lines(FIX), lines(out)
+    assert a[:4] == b[:4]
+    assert sorted(a[4:]) == sorted(b[4:])
+    # x fastest: the second line is the cell right of the first
+    assert b[5].split("\t")[0] == "1.5450000000e+16" and b[5].split("\t")[1] == "5.1500000000e+15"
+
+
+def test_code_units_round_trip(tmp_path):
+    """Through the reference's unit scalings (to code units and back, as a
+    run reads and prints), gzip-compressed: every value within 2 units in
+    the last printed digit."""
+    Q, h = ta.snapshot_read(FIX)
+    q = Q.copy()
+    tc = lambda v, m, l, t: ta.units(SCALES, v, m, l, t)  # noqa: E731
+    q[0] *= tc(1.0, 1, -3, 0)
+    q[1] *= tc(1.0, 1, -1, -2)
+    q[4:7] *= tc(1.0, 0, 1, -1)
+    out = str(tmp_path / "out.txt.gz")
+    ta.snapshot_write(out, tc(h["time"], 0, 0, 1), q, tc(h["side_length"], 0, 1, 0), scales=SCALES)
+    Q2, h2 = ta.snapshot_read(out)
+    assert h2["shape"] == h["shape"] and h2["time"] == pytest.approx(h["time"], rel=1e-10)
+    for v in (0, 1, 2, 4, 5):
+        np.testing.assert_allclose(Q2[v], Q[v], rtol=2e-10, atol=0)
+
+
+def test_three_dimensional_state(tmp_path):
+    """A 3-D state: cell centres (i + 1/2) dx, the velocity columns of all
+    three axes, and reading places every cell by its coordinates."""
+    rng = np.random.default_rng(5)
+    Q = np.zeros((7, 3, 4, 5))
+    Q[0] = rng.uniform(0.1, 2.0, Q.shape[1:])
+    Q[1] = rng.uniform(0.1, 2.0, Q.shape[1:])
+    Q[2] = rng.uniform(0.0, 1.0, Q.shape[1:])
+    Q[4:7] = rng.normal(size=(3,) + Q.shape[1:])
+    out = str(tmp_path / "s.txt")
+    ta.snapshot_write(out, 0.25, Q, 1.0)
+    ls = lines(out)
+    assert ls[:4] == ["2.5000000000e-01", "5", "4", "3"]
+    first = ls[4].split("\t")
+    assert len(first) == 3 + 3 + 3 and first[:3] == ["1.0000000000e-01"] * 3
+    Q2, h = ta.snapshot_read(out)
+    assert h["nd"] == 3 and h["side_length"] == pytest.approx(1.0, rel=1e-12)
+    for v in (0, 1, 2, 4, 5, 6):
+        np.testing.assert_allclose(Q2[v], Q[v], rtol=1e-10, atol=1e-10)
+    assert not Q2[3].any()  # ADV is not stored (lossy resume, as the reference)

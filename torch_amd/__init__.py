@@ -67,6 +67,10 @@ def lib() -> ctypes.CDLL:
         L.torchgpu_destroy.restype = None
         L.torchgpu_last_error.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
         L.torchgpu_state_ptr.argtypes = [vp, ctypes.POINTER(vp)]
+        L.torchgpu_set_time.argtypes = [vp, d]
+        L.torchgpu_pause_at_checkpoints.argtypes = [vp, i]
+        L.torchgpu_checkpoint.argtypes = [vp, ctypes.POINTER(i), ctypes.POINTER(i)]
+        L.torchgpu_get_print_state.argtypes = [vp, vp, i]
         for name in ("torchgpu_set_primitive", "torchgpu_set_conserved", "torchgpu_get_conserved"):
             getattr(L, name).argtypes = [vp, vp, i]
         L.torchgpu_get_primitive.argtypes = [vp, vp, i, i]
@@ -209,6 +213,28 @@ class Grid:
     def reset_clock(self):
         self._check(lib().torchgpu_reset_clock(self._h))
 
+    def set_time(self, t: float):
+        """Simulation time (code units) and its checkpoint count, as
+        Torch::initialise sets them from a snapshot."""
+        self._check(lib().torchgpu_set_time(self._h, t))
+
+    def pause_at_checkpoints(self, on: bool = True):
+        self._check(lib().torchgpu_pause_at_checkpoints(self._h, int(on)))
+
+    def checkpoint(self):
+        """(checkpoint count, due): due when a run paused at a new
+        checkpoint (the pause is lifted by this call)."""
+        count, due = ctypes.c_int(), ctypes.c_int()
+        self._check(lib().torchgpu_checkpoint(self._h, ctypes.byref(count), ctypes.byref(due)))
+        return count.value, bool(due.value)
+
+    def print_state(self, out=None):
+        """The primitive state Torch::run prints at this point (see
+        torchgpu_get_print_state)."""
+        out = np.empty((7, *self.shape)) if out is None else out
+        self._check(lib().torchgpu_get_print_state(self._h, ctypes.c_void_p(_ptr(out)), self._is_dev(out)))
+        return out
+
     def synchronize(self):
         self._check(lib().torchgpu_synchronize(self._h))
 
@@ -253,6 +279,110 @@ def units(scales, value, m, l, t, to_code=True) -> float:
     if rc:
         raise TorchGpuError(rc, "units")
     return out.value
+
+
+# -- snapshots (DataPrinter::print2D / DataReader text format) ----------------
+
+def _bind_snapshot(L):
+    if getattr(L, "_snapshot_bound", False):
+        return
+    d, i, vp, cp = ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_char_p
+    L.torchgpu_snapshot_write.argtypes = [cp, i, d, vp, i, i, i, d, d, d, d]
+    L.torchgpu_snapshot_header.argtypes = [cp] + [ctypes.POINTER(d)] + [ctypes.POINTER(i)] * 4 + [ctypes.POINTER(d)]
+    L.torchgpu_snapshot_read.argtypes = [cp, vp, i, i, i]
+    L._snapshot_bound = True
+
+
+def snapshot_write(path: str, t: float, Q, side_length: float, scales=(1.0, 1.0, 1.0), gz=None):
+    """Writes time t and the primitive state Q (host float64 [7][nz][ny][nx],
+    code units) in the reference's data2D text format (gzip when `gz`, by
+    default when `path` ends in .gz)."""
+    import numpy as np
+
+    L = lib()
+    _bind_snapshot(L)
+    Q = np.ascontiguousarray(Q, dtype=np.float64)
+    nz, ny, nx = Q.shape[1:]
+    gz = path.endswith(".gz") if gz is None else gz
+    rc = L.torchgpu_snapshot_write(path.encode(), int(gz), t, Q.ctypes.data, nx, ny, nz, side_length, *scales)
+    if rc:
+        raise TorchGpuError(rc, f"snapshot_write: cannot write {path}")
+
+
+def snapshot_header(path: str) -> dict:
+    """Time (s), shape (nz, ny, nx), nd and side length (cm) of a snapshot."""
+    L = lib()
+    _bind_snapshot(L)
+    t, side = ctypes.c_double(), ctypes.c_double()
+    nx, ny, nz, nd = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    rc = L.torchgpu_snapshot_header(path.encode(), ctypes.byref(t), ctypes.byref(nx), ctypes.byref(ny),
+                                    ctypes.byref(nz), ctypes.byref(nd), ctypes.byref(side))
+    if rc:
+        raise TorchGpuError(rc, f"snapshot_header: invalid input file {path}")
+    return {"time": t.value, "shape": (nz.value, ny.value, nx.value), "nd": nd.value, "side_length": side.value}
+
+
+def snapshot_read(path: str, shape=None):
+    """(Q, header): density, pressure, HII fraction and velocities of the
+    snapshot's cells (cgs, as stored) in a zero-initialised [7][nz][ny][nx]
+    array of `shape` (default: the snapshot's own)."""
+    import numpy as np
+
+    L = lib()
+    _bind_snapshot(L)
+    h = snapshot_header(path)
+    nz, ny, nx = shape or h["shape"]
+    Q = np.zeros((7, nz, ny, nx))
+    rc = L.torchgpu_snapshot_read(path.encode(), Q.ctypes.data, nx, ny, nz)
+    if rc:
+        raise TorchGpuError(rc, f"snapshot_read: invalid input file {path}")
+    return Q, h
+
+
+def run_printing(grid: "Grid", nsteps: int, outdir: str, scales=(1.0, 1.0, 1.0), cooling=None, thermo=None,
+                 surrogate: bool = False):
+    """Torch::run with its snapshots (Torch.cpp:240-302): prints
+    data2D_<count>.txt.gz (6-digit count) into `outdir` for the initial
+    state, at every checkpoint (the run pauses on the device there) and at
+    the end when tmax is reached; at most `nsteps` steps (hydro only, or
+    operator-split with `cooling` and its `thermo` keyword arguments).
+    Returns the list of files written."""
+    import os
+
+    p = grid.params
+    os.makedirs(outdir, exist_ok=True)
+    written = []
+
+    def emit(count):
+        path = os.path.join(outdir, f"data2D_{count:06d}.txt.gz")
+        t, _, _ = grid.clock()
+        snapshot_write(path, t, grid.print_state(), p.side_length, scales)
+        written.append(path)
+
+    count, _ = grid.checkpoint()
+    emit(count)
+    grid.pause_at_checkpoints(True)
+    final_print = False
+    done = 0
+    try:
+        while done < nsteps:
+            n0 = grid.clock()[2]
+            if cooling is None:
+                grid.run(nsteps - done)
+            else:
+                cooling.run_split(grid, nsteps - done, surrogate=surrogate, **(thermo or {}))
+            grid.synchronize()
+            done += grid.clock()[2] - n0
+            count, due = grid.checkpoint()
+            if not due:
+                break
+            emit(count)
+            final_print = count != p.ncheckpoints
+        if final_print and not grid.clock()[0] < p.tmax:
+            emit(p.ncheckpoints)
+    finally:
+        grid.pause_at_checkpoints(False)
+    return written
 
 
 def thermo_params(scales=(1e-20, 1e-28, 1e11), thermo_hii_switch=1e-2, heating_amplification=1.0,

@@ -20,7 +20,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtorchgpu.so")
-SOURCES = ["hydro.hip", "hydro_mixed.hip", "thermo.hip", "mlp.hip", "api.hip", "units.cpp", "mlp_api.cpp"]
+SOURCES = ["hydro.hip", "hydro_mixed.hip", "thermo.hip", "mlp.hip", "api.hip", "units.cpp", "mlp_api.cpp",
+           "snapshot.cpp"]
 ARCH = os.environ.get("TORCHGPU_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -60,7 +61,7 @@ def build(verbose: bool = False) -> str:
         msg = "\n".join(f"--- {s} ---\n{o}" for s, o in failed)
         raise RuntimeError(f"hipcc failed:\n{msg}")
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lz"]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     build_host(verbose)

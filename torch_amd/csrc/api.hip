@@ -23,6 +23,15 @@ struct torchgpu_ctx {
     int qcur = 0;
     bool qvalid = false, qstar_valid = false;
     int step_counter = 0;  // Torch::stepCounter of the operator-split loop
+    // the primitive state the reference's Fluid holds after the last step,
+    // which Torch::run prints: 1 = QfromU(U*) (torchgpu_run, hydro_step),
+    // 2 = fixPrimitives(QfromU(U*)) with U* the input of the last sub-step
+    // (torchgpu_run_split); before any step, fixPrimitives(QfromU(U))
+    int last_mode = 0;
+    int state_step = 0;   // StepState::step when U was last set from outside
+    bool pause_ckpt = false;
+    double* qprint = nullptr;
+    double* pstate = nullptr;  // operator-split: input of the last sub-step
     std::string err;
 };
 
@@ -68,10 +77,14 @@ int check_device_errors(torchgpu_ctx* c) {
 }
 
 // Mixed path: the power-of-two state scale from the current state (the
-// predictor kernels keep it up to date from then on).  Synchronises.
+// predictor kernels keep it up to date from then on).  Synchronises; also
+// records that U was set from outside (for the printed state).
 int seed_scale(torchgpu_ctx* c) {
     if (c->p.precision == TORCHGPU_MIXED) HIP_TRY(c, tg::launch_mixed_scale_seed(c->g, c->U, c->st, c->stream));
+    tg::StepState st;
+    HIP_TRY(c, hipMemcpyAsync(&st, c->st, sizeof st, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->state_step = st.step;
     return TORCHGPU_OK;
 }
 
@@ -135,6 +148,12 @@ int run_stage(torchgpu_ctx* c, int mode, double dtf) {
 
 int tg::ctx_run_stage(torchgpu_ctx* c, int mode, double dtf) { return run_stage(c, mode, dtf); }
 int& tg::ctx_step_counter(torchgpu_ctx* c) { return c->step_counter; }
+void tg::ctx_set_print_mode(torchgpu_ctx* c, int mode) { c->last_mode = mode; }
+double* tg::ctx_print_buffer(torchgpu_ctx* c) {
+    if (!c->pause_ckpt) return nullptr;
+    if (!c->pstate && hipMalloc(&c->pstate, state_bytes(c)) != hipSuccess) return nullptr;
+    return c->pstate;
+}
 
 extern "C" {
 
@@ -208,6 +227,8 @@ void torchgpu_destroy(torchgpu_ctx* c) {
     if (c->U) (void)hipFree(c->U);
     if (c->Ustar) (void)hipFree(c->Ustar);
     if (c->st) (void)hipFree(c->st);
+    if (c->qprint) (void)hipFree(c->qprint);
+    if (c->pstate) (void)hipFree(c->pstate);
     if (c->qf[0]) (void)hipFree(c->qf[0]);
     if (c->qf[1]) (void)hipFree(c->qf[1]);
     delete c;
@@ -296,6 +317,7 @@ static int hydro_step_device(torchgpu_ctx* c) {
 
 int torchgpu_hydro_step(torchgpu_ctx* c, double dt) {
     if (!c) return TORCHGPU_ERR_INVALID_ARG;
+    c->last_mode = 1;
     // an explicit dt: step_begin with use_cfl = 0 still honours tmax halting
     // bookkeeping but takes dt as given
     HIP_TRY(c, tg::launch_step_begin(c->st, 0, dt, c->stream));
@@ -325,6 +347,7 @@ int torchgpu_hydro_stage_dt(torchgpu_ctx* c, int stage, double dtf) {
 
 int torchgpu_run(torchgpu_ctx* c, int nsteps) {
     if (!c || nsteps < 0) return TORCHGPU_ERR_INVALID_ARG;
+    c->last_mode = 1;
     for (int s = 0; s < nsteps; ++s) {
         HIP_TRY(c, tg::launch_step_begin(c->st, 1, 0.0, c->stream));
         int rc = hydro_step_device(c);
@@ -339,6 +362,56 @@ int torchgpu_reset_clock(torchgpu_ctx* c) {
     int rc = reset_clock(c);
     if (rc) return rc;
     return seed_scale(c);
+}
+
+int torchgpu_set_time(torchgpu_ctx* c, double t) {
+    if (!c || !(t >= 0.0)) return TORCHGPU_ERR_INVALID_ARG;
+    HIP_TRY(c, tg::launch_set_time(c->st, t, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return TORCHGPU_OK;
+}
+
+int torchgpu_pause_at_checkpoints(torchgpu_ctx* c, int on) {
+    if (!c) return TORCHGPU_ERR_INVALID_ARG;
+    tg::StepState st;
+    HIP_TRY(c, hipMemcpyAsync(&st, c->st, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    st.pause_ckpt = on ? 1 : 0;
+    HIP_TRY(c, hipMemcpyAsync(c->st, &st, sizeof st, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->pause_ckpt = on != 0;
+    return TORCHGPU_OK;
+}
+
+int torchgpu_checkpoint(torchgpu_ctx* c, int* count, int* due) {
+    if (!c || !count || !due) return TORCHGPU_ERR_INVALID_ARG;
+    tg::StepState st;
+    HIP_TRY(c, hipMemcpyAsync(&st, c->st, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    *count = st.ckpt_count;
+    *due = st.paused;
+    if (st.paused) {
+        HIP_TRY(c, tg::launch_resume(c->st, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
+    return TORCHGPU_OK;
+}
+
+int torchgpu_get_print_state(torchgpu_ctx* c, double* Q, int on_device) {
+    if (!c || !Q) return TORCHGPU_ERR_INVALID_ARG;
+    tg::StepState st;
+    HIP_TRY(c, hipMemcpyAsync(&st, c->st, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const int mode = st.step > c->state_step ? c->last_mode : 0;
+    if (!c->qprint) HIP_TRY(c, hipMalloc(&c->qprint, state_bytes(c)));
+    if (mode == 2 && !c->pstate) return fail(c, TORCHGPU_ERR_INVALID_ARG, "print state: pause at checkpoints first");
+    if (mode == 0) HIP_TRY(c, tg::launch_q_from_u(c->g, c->U, c->qprint, 1, c->stream));
+    else if (mode == 1) HIP_TRY(c, tg::launch_q_from_u(c->g, c->Ustar, c->qprint, 0, c->stream));
+    else HIP_TRY(c, tg::launch_q_from_u(c->g, c->pstate, c->qprint, 1, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(Q, c->qprint, state_bytes(c), on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return TORCHGPU_OK;
 }
 
 int torchgpu_synchronize(torchgpu_ctx* c) {

@@ -166,10 +166,13 @@ __global__ __launch_bounds__(NTHREADS) void hydro_stage_kernel(HydroGeom g, cons
                                                               const double* Ubase, double* Udst, StepState* st,
                                                               double dtf_host) {
     __shared__ TileLDS s;
-    if (st->halted) return;
     const int tile = tile_of_block(blockIdx.x, g.ntiles);
     const int tx = tile % g.ntx, ty = (tile / g.ntx) % g.nty, tz = tile / (g.ntx * g.nty);
     const int o[3] = {tx * TILE, ty * TILE, tz * TILE};
+    if (st->halted) {
+        copy_through_halted(g, dtf_host, Ubase, Udst, o);
+        return;
+    }
     const int tid = threadIdx.x;
     const GasConsts<double> gc = {g.g, g.gm1, g.g2};
 
@@ -242,6 +245,31 @@ __global__ __launch_bounds__(256) void cfl_kernel(HydroGeom g, const double* __r
     }
 }
 
+// Checkpointer::update's count for time t (Checkpointer.cpp:14-26): the
+// number of the checkpoint percentages 100 / ncheckpoints passed.
+__device__ __forceinline__ int checkpoint_count(const StepState* st, double t) {
+    const double delta = 100.0 / st->ncheckpoints;
+    const double pct = smin(100.0, 100.0 * t / st->tmax);
+    int count = st->ckpt_count;
+    while (pct - count * delta > delta - 1.0e-8) count += 1;
+    return count;
+}
+
+__global__ void set_time_kernel(StepState* st, double t) {
+    st->time = t;
+    st->ckpt_count = 0;
+    st->ckpt_count = checkpoint_count(st, t);
+    st->paused = 0;
+    st->halted = 0;
+}
+
+__global__ void resume_kernel(StepState* st) {
+    if (st->paused) {
+        st->paused = 0;
+        st->halted = 0;
+    }
+}
+
 // Start of a step: dt = min(dt_nextCheckpoint, dt_cfl) as Torch::run +
 // Torch::fullStep + Torch::calculateTimeStep (Torch.cpp:266-270,304-339,397),
 // with the checkpoint arithmetic of Checkpointer::update (Checkpointer.cpp:14-26).
@@ -260,11 +288,17 @@ __global__ void step_begin_kernel(StepState* st, int use_cfl, double dt_override
         st->halted = 1;
         return;
     }
+    const int count = checkpoint_count(st, st->time);
+    if (st->pause_ckpt && count > st->ckpt_count) {
+        // a new checkpoint is due before this step: Torch::run prints here
+        st->ckpt_count = count;
+        st->paused = 1;
+        st->halted = 1;
+        return;
+    }
+    st->ckpt_count = count;
     const double delta = 100.0 / st->ncheckpoints;
     double pct = smin(100.0, 100.0 * st->time / st->tmax);
-    int count = st->ckpt_count;
-    while (pct - count * delta > delta - 1.0e-8) count += 1;
-    st->ckpt_count = count;
     const double dt_next = ((count + 1) * delta - pct) * st->tmax / 100.0;
     double dt;
     if (st->first) {
@@ -382,6 +416,32 @@ hipError_t launch_cfl(const HydroGeom& g, const double* U, StepState* st, hipStr
 hipError_t launch_step_begin(StepState* st, int use_cfl, double dt_override, hipStream_t stream,
                              const unsigned long long* thermo_dt_bits) {
     hipLaunchKernelGGL(step_begin_kernel, dim3(1), dim3(1), 0, stream, st, use_cfl, dt_override, thermo_dt_bits);
+    return hipGetLastError();
+}
+
+// dst = src (n doubles) unless the run has halted.
+__global__ void copy_if_running_kernel(const StepState* st, const double* __restrict__ src, double* __restrict__ dst,
+                                       long long n) {
+    if (st->halted) return;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+hipError_t launch_copy_if_running(const StepState* st, const double* src, double* dst, long long n,
+                                  hipStream_t stream) {
+    long long nb = (n + 255) / 256;
+    if (nb > 65536) nb = 65536;
+    hipLaunchKernelGGL(copy_if_running_kernel, dim3((unsigned)nb), dim3(256), 0, stream, st, src, dst, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_time(StepState* st, double t, hipStream_t stream) {
+    hipLaunchKernelGGL(set_time_kernel, dim3(1), dim3(1), 0, stream, st, t);
+    return hipGetLastError();
+}
+
+hipError_t launch_resume(StepState* st, hipStream_t stream) {
+    hipLaunchKernelGGL(resume_kernel, dim3(1), dim3(1), 0, stream, st);
     return hipGetLastError();
 }
 

@@ -1,0 +1,208 @@
+// snapshot.cpp -- the reference's data2D snapshot text format (host code).
+//
+//   torchgpu_snapshot_write   DataPrinter::print2D   (src/IO/DataPrinter.cpp:139-174)
+//   torchgpu_snapshot_header  DataReader::readDataParameters (src/IO/DataReader.cpp:17-68)
+//   torchgpu_snapshot_read    DataReader::readGrid   (src/IO/DataReader.cpp:70-125)
+//
+// Format: the time (s), then nx, ny and nz, one per line; then one line per
+// cell, x fastest, with the nd cell-centre coordinates (cm), density
+// (g/cm^3), pressure (dyn/cm^2), HII fraction and the nd velocity components
+// (cm/s), tab separated, each double as std::scientific with precision 10
+// (printf "%.10e").  nd = 1 when ny == 1, 2 when nz == 1, else 3.  The writer
+// gzips like the reference's OutputStreamGZ when asked; the readers take
+// plain or gzip text (zlib reads both).
+#include <zlib.h>
+
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/torchgpu.h"
+
+namespace {
+
+struct TextOut {
+    FILE* f = nullptr;
+    gzFile z = nullptr;
+    bool open(const char* path, bool gz) {
+        if (gz) z = gzopen(path, "wb6");
+        else f = std::fopen(path, "wb");
+        return z || f;
+    }
+    bool write(const char* s, size_t n) {
+        if (z) return gzwrite(z, s, (unsigned)n) == (int)n;
+        return std::fwrite(s, 1, n, f) == n;
+    }
+    bool close() {
+        bool ok = true;
+        if (z) ok = gzclose(z) == Z_OK;
+        if (f) ok = std::fclose(f) == 0;
+        z = nullptr;
+        f = nullptr;
+        return ok;
+    }
+};
+
+// Whitespace-separated tokens of a plain or gzip text file.
+struct TokenIn {
+    gzFile z = nullptr;
+    std::string buf;
+    size_t pos = 0;
+    bool eof = false;
+    bool open(const char* path) {
+        z = gzopen(path, "rb");
+        return z != nullptr;
+    }
+    ~TokenIn() {
+        if (z) gzclose(z);
+    }
+    bool fill() {
+        if (eof) return false;
+        char tmp[1 << 16];
+        const int n = gzread(z, tmp, sizeof tmp);
+        if (n <= 0) {
+            eof = true;
+            return false;
+        }
+        buf.erase(0, pos);
+        pos = 0;
+        buf.append(tmp, (size_t)n);
+        return true;
+    }
+    bool next(std::string& tok) {
+        tok.clear();
+        for (;;) {
+            while (pos < buf.size() && std::isspace((unsigned char)buf[pos])) ++pos;
+            if (pos < buf.size()) break;
+            if (!fill()) return false;
+        }
+        for (;;) {
+            while (pos < buf.size() && !std::isspace((unsigned char)buf[pos])) tok.push_back(buf[pos++]);
+            if (pos < buf.size() || !fill()) return !tok.empty();
+        }
+    }
+    bool number(double& v) {
+        std::string t;
+        if (!next(t)) return false;
+        char* end = nullptr;
+        v = std::strtod(t.c_str(), &end);
+        return end && *end == '\0';
+    }
+};
+
+int dims_of(int ny, int nz) { return ny == 1 ? 1 : (nz == 1 ? 2 : 3); }
+
+double from_code(double scales[3], double v, double m, double l, double t) {
+    double out = 0.0;
+    torchgpu_units(scales[0], scales[1], scales[2], v, m, l, t, 0, &out);
+    return out;
+}
+
+}  // namespace
+
+extern "C" int torchgpu_snapshot_write(const char* path, int gz, double t, const double* Q, int nx, int ny, int nz,
+                                       double side_length, double density_scale, double pressure_scale,
+                                       double time_scale) {
+    if (!path || !Q || nx < 1 || ny < 1 || nz < 1 || !(side_length > 0.0)) return TORCHGPU_ERR_INVALID_ARG;
+    double sc[3] = {density_scale, pressure_scale, time_scale};
+    const int nd = dims_of(ny, nz);
+    const long long n = (long long)nx * ny * nz;
+    // Grid::initialise: dx[i] = sideLength / ncells[0] (Grid.cpp:471-472);
+    // cell centres xc = index + 0.5 in the nd active dimensions (Grid.cpp:537-541)
+    const double dx = side_length / (double)nx;
+    // unit factors as Converter::fromCodeUnits applies them (one conversion
+    // per value, as the reference prints)
+    TextOut out;
+    if (!out.open(path, gz != 0)) return TORCHGPU_ERR_INVALID_ARG;
+    char line[512];
+    int k = std::snprintf(line, sizeof line, "%.10e\n%d\n%d\n%d\n", from_code(sc, t, 0, 0, 1), nx, ny, nz);
+    bool ok = out.write(line, (size_t)k);
+    for (long long c = 0; c < n && ok; ++c) {
+        const long long idx[3] = {c % nx, (c / nx) % ny, c / ((long long)nx * ny)};
+        k = 0;
+        for (int d = 0; d < nd; ++d)
+            k += std::snprintf(line + k, sizeof line - k, "%.10e\t",
+                               from_code(sc, ((double)idx[d] + 0.5) * dx, 0, 1, 0));
+        k += std::snprintf(line + k, sizeof line - k, "%.10e", from_code(sc, Q[0 * n + c], 1, -3, 0));
+        k += std::snprintf(line + k, sizeof line - k, "\t%.10e", from_code(sc, Q[1 * n + c], 1, -1, -2));
+        k += std::snprintf(line + k, sizeof line - k, "\t%.10e", Q[2 * n + c]);
+        for (int d = 0; d < nd; ++d)
+            k += std::snprintf(line + k, sizeof line - k, "\t%.10e", from_code(sc, Q[(4 + d) * n + c], 0, 1, -1));
+        line[k++] = '\n';
+        ok = out.write(line, (size_t)k);
+    }
+    ok = out.close() && ok;
+    return ok ? TORCHGPU_OK : TORCHGPU_ERR_INVALID_ARG;
+}
+
+extern "C" int torchgpu_snapshot_header(const char* path, double* t, int* nx, int* ny, int* nz, int* nd,
+                                        double* side_length) {
+    if (!path || !t || !nx || !ny || !nz || !nd || !side_length) return TORCHGPU_ERR_INVALID_ARG;
+    TokenIn in;
+    if (!in.open(path)) return TORCHGPU_ERR_INVALID_ARG;
+    double v[4];
+    for (double& x : v)
+        if (!in.number(x)) return TORCHGPU_ERR_INVALID_ARG;
+    *t = v[0];
+    *nx = (int)v[1];
+    *ny = (int)v[2];
+    *nz = (int)v[3];
+    *nd = dims_of(*ny, *nz);
+    // the first two cells' coordinates give dx (DataReader.cpp:34-58)
+    double x0[3] = {0, 0, 0}, x1[3] = {0, 0, 0}, skip;
+    for (int d = 0; d < *nd; ++d)
+        if (!in.number(x0[d])) return TORCHGPU_ERR_INVALID_ARG;
+    for (int i = 0; i < 3 + *nd; ++i)
+        if (!in.number(skip)) return TORCHGPU_ERR_INVALID_ARG;
+    for (int d = 0; d < *nd; ++d)
+        if (!in.number(x1[d])) return TORCHGPU_ERR_INVALID_ARG;
+    double dx = 0.0;
+    for (int d = 0; d < *nd; ++d)
+        if (std::abs(x1[d] - x0[d]) > std::numeric_limits<double>::min()) {
+            dx = std::abs(x1[d] - x0[d]);
+            break;
+        }
+    if (!(dx > 0.0)) return TORCHGPU_ERR_INVALID_ARG;
+    *side_length = dx * (*nx);
+    return TORCHGPU_OK;
+}
+
+extern "C" int torchgpu_snapshot_read(const char* path, double* Q, int nx, int ny, int nz) {
+    if (!path || !Q || nx < 1 || ny < 1 || nz < 1) return TORCHGPU_ERR_INVALID_ARG;
+    double t, side;
+    int hx, hy, hz, nd;
+    int rc = torchgpu_snapshot_header(path, &t, &hx, &hy, &hz, &nd, &side);
+    if (rc) return rc;
+    const double dx = side / hx;
+    TokenIn in;
+    if (!in.open(path)) return TORCHGPU_ERR_INVALID_ARG;
+    double skip;
+    for (int i = 0; i < 4; ++i)
+        if (!in.number(skip)) return TORCHGPU_ERR_INVALID_ARG;
+    // one line per cell of this grid, as DataReader::readGrid reads them
+    const long long n = (long long)nx * ny * nz;
+    for (long long c = 0; c < n; ++c) {
+        int xc[3] = {0, 0, 0};
+        for (int d = 0; d < nd; ++d) {
+            double x;
+            if (!in.number(x)) return TORCHGPU_ERR_INVALID_ARG;
+            xc[d] = (int)(x / dx);
+        }
+        double rho, pre, hii, vel[3] = {0, 0, 0};
+        if (!in.number(rho) || !in.number(pre) || !in.number(hii)) return TORCHGPU_ERR_INVALID_ARG;
+        for (int d = 0; d < nd; ++d)
+            if (!in.number(vel[d])) return TORCHGPU_ERR_INVALID_ARG;
+        // Grid::locate: cells outside this grid are skipped
+        if (xc[0] < 0 || xc[0] >= nx || xc[1] < 0 || xc[1] >= ny || xc[2] < 0 || xc[2] >= nz) continue;
+        const long long i = xc[0] + (long long)nx * (xc[1] + (long long)ny * xc[2]);
+        Q[0 * n + i] = rho;
+        Q[1 * n + i] = pre;
+        Q[2 * n + i] = hii;
+        for (int d = 0; d < nd; ++d) Q[(4 + d) * n + i] = vel[d];
+    }
+    return TORCHGPU_OK;
+}

@@ -970,6 +970,8 @@ int torchgpu_run_split(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* T_m
     if (surrogate && !c->has_surrogate) return TORCHGPU_ERR_NOT_TRAINED;
     tg::CtxView v = tg::ctx_view(ctx);
     if (v.stream != c->stream) return TORCHGPU_ERR_INVALID_ARG;  // one stream orders the two handles' work
+    tg::ctx_set_print_mode(ctx, 2);
+    double* print_state = tg::ctx_print_buffer(ctx);
     int rc;
     for (int s = 0; s < nsteps; ++s) {
         v = tg::ctx_view(ctx);  // U and U* swap roles after every hydro sub-step
@@ -985,6 +987,13 @@ int torchgpu_run_split(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* T_m
         for (int k = 0; k < 3; ++k) {
             const bool hydro = ((k == 1 ? 1 : 0) + counter) % 2 == 0;
             const double code = (k == 1) ? tg::DT_FULL : tg::DT_HALF;
+            if (k == 2 && print_state) {
+                // the reference's Fluid::Q after the step is fixPrimitives(Q)
+                // of the last sub-step's input (Torch.cpp:352-358): kept for
+                // the snapshots printed at checkpoints
+                tg::CtxView w = tg::ctx_view(ctx);
+                HIP_OK(tg::launch_copy_if_running(w.st, w.U, print_state, 7 * w.g->ncell, w.stream));
+            }
             if (hydro) {
                 if ((rc = tg::ctx_run_stage(ctx, 0, code))) return rc;
             } else if ((rc = grid_substep_dev(ctx, c, T_min, col_den, F, excluded, code, surrogate))) {

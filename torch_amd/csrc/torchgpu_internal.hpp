@@ -50,6 +50,11 @@ struct StepState {
     // records for the next one (hydro_mixed.hip)
     unsigned int qmax[2];
     unsigned int qmax_next[2];
+    // Torch::run prints a snapshot when a new checkpoint is due
+    // (Torch.cpp:268-280): with pause_ckpt set, the run stops before such a
+    // step (paused, halted) until the host has printed and resumes it
+    int pause_ckpt;
+    int paused;
 };
 
 // Negative dt arguments of the stage and cooling kernels take the dt of the
@@ -112,12 +117,23 @@ void ctx_state_changed(torchgpu_ctx* c);
 int ctx_run_stage(torchgpu_ctx* c, int mode, double dtf);
 // The context's Strang alternation counter (Torch::stepCounter).
 int& ctx_step_counter(torchgpu_ctx* c);
+// Bookkeeping of the state Torch::run prints (see torchgpu_get_print_state):
+// the mode of the last run, and (when pausing at checkpoints) the buffer
+// that keeps the input of an operator-split step's last sub-step.
+void ctx_set_print_mode(torchgpu_ctx* c, int mode);
+double* ctx_print_buffer(torchgpu_ctx* c);
 hipError_t launch_cfl(const HydroGeom& g, const double* U, StepState* st, hipStream_t stream);
 // thermo_dt_bits: the bit pattern of the cooling operator's minimum
 // |E/RATE| (a Torch::fullStep with thermodynamics), or null (hydro only).
 hipError_t launch_step_begin(StepState* st, int use_cfl, double dt_override, hipStream_t stream,
                              const unsigned long long* thermo_dt_bits = nullptr);
 hipError_t launch_step_end(StepState* st, hipStream_t stream);
+// Clock control for Torch::run's checkpoint printing: set the time (and the
+// checkpoint count Checkpointer::update gives for it), resume a paused run.
+hipError_t launch_set_time(StepState* st, double t, hipStream_t stream);
+hipError_t launch_copy_if_running(const StepState* st, const double* src, double* dst, long long n,
+                                  hipStream_t stream);
+hipError_t launch_resume(StepState* st, hipStream_t stream);
 hipError_t launch_q_from_u(const HydroGeom& g, const double* U, double* Q, int fix, hipStream_t stream);
 hipError_t launch_u_from_q(const HydroGeom& g, double* Q, double* U, hipStream_t stream);
 

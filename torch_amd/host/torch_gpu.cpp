@@ -3,7 +3,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <filesystem>
 #include <sstream>
 #include <vector>
 
@@ -212,7 +214,7 @@ void Fluid::advSolution(double dt) {
         check_rc(torchgpu_hydro_stage_dt(ctx_, stage, dt), "hydro stage", ctx_);
         if (stage == 2) w_saved_ = u_is_w_ = false;
     } else if (pending == Pending::THERMO) {
-        check_rc(torchgpu_thermo_grid_substep(ctx_, cooling, nullptr, nullptr, nullptr, 0, dt, 0), "thermo substep",
+        check_rc(torchgpu_thermo_grid_substep(ctx_, cooling, nullptr, nullptr, nullptr, -1, dt, 0), "thermo substep",
                  ctx_);
     }
     pending = Pending::NONE;
@@ -255,7 +257,7 @@ void Thermodynamics::preTimeStepCalculations(Fluid&) const {}  // rates are eval
 
 double Thermodynamics::calculateTimeStep(double dt_max, Fluid& fluid) const {
     double dt = 0;
-    check_rc(torchgpu_thermo_grid_timestep(fluid.ctx(), cool_, nullptr, nullptr, nullptr, 0, dt_max, &dt),
+    check_rc(torchgpu_thermo_grid_timestep(fluid.ctx(), cool_, nullptr, nullptr, nullptr, -1, dt_max, &dt),
              "thermo timestep", fluid.ctx());
     return dt;
 }
@@ -271,6 +273,18 @@ void Thermodynamics::updateSourceTerms(double, Fluid&) const {}
 // ---------------------------------------------------------------------------
 void Torch::initialise(const TorchParameters& p_in, void* stream) {
     params = p_in;
+    // Integration.initial_conditions (Torch.cpp:38-45): cell counts, side
+    // length (cm) and dimensions from the snapshot's header
+    double t0 = 0, side = 0;
+    int nx = 0, ny = 0, nz = 0, nd = 0;
+    const std::string& ic = params.initialConditions;
+    if (!ic.empty()) {
+        if (torchgpu_snapshot_header(ic.c_str(), &t0, &nx, &ny, &nz, &nd, &side) != TORCHGPU_OK)
+            throw std::runtime_error("DataReader::readDataParameters: invalid input file " + ic + ".");
+        params.ncells = {{nx, ny, nz}};
+        params.sideLength = side;
+        params.nd = nd;
+    }
     params.initialise();
     fluid.initialise(params, stream);
     if (params.cooling_on) thermodynamics.initialise(params, stream);
@@ -278,6 +292,16 @@ void Torch::initialise(const TorchParameters& p_in, void* stream) {
     steps = 0;
     first_time_ = true;
     stepCounter_ = 0;
+    if (!ic.empty()) {
+        // DataReader::readGrid, then the tail of Torch::initialise
+        // (Torch.cpp:58, 100-103, 117-119)
+        std::vector<double> Q((size_t)7 * nx * ny * nz, 0.0);
+        if (torchgpu_snapshot_read(ic.c_str(), Q.data(), nx, ny, nz) != TORCHGPU_OK)
+            throw std::runtime_error("DataReader::readGrid: invalid input file " + ic + ".");
+        setPrimitivesCgs(Q.data());
+        currentTime = params.toCodeUnits(t0, 0, 0, 1);
+        check_rc(torchgpu_set_time(fluid.ctx(), currentTime), "set time", fluid.ctx());
+    }
 }
 
 void Torch::setPrimitivesCgs(const double* Qcgs) {
@@ -362,7 +386,58 @@ double Torch::fullStep(double dt_nextCheckPoint) {  // Torch.cpp:389-418
     return dt;
 }
 
-void Torch::run(int max_steps) {  // Torch.cpp:240-302 (printing omitted), capped at max_steps
+void Torch::print2D(int count) const {
+    // DataPrinter::print2D (DataPrinter.cpp:139-174) of the state Fluid::Q
+    // holds at this point of Torch::run
+    const torchgpu_params g = params.gridParams();
+    std::vector<double> Q((size_t)7 * g.nx * g.ny * g.nz);
+    check_rc(torchgpu_get_print_state(fluid.ctx(), Q.data(), 0), "print state", fluid.ctx());
+    double t = 0, dt = 0;
+    int n = 0;
+    check_rc(torchgpu_get_clock(fluid.ctx(), &t, &dt, &n), "clock", fluid.ctx());
+    std::filesystem::create_directories(params.outputDirectory);
+    char name[64];
+    std::snprintf(name, sizeof name, "data2D_%06d.txt.gz", count);
+    const std::string path = (std::filesystem::path(params.outputDirectory) / name).string();
+    if (torchgpu_snapshot_write(path.c_str(), 1, t, Q.data(), g.nx, g.ny, g.nz, params.sideLength, params.dscale,
+                                params.pscale, params.tscale) != TORCHGPU_OK)
+        throw std::runtime_error("DataPrinter::print2D: unable to open" + path);
+}
+
+void Torch::run(int max_steps, bool print) {  // Torch.cpp:240-302, capped at max_steps
+    if (print) {
+        // the device loop pauses at every checkpoint for the print
+        torchgpu_ctx* c = fluid.ctx();
+        int count = 0, due = 0;
+        check_rc(torchgpu_checkpoint(c, &count, &due), "checkpoint", c);
+        print2D(count);
+        check_rc(torchgpu_pause_at_checkpoints(c, 1), "pause at checkpoints", c);
+        bool final_print = false;
+        int done = 0;
+        double t = 0, dt = 0;
+        int n0 = 0, n1 = 0;
+        while (done < max_steps) {
+            check_rc(torchgpu_get_clock(c, &t, &dt, &n0), "clock", c);
+            if (params.cooling_on)
+                check_rc(torchgpu_run_split(c, thermodynamics.handle(), nullptr, nullptr, nullptr, -1,
+                                            max_steps - done, 0), "run", c);
+            else
+                check_rc(torchgpu_run(c, max_steps - done), "run", c);
+            fluid.check();
+            check_rc(torchgpu_get_clock(c, &t, &dt, &n1), "clock", c);
+            done += n1 - n0;
+            check_rc(torchgpu_checkpoint(c, &count, &due), "checkpoint", c);
+            if (!due) break;
+            print2D(count);
+            final_print = count != params.ncheckpoints;
+        }
+        check_rc(torchgpu_get_clock(c, &t, &dt, &n1), "clock", c);
+        if (final_print && !(t < params.tmax)) print2D(params.ncheckpoints);
+        check_rc(torchgpu_pause_at_checkpoints(c, 0), "pause at checkpoints", c);
+        currentTime = t;
+        steps = n1;
+        return;
+    }
     if (!params.cooling_on) {
         // hydro-only: the whole loop stays on the device (dt chosen there)
         check_rc(torchgpu_run(fluid.ctx(), max_steps), "run", fluid.ctx());

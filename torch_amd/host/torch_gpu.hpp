@@ -21,6 +21,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../include/torchgpu.h"
 #include "lua_config.hpp"
@@ -170,7 +171,10 @@ class Torch {
 public:
     void initialise(const TorchParameters& p, void* stream = nullptr);
     void setPrimitivesCgs(const double* Q);   // Torch::toCodeUnits + fixPrimitives + globalUfromQ
-    void run(int max_steps);                  // Torch::run, capped
+    // Torch::run, capped; with `print`, the data2D snapshots of
+    // Integration.output_directory at the start, every checkpoint and the end
+    void run(int max_steps, bool print = false);
+    void print2D(int count) const;
     double fullStep(double dt_nextCheckpoint);
     void hydroStep(double dt, bool hasCalculatedHeatFlux);
     void subStep(double dt, bool hasCalculatedHeatFlux, Integrator& comp);

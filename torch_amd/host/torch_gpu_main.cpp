@@ -1,13 +1,18 @@
 // torch_gpu_main.cpp -- command-line driver: reads a TORCH Lua parameter
 // file, sets the initial state and runs Torch::run on the GPU.
 //
-//   torch_gpu --paramfile=FILE [--setup=sod|FILE.bin] [--steps=N]
-//             [--output=FILE.bin] [--dump-params]
+//   torch_gpu --paramfile=FILE [--setup=sod|FILE.bin|FILE.txt[.gz]] [--steps=N]
+//             [--output=FILE.bin] [--snapshots] [--dump-params]
 //
 // --setup=sod builds the Sod shock tube along x; FILE.bin holds primitive
 // variables in cgs, float64 [7][nz][ny][nx] (UID order), the values a Lua
 // set-up function would return (the reference's Torch::setUpLua,
-// Torch.cpp:192-228, needs a Lua runtime).
+// Torch.cpp:192-228, needs a Lua runtime).  A data2D snapshot (.txt or
+// .txt.gz) is read as Integration.initial_conditions is (grid shape, side
+// length, state and time from the file); a non-empty initial_conditions in
+// the parameter file does the same.  --snapshots writes the data2D files of
+// Torch::run to Integration.output_directory: the initial state, every
+// checkpoint and the final state.
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -45,7 +50,8 @@ void dump(const tg::TorchParameters& p) {
     tg::TorchParameters c = p;
     c.initialise();
     std::printf(" \"code_units\": {\"side_length\": %.17g, \"tmax\": %.17g, \"dt_max\": %.17g, \"dfloor\": %.17g, "
-                "\"pfloor\": %.17g}}\n", c.sideLength, c.tmax, c.dt_max, c.dfloor, c.pfloor);
+                "\"pfloor\": %.17g, \"spec_gas_const\": %.17g}}\n", c.sideLength, c.tmax, c.dt_max, c.dfloor, c.pfloor,
+                c.specificGasConstant);
 }
 
 }  // namespace
@@ -53,7 +59,8 @@ void dump(const tg::TorchParameters& p) {
 int main(int argc, char** argv) {
     const std::string paramfile = arg(argc, argv, "--paramfile", "");
     if (paramfile.empty()) {
-        std::fprintf(stderr, "usage: torch_gpu --paramfile=FILE [--setup=sod|FILE.bin] [--steps=N] [--output=FILE.bin] [--dump-params]\n");
+        std::fprintf(stderr, "usage: torch_gpu --paramfile=FILE [--setup=sod|FILE.bin|FILE.txt[.gz]] [--steps=N] "
+                             "[--output=FILE.bin] [--snapshots] [--dump-params]\n");
         return 2;
     }
     try {
@@ -64,13 +71,19 @@ int main(int argc, char** argv) {
             dump(p);
             return 0;
         }
+        const std::string setup = arg(argc, argv, "--setup", p.initialConditions.empty() ? "sod" : "");
+        auto ends_with = [](const std::string& a, const std::string& b) {
+            return a.size() >= b.size() && a.compare(a.size() - b.size(), b.size(), b) == 0;
+        };
+        if (ends_with(setup, ".txt") || ends_with(setup, ".txt.gz")) p.initialConditions = setup;
         tg::Torch torch;
         torch.initialise(p);
-        const int nx = p.ncells[0], ny = p.ncells[1], nz = p.ncells[2];
+        const int nx = torch.params.ncells[0], ny = torch.params.ncells[1], nz = torch.params.ncells[2];
         const size_t n = (size_t)nx * ny * nz;
         std::vector<double> Q(7 * n, 0.0);
-        const std::string setup = arg(argc, argv, "--setup", "sod");
-        if (setup == "sod") {
+        if (!torch.params.initialConditions.empty()) {
+            // state and time already read by Torch::initialise
+        } else if (setup == "sod") {
             for (int k = 0; k < nz; ++k)
                 for (int j = 0; j < ny; ++j)
                     for (int i = 0; i < nx; ++i) {
@@ -84,10 +97,10 @@ int main(int argc, char** argv) {
             if (!f.read(reinterpret_cast<char*>(Q.data()), (std::streamsize)(Q.size() * sizeof(double))))
                 throw std::runtime_error("cannot read " + setup);
         }
-        torch.setPrimitivesCgs(Q.data());
+        if (torch.params.initialConditions.empty()) torch.setPrimitivesCgs(Q.data());
         const int steps = std::stoi(arg(argc, argv, "--steps", "100"));
         auto t0 = std::chrono::steady_clock::now();
-        torch.run(steps);
+        torch.run(steps, arg(argc, argv, "--snapshots", "") == "1");
         auto t1 = std::chrono::steady_clock::now();
         const double secs = std::chrono::duration<double>(t1 - t0).count();
         std::printf("{\"steps\": %d, \"time\": %.17g, \"seconds\": %.6f, \"mcell_updates_per_s\": %.3f}\n", torch.steps,
