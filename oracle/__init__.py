@@ -106,6 +106,12 @@ def lib() -> ctypes.CDLL:
         L.orc_log_spline.restype = d
         L.orc_thermo_rates.argtypes = [P(ThermoParams), d, d, d, d, d, dp]
         L.orc_thermo_batch.argtypes = [P(ThermoParams), ctypes.c_int64, dp, dp, ctypes.c_int]
+        i3 = ctypes.POINTER(ctypes.c_int)
+        i64 = ctypes.POINTER(ctypes.c_int64)
+        L.orc_star_geometry.argtypes = [i3, i3, d, d, i64, dp, dp, dp]
+        L.orc_star_wind.argtypes = [i3, i3, ctypes.c_int, ctypes.POINTER(ctypes.c_ubyte)]
+        L.orc_thermo_ray_trace.argtypes = [i3, i3, i64, dp, dp, d, dp, dp, dp]
+        L.orc_star_fuv.argtypes = [i3, i3, d, d, d, dp]
         _lib = L
     return _lib
 
@@ -272,3 +278,51 @@ def thermo_rates(t: ThermoParams, nH, x, T, Av, F) -> np.ndarray:
 
 def log_spline(t: ThermoParams, x: float) -> float:
     return lib().orc_log_spline(ctypes.byref(t), x)
+
+
+def _i3(v):
+    return (ctypes.c_int * 3)(*[int(x) for x in v])
+
+
+def star_geometry(shape, star, dx: float, pi: float):
+    """Grid::calculateNearestNeighbours + Radiation::cellPathLength /
+    shellVolume for a star at cell `star` = (i, j, k) of a grid of `shape` =
+    (nz, ny, nx): (nb int64 [nz, ny, nx, 4], w [.., 4], ds, shell_vol)."""
+    nz, ny, nx = shape
+    nb = np.zeros((nz, ny, nx, 4), dtype=np.int64)
+    w = np.zeros((nz, ny, nx, 4))
+    ds = np.zeros((nz, ny, nx))
+    sv = np.zeros((nz, ny, nx))
+    lib().orc_star_geometry(_i3((nx, ny, nz)), _i3(star), dx, pi, nb.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                            _dp(w), _dp(ds), _dp(sv))
+    return nb, w, ds, sv
+
+
+def star_wind(shape, star, wind_radius: int) -> np.ndarray:
+    """CausalWind membership (bool [nz, ny, nx]) of Fluid::initialiseGrid."""
+    nz, ny, nx = shape
+    out = np.zeros((nz, ny, nx), dtype=np.uint8)
+    lib().orc_star_wind(_i3((nx, ny, nz)), _i3(star), int(wind_radius),
+                        out.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)))
+    return out.astype(bool)
+
+
+def thermo_ray_trace(shape, star, geom, hydrogen_mass: float, den: np.ndarray):
+    """Thermodynamics::rayTrace: (COL_DEN, DCOL_DEN) [nz, ny, nx] of the
+    density field `den` (code units) with the geometry of star_geometry."""
+    nz, ny, nx = shape
+    nb, w, ds, _ = geom
+    den = np.ascontiguousarray(den, dtype=np.float64)
+    col = np.zeros((nz, ny, nx))
+    dcol = np.zeros((nz, ny, nx))
+    lib().orc_thermo_ray_trace(_i3((nx, ny, nz)), _i3(star), nb.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                               _dp(w), _dp(ds), hydrogen_mass, _dp(den), _dp(col), _dp(dcol))
+    return col, dcol
+
+
+def star_fuv(shape, star, dx: float, pi: float, photon_rate: float) -> np.ndarray:
+    """F_FUV [nz, ny, nx] of a star-on thermo preTimeStep (code units)."""
+    nz, ny, nx = shape
+    F = np.zeros((nz, ny, nx))
+    lib().orc_star_fuv(_i3((nx, ny, nz)), _i3(star), dx, pi, photon_rate, _dp(F))
+    return F

@@ -15,6 +15,7 @@
 //   riemann <solver> <gamma> <n> <in.bin> <out.bin>
 //   thermo  <case.txt>              per-cell preTimeStep + integrate
 //   consts  <dscale> <pscale> <tscale>   print unit-converted constants
+//   raytrace <case.txt>             star geometry + thermo column densities
 
 #include <algorithm>
 #include <array>
@@ -43,6 +44,7 @@
 #include "Integrators/Riemann.hpp"
 #include "Integrators/SlopeLimiter.hpp"
 #include "Integrators/Thermodynamics.hpp"
+#include "Integrators/Radiation.hpp"
 #include "MPI/MPI_Wrapper.hpp"
 #include "Torch/Constants.hpp"
 #include "Torch/Parameters.hpp"
@@ -120,7 +122,8 @@ static void init_sim(Sim& s, const KV& kv) {
     p.minTempInitialState = geti(kv, "min_temp_initial_state", 0) != 0;
     p.star_on = geti(kv, "star_on", 0) != 0;
     p.photonRate = getd(kv, "photon_rate", 0.0);
-    p.star_position = {{0, 0, 0}};
+    p.star_position = {{geti(kv, "star_x", 0), geti(kv, "star_y", 0), geti(kv, "star_z", 0)}};
+    p.windCellRadius = geti(kv, "wind_radius", 0);
 
     s.consts = std::make_shared<Constants>();
     s.consts->initialise_DPT(p.dscale, p.pscale, p.tscale);
@@ -421,6 +424,59 @@ static int run_thermo(const KV& kv_in) {
     return 0;
 }
 
+// The thermo column-density ray trace of a star-on run: Radiation::initField
+// (nearest neighbours, weights, path lengths, shell volumes;
+// Radiation.cpp:78-96, Grid.cpp:922-951) and Thermodynamics::
+// preTimeStepCalculations with its rayTrace (Thermodynamics.cpp:218-262,
+// 339-417) on a grid whose primitives (code units) come from input_q.
+// Output per cell (15 doubles): neighbour IDs 0-3, weights 0-3, ds,
+// shellVol, T[COL_DEN], T[DCOL_DEN], T[RATE], T[HEAT], 1 if in CausalWind.
+static int run_raytrace(const KV& kv_in) {
+    KV kv = kv_in;
+    kv["star_on"] = "1";
+    kv["cooling_on"] = "1";
+    Sim s;
+    init_sim(s, kv);
+    Grid& grid = s.fluid.getGrid();
+    const int n = grid.ncells[0] * grid.ncells[1] * grid.ncells[2];
+    std::vector<double> q(7 * (size_t)n);
+    FILE* f = fopen(gets(kv, "input_q", "").c_str(), "rb");
+    if (fread(q.data(), 8, q.size(), f) != q.size()) throw std::runtime_error("short raytrace input");
+    fclose(f);
+    for (int c = 0; c < n; ++c) {
+        GridCell& cell = grid.getCell(c);
+        for (int v = 0; v < 7; ++v) cell.Q[v] = q[(size_t)v * n + c];
+    }
+    s.fluid.globalUfromQ();
+    s.thermo.initialiseMinTempField(s.fluid);
+    Radiation rad;
+    rad.initialise(s.consts, s.p.getRadiationParameters());
+    rad.initField(s.fluid);
+    s.thermo.preTimeStepCalculations(s.fluid);
+    std::vector<char> wind(n, 0);
+    for (int id : grid.getOrderedIndices("CausalWind")) wind[id] = 1;
+    std::vector<double> out(15 * (size_t)n);
+    for (int c = 0; c < n; ++c) {
+        GridCell& cell = grid.getCell(c);
+        double* o = &out[15 * (size_t)c];
+        for (int i = 0; i < 4; ++i) o[i] = cell.neighbourIDs[i];
+        for (int i = 0; i < 4; ++i) o[4 + i] = cell.neighbourWeights[i];
+        o[8] = cell.ds;
+        o[9] = cell.shellVol;
+        o[10] = cell.T[TID::COL_DEN];
+        o[11] = cell.T[TID::DCOL_DEN];
+        o[12] = cell.T[TID::RATE];
+        o[13] = cell.T[TID::HEAT];
+        o[14] = wind[c];
+    }
+    f = fopen(gets(kv, "output", "").c_str(), "wb");
+    fwrite(out.data(), 8, out.size(), f);
+    fclose(f);
+    printf("{\"photon_rate\": %.17g, \"hydrogen_mass\": %.17g, \"pi\": %.17g, \"dx\": %.17g}\n", s.p.photonRate,
+           s.consts->hydrogenMass, s.consts->pi, grid.dx[0]);
+    return 0;
+}
+
 static int run_consts(int argc, char** argv) {
     KV kv;
     kv["density_scale"] = argv[2];
@@ -464,6 +520,7 @@ int main(int argc, char** argv) {
         if (mode == "riemann") return run_riemann(argc, argv);
         if (mode == "thermo") return run_thermo(read_case(argv[2]));
         if (mode == "consts") return run_consts(argc, argv);
+        if (mode == "raytrace") return run_raytrace(read_case(argv[2]));
     } catch (std::exception& e) {
         fprintf(stderr, "ref_harness error: %s\n", e.what());
         return 1;

@@ -928,3 +928,184 @@ void orc_thermo_batch(const orc_thermo_params* t, int64_t n, const double* in, d
         o[0] = heat; o[1] = rp; o[2] = rate; o[3] = (double)nsub; o[4] = ps;
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Star geometry and the thermo column-density ray trace                       */
+/* ------------------------------------------------------------------------ */
+/* Cell centres sit at xc = index + 0.5 and the star at xs = cell + 0.5, in
+ * cell units (Grid::buildCells, Grid.cpp:537-541; Star::initialise without
+ * face snapping, Star.cpp:17-21).  3-D grids; the star inside the grid. */
+
+/* Grid::getRayPlane, Grid.cpp:909-920: the axis of the largest |xc - xs|,
+ * the first one on ties. */
+static int ray_plane(const double xc[3], const double xs[3]) {
+    int plane = 0;
+    double m = 0;
+    for (int i = 0; i < 3; ++i) {
+        double a = fabs(xc[i] - xs[i]);
+        if (a > m) {
+            m = a;
+            plane = i;
+        }
+    }
+    return plane;
+}
+
+/* Grid::traverse3D, Grid.cpp:167-178: the cell dc[a] steps along axis
+ * ax[a], or -1 off the grid. */
+static int64_t step_cell(const int n[3], const int ijk[3], const int ax[3], const int dc[3]) {
+    int c[3] = {ijk[0], ijk[1], ijk[2]};
+    for (int a = 0; a < 3; ++a) c[ax[a]] += dc[a];
+    for (int a = 0; a < 3; ++a)
+        if (c[a] < 0 || c[a] >= n[a]) return -1;
+    return c[0] + (int64_t)n[0] * (c[1] + (int64_t)n[1] * c[2]);
+}
+
+/* Grid::calculateNearestNeighbours (Grid.cpp:922-951), Radiation::
+ * cellPathLength (Radiation.cpp:102-117) and shellVolume (:119-138) of the
+ * cell (i, j, k): the four upstream neighbours and their weights, the path
+ * length ds and the shell volume (code units, dx the cell width). */
+void orc_star_cell_geometry(const int n[3], const int star[3], double dx, double pi, int i, int j, int k,
+                            int64_t nb[4], double w[4], double* ds, double* shell_vol) {
+    const int ijk[3] = {i, j, k};
+    double xc[3], xs[3];
+    for (int a = 0; a < 3; ++a) {
+        xc[a] = (double)ijk[a] + 0.5;
+        xs[a] = (double)star[a] + 0.5;
+    }
+    const int plane = ray_plane(xc, xs);
+    const int irot[3] = {(plane + 1) % 3, (plane + 2) % 3, plane % 3};
+    double d[3];
+    int s[3], LR[3];
+    for (int a = 0; a < 3; ++a) {
+        d[a] = xc[irot[a]] - xs[irot[a]];
+        s[a] = d[a] < -1.0 / 10.0 ? -1 : 1;
+        LR[a] = fabs(d[a]) < 1.0 / 10.0 ? 0 : s[a];
+    }
+    const int o0[3] = {0, 0, -LR[2]}, o1[3] = {0, -LR[1], -LR[2]}, o2[3] = {-LR[0], 0, -LR[2]},
+              o3[3] = {-LR[0], -LR[1], -LR[2]};
+    nb[0] = step_cell(n, ijk, irot, o0);
+    nb[1] = step_cell(n, ijk, irot, o1);
+    nb[2] = step_cell(n, ijk, irot, o2);
+    nb[3] = step_cell(n, ijk, irot, o3);
+    const double i0 = (double)(int)xc[irot[0]], i1 = (double)(int)xc[irot[1]];
+    const double ic0 = i0 - 0.5 * (s[2] * d[0] / d[2]);
+    const double ic1 = i1 - 0.5 * (s[2] * d[1] / d[2]);
+    const double del0 = fabs(2.0 * ic0 - 2.0 * i0 + s[0]);
+    const double del1 = fabs(2.0 * ic1 - 2.0 * i1 + s[1]);
+    const int f0 = fabs(d[0]) > 0.9, f1 = fabs(d[1]) > 0.9, f2 = fabs(d[2]) > 0.9;
+    w[0] = f2 ? del0 * del1 : 0;
+    w[1] = (f1 && f2) ? del0 * (1.0 - del1) : 0;
+    w[2] = (f0 && f2) ? (1.0 - del0) * del1 : 0;
+    w[3] = (f0 && f1 && f2) ? (1.0 - del0) * (1.0 - del1) : 0;
+    /* cellPathLength: distance over the largest |d| (ties to z, then y) */
+    double e[3];
+    for (int a = 0; a < 3; ++a) e[a] = fabs(xc[a] - xs[a]);
+    double denom;
+    if (e[2] >= e[1] && e[2] >= e[0]) denom = e[2];
+    else if (e[1] > e[2] && e[1] >= e[0]) denom = e[1];
+    else denom = e[0];
+    *ds = denom != 0 ? sqrt(e[0] * e[0] * dx * dx + e[1] * e[1] * dx * dx + e[2] * e[2] * dx * dx) / denom : 0.0;
+    double r2 = 0;
+    for (int a = 0; a < 3; ++a) r2 += (xc[a] - xs[a]) * (xc[a] - xs[a]) * dx * dx;
+    if (r2 <= 100.0 * (*ds) * (*ds)) {
+        double r = sqrt(r2), rmin = r - 0.5 * (*ds), rmax = r + 0.5 * (*ds);
+        *shell_vol = 4.0 * pi * (rmax * rmax * rmax - rmin * rmin * rmin) / 3.0;
+    } else {
+        *shell_vol = 4.0 * pi * r2 * (*ds);
+    }
+}
+
+void orc_star_geometry(const int n[3], const int star[3], double dx, double pi, int64_t* nb, double* w, double* ds,
+                       double* shell_vol) {
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) {
+                const int64_t c = i + (int64_t)n[0] * (j + (int64_t)n[1] * k);
+                orc_star_cell_geometry(n, star, dx, pi, i, j, k, nb + 4 * c, w + 4 * c, ds + c, shell_vol + c);
+            }
+}
+
+/* Thermodynamics::rayTrace + updateColDen (Thermodynamics.cpp:339-417):
+ * COL_DEN of every cell from its upstream neighbours' COL_DEN + DCOL_DEN
+ * (weights divided by the neighbour's column, renormalised), DCOL_DEN =
+ * (den / m_H) ds.  The reference visits the cells in its causal order
+ * (Grid::buildCausal); every upstream neighbour of a cell lies one
+ * Chebyshev shell closer to the star, so visiting the cells by increasing
+ * shell gives the same values. */
+void orc_thermo_ray_trace(const int n[3], const int star[3], const int64_t* nb, const double* w, const double* ds,
+                          double hydrogen_mass, const double* den, double* col, double* dcol) {
+    const int64_t ncell = (int64_t)n[0] * n[1] * n[2];
+    int kmax = 0;
+    for (int a = 0; a < 3; ++a) {
+        int m = star[a] > n[a] - 1 - star[a] ? star[a] : n[a] - 1 - star[a];
+        if (m > kmax) kmax = m;
+    }
+    for (int shell = 0; shell <= kmax; ++shell) {
+        for (int64_t c = 0; c < ncell; ++c) {
+            const int i = (int)(c % n[0]), j = (int)((c / n[0]) % n[1]), k = (int)(c / ((int64_t)n[0] * n[1]));
+            const int ijk[3] = {i, j, k};
+            int cheb = 0;
+            for (int a = 0; a < 3; ++a) {
+                int e = abs(ijk[a] - star[a]);
+                if (e > cheb) cheb = e;
+            }
+            if (cheb != shell) continue;
+            double dist2 = 0;
+            for (int a = 0; a < 3; ++a) {
+                double e = ((double)ijk[a] + 0.5) - ((double)star[a] + 0.5);
+                dist2 += e * e;
+            }
+            if (dist2 > 0.95 * 0.95) {
+                double cd[4] = {0, 0, 0, 0}, wr[4];
+                for (int q = 0; q < 4; ++q) {
+                    if (nb[4 * c + q] != -1) cd[q] = col[nb[4 * c + q]] + dcol[nb[4 * c + q]];
+                    wr[q] = cd[q] == 0 ? 0 : w[4 * c + q] / cd[q];
+                }
+                double sum_w = wr[0] + wr[1] + wr[2] + wr[3];
+                double v = 0.0;
+                for (int q = 0; q < 4 && sum_w != 0; ++q) {
+                    wr[q] = wr[q] / sum_w;
+                    v += wr[q] * cd[q];
+                }
+                col[c] = v;
+            } else {
+                col[c] = 0;
+            }
+            dcol[c] = (den[c] / hydrogen_mass) * ds[c];
+        }
+    }
+}
+
+/* Fluid::initialiseGrid's CausalWind test (Fluid.cpp:41-54): a cell is a
+ * wind cell when sum_d ((int)xc_d - p_d) (xc_d - p_d) <= R^2 (p the star's
+ * cell, R the wind radius in cells) or it is the star's cell. */
+void orc_star_wind(const int n[3], const int star[3], int wind_radius, unsigned char* wind) {
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) {
+                const int ijk[3] = {i, j, k};
+                double d2 = 0;
+                for (int a = 0; a < 3; ++a) d2 += (ijk[a] - star[a]) * (((double)ijk[a] + 0.5) - star[a]);
+                const int here = i == star[0] && j == star[1] && k == star[2];
+                wind[i + (int64_t)n[0] * (j + (int64_t)n[1] * k)] = (d2 <= wind_radius * wind_radius) || here;
+            }
+}
+
+/* The FUV flux of a star-on thermo preTimeStep (Thermodynamics.cpp:237-243,
+ * fluxFUV :113-118): F = 0.5 S / (1.2e7 * 4 pi r^2), r^2 = sum_d
+ * (xc_d - xs_d)^2 dx^2, and 0 at the star. */
+void orc_star_fuv(const int n[3], const int star[3], double dx, double pi, double photon_rate, double* F) {
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) {
+                const int ijk[3] = {i, j, k};
+                double r2 = 0;
+                for (int a = 0; a < 3; ++a) {
+                    double e = ((double)ijk[a] + 0.5) - ((double)star[a] + 0.5);
+                    r2 += e * e * dx * dx;
+                }
+                const double q = 0.5 * photon_rate;
+                F[i + (int64_t)n[0] * (j + (int64_t)n[1] * k)] = r2 != 0 ? q / (1.2e7 * 4 * pi * r2) : 0;
+            }
+}

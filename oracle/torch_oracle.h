@@ -137,6 +137,19 @@ void orc_thermo_cell(const orc_thermo_params* t, double den, double pre, double 
 void orc_thermo_batch(const orc_thermo_params* t, int64_t n, const double* in /* n x 9 */,
                       double* out /* n x 5 */, int nthreads);
 
+/* Star geometry (Grid::calculateNearestNeighbours, Radiation::cellPathLength,
+ * shellVolume) of one cell and of the whole grid (x fastest; nb, w 4 per
+ * cell), the CausalWind cells, and the thermo column-density ray trace
+ * (Thermodynamics::rayTrace); n = {nx, ny, nz}, star = its cell. */
+void orc_star_cell_geometry(const int n[3], const int star[3], double dx, double pi, int i, int j, int k,
+                            int64_t nb[4], double w[4], double* ds, double* shell_vol);
+void orc_star_geometry(const int n[3], const int star[3], double dx, double pi, int64_t* nb, double* w, double* ds,
+                       double* shell_vol);
+void orc_star_wind(const int n[3], const int star[3], int wind_radius, unsigned char* wind);
+void orc_star_fuv(const int n[3], const int star[3], double dx, double pi, double photon_rate, double* F);
+void orc_thermo_ray_trace(const int n[3], const int star[3], const int64_t* nb, const double* w, const double* ds,
+                          double hydrogen_mass, const double* den, double* col, double* dcol);
+
 #ifdef __cplusplus
 }
 #endif

@@ -190,6 +190,52 @@ def thermo():
     print("thermo", n)
 
 
+def raytrace():
+    """The star-on thermo preTimeStep on small grids: the reference's star
+    geometry (Radiation::initField -> Grid::calculateNearestNeighbours,
+    cellPathLength, shellVolume), CausalWind cells, the column-density ray
+    trace (Thermodynamics::rayTrace) and the pre-step RATE / HEAT.  Random
+    clumpy media in code units of the 1e-20/1e-28/1e11 scales; side 1 pc;
+    photon rate 1e49 /s."""
+    c = oracle.converter(1e-20, 1e-28, 1e11)
+    tc = lambda v, m, l, t: oracle.to_code(c, v, m, l, t)  # noqa: E731
+    cases = {"a": ((7, 6, 5), (2, 1, 4), 1), "b": ((8, 8, 8), (0, 0, 0), 0), "c": ((9, 10, 11), (7, 3, 2), 2),
+             "d": ((6, 6, 6), (5, 5, 5), 0)}
+    rng = np.random.default_rng(23)
+    out = {}
+    for name, (shape, star, wind) in cases.items():
+        nz, ny, nx = shape
+        rho = 10.0 ** rng.uniform(-24, -20, shape)
+        T = 10.0 ** rng.uniform(2, 4, shape)
+        hii = rng.uniform(0, 1, shape)
+        Q = np.zeros((7,) + shape)
+        Q[0] = rho * tc(1.0, 1, -3, 0)
+        Q[1] = rho * R_CGS * (hii + 1.0) * T * tc(1.0, 1, -1, -2)
+        Q[2] = hii
+        Q[3] = np.where(rng.uniform(0, 1, shape) < 0.1, 0.005, 1.0)
+        with tempfile.TemporaryDirectory() as d:
+            Q.tofile(f"{d}/q.bin")
+            kv = {"nx": nx, "ny": ny, "nz": nz, "side_length": 3.0857e18, "density_scale": 1e-20,
+                  "pressure_scale": 1e-28, "time_scale": 1e11, "gamma": 1.67, "density_floor": 1e-30,
+                  "temperature_floor": 0.1, "mass_fraction_hydrogen": 1.0, "thermo_hii_switch": 1e-2,
+                  "star_x": star[0], "star_y": star[1], "star_z": star[2], "wind_radius": wind,
+                  "photon_rate": 1e49, "input_q": f"{d}/q.bin", "output": f"{d}/out.bin"}
+            with open(f"{d}/case.txt", "w") as f:
+                for k, v in kv.items():
+                    f.write(f"{k} {v}\n")
+            r = subprocess.run([HARNESS, "raytrace", f"{d}/case.txt"], capture_output=True, text=True, check=True)
+            o = np.fromfile(f"{d}/out.bin").reshape(nz, ny, nx, 15)
+        info = json.loads(r.stdout.strip().splitlines()[-1])
+        out[f"{name}_shape"] = np.array(shape)
+        out[f"{name}_star"] = np.array(star)
+        out[f"{name}_wind_radius"] = np.array(wind)
+        out[f"{name}_Q"] = Q
+        out[f"{name}_out"] = o
+        out[f"{name}_info"] = np.array([info["photon_rate"], info["hydrogen_mass"], info["pi"], info["dx"]])
+    np.savez_compressed(f"{HERE}/raytrace.npz", **out)
+    print("raytrace", list(cases))
+
+
 if __name__ == "__main__":
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle ref")
