@@ -34,6 +34,9 @@
  *   Torch::initialise start time from initial conditions  torchgpu_set_time
  *     (Torch.cpp:100-103)
  *   DataPrinter::print2D, DataReader (src/IO/)            torchgpu_snapshot_*
+ *   Star-on thermo preTimeStep: Thermodynamics::rayTrace  torchgpu_cooling_set_star,
+ *     (Thermodynamics.cpp:218-262,339-417), star geometry   torchgpu_star_geometry,
+ *     (Grid.cpp:922-951, Radiation.cpp:78-138)              torchgpu_thermo_grid_inputs
  *   MLP surrogate of the subcycled update (no reference   torchgpu_mlp_*,
  *     counterpart; FP64 integrate remains the fallback)     torchgpu_thermo_cells_surrogate
  *
@@ -283,6 +286,39 @@ int torchgpu_thermo_grid_substep(torchgpu_ctx* ctx, torchgpu_cooling* c, const d
  * end the run and surface at the next synchronizing call. */
 int torchgpu_run_split(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* T_min, const double* col_den,
                        const double* F_FUV, int64_t excluded, int nsteps, int surrogate);
+
+/* A star on the grid (Parameters.Star: cell_position_x/y/z,
+ * wind_radius_in_cells, photon_rate; src/main.cpp:192-204), at the centre
+ * of its cell (snap_to_face_left_* off).  photon_rate in code units. */
+typedef struct torchgpu_star {
+    int cell[3];
+    int wind_cell_radius;
+    double photon_rate;
+} torchgpu_star;
+/* Star-on thermodynamics (Star.on = true) for the grid of ctx: from now on
+ * the grid thermo calls (grid_timestep, grid_substep, run_split,
+ * grid_inputs) compute COL_DEN by the causal column-density ray trace of
+ * Thermodynamics::rayTrace (Thermodynamics.cpp:339-417, on the state's
+ * fixed primitives) where no col_den array is passed, take F_FUV =
+ * fluxFUV(0.5 photon_rate, r^2) (Thermodynamics.cpp:237-243) where no F_FUV
+ * array is passed, and skip the CausalWind cells (the star's cell and those
+ * within the wind radius, Fluid.cpp:41-54) as preTimeStepCalculations does.
+ * The neighbour geometry is Grid::calculateNearestNeighbours
+ * (Grid.cpp:922-951) and Radiation::cellPathLength (Radiation.cpp:102-117).
+ * NULL switches the star off.  The star cell must lie inside the grid. */
+int torchgpu_cooling_set_star(torchgpu_cooling* c, torchgpu_ctx* ctx, const torchgpu_star* star);
+/* The star's per-cell geometry, as Radiation::initField sets it
+ * (Radiation.cpp:78-96): the four upstream neighbours (flat indices, -1
+ * none) and weights (int64 / double [n][4]), the path length ds and the
+ * shell volume ([n]); host arrays unless `on_device`.  Synchronises. */
+int torchgpu_star_geometry(torchgpu_cooling* c, int64_t* nb, double* w, double* ds, double* shell_vol, int on_device);
+/* The per-cell inputs [9][n] (den, pre, hii, adv (-1: skipped), E, T_min,
+ * COL_DEN, F_FUV, dt = 0) the grid thermo operator forms for the current
+ * state, including the ray-traced COL_DEN and, into dcol_den [n] when not
+ * NULL, its DCOL_DEN (Thermodynamics::preTimeStepCalculations' inputs).
+ * Synchronises. */
+int torchgpu_thermo_grid_inputs(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* T_min, const double* col_den,
+                                const double* F_FUV, int64_t excluded, double* in9, double* dcol_den, int on_device);
 
 /* ------------------------------------------------------------------------ */
 /* Cooling surrogate (no reference counterpart: a learned replacement of the  */

@@ -255,6 +255,22 @@ class ThermoParams(ctypes.Structure):
         ("subcycling", ctypes.c_int), ("pad", ctypes.c_int)]
 
 
+class Star(ctypes.Structure):
+    """torchgpu_star: Parameters.Star cell position, wind radius (cells) and
+    photon rate (code units)."""
+    _fields_ = [("cell", ctypes.c_int * 3), ("wind_cell_radius", ctypes.c_int), ("photon_rate", ctypes.c_double)]
+
+
+def make_star(cell, wind_cell_radius: int = 0, photon_rate: float = 0.0) -> Star:
+    """cell = (i, j, k), x first."""
+    s = Star()
+    for a in range(3):
+        s.cell[a] = int(cell[a])
+    s.wind_cell_radius = int(wind_cell_radius)
+    s.photon_rate = float(photon_rate)
+    return s
+
+
 def _bind_cooling(L):
     vp, d, i = ctypes.c_void_p, ctypes.c_double, ctypes.c_int
     L.torchgpu_units.argtypes = [d, d, d, d, d, d, d, i, ctypes.POINTER(d)]
@@ -267,6 +283,9 @@ def _bind_cooling(L):
     L.torchgpu_thermo_grid_timestep.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int64, d, ctypes.POINTER(d)]
     L.torchgpu_thermo_grid_substep.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int64, d, i]
     L.torchgpu_run_split.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int64, i, i]
+    L.torchgpu_cooling_set_star.argtypes = [vp, vp, ctypes.POINTER(Star)]
+    L.torchgpu_star_geometry.argtypes = [vp, vp, vp, vp, vp, i]
+    L.torchgpu_thermo_grid_inputs.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int64, vp, vp, i]
 
 
 def units(scales, value, m, l, t, to_code=True) -> float:
@@ -458,6 +477,43 @@ class Cooling:
         the Strang alternation on the device (no host round trip)."""
         self._check(lib().torchgpu_run_split(grid._h, self._h, self._dptr(T_min), self._dptr(col_den),
                                              self._dptr(F_FUV), excluded, nsteps, int(surrogate)))
+
+    def set_star(self, grid: "Grid", star: "Star | None"):
+        """Star-on thermodynamics on `grid` (torchgpu_cooling_set_star): ray
+        traced COL_DEN, the star's F_FUV, CausalWind cells skipped; None
+        switches it off."""
+        self._check(lib().torchgpu_cooling_set_star(self._h, grid._h, None if star is None else ctypes.byref(star)))
+        p = grid.params
+        self._star_n = None if star is None else p.nx * p.ny * p.nz
+
+    def star_geometry(self):
+        """(nb int64 [n, 4], w [n, 4], ds [n], shell_vol [n]) of the star set
+        on this handle (host arrays)."""
+        import numpy as np
+
+        n = getattr(self, "_star_n", None)
+        if n is None:
+            raise TorchGpuError(1, "star_geometry: no star set")
+        nb = np.zeros((n, 4), dtype=np.int64)
+        w = np.zeros((n, 4))
+        ds = np.zeros(n)
+        sv = np.zeros(n)
+        self._check(lib().torchgpu_star_geometry(self._h, nb.ctypes.data, w.ctypes.data, ds.ctypes.data,
+                                                 sv.ctypes.data, 0))
+        return nb, w, ds, sv
+
+    def grid_inputs(self, grid: "Grid", T_min=None, col_den=None, F_FUV=None, excluded: int = -1):
+        """(in9 [9, n], DCOL_DEN [n]) the grid thermo operator forms for the
+        current state (host arrays)."""
+        import numpy as np
+
+        n = grid.params.nx * grid.params.ny * grid.params.nz
+        in9 = np.zeros((9, n))
+        dcol = np.zeros(n)
+        self._check(lib().torchgpu_thermo_grid_inputs(grid._h, self._h, self._dptr(T_min), self._dptr(col_den),
+                                                      self._dptr(F_FUV), excluded, in9.ctypes.data, dcol.ctypes.data,
+                                                      0))
+        return in9, dcol
 
     def thermo_cells(self, inp, out=None, count=True):
         """inp: device float64 tensor [9][n]; returns out [5][n] (device) and

@@ -1,0 +1,153 @@
+// raytrace.hip -- the causal column-density ray trace of the star-on thermo
+// operator (Thermodynamics::rayTrace + updateColDen,
+// src/Integrators/Thermodynamics.cpp:339-417) and the star geometry kernel.
+//
+// The reference walks the cells in a causal order (Grid::buildCausal) so
+// that a cell's four upstream neighbours (Grid::calculateNearestNeighbours)
+// are done before it.  Those neighbours always lie exactly one Chebyshev
+// shell (max_d |i_d - star_d|) closer to the star: one step along the ray
+// plane and at most one along the other axes, whose offsets are no larger.
+// So the GPU sweeps the shells outwards, one launch per shell, every cell
+// of a shell in parallel: shell k reads only shell k - 1, written by the
+// previous launch on the same stream.  Neighbours of zero weight may be in
+// the same shell (ties between axes); their values are multiplied by zero
+// and drop out, as in the reference, so a concurrent write there cannot
+// change a result (the buffers are kept finite).
+//
+// Per cell: 8 B density read, 2 x 8 B written, and 4 x 16 B neighbour reads
+// that hit in L2 (the previous shell was just written).
+#include <hip/hip_runtime.h>
+
+#include "star_device.hpp"
+#include "torchgpu_internal.hpp"
+
+namespace tg {
+
+__device__ __forceinline__ void shell_cell(const ShellFaces& f, long long t, int ijk[3]) {
+    int q = 0;
+    while (q + 1 < f.nf && t >= f.start[q + 1]) ++q;
+    const long long l = t - f.start[q];
+    const int a = f.axis[q];
+    ijk[a] = f.fixed[q];
+    ijk[a == 0 ? 1 : 0] = f.lo0[q] + (int)(l % f.w0[q]);
+    ijk[a == 2 ? 1 : 2] = f.lo1[q] + (int)(l / f.w0[q]);
+}
+
+// updateColDen (Thermodynamics.cpp:339-363) for the cells of one shell.
+__global__ __launch_bounds__(256) void coldens_shell_kernel(StarGeom s, ShellFaces f, const double* __restrict__ den,
+                                                            double* col, double* dcol) {
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= f.start[f.nf]) return;
+    int ijk[3];
+    shell_cell(f, t, ijk);
+    const long long c = star_flat(s, ijk[0], ijk[1], ijk[2]);
+    // distance to the star in cell units
+    double dist2 = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double e = star_offset(s, ijk, a);
+        dist2 += e * e;
+    }
+    double v = 0;
+    if (dist2 > 0.95 * 0.95) {
+        long long nb[4];
+        double w[4], cd[4], wr[4];
+        star_neighbours(s, ijk, nb, w);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            cd[q] = nb[q] != -1 ? col[nb[q]] + dcol[nb[q]] : 0.0;
+            wr[q] = cd[q] == 0 ? 0.0 : w[q] / cd[q];
+        }
+        const double sum_w = wr[0] + wr[1] + wr[2] + wr[3];
+        if (sum_w != 0) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v += (wr[q] / sum_w) * cd[q];
+        }
+    }
+    col[c] = v;
+    dcol[c] = (den[c] / s.hydrogen_mass) * star_path_length(s, ijk);
+}
+
+__global__ void star_geometry_kernel(StarGeom s, long long* __restrict__ nb, double* __restrict__ w,
+                                     double* __restrict__ ds, double* __restrict__ shell_vol) {
+    const long long n = (long long)s.n[0] * s.n[1] * s.n[2];
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+        const int ijk[3] = {(int)(c % s.n[0]), (int)((c / s.n[0]) % s.n[1]), (int)(c / ((long long)s.n[0] * s.n[1]))};
+        long long b[4];
+        double wq[4];
+        star_neighbours(s, ijk, b, wq);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            nb[4 * c + q] = b[q];
+            w[4 * c + q] = wq[q];
+        }
+        const double d = star_path_length(s, ijk);
+        ds[c] = d;
+        shell_vol[c] = star_shell_volume(s, ijk, d);
+    }
+}
+
+namespace {
+
+// Faces of shell k clipped to the grid (see ShellFaces).
+void shell_faces(const StarGeom& s, int k, ShellFaces& f) {
+    f.nf = 0;
+    f.start[0] = 0;
+    auto add = [&](int axis, int fixed, int lo0, int hi0, int lo1, int hi1) {
+        const int b0 = axis == 0 ? 1 : 0, b1 = axis == 2 ? 1 : 2;
+        if (fixed < 0 || fixed >= s.n[axis]) return;
+        lo0 = lo0 < 0 ? 0 : lo0;
+        lo1 = lo1 < 0 ? 0 : lo1;
+        hi0 = hi0 > s.n[b0] - 1 ? s.n[b0] - 1 : hi0;
+        hi1 = hi1 > s.n[b1] - 1 ? s.n[b1] - 1 : hi1;
+        if (hi0 < lo0 || hi1 < lo1) return;
+        const int q = f.nf++;
+        f.axis[q] = axis;
+        f.fixed[q] = fixed;
+        f.lo0[q] = lo0;
+        f.lo1[q] = lo1;
+        f.w0[q] = hi0 - lo0 + 1;
+        f.start[q + 1] = f.start[q] + (long long)f.w0[q] * (hi1 - lo1 + 1);
+    };
+    const int x = s.s[0], y = s.s[1], z = s.s[2];
+    if (k == 0) {
+        add(0, x, y, y, z, z);
+        return;
+    }
+    for (int sgn = -1; sgn <= 1; sgn += 2) add(0, x + sgn * k, y - k, y + k, z - k, z + k);
+    for (int sgn = -1; sgn <= 1; sgn += 2) add(1, y + sgn * k, x - k + 1, x + k - 1, z - k, z + k);
+    for (int sgn = -1; sgn <= 1; sgn += 2) add(2, z + sgn * k, x - k + 1, x + k - 1, y - k + 1, y + k - 1);
+}
+
+}  // namespace
+
+hipError_t launch_thermo_ray_trace(const StarGeom& s, const double* den, double* col, double* dcol,
+                                   hipStream_t stream) {
+    int kmax = 0;
+    for (int a = 0; a < 3; ++a) {
+        const int m = s.s[a] > s.n[a] - 1 - s.s[a] ? s.s[a] : s.n[a] - 1 - s.s[a];
+        kmax = m > kmax ? m : kmax;
+    }
+    ShellFaces f;
+    for (int k = 0; k <= kmax; ++k) {
+        shell_faces(s, k, f);
+        const long long cells = f.start[f.nf];
+        if (cells == 0) continue;
+        const long long nb = (cells + 255) / 256;
+        hipLaunchKernelGGL(coldens_shell_kernel, dim3((unsigned)nb), dim3(256), 0, stream, s, f, den, col, dcol);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_star_geometry(const StarGeom& s, long long* nb, double* w, double* ds, double* shell_vol,
+                                hipStream_t stream) {
+    const long long n = (long long)s.n[0] * s.n[1] * s.n[2];
+    long long b = (n + 255) / 256;
+    if (b > 65536) b = 65536;
+    hipLaunchKernelGGL(star_geometry_kernel, dim3((unsigned)b), dim3(256), 0, stream, s, nb, w, ds, shell_vol);
+    return hipGetLastError();
+}
+
+}  // namespace tg

@@ -37,6 +37,7 @@
 #include "../../include/torchgpu.h"
 #include "hydro_math.hpp"
 #include "mlp.hpp"
+#include "star_device.hpp"
 #include "thermo_math.hpp"
 #include "torchgpu_internal.hpp"
 
@@ -436,13 +437,15 @@ __global__ void surrogate_targets_kernel(const unsigned long long* __restrict__ 
 // ---------------------------------------------------------------------------
 // Per-cell inputs of the cooling kernels from the grid: Q =
 // fixPrimitives(QfromU(U)) (Fluid.cpp:138-165), E = U[PRE], T_min, COL_DEN,
-// F_FUV (arrays or defaults) and dt.  The excluded cell (the star's cell,
-// which Fluid::initialiseGrid puts in CausalWind, Fluid.cpp:41-54) gets
-// ADV = -1 so that the gate makes it a no-op.
+// F_FUV (arrays or defaults) and dt.  The excluded cell gets ADV = -1 so
+// that the gate makes it a no-op; with a star, so do its CausalWind cells
+// (Fluid::initialiseGrid, Fluid.cpp:41-54), which the reference's
+// preTimeStepCalculations skips, and F_FUV is the star's when no array is
+// given (Thermodynamics.cpp:237-243).
 __global__ void thermo_gather_kernel(HydroGeom g, const double* __restrict__ U, const double* __restrict__ T_min,
                                      double T_min_default, const double* __restrict__ col_den,
                                      const double* __restrict__ F, double dt, const StepState* st,
-                                     long long excluded, double* __restrict__ in9) {
+                                     long long excluded, StarGeom star, double* __restrict__ in9) {
     const long long n = g.ncell;
     // DT_HALF / DT_FULL: the sub-step of the step in flight
     if (dt < 0.0) dt = (dt == DT_HALF) ? 0.5 * st->dt : st->dt;
@@ -461,11 +464,18 @@ __global__ void thermo_gather_kernel(HydroGeom g, const double* __restrict__ U, 
         in9[0 * n + c] = q[DEN];
         in9[1 * n + c] = q[PRE];
         in9[2 * n + c] = q[HII];
-        in9[3 * n + c] = (c == excluded) ? -1.0 : q[ADV];
+        bool skip = c == excluded;
+        double f = F ? F[c] : 0.0;
+        if (star.on) {
+            const int ijk[3] = {(int)(c % g.nx), (int)((c / g.nx) % g.ny), (int)(c / ((long long)g.nx * g.ny))};
+            skip = skip || star_wind_cell(star, ijk);
+            if (!F) f = star_fuv(star, ijk);
+        }
+        in9[3 * n + c] = skip ? -1.0 : q[ADV];
         in9[4 * n + c] = u[PRE];
         in9[5 * n + c] = T_min ? T_min[c] : T_min_default;
         in9[6 * n + c] = col_den ? col_den[c] : 0.0;
-        in9[7 * n + c] = F ? F[c] : 0.0;
+        in9[7 * n + c] = f;
         in9[8 * n + c] = dt;
     }
 }
@@ -503,6 +513,11 @@ __global__ __launch_bounds__(256) void thermo_timestep_kernel(ThermoConsts t, co
 }
 
 __global__ void set_u64_kernel(unsigned long long* p, unsigned long long v) { *p = v; }
+
+__global__ void zero_f64_kernel(double* p, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
 
 // ---------------------------------------------------------------------------
 // Longest-first work order for the persistent FP64 kernel.  A cell's cost is
@@ -611,6 +626,11 @@ struct torchgpu_cooling {
     double* in9 = nullptr;
     double* out5 = nullptr;
     unsigned long long* dtbits = nullptr;
+    // star (torchgpu_cooling_set_star): geometry and the DCOL_DEN of the
+    // ray trace (COL_DEN goes straight into in9)
+    tg::StarGeom star{};
+    double* dcol = nullptr;
+    long long dcol_cap = 0;
 };
 
 // the MLP handle is defined in mlp_api.cpp; the cooling operator only needs
@@ -768,7 +788,7 @@ void torchgpu_cooling_destroy(torchgpu_cooling* c) {
     (void)hipStreamSynchronize(c->stream);
     for (void* ptr : {(void*)c->tables, (void*)c->counters, (void*)c->err, (void*)c->mlp_list, (void*)c->fb_list,
                       (void*)c->feat, (void*)c->y, (void*)c->sdata, (void*)c->in9, (void*)c->out5, (void*)c->dtbits,
-                      (void*)c->lpt_order, (void*)c->lpt_bucket, (void*)c->lpt_hist})
+                      (void*)c->lpt_order, (void*)c->lpt_bucket, (void*)c->lpt_hist, (void*)c->dcol})
         if (ptr) (void)hipFree(ptr);
     delete c;
 }
@@ -891,6 +911,9 @@ static int ensure_grid(torchgpu_cooling* c, long long n) {
     c->grid_cap = 0;
     HIP_OK(hipMalloc(&c->in9, 9 * n * sizeof(double)));
     HIP_OK(hipMalloc(&c->out5, 5 * n * sizeof(double)));
+    // finite contents everywhere: the ray trace reads columns of zero weight
+    hipLaunchKernelGGL(tg::zero_f64_kernel, dim3(1024), dim3(256), 0, c->stream, c->in9, 9 * n);
+    HIP_OK(hipGetLastError());
     if (!c->dtbits) HIP_OK(hipMalloc(&c->dtbits, sizeof(unsigned long long)));
     c->grid_cap = n;
     return TORCHGPU_OK;
@@ -904,9 +927,16 @@ static int gather(torchgpu_cooling* c, torchgpu_ctx* ctx, const double* T_min, c
     if (rc) return rc;
     long long nb = (n + 255) / 256;
     if (nb > 65536) nb = 65536;
+    if (c->star.on && (c->star.n[0] != v.g->nx || c->star.n[1] != v.g->ny || c->star.n[2] != v.g->nz))
+        return TORCHGPU_ERR_INVALID_ARG;
     hipLaunchKernelGGL(tg::thermo_gather_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, *v.g, v.U, T_min,
-                       100.0, col_den, F, dt, v.st, excluded, c->in9);
+                       100.0, col_den, F, dt, v.st, excluded, c->star, c->in9);
     HIP_OK(hipGetLastError());
+    if (c->star.on && !col_den) {
+        // Thermodynamics::preTimeStepCalculations starts with rayTrace on
+        // the fixed primitives (Thermodynamics.cpp:218-220)
+        HIP_OK(tg::launch_thermo_ray_trace(c->star, c->in9, c->in9 + 6 * n, c->dcol, c->stream));
+    }
     return TORCHGPU_OK;
 }
 
@@ -962,6 +992,87 @@ int torchgpu_thermo_grid_substep(torchgpu_ctx* ctx, torchgpu_cooling* c, const d
     if (!ctx || !c || !(dt >= 0.0)) return TORCHGPU_ERR_INVALID_ARG;
     if (surrogate && !c->has_surrogate) return TORCHGPU_ERR_NOT_TRAINED;
     return grid_substep_dev(ctx, c, T_min, col_den, F, excluded, dt, surrogate);
+}
+
+int torchgpu_cooling_set_star(torchgpu_cooling* c, torchgpu_ctx* ctx, const torchgpu_star* star) {
+    if (!c || !ctx) return TORCHGPU_ERR_INVALID_ARG;
+    if (!star) {
+        c->star = tg::StarGeom{};
+        return TORCHGPU_OK;
+    }
+    const tg::HydroGeom& g = *tg::ctx_view(ctx).g;
+    const int n[3] = {g.nx, g.ny, g.nz};
+    for (int a = 0; a < 3; ++a)
+        if (star->cell[a] < 0 || star->cell[a] >= n[a]) {
+            snprintf(c->msg, sizeof c->msg, "star cell (%d, %d, %d) outside the %d x %d x %d grid", star->cell[0],
+                     star->cell[1], star->cell[2], n[0], n[1], n[2]);
+            return TORCHGPU_ERR_INVALID_ARG;
+        }
+    if (star->wind_cell_radius < 0 || !(star->photon_rate >= 0.0)) return TORCHGPU_ERR_INVALID_ARG;
+    const long long ncell = g.ncell;
+    if (ncell > c->dcol_cap) {
+        (void)hipStreamSynchronize(c->stream);
+        if (c->dcol) (void)hipFree(c->dcol);
+        c->dcol = nullptr;
+        c->dcol_cap = 0;
+        HIP_OK(hipMalloc(&c->dcol, ncell * sizeof(double)));
+        c->dcol_cap = ncell;
+    }
+    hipLaunchKernelGGL(tg::zero_f64_kernel, dim3(1024), dim3(256), 0, c->stream, c->dcol, ncell);
+    HIP_OK(hipGetLastError());
+    tg::StarGeom s{};
+    s.on = 1;
+    for (int a = 0; a < 3; ++a) {
+        s.s[a] = star->cell[a];
+        s.n[a] = n[a];
+    }
+    s.wind_r = star->wind_cell_radius;
+    s.dx = g.dx;
+    s.photon_rate = star->photon_rate;
+    s.pi = c->p.pi;
+    s.hydrogen_mass = c->p.hydrogen_mass;
+    c->star = s;
+    return TORCHGPU_OK;
+}
+
+int torchgpu_star_geometry(torchgpu_cooling* c, int64_t* nb, double* w, double* ds, double* shell_vol, int on_device) {
+    if (!c || !nb || !w || !ds || !shell_vol || !c->star.on) return TORCHGPU_ERR_INVALID_ARG;
+    const long long n = (long long)c->star.n[0] * c->star.n[1] * c->star.n[2];
+    long long* d_nb = reinterpret_cast<long long*>(nb);
+    double *d_w = w, *d_ds = ds, *d_sv = shell_vol;
+    void* tmp = nullptr;
+    if (!on_device) {
+        HIP_OK(hipMalloc(&tmp, n * (4 * sizeof(long long) + 6 * sizeof(double))));
+        d_nb = static_cast<long long*>(tmp);
+        d_w = reinterpret_cast<double*>(d_nb + 4 * n);
+        d_ds = d_w + 4 * n;
+        d_sv = d_ds + n;
+    }
+    hipError_t e = tg::launch_star_geometry(c->star, d_nb, d_w, d_ds, d_sv, c->stream);
+    if (e == hipSuccess && !on_device) {
+        e = hipMemcpyAsync(nb, d_nb, 4 * n * sizeof(long long), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(w, d_w, 4 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(ds, d_ds, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(shell_vol, d_sv, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (tmp) (void)hipFree(tmp);
+    HIP_OK(e);
+    return TORCHGPU_OK;
+}
+
+int torchgpu_thermo_grid_inputs(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* T_min, const double* col_den,
+                                const double* F, int64_t excluded, double* in9, double* dcol_den, int on_device) {
+    if (!ctx || !c || !in9) return TORCHGPU_ERR_INVALID_ARG;
+    int rc = gather(c, ctx, T_min, col_den, F, excluded, 0.0);
+    if (rc) return rc;
+    const long long n = tg::ctx_view(ctx).g->ncell;
+    const hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    HIP_OK(hipMemcpyAsync(in9, c->in9, 9 * n * sizeof(double), k, c->stream));
+    if (dcol_den && c->star.on && !col_den)
+        HIP_OK(hipMemcpyAsync(dcol_den, c->dcol, n * sizeof(double), k, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return check_err(c);
 }
 
 int torchgpu_run_split(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* T_min, const double* col_den,

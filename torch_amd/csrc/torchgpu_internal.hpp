@@ -30,6 +30,30 @@ struct HydroGeom {
     double igm1, idx;       // 1 / gm1, 1 / dx (mixed path)
 };
 
+// A star at a cell centre of a 3-D grid (cell units: cells at index + 0.5,
+// the star at its cell + 0.5), with its wind radius and photon rate, for the
+// star-on thermo operator (Star::initialise without face snapping,
+// Star.cpp:17-21; Fluid::initialiseGrid, Fluid.cpp:27-57).
+struct StarGeom {
+    int on;
+    int s[3];               // the star's cell
+    int n[3];               // nx, ny, nz
+    int wind_r;             // Star.wind_radius_in_cells
+    double dx;              // cell width (code units)
+    double photon_rate;     // code units
+    double pi;              // Constants::pi
+    double hydrogen_mass;   // code units
+};
+
+// One Chebyshev shell of cells around the star as up to six faces clipped to
+// the grid: face q fixes coordinate `axis[q]` at `fixed[q]` and spans the two
+// other axes (lower index first) from lo0/lo1, w0 cells along the first.
+struct ShellFaces {
+    int nf;
+    int axis[6], fixed[6], lo0[6], lo1[6], w0[6];
+    long long start[7];     // prefix sums of the face sizes
+};
+
 // Device-resident state of a time-stepping run (one per context).
 struct StepState {
     double dt;              // dt of the step in flight
@@ -131,6 +155,12 @@ hipError_t launch_step_end(StepState* st, hipStream_t stream);
 // Clock control for Torch::run's checkpoint printing: set the time (and the
 // checkpoint count Checkpointer::update gives for it), resume a paused run.
 hipError_t launch_set_time(StepState* st, double t, hipStream_t stream);
+// The column-density ray trace (Thermodynamics::rayTrace): one launch per
+// shell, den/col/dcol [n] device arrays; the star's geometry per cell.
+hipError_t launch_thermo_ray_trace(const StarGeom& s, const double* den, double* col, double* dcol,
+                                   hipStream_t stream);
+hipError_t launch_star_geometry(const StarGeom& s, long long* nb, double* w, double* ds, double* shell_vol,
+                                hipStream_t stream);
 hipError_t launch_copy_if_running(const StepState* st, const double* src, double* dst, long long n,
                                   hipStream_t stream);
 hipError_t launch_resume(StepState* st, hipStream_t stream);
