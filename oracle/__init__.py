@@ -112,6 +112,16 @@ def lib() -> ctypes.CDLL:
         L.orc_star_wind.argtypes = [i3, i3, ctypes.c_int, ctypes.POINTER(ctypes.c_ubyte)]
         L.orc_thermo_ray_trace.argtypes = [i3, i3, i64, dp, dp, d, dp, dp, dp]
         L.orc_star_fuv.argtypes = [i3, i3, d, d, d, dp]
+        L.orc_rad_init.argtypes = [ctypes.POINTER(RadParams), ctypes.POINTER(Converter), dp, d, d, d, d, d, d,
+                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, d, d, d, d]
+        L.orc_rad_recombination.argtypes = [ctypes.POINTER(RadParams), d]
+        L.orc_rad_recombination.restype = d
+        L.orc_first_nonwind_cell.argtypes = [i3, i3, ctypes.c_int]
+        L.orc_first_nonwind_cell.restype = ctypes.c_int64
+        L.orc_rad_init_field.argtypes = [ctypes.POINTER(RadParams), i3, i3, d, d, dp, dp]
+        L.orc_rad_run.argtypes = [ctypes.POINTER(RadParams), P(HydroParams), i3, ctypes.c_int, d,
+                                  ctypes.POINTER(ctypes.c_int), dp, dp, dp, ctypes.c_int, dp]
+        L.orc_rad_run.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -326,3 +336,65 @@ def star_fuv(shape, star, dx: float, pi: float, photon_rate: float) -> np.ndarra
     F = np.zeros((nz, ny, nx))
     lib().orc_star_fuv(_i3((nx, ny, nz)), _i3(star), dx, pi, photon_rate, _dp(F))
     return F
+
+
+class RadParams(ctypes.Structure):
+    _fields_ = ([(k, ctypes.c_double * 4) for k in ("K",)]
+                + [(k, ctypes.c_double) for k in ("sigma", "alphaB", "tau0", "minX", "THI", "THII")]
+                + [(k, ctypes.c_int) for k in ("scheme", "coupling", "collisions_on")]
+                + [(k, ctypes.c_double) for k in ("heating_amplification", "mass_frac_h", "hydrogen_mass",
+                                                  "spec_gas_const", "boltzmann", "rydberg", "voronov_A",
+                                                  "voronov_X", "voronov_K", "voronov_P", "voronov_DE",
+                                                  "photon_energy", "photon_rate")]
+                + [("rec_x", ctypes.c_double * 31), ("rec_y", ctypes.c_double * 31), ("rec_y2", ctypes.c_double * 31),
+                   ("rec_slope", ctypes.c_double * 2), ("cool_x", ctypes.c_double * 31),
+                   ("cool_y", ctypes.c_double * 31), ("cool_y2", ctypes.c_double * 31),
+                   ("cool_slope", ctypes.c_double * 2)])
+
+
+SCHEME = {"implicit": 0, "implicit2": 1, "explicit": 2}
+COUPLING = {"tti": 0, "neq": 1, "off": 2}
+
+
+def rad_params(c: Converter, K=(0.0, 0.0, 0.0, 0.0), sigma=6.3e-18, alphaB=2.59e-13, tau0=0.6, minX=0.0, THI=300.0,
+               THII=1e4, scheme="implicit", coupling="off", collisions_on=False, heating_amplification=1.0,
+               mass_frac_h=1.0, photon_energy=0.0, photon_rate=0.0) -> RadParams:
+    """Radiation::initialise with the Lua-level (cgs) values; unknown scheme /
+    coupling strings fall back to implicit / off as the reference's."""
+    r = RadParams()
+    Ka = (ctypes.c_double * 4)(*K)
+    lib().orc_rad_init(ctypes.byref(r), ctypes.byref(c), Ka, sigma, alphaB, tau0, minX, THI, THII,
+                       SCHEME.get(scheme, 0), COUPLING.get(coupling, 2), int(collisions_on), heating_amplification,
+                       mass_frac_h, photon_energy, photon_rate)
+    return r
+
+
+def first_nonwind_cell(shape, star, wind_radius: int) -> int:
+    nz, ny, nx = shape
+    return lib().orc_first_nonwind_cell(_i3((nx, ny, nz)), _i3(star), int(wind_radius))
+
+
+def rad_init_field(r: RadParams, shape, star, dx: float, Q: np.ndarray) -> np.ndarray:
+    """Radiation::initField: R [6][nz][ny][nx] (RID order) from the fixed
+    primitives Q."""
+    nz, ny, nx = shape
+    Q = np.ascontiguousarray(Q, dtype=np.float64)
+    R = np.zeros((6, nz, ny, nx))
+    lib().orc_rad_init_field(ctypes.byref(r), _i3((nx, ny, nz)), _i3(star), dx, 3.14159265359, _dp(Q), _dp(R))
+    return R
+
+
+def rad_run(r: RadParams, p: HydroParams, star, wind_radius: int, U: np.ndarray, R: np.ndarray, nsteps: int,
+            T_min: float = 100.0, first: bool = True):
+    """The radiation-only step sequence (preTimeStep, dt, two Torch::subStep
+    calls per step).  Returns (U, R, Q[HII, ADV] [2][...], dts, first)."""
+    U = np.ascontiguousarray(U, dtype=np.float64).copy()
+    R = np.ascontiguousarray(R, dtype=np.float64).copy()
+    Qo = np.zeros((2,) + U.shape[1:])
+    dts = np.zeros(nsteps)
+    f = ctypes.c_int(int(first))
+    err = lib().orc_rad_run(ctypes.byref(r), ctypes.byref(p), _i3(star), int(wind_radius), T_min, ctypes.byref(f),
+                            _dp(U), _dp(R), _dp(Qo), nsteps, _dp(dts))
+    if err:
+        raise RuntimeError(f"radiation run failed (code {err})")
+    return U, R, Qo, dts, bool(f.value)

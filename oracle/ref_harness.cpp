@@ -16,6 +16,7 @@
 //   thermo  <case.txt>              per-cell preTimeStep + integrate
 //   consts  <dscale> <pscale> <tscale>   print unit-converted constants
 //   raytrace <case.txt>             star geometry + thermo column densities
+//   radiation <case.txt>            Radiation integrator sub-steps
 
 #include <algorithm>
 #include <array>
@@ -124,6 +125,20 @@ static void init_sim(Sim& s, const KV& kv) {
     p.photonRate = getd(kv, "photon_rate", 0.0);
     p.star_position = {{geti(kv, "star_x", 0), geti(kv, "star_y", 0), geti(kv, "star_z", 0)}};
     p.windCellRadius = geti(kv, "wind_radius", 0);
+    p.photonEnergy = getd(kv, "photon_energy", 0.0);
+    p.K1 = getd(kv, "K1", 0.0);
+    p.K2 = getd(kv, "K2", 0.0);
+    p.K3 = getd(kv, "K3", 0.0);
+    p.K4 = getd(kv, "K4", 0.0);
+    p.photoIonCrossSection = getd(kv, "photoion_cross_section", 0.0);
+    p.alphaB = getd(kv, "case_b_recombination_coeff", 0.0);
+    p.tau0 = getd(kv, "tau_0", 0.0);
+    p.minX = getd(kv, "minimum_hii_fraction", 0.0);
+    p.THI = getd(kv, "temperature_hi", 0.0);
+    p.THII = getd(kv, "temperature_hii", 0.0);
+    p.collisions_on = geti(kv, "collisions_on", 0) != 0;
+    p.rt_coupling = gets(kv, "coupling", "off");
+    p.rt_scheme = gets(kv, "integration_scheme", "implicit");
 
     s.consts = std::make_shared<Constants>();
     s.consts->initialise_DPT(p.dscale, p.pscale, p.tscale);
@@ -446,6 +461,7 @@ static int run_raytrace(const KV& kv_in) {
     for (int c = 0; c < n; ++c) {
         GridCell& cell = grid.getCell(c);
         for (int v = 0; v < 7; ++v) cell.Q[v] = q[(size_t)v * n + c];
+        cell.heatCapacityRatio = s.fluid.heatCapacityRatio;  // Torch.cpp:184,224
     }
     s.fluid.globalUfromQ();
     s.thermo.initialiseMinTempField(s.fluid);
@@ -474,6 +490,70 @@ static int run_raytrace(const KV& kv_in) {
     fclose(f);
     printf("{\"photon_rate\": %.17g, \"hydrogen_mass\": %.17g, \"pi\": %.17g, \"dx\": %.17g}\n", s.p.photonRate,
            s.consts->hydrogenMass, s.consts->pi, grid.dx[0]);
+    return 0;
+}
+
+// The Radiation integrator on its own (Radiation.cpp): set-up as
+// Torch::initialise (primitives of input_q in code units, fixPrimitives,
+// globalUfromQ, Radiation::initField, Torch.cpp:116-122), then `nsteps`
+// steps of globalQfromU + fixPrimitives + preTimeStepCalculations +
+// calculateTimeStep (dt = min(dt_max, dt_rad)), then two Torch::subStep
+// calls of the radiation component (the first with the heat flux already
+// computed, the second recomputing it; Torch.cpp:352-364).  Writes per step
+// U [7][n], R [6][n] and Q[HII], Q[ADV] [2][n] to <prefix>_<step>.bin and
+// the dts to <prefix>_dt.bin.
+static int run_radiation(const KV& kv_in) {
+    KV kv = kv_in;
+    kv["star_on"] = gets(kv, "star_on", "1");
+    Sim s;
+    init_sim(s, kv);
+    Grid& grid = s.fluid.getGrid();
+    const int n = grid.ncells[0] * grid.ncells[1] * grid.ncells[2];
+    std::vector<double> q(7 * (size_t)n);
+    FILE* f = fopen(gets(kv, "input_q", "").c_str(), "rb");
+    if (fread(q.data(), 8, q.size(), f) != q.size()) throw std::runtime_error("short radiation input");
+    fclose(f);
+    for (int c = 0; c < n; ++c) {
+        GridCell& cell = grid.getCell(c);
+        for (int v = 0; v < 7; ++v) cell.Q[v] = q[(size_t)v * n + c];
+        cell.heatCapacityRatio = s.fluid.heatCapacityRatio;  // Torch.cpp:184,224
+    }
+    s.thermo.initialiseMinTempField(s.fluid);  // Torch.cpp:114
+    s.fluid.fixPrimitives();
+    s.fluid.globalUfromQ();
+    Radiation rad;
+    rad.initialise(s.consts, s.p.getRadiationParameters());
+    rad.initField(s.fluid);
+    const int nsteps = geti(kv, "nsteps", 2);
+    const std::string prefix = gets(kv, "output_prefix", "rad");
+    std::vector<double> dts;
+    for (int step = 1; step <= nsteps; ++step) {
+        s.fluid.globalQfromU();
+        s.fluid.fixPrimitives();
+        rad.preTimeStepCalculations(s.fluid);
+        const double dt = std::min(s.p.dt_max, rad.calculateTimeStep(s.p.dt_max, s.fluid));
+        dts.push_back(dt);
+        sub_step(s, dt, true, rad);
+        sub_step(s, dt, false, rad);
+        std::vector<double> out(15 * (size_t)n);
+        for (int c = 0; c < n; ++c) {
+            GridCell& cell = grid.getCell(c);
+            for (int v = 0; v < 7; ++v) out[(size_t)v * n + c] = cell.U[v];
+            for (int r = 0; r < 6; ++r) out[(size_t)(7 + r) * n + c] = cell.R[r];
+            out[(size_t)13 * n + c] = cell.Q[UID::HII];
+            out[(size_t)14 * n + c] = cell.Q[UID::ADV];
+        }
+        char name[512];
+        snprintf(name, sizeof name, "%s_%04d.bin", prefix.c_str(), step);
+        FILE* o = fopen(name, "wb");
+        fwrite(out.data(), 8, out.size(), o);
+        fclose(o);
+    }
+    FILE* o = fopen((prefix + "_dt.bin").c_str(), "wb");
+    fwrite(dts.data(), 8, dts.size(), o);
+    fclose(o);
+    printf("{\"photon_rate\": %.17g, \"photon_energy\": %.17g, \"sigma\": %.17g, \"dt_max\": %.17g}\n",
+           s.p.photonRate, s.p.photonEnergy, s.p.photoIonCrossSection, s.p.dt_max);
     return 0;
 }
 
@@ -521,6 +601,7 @@ int main(int argc, char** argv) {
         if (mode == "thermo") return run_thermo(read_case(argv[2]));
         if (mode == "consts") return run_consts(argc, argv);
         if (mode == "raytrace") return run_raytrace(read_case(argv[2]));
+        if (mode == "radiation") return run_radiation(read_case(argv[2]));
     } catch (std::exception& e) {
         fprintf(stderr, "ref_harness error: %s\n", e.what());
         return 1;

@@ -1109,3 +1109,511 @@ void orc_star_fuv(const int n[3], const int star[3], double dx, double pi, doubl
                 F[i + (int64_t)n[0] * (j + (int64_t)n[1] * k)] = r2 != 0 ? q / (1.2e7 * 4 * pi * r2) : 0;
             }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Radiation: reference src/Integrators/Radiation.cpp                        */
+/* ------------------------------------------------------------------------ */
+
+/* Radiation::initialise + initRecombinationHummer (Radiation.cpp:24-56,
+ * 184-212), the Constants it reads (Constants.cpp:27-40) and the star's
+ * photon energy and rate (Parameters.cpp:15-20).  Inputs in cgs. */
+void orc_rad_init(orc_rad_params* r, const orc_converter* c, const double K[4], double sigma, double alphaB,
+                  double tau0, double minX, double THI, double THII, int scheme, int coupling, int collisions_on,
+                  double heating_amplification, double mass_frac_h, double photon_energy, double photon_rate) {
+    memset(r, 0, sizeof *r);
+    for (int i = 0; i < 4; ++i) r->K[i] = K[i];
+    r->sigma = orc_to_code_units(c, sigma, 0, 2, 0);
+    r->alphaB = orc_to_code_units(c, alphaB, 0, 3, -1);
+    r->tau0 = tau0;
+    r->minX = minX;
+    r->THI = THI;
+    r->THII = THII;
+    r->scheme = scheme;
+    r->coupling = coupling;
+    r->collisions_on = collisions_on;
+    r->heating_amplification = heating_amplification;
+    r->mass_frac_h = mass_frac_h;
+    r->hydrogen_mass = orc_to_code_units(c, 1.674e-24, 1, 0, 0);
+    r->spec_gas_const = orc_to_code_units(c, 8.314462e7, 0, 2, -2);
+    r->boltzmann = orc_to_code_units(c, 1.3806488e-16, 1, 2, -2);
+    r->rydberg = orc_to_code_units(c, 13.6 * 1.60217646e-12, 1, 2, -2);
+    r->voronov_A = orc_to_code_units(c, 2.91e-8, 0, 3, -1);
+    r->voronov_X = 0.232;
+    r->voronov_K = 0.39;
+    r->voronov_P = 0.0;
+    r->voronov_DE = r->rydberg / r->boltzmann;
+    r->photon_energy = orc_to_code_units(c, photon_energy, 1, 2, -2);
+    r->photon_rate = orc_to_code_units(c, photon_rate, 0, 0, -1);
+    static const double alpha[31] = {9.283e-11, 8.823e-11, 8.361e-11, 7.898e-11, 7.435e-11, 6.973e-11, 6.512e-11,
+        6.054e-11, 5.599e-11, 5.147e-11, 4.700e-11, 4.258e-11, 3.823e-11, 3.397e-11, 2.983e-11, 2.584e-11,
+        2.204e-11, 1.847e-11, 1.520e-11, 1.226e-11, 9.696e-12, 7.514e-12, 5.710e-12, 4.257e-12, 3.117e-12,
+        2.244e-12, 1.590e-12, 1.110e-12, 7.642e-13, 5.199e-13, 3.498e-13};
+    static const double cool[31] = {8.287e-11, 7.821e-11, 7.356e-11, 6.892e-11, 6.430e-11, 5.971e-11, 5.515e-11,
+        5.062e-11, 4.614e-11, 4.170e-11, 3.734e-11, 3.306e-11, 2.888e-11, 2.484e-11, 2.098e-11, 1.736e-11,
+        1.402e-11, 1.103e-11, 8.442e-12, 6.279e-12, 4.539e-12, 3.192e-12, 2.185e-12, 1.458e-12, 9.484e-13,
+        6.023e-13, 3.738e-13, 2.268e-13, 1.348e-13, 7.859e-14, 4.499e-14};
+    for (int i = 0; i < 31; ++i) {
+        /* Hummer (1994) case-B coefficients at log10 T = 1 + 0.2 i, over sqrt T */
+        double T = exp(log(10.0) * (1.0 + 0.2 * (double)i));
+        double sT = sqrt(T);
+        r->rec_x[i] = r->cool_x[i] = T;
+        r->rec_y[i] = orc_to_code_units(c, alpha[i] / sT, 0, 3, -1);
+        r->cool_y[i] = orc_to_code_units(c, cool[i] / sT, 0, 3, -1);
+    }
+    spline_y2(r->rec_x, r->rec_y, 31, r->rec_y2);
+    spline_y2(r->cool_x, r->cool_y, 31, r->cool_y2);
+    /* LinearSplineData: logarithmic end slopes (SplineData.cpp:97-100) */
+    r->rec_slope[0] = (log10(r->rec_y[1]) - log10(r->rec_y[0])) / (log10(r->rec_x[1]) - log10(r->rec_x[0]));
+    r->rec_slope[1] = (log10(r->rec_y[30]) - log10(r->rec_y[29])) / (log10(r->rec_x[30]) - log10(r->rec_x[29]));
+    r->cool_slope[0] = (log10(r->cool_y[1]) - log10(r->cool_y[0])) / (log10(r->cool_x[1]) - log10(r->cool_x[0]));
+    r->cool_slope[1] =
+        (log10(r->cool_y[30]) - log10(r->cool_y[29])) / (log10(r->cool_x[30]) - log10(r->cool_x[29]));
+}
+
+/* LinearSplineData::interpolate, SplineData.cpp:108-119. */
+static double linear_spline(const double* x, const double* y, const double* y2, const double* slope, double v) {
+    if (v > x[30]) return y[30] * pow(v / x[30], slope[1]);
+    if (v < x[0]) return y[0] * pow(v / x[0], slope[0]);
+    return splint(x, y, y2, 31, v);
+}
+
+double orc_rad_recombination(const orc_rad_params* r, double T) {  /* :214-218 */
+    return linear_spline(r->rec_x, r->rec_y, r->rec_y2, r->rec_slope, T);
+}
+
+static double rad_recomb_cooling(const orc_rad_params* r, double nH, double x, double T) {  /* :225-228 */
+    double rate = linear_spline(r->cool_x, r->cool_y, r->cool_y2, r->cool_slope, T);
+    return x * x * nH * nH * r->boltzmann * T * rate;
+}
+
+static double rad_collisional(const orc_rad_params* r, double T) {  /* :236-241, Voronov (1997) */
+    if (!r->collisions_on || T < 5.0e3) return 0.0;
+    double U = r->voronov_DE / T;
+    return r->voronov_A * (1. + r->voronov_P * sqrt(U)) * exp(r->voronov_K * log(U) - U) / (r->voronov_X + U);
+}
+
+static double rad_photoion(double nHI, double tau, double dtau, double shell_vol, double S) {  /* :243-251 */
+    if (nHI == 0 || shell_vol == 0) return 0.0;
+    double emT = exp(-tau), emdT = exp(-dtau);
+    return S * emT * (1.0 - emdT) / (nHI * shell_vol);
+}
+
+static double rad_frac_rate(double A_pi, double A_ci, double A_rr, double nH, double x) {  /* :253-255 */
+    return (1.0 - x) * (A_pi + x * nH * A_ci) - x * x * nH * A_rr;
+}
+
+static double rad_temperature(const orc_rad_params* r, double hii, double pre, double den) {  /* Fluid.cpp:72-75 */
+    double mu_inv = r->mass_frac_h * (hii + 1.0) + (1.0 - r->mass_frac_h) * 0.25;
+    return (pre / den) * (1.0 / mu_inv) / r->spec_gas_const;
+}
+
+/* Radiation::doric, Radiation.cpp:140-182. */
+static void rad_doric(double dt, double* x_avg, double* x, double Api, double nHII_aB, double nHII_Aci) {
+    const double eps = 1.0e-8;
+    double xeq, inv_ti, old = *x;
+    if (nHII_aB == 0.0) xeq = 1.0;
+    else xeq = (Api + nHII_Aci) / (Api + nHII_Aci + nHII_aB);
+    inv_ti = Api + nHII_Aci + nHII_aB;
+    if (dt * inv_ti < 1.0e-8) {
+        *x = old;
+        *x = smax(smin(*x, 1.0), 0.0);
+        if (1.0 - *x < eps) *x = 1.0 - eps;
+        *x_avg = *x;
+    } else {
+        double e = exp(-dt * inv_ti);
+        *x = xeq + (old - xeq) * e;
+        *x = smax(smin(*x, 1.0), 0.0);
+        if (1.0 - *x < eps) *x = 1.0 - eps;
+        *x_avg = xeq + (old - xeq) * (1.0 - e) / (dt * inv_ti);
+        *x_avg = smax(smin(*x_avg, 1.0), 0.0);
+        if (1.0 - *x_avg < eps) *x_avg = 1.0 - eps;
+    }
+}
+
+/* The first cell of the reference's CausalNonWind list (Grid::buildCausal
+ * with nextCausalCoords / nextSnakeCoords, Grid.cpp:294-356,568-574; the
+ * wind test of Fluid.cpp:41-54): octants in the order (+,+,+), (-,+,+),
+ * (+,-,+), (-,-,+), (+,+,-), (-,+,-), (+,-,-), (-,-,-), each walked z-major,
+ * then y, then x, away from the star (negative octants start one cell off
+ * it).  Radiation::calculateTimeStep treats this cell specially on its
+ * first call.  -1 if every cell is a wind cell. */
+int64_t orc_first_nonwind_cell(const int n[3], const int star[3], int wind_radius) {
+    unsigned char* wind = malloc((size_t)n[0] * n[1] * n[2]);
+    orc_star_wind(n, star, wind_radius, wind);
+    int64_t found = -1;
+    for (int oct = 0; oct < 8 && found < 0; ++oct) {
+        const int dir[3] = {(oct & 1) ? -1 : 1, (oct & 2) ? -1 : 1, (oct & 4) ? -1 : 1};
+        int b[3];
+        for (int a = 0; a < 3; ++a) b[a] = star[a] + (dir[a] < 0 ? -1 : 0);
+        for (int k = b[2]; k >= 0 && k < n[2] && found < 0; k += dir[2])
+            for (int j = b[1]; j >= 0 && j < n[1] && found < 0; j += dir[1])
+                for (int i = b[0]; i >= 0 && i < n[0] && found < 0; i += dir[0]) {
+                    int64_t c = i + (int64_t)n[0] * (j + (int64_t)n[1] * k);
+                    if (!wind[c]) found = c;
+                }
+    }
+    free(wind);
+    return found;
+}
+
+/* Torch::initialise's Radiation::initField (Radiation.cpp:78-96): DTAU and
+ * DTAU_A of every cell from the fixed primitives Q (R zero on entry). */
+void orc_rad_init_field(const orc_rad_params* r, const int n[3], const int star[3], double dx, double pi,
+                        const double* Q, double* R) {
+    const int64_t nc = (int64_t)n[0] * n[1] * n[2];
+    for (int64_t c = 0; c < nc; ++c) {
+        int64_t nb[4];
+        double w[4], ds, sv;
+        orc_star_cell_geometry(n, star, dx, pi, (int)(c % n[0]), (int)((c / n[0]) % n[1]),
+                               (int)(c / ((int64_t)n[0] * n[1])), nb, w, &ds, &sv);
+        double nH = r->mass_frac_h * Q[DEN * nc + c] / r->hydrogen_mass;
+        R[ORC_R_DTAU * nc + c] = ((1.0 - Q[HII * nc + c]) * nH) * r->sigma * ds;
+        R[ORC_R_DTAU_A * nc + c] = ((1.0 - R[ORC_R_HII_A * nc + c]) * nH) * r->sigma * ds;
+    }
+}
+
+/* Radiation::preTimeStepCalculations (Radiation.cpp:257-296) on the
+ * CausalNonWind cells: R[HEAT].  Returns 1 on a NaN rate. */
+static int rad_pre_step(const orc_rad_params* r, const int n[3], const int star[3], double dx, double pi,
+                        const unsigned char* wind, const double* Q, double T_min, double* R) {
+    const int64_t nc = (int64_t)n[0] * n[1] * n[2];
+    for (int64_t c = 0; c < nc; ++c) {
+        if (wind[c]) continue;
+        int64_t nb[4];
+        double w[4], ds, sv;
+        orc_star_cell_geometry(n, star, dx, pi, (int)(c % n[0]), (int)((c / n[0]) % n[1]),
+                               (int)(c / ((int64_t)n[0] * n[1])), nb, w, &ds, &sv);
+        const double den = Q[DEN * nc + c], pre = Q[PRE * nc + c], hii = Q[HII * nc + c];
+        double n_H = (r->mass_frac_h * den / r->hydrogen_mass);
+        double excess = r->photon_energy - r->rydberg;
+        double T = rad_temperature(r, hii, pre, den);
+        double nHI = (1.0 - hii) * n_H;
+        double A_pi = rad_photoion(nHI, R[ORC_R_TAU * nc + c], R[ORC_R_DTAU * nc + c], sv, r->photon_rate);
+        double photoion = n_H * (1.0 - hii) * A_pi * excess;
+        double recomb = rad_recomb_cooling(r, n_H, hii, T);
+        double coll = hii * (1.0 - hii) * n_H * n_H * rad_collisional(r, T);
+        double rate = photoion - recomb - coll;
+        double soft = rate;
+        if (T < T_min + 200 && rate < 0.0) soft = smin(0.0, soft * (T - T_min) / 200.0);
+        soft *= r->heating_amplification;
+        R[ORC_R_HEAT * nc + c] = soft;
+        if (rate != rate) return 1;
+    }
+    return 0;
+}
+
+/* Radiation::calculateTimeStep (Radiation.cpp:298-351): min over the
+ * CausalNonWind cells of the coupling and K1..K4 criteria.  *first is the
+ * reference's function-static `once` (cleared by the call). */
+static int rad_timestep(const orc_rad_params* r, const int n[3], const int star[3], double dx, double pi,
+                        const unsigned char* wind, int64_t first_cell, int* first, const double* Q, const double* U,
+                        const double* R, double dt_max, double* dt_out) {
+    const int64_t nc = (int64_t)n[0] * n[1] * n[2];
+    double dt = dt_max;
+    /* the reference's causal order matters only for the `once` cell, which
+     * is visited first */
+    for (int64_t m = -1; m < nc; ++m) {
+        const int64_t c = m < 0 ? first_cell : m;
+        if (c < 0 || wind[c] || (m >= 0 && c == first_cell)) continue;
+        int64_t nb[4];
+        double w[4], ds, sv;
+        orc_star_cell_geometry(n, star, dx, pi, (int)(c % n[0]), (int)((c / n[0]) % n[1]),
+                               (int)(c / ((int64_t)n[0] * n[1])), nb, w, &ds, &sv);
+        const double den = Q[DEN * nc + c], pre = Q[PRE * nc + c], hii = Q[HII * nc + c];
+        double dtc = dt_max, dt1 = dt_max, dt2 = dt_max, dt3 = dt_max, dt4 = dt_max;
+        if (r->coupling == ORC_COUPLING_NEQ && R[ORC_R_HEAT * nc + c] != 0)
+            dtc = fabs(U[PRE * nc + c] / R[ORC_R_HEAT * nc + c]);
+        if (r->K[0] != 0.0) {
+            double T = rad_temperature(r, hii, pre, den);
+            if (*first) {
+                dt1 = r->K[0] * r->hydrogen_mass / (r->mass_frac_h * den * orc_rad_recombination(r, T));
+                *first = 0;
+            } else if (hii != 0) {
+                dt1 = r->K[0] * r->hydrogen_mass / (r->mass_frac_h * den * hii * orc_rad_recombination(r, T));
+            }
+        }
+        if (r->K[1] != 0.0) dt2 = dt_max;
+        for (int q = 2; q < 4; ++q) {
+            if (r->K[q] == 0.0) continue;
+            double nH = r->mass_frac_h * den / r->hydrogen_mass;
+            double nHI = (1.0 - hii) * nH;
+            double A_pi = rad_photoion(nHI, R[ORC_R_TAU * nc + c], R[ORC_R_DTAU * nc + c], sv, r->photon_rate);
+            double A_ci = rad_collisional(r, rad_temperature(r, hii, pre, den));
+            double A_rr = orc_rad_recombination(r, rad_temperature(r, hii, pre, den));
+            double fr = rad_frac_rate(A_pi, A_ci, A_rr, nH, hii);
+            if (fr != 0.0) {
+                if (q == 2) dt3 = r->K[2] * smax(0.05, 1.0 - hii) / fabs(fr);
+                else dt4 = r->K[3] * 1.0 / fabs(fr);
+            }
+        }
+        dt = smin(dt, smin(dt_max, smin(dtc, smin(dt1, smin(dt2, smin(dt3, dt4))))));
+        if (dt == 0.0 || dt != dt || isinf(dt)) return 1;
+    }
+    *dt_out = dt;
+    return 0;
+}
+
+/* Radiation::updateTauSC, Radiation.cpp:353-373 (avg selects TAU_A). */
+static void rad_update_tau(const orc_rad_params* r, int64_t nc, int64_t c, const int64_t nb[4], const double w[4],
+                           double dist2, int avg, double* R) {
+    const int T_ = avg ? ORC_R_TAU_A : ORC_R_TAU, D_ = avg ? ORC_R_DTAU_A : ORC_R_DTAU;
+    if (dist2 > 0.95) {
+        double tau[4] = {0, 0, 0, 0}, wr[4];
+        for (int i = 0; i < 4; ++i) {
+            if (nb[i] != -1) tau[i] = R[T_ * nc + nb[i]] + R[D_ * nc + nb[i]];
+            wr[i] = w[i] / smax(r->tau0, tau[i]);
+        }
+        double sum_w = wr[0] + wr[1] + wr[2] + wr[3];
+        double v = 0.0;
+        for (int i = 0; i < 4; ++i) {
+            wr[i] = wr[i] / sum_w;
+            v += wr[i] * tau[i];
+        }
+        R[T_ * nc + c] = v;
+    } else {
+        R[T_ * nc + c] = 0;
+    }
+}
+
+/* Radiation::update_HIIfrac, Radiation.cpp:375-478, for a non-wind cell.
+ * Q holds den, pre, hii, adv in place.  Returns 1 when the implicit
+ * iteration fails. */
+static int rad_update_hii(const orc_rad_params* r, int64_t nc, int64_t c, double ds, double sv, double dt, double* Q,
+                          double* R) {
+    const double den = Q[DEN * nc + c], pre = Q[PRE * nc + c];
+    double n_H = r->mass_frac_h * den / r->hydrogen_mass;
+    double A_pi = 0;
+    double x = Q[HII * nc + c];
+    double x_avg = x;
+    double T = rad_temperature(r, Q[HII * nc + c], pre, den);
+    double alphaB = orc_rad_recombination(r, T);
+    double A_ci = rad_collisional(r, T);
+    if (r->scheme == ORC_SCHEME_IMPLICIT || r->scheme == ORC_SCHEME_IMPLICIT2) {
+        const double conv2 = 1.0e-3, conv_frac = 1.0e-5;
+        double old;
+        int niter = 0, miter = 0, converged = 0;
+        double tau_avg = R[ORC_R_TAU_A * nc + c];
+        if (r->scheme == ORC_SCHEME_IMPLICIT2) tau_avg = R[ORC_R_TAU * nc + c];
+        while (!converged) {
+            niter++;
+            old = x_avg;
+            x = Q[HII * nc + c];
+            double dtau_avg = (1.0 - x_avg) * n_H * r->sigma * ds;
+            double nHI = (1.0 - x_avg) * n_H;
+            A_pi = rad_photoion(nHI, tau_avg, dtau_avg, sv, r->photon_rate);
+            double nHII_aB = x_avg * n_H * alphaB;
+            double nHII_Aci = x_avg * n_H * A_ci;
+            rad_doric(dt, &x_avg, &x, A_pi, nHII_aB, nHII_Aci);
+            if (fabs((x_avg - old) / x_avg) < conv2 || (x_avg < conv_frac) || A_pi == 0) converged = 1;
+            if (niter > 10005) {
+                miter++;
+                niter = 0;
+                x_avg = 0.5 * (x_avg + old);
+            }
+            if (miter > 5 || x != x) return 1;
+        }
+        R[ORC_R_HII_A * nc + c] = x_avg;
+    } else {
+        double tau = R[ORC_R_TAU * nc + c], dtau = R[ORC_R_DTAU * nc + c];
+        double nHI = (1.0 - x) * (r->mass_frac_h * den / r->hydrogen_mass);
+        A_pi = rad_photoion(nHI, tau, dtau, sv, r->photon_rate);
+        double nHII_aB = x_avg * n_H * alphaB;
+        double nHII_Aci = x_avg * n_H * A_ci;
+        double dummy = x;
+        rad_doric(dt, &x, &dummy, A_pi, nHII_aB, nHII_Aci);
+    }
+    Q[HII * nc + c] = x;
+    if (x > 0.001) Q[ADV * nc + c] = 1;
+    return 0;
+}
+
+/* Radiation::transferRadiation with a star (Radiation.cpp:569-651): wind
+ * cells reset; the others, from the star outwards (each cell's upstream
+ * neighbours one Chebyshev shell closer, as in the column-density ray
+ * trace): TAU, TAU_A, the HII update, then DTAU, DTAU_A. */
+static int rad_transfer(const orc_rad_params* r, const int n[3], const int star[3], double dx, double pi,
+                        const unsigned char* wind, double dt, double* Q, double* R) {
+    const int64_t nc = (int64_t)n[0] * n[1] * n[2];
+    for (int64_t c = 0; c < nc; ++c) {
+        if (!wind[c]) continue;
+        R[ORC_R_TAU * nc + c] = 0;
+        R[ORC_R_TAU_A * nc + c] = 0;
+        R[ORC_R_DTAU * nc + c] = 0;
+        R[ORC_R_DTAU_A * nc + c] = 0;
+        Q[HII * nc + c] = 1;
+        R[ORC_R_HII_A * nc + c] = 1;
+    }
+    int kmax = 0;
+    for (int a = 0; a < 3; ++a) {
+        int m = star[a] > n[a] - 1 - star[a] ? star[a] : n[a] - 1 - star[a];
+        if (m > kmax) kmax = m;
+    }
+    for (int shell = 0; shell <= kmax; ++shell)
+        for (int64_t c = 0; c < nc; ++c) {
+            if (wind[c]) continue;
+            const int ijk[3] = {(int)(c % n[0]), (int)((c / n[0]) % n[1]), (int)(c / ((int64_t)n[0] * n[1]))};
+            int cheb = 0;
+            double dist2 = 0;
+            for (int a = 0; a < 3; ++a) {
+                int e = abs(ijk[a] - star[a]);
+                if (e > cheb) cheb = e;
+                double d = ((double)ijk[a] + 0.5) - ((double)star[a] + 0.5);
+                dist2 += d * d;
+            }
+            if (cheb != shell) continue;
+            int64_t nb[4];
+            double w[4], ds, sv;
+            orc_star_cell_geometry(n, star, dx, pi, ijk[0], ijk[1], ijk[2], nb, w, &ds, &sv);
+            rad_update_tau(r, nc, c, nb, w, dist2, 0, R);
+            rad_update_tau(r, nc, c, nb, w, dist2, 1, R);
+            if (rad_update_hii(r, nc, c, ds, sv, dt, Q, R)) return 1;
+            double nH = r->mass_frac_h * Q[DEN * nc + c] / r->hydrogen_mass;
+            R[ORC_R_DTAU * nc + c] = ((1.0 - Q[HII * nc + c]) * nH) * r->sigma * ds;
+            R[ORC_R_DTAU_A * nc + c] = ((1.0 - R[ORC_R_HII_A * nc + c]) * nH) * r->sigma * ds;
+        }
+    return 0;
+}
+
+/* Radiation::transferRadiation2 with a star (Radiation.cpp:510-567), the
+ * integrate of the implicit2 and explicit schemes: Radiation::rayTrace
+ * first (:480-508: wind cells reset; the others, from the star outwards,
+ * TAU from the upstream neighbours, then DTAU and DTAU_A from the HII
+ * fractions before the update), then the HII update of every non-wind
+ * cell (independent cells). */
+static int rad_transfer2(const orc_rad_params* r, const int n[3], const int star[3], double dx, double pi,
+                         const unsigned char* wind, double dt, double* Q, double* R) {
+    const int64_t nc = (int64_t)n[0] * n[1] * n[2];
+    for (int64_t c = 0; c < nc; ++c) {
+        if (!wind[c]) continue;
+        R[ORC_R_TAU * nc + c] = 0;
+        R[ORC_R_TAU_A * nc + c] = 0;
+        R[ORC_R_DTAU * nc + c] = 0;
+        R[ORC_R_DTAU_A * nc + c] = 0;
+        Q[HII * nc + c] = 1;
+        R[ORC_R_HII_A * nc + c] = 1;
+    }
+    int kmax = 0;
+    for (int a = 0; a < 3; ++a) {
+        int m = star[a] > n[a] - 1 - star[a] ? star[a] : n[a] - 1 - star[a];
+        if (m > kmax) kmax = m;
+    }
+    for (int shell = 0; shell <= kmax; ++shell)
+        for (int64_t c = 0; c < nc; ++c) {
+            if (wind[c]) continue;
+            const int ijk[3] = {(int)(c % n[0]), (int)((c / n[0]) % n[1]), (int)(c / ((int64_t)n[0] * n[1]))};
+            int cheb = 0;
+            double dist2 = 0;
+            for (int a = 0; a < 3; ++a) {
+                int e = abs(ijk[a] - star[a]);
+                if (e > cheb) cheb = e;
+                double d = ((double)ijk[a] + 0.5) - ((double)star[a] + 0.5);
+                dist2 += d * d;
+            }
+            if (cheb != shell) continue;
+            int64_t nb[4];
+            double w[4], ds, sv;
+            orc_star_cell_geometry(n, star, dx, pi, ijk[0], ijk[1], ijk[2], nb, w, &ds, &sv);
+            rad_update_tau(r, nc, c, nb, w, dist2, 0, R);
+            double nH = r->mass_frac_h * Q[DEN * nc + c] / r->hydrogen_mass;
+            R[ORC_R_DTAU * nc + c] = ((1.0 - Q[HII * nc + c]) * nH) * r->sigma * ds;
+            R[ORC_R_DTAU_A * nc + c] = ((1.0 - R[ORC_R_HII_A * nc + c]) * nH) * r->sigma * ds;
+        }
+    for (int64_t c = 0; c < nc; ++c) {
+        if (wind[c]) continue;
+        int64_t nb[4];
+        double w[4], ds, sv;
+        orc_star_cell_geometry(n, star, dx, pi, (int)(c % n[0]), (int)((c / n[0]) % n[1]),
+                               (int)(c / ((int64_t)n[0] * n[1])), nb, w, &ds, &sv);
+        if (rad_update_hii(r, nc, c, ds, sv, dt, Q, R)) return 1;
+    }
+    return 0;
+}
+
+/* Radiation::updateSourceTerms (Radiation.cpp:653-680) + Fluid::advSolution
+ * + fixSolution for a star-on run. */
+static int rad_update_source(const orc_rad_params* r, const orc_hydro_params* p, double dt, const double* Q,
+                             const double* R, double* U) {
+    const int64_t nc = ncells(p);
+    double* UDOT = calloc(7 * (size_t)nc, sizeof(double));
+    for (int64_t c = 0; c < nc; ++c) {
+        const double den = Q[DEN * nc + c];
+        if (r->coupling == ORC_COUPLING_TTI) {
+            double x = Q[HII * nc + c];
+            double mu_inv = r->mass_frac_h * (x + 1.0) + (1.0 - r->mass_frac_h) * 0.25;
+            double T_new = r->THI + (r->THII - r->THI) * x;
+            double pre_new = r->spec_gas_const * mu_inv * den * T_new;
+            double ke = 0.0;
+            for (int d = 0; d < 3; ++d) ke += 0.5 * U[(VEL + d) * nc + c] * U[(VEL + d) * nc + c] / U[DEN * nc + c];
+            double E_new = pre_new / (p->gamma - 1.0) + ke;
+            UDOT[PRE * nc + c] += (E_new - U[PRE * nc + c]) / dt;
+        } else if (r->coupling == ORC_COUPLING_NEQ) {
+            UDOT[PRE * nc + c] += R[ORC_R_HEAT * nc + c];
+        }
+        UDOT[HII * nc + c] += (Q[HII * nc + c] * den - U[HII * nc + c]) / dt;
+        UDOT[ADV * nc + c] += (Q[ADV * nc + c] * den - U[ADV * nc + c]) / dt;
+    }
+    adv_solution(p, U, UDOT, dt);
+    free(UDOT);
+    return orc_fixSolution(p, U) ? 2 : 0;
+}
+
+static int rad_check_values(const orc_hydro_params* p, const double* U, const double* Q) {  /* Torch.cpp:420-443 */
+    const int64_t nc = ncells(p);
+    for (int64_t c = 0; c < nc; ++c) {
+        if (Q[DEN * nc + c] == 0 || Q[PRE * nc + c] == 0) return 1;
+        for (int v = 0; v < NV; ++v)
+            if (U[v * nc + c] != U[v * nc + c] || isinf(U[v * nc + c])) return 1;
+    }
+    return 0;
+}
+
+/* The radiation-only sequence of the reference harness's radiation mode:
+ * per step Q = fixPrimitives(QfromU(U)), preTimeStepCalculations, dt =
+ * min(dt_max, calculateTimeStep), then Torch::subStep(dt, true) and
+ * subStep(dt, false) of the radiation component (Torch.cpp:352-364).  U
+ * and R (6 n, RID order) are updated; Qout (2 n) receives Q[HII], Q[ADV]
+ * after the last step; dts the dt of each step.  *first as rad_timestep.
+ * Returns 0, or 1 (NaN heating / timestep / implicit iteration), 2
+ * (fixSolution), 3 (checkValues). */
+int orc_rad_run(const orc_rad_params* r, const orc_hydro_params* p, const int star[3], int wind_radius,
+                double T_min, int* first, double* U, double* R, double* Qout, int nsteps, double* dts) {
+    const int n[3] = {p->nx, p->ny, p->nz};
+    const int64_t nc = ncells(p);
+    double* Q = malloc(sizeof(double) * 7 * nc);
+    unsigned char* wind = malloc((size_t)nc);
+    orc_star_wind(n, star, wind_radius, wind);
+    const int64_t first_cell = orc_first_nonwind_cell(n, star, wind_radius);
+    const double pi = 3.14159265359;
+    int err = 0;
+    for (int s = 0; s < nsteps && !err; ++s) {
+        orc_globalQfromU(p, U, Q);
+        orc_fixPrimitives(p, Q, U);
+        if ((err = rad_pre_step(r, n, star, p->dx, pi, wind, Q, T_min, R))) break;
+        double dt;
+        if ((err = rad_timestep(r, n, star, p->dx, pi, wind, first_cell, first, Q, U, R, p->dt_max, &dt))) break;
+        dt = smin(p->dt_max, dt);
+        dts[s] = dt;
+        for (int sub = 0; sub < 2 && !err; ++sub) {
+            if (rad_check_values(p, U, Q)) { err = 3; break; }
+            if (sub == 1) {
+                orc_globalQfromU(p, U, Q);
+                orc_fixPrimitives(p, Q, U);
+                if ((err = rad_pre_step(r, n, star, p->dx, pi, wind, Q, T_min, R))) break;
+            }
+            /* Radiation::integrate (Radiation.cpp:58-63) */
+            if (r->scheme == ORC_SCHEME_IMPLICIT) err = rad_transfer(r, n, star, p->dx, pi, wind, dt, Q, R);
+            else err = rad_transfer2(r, n, star, p->dx, pi, wind, dt, Q, R);
+            if (err) break;
+            if ((err = rad_update_source(r, p, dt, Q, R, U))) break;
+            /* checkValues(" after") sees U and the Q the transfer left */
+            if (rad_check_values(p, U, Q)) err = 3;
+        }
+    }
+    for (int64_t c = 0; c < nc; ++c) {
+        Qout[c] = Q[HII * nc + c];
+        Qout[nc + c] = Q[ADV * nc + c];
+    }
+    free(Q);
+    free(wind);
+    return err;
+}

@@ -150,6 +150,32 @@ void orc_star_fuv(const int n[3], const int star[3], double dx, double pi, doubl
 void orc_thermo_ray_trace(const int n[3], const int star[3], const int64_t* nb, const double* w, const double* ds,
                           double hydrogen_mass, const double* den, double* col, double* dcol);
 
+/* Radiation (src/Integrators/Radiation.cpp): parameters in code units,
+ * the Hummer recombination splines, the per-cell state R [6][n] in the
+ * reference's RID order. */
+enum { ORC_R_HII_A = 0, ORC_R_TAU = 1, ORC_R_TAU_A = 2, ORC_R_DTAU = 3, ORC_R_DTAU_A = 4, ORC_R_HEAT = 5 };
+enum { ORC_SCHEME_IMPLICIT = 0, ORC_SCHEME_IMPLICIT2 = 1, ORC_SCHEME_EXPLICIT = 2 };
+enum { ORC_COUPLING_TTI = 0, ORC_COUPLING_NEQ = 1, ORC_COUPLING_OFF = 2 };
+typedef struct {
+    double K[4], sigma, alphaB, tau0, minX, THI, THII;
+    int scheme, coupling, collisions_on;
+    double heating_amplification, mass_frac_h;
+    double hydrogen_mass, spec_gas_const, boltzmann, rydberg;
+    double voronov_A, voronov_X, voronov_K, voronov_P, voronov_DE;
+    double photon_energy, photon_rate;
+    double rec_x[31], rec_y[31], rec_y2[31], rec_slope[2];
+    double cool_x[31], cool_y[31], cool_y2[31], cool_slope[2];
+} orc_rad_params;
+void orc_rad_init(orc_rad_params* r, const orc_converter* c, const double K[4], double sigma, double alphaB,
+                  double tau0, double minX, double THI, double THII, int scheme, int coupling, int collisions_on,
+                  double heating_amplification, double mass_frac_h, double photon_energy, double photon_rate);
+double orc_rad_recombination(const orc_rad_params* r, double T);
+int64_t orc_first_nonwind_cell(const int n[3], const int star[3], int wind_radius);
+void orc_rad_init_field(const orc_rad_params* r, const int n[3], const int star[3], double dx, double pi,
+                        const double* Q, double* R);
+int orc_rad_run(const orc_rad_params* r, const orc_hydro_params* p, const int star[3], int wind_radius,
+                double T_min, int* first, double* U, double* R, double* Qout, int nsteps, double* dts);
+
 #ifdef __cplusplus
 }
 #endif

@@ -236,6 +236,70 @@ def raytrace():
     print("raytrace", list(cases))
 
 
+RAD_CASES = {
+    # the reference config's radiation settings (config/torch-config.lua):
+    # implicit scheme, neq coupling, K1 = 0.2
+    "neq": dict(shape=(8, 9, 10), star=(3, 4, 2), wind=1, nsteps=3, side=3.0857e17, T=(1.5, 2.5), hii=0.01,
+                kv=dict(K1=0.2, coupling="neq", integration_scheme="implicit", collisions_on=0)),
+    # two-temperature isothermal coupling, explicit scheme, collisions, K3/K4
+    "tti": dict(shape=(7, 6, 9), star=(0, 0, 0), wind=0, nsteps=2, side=6e17, T=(3.0, 4.3), hii=0.5,
+                kv=dict(K3=0.1, K4=0.05, coupling="tti", integration_scheme="explicit", collisions_on=1)),
+    # implicit2, no coupling, K1 + K2, heating amplification, wind radius 2
+    "off": dict(shape=(9, 7, 12), star=(6, 3, 4), wind=2, nsteps=2, side=1e18, T=(1.0, 3.5), hii=0.2,
+                kv=dict(K1=0.5, K2=1.0, coupling="off", integration_scheme="implicit2", collisions_on=1,
+                        heating_amplification=2.0)),
+}
+
+
+def radiation():
+    """The Radiation integrator (Radiation.cpp) on its own: initField, then
+    per step preTimeStepCalculations, calculateTimeStep and two subSteps of
+    the radiation component; random clumpy neutral-to-partly-ionised media,
+    a star of 1e49 photons/s of 2.976e-11 erg (code units of the
+    1e-20/1e-28/1e11 scales).  Stores the inputs, the last step's U, R and
+    Q[HII, ADV], and every dt."""
+    c = oracle.converter(1e-20, 1e-28, 1e11)
+    tc = lambda v, m, l, t: oracle.to_code(c, v, m, l, t)  # noqa: E731
+    rng = np.random.default_rng(31)
+    out = {}
+    for name, cs in RAD_CASES.items():
+        shape = cs["shape"]
+        nz, ny, nx = shape
+        rho = 10.0 ** rng.uniform(-22, -20, shape)
+        T = 10.0 ** rng.uniform(cs["T"][0], cs["T"][1], shape)
+        hii = rng.uniform(0, cs["hii"], shape)
+        Q = np.zeros((7,) + shape)
+        Q[0] = rho * tc(1.0, 1, -3, 0)
+        Q[1] = rho * R_CGS * (hii + 1.0) * T * tc(1.0, 1, -1, -2)
+        Q[2] = hii
+        Q[3] = np.where(rng.uniform(0, 1, shape) < 0.5, 0.0, 1.0)
+        Q[4:7] = rng.normal(size=(3,) + shape) * 1e5 * tc(1.0, 0, 1, -1)
+        with tempfile.TemporaryDirectory() as d:
+            Q.tofile(f"{d}/q.bin")
+            kv = {"nx": nx, "ny": ny, "nz": nz, "side_length": cs["side"], "density_scale": 1e-20,
+                  "pressure_scale": 1e-28, "time_scale": 1e11, "gamma": 1.67, "density_floor": 1e-30,
+                  "temperature_floor": 0.1, "mass_fraction_hydrogen": 1.0, "simulation_time": 1e13,
+                  "star_x": cs["star"][0], "star_y": cs["star"][1], "star_z": cs["star"][2],
+                  "wind_radius": cs["wind"], "photon_rate": 1e49, "photon_energy": 2.976e-11,
+                  "photoion_cross_section": 6.3e-18, "case_b_recombination_coeff": 2.59e-13, "tau_0": 0.6,
+                  "temperature_hi": 300, "temperature_hii": 10000, "nsteps": cs["nsteps"], "input_q": f"{d}/q.bin",
+                  "output_prefix": f"{d}/rad"}
+            kv.update(cs["kv"])
+            with open(f"{d}/case.txt", "w") as f:
+                for k, v in kv.items():
+                    f.write(f"{k} {v}\n")
+            subprocess.run([HARNESS, "radiation", f"{d}/case.txt"], capture_output=True, text=True, check=True)
+            o = np.fromfile(f"{d}/rad_{cs['nsteps']:04d}.bin").reshape((15,) + shape)
+            dts = np.fromfile(f"{d}/rad_dt.bin")
+        out[f"{name}_Q"] = Q
+        out[f"{name}_U"] = o[:7]
+        out[f"{name}_R"] = o[7:13]
+        out[f"{name}_q"] = o[13:15]
+        out[f"{name}_dts"] = dts
+    np.savez_compressed(f"{HERE}/radiation.npz", **out)
+    print("radiation", list(RAD_CASES))
+
+
 if __name__ == "__main__":
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle ref")
