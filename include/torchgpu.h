@@ -34,6 +34,9 @@
  *   Torch::initialise start time from initial conditions  torchgpu_set_time
  *     (Torch.cpp:100-103)
  *   DataPrinter::print2D, DataReader (src/IO/)            torchgpu_snapshot_*
+ *   Radiation::initField / preTimeStepCalculations /      torchgpu_radiation_*
+ *     calculateTimeStep / integrate / updateSourceTerms
+ *     (src/Integrators/Radiation.cpp:78-680)
  *   Star-on thermo preTimeStep: Thermodynamics::rayTrace  torchgpu_cooling_set_star,
  *     (Thermodynamics.cpp:218-262,339-417), star geometry   torchgpu_star_geometry,
  *     (Grid.cpp:922-951, Radiation.cpp:78-138)              torchgpu_thermo_grid_inputs
@@ -72,7 +75,10 @@ enum {
     TORCHGPU_ERR_UNKNOWN_SOLVER = 7, /* RiemannSolverFactory::create unknown type (Riemann.cpp:256) */
     TORCHGPU_ERR_UNKNOWN_LIMITER = 8,/* SlopeLimiterFactory::create unknown type (SlopeLimiter.cpp:331) */
     TORCHGPU_ERR_NSTEPS_RANGE = 9,   /* subcycle count beyond int range (Thermodynamics.cpp:307) */
-    TORCHGPU_ERR_NOT_TRAINED = 10    /* surrogate used before weights were set */
+    TORCHGPU_ERR_NOT_TRAINED = 10,   /* surrogate used before weights were set */
+    TORCHGPU_ERR_RADIATION = 11      /* Radiation throws: NaN heating rate, invalid timestep, implicit
+                                        HII iteration not converging, "Radiationbefore produced an error."
+                                        (Radiation.cpp:284-295,346-347,436-455; Torch.cpp:420-443) */
 };
 
 /* Boundary conditions, Condition enum order of src/Torch/Common.hpp:27. */
@@ -319,6 +325,73 @@ int torchgpu_star_geometry(torchgpu_cooling* c, int64_t* nb, double* w, double* 
  * Synchronises. */
 int torchgpu_thermo_grid_inputs(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* T_min, const double* col_den,
                                 const double* F_FUV, int64_t excluded, double* in9, double* dcol_den, int on_device);
+
+/* ------------------------------------------------------------------------ */
+/* Radiation (photoionisation by the star; src/Integrators/Radiation.cpp)     */
+/* ------------------------------------------------------------------------ */
+/* Radiation.integration_scheme and Radiation.coupling (Radiation.cpp:40-52,
+ * enum order of src/Torch/Common.hpp:38-39). */
+enum { TORCHGPU_RT_IMPLICIT = 0, TORCHGPU_RT_IMPLICIT2 = 1, TORCHGPU_RT_EXPLICIT = 2 };
+enum { TORCHGPU_RT_TTI = 0, TORCHGPU_RT_NEQ = 1, TORCHGPU_RT_OFF = 2 };
+/* Radiation::initialise's state in code units (fill with
+ * torchgpu_rad_params_init): K1..K4, cross-section, tau_0, temperatures,
+ * scheme, coupling, constants, the Hummer recombination (rec) and
+ * recombination-cooling (cool) linear splines (LinearSplineData, 31 knots). */
+typedef struct torchgpu_rad_params {
+    double K[4];
+    double sigma, tau0, min_hii, T_hi, T_hii;
+    int scheme, coupling, collisions_on, pad;
+    double heating_amplification, mass_fraction_h;
+    double hydrogen_mass, spec_gas_const, boltzmann, rydberg, pi;
+    double voronov_A, voronov_X, voronov_K, voronov_P, voronov_DE;
+    double photon_energy, T_min;
+    double rec_x[31], rec_y[31], rec_y2[31], rec_slope[2];
+    double cool_y[31], cool_y2[31], cool_slope[2];
+} torchgpu_rad_params;
+/* Radiation::initialise + initRecombinationHummer (Radiation.cpp:24-56,
+ * 184-212) from the Lua-level values: Radiation.K1..K4,
+ * photoion_cross_section (cm^2), tau_0, minimum_hii_fraction,
+ * temperature_hi/hii (K), integration_scheme, coupling, collisions_on,
+ * heating_amplification, mass_fraction_hydrogen and Star.photon_energy
+ * (erg), with the unit scales.  T_min is the reference's default cell
+ * minimum temperature, 100 K. */
+int torchgpu_rad_params_init(torchgpu_rad_params* r, double density_scale, double pressure_scale, double time_scale,
+                             const double K[4], double photoion_cross_section, double tau0, double min_hii_fraction,
+                             double temperature_hi, double temperature_hii, int scheme, int coupling,
+                             int collisions_on, double heating_amplification, double mass_fraction_h,
+                             double photon_energy);
+
+typedef struct torchgpu_radiation torchgpu_radiation;
+/* The Radiation integrator on the grid of ctx with a star (Star.on; its
+ * photon_rate in code units): per-cell state R [6][n] (HII_A, TAU, TAU_A,
+ * DTAU, DTAU_A, HEAT, the reference's RID order) on the device, the star
+ * geometry of torchgpu_star_geometry.  Uses ctx's stream; errors surface
+ * at torchgpu_synchronize(ctx) as TORCHGPU_ERR_RADIATION (or the
+ * fixSolution error). */
+int torchgpu_radiation_create(const torchgpu_rad_params* p, torchgpu_ctx* ctx, const torchgpu_star* star,
+                              torchgpu_radiation** out);
+void torchgpu_radiation_destroy(torchgpu_radiation* r);
+/* Radiation::initField (Radiation.cpp:78-96) as Torch::initialise calls it
+ * after globalUfromQ (Torch.cpp:121): DTAU and DTAU_A from the density and
+ * HII fraction of Q -- the primitives [7][n] given to set_primitive, fixed
+ * as there (host unless `on_device`), or with Q NULL those of
+ * fixPrimitives(QfromU(U)); the rest of R zero. */
+int torchgpu_radiation_init_field(torchgpu_radiation* r, const double* Q, int on_device);
+/* fullStep's radiation part of the step start (Torch.cpp:390-397): Q =
+ * fixPrimitives(QfromU(U)), Radiation::preTimeStepCalculations (heating,
+ * Radiation.cpp:257-296) and calculateTimeStep (:298-351, the reference's
+ * first-call rule for K1 included) -> *dt (<= dt_max).  Synchronises. */
+int torchgpu_radiation_timestep(torchgpu_radiation* r, double dt_max, double* dt);
+/* Torch::subStep of the radiation component (Torch.cpp:352-364): checkValues,
+ * unless `has_heat` (the state is the one of the last timestep call) Q and
+ * preTimeStepCalculations again, integrate (transferRadiation for the
+ * implicit scheme, Radiation.cpp:569-651, else transferRadiation2,
+ * :510-567: the causal sweep from the star), updateSourceTerms (:653-680),
+ * advSolution(dt), fixSolution.  Asynchronous. */
+int torchgpu_radiation_substep(torchgpu_radiation* r, double dt, int has_heat);
+/* R [6][n] and Q[HII], Q[ADV] [2][n] as the last sub-step left them; host
+ * arrays unless `on_device`.  Synchronises. */
+int torchgpu_radiation_get_state(torchgpu_radiation* r, double* R, double* q_hii_adv, int on_device);
 
 /* ------------------------------------------------------------------------ */
 /* Cooling surrogate (no reference counterpart: a learned replacement of the  */

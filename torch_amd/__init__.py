@@ -20,7 +20,7 @@ ERRORS = {
     4: "Fluid::fixSolution: invalid value", 5: "hydro before produced an error",
     6: "Hydrodynamics::calcFluxes: invalid order", 7: "RiemannSolverFactory::create: unknown type",
     8: "SlopeLimiterFactory::create: unknown type", 9: "subcycle count out of range",
-    10: "surrogate not trained",
+    10: "surrogate not trained", 11: "radiation error",
 }
 BC = {"free": 0, "reflecting": 1, "outflow": 2, "inflow": 3, "periodic": 4, "default": 0}
 SOLVER = {"HLL": 0, "HLLC": 1, "RotatedHLLC": 2, "default": 2}
@@ -528,3 +528,115 @@ class Cooling:
                                                 ctypes.c_void_p(out.data_ptr()),
                                                 ctypes.byref(tot) if count else None))
         return out, tot.value
+
+
+# -- radiation (Radiation.cpp) -------------------------------------------------
+
+class RadParams(ctypes.Structure):
+    """torchgpu_rad_params (fill with rad_params)."""
+    _fields_ = ([("K", ctypes.c_double * 4)]
+                + [(k, ctypes.c_double) for k in ("sigma", "tau0", "min_hii", "T_hi", "T_hii")]
+                + [(k, ctypes.c_int) for k in ("scheme", "coupling", "collisions_on", "pad")]
+                + [(k, ctypes.c_double) for k in ("heating_amplification", "mass_fraction_h", "hydrogen_mass",
+                                                  "spec_gas_const", "boltzmann", "rydberg", "pi", "voronov_A",
+                                                  "voronov_X", "voronov_K", "voronov_P", "voronov_DE",
+                                                  "photon_energy", "T_min")]
+                + [("rec_x", ctypes.c_double * 31), ("rec_y", ctypes.c_double * 31), ("rec_y2", ctypes.c_double * 31),
+                   ("rec_slope", ctypes.c_double * 2), ("cool_y", ctypes.c_double * 31),
+                   ("cool_y2", ctypes.c_double * 31), ("cool_slope", ctypes.c_double * 2)])
+
+
+RT_SCHEME = {"implicit": 0, "implicit2": 1, "explicit": 2}
+RT_COUPLING = {"tti": 0, "neq": 1, "off": 2}
+
+
+def _bind_radiation(L):
+    if getattr(L, "_radiation_bound", False):
+        return
+    vp, d, i = ctypes.c_void_p, ctypes.c_double, ctypes.c_int
+    L.torchgpu_rad_params_init.argtypes = [ctypes.POINTER(RadParams), d, d, d, ctypes.POINTER(d), d, d, d, d, d, i, i,
+                                           i, d, d, d]
+    L.torchgpu_radiation_create.argtypes = [ctypes.POINTER(RadParams), vp, ctypes.POINTER(Star), ctypes.POINTER(vp)]
+    L.torchgpu_radiation_destroy.argtypes = [vp]
+    L.torchgpu_radiation_destroy.restype = None
+    L.torchgpu_radiation_init_field.argtypes = [vp, vp, i]
+    L.torchgpu_radiation_timestep.argtypes = [vp, d, ctypes.POINTER(d)]
+    L.torchgpu_radiation_substep.argtypes = [vp, d, i]
+    L.torchgpu_radiation_get_state.argtypes = [vp, vp, vp, i]
+    L._radiation_bound = True
+
+
+def rad_params(scales=(1e-20, 1e-28, 1e11), K=(0.0, 0.0, 0.0, 0.0), photoion_cross_section=6.3e-18, tau0=0.6,
+               min_hii_fraction=0.0, temperature_hi=300.0, temperature_hii=1e4, scheme="implicit", coupling="off",
+               collisions_on=False, heating_amplification=1.0, mass_fraction_h=1.0,
+               photon_energy=2.976e-11) -> RadParams:
+    """Radiation::initialise from the Lua-level (cgs) values; unknown scheme
+    and coupling strings fall back to implicit and off as the reference's."""
+    L = lib()
+    _bind_radiation(L)
+    r = RadParams()
+    Ka = (ctypes.c_double * 4)(*K)
+    rc = L.torchgpu_rad_params_init(ctypes.byref(r), scales[0], scales[1], scales[2], Ka, photoion_cross_section,
+                                    tau0, min_hii_fraction, temperature_hi, temperature_hii,
+                                    RT_SCHEME.get(scheme, 0), RT_COUPLING.get(coupling, 2), int(collisions_on),
+                                    heating_amplification, mass_fraction_h, photon_energy)
+    if rc:
+        raise TorchGpuError(rc, "rad_params_init")
+    return r
+
+
+class Radiation:
+    """The Radiation integrator on a grid with a star (torchgpu_radiation)."""
+
+    def __init__(self, params: RadParams, grid: "Grid", star: Star):
+        L = lib()
+        _bind_radiation(L)
+        self.grid = grid
+        self._h = ctypes.c_void_p()
+        rc = L.torchgpu_radiation_create(ctypes.byref(params), grid._h, ctypes.byref(star), ctypes.byref(self._h))
+        if rc:
+            raise TorchGpuError(rc, ERRORS.get(rc, "radiation create failed"))
+
+    def _check(self, rc):
+        if rc:
+            raise TorchGpuError(rc, ERRORS.get(rc, "radiation error"))
+
+    def close(self):
+        if self._h:
+            lib().torchgpu_radiation_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init_field(self, Q=None):
+        """Radiation::initField; Q: the primitives given to set_primitive."""
+        import numpy as np
+
+        if Q is not None:
+            Q = np.ascontiguousarray(Q, dtype=np.float64)
+        self._check(lib().torchgpu_radiation_init_field(self._h, None if Q is None else ctypes.c_void_p(Q.ctypes.data),
+                                                         0))
+
+    def timestep(self, dt_max: float) -> float:
+        dt = ctypes.c_double()
+        self._check(lib().torchgpu_radiation_timestep(self._h, dt_max, ctypes.byref(dt)))
+        return dt.value
+
+    def substep(self, dt: float, has_heat: bool):
+        self._check(lib().torchgpu_radiation_substep(self._h, dt, int(has_heat)))
+
+    def state(self):
+        """(R [6][nz][ny][nx], Q[HII, ADV] [2][nz][ny][nx]) host arrays."""
+        import numpy as np
+
+        p = self.grid.params
+        shape = (p.nz, p.ny, p.nx)
+        R = np.zeros((6,) + shape)
+        q = np.zeros((2,) + shape)
+        self._check(lib().torchgpu_radiation_get_state(self._h, ctypes.c_void_p(R.ctypes.data),
+                                                        ctypes.c_void_p(q.ctypes.data), 0))
+        return R, q

@@ -20,8 +20,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtorchgpu.so")
-SOURCES = ["hydro.hip", "hydro_mixed.hip", "thermo.hip", "raytrace.hip", "mlp.hip", "api.hip", "units.cpp",
-           "mlp_api.cpp", "snapshot.cpp"]
+SOURCES = ["hydro.hip", "hydro_mixed.hip", "thermo.hip", "raytrace.hip", "radiation.hip", "mlp.hip", "api.hip",
+           "units.cpp", "mlp_api.cpp", "snapshot.cpp"]
 ARCH = os.environ.get("TORCHGPU_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

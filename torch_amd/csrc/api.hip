@@ -71,6 +71,14 @@ int check_device_errors(torchgpu_ctx* c) {
         return fail(c, TORCHGPU_ERR_RIEMANN_NAN, "RiemannSolver::solve: HII flux is NaN" + at);
     case tg::ERR_THERMO_NSTEPS:
         return fail(c, TORCHGPU_ERR_NSTEPS_RANGE, "Thermodynamics::integrate: subcycle count out of int range" + at);
+    case tg::ERR_RAD_CHECK:
+        return fail(c, TORCHGPU_ERR_RADIATION, "Radiationbefore produced an error." + at);
+    case tg::ERR_RAD_NAN:
+        return fail(c, TORCHGPU_ERR_RADIATION, "heating rate in radiation module is NaN" + at);
+    case tg::ERR_RAD_ITER:
+        return fail(c, TORCHGPU_ERR_RADIATION, "Radiation::calculate_HIIfrac: implicit method not converging." + at);
+    case tg::ERR_RAD_DT:
+        return fail(c, TORCHGPU_ERR_RADIATION, "Radiation::calculateTimeStep(): invalid timestep" + at);
     default:
         return fail(c, TORCHGPU_ERR_FIX_SOLUTION, "Fluid::fixSolution: invalid value." + at);
     }

@@ -23,16 +23,6 @@
 
 namespace tg {
 
-__device__ __forceinline__ void shell_cell(const ShellFaces& f, long long t, int ijk[3]) {
-    int q = 0;
-    while (q + 1 < f.nf && t >= f.start[q + 1]) ++q;
-    const long long l = t - f.start[q];
-    const int a = f.axis[q];
-    ijk[a] = f.fixed[q];
-    ijk[a == 0 ? 1 : 0] = f.lo0[q] + (int)(l % f.w0[q]);
-    ijk[a == 2 ? 1 : 2] = f.lo1[q] + (int)(l / f.w0[q]);
-}
-
 // updateColDen (Thermodynamics.cpp:339-363) for the cells of one shell.
 __global__ __launch_bounds__(256) void coldens_shell_kernel(StarGeom s, ShellFaces f, const double* __restrict__ den,
                                                             double* col, double* dcol) {
@@ -87,8 +77,6 @@ __global__ void star_geometry_kernel(StarGeom s, long long* __restrict__ nb, dou
     }
 }
 
-namespace {
-
 // Faces of shell k clipped to the grid (see ShellFaces).
 void shell_faces(const StarGeom& s, int k, ShellFaces& f) {
     f.nf = 0;
@@ -119,15 +107,18 @@ void shell_faces(const StarGeom& s, int k, ShellFaces& f) {
     for (int sgn = -1; sgn <= 1; sgn += 2) add(2, z + sgn * k, x - k + 1, x + k - 1, y - k + 1, y + k - 1);
 }
 
-}  // namespace
-
-hipError_t launch_thermo_ray_trace(const StarGeom& s, const double* den, double* col, double* dcol,
-                                   hipStream_t stream) {
+int shell_max(const StarGeom& s) {
     int kmax = 0;
     for (int a = 0; a < 3; ++a) {
         const int m = s.s[a] > s.n[a] - 1 - s.s[a] ? s.s[a] : s.n[a] - 1 - s.s[a];
         kmax = m > kmax ? m : kmax;
     }
+    return kmax;
+}
+
+hipError_t launch_thermo_ray_trace(const StarGeom& s, const double* den, double* col, double* dcol,
+                                   hipStream_t stream) {
+    const int kmax = shell_max(s);
     ShellFaces f;
     for (int k = 0; k <= kmax; ++k) {
         shell_faces(s, k, f);

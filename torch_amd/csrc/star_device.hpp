@@ -117,6 +117,18 @@ __device__ __forceinline__ bool star_wind_cell(const StarGeom& s, const int ijk[
     return d2 <= (double)(s.wind_r * s.wind_r) || here;
 }
 
+// Cell t of a shell's enumeration (face by face, the face's first axis
+// fastest).
+__device__ __forceinline__ void shell_cell(const ShellFaces& f, long long t, int ijk[3]) {
+    int q = 0;
+    while (q + 1 < f.nf && t >= f.start[q + 1]) ++q;
+    const long long l = t - f.start[q];
+    const int a = f.axis[q];
+    ijk[a] = f.fixed[q];
+    ijk[a == 0 ? 1 : 0] = f.lo0[q] + (int)(l % f.w0[q]);
+    ijk[a == 2 ? 1 : 2] = f.lo1[q] + (int)(l / f.w0[q]);
+}
+
 __device__ __forceinline__ double star_fuv(const StarGeom& s, const int ijk[3]) {
     const double r2 = star_r2(s, ijk);
     return r2 != 0 ? (0.5 * s.photon_rate) / (1.2e7 * 4 * s.pi * r2) : 0.0;

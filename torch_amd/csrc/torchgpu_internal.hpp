@@ -98,7 +98,11 @@ enum : unsigned int {
     ERR_RIEMANN_NAN_2 = 8u,   // ... corrector
     ERR_FIX_SOLUTION_2 = 16u, // ... corrector
     ERR_THERMO_NSTEPS = 32u,  // subcycle count beyond the int range of Thermodynamics.cpp:307
-    ERR_THERMO_FIX = 64u,     // fixSolution throw after a thermo sub-step
+    ERR_THERMO_FIX = 64u,     // fixSolution throw after a thermo or radiation sub-step
+    ERR_RAD_CHECK = 128u,     // checkValues("Radiationbefore") throw (Torch.cpp:353,420-443)
+    ERR_RAD_NAN = 256u,       // "heating rate in radiation module is NaN" (Radiation.cpp:284-295)
+    ERR_RAD_ITER = 512u,      // "implicit method not converging" (Radiation.cpp:436-455)
+    ERR_RAD_DT = 1024u,       // "Radiation::calculateTimeStep(): invalid timestep" (Radiation.cpp:346-347)
 };
 
 // Device view of a grid context (api.hip), for operators implemented in
@@ -159,6 +163,9 @@ hipError_t launch_set_time(StepState* st, double t, hipStream_t stream);
 // shell, den/col/dcol [n] device arrays; the star's geometry per cell.
 hipError_t launch_thermo_ray_trace(const StarGeom& s, const double* den, double* col, double* dcol,
                                    hipStream_t stream);
+// The cells of Chebyshev shell k around the star, and the largest k.
+void shell_faces(const StarGeom& s, int k, ShellFaces& f);
+int shell_max(const StarGeom& s);
 hipError_t launch_star_geometry(const StarGeom& s, long long* nb, double* w, double* ds, double* shell_vol,
                                 hipStream_t stream);
 hipError_t launch_copy_if_running(const StepState* st, const double* src, double* dst, long long n,

@@ -49,7 +49,7 @@ double Converter::convert(double val, double mi, double li, double ti, bool to_c
     return result;
 }
 
-static void natural_spline(const double* x, const double* f, int n, double* y2) {
+void natural_spline(const double* x, const double* f, int n, double* y2) {
     double u[64];
     y2[0] = 0.0;
     u[0] = 0.0;
@@ -130,5 +130,72 @@ extern "C" int torchgpu_thermo_params_init(torchgpu_thermo_params* t, double den
     const double dfloor = code(density_floor_cgs, 1, -3, 0);
     t->pfloor = 0.1 * t->spec_gas_const * dfloor * temperature_floor;
     t->subcycling = subcycling;
+    return TORCHGPU_OK;
+}
+
+extern "C" int torchgpu_rad_params_init(torchgpu_rad_params* r, double density_scale, double pressure_scale,
+                                        double time_scale, const double K[4], double photoion_cross_section,
+                                        double tau0, double min_hii_fraction, double temperature_hi,
+                                        double temperature_hii, int scheme, int coupling, int collisions_on,
+                                        double heating_amplification, double mass_fraction_h, double photon_energy) {
+    // Radiation::initialise + initRecombinationHummer (Radiation.cpp:24-56,
+    // 184-212), Constants::initialise (Constants.cpp:27-40) and the
+    // conversions of TorchParameters::initialise (Parameters.cpp:15-19)
+    if (!r || !K) return TORCHGPU_ERR_INVALID_ARG;
+    if (scheme < TORCHGPU_RT_IMPLICIT || scheme > TORCHGPU_RT_EXPLICIT) return TORCHGPU_ERR_INVALID_ARG;
+    if (coupling < TORCHGPU_RT_TTI || coupling > TORCHGPU_RT_OFF) return TORCHGPU_ERR_INVALID_ARG;
+    tg::Converter c;
+    c.set_rho_pressure_time(density_scale, pressure_scale, time_scale);
+    std::memset(r, 0, sizeof *r);
+    for (int i = 0; i < 4; ++i) r->K[i] = K[i];
+    r->sigma = c.convert(photoion_cross_section, 0, 2, 0, true);
+    r->tau0 = tau0;
+    r->min_hii = min_hii_fraction;
+    r->T_hi = temperature_hi;
+    r->T_hii = temperature_hii;
+    r->scheme = scheme;
+    r->coupling = coupling;
+    r->collisions_on = collisions_on != 0;
+    r->heating_amplification = heating_amplification;
+    r->mass_fraction_h = mass_fraction_h;
+    r->hydrogen_mass = c.convert(1.674e-24, 1, 0, 0, true);
+    r->spec_gas_const = c.convert(8.314462e7, 0, 2, -2, true);
+    r->boltzmann = c.convert(1.3806488e-16, 1, 2, -2, true);
+    r->rydberg = c.convert(13.6 * 1.60217646e-12, 1, 2, -2, true);  // EV_2_ERGS(13.6)
+    r->pi = 3.14159265359;
+    r->voronov_A = c.convert(2.91e-8, 0, 3, -1, true);
+    r->voronov_X = 0.232;
+    r->voronov_K = 0.39;
+    r->voronov_P = 0.0;
+    r->voronov_DE = r->rydberg / r->boltzmann;
+    r->photon_energy = c.convert(photon_energy, 1, 2, -2, true);
+    r->T_min = 100.0;  // Thermodynamics::initialiseMinTempField (Thermodynamics.cpp:52,102-111)
+    // Hummer (1994) case-B recombination and recombination-cooling
+    // coefficients at log10 T = 1, 1.2, ..., 7, divided by sqrt(T)
+    static const double alpha[31] = {9.283e-11, 8.823e-11, 8.361e-11, 7.898e-11, 7.435e-11, 6.973e-11, 6.512e-11,
+        6.054e-11, 5.599e-11, 5.147e-11, 4.700e-11, 4.258e-11, 3.823e-11, 3.397e-11, 2.983e-11, 2.584e-11,
+        2.204e-11, 1.847e-11, 1.520e-11, 1.226e-11, 9.696e-12, 7.514e-12, 5.710e-12, 4.257e-12, 3.117e-12,
+        2.244e-12, 1.590e-12, 1.110e-12, 7.642e-13, 5.199e-13, 3.498e-13};
+    static const double cool[31] = {8.287e-11, 7.821e-11, 7.356e-11, 6.892e-11, 6.430e-11, 5.971e-11, 5.515e-11,
+        5.062e-11, 4.614e-11, 4.170e-11, 3.734e-11, 3.306e-11, 2.888e-11, 2.484e-11, 2.098e-11, 1.736e-11,
+        1.402e-11, 1.103e-11, 8.442e-12, 6.279e-12, 4.539e-12, 3.192e-12, 2.185e-12, 1.458e-12, 9.484e-13,
+        6.023e-13, 3.738e-13, 2.268e-13, 1.348e-13, 7.859e-14, 4.499e-14};
+    for (int i = 0; i < 31; ++i) {
+        const double T = std::exp(std::log(10.0) * (1.0 + 0.2 * static_cast<double>(i)));
+        const double sT = std::sqrt(T);
+        r->rec_x[i] = T;
+        r->rec_y[i] = c.convert(alpha[i] / sT, 0, 3, -1, true);
+        r->cool_y[i] = c.convert(cool[i] / sT, 0, 3, -1, true);
+    }
+    tg::natural_spline(r->rec_x, r->rec_y, 31, r->rec_y2);
+    tg::natural_spline(r->rec_x, r->cool_y, 31, r->cool_y2);
+    // LinearSplineData's logarithmic end slopes (SplineData.cpp:97-100)
+    auto slope = [&](const double* y, int a, int b) {
+        return (std::log10(y[b]) - std::log10(y[a])) / (std::log10(r->rec_x[b]) - std::log10(r->rec_x[a]));
+    };
+    r->rec_slope[0] = slope(r->rec_y, 0, 1);
+    r->rec_slope[1] = slope(r->rec_y, 29, 30);
+    r->cool_slope[0] = slope(r->cool_y, 0, 1);
+    r->cool_slope[1] = slope(r->cool_y, 29, 30);
     return TORCHGPU_OK;
 }

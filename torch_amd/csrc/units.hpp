@@ -11,4 +11,8 @@ struct Converter {
     double convert(double val, double mass_index, double length_index, double time_index, bool to_code) const;
 };
 
+// Natural cubic spline second derivatives (SplineData::spline with both end
+// derivatives 1e99, SplineData.cpp:15-55).
+void natural_spline(const double* x, const double* f, int n, double* y2);
+
 }  // namespace tg
