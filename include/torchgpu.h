@@ -377,10 +377,14 @@ void torchgpu_radiation_destroy(torchgpu_radiation* r);
  * as there (host unless `on_device`), or with Q NULL those of
  * fixPrimitives(QfromU(U)); the rest of R zero. */
 int torchgpu_radiation_init_field(torchgpu_radiation* r, const double* Q, int on_device);
-/* fullStep's radiation part of the step start (Torch.cpp:390-397): Q =
- * fixPrimitives(QfromU(U)), Radiation::preTimeStepCalculations (heating,
- * Radiation.cpp:257-296) and calculateTimeStep (:298-351, the reference's
- * first-call rule for K1 included) -> *dt (<= dt_max).  Synchronises. */
+/* Q = fixPrimitives(QfromU(U)) and Radiation::preTimeStepCalculations
+ * (heating, Radiation.cpp:257-296) on the current state, as Torch::fullStep
+ * does before the time step (Torch.cpp:390-395).  Asynchronous. */
+int torchgpu_radiation_pre_step(torchgpu_radiation* r);
+/* Radiation::calculateTimeStep (Radiation.cpp:298-351; the reference's
+ * first-call rule for K1 included, so call it only where Torch::
+ * calculateTimeStep does -- not on the first step) on the state of the last
+ * pre_step (made now if there is none) -> *dt (<= dt_max).  Synchronises. */
 int torchgpu_radiation_timestep(torchgpu_radiation* r, double dt_max, double* dt);
 /* Torch::subStep of the radiation component (Torch.cpp:352-364): checkValues,
  * unless `has_heat` (the state is the one of the last timestep call) Q and

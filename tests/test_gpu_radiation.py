@@ -51,6 +51,7 @@ def _run_gpu(gpu, shape, Q, side_cgs, star, wind, kv, nsteps):
     rad.init_field(Q)
     dts = []
     for _ in range(nsteps):
+        rad.pre_step()
         dt = min(p.dt_max, rad.timestep(p.dt_max))
         dts.append(dt)
         rad.substep(dt, True)

@@ -261,3 +261,106 @@ def test_driver_cooling_matches_device_loop(tmp_path, gpu):
     assert not np.array_equal(Upy[1], U0[1])
     np.testing.assert_array_equal(res["device"], Upy)
     np.testing.assert_array_equal(res["host"], Upy)
+
+
+STAR_LUA = """
+Parameters = {
+    Integration = {
+        density_scale = 1.0e-20, pressure_scale = 1.0e-28, time_scale = 1.0e11,
+        spatial_order = 1, temporal_order = 2, simulation_time = 1.0e13,
+        radiation_on = true, cooling_on = true, ncheckpoints = 10,
+    },
+    Grid = {
+        no_dimensions = 3, no_cells_x = 16, no_cells_y = 16, no_cells_z = 16,
+        geometry = "cartesian", side_length = 1.0e18,
+        left_boundary_condition_x = "outflow", right_boundary_condition_x = "outflow",
+        left_boundary_condition_y = "outflow", right_boundary_condition_y = "outflow",
+        left_boundary_condition_z = "outflow", right_boundary_condition_z = "outflow",
+    },
+    Hydrodynamics = {
+        gamma = 1.67, density_floor = 1.0e-30, temperature_floor = 0.1,
+        riemann_solver = "RotatedHLLC", slope_limiter = "albada",
+    },
+    Radiation = {
+        K1 = 0.2, K2 = 0.0, K3 = 0.0, K4 = 0.0,
+        photoion_cross_section = 6.3e-18, case_b_recombination_coeff = 2.59e-13, tau_0 = 0.6,
+        minimum_hii_fraction = 0, temperature_hi = 300, temperature_hii = 10000,
+        mass_fraction_hydrogen = 1.0, integration_scheme = "implicit", collisions_on = false, coupling = "neq",
+    },
+    Thermodynamics = { thermo_hii_switch = 1.0e-2, heating_amplification = 1.0 },
+    Star = {
+        on = true, cell_position_x = 7, cell_position_y = 8, cell_position_z = 6,
+        snap_to_face_left_x = false, snap_to_face_left_y = false, snap_to_face_left_z = false,
+        photon_energy = 2.976e-11, photon_rate = 4.9e48, wind_radius_in_cells = 1,
+    },
+}
+"""
+
+
+@pytest.mark.gpu
+def test_driver_star_radiation_cooling_matches_python_loop(tmp_path, gpu):
+    """Torch::fullStep with hydrodynamics, cooling and radiation around a star
+    (Lua Star and Radiation tables): the C++ driver's host-driven sequence
+    against torch_amd.fullstep.FullStep over the same library calls with the
+    parameters the driver reports: identical conserved states after 5 steps
+    (the first with dt_max 1e-20, then min(hydro, radiation, thermo))."""
+    _exe_or_skip()
+    from torch_amd import fullstep, setups
+
+    n, steps = 16, 5
+    f = tmp_path / "star.lua"
+    f.write_text(STAR_LUA)
+    d = json.loads(subprocess.run([EXE, f"--paramfile={f}", "--dump-params"], capture_output=True, text=True,
+                                  timeout=60).stdout)
+    assert d["star_on"] and d["radiation_on"] and d["star_cell"] == [7, 8, 6] and d["coupling"] == "neq"
+    Qt = setups.turbulent(n, 0.5, 0.5)
+    rho = 1.0e-21 * Qt[0]
+    Q = np.zeros((7, n, n, n))
+    Q[0] = rho
+    Q[1] = rho * 8.314462e7 * 100.0
+    Q[3] = 1.0
+    Q[4:7] = Qt[4:7] / np.abs(Qt[4:7]).max() * 1.0e5
+    setup = tmp_path / "Q.bin"
+    Q.tofile(setup)
+    outs = {}
+    for k in (0, steps):
+        out = tmp_path / f"U_{k}.bin"
+        r = subprocess.run([EXE, f"--paramfile={f}", f"--setup={setup}", f"--steps={k}", f"--output={out}"],
+                           capture_output=True, text=True, timeout=120, cwd=tmp_path)
+        assert r.returncode == 0, r.stderr
+        outs[k] = np.fromfile(out, dtype=np.float64).reshape(7, n, n, n)
+    scales = tuple(d["scales"])
+    cu = d["code_units"]
+    p = gpu.make_params((n, n, n), side_length=cu["side_length"], gamma=d["gamma"], dfloor=cu["dfloor"],
+                        pfloor=cu["pfloor"], tfloor=d["tfloor"], spec_gas_const=cu["spec_gas_const"],
+                        tmax=cu["tmax"], ncheckpoints=d["ncheckpoints"])
+    g = gpu.Grid(p)
+    g.set_conserved(outs[0])
+    star = gpu.make_star(d["star_cell"], d["wind_radius"], cu["photon_rate"])
+    tp = gpu.thermo_params(scales=scales, thermo_hii_switch=d["thermo_hii_switch"],
+                           mass_fraction_h=d["mass_fraction_h"], gamma=d["gamma"], density_floor=d["dfloor"],
+                           temperature_floor=d["tfloor"])
+    tp.pfloor = cu["pfloor"]
+    cool = gpu.Cooling(tp)
+    cool.set_star(g, star)
+    rp = gpu.rad_params(scales, K=tuple(d["K"]), photoion_cross_section=d["photoion_cross_section"],
+                        tau0=d["tau_0"], temperature_hi=d["temperature_hi"], temperature_hii=d["temperature_hii"],
+                        scheme=d["integration_scheme"], coupling=d["coupling"], collisions_on=d["collisions_on"],
+                        heating_amplification=d["heating_amplification"], mass_fraction_h=d["mass_fraction_h"],
+                        photon_energy=d["photon_energy"])
+    rad = gpu.Radiation(rp, g, star)
+    # the code-unit primitives Torch::initialise hands to Radiation::initField
+    to = np.vectorize(lambda v, m, l, t: gpu.units(scales, v, m, l, t))
+    Qc = Q.copy()
+    Qc[0] = to(Q[0], 1, -3, 0)
+    Qc[1] = to(Q[1], 1, -1, -2)
+    for a in range(3):
+        Qc[4 + a] = to(Q[4 + a], 0, 1, -1)
+    rad.init_field(Qc)
+    loop = fullstep.FullStep(g, cooling=cool, radiation=rad)
+    dts = [loop.step() for _ in range(steps)]
+    g.synchronize()
+    U = g.get_conserved()
+    assert np.isfinite(U).all() and dts[1] > dts[0]
+    assert not np.array_equal(U[2], outs[0][2])  # the star ionised gas
+    np.testing.assert_array_equal(outs[steps], U)

@@ -561,6 +561,7 @@ def _bind_radiation(L):
     L.torchgpu_radiation_destroy.restype = None
     L.torchgpu_radiation_init_field.argtypes = [vp, vp, i]
     L.torchgpu_radiation_timestep.argtypes = [vp, d, ctypes.POINTER(d)]
+    L.torchgpu_radiation_pre_step.argtypes = [vp]
     L.torchgpu_radiation_substep.argtypes = [vp, d, i]
     L.torchgpu_radiation_get_state.argtypes = [vp, vp, vp, i]
     L._radiation_bound = True
@@ -620,6 +621,10 @@ class Radiation:
             Q = np.ascontiguousarray(Q, dtype=np.float64)
         self._check(lib().torchgpu_radiation_init_field(self._h, None if Q is None else ctypes.c_void_p(Q.ctypes.data),
                                                          0))
+
+    def pre_step(self):
+        """Q and Radiation::preTimeStepCalculations on the current state."""
+        self._check(lib().torchgpu_radiation_pre_step(self._h))
 
     def timestep(self, dt_max: float) -> float:
         dt = ctypes.c_double()

@@ -607,11 +607,19 @@ int torchgpu_radiation_init_field(torchgpu_radiation* r, const double* Q, int on
     return TORCHGPU_OK;
 }
 
-int torchgpu_radiation_timestep(torchgpu_radiation* r, double dt_max, double* dt) {
-    if (!r || !dt || !(dt_max > 0.0)) return TORCHGPU_ERR_INVALID_ARG;
+int torchgpu_radiation_pre_step(torchgpu_radiation* r) {
+    if (!r) return TORCHGPU_ERR_INVALID_ARG;
     int rc = rad_gather(r, 0);
     if (!rc) rc = rad_heat(r);
-    if (rc) return rc;
+    return rc;
+}
+
+int torchgpu_radiation_timestep(torchgpu_radiation* r, double dt_max, double* dt) {
+    if (!r || !dt || !(dt_max > 0.0)) return TORCHGPU_ERR_INVALID_ARG;
+    if (!r->q_valid) {
+        const int rc = torchgpu_radiation_pre_step(r);
+        if (rc) return rc;
+    }
     tg::CtxView v = tg::ctx_view(r->ctx);
     tg::RadConsts c = r->rc;
     c.use_once = r->once_pending && c.K[0] != 0.0 && c.once_cell >= 0;

@@ -161,7 +161,20 @@ void parseParameters(const LuaConfig& cfg, TorchParameters& p, std::string* w) {
     take(H["riemann_solver"], p.riemannSolver, "riemann_solver", w);
     take(H["slope_limiter"], p.slopeLimiter, "slope_limiter", w);
     if (H["precision"].exists()) take(H["precision"], p.precision, "precision", w);
+    take(R["K1"], p.K1, "K1", w);
+    take(R["K2"], p.K2, "K2", w);
+    take(R["K3"], p.K3, "K3", w);
+    take(R["K4"], p.K4, "K4", w);
+    take(R["photoion_cross_section"], p.photoIonCrossSection, "photoion_cross_section", w);
+    take(R["case_b_recombination_coeff"], p.alphaB, "case_b_recombination_coeff", w);
+    take(R["tau_0"], p.tau0, "tau_0", w);
+    take(R["minimum_hii_fraction"], p.minX, "minimum_hii_fraction", w);
+    take(R["temperature_hi"], p.THI, "temperature_hi", w);
+    take(R["temperature_hii"], p.THII, "temperature_hii", w);
     take(R["mass_fraction_hydrogen"], p.massFractionH, "mass_fraction_hydrogen", w);
+    take(R["collisions_on"], p.collisions_on, "collisions_on", w);
+    take(R["coupling"], p.rt_coupling, "coupling", w);
+    take(R["integration_scheme"], p.rt_scheme, "integration_scheme", w);
     take(T["thermo_hii_switch"], p.thermoHII_Switch, "thermo_hii_switch", w);
     take(T["heating_amplification"], p.heatingAmplification, "heating_amplification", w);
     take(T["thermo_subcycling"], p.thermoSubcycling, "thermo_subcycling", w);
@@ -170,7 +183,15 @@ void parseParameters(const LuaConfig& cfg, TorchParameters& p, std::string* w) {
     take(S["cell_position_x"], p.star_position[0], "cell_position_x", w);
     take(S["cell_position_y"], p.star_position[1], "cell_position_y", w);
     take(S["cell_position_z"], p.star_position[2], "cell_position_z", w);
+    take(S["snap_to_face_left_x"], p.faceSnap[0], "snap_to_face_left_x", w);
+    take(S["snap_to_face_left_y"], p.faceSnap[1], "snap_to_face_left_y", w);
+    take(S["snap_to_face_left_z"], p.faceSnap[2], "snap_to_face_left_z", w);
+    take(S["photon_energy"], p.photonEnergy, "photon_energy", w);
     take(S["photon_rate"], p.photonRate, "photon_rate", w);
+    take(S["wind_radius_in_cells"], p.windCellRadius, "wind_radius_in_cells", w);
+    take(S["mass_loss_rate"], p.massLossRate, "mass_loss_rate", w);
+    take(S["wind_velocity"], p.windVelocity, "wind_velocity", w);
+    take(S["wind_temperature"], p.windTemperature, "wind_temperature", w);
 }
 
 // ---------------------------------------------------------------------------
@@ -216,6 +237,9 @@ void Fluid::advSolution(double dt) {
     } else if (pending == Pending::THERMO) {
         check_rc(torchgpu_thermo_grid_substep(ctx_, cooling, nullptr, nullptr, nullptr, -1, dt, 0), "thermo substep",
                  ctx_);
+    } else if (pending == Pending::RAD) {
+        // Torch::subStep has made preTimeStepCalculations on this state
+        check_rc(torchgpu_radiation_substep(radiation, dt, 1), "radiation substep", ctx_);
     }
     pending = Pending::NONE;
 }
@@ -268,6 +292,62 @@ void Thermodynamics::integrate(double, Fluid& fluid) const {
 }
 void Thermodynamics::updateSourceTerms(double, Fluid&) const {}
 
+torchgpu_star starOf(const TorchParameters& p) {
+    // Star::initialise (Star.cpp:17-21): snapped stars sit on a cell face,
+    // which the shell sweeps of the library do not cover
+    if (p.faceSnap[0] || p.faceSnap[1] || p.faceSnap[2])
+        throw std::runtime_error("Star.snap_to_face_left_*: stars on a cell face are not supported on the GPU path");
+    torchgpu_star s;
+    for (int a = 0; a < 3; ++a) s.cell[a] = p.star_position[a];
+    s.wind_cell_radius = p.windCellRadius;
+    s.photon_rate = p.photonRate;  // code units after TorchParameters::initialise
+    return s;
+}
+
+Radiation::~Radiation() {
+    if (rad_) torchgpu_radiation_destroy(rad_);
+}
+
+void Radiation::initialise(const TorchParameters& p, Fluid& fluid) {
+    // Radiation::initialise (Radiation.cpp:24-56) with the Lua-level values
+    // (the cross-section and photon energy stay in cgs here; the library
+    // converts them as TorchParameters::initialise does)
+    torchgpu_rad_params r;
+    const double K[4] = {p.K1, p.K2, p.K3, p.K4};
+    const int scheme = p.rt_scheme == "implicit2" ? TORCHGPU_RT_IMPLICIT2
+                       : p.rt_scheme == "explicit" ? TORCHGPU_RT_EXPLICIT
+                                                   : TORCHGPU_RT_IMPLICIT;
+    const int coupling = p.rt_coupling == "neq" ? TORCHGPU_RT_NEQ
+                         : p.rt_coupling == "tti" ? TORCHGPU_RT_TTI
+                                                  : TORCHGPU_RT_OFF;
+    check_rc(torchgpu_rad_params_init(&r, p.dscale, p.pscale, p.tscale, K, p.photoIonCrossSection, p.tau0, p.minX,
+                                      p.THI, p.THII, scheme, coupling, p.collisions_on ? 1 : 0,
+                                      p.heatingAmplification, p.massFractionH, p.photonEnergy),
+             "radiation params");
+    const torchgpu_star star = starOf(p);
+    check_rc(torchgpu_radiation_create(&r, fluid.ctx(), &star, &rad_), "radiation create", fluid.ctx());
+}
+
+void Radiation::initField(const double* Q) {
+    check_rc(torchgpu_radiation_init_field(rad_, Q, 0), "radiation initField");
+}
+
+void Radiation::preTimeStepCalculations(Fluid& fluid) const {
+    check_rc(torchgpu_radiation_pre_step(rad_), "radiation preTimeStep", fluid.ctx());
+}
+
+double Radiation::calculateTimeStep(double dt_max, Fluid& fluid) const {
+    double dt = 0;
+    check_rc(torchgpu_radiation_timestep(rad_, dt_max, &dt), "radiation timestep", fluid.ctx());
+    return dt;
+}
+
+void Radiation::integrate(double, Fluid& fluid) const {
+    fluid.radiation = rad_;
+    fluid.pending = Fluid::Pending::RAD;
+}
+void Radiation::updateSourceTerms(double, Fluid&) const {}
+
 // ---------------------------------------------------------------------------
 // driver (Torch.cpp)
 // ---------------------------------------------------------------------------
@@ -287,7 +367,21 @@ void Torch::initialise(const TorchParameters& p_in, void* stream) {
     }
     params.initialise();
     fluid.initialise(params, stream);
-    if (params.cooling_on) thermodynamics.initialise(params, stream);
+    if (params.cooling_on) {
+        thermodynamics.initialise(params, stream);
+        if (params.star_on) {
+            // Thermodynamics::preTimeStepCalculations with Star.on
+            const torchgpu_star star = starOf(params);
+            check_rc(torchgpu_cooling_set_star(thermodynamics.handle(), fluid.ctx(), &star), "cooling star",
+                     fluid.ctx());
+        }
+    }
+    if (params.radiation_on) {
+        if (!params.star_on)
+            throw std::runtime_error("radiation_on without Star.on: the reference's Radiation then changes nothing "
+                                     "but the printed HII fraction, which the GPU path does not model");
+        radiation.initialise(params, fluid);
+    }
     currentTime = 0;
     steps = 0;
     first_time_ = true;
@@ -318,6 +412,8 @@ void Torch::setPrimitivesCgs(const double* Qcgs) {
         for (int d = 0; d < 3; ++d) Q[(4 + d) * n + c] = params.toCodeUnits(Q[(4 + d) * n + c], 0, 1, -1);
     }
     fluid.setPrimitives(Q.data());
+    // Torch::initialise: radiation.initField after globalUfromQ (Torch.cpp:121)
+    if (params.radiation_on) radiation.initField(Q.data());
 }
 
 double Torch::calculateTimeStep() {  // Torch.cpp:304-339 (single process: the MPI min is the identity)
@@ -328,6 +424,7 @@ double Torch::calculateTimeStep() {  // Torch.cpp:304-339 (single process: the M
     } else {
         double dt_hydro = hydrodynamics.calculateTimeStep(params.dt_max, fluid);
         double dt_rad = dt_hydro, dt_thermo = dt_hydro;
+        if (params.radiation_on) dt_rad = radiation.calculateTimeStep(params.dt_max, fluid);
         if (params.cooling_on) dt_thermo = thermodynamics.calculateTimeStep(params.dt_max, fluid);
         dt = std::min(std::min(dt_hydro, dt_rad), dt_thermo);
     }
@@ -369,9 +466,12 @@ double Torch::fullStep(double dt_nextCheckPoint) {  // Torch.cpp:389-418
     fluid.globalQfromU();
     fluid.fixPrimitives();
     if (params.cooling_on) thermodynamics.preTimeStepCalculations(fluid);
+    if (params.radiation_on) radiation.preTimeStepCalculations(fluid);
     double dt = std::min(dt_nextCheckPoint, calculateTimeStep());
+    // activeComponents of Torch::run (Torch.cpp:256-261)
     std::vector<Integrator*> comps{&hydrodynamics};
     if (params.cooling_on) comps.push_back(&thermodynamics);
+    if (params.radiation_on) comps.push_back(&radiation);
     const int ncomps = (int)comps.size();
     if (ncomps == 1) {
         hydroStep(dt, true);
@@ -405,6 +505,9 @@ void Torch::print2D(int count) const {
 }
 
 void Torch::run(int max_steps, bool print) {  // Torch.cpp:240-302, capped at max_steps
+    if (print && params.radiation_on)
+        throw std::runtime_error("--snapshots: checkpoint printing runs on the device loops (hydro, hydro + cooling); "
+                                 "radiation_on runs the host-driven loop");
     if (print) {
         // the device loop pauses at every checkpoint for the print
         torchgpu_ctx* c = fluid.ctx();
@@ -438,7 +541,7 @@ void Torch::run(int max_steps, bool print) {  // Torch.cpp:240-302, capped at ma
         steps = n1;
         return;
     }
-    if (!params.cooling_on) {
+    if (!params.cooling_on && !params.radiation_on) {
         // hydro-only: the whole loop stays on the device (dt chosen there)
         check_rc(torchgpu_run(fluid.ctx(), max_steps), "run", fluid.ctx());
         fluid.check();

@@ -53,7 +53,16 @@ struct TorchParameters {
     double dscale = 0, pscale = 0, tscale = 0;
     bool star_on = false;
     std::array<int, 3> star_position{{0, 0, 0}};
+    std::array<bool, 3> faceSnap{{false, false, false}};
     double photonRate = 0;
+    double photonEnergy = 0;
+    int windCellRadius = 0;
+    double massLossRate = 0, windVelocity = 0, windTemperature = 0;
+    double K1 = 0, K2 = 0, K3 = 0, K4 = 0;
+    double photoIonCrossSection = 0, alphaB = 0, tau0 = 0, minX = 0, THI = 0, THII = 0;
+    bool collisions_on = false;
+    std::string rt_coupling = "off";
+    std::string rt_scheme = "implicit";
     double thermoHII_Switch = 0;
     double heatingAmplification = 1.0;
     bool thermoSubcycling = true;
@@ -125,9 +134,10 @@ public:
     void fixSolution();
 
     // pending work recorded by the integrators
-    enum class Pending { NONE, HYDRO, THERMO };
+    enum class Pending { NONE, HYDRO, THERMO, RAD };
     Pending pending = Pending::NONE;
-    torchgpu_cooling* cooling = nullptr;  // set by Thermodynamics
+    torchgpu_cooling* cooling = nullptr;      // set by Thermodynamics
+    torchgpu_radiation* radiation = nullptr;  // set by Radiation
 
     void check() const;  // reports device errors as std::runtime_error
 
@@ -165,6 +175,27 @@ private:
     torchgpu_cooling* cool_ = nullptr;
 };
 
+// Radiation (src/Integrators/Radiation.cpp) for a star-on run.
+class Radiation : public Integrator {
+public:
+    Radiation() : Integrator("Radiation") {}
+    ~Radiation();
+    void initialise(const TorchParameters& p, Fluid& fluid);
+    void initField(const double* Q);  // Radiation::initField on the primitives of Torch::initialise
+    void preTimeStepCalculations(Fluid& fluid) const override;
+    double calculateTimeStep(double dt_max, Fluid& fluid) const override;
+    void integrate(double dt, Fluid& fluid) const override;
+    void updateSourceTerms(double dt, Fluid& fluid) const override;
+    torchgpu_radiation* handle() const { return rad_; }
+
+private:
+    torchgpu_radiation* rad_ = nullptr;
+};
+
+// The star of Parameters.Star as the library takes it (cell centre; face
+// snapping is rejected).
+torchgpu_star starOf(const TorchParameters& p);
+
 // The driver of src/Torch/Torch.cpp for hydro-only runs (device-resident
 // time loop) and the operator-split sequence with the integrators above.
 class Torch {
@@ -183,6 +214,7 @@ public:
     Fluid fluid;
     Hydrodynamics hydrodynamics;
     Thermodynamics thermodynamics;
+    Radiation radiation;
     TorchParameters params;
     double currentTime = 0;
     int steps = 0;

@@ -46,12 +46,21 @@ void dump(const tg::TorchParameters& p) {
                 p.leftBC[2].c_str(), p.rightBC[2].c_str(), p.cooling_on ? "true" : "false", p.ncheckpoints);
     std::printf(" \"mass_fraction_h\": %.17g, \"thermo_hii_switch\": %.17g, \"precision\": \"%s\",\n", p.massFractionH,
                 p.thermoHII_Switch, p.precision.c_str());
+    std::printf(" \"star_on\": %s, \"star_cell\": [%d, %d, %d], \"wind_radius\": %d, \"photon_rate\": %.17g, "
+                "\"photon_energy\": %.17g, \"radiation_on\": %s,\n", p.star_on ? "true" : "false", p.star_position[0],
+                p.star_position[1], p.star_position[2], p.windCellRadius, p.photonRate, p.photonEnergy,
+                p.radiation_on ? "true" : "false");
+    std::printf(" \"K\": [%.17g, %.17g, %.17g, %.17g], \"photoion_cross_section\": %.17g, \"tau_0\": %.17g, "
+                "\"temperature_hi\": %.17g, \"temperature_hii\": %.17g, \"coupling\": \"%s\", "
+                "\"integration_scheme\": \"%s\", \"collisions_on\": %s, \"heating_amplification\": %.17g,\n",
+                p.K1, p.K2, p.K3, p.K4, p.photoIonCrossSection, p.tau0, p.THI, p.THII, p.rt_coupling.c_str(),
+                p.rt_scheme.c_str(), p.collisions_on ? "true" : "false", p.heatingAmplification);
     // the same after TorchParameters::initialise (code units)
     tg::TorchParameters c = p;
     c.initialise();
     std::printf(" \"code_units\": {\"side_length\": %.17g, \"tmax\": %.17g, \"dt_max\": %.17g, \"dfloor\": %.17g, "
-                "\"pfloor\": %.17g, \"spec_gas_const\": %.17g}}\n", c.sideLength, c.tmax, c.dt_max, c.dfloor, c.pfloor,
-                c.specificGasConstant);
+                "\"pfloor\": %.17g, \"spec_gas_const\": %.17g, \"photon_rate\": %.17g}}\n", c.sideLength, c.tmax,
+                c.dt_max, c.dfloor, c.pfloor, c.specificGasConstant, c.photonRate);
 }
 
 }  // namespace
