@@ -225,6 +225,40 @@ def cooling_rates(log2_cells: int, weights: str, gate: float) -> dict:
     }
 
 
+def star_cpu_baseline(m: int) -> dict:
+    """The oracle's single-threaded C restatement of the star operators on an
+    m^3 grid of the same medium: the column-density ray trace and one
+    radiation sub-step (half of a two-sub-step radiation step)."""
+    import numpy as np
+
+    import oracle
+    from torch_amd import star_bench
+
+    c = oracle.converter(*star_bench.SCALES)
+    units = lambda v, mi, li, ti: oracle.to_code(c, v, mi, li, ti)  # noqa: E731
+    Q, side = star_bench.medium(m, units)
+    dx = units(side, 0, 1, 0) / m
+    st = (m // 2, m // 2, m // 2)
+    geom = oracle.star_geometry((m, m, m), st, dx, 3.14159265359)
+    t0 = time.perf_counter()
+    oracle.thermo_ray_trace((m, m, m), st, geom, units(1.674e-24, 1, 0, 0), Q[0])
+    t_rt = time.perf_counter() - t0
+    R = units(8.314462e7, 0, 2, -2)
+    dfloor = units(1e-30, 1, -3, 0)
+    p = oracle.make_params((m, m, m), dx, gamma=1.67, dfloor=dfloor, pfloor=0.1 * R * dfloor * 0.1, tfloor=0.1,
+                           spec_gas_const=R, dt_max=units(1e13, 0, 0, 1) / 10.0, nthreads=1)
+    r = oracle.rad_params(c, K=(0.2, 0, 0, 0), coupling="neq", photon_energy=2.976e-11, photon_rate=4.9e48)
+    Qf = oracle.fix_primitives(p, Q, np.zeros_like(Q))
+    U0 = oracle.UfromQ_grid(Qf, p.gamma)
+    R0 = oracle.rad_init_field(r, (m, m, m), st, dx, Qf)
+    t0 = time.perf_counter()
+    oracle.rad_run(r, p, st, 2, U0, R0, 1)
+    t_rad = (time.perf_counter() - t0) / 2.0
+    return {"n": m, "cores": 1, "kind": "port", "ray_trace_mcells_per_s": round(m ** 3 / t_rt / 1e6, 3),
+            "radiation_mcells_per_s": round(m ** 3 / t_rad / 1e6, 3),
+            "sample": f"{m}^3 star workload: one ray trace, one radiation step of two sub-steps"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -238,6 +272,8 @@ def main():
     ap.add_argument("--no-cooling", action="store_true")
     ap.add_argument("--surrogate", default=os.path.join(ROOT, "torch_amd", "data", "surrogate_cooling.npz"))
     ap.add_argument("--gate", type=float, default=0.01)
+    ap.add_argument("--star-n", type=int, default=256, help="star workload (ray trace, radiation) grid")
+    ap.add_argument("--no-star", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}); "
@@ -257,6 +293,15 @@ def main():
         t0 = time.perf_counter()
         cool = cooling_rates(args.cooling_log2, args.surrogate, args.gate)
         log(f"cooling {cool} ({time.perf_counter() - t0:.1f}s)")
+    star = None
+    if not args.no_star:
+        from torch_amd import star_bench
+
+        t0 = time.perf_counter()
+        star = star_bench.gpu_rates(args.star_n, 5)
+        if not args.no_cpu:
+            star["cpu_baseline"] = star_cpu_baseline(48)
+        log(f"star {star} ({time.perf_counter() - t0:.1f}s)")
     ncell = args.n ** 3
     achieved = BYTES_STAGE2 * ncell / (mx["stage2_ms"] * 1e-3) / 1e9
     traffic = measured_traffic(args.n, "void tg::hydro_stage_mixed_kernel<2, true, 2>")
@@ -286,6 +331,7 @@ def main():
                      "bytes_per_cell": BYTES_STAGE2, "launch_ms": round(mx["stage2_ms"], 4)},
         "cpu_baseline": cpu,
         "cooling": cool,
+        "star": star,
     }
     print(json.dumps(out), flush=True)
 

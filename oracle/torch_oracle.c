@@ -1016,6 +1016,36 @@ void orc_star_cell_geometry(const int n[3], const int star[3], double dx, double
     }
 }
 
+/* The cells sorted by Chebyshev distance max_d |i_d - star_d| to the star
+ * (a counting sort; within a shell by flat index).  Every upstream
+ * neighbour of a cell is in the previous shell, so this order is causal. */
+static int64_t* shell_order(const int n[3], const int star[3]) {
+    const int64_t nc = (int64_t)n[0] * n[1] * n[2];
+    int kmax = 0;
+    for (int a = 0; a < 3; ++a) {
+        int m = star[a] > n[a] - 1 - star[a] ? star[a] : n[a] - 1 - star[a];
+        if (m > kmax) kmax = m;
+    }
+    int64_t* start = calloc((size_t)kmax + 2, sizeof(int64_t));
+    int* shell = malloc(sizeof(int) * (size_t)nc);
+    for (int64_t c = 0; c < nc; ++c) {
+        const int ijk[3] = {(int)(c % n[0]), (int)((c / n[0]) % n[1]), (int)(c / ((int64_t)n[0] * n[1]))};
+        int cheb = 0;
+        for (int a = 0; a < 3; ++a) {
+            int e = abs(ijk[a] - star[a]);
+            if (e > cheb) cheb = e;
+        }
+        shell[c] = cheb;
+        start[cheb + 1]++;
+    }
+    for (int k = 0; k <= kmax; ++k) start[k + 1] += start[k];
+    int64_t* order = malloc(sizeof(int64_t) * (size_t)nc);
+    for (int64_t c = 0; c < nc; ++c) order[start[shell[c]]++] = c;
+    free(start);
+    free(shell);
+    return order;
+}
+
 void orc_star_geometry(const int n[3], const int star[3], double dx, double pi, int64_t* nb, double* w, double* ds,
                        double* shell_vol) {
     for (int k = 0; k < n[2]; ++k)
@@ -1036,21 +1066,12 @@ void orc_star_geometry(const int n[3], const int star[3], double dx, double pi, 
 void orc_thermo_ray_trace(const int n[3], const int star[3], const int64_t* nb, const double* w, const double* ds,
                           double hydrogen_mass, const double* den, double* col, double* dcol) {
     const int64_t ncell = (int64_t)n[0] * n[1] * n[2];
-    int kmax = 0;
-    for (int a = 0; a < 3; ++a) {
-        int m = star[a] > n[a] - 1 - star[a] ? star[a] : n[a] - 1 - star[a];
-        if (m > kmax) kmax = m;
-    }
-    for (int shell = 0; shell <= kmax; ++shell) {
-        for (int64_t c = 0; c < ncell; ++c) {
+    int64_t* order = shell_order(n, star);
+    {
+        for (int64_t o = 0; o < ncell; ++o) {
+            const int64_t c = order[o];
             const int i = (int)(c % n[0]), j = (int)((c / n[0]) % n[1]), k = (int)(c / ((int64_t)n[0] * n[1]));
             const int ijk[3] = {i, j, k};
-            int cheb = 0;
-            for (int a = 0; a < 3; ++a) {
-                int e = abs(ijk[a] - star[a]);
-                if (e > cheb) cheb = e;
-            }
-            if (cheb != shell) continue;
             double dist2 = 0;
             for (int a = 0; a < 3; ++a) {
                 double e = ((double)ijk[a] + 0.5) - ((double)star[a] + 0.5);
@@ -1075,6 +1096,7 @@ void orc_thermo_ray_trace(const int n[3], const int star[3], const int64_t* nb, 
             dcol[c] = (den[c] / hydrogen_mass) * ds[c];
         }
     }
+    free(order);
 }
 
 /* Fluid::initialiseGrid's CausalWind test (Fluid.cpp:41-54): a cell is a
@@ -1443,35 +1465,32 @@ static int rad_transfer(const orc_rad_params* r, const int n[3], const int star[
         Q[HII * nc + c] = 1;
         R[ORC_R_HII_A * nc + c] = 1;
     }
-    int kmax = 0;
-    for (int a = 0; a < 3; ++a) {
-        int m = star[a] > n[a] - 1 - star[a] ? star[a] : n[a] - 1 - star[a];
-        if (m > kmax) kmax = m;
-    }
-    for (int shell = 0; shell <= kmax; ++shell)
-        for (int64_t c = 0; c < nc; ++c) {
+    int64_t* order = shell_order(n, star);
+    int err = 0;
+    for (int64_t o = 0; o < nc && !err; ++o) {
+            const int64_t c = order[o];
             if (wind[c]) continue;
             const int ijk[3] = {(int)(c % n[0]), (int)((c / n[0]) % n[1]), (int)(c / ((int64_t)n[0] * n[1]))};
-            int cheb = 0;
             double dist2 = 0;
             for (int a = 0; a < 3; ++a) {
-                int e = abs(ijk[a] - star[a]);
-                if (e > cheb) cheb = e;
                 double d = ((double)ijk[a] + 0.5) - ((double)star[a] + 0.5);
                 dist2 += d * d;
             }
-            if (cheb != shell) continue;
             int64_t nb[4];
             double w[4], ds, sv;
             orc_star_cell_geometry(n, star, dx, pi, ijk[0], ijk[1], ijk[2], nb, w, &ds, &sv);
             rad_update_tau(r, nc, c, nb, w, dist2, 0, R);
             rad_update_tau(r, nc, c, nb, w, dist2, 1, R);
-            if (rad_update_hii(r, nc, c, ds, sv, dt, Q, R)) return 1;
+            if (rad_update_hii(r, nc, c, ds, sv, dt, Q, R)) {
+                err = 1;
+                break;
+            }
             double nH = r->mass_frac_h * Q[DEN * nc + c] / r->hydrogen_mass;
             R[ORC_R_DTAU * nc + c] = ((1.0 - Q[HII * nc + c]) * nH) * r->sigma * ds;
             R[ORC_R_DTAU_A * nc + c] = ((1.0 - R[ORC_R_HII_A * nc + c]) * nH) * r->sigma * ds;
-        }
-    return 0;
+    }
+    free(order);
+    return err;
 }
 
 /* Radiation::transferRadiation2 with a star (Radiation.cpp:510-567), the
@@ -1492,24 +1511,17 @@ static int rad_transfer2(const orc_rad_params* r, const int n[3], const int star
         Q[HII * nc + c] = 1;
         R[ORC_R_HII_A * nc + c] = 1;
     }
-    int kmax = 0;
-    for (int a = 0; a < 3; ++a) {
-        int m = star[a] > n[a] - 1 - star[a] ? star[a] : n[a] - 1 - star[a];
-        if (m > kmax) kmax = m;
-    }
-    for (int shell = 0; shell <= kmax; ++shell)
-        for (int64_t c = 0; c < nc; ++c) {
+    int64_t* order = shell_order(n, star);
+    int err = 0;
+    for (int64_t o = 0; o < nc && !err; ++o) {
+            const int64_t c = order[o];
             if (wind[c]) continue;
             const int ijk[3] = {(int)(c % n[0]), (int)((c / n[0]) % n[1]), (int)(c / ((int64_t)n[0] * n[1]))};
-            int cheb = 0;
             double dist2 = 0;
             for (int a = 0; a < 3; ++a) {
-                int e = abs(ijk[a] - star[a]);
-                if (e > cheb) cheb = e;
                 double d = ((double)ijk[a] + 0.5) - ((double)star[a] + 0.5);
                 dist2 += d * d;
             }
-            if (cheb != shell) continue;
             int64_t nb[4];
             double w[4], ds, sv;
             orc_star_cell_geometry(n, star, dx, pi, ijk[0], ijk[1], ijk[2], nb, w, &ds, &sv);
@@ -1517,7 +1529,9 @@ static int rad_transfer2(const orc_rad_params* r, const int n[3], const int star
             double nH = r->mass_frac_h * Q[DEN * nc + c] / r->hydrogen_mass;
             R[ORC_R_DTAU * nc + c] = ((1.0 - Q[HII * nc + c]) * nH) * r->sigma * ds;
             R[ORC_R_DTAU_A * nc + c] = ((1.0 - R[ORC_R_HII_A * nc + c]) * nH) * r->sigma * ds;
-        }
+    }
+    free(order);
+    (void)err;
     for (int64_t c = 0; c < nc; ++c) {
         if (wind[c]) continue;
         int64_t nb[4];

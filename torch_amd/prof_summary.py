@@ -6,6 +6,8 @@
         --sq Q/bench_counter_collection.csv --commands cmds.txt \\
         --md profiles/hydro_stage_turb256.md --json profiles/hydro_stage_turb256_traffic.json
 
+    python -m torch_amd.prof_summary db RESULTS.db [REGEX]   # kernel table of a rocprofv3 database
+
 Inputs are the CSV files rocprofv3 writes for `--kernel-trace --stats` and
 for `--pmc` runs of bench.py and of torch_amd.hbm_probe (a device copy of
 --copy-mib MiB, read and written once per dispatch).  FETCH_SIZE and
@@ -22,6 +24,7 @@ import collections
 import csv
 import json
 import re
+import sys
 
 KEEP = re.compile(r"hydro_stage|q_float|q_from_u|u_from_q|step_begin|step_end|cfl_kernel|scale_rotate|state_max")
 
@@ -55,7 +58,29 @@ def copy_factor(path: str, counter: str, copy_bytes: float) -> float:
     return copy_bytes / (max(vals) * 1024.0)
 
 
+def db_kernel_table(path: str, keep: str = ".") -> list[str]:
+    """Markdown rows (calls, total, average, min, max in µs) per kernel from a
+    rocprofv3 SQLite database (its `kernels` view), for kernel names matching
+    the regular expression `keep`, largest total first."""
+    import sqlite3
+
+    rows = sqlite3.connect(path).execute(
+        "select name, count(*), sum(duration), avg(duration), min(duration), max(duration) from kernels "
+        "group by name order by sum(duration) desc").fetchall()
+    out = ["| kernel | calls | total µs | avg µs | min µs | max µs |", "|---|---|---|---|---|---|"]
+    pat = re.compile(keep)
+    for name, calls, tot, avg, mn, mx in rows:
+        k = short(name)
+        if pat.search(k):
+            out.append(f"| `{k}` | {calls} | {tot / 1e3:.1f} | {avg / 1e3:.2f} | {mn / 1e3:.2f} | {mx / 1e3:.2f} |")
+    return out
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "db":
+        # python -m torch_amd.prof_summary db RESULTS.db [REGEX]
+        print("\n".join(db_kernel_table(sys.argv[2], sys.argv[3] if len(sys.argv) > 3 else ".")))
+        return
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--grid", type=int, required=True)
     ap.add_argument("--stats", required=True)
