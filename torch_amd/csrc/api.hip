@@ -125,6 +125,8 @@ int run_stage(torchgpu_ctx* c, int mode, double dtf) {
     if (c->p.precision == TORCHGPU_FP64) {
         if (mode == 2) HIP_TRY(c, tg::launch_hydro_stage(c->g, solver, 2, c->Ustar, c->U, c->U, c->st, c->stream, dtf));
         else HIP_TRY(c, tg::launch_hydro_stage(c->g, solver, 1, c->U, c->U, c->Ustar, c->st, c->stream, dtf));
+        if (mode == 0 && (dtf == tg::DT_HALF || dtf == tg::DT_FULL))
+            HIP_TRY(c, tg::launch_halted_copy(c->g, c->st, c->U, c->Ustar, nullptr, nullptr, c->stream));
         if (mode == 0) std::swap(c->U, c->Ustar);
         return TORCHGPU_OK;
     }
@@ -140,6 +142,8 @@ int run_stage(torchgpu_ctx* c, int mode, double dtf) {
     if (!c->qvalid) HIP_TRY(c, tg::launch_q_float(c->g, c->U, q0, 1, c->stream));
     HIP_TRY(c, tg::launch_hydro_stage_mixed(c->g, solver, 1, q0, c->U, c->Ustar, q1, mode == 0, c->st, c->stream,
                                             dtf));
+    if (mode == 0 && (dtf == tg::DT_HALF || dtf == tg::DT_FULL))
+        HIP_TRY(c, tg::launch_halted_copy(c->g, c->st, c->U, c->Ustar, q0, q1, c->stream));
     if (mode == 1) {
         c->qvalid = true;
         c->qstar_valid = true;

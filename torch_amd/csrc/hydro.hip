@@ -166,13 +166,10 @@ __global__ __launch_bounds__(NTHREADS) void hydro_stage_kernel(HydroGeom g, cons
                                                               const double* Ubase, double* Udst, StepState* st,
                                                               double dtf_host) {
     __shared__ TileLDS s;
+    if (st->halted) return;  // a halted sub-step's pass-through is launch_halted_copy's
     const int tile = tile_of_block(blockIdx.x, g.ntiles);
     const int tx = tile % g.ntx, ty = (tile / g.ntx) % g.nty, tz = tile / (g.ntx * g.nty);
     const int o[3] = {tx * TILE, ty * TILE, tz * TILE};
-    if (st->halted) {
-        copy_through_halted(g, dtf_host, Ubase, Udst, o);
-        return;
-    }
     const int tid = threadIdx.x;
     const GasConsts<double> gc = {g.g, g.gm1, g.g2};
 
@@ -376,6 +373,33 @@ __global__ void apply_rate_fix_kernel(HydroGeom g, double* __restrict__ U, const
 #pragma unroll
         for (int v = 0; v < 7; ++v) U[v * g.ncell + c] = u[v];
     }
+}
+
+// A halted run's single-stage sub-step (dt code DT_HALF / DT_FULL) still
+// passes its input through to its output, float primitives included: the
+// host swaps the two state buffers after every such sub-step without knowing
+// whether the run halted.  Launched after each such sub-step; returns at
+// once while the run is going.
+__global__ void halted_copy_kernel(HydroGeom g, const StepState* st, const double* __restrict__ Usrc,
+                                   double* __restrict__ Udst, const float* __restrict__ qin,
+                                   float* __restrict__ qout) {
+    if (!st->halted) return;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < g.ncell; c += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int v = 0; v < 7; ++v) Udst[v * g.ncell + c] = Usrc[v * g.ncell + c];
+        if (qin) {
+#pragma unroll
+            for (int v = 0; v < 7; ++v) qout[v * g.ncell + c] = qin[v * g.ncell + c];
+        }
+    }
+}
+
+hipError_t launch_halted_copy(const HydroGeom& g, const StepState* st, const double* Usrc, double* Udst,
+                              const float* qin, float* qout, hipStream_t stream) {
+    long long nb = (g.ncell + 255) / 256;
+    if (nb > 8192) nb = 8192;
+    hipLaunchKernelGGL(halted_copy_kernel, dim3((unsigned)nb), dim3(256), 0, stream, g, st, Usrc, Udst, qin, qout);
+    return hipGetLastError();
 }
 
 hipError_t launch_fix_solution(const HydroGeom& g, double* U, const double* rate, double dtf, StepState* st,

@@ -143,24 +143,6 @@ __device__ __forceinline__ double stage_dtf(double dtf, const StepState* st, int
     return stage == 1 ? st->dt / 2.0 : st->dt;
 }
 
-// A halted run's single-stage sub-step (dt code DT_HALF / DT_FULL) still
-// copies its input to its output, tile by tile: the host swaps the two state
-// buffers after every such sub-step without knowing whether the run halted.
-__device__ __forceinline__ bool copy_through_halted(const HydroGeom& g, double dtf, const double* __restrict__ Usrc,
-                                                    double* __restrict__ Udst, const int o[3]) {
-    if (!(dtf == DT_HALF || dtf == DT_FULL) || Udst == Usrc) return false;
-    const int tid = threadIdx.x;
-    if (tid < TCELLS) {
-        const int x = o[0] + (tid & 7), y = o[1] + ((tid >> 3) & 7), z = o[2] + (tid >> 6);
-        if (x < g.nx && y < g.ny && z < g.nz) {
-            const long long c = (long long)x + (long long)g.nx * ((long long)y + (long long)g.ny * z);
-#pragma unroll
-            for (int v = 0; v < 7; ++v) Udst[v * g.ncell + c] = Usrc[v * g.ncell + c];
-        }
-    }
-    return true;
-}
-
 // Records error `bit` for the current step unless an earlier step failed
 // first.  (Lane aggregation with a ballot and coherent pre-check loads here
 // slowed the mixed stage kernel by half even with no error raised: a failing

@@ -314,23 +314,13 @@ __global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(6))) v
     HydroGeom g, const float* __restrict__ qin, const double* Ubase, double* Udst, float* __restrict__ qout,
     int qout_fix, StepState* st, double dtf_host) {
     __shared__ MixedLDS s;
+    // A halted sub-step's pass-through is launch_halted_copy's: a copy branch
+    // here (never taken while running) made every stage 45% slower at 256^3.
+    if (st->halted) return;
     const int tile = tile_of_block(blockIdx.x, g.ntiles);
     const int tx = tile % g.ntx, ty = (tile / g.ntx) % g.nty, tz = tile / (g.ntx * g.nty);
     const int o[3] = {tx * TILE, ty * TILE, tz * TILE};
     const int tid = threadIdx.x;
-    if (st->halted) {
-        // a halted single-stage sub-step passes its input through, float
-        // primitives included (they are fixed ones for a single stage)
-        if (copy_through_halted(g, dtf_host, Ubase, Udst, o) && tid < TCELLS) {
-            const int x = o[0] + (tid & 7), y = o[1] + ((tid >> 3) & 7), z = o[2] + (tid >> 6);
-            if (x < g.nx && y < g.ny && z < g.nz) {
-                const long long c = (long long)x + (long long)g.nx * ((long long)y + (long long)g.ny * z);
-#pragma unroll
-                for (int v = 0; v < 7; ++v) qout[v * g.ncell + c] = qin[v * g.ncell + c];
-            }
-        }
-        return;
-    }
     const GasConsts<float> gc = {g.gf, g.gm1f, g.g2f};
     const MixedScale m = mixed_scale(st);
     const int ks[7] = {m.kr, m.kp, 0, 0, m.kv, m.kv, m.kv};

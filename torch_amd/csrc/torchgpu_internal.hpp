@@ -168,6 +168,10 @@ void shell_faces(const StarGeom& s, int k, ShellFaces& f);
 int shell_max(const StarGeom& s);
 hipError_t launch_star_geometry(const StarGeom& s, long long* nb, double* w, double* ds, double* shell_vol,
                                 hipStream_t stream);
+// Pass-through of a halted single-stage sub-step (Usrc -> Udst, and the
+// float primitives qin -> qout when qin is given); a no-op while running.
+hipError_t launch_halted_copy(const HydroGeom& g, const StepState* st, const double* Usrc, double* Udst,
+                              const float* qin, float* qout, hipStream_t stream);
 hipError_t launch_copy_if_running(const StepState* st, const double* src, double* dst, long long n,
                                   hipStream_t stream);
 hipError_t launch_resume(StepState* st, hipStream_t stream);
