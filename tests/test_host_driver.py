@@ -364,3 +364,30 @@ def test_driver_star_radiation_cooling_matches_python_loop(tmp_path, gpu):
     assert np.isfinite(U).all() and dts[1] > dts[0]
     assert not np.array_equal(U[2], outs[0][2])  # the star ionised gas
     np.testing.assert_array_equal(outs[steps], U)
+
+
+@pytest.mark.gpu
+def test_driver_lua_setupfile_run(tmp_path):
+    """--setupfile=FILE.lua (Torch::setUpLua) drives a run: 4 steps from the
+    cloud set-up equal 4 steps from the primitives the same file produces
+    through --dump-setup (bitwise)."""
+    _exe_or_skip()
+    from test_lua_setup import CLOUD, PARAMS
+
+    pf = tmp_path / "p.lua"
+    pf.write_text(PARAMS)
+    sf = tmp_path / "cloud.lua"
+    sf.write_text(CLOUD)
+    qbin = tmp_path / "q.bin"
+    r = subprocess.run([EXE, f"--paramfile={pf}", f"--setupfile={sf}", f"--dump-setup={qbin}"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    outs = []
+    for setup in (f"--setupfile={sf}", f"--setup={qbin}"):
+        out = tmp_path / f"U{len(outs)}.bin"
+        r = subprocess.run([EXE, f"--paramfile={pf}", setup, "--steps=4", f"--output={out}"], capture_output=True,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        outs.append(np.fromfile(out))
+    assert np.isfinite(outs[0]).all()
+    np.testing.assert_array_equal(outs[0], outs[1])

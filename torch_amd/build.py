@@ -68,7 +68,7 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
-HOST_SOURCES = ["lua_config.cpp", "torch_gpu.cpp", "torch_gpu_main.cpp"]
+HOST_SOURCES = ["lua_config.cpp", "lua_setup.cpp", "torch_gpu.cpp", "torch_gpu_main.cpp"]
 HOST_EXE = os.path.join(HERE, "torch_gpu")
 
 

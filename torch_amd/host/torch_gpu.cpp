@@ -1,6 +1,8 @@
 // torch_gpu.cpp -- host layer over the C ABI (see torch_gpu.hpp).
 #include "torch_gpu.hpp"
 
+#include "lua_setup.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -302,6 +304,62 @@ torchgpu_star starOf(const TorchParameters& p) {
     s.wind_cell_radius = p.windCellRadius;
     s.photon_rate = p.photonRate;  // code units after TorchParameters::initialise
     return s;
+}
+
+std::vector<double> evalLuaSetup(const TorchParameters& p, const std::string& filename) {
+    const torchgpu_params g = p.gridParams();
+    const size_t n = (size_t)g.nx * g.ny * g.nz;
+    std::vector<double> Q(7 * n, 0.0);
+    lua::State L;
+    try {
+        L.load_file(filename);
+    } catch (lua::Error& e) {
+        throw std::runtime_error("Torch::setUpLua: could not open lua file: " + filename + "\n" + e.what());
+    }
+    // Grid::initialise: dx = sideLength / ncells[0]; cell centres at index +
+    // 0.5, the star at its cell + 0.5 (Star.cpp:17-21), both times dx and
+    // converted to cm as setUpLua does
+    const double dx = p.sideLength / g.nx;
+    double xs[3];
+    for (int a = 0; a < 3; ++a) {
+        const double mod = (p.faceSnap[a] && p.star_position[a] == 0) ? 0.5 : 0.0;
+        xs[a] = convert(p, (p.star_position[a] + 0.5 - mod) * dx, 0, 1, 0, false);
+    }
+    for (int k = 0; k < g.nz; ++k)
+        for (int j = 0; j < g.ny; ++j)
+            for (int i = 0; i < g.nx; ++i) {
+                const int idx[3] = {i, j, k};
+                std::vector<double> args(6);
+                for (int a = 0; a < 3; ++a) {
+                    args[a] = convert(p, ((double)idx[a] + 0.5) * dx, 0, 1, 0, false);
+                    args[3 + a] = xs[a];
+                }
+                std::vector<lua::Value> r;
+                try {
+                    r = L.call("initialise", args);
+                } catch (lua::Error& e) {
+                    throw std::runtime_error(std::string("Torch::setUpLua: ") + e.what());
+                }
+                double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                for (size_t q = 0; q < 9 && q < r.size(); ++q) {
+                    if (r[q].kind == lua::Value::NUMBER) v[q] = r[q].num;
+                    else if (r[q].kind != lua::Value::NIL)
+                        throw std::runtime_error("Torch::setUpLua: initialise returned a non-number");
+                }
+                if (v[6] != 0 || v[7] != 0 || v[8] != 0)
+                    throw std::runtime_error("Torch::setUpLua: gravity (returns 7-9) is not modelled on the GPU path");
+                const size_t c = (size_t)i + (size_t)g.nx * ((size_t)j + (size_t)g.ny * k);
+                Q[0 * n + c] = v[0];
+                Q[1 * n + c] = v[1];
+                Q[2 * n + c] = v[2];
+                for (int d = 0; d < 3; ++d) Q[(4 + d) * n + c] = v[3 + d];
+            }
+    return Q;
+}
+
+void Torch::setUpLua(const std::string& filename) {
+    const std::vector<double> Q = evalLuaSetup(params, filename);
+    setPrimitivesCgs(Q.data());
 }
 
 Radiation::~Radiation() {

@@ -196,12 +196,20 @@ private:
 // snapping is rejected).
 torchgpu_star starOf(const TorchParameters& p);
 
+// Torch::setUpLua (Torch.cpp:192-228) without the GPU: runs the set-up file
+// and calls its initialise(x, y, z, xs, ys, zs) for every cell with the cell
+// centre and the star position in cm; returns the primitives in cgs
+// [7][nz][ny][nx] (DEN, PRE, HII, VEL0..2; ADV zero).  p must have been
+// initialised (code-unit side length).  Gravity must come back zero.
+std::vector<double> evalLuaSetup(const TorchParameters& p, const std::string& filename);
+
 // The driver of src/Torch/Torch.cpp for hydro-only runs (device-resident
 // time loop) and the operator-split sequence with the integrators above.
 class Torch {
 public:
     void initialise(const TorchParameters& p, void* stream = nullptr);
     void setPrimitivesCgs(const double* Q);   // Torch::toCodeUnits + fixPrimitives + globalUfromQ
+    void setUpLua(const std::string& filename);  // Torch::setUpLua + the tail of Torch::initialise
     // Torch::run, capped; with `print`, the data2D snapshots of
     // Integration.output_directory at the start, every checkpoint and the end
     void run(int max_steps, bool print = false);

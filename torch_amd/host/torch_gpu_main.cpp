@@ -12,7 +12,11 @@
 // length, state and time from the file); a non-empty initial_conditions in
 // the parameter file does the same.  --snapshots writes the data2D files of
 // Torch::run to Integration.output_directory: the initial state, every
-// checkpoint and the final state.
+// checkpoint and the final state.  --setupfile=FILE.lua runs the set-up
+// file's initialise(x, y, z, xs, ys, zs) for every cell, as the reference's
+// --setupfile (main.cpp:36-50, Torch::setUpLua); --dump-setup=FILE.bin
+// writes the primitives it returns (cgs, float64 [7][nz][ny][nx]) and exits
+// without touching the GPU.
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -68,8 +72,8 @@ void dump(const tg::TorchParameters& p) {
 int main(int argc, char** argv) {
     const std::string paramfile = arg(argc, argv, "--paramfile", "");
     if (paramfile.empty()) {
-        std::fprintf(stderr, "usage: torch_gpu --paramfile=FILE [--setup=sod|FILE.bin|FILE.txt[.gz]] [--steps=N] "
-                             "[--output=FILE.bin] [--snapshots] [--dump-params]\n");
+        std::fprintf(stderr, "usage: torch_gpu --paramfile=FILE [--setupfile=FILE.lua | --setup=sod|FILE.bin|FILE.txt[.gz]] "
+                             "[--steps=N] [--output=FILE.bin] [--snapshots] [--dump-params] [--dump-setup=FILE.bin]\n");
         return 2;
     }
     try {
@@ -79,6 +83,17 @@ int main(int argc, char** argv) {
         if (arg(argc, argv, "--dump-params", "") == "1") {
             dump(p);
             return 0;
+        }
+        const std::string setupfile = arg(argc, argv, "--setupfile", "");
+        const std::string dump_setup = arg(argc, argv, "--dump-setup", "");
+        if (!dump_setup.empty()) {
+            if (setupfile.empty()) throw std::runtime_error("--dump-setup needs --setupfile");
+            tg::TorchParameters c = p;
+            c.initialise();
+            const std::vector<double> Q = tg::evalLuaSetup(c, setupfile);
+            std::ofstream o(dump_setup, std::ios::binary);
+            o.write(reinterpret_cast<const char*>(Q.data()), (std::streamsize)(Q.size() * sizeof(double)));
+            return o ? 0 : 1;
         }
         const std::string setup = arg(argc, argv, "--setup", p.initialConditions.empty() ? "sod" : "");
         auto ends_with = [](const std::string& a, const std::string& b) {
@@ -92,6 +107,8 @@ int main(int argc, char** argv) {
         std::vector<double> Q(7 * n, 0.0);
         if (!torch.params.initialConditions.empty()) {
             // state and time already read by Torch::initialise
+        } else if (!setupfile.empty()) {
+            torch.setUpLua(setupfile);
         } else if (setup == "sod") {
             for (int k = 0; k < nz; ++k)
                 for (int j = 0; j < ny; ++j)
@@ -106,7 +123,7 @@ int main(int argc, char** argv) {
             if (!f.read(reinterpret_cast<char*>(Q.data()), (std::streamsize)(Q.size() * sizeof(double))))
                 throw std::runtime_error("cannot read " + setup);
         }
-        if (torch.params.initialConditions.empty()) torch.setPrimitivesCgs(Q.data());
+        if (torch.params.initialConditions.empty() && setupfile.empty()) torch.setPrimitivesCgs(Q.data());
         const int steps = std::stoi(arg(argc, argv, "--steps", "100"));
         auto t0 = std::chrono::steady_clock::now();
         torch.run(steps, arg(argc, argv, "--snapshots", "") == "1");
