@@ -219,6 +219,17 @@ int torchgpu_snapshot_header(const char* path, double* t, int* nx, int* ny, int*
  * [7][nz][ny][nx]); cells outside the grid are skipped, other entries of Q
  * are left as they are. */
 int torchgpu_snapshot_read(const char* path, double* Q, int nx, int ny, int nz);
+/* DataReader::patchGrid (src/IO/DataReader.cpp:126-231), on host primitives
+ * Q [7][nz][ny][nx] as they stand before Torch::toCodeUnits: each patch
+ * cell lands in grid cell (int)(coordinate / grid_dx + offset); the covered
+ * cells are zeroed, then receive the sum of their patch cells' density,
+ * pressure, HII fraction and velocities divided by r = ratio^nd, ratio =
+ * grid_dx / patch dx.  grid_dx is the grid's cell width as the reference's
+ * Grid holds it (code units -- the reference divides the patch's cgs
+ * coordinates by it).  The ratio and the covered cell counts must be
+ * integers and nd must match. */
+int torchgpu_snapshot_patch(const char* path, double* Q, int nx, int ny, int nz, double grid_dx, int ox, int oy,
+                            int oz);
 
 /* ------------------------------------------------------------------------ */
 /* Thermodynamics (cooling) operator                                         */

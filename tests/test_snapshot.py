@@ -78,3 +78,53 @@ def test_three_dimensional_state(tmp_path):
     for v in (0, 1, 2, 4, 5, 6):
         np.testing.assert_allclose(Q2[v], Q[v], rtol=1e-10, atol=1e-10)
     assert not Q2[3].any()  # ADV is not stored (lossy resume, as the reference)
+
+
+def test_patch_averages_covered_cells(tmp_path):
+    """DataReader::patchGrid: a 4 x 4 x 4 patch of half-width cells at offset
+    (1, 2, 0) zeroes the 2 x 2 x 2 grid cells it covers and fills each with
+    the mean of its 8 patch cells; the rest of the grid is untouched."""
+    rng = np.random.default_rng(2)
+    P = np.zeros((7, 4, 4, 4))
+    P[0] = rng.uniform(1, 2, (4, 4, 4))
+    P[1] = rng.uniform(1, 2, (4, 4, 4))
+    P[2] = rng.uniform(0, 1, (4, 4, 4))
+    P[4:7] = rng.normal(size=(3, 4, 4, 4))
+    f = str(tmp_path / "patch.txt")
+    ta.snapshot_write(f, 0.0, P, 4 * 0.5)  # patch dx 0.5
+    P, _ = ta.snapshot_read(f)  # the values as printed (%.10e)
+    Q = np.full((7, 3, 5, 6), 7.0)
+    Q[3] = 0.25
+    ta.snapshot_patch(f, Q, 1.0, offset=(1, 2, 0))
+    blocks = P.reshape(7, 2, 2, 2, 2, 2, 2)
+    for k in range(3):
+        for j in range(5):
+            for i in range(6):
+                inside = 1 <= i < 3 and 2 <= j < 4 and 0 <= k < 2
+                for v in (0, 1, 2, 4, 5, 6):
+                    want = blocks[v, k, :, j - 2, :, i - 1, :].mean() if inside else 7.0
+                    assert Q[v, k, j, i] == pytest.approx(want, rel=1e-15), (v, k, j, i)
+                assert Q[3, k, j, i] == 0.25  # ADV is not patched
+
+
+def test_patch_rejects_non_integer_ratio(tmp_path):
+    P = np.ones((7, 4, 4, 4))
+    f = str(tmp_path / "patch.txt")
+    ta.snapshot_write(f, 0.0, P, 4 * 0.3)
+    with pytest.raises(ta.TorchGpuError):
+        ta.snapshot_patch(f, np.ones((7, 4, 4, 4)), 1.0)
+
+
+def test_reference_file_patch():
+    """The reference's 150 x 200 snapshot as a patch: at the file's own cell
+    width it reproduces the file's cells (sum of one cell / 1); at twice the
+    width it is refused, as the reference's check of the covered cell counts
+    runs over all three axes and a 2-D file has one cell along z."""
+    Qf, h = ta.snapshot_read(FIX)
+    Q = np.full((7, 1, 200, 150), 3.0)
+    ta.snapshot_patch(FIX, Q, 1.03e16)  # the file's dx, 1.03e16 cm
+    for v in (0, 1, 2, 4, 5):
+        np.testing.assert_array_equal(Q[v], Qf[v])
+    assert (Q[3] == 3.0).all() and (Q[6] == 3.0).all()  # ADV and the third velocity untouched in 2-D
+    with pytest.raises(ta.TorchGpuError):
+        ta.snapshot_patch(FIX, np.zeros((7, 1, 100, 75)), 2 * 1.03e16)

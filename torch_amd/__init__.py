@@ -309,6 +309,7 @@ def _bind_snapshot(L):
     L.torchgpu_snapshot_write.argtypes = [cp, i, d, vp, i, i, i, d, d, d, d]
     L.torchgpu_snapshot_header.argtypes = [cp] + [ctypes.POINTER(d)] + [ctypes.POINTER(i)] * 4 + [ctypes.POINTER(d)]
     L.torchgpu_snapshot_read.argtypes = [cp, vp, i, i, i]
+    L.torchgpu_snapshot_patch.argtypes = [cp, vp, i, i, i, d, i, i, i]
     L._snapshot_bound = True
 
 
@@ -339,6 +340,22 @@ def snapshot_header(path: str) -> dict:
     if rc:
         raise TorchGpuError(rc, f"snapshot_header: invalid input file {path}")
     return {"time": t.value, "shape": (nz.value, ny.value, nx.value), "nd": nd.value, "side_length": side.value}
+
+
+def snapshot_patch(path: str, Q, grid_dx: float, offset=(0, 0, 0)):
+    """DataReader::patchGrid on host primitives Q [7][nz][ny][nx] (modified
+    in place): the patch file's cells averaged into the grid cells they
+    cover, at `offset` cells."""
+    import numpy as np
+
+    L = lib()
+    _bind_snapshot(L)
+    assert isinstance(Q, np.ndarray) and Q.dtype == np.float64 and Q.flags.c_contiguous
+    nz, ny, nx = Q.shape[1:]
+    rc = L.torchgpu_snapshot_patch(path.encode(), Q.ctypes.data, nx, ny, nz, grid_dx, *[int(o) for o in offset])
+    if rc:
+        raise TorchGpuError(rc, f"DataReader::patchGrid: cannot patch with {path}")
+    return Q
 
 
 def snapshot_read(path: str, shape=None):

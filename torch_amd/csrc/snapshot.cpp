@@ -3,6 +3,7 @@
 //   torchgpu_snapshot_write   DataPrinter::print2D   (src/IO/DataPrinter.cpp:139-174)
 //   torchgpu_snapshot_header  DataReader::readDataParameters (src/IO/DataReader.cpp:17-68)
 //   torchgpu_snapshot_read    DataReader::readGrid   (src/IO/DataReader.cpp:70-125)
+//   torchgpu_snapshot_patch   DataReader::patchGrid  (src/IO/DataReader.cpp:126-231)
 //
 // Format: the time (s), then nx, ny and nz, one per line; then one line per
 // cell, x fastest, with the nd cell-centre coordinates (cm), density
@@ -139,20 +140,20 @@ extern "C" int torchgpu_snapshot_write(const char* path, int gz, double t, const
     return ok ? TORCHGPU_OK : TORCHGPU_ERR_INVALID_ARG;
 }
 
-extern "C" int torchgpu_snapshot_header(const char* path, double* t, int* nx, int* ny, int* nz, int* nd,
-                                        double* side_length) {
-    if (!path || !t || !nx || !ny || !nz || !nd || !side_length) return TORCHGPU_ERR_INVALID_ARG;
+namespace {
+
+// DataReader::readDataParameters (DataReader.cpp:17-68): time, cell counts,
+// nd and dx, the smallest nonzero coordinate difference of the first two
+// cells listed.
+int read_header(const char* path, double* t, int n[3], int* nd, double* dx) {
     TokenIn in;
     if (!in.open(path)) return TORCHGPU_ERR_INVALID_ARG;
     double v[4];
     for (double& x : v)
         if (!in.number(x)) return TORCHGPU_ERR_INVALID_ARG;
     *t = v[0];
-    *nx = (int)v[1];
-    *ny = (int)v[2];
-    *nz = (int)v[3];
-    *nd = dims_of(*ny, *nz);
-    // the first two cells' coordinates give dx (DataReader.cpp:34-58)
+    for (int a = 0; a < 3; ++a) n[a] = (int)v[1 + a];
+    *nd = dims_of(n[1], n[2]);
     double x0[3] = {0, 0, 0}, x1[3] = {0, 0, 0}, skip;
     for (int d = 0; d < *nd; ++d)
         if (!in.number(x0[d])) return TORCHGPU_ERR_INVALID_ARG;
@@ -160,14 +161,89 @@ extern "C" int torchgpu_snapshot_header(const char* path, double* t, int* nx, in
         if (!in.number(skip)) return TORCHGPU_ERR_INVALID_ARG;
     for (int d = 0; d < *nd; ++d)
         if (!in.number(x1[d])) return TORCHGPU_ERR_INVALID_ARG;
-    double dx = 0.0;
-    for (int d = 0; d < *nd; ++d)
-        if (std::abs(x1[d] - x0[d]) > std::numeric_limits<double>::min()) {
-            dx = std::abs(x1[d] - x0[d]);
-            break;
+    bool any = false;
+    double m = 0.0;
+    for (int d = 0; d < *nd; ++d) {
+        const double e = std::abs(x1[d] - x0[d]);
+        if (e > std::numeric_limits<double>::min() && (!any || e < m)) {
+            m = e;
+            any = true;
         }
-    if (!(dx > 0.0)) return TORCHGPU_ERR_INVALID_ARG;
-    *side_length = dx * (*nx);
+    }
+    if (!any) return TORCHGPU_ERR_INVALID_ARG;
+    *dx = m;
+    return TORCHGPU_OK;
+}
+
+}  // namespace
+
+extern "C" int torchgpu_snapshot_header(const char* path, double* t, int* nx, int* ny, int* nz, int* nd,
+                                        double* side_length) {
+    if (!path || !t || !nx || !ny || !nz || !nd || !side_length) return TORCHGPU_ERR_INVALID_ARG;
+    int n[3];
+    double dx;
+    const int rc = read_header(path, t, n, nd, &dx);
+    if (rc) return rc;
+    *nx = n[0];
+    *ny = n[1];
+    *nz = n[2];
+    *side_length = dx * n[0];
+    return TORCHGPU_OK;
+}
+
+extern "C" int torchgpu_snapshot_patch(const char* path, double* Q, int nx, int ny, int nz, double grid_dx, int ox,
+                                       int oy, int oz) {
+    // DataReader::patchGrid (DataReader.cpp:126-231): the patch's cells
+    // (coordinates / grid_dx + offset, truncated) first zero the grid cells
+    // they land in, then add their values / r, r = (grid_dx / patch dx)^nd
+    if (!path || !Q || nx < 1 || ny < 1 || nz < 1 || !(grid_dx > 0.0)) return TORCHGPU_ERR_INVALID_ARG;
+    double t, pdx;
+    int pn[3], nd;
+    int rc = read_header(path, &t, pn, &nd, &pdx);
+    if (rc) return rc;
+    if (nd != dims_of(ny, nz)) return TORCHGPU_ERR_INVALID_ARG;  // "patch ndims != grid ndims"
+    const double ratio = grid_dx / pdx;
+    if (std::abs(ratio - (int)ratio) > std::numeric_limits<double>::min()) return TORCHGPU_ERR_INVALID_ARG;
+    for (int a = 0; a < 3; ++a) {
+        const double covered = pn[a] / ratio;
+        if (std::abs(covered - (int)covered) > std::numeric_limits<double>::min()) return TORCHGPU_ERR_INVALID_ARG;
+    }
+    const int r = (int)std::pow((int)(ratio + 0.5), nd);
+    const int off[3] = {ox, oy, oz};
+    const long long n = (long long)nx * ny * nz;
+    const long long npatch = (long long)pn[0] * pn[1] * pn[2];
+    for (int pass = 0; pass < 2; ++pass) {
+        TokenIn in;
+        if (!in.open(path)) return TORCHGPU_ERR_INVALID_ARG;
+        double skip;
+        for (int i = 0; i < 4; ++i)
+            if (!in.number(skip)) return TORCHGPU_ERR_INVALID_ARG;
+        for (long long p = 0; p < npatch; ++p) {
+            int xc[3] = {0, 0, 0};
+            for (int d = 0; d < nd; ++d) {
+                double x;
+                if (!in.number(x)) return TORCHGPU_ERR_INVALID_ARG;
+                xc[d] = (int)((x / grid_dx) + off[d]);
+            }
+            double val[6] = {0, 0, 0, 0, 0, 0};
+            for (int q = 0; q < 3 + nd; ++q)
+                if (!in.number(val[q])) return TORCHGPU_ERR_INVALID_ARG;
+            if (xc[0] < 0 || xc[0] > nx) continue;
+            if (xc[0] >= nx || xc[1] < 0 || xc[1] >= ny || xc[2] < 0 || xc[2] >= nz) continue;  // Grid::locate
+            const long long c = xc[0] + (long long)nx * (xc[1] + (long long)ny * xc[2]);
+            if (pass == 0) {
+                Q[0 * n + c] = 0;
+                Q[1 * n + c] = 0;
+                Q[2 * n + c] = 0;
+                for (int d = 0; d < nd; ++d) Q[(4 + d) * n + c] = 0;
+            } else {
+                Q[0 * n + c] += val[0] / r;
+                Q[1 * n + c] += val[1] / r;
+                Q[2 * n + c] += val[2] / r;
+                for (int d = 0; d < nd; ++d) Q[(4 + d) * n + c] += val[3 + d] / r;
+            }
+        }
+    }
     return TORCHGPU_OK;
 }
 

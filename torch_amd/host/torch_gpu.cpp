@@ -151,6 +151,13 @@ void parseParameters(const LuaConfig& cfg, TorchParameters& p, std::string* w) {
     take(G["no_cells_z"], p.ncells[2], "no_cells_z", w);
     take(G["side_length"], p.sideLength, "side_length", w);
     take(G["geometry"], p.geometry, "geometry", w);
+    if (G["Patch"].exists()) {
+        const LuaValue& Pt = G["Patch"];
+        take(Pt["filename"], p.patchfilename, "filename", w);
+        take(Pt["offset_x"], p.patchoffset[0], "offset_x", w);
+        take(Pt["offset_y"], p.patchoffset[1], "offset_y", w);
+        take(Pt["offset_z"], p.patchoffset[2], "offset_z", w);
+    }
     const char* xyz[3] = {"x", "y", "z"};
     for (int d = 0; d < 3; ++d) {
         take(G[std::string("left_boundary_condition_") + xyz[d]], p.leftBC[d], "left_boundary_condition", w);
@@ -461,6 +468,12 @@ void Torch::setPrimitivesCgs(const double* Qcgs) {
     const torchgpu_params g = params.gridParams();
     const size_t n = (size_t)g.nx * g.ny * g.nz;
     std::vector<double> Q(Qcgs, Qcgs + 7 * n);
+    if (!params.patchfilename.empty()) {
+        // Grid.Patch: DataReader::patchGrid after the set-up (Torch.cpp:110-111)
+        if (torchgpu_snapshot_patch(params.patchfilename.c_str(), Q.data(), g.nx, g.ny, g.nz, g.side_length / g.nx,
+                                    params.patchoffset[0], params.patchoffset[1], params.patchoffset[2]) != TORCHGPU_OK)
+            throw std::runtime_error("DataReader::patchGrid: cannot patch the grid with " + params.patchfilename);
+    }
     const double kd = params.toCodeUnits(1.0, 1, -3, 0), kp = params.toCodeUnits(1.0, 1, -1, -2),
                  kv = params.toCodeUnits(1.0, 0, 1, -1);
     const bool unit = (kd == 1.0 && kp == 1.0 && kv == 1.0);

@@ -32,6 +32,8 @@ struct TorchParameters {
     // defaults of src/Torch/Parameters.hpp:25-97
     std::string setupFile = "";
     std::string initialConditions = "";
+    std::string patchfilename = "";
+    std::array<int, 3> patchoffset{{0, 0, 0}};
     int nd = 0;
     double sideLength = 0;
     std::array<int, 3> ncells{{0, 0, 0}};
