@@ -310,7 +310,7 @@ __device__ __forceinline__ void load_tile(const HydroGeom& g, const float* __res
 // qin: float Q of the input state (fixPrimitives applied for stage 1);
 // qout: float Q of the output state for the next stage, fixed if qout_fix.
 template <int SOLVER, bool ALB, int STAGE>
-__global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(6))) void hydro_stage_mixed_kernel(
+__global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(7))) void hydro_stage_mixed_kernel(
     HydroGeom g, const float* __restrict__ qin, const double* Ubase, double* Udst, float* __restrict__ qout,
     int qout_fix, StepState* st, double dtf_host) {
     __shared__ MixedLDS s;
