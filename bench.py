@@ -226,13 +226,49 @@ def cooling_rates(log2_cells: int, weights: str, gate: float) -> dict:
 
 
 def star_cpu_baseline(m: int) -> dict:
-    """The oracle's single-threaded C restatement of the star operators on an
-    m^3 grid of the same medium: the column-density ray trace and one
-    radiation sub-step (half of a two-sub-step radiation step)."""
+    """The star operators on the host on an m^3 grid of the same medium: the
+    reference's own Thermodynamics::rayTrace and one Radiation sub-step
+    (implicit, neq; half of a two-sub-step step) through the reference
+    harness when it is present, else the oracle's single-threaded C
+    restatement of the same."""
     import numpy as np
 
-    import oracle
     from torch_amd import star_bench
+
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if os.path.exists(harness):
+        try:
+            import torch_amd as ta
+
+            units = lambda v, mi, li, ti: ta.units(star_bench.SCALES, v, mi, li, ti)  # noqa: E731
+            Q, side = star_bench.medium(m, units)
+            common = {"nx": m, "ny": m, "nz": m, "side_length": side, "density_scale": 1e-20, "pressure_scale": 1e-28,
+                      "time_scale": 1e11, "gamma": 1.67, "density_floor": 1e-30, "temperature_floor": 0.1,
+                      "mass_fraction_hydrogen": 1.0, "simulation_time": 1e13, "star_x": m // 2, "star_y": m // 2,
+                      "star_z": m // 2, "wind_radius": 2, "photon_rate": 4.9e48}
+            rad = {"photon_energy": 2.976e-11, "K1": 0.2, "coupling": "neq", "integration_scheme": "implicit",
+                   "photoion_cross_section": 6.3e-18, "case_b_recombination_coeff": 2.59e-13, "tau_0": 0.6,
+                   "temperature_hi": 300, "temperature_hii": 10000, "nsteps": 1}
+            with tempfile.TemporaryDirectory() as d:
+                Q.tofile(f"{d}/q.bin")
+                res = {}
+                for mode, extra in (("raytrace", {"output": f"{d}/rt.bin"}), ("radiation", dict(rad, output_prefix=f"{d}/rad"))):
+                    kv = dict(common, input_q=f"{d}/q.bin", **extra)
+                    with open(f"{d}/{mode}.txt", "w") as f:
+                        for k, v in kv.items():
+                            f.write(f"{k} {v}\n")
+                    r = subprocess.run([harness, mode, f"{d}/{mode}.txt"], capture_output=True, text=True, timeout=300)
+                    if r.returncode != 0:
+                        raise RuntimeError(r.stderr[-500:])
+                    res[mode] = json.loads(r.stdout.strip().splitlines()[-1])
+            return {"n": m, "cores": 1, "kind": "reference",
+                    "ray_trace_mcells_per_s": round(m ** 3 / res["raytrace"]["ray_trace_s"] / 1e6, 3),
+                    "radiation_mcells_per_s": round(m ** 3 / res["radiation"]["substep_s"] / 1e6, 3),
+                    "sample": f"{m}^3 star workload through the reference's own units (one process): "
+                              "Thermodynamics::rayTrace, one Radiation sub-step"}
+        except Exception as e:  # noqa: BLE001
+            log("reference harness unavailable for the star baseline:", e)
+    import oracle
 
     c = oracle.converter(*star_bench.SCALES)
     units = lambda v, mi, li, ti: oracle.to_code(c, v, mi, li, ti)  # noqa: E731
@@ -256,7 +292,7 @@ def star_cpu_baseline(m: int) -> dict:
     t_rad = (time.perf_counter() - t0) / 2.0
     return {"n": m, "cores": 1, "kind": "port", "ray_trace_mcells_per_s": round(m ** 3 / t_rt / 1e6, 3),
             "radiation_mcells_per_s": round(m ** 3 / t_rad / 1e6, 3),
-            "sample": f"{m}^3 star workload: one ray trace, one radiation step of two sub-steps"}
+            "sample": f"{m}^3 star workload, oracle restatement: one ray trace, one radiation step of two sub-steps"}
 
 
 def main():
