@@ -24,13 +24,35 @@
 namespace tg {
 
 // updateColDen (Thermodynamics.cpp:339-363) for the cells of one shell.
-__global__ __launch_bounds__(256) void coldens_shell_kernel(StarGeom s, ShellFaces f, const double* __restrict__ den,
-                                                            double* col, double* dcol) {
+//
+// The x-fixed faces of a shell have no contiguous direction in the x-fastest
+// grid, so their cells also keep COL_DEN and DCOL_DEN in face-contiguous
+// buffers xf (per side, y fastest, in a frame centred on the star, two
+// shells ping-ponged): every upstream neighbour of nonzero weight of an
+// x-face cell is on the same side's x-face of the previous shell.
+// Neighbours of zero weight are not read (their term is 0 x a finite
+// value, as the reference adds it).
+struct XFaces {
+    double* col;   // [2 parities][2 sides][w * w]
+    double* dcol;
+    int K, w;      // frame half-width (largest shell) and w = 2K + 1
+};
+
+__device__ __forceinline__ long long xface_index(const StarGeom& s, const XFaces& xf, int parity, int side, int y,
+                                                 int z) {
+    return ((long long)(parity * 2 + side) * xf.w + (z - s.s[2] + xf.K)) * xf.w + (y - s.s[1] + xf.K);
+}
+
+__global__ __launch_bounds__(256) void coldens_shell_kernel(StarGeom s, ShellFaces f, int k, XFaces xf,
+                                                            const double* __restrict__ den, double* col,
+                                                            double* dcol) {
     const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
     if (t >= f.start[f.nf]) return;
     int ijk[3];
     shell_cell(f, t, ijk);
     const long long c = star_flat(s, ijk[0], ijk[1], ijk[2]);
+    const bool xface = ijk[0] - s.s[0] == k || s.s[0] - ijk[0] == k;
+    const int side = ijk[0] > s.s[0] ? 1 : 0;
     // distance to the star in cell units
     double dist2 = 0;
 #pragma unroll
@@ -40,12 +62,22 @@ __global__ __launch_bounds__(256) void coldens_shell_kernel(StarGeom s, ShellFac
     }
     double v = 0;
     if (dist2 > 0.95 * 0.95) {
-        long long nb[4];
+        int nbc[4][3];
+        bool valid[4];
         double w[4], cd[4], wr[4];
-        star_neighbours(s, ijk, nb, w);
+        star_neighbour_cells(s, ijk, nbc, valid, w);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            cd[q] = nb[q] != -1 ? col[nb[q]] + dcol[nb[q]] : 0.0;
+            cd[q] = 0.0;
+            if (valid[q] && w[q] != 0.0) {
+                if (xface) {
+                    const long long i = xface_index(s, xf, (k - 1) & 1, side, nbc[q][1], nbc[q][2]);
+                    cd[q] = xf.col[i] + xf.dcol[i];
+                } else {
+                    const long long i = star_flat(s, nbc[q][0], nbc[q][1], nbc[q][2]);
+                    cd[q] = col[i] + dcol[i];
+                }
+            }
             wr[q] = cd[q] == 0 ? 0.0 : w[q] / cd[q];
         }
         const double sum_w = wr[0] + wr[1] + wr[2] + wr[3];
@@ -54,8 +86,17 @@ __global__ __launch_bounds__(256) void coldens_shell_kernel(StarGeom s, ShellFac
             for (int q = 0; q < 4; ++q) v += (wr[q] / sum_w) * cd[q];
         }
     }
+    const double dc = (den[c] / s.hydrogen_mass) * star_path_length(s, ijk);
     col[c] = v;
-    dcol[c] = (den[c] / s.hydrogen_mass) * star_path_length(s, ijk);
+    dcol[c] = dc;
+    if (xface) {
+        // the star's cell (k = 0) feeds both sides of shell 1
+        for (int sd = (k == 0 ? 0 : side); sd <= (k == 0 ? 1 : side); ++sd) {
+            const long long i = xface_index(s, xf, k & 1, sd, ijk[1], ijk[2]);
+            xf.col[i] = v;
+            xf.dcol[i] = dc;
+        }
+    }
 }
 
 __global__ void star_geometry_kernel(StarGeom s, long long* __restrict__ nb, double* __restrict__ w,
@@ -116,16 +157,27 @@ int shell_max(const StarGeom& s) {
     return kmax;
 }
 
-hipError_t launch_thermo_ray_trace(const StarGeom& s, const double* den, double* col, double* dcol,
+long long xface_doubles(const StarGeom& s) {
+    const long long w = 2LL * shell_max(s) + 1;
+    return 2 * 2 * 2 * w * w;  // col and dcol, 2 parities, 2 sides
+}
+
+hipError_t launch_thermo_ray_trace(const StarGeom& s, const double* den, double* col, double* dcol, double* xface,
                                    hipStream_t stream) {
     const int kmax = shell_max(s);
+    XFaces xf;
+    xf.K = kmax;
+    xf.w = 2 * kmax + 1;
+    xf.col = xface;
+    xf.dcol = xface + 2LL * 2 * xf.w * xf.w;
     ShellFaces f;
     for (int k = 0; k <= kmax; ++k) {
         shell_faces(s, k, f);
         const long long cells = f.start[f.nf];
         if (cells == 0) continue;
         const long long nb = (cells + 255) / 256;
-        hipLaunchKernelGGL(coldens_shell_kernel, dim3((unsigned)nb), dim3(256), 0, stream, s, f, den, col, dcol);
+        hipLaunchKernelGGL(coldens_shell_kernel, dim3((unsigned)nb), dim3(256), 0, stream, s, f, k, xf, den, col,
+                           dcol);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

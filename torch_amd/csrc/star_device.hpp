@@ -75,6 +75,51 @@ __device__ __forceinline__ void star_neighbours(const StarGeom& s, const int ijk
     w[3] = (f0 && f1 && f2) ? (1.0 - del0) * (1.0 - del1) : 0.0;
 }
 
+// The same neighbours as coordinates (valid[q] false off the grid).
+__device__ __forceinline__ void star_neighbour_cells(const StarGeom& s, const int ijk[3], int nbc[4][3], bool valid[4],
+                                                     double w[4]) {
+    int plane = 0;
+    double m = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double e = fabs(star_offset(s, ijk, a));
+        if (e > m) {
+            m = e;
+            plane = a;
+        }
+    }
+    const int ax[3] = {(plane + 1) % 3, (plane + 2) % 3, plane};
+    double d[3];
+    int sg[3], step[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        d[a] = star_offset(s, ijk, ax[a]);
+        sg[a] = d[a] < -1.0 / 10.0 ? -1 : 1;
+        step[a] = fabs(d[a]) < 1.0 / 10.0 ? 0 : -sg[a];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        nbc[q][0] = ijk[0];
+        nbc[q][1] = ijk[1];
+        nbc[q][2] = ijk[2];
+        nbc[q][ax[2]] += step[2];
+        if (q == 1 || q == 3) nbc[q][ax[1]] += step[1];
+        if (q == 2 || q == 3) nbc[q][ax[0]] += step[0];
+        valid[q] = nbc[q][0] >= 0 && nbc[q][0] < s.n[0] && nbc[q][1] >= 0 && nbc[q][1] < s.n[1] && nbc[q][2] >= 0 &&
+                   nbc[q][2] < s.n[2];
+    }
+    const double i0 = (double)ijk[ax[0]], i1 = (double)ijk[ax[1]];
+    const double ic0 = i0 - 0.5 * ((double)sg[2] * d[0] / d[2]);
+    const double ic1 = i1 - 0.5 * ((double)sg[2] * d[1] / d[2]);
+    const double del0 = fabs(2.0 * ic0 - 2.0 * i0 + (double)sg[0]);
+    const double del1 = fabs(2.0 * ic1 - 2.0 * i1 + (double)sg[1]);
+    const bool f0 = fabs(d[0]) > 0.9, f1 = fabs(d[1]) > 0.9, f2 = fabs(d[2]) > 0.9;
+    w[0] = f2 ? del0 * del1 : 0.0;
+    w[1] = (f1 && f2) ? del0 * (1.0 - del1) : 0.0;
+    w[2] = (f0 && f2) ? (1.0 - del0) * del1 : 0.0;
+    w[3] = (f0 && f1 && f2) ? (1.0 - del0) * (1.0 - del1) : 0.0;
+}
+
 // Path length of the ray through the cell: distance / largest |offset|
 // (ties go to z, then y); 0 at the star.
 __device__ __forceinline__ double star_path_length(const StarGeom& s, const int ijk[3]) {

@@ -631,6 +631,8 @@ struct torchgpu_cooling {
     tg::StarGeom star{};
     double* dcol = nullptr;
     long long dcol_cap = 0;
+    double* xface = nullptr;   // the ray trace's x-face buffers
+    long long xface_cap = 0;
 };
 
 // the MLP handle is defined in mlp_api.cpp; the cooling operator only needs
@@ -788,7 +790,8 @@ void torchgpu_cooling_destroy(torchgpu_cooling* c) {
     (void)hipStreamSynchronize(c->stream);
     for (void* ptr : {(void*)c->tables, (void*)c->counters, (void*)c->err, (void*)c->mlp_list, (void*)c->fb_list,
                       (void*)c->feat, (void*)c->y, (void*)c->sdata, (void*)c->in9, (void*)c->out5, (void*)c->dtbits,
-                      (void*)c->lpt_order, (void*)c->lpt_bucket, (void*)c->lpt_hist, (void*)c->dcol})
+                      (void*)c->lpt_order, (void*)c->lpt_bucket, (void*)c->lpt_hist, (void*)c->dcol,
+                      (void*)c->xface})
         if (ptr) (void)hipFree(ptr);
     delete c;
 }
@@ -935,7 +938,7 @@ static int gather(torchgpu_cooling* c, torchgpu_ctx* ctx, const double* T_min, c
     if (c->star.on && !col_den) {
         // Thermodynamics::preTimeStepCalculations starts with rayTrace on
         // the fixed primitives (Thermodynamics.cpp:218-220)
-        HIP_OK(tg::launch_thermo_ray_trace(c->star, c->in9, c->in9 + 6 * n, c->dcol, c->stream));
+        HIP_OK(tg::launch_thermo_ray_trace(c->star, c->in9, c->in9 + 6 * n, c->dcol, c->xface, c->stream));
     }
     return TORCHGPU_OK;
 }
@@ -1031,6 +1034,15 @@ int torchgpu_cooling_set_star(torchgpu_cooling* c, torchgpu_ctx* ctx, const torc
     s.photon_rate = star->photon_rate;
     s.pi = c->p.pi;
     s.hydrogen_mass = c->p.hydrogen_mass;
+    const long long nx = tg::xface_doubles(s);
+    if (nx > c->xface_cap) {
+        (void)hipStreamSynchronize(c->stream);
+        if (c->xface) (void)hipFree(c->xface);
+        c->xface = nullptr;
+        c->xface_cap = 0;
+        HIP_OK(hipMalloc(&c->xface, nx * sizeof(double)));
+        c->xface_cap = nx;
+    }
     c->star = s;
     return TORCHGPU_OK;
 }
