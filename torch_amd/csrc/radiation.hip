@@ -316,19 +316,39 @@ __global__ __launch_bounds__(256) void rad_dt_kernel(StarGeom s, RadConsts r, co
     }
 }
 
+// Face-contiguous copies of TAU, DTAU, TAU_A, DTAU_A for the x-fixed faces
+// of the last two shells (see raytrace.hip's coldens_shell_kernel): fields
+// [2 parities][2 sides][4][w * w], y fastest, frame centred on the star.
+struct RadXFaces {
+    double* buf;
+    int K, w;
+};
+
+__device__ __forceinline__ long long rad_xface_index(const StarGeom& s, const RadXFaces& xf, int parity, int side,
+                                                     int field, int y, int z) {
+    return (((long long)(parity * 2 + side) * 4 + field) * xf.w + (z - s.s[2] + xf.K)) * xf.w + (y - s.s[1] + xf.K);
+}
+
 // One Chebyshev shell of the causal sweep.  IMPL: transferRadiation
 // (TAU and TAU_A, the HII update, then DTAU and DTAU_A); otherwise
 // Radiation::rayTrace (TAU, then DTAU and DTAU_A from the current HII).
+// With tau_0 > 0 a neighbour of zero weight adds exactly 0 (0 / tau_0 x a
+// finite tau), so it is not read, and x-face cells read their upstream
+// neighbours from the previous shell's x-face copies (`fast`).
 template <bool IMPL>
 __global__ __launch_bounds__(256) void rad_sweep_kernel(StarGeom s, RadConsts r, const double* __restrict__ tab,
-                                                        ShellFaces f, double dt, double* q4, double* R,
-                                                        StepState* st) {
+                                                        ShellFaces f, int k, RadXFaces xf, double dt, double* q4,
+                                                        double* R, StepState* st) {
     const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
     if (t >= f.start[f.nf]) return;
     const long long n = (long long)s.n[0] * s.n[1] * s.n[2];
     int ijk[3];
     shell_cell(f, t, ijk);
     const long long c = star_flat(s, ijk[0], ijk[1], ijk[2]);
+    const bool fast = r.tau0 > 0.0;
+    const bool xface = fast && (ijk[0] - s.s[0] == k || s.s[0] - ijk[0] == k);
+    const int side = ijk[0] > s.s[0] ? 1 : 0;
+    double out[4];  // TAU, DTAU, TAU_A, DTAU_A of this cell after the sweep
     if (star_wind_cell(s, ijk)) {
         R[R_TAU * n + c] = 0;
         R[R_TAU_A * n + c] = 0;
@@ -336,44 +356,66 @@ __global__ __launch_bounds__(256) void rad_sweep_kernel(StarGeom s, RadConsts r,
         R[R_DTAU_A * n + c] = 0;
         q4[2 * n + c] = 1;
         R[R_HII_A * n + c] = 1;
-        return;
-    }
-    double dist2 = 0;
+        out[0] = out[1] = out[2] = out[3] = 0.0;
+    } else {
+        double dist2 = 0;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const double e = star_offset(s, ijk, a);
-        dist2 += e * e;
-    }
-    long long nb[4];
-    double w[4];
-    star_neighbours(s, ijk, nb, w);
-    // Radiation::updateTauSC (Radiation.cpp:353-373)
-#pragma unroll
-    for (int pass = 0; pass < (IMPL ? 2 : 1); ++pass) {
-        const int T_ = pass ? R_TAU_A : R_TAU, D_ = pass ? R_DTAU_A : R_DTAU;
-        double v = 0;
-        if (dist2 > 0.95) {
-            double tau[4], wr[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                tau[q] = nb[q] != -1 ? R[T_ * n + nb[q]] + R[D_ * n + nb[q]] : 0.0;
-                wr[q] = w[q] / smax(r.tau0, tau[q]);
-            }
-            const double sum_w = wr[0] + wr[1] + wr[2] + wr[3];
-            v = 0.0;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) v += (wr[q] / sum_w) * tau[q];
+        for (int a = 0; a < 3; ++a) {
+            const double e = star_offset(s, ijk, a);
+            dist2 += e * e;
         }
-        R[T_ * n + c] = v;
+        int nbc[4][3];
+        bool valid[4];
+        double w[4];
+        star_neighbour_cells(s, ijk, nbc, valid, w);
+        // Radiation::updateTauSC (Radiation.cpp:353-373)
+#pragma unroll
+        for (int pass = 0; pass < (IMPL ? 2 : 1); ++pass) {
+            const int T_ = pass ? R_TAU_A : R_TAU, D_ = pass ? R_DTAU_A : R_DTAU;
+            double v = 0;
+            if (dist2 > 0.95) {
+                double tau[4], wr[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    tau[q] = 0.0;
+                    if (valid[q] && (!fast || w[q] != 0.0)) {
+                        if (xface) {
+                            const int p = (k - 1) & 1;
+                            tau[q] = xf.buf[rad_xface_index(s, xf, p, side, 2 * pass, nbc[q][1], nbc[q][2])] +
+                                     xf.buf[rad_xface_index(s, xf, p, side, 2 * pass + 1, nbc[q][1], nbc[q][2])];
+                        } else {
+                            const long long i = star_flat(s, nbc[q][0], nbc[q][1], nbc[q][2]);
+                            tau[q] = R[T_ * n + i] + R[D_ * n + i];
+                        }
+                    }
+                    wr[q] = w[q] / smax(r.tau0, tau[q]);
+                }
+                const double sum_w = wr[0] + wr[1] + wr[2] + wr[3];
+                v = 0.0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) v += (wr[q] / sum_w) * tau[q];
+            }
+            R[T_ * n + c] = v;
+            out[2 * pass] = v;
+        }
+        const double ds = star_path_length(s, ijk);
+        if (IMPL) {
+            const double sv = star_shell_volume(s, ijk, ds);
+            if (!rad_update_hii(r, tab, n, c, ds, sv, dt, q4, R)) set_error(st, ERR_RAD_ITER);
+        }
+        const double nH = r.X * q4[c] / r.mh;
+        out[1] = ((1.0 - q4[2 * n + c]) * nH) * r.sigma * ds;
+        out[3] = ((1.0 - R[R_HII_A * n + c]) * nH) * r.sigma * ds;
+        R[R_DTAU * n + c] = out[1];
+        R[R_DTAU_A * n + c] = out[3];
     }
-    const double ds = star_path_length(s, ijk);
-    if (IMPL) {
-        const double sv = star_shell_volume(s, ijk, ds);
-        if (!rad_update_hii(r, tab, n, c, ds, sv, dt, q4, R)) set_error(st, ERR_RAD_ITER);
+    if (xface) {
+        // the star's cell (k = 0) feeds both sides of shell 1
+        for (int sd = (k == 0 ? 0 : side); sd <= (k == 0 ? 1 : side); ++sd)
+#pragma unroll
+            for (int fld = 0; fld < (IMPL ? 4 : 2); ++fld)
+                xf.buf[rad_xface_index(s, xf, k & 1, sd, fld, ijk[1], ijk[2])] = out[fld];
     }
-    const double nH = r.X * q4[c] / r.mh;
-    R[R_DTAU * n + c] = ((1.0 - q4[2 * n + c]) * nH) * r.sigma * ds;
-    R[R_DTAU_A * n + c] = ((1.0 - R[R_HII_A * n + c]) * nH) * r.sigma * ds;
 }
 
 __global__ void rad_hii_kernel(StarGeom s, RadConsts r, const double* __restrict__ tab, double dt, double* q4,
@@ -440,6 +482,7 @@ struct torchgpu_radiation {
     unsigned long long* dtbits = nullptr;
     bool once_pending = true;  // calculateTimeStep's function-static `once`
     bool q_valid = false;      // q4 holds a gather (checkValues' Q)
+    double* xface = nullptr;   // the sweep's x-face copies
 };
 
 namespace {
@@ -560,6 +603,8 @@ int torchgpu_radiation_create(const torchgpu_rad_params* p, torchgpu_ctx* ctx, c
     bool ok = hipMalloc(&r->tables, sizeof tab) == hipSuccess && hipMalloc(&r->R, 6 * nc * sizeof(double)) == hipSuccess &&
               hipMalloc(&r->q4, 4 * nc * sizeof(double)) == hipSuccess &&
               hipMalloc(&r->dtbits, sizeof(unsigned long long)) == hipSuccess;
+    const long long wx = 2LL * tg::shell_max(s) + 1;
+    ok = ok && hipMalloc(&r->xface, 16 * wx * wx * sizeof(double)) == hipSuccess;
     ok = ok && hipMemcpyAsync(r->tables, tab, sizeof tab, hipMemcpyHostToDevice, r->stream) == hipSuccess;
     if (ok) {
         hipLaunchKernelGGL(tg::rad_zero_kernel, dim3(grid_blocks(6 * nc)), dim3(256), 0, r->stream, r->R, 6 * nc);
@@ -577,7 +622,7 @@ int torchgpu_radiation_create(const torchgpu_rad_params* p, torchgpu_ctx* ctx, c
 void torchgpu_radiation_destroy(torchgpu_radiation* r) {
     if (!r) return;
     (void)hipStreamSynchronize(r->stream);
-    for (void* p : {(void*)r->tables, (void*)r->R, (void*)r->q4, (void*)r->dtbits})
+    for (void* p : {(void*)r->tables, (void*)r->R, (void*)r->q4, (void*)r->dtbits, (void*)r->xface})
         if (p) (void)hipFree(p);
     delete r;
 }
@@ -656,6 +701,10 @@ int torchgpu_radiation_substep(torchgpu_radiation* r, double dt, int has_heat) {
     // Radiation::integrate (Radiation.cpp:58-63)
     const bool impl = r->rc.scheme == TORCHGPU_RT_IMPLICIT;
     const int kmax = tg::shell_max(r->star);
+    tg::RadXFaces xf;
+    xf.buf = r->xface;
+    xf.K = kmax;
+    xf.w = 2 * kmax + 1;
     tg::ShellFaces f;
     for (int k = 0; k <= kmax; ++k) {
         tg::shell_faces(r->star, k, f);
@@ -664,10 +713,10 @@ int torchgpu_radiation_substep(torchgpu_radiation* r, double dt, int has_heat) {
         const unsigned nb = (unsigned)((cells + 255) / 256);
         if (impl)
             hipLaunchKernelGGL(tg::rad_sweep_kernel<true>, dim3(nb), dim3(256), 0, r->stream, r->star, r->rc, r->tables,
-                               f, dt, r->q4, r->R, v.st);
+                               f, k, xf, dt, r->q4, r->R, v.st);
         else
             hipLaunchKernelGGL(tg::rad_sweep_kernel<false>, dim3(nb), dim3(256), 0, r->stream, r->star, r->rc,
-                               r->tables, f, dt, r->q4, r->R, v.st);
+                               r->tables, f, k, xf, dt, r->q4, r->R, v.st);
         RAD_OK(hipGetLastError());
     }
     if (!impl) {
