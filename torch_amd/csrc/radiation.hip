@@ -93,6 +93,13 @@ __device__ __forceinline__ double rad_photoion(double nHI, double tau, double dt
     return S * emT * (1.0 - emdT) / (nHI * sv);
 }
 
+// The same with exp(-tau) given (loop-invariant in the implicit iteration).
+__device__ __forceinline__ double rad_photoion_emT(double nHI, double emT, double dtau, double sv, double S) {
+    if (nHI == 0 || sv == 0) return 0.0;
+    const double emdT = exp(-dtau);
+    return S * emT * (1.0 - emdT) / (nHI * sv);
+}
+
 __device__ __forceinline__ double rad_temperature(const RadConsts& r, double hii, double pre, double den) {
     const double mu_inv = r.X * (hii + 1.0) + r.mu_c;
     return (pre / den) * (1.0 / mu_inv) / r.Rgas;
@@ -134,6 +141,7 @@ __device__ __forceinline__ bool rad_update_hii(const RadConsts& r, const double*
     const double A_ci = rad_collisional(r, T);
     if (r.scheme != TORCHGPU_RT_EXPLICIT) {
         const double tau_avg = r.scheme == TORCHGPU_RT_IMPLICIT2 ? R[R_TAU * n + c] : R[R_TAU_A * n + c];
+        const double em_tau_avg = exp(-tau_avg);
         int niter = 0, miter = 0;
         bool converged = false;
         while (!converged) {
@@ -142,7 +150,7 @@ __device__ __forceinline__ bool rad_update_hii(const RadConsts& r, const double*
             x = q_hii;
             const double dtau_avg = (1.0 - x_avg) * n_H * r.sigma * ds;
             const double nHI = (1.0 - x_avg) * n_H;
-            const double A_pi = rad_photoion(nHI, tau_avg, dtau_avg, sv, r.photon_rate);
+            const double A_pi = rad_photoion_emT(nHI, em_tau_avg, dtau_avg, sv, r.photon_rate);
             const double nHII_aB = x_avg * n_H * alphaB;
             const double nHII_Aci = x_avg * n_H * A_ci;
             rad_doric(dt, x_avg, x, A_pi, nHII_aB, nHII_Aci);
