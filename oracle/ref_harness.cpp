@@ -89,6 +89,7 @@ struct Sim {
     Fluid fluid;
     Hydrodynamics hydro;
     Thermodynamics thermo;
+    Radiation rad;
 };
 
 // Torch::initialise (Torch.cpp:29-99) minus IO, Lua, radiation.
@@ -101,7 +102,7 @@ static void init_sim(Sim& s, const KV& kv) {
     p.tmax = getd(kv, "simulation_time", 1e3);
     p.ncheckpoints = geti(kv, "ncheckpoints", 1);
     p.cooling_on = geti(kv, "cooling_on", 0) != 0;
-    p.radiation_on = false;
+    p.radiation_on = geti(kv, "radiation_on", 0) != 0;
     p.nd = 3;
     p.ncells = {{geti(kv, "nx", 8), geti(kv, "ny", 8), geti(kv, "nz", 8)}};
     p.sideLength = getd(kv, "side_length", 1.0);
@@ -161,6 +162,7 @@ static void init_sim(Sim& s, const KV& kv) {
         std::cerr << "warning: " << e.what();
     }
     s.thermo.initialise(s.consts, p.getThermoParameters());
+    s.rad.initialise(s.consts, p.getRadiationParameters());
 }
 
 // Torch::toCodeUnits, Torch.cpp:146-155.
@@ -260,6 +262,7 @@ static int run_hydro(const KV& kv) {
     to_code_units(s);
     s.fluid.fixPrimitives();
     s.fluid.globalUfromQ();
+    if (s.p.radiation_on) s.rad.initField(s.fluid);  // Torch.cpp:121
 
     const int nsteps = geti(kv, "nsteps", 1);
     std::vector<int> dumps;
@@ -292,6 +295,7 @@ static int run_hydro(const KV& kv) {
     std::vector<Integrator*> comps;
     comps.push_back(&s.hydro);
     if (s.p.cooling_on) comps.push_back(&s.thermo);
+    if (s.p.radiation_on) comps.push_back(&s.rad);
 
     mpi.barrier();
     auto t0 = std::chrono::steady_clock::now();
@@ -301,15 +305,17 @@ static int run_hydro(const KV& kv) {
         s.fluid.globalQfromU();
         s.fluid.fixPrimitives();
         if (s.p.cooling_on) s.thermo.preTimeStepCalculations(s.fluid);
+        if (s.p.radiation_on) s.rad.preTimeStepCalculations(s.fluid);
         double dt;
         if (first_time) {
             dt = s.p.dt_max * 1.0e-20;
             first_time = false;
         } else {
             double dt_hydro = s.hydro.calculateTimeStep(s.p.dt_max, s.fluid);
-            double dt_thermo = dt_hydro;
+            double dt_rad = dt_hydro, dt_thermo = dt_hydro;
+            if (s.p.radiation_on) dt_rad = s.rad.calculateTimeStep(s.p.dt_max, s.fluid);
             if (s.p.cooling_on) dt_thermo = s.thermo.calculateTimeStep(s.p.dt_max, s.fluid);
-            dt = std::min(std::min(dt_hydro, dt_hydro), dt_thermo);
+            dt = std::min(std::min(dt_hydro, dt_rad), dt_thermo);
         }
         dt = mpi.minimum(dt);
         dt = std::min(dt_next, dt);

@@ -300,6 +300,46 @@ def radiation():
     print("radiation", list(RAD_CASES))
 
 
+STAR_FULLSTEP = dict(n=16, star=(8, 7, 9), wind=1, nsteps=4, side=3.0857e18, seed=41,
+                     kv={"cooling_on": 1, "radiation_on": 1, "star_on": 1, "photon_rate": 4.9e48,
+                         "photon_energy": 2.976e-11, "K1": 0.2, "coupling": "neq", "integration_scheme": "implicit",
+                         "photoion_cross_section": 6.3e-18, "case_b_recombination_coeff": 2.59e-13, "tau_0": 0.6,
+                         "temperature_hi": 300, "temperature_hii": 10000, "thermo_hii_switch": 1e-2})
+
+
+def star_fullstep_q():
+    """The cgs primitives of the composed star-on case: a clumpy neutral
+    medium (1e-22..1e-20 g/cm^3, 100 K, ADV = 1) with small random motions."""
+    cs = STAR_FULLSTEP
+    n = cs["n"]
+    rng = np.random.default_rng(cs["seed"])
+    rho = 10.0 ** rng.uniform(-22, -20, (n, n, n))
+    Q = np.zeros((7, n, n, n))
+    Q[0] = rho
+    Q[1] = rho * R_CGS * 100.0
+    Q[3] = 1.0
+    Q[4:7] = rng.normal(size=(3, n, n, n)) * 1.0e5
+    return Q
+
+
+def star_fullstep():
+    """Torch::fullStep with hydrodynamics, thermodynamics and radiation
+    around a star (the reference harness's driver loop with all three
+    components): 4 steps of a 16^3 clumpy medium; U after every step and
+    every dt."""
+    cs = STAR_FULLSTEP
+    n = cs["n"]
+    Q = star_fullstep_q()
+    extra = {"density_scale": 1e-20, "pressure_scale": 1e-28, "time_scale": 1e11, "side_length": cs["side"],
+             "temperature_floor": 0.1, "simulation_time": 1e13, "ncheckpoints": 10, "star_x": cs["star"][0],
+             "star_y": cs["star"][1], "star_z": cs["star"][2], "wind_radius": cs["wind"], **cs["kv"]}
+    dumps = [1, 2, cs["nsteps"]]
+    out, dts, info = run_hydro_case(Q, cs["nsteps"], dumps, tfloor=0.1, extra=extra)
+    np.savez_compressed(f"{HERE}/star_fullstep.npz", Q=Q, dts=dts, time=info["time"],
+                        **{f"U_{k}": out[k] for k in dumps})
+    print("star_fullstep", info, dts)
+
+
 if __name__ == "__main__":
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle ref")
