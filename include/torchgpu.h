@@ -26,8 +26,9 @@
  *   Thermodynamics::calculateTimeStep / the thermo part   torchgpu_thermo_grid_timestep /
  *     of Torch::subStep on the grid state                   torchgpu_thermo_grid_substep
  *     (Thermodynamics.cpp:463-485, Torch.cpp:352-364)
- *   Torch::run / fullStep / calculateTimeStep with        torchgpu_run_split
- *     hydrodynamics and thermodynamics (Torch.cpp:240-418)
+ *   Torch::run / fullStep / calculateTimeStep with        torchgpu_run_split,
+ *     hydrodynamics and thermodynamics and/or radiation     torchgpu_run_full
+ *     (Torch.cpp:240-418)
  *   Torch::run's Checkpointer test and the state it       torchgpu_pause_at_checkpoints /
  *     prints (Torch.cpp:268-277, Checkpointer.cpp:14-40)     torchgpu_checkpoint /
  *                                                            torchgpu_get_print_state
@@ -179,9 +180,9 @@ int torchgpu_get_clock(torchgpu_ctx* ctx, double* time, double* last_dt, int* st
 /* Torch::initialise's start time from an initial-conditions snapshot
  * (src/Torch/Torch.cpp:100-103, code units; t >= 0).  Synchronises. */
 int torchgpu_set_time(torchgpu_ctx* ctx, double t);
-/* With `on`, torchgpu_run and torchgpu_run_split stop at the end of a step
- * that reaches a checkpoint (the Checkpointer test of Torch::run,
- * src/Torch/Torch.cpp:268-277, src/IO/Checkpointer.cpp:14-40): later steps
+/* With `on`, torchgpu_run, torchgpu_run_split and torchgpu_run_full stop at
+ * the end of a step that reaches a checkpoint (the Checkpointer test of
+ * Torch::run, src/Torch/Torch.cpp:268-277, src/IO/Checkpointer.cpp:14-40): later steps
  * of the same call do nothing until torchgpu_checkpoint resumes the run.
  * Also keeps the state Torch::run prints at that point.  Synchronises. */
 int torchgpu_pause_at_checkpoints(torchgpu_ctx* ctx, int on);
@@ -407,6 +408,18 @@ int torchgpu_radiation_substep(torchgpu_radiation* r, double dt, int has_heat);
 /* R [6][n] and Q[HII], Q[ADV] [2][n] as the last sub-step left them; host
  * arrays unless `on_device`.  Synchronises. */
 int torchgpu_radiation_get_state(torchgpu_radiation* r, double* R, double* q_hii_adv, int on_device);
+/* torchgpu_run_split with any of the reference's split components: the
+ * cooling handle `c` and/or the radiation handle `r` (either may be NULL;
+ * both NULL is torchgpu_run).  Each step: the preTimeStepCalculations of
+ * thermodynamics and radiation, dt = min(dt_nextCheckpoint, dt_max * 1e-20
+ * first, then min(min(dt_hydro, dt_rad), dt_thermo)) chosen on the device
+ * (Torch.cpp:304-339,388-397), then the Strang sub-steps over (hydro,
+ * thermo, radiation) rotated by stepCounter (Torch.cpp:399-415), each a
+ * Torch::subStep.  With radiation, the state printed at checkpoints carries
+ * the HII fraction its last sub-step left in Fluid::Q.  All handles must
+ * use the same stream.  Asynchronous, as torchgpu_run_split. */
+int torchgpu_run_full(torchgpu_ctx* ctx, torchgpu_cooling* c, torchgpu_radiation* r, const double* T_min,
+                      const double* col_den, const double* F_FUV, int64_t excluded, int nsteps, int surrogate);
 
 /* ------------------------------------------------------------------------ */
 /* Cooling surrogate (no reference counterpart: a learned replacement of the  */

@@ -376,12 +376,13 @@ def snapshot_read(path: str, shape=None):
 
 
 def run_printing(grid: "Grid", nsteps: int, outdir: str, scales=(1.0, 1.0, 1.0), cooling=None, thermo=None,
-                 surrogate: bool = False):
+                 surrogate: bool = False, radiation=None):
     """Torch::run with its snapshots (Torch.cpp:240-302): prints
     data2D_<count>.txt.gz (6-digit count) into `outdir` for the initial
     state, at every checkpoint (the run pauses on the device there) and at
     the end when tmax is reached; at most `nsteps` steps (hydro only, or
-    operator-split with `cooling` and its `thermo` keyword arguments).
+    operator-split with `cooling` and its `thermo` keyword arguments and/or
+    `radiation`).
     Returns the list of files written."""
     import os
 
@@ -403,7 +404,9 @@ def run_printing(grid: "Grid", nsteps: int, outdir: str, scales=(1.0, 1.0, 1.0),
     try:
         while done < nsteps:
             n0 = grid.clock()[2]
-            if cooling is None:
+            if radiation is not None:
+                radiation.run_full(nsteps - done, cooling=cooling, surrogate=surrogate, **(thermo or {}))
+            elif cooling is None:
                 grid.run(nsteps - done)
             else:
                 cooling.run_split(grid, nsteps - done, surrogate=surrogate, **(thermo or {}))
@@ -581,6 +584,7 @@ def _bind_radiation(L):
     L.torchgpu_radiation_pre_step.argtypes = [vp]
     L.torchgpu_radiation_substep.argtypes = [vp, d, i]
     L.torchgpu_radiation_get_state.argtypes = [vp, vp, vp, i]
+    L.torchgpu_run_full.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_int64, i, i]
     L._radiation_bound = True
 
 
@@ -650,6 +654,16 @@ class Radiation:
 
     def substep(self, dt: float, has_heat: bool):
         self._check(lib().torchgpu_radiation_substep(self._h, dt, int(has_heat)))
+
+    def run_full(self, nsteps: int, cooling: "Cooling | None" = None, surrogate: bool = False, T_min=None,
+                 col_den=None, F_FUV=None, excluded: int = -1):
+        """`nsteps` Torch::fullStep iterations of hydro + (thermo) + radiation
+        with dt, the pre-step calculations and the Strang alternation on the
+        device (torchgpu_run_full, no host round trip)."""
+        dptr = cooling._dptr if cooling is not None else (lambda a: None)
+        self._check(lib().torchgpu_run_full(self.grid._h, None if cooling is None else cooling._h, self._h,
+                                            dptr(T_min), dptr(col_den), dptr(F_FUV), excluded, nsteps,
+                                            int(surrogate)))
 
     def state(self):
         """(R [6][nz][ny][nx], Q[HII, ADV] [2][nz][ny][nx]) host arrays."""

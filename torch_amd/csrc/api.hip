@@ -416,10 +416,11 @@ int torchgpu_get_print_state(torchgpu_ctx* c, double* Q, int on_device) {
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     const int mode = st.step > c->state_step ? c->last_mode : 0;
     if (!c->qprint) HIP_TRY(c, hipMalloc(&c->qprint, state_bytes(c)));
-    if (mode == 2 && !c->pstate) return fail(c, TORCHGPU_ERR_INVALID_ARG, "print state: pause at checkpoints first");
+    if (mode >= 2 && !c->pstate) return fail(c, TORCHGPU_ERR_INVALID_ARG, "print state: pause at checkpoints first");
     if (mode == 0) HIP_TRY(c, tg::launch_q_from_u(c->g, c->U, c->qprint, 1, c->stream));
     else if (mode == 1) HIP_TRY(c, tg::launch_q_from_u(c->g, c->Ustar, c->qprint, 0, c->stream));
-    else HIP_TRY(c, tg::launch_q_from_u(c->g, c->pstate, c->qprint, 1, c->stream));
+    else if (mode == 2) HIP_TRY(c, tg::launch_q_from_u(c->g, c->pstate, c->qprint, 1, c->stream));
+    else HIP_TRY(c, hipMemcpyAsync(c->qprint, c->pstate, state_bytes(c), hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(Q, c->qprint, state_bytes(c), on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                               c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));

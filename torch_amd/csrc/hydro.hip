@@ -272,7 +272,8 @@ __global__ void resume_kernel(StepState* st) {
 // with the checkpoint arithmetic of Checkpointer::update (Checkpointer.cpp:14-26).
 // use_cfl = 0 takes dt_override instead of the CFL/first-step value.
 __global__ void step_begin_kernel(StepState* st, int use_cfl, double dt_override,
-                                  const unsigned long long* thermo_dt_bits) {
+                                  const unsigned long long* thermo_dt_bits,
+                                  const unsigned long long* rad_dt_bits) {
     if (!use_cfl) {  // an explicit Torch::hydroStep(dt): no clock rules apply
         st->halted = 0;
         st->dt = dt_override;
@@ -310,6 +311,11 @@ __global__ void step_begin_kernel(StepState* st, int use_cfl, double dt_override
             // Thermodynamics::calculateTimeStep: dt_max lowered by every
             // smaller |f E/RATE| (Thermodynamics.cpp:465-474)
             const double m = __longlong_as_double((long long)*thermo_dt_bits);
+            dt = smin(dt, smin(m, st->dt_max));
+        }
+        if (rad_dt_bits) {
+            // Radiation::calculateTimeStep's minimum, at most dt_max
+            const double m = __longlong_as_double((long long)*rad_dt_bits);
             dt = smin(dt, smin(m, st->dt_max));
         }
     }
@@ -438,8 +444,9 @@ hipError_t launch_cfl(const HydroGeom& g, const double* U, StepState* st, hipStr
 }
 
 hipError_t launch_step_begin(StepState* st, int use_cfl, double dt_override, hipStream_t stream,
-                             const unsigned long long* thermo_dt_bits) {
-    hipLaunchKernelGGL(step_begin_kernel, dim3(1), dim3(1), 0, stream, st, use_cfl, dt_override, thermo_dt_bits);
+                             const unsigned long long* thermo_dt_bits, const unsigned long long* rad_dt_bits) {
+    hipLaunchKernelGGL(step_begin_kernel, dim3(1), dim3(1), 0, stream, st, use_cfl, dt_override, thermo_dt_bits,
+                       rad_dt_bits);
     return hipGetLastError();
 }
 
@@ -478,6 +485,28 @@ hipError_t launch_q_from_u(const HydroGeom& g, const double* U, double* Q, int f
     long long nb = (g.ncell + 255) / 256;
     if (nb > 65536) nb = 65536;
     hipLaunchKernelGGL(q_from_u_kernel, dim3((unsigned)nb), dim3(256), 0, stream, g, U, Q, fix);
+    return hipGetLastError();
+}
+
+__global__ void q_from_u_if_running_kernel(HydroGeom g, const StepState* st, const double* __restrict__ U,
+                                           double* __restrict__ Q) {
+    if (st->halted) return;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < g.ncell; c += (long long)gridDim.x * blockDim.x) {
+        double u[7], q[7];
+#pragma unroll
+        for (int v = 0; v < 7; ++v) u[v] = U[v * g.ncell + c];
+        QfromU(q, u, g.gm1);
+        fix_primitives(g, q, u[DEN]);
+#pragma unroll
+        for (int v = 0; v < 7; ++v) Q[v * g.ncell + c] = q[v];
+    }
+}
+
+hipError_t launch_q_from_u_if_running(const HydroGeom& g, const StepState* st, const double* U, double* Q,
+                                      hipStream_t stream) {
+    long long nb = (g.ncell + 255) / 256;
+    if (nb > 65536) nb = 65536;
+    hipLaunchKernelGGL(q_from_u_if_running_kernel, dim3((unsigned)nb), dim3(256), 0, stream, g, st, U, Q);
     return hipGetLastError();
 }
 

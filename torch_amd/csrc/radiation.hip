@@ -183,7 +183,8 @@ __device__ __forceinline__ void cell_ijk(const StarGeom& s, long long c, int ijk
 }
 
 __global__ void rad_gather_kernel(HydroGeom g, const double* __restrict__ U, double* __restrict__ q4, StepState* st,
-                                  int check, int check_old) {
+                                  int check, int check_old, int gated) {
+    if (gated && st->halted) return;
     const long long n = g.ncell;
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
         double u[7], q[7];
@@ -209,7 +210,8 @@ __global__ void rad_gather_kernel(HydroGeom g, const double* __restrict__ U, dou
 
 // Torch::checkValues before a sub-step whose Q is the last gather's.
 __global__ void rad_check_kernel(HydroGeom g, const double* __restrict__ U, const double* __restrict__ q4,
-                                 StepState* st) {
+                                 StepState* st, int gated) {
+    if (gated && st->halted) return;
     const long long n = g.ncell;
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
         bool bad = q4[c] == 0 || q4[n + c] == 0;
@@ -248,7 +250,8 @@ __global__ void rad_init_field_kernel(HydroGeom g, StarGeom s, RadConsts r, cons
 }
 
 __global__ void rad_heat_kernel(StarGeom s, RadConsts r, const double* __restrict__ tab, const double* __restrict__ q4,
-                                double* __restrict__ R, StepState* st) {
+                                double* __restrict__ R, StepState* st, int gated) {
+    if (gated && st->halted) return;
     const long long n = (long long)s.n[0] * s.n[1] * s.n[2];
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
         int ijk[3];
@@ -275,8 +278,13 @@ __global__ void rad_heat_kernel(StarGeom s, RadConsts r, const double* __restric
 __global__ __launch_bounds__(256) void rad_dt_kernel(StarGeom s, RadConsts r, const double* __restrict__ tab,
                                                      const double* __restrict__ q4, const double* __restrict__ U,
                                                      const double* __restrict__ R, double dt_max,
-                                                     unsigned long long* dtbits, StepState* st) {
+                                                     unsigned long long* dtbits, StepState* st, unsigned long long* once,
+                                                     int gated) {
     __shared__ double red[4];
+    // in the device loop: no time step is chosen on the first step or after
+    // the run halted, and dt_max is the run's
+    if (gated && (st->halted || st->first)) return;
+    if (gated) dt_max = st->dt_max;
     const long long n = (long long)s.n[0] * s.n[1] * s.n[2];
     double m = dt_max;
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
@@ -288,8 +296,10 @@ __global__ __launch_bounds__(256) void rad_dt_kernel(StarGeom s, RadConsts r, co
         if (r.coupling == TORCHGPU_RT_NEQ && R[R_HEAT * n + c] != 0) dtc = fabs(U[PRE * n + c] / R[R_HEAT * n + c]);
         if (r.K[0] != 0.0) {
             const double T = rad_temperature(r, hii, pre, den);
-            if (r.use_once && c == r.once_cell) dt1 = r.K[0] * r.mh / (r.X * den * rad_alpha(r, tab, T));
-            else if (hii != 0) dt1 = r.K[0] * r.mh / (r.X * den * hii * rad_alpha(r, tab, T));
+            if (r.use_once && c == r.once_cell && *once) {
+                dt1 = r.K[0] * r.mh / (r.X * den * rad_alpha(r, tab, T));
+                *once = 2;  // used: rad_once_commit_kernel clears it
+            } else if (hii != 0) dt1 = r.K[0] * r.mh / (r.X * den * hii * rad_alpha(r, tab, T));
         }
         if (r.K[1] != 0.0) dt2 = dt_max;
         if (r.K[2] != 0.0 || r.K[3] != 0.0) {
@@ -349,6 +359,8 @@ __global__ __launch_bounds__(256) void rad_sweep_kernel(StarGeom s, RadConsts r,
                                                         double* R, StepState* st) {
     const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
     if (t >= f.start[f.nf]) return;
+    if (dt < 0.0 && st->halted) return;  // a DT_* code: the device loop's step
+    dt = stage_dtf(dt, st, 0);
     const long long n = (long long)s.n[0] * s.n[1] * s.n[2];
     int ijk[3];
     shell_cell(f, t, ijk);
@@ -428,6 +440,8 @@ __global__ __launch_bounds__(256) void rad_sweep_kernel(StarGeom s, RadConsts r,
 
 __global__ void rad_hii_kernel(StarGeom s, RadConsts r, const double* __restrict__ tab, double dt, double* q4,
                                double* R, StepState* st) {
+    if (dt < 0.0 && st->halted) return;
+    dt = stage_dtf(dt, st, 0);
     const long long n = (long long)s.n[0] * s.n[1] * s.n[2];
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
         int ijk[3];
@@ -440,6 +454,8 @@ __global__ void rad_hii_kernel(StarGeom s, RadConsts r, const double* __restrict
 
 __global__ void rad_source_kernel(HydroGeom g, RadConsts r, const double* __restrict__ q4,
                                   const double* __restrict__ R, double* __restrict__ U, double dt, StepState* st) {
+    if (dt < 0.0 && st->halted) return;
+    dt = stage_dtf(dt, st, 0);
     const long long n = g.ncell;
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
         double u[7];
@@ -477,6 +493,20 @@ __global__ void rad_zero_kernel(double* p, long long n) {
 
 __global__ void rad_set_u64_kernel(unsigned long long* p, unsigned long long v) { *p = v; }
 
+__global__ void rad_copy_if_running_kernel(const StepState* st, const double* __restrict__ src,
+                                           double* __restrict__ dst, long long n) {
+    if (st->halted) return;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// calculateTimeStep's function-static `once` (Radiation.cpp:307-316): set
+// at creation, marked 2 by the time-step kernel that used it, cleared here;
+// restored when the device loop's step did not run (a checkpoint pause).
+__global__ void rad_once_commit_kernel(unsigned long long* once, const StepState* st, int gated) {
+    if (*once == 2) *once = (gated && st->halted) ? 1ull : 0ull;
+}
+
 }  // namespace tg
 
 struct torchgpu_radiation {
@@ -487,9 +517,8 @@ struct torchgpu_radiation {
     double* tables = nullptr;
     double* R = nullptr;
     double* q4 = nullptr;
-    unsigned long long* dtbits = nullptr;
-    bool once_pending = true;  // calculateTimeStep's function-static `once`
-    bool q_valid = false;      // q4 holds a gather (checkValues' Q)
+    unsigned long long* dtbits = nullptr;  // [0] time-step minimum, [1] `once`
+    bool q_valid = false;                  // q4 holds a gather (checkValues' Q)
     double* xface = nullptr;   // the sweep's x-face copies
 };
 
@@ -529,24 +558,118 @@ long long first_nonwind_cell(const tg::StarGeom& s) {
         if ((expr) != hipSuccess) return TORCHGPU_ERR_HIP;                            \
     } while (0)
 
-int rad_gather(torchgpu_radiation* r, int check) {
+int rad_gather(torchgpu_radiation* r, int check, int gated) {
     tg::CtxView v = tg::ctx_view(r->ctx);
     hipLaunchKernelGGL(tg::rad_gather_kernel, dim3(grid_blocks(v.g->ncell)), dim3(256), 0, r->stream, *v.g, v.U, r->q4,
-                       v.st, check, (int)r->q_valid);
+                       v.st, check, (int)r->q_valid, gated);
     RAD_OK(hipGetLastError());
     r->q_valid = true;
     return TORCHGPU_OK;
 }
 
-int rad_heat(torchgpu_radiation* r) {
+int rad_heat(torchgpu_radiation* r, int gated) {
     tg::CtxView v = tg::ctx_view(r->ctx);
     hipLaunchKernelGGL(tg::rad_heat_kernel, dim3(grid_blocks(v.g->ncell)), dim3(256), 0, r->stream, r->star, r->rc,
-                       r->tables, r->q4, r->R, v.st);
+                       r->tables, r->q4, r->R, v.st, gated);
     RAD_OK(hipGetLastError());
     return TORCHGPU_OK;
 }
 
 }  // namespace
+
+namespace tg {
+
+hipStream_t rad_stream(const torchgpu_radiation* r) { return r->stream; }
+const unsigned long long* rad_dt_bits(const torchgpu_radiation* r) { return r->dtbits; }
+
+// Radiation::preTimeStepCalculations on the fixed primitives of U.
+int rad_pre_step_dev(torchgpu_radiation* r, int gated) {
+    int rc = rad_gather(r, 0, gated);
+    if (!rc) rc = rad_heat(r, gated);
+    return rc;
+}
+
+// Radiation::calculateTimeStep's minimum into dtbits[0] (DBL_MAX when no
+// cell lowers it), no host sync.
+int rad_timestep_dev(torchgpu_radiation* r, double dt_max, int gated) {
+    tg::CtxView v = tg::ctx_view(r->ctx);
+    hipLaunchKernelGGL(rad_set_u64_kernel, dim3(1), dim3(1), 0, r->stream, r->dtbits, 0x7fefffffffffffffull);
+    const unsigned nb = grid_blocks(v.g->ncell) < 2048 ? grid_blocks(v.g->ncell) : 2048;
+    hipLaunchKernelGGL(rad_dt_kernel, dim3(nb), dim3(256), 0, r->stream, r->star, r->rc, r->tables, r->q4, v.U, r->R,
+                       dt_max, r->dtbits, v.st, r->dtbits + 1, gated);
+    RAD_OK(hipGetLastError());
+    return TORCHGPU_OK;
+}
+
+int rad_once_commit(torchgpu_radiation* r, int gated) {
+    hipLaunchKernelGGL(rad_once_commit_kernel, dim3(1), dim3(1), 0, r->stream, r->dtbits + 1,
+                       tg::ctx_view(r->ctx).st, gated);
+    RAD_OK(hipGetLastError());
+    return TORCHGPU_OK;
+}
+
+// Torch::subStep for the radiation component (Torch.cpp:351-363): dt
+// explicit, or a DT_* code of the device loop's step (then every kernel
+// returns once the run has halted).
+int rad_substep_dev(torchgpu_radiation* r, double dt, int has_heat) {
+    tg::CtxView v = tg::ctx_view(r->ctx);
+    const long long nc = v.g->ncell;
+    const int gated = dt < 0.0;
+    int rc;
+    if (!has_heat || !r->q_valid) {
+        // checkValues on the current U and Q, then Q and
+        // preTimeStepCalculations anew
+        if ((rc = rad_gather(r, 1, gated)) || (rc = rad_heat(r, gated))) return rc;
+    } else {
+        hipLaunchKernelGGL(rad_check_kernel, dim3(grid_blocks(nc)), dim3(256), 0, r->stream, *v.g, v.U, r->q4, v.st,
+                           gated);
+        RAD_OK(hipGetLastError());
+    }
+    // Radiation::integrate (Radiation.cpp:58-63)
+    const bool impl = r->rc.scheme == TORCHGPU_RT_IMPLICIT;
+    const int kmax = shell_max(r->star);
+    RadXFaces xf;
+    xf.buf = r->xface;
+    xf.K = kmax;
+    xf.w = 2 * kmax + 1;
+    ShellFaces f;
+    for (int k = 0; k <= kmax; ++k) {
+        shell_faces(r->star, k, f);
+        const long long cells = f.start[f.nf];
+        if (cells == 0) continue;
+        const unsigned nb = (unsigned)((cells + 255) / 256);
+        if (impl)
+            hipLaunchKernelGGL(rad_sweep_kernel<true>, dim3(nb), dim3(256), 0, r->stream, r->star, r->rc, r->tables, f,
+                               k, xf, dt, r->q4, r->R, v.st);
+        else
+            hipLaunchKernelGGL(rad_sweep_kernel<false>, dim3(nb), dim3(256), 0, r->stream, r->star, r->rc, r->tables,
+                               f, k, xf, dt, r->q4, r->R, v.st);
+        RAD_OK(hipGetLastError());
+    }
+    if (!impl) {
+        hipLaunchKernelGGL(rad_hii_kernel, dim3(grid_blocks(nc)), dim3(256), 0, r->stream, r->star, r->rc, r->tables,
+                           dt, r->q4, r->R, v.st);
+        RAD_OK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(rad_source_kernel, dim3(grid_blocks(nc)), dim3(256), 0, r->stream, *v.g, r->rc, r->q4, r->R,
+                       v.U, dt, v.st);
+    RAD_OK(hipGetLastError());
+    ctx_state_changed(r->ctx);
+    return TORCHGPU_OK;
+}
+
+// The HII fraction of the radiation's Q (q4) into a printed Q [7][n] unless
+// the run has halted: Radiation::integrate leaves it in Fluid::Q.
+int rad_print_hii(torchgpu_radiation* r, double* P) {
+    tg::CtxView v = tg::ctx_view(r->ctx);
+    const long long nc = v.g->ncell;
+    hipLaunchKernelGGL(rad_copy_if_running_kernel, dim3(grid_blocks(nc)), dim3(256), 0, r->stream, v.st,
+                       r->q4 + 2 * nc, P + (long long)HII * nc, nc);
+    RAD_OK(hipGetLastError());
+    return TORCHGPU_OK;
+}
+
+}  // namespace tg
 
 extern "C" {
 
@@ -601,6 +724,7 @@ int torchgpu_radiation_create(const torchgpu_rad_params* p, torchgpu_ctx* ctx, c
     c.cool_slope0 = p->cool_slope[0];
     c.cool_slope1 = p->cool_slope[1];
     c.once_cell = first_nonwind_cell(s);
+    c.use_once = c.K[0] != 0.0 && c.once_cell >= 0;
     double tab[tg::TAB_N];
     std::memcpy(tab + tg::TAB_X, p->rec_x, sizeof p->rec_x);
     std::memcpy(tab + tg::TAB_REC, p->rec_y, sizeof p->rec_y);
@@ -610,13 +734,14 @@ int torchgpu_radiation_create(const torchgpu_rad_params* p, torchgpu_ctx* ctx, c
     const long long nc = g.ncell;
     bool ok = hipMalloc(&r->tables, sizeof tab) == hipSuccess && hipMalloc(&r->R, 6 * nc * sizeof(double)) == hipSuccess &&
               hipMalloc(&r->q4, 4 * nc * sizeof(double)) == hipSuccess &&
-              hipMalloc(&r->dtbits, sizeof(unsigned long long)) == hipSuccess;
+              hipMalloc(&r->dtbits, 2 * sizeof(unsigned long long)) == hipSuccess;
     const long long wx = 2LL * tg::shell_max(s) + 1;
     ok = ok && hipMalloc(&r->xface, 16 * wx * wx * sizeof(double)) == hipSuccess;
     ok = ok && hipMemcpyAsync(r->tables, tab, sizeof tab, hipMemcpyHostToDevice, r->stream) == hipSuccess;
     if (ok) {
         hipLaunchKernelGGL(tg::rad_zero_kernel, dim3(grid_blocks(6 * nc)), dim3(256), 0, r->stream, r->R, 6 * nc);
         hipLaunchKernelGGL(tg::rad_zero_kernel, dim3(grid_blocks(4 * nc)), dim3(256), 0, r->stream, r->q4, 4 * nc);
+        hipLaunchKernelGGL(tg::rad_set_u64_kernel, dim3(1), dim3(1), 0, r->stream, r->dtbits + 1, 1ull);
         ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(r->stream) == hipSuccess;
     }
     if (!ok) {
@@ -637,7 +762,7 @@ void torchgpu_radiation_destroy(torchgpu_radiation* r) {
 
 int torchgpu_radiation_init_field(torchgpu_radiation* r, const double* Q, int on_device) {
     if (!r) return TORCHGPU_ERR_INVALID_ARG;
-    int rc = rad_gather(r, 0);
+    int rc = rad_gather(r, 0, 0);
     if (rc) return rc;
     tg::CtxView v = tg::ctx_view(r->ctx);
     const long long nc = v.g->ncell;
@@ -662,9 +787,7 @@ int torchgpu_radiation_init_field(torchgpu_radiation* r, const double* Q, int on
 
 int torchgpu_radiation_pre_step(torchgpu_radiation* r) {
     if (!r) return TORCHGPU_ERR_INVALID_ARG;
-    int rc = rad_gather(r, 0);
-    if (!rc) rc = rad_heat(r);
-    return rc;
+    return tg::rad_pre_step_dev(r, 0);
 }
 
 int torchgpu_radiation_timestep(torchgpu_radiation* r, double dt_max, double* dt) {
@@ -673,17 +796,9 @@ int torchgpu_radiation_timestep(torchgpu_radiation* r, double dt_max, double* dt
         const int rc = torchgpu_radiation_pre_step(r);
         if (rc) return rc;
     }
-    tg::CtxView v = tg::ctx_view(r->ctx);
-    tg::RadConsts c = r->rc;
-    c.use_once = r->once_pending && c.K[0] != 0.0 && c.once_cell >= 0;
-    if (c.use_once) r->once_pending = false;
-    unsigned long long init;
-    std::memcpy(&init, &dt_max, sizeof init);
-    hipLaunchKernelGGL(tg::rad_set_u64_kernel, dim3(1), dim3(1), 0, r->stream, r->dtbits, init);
-    const unsigned nb = grid_blocks(v.g->ncell) < 2048 ? grid_blocks(v.g->ncell) : 2048;
-    hipLaunchKernelGGL(tg::rad_dt_kernel, dim3(nb), dim3(256), 0, r->stream, r->star, c, r->tables, r->q4, v.U, r->R,
-                       dt_max, r->dtbits, v.st);
-    RAD_OK(hipGetLastError());
+    int rc = tg::rad_timestep_dev(r, dt_max, 0);
+    if (!rc) rc = tg::rad_once_commit(r, 0);
+    if (rc) return rc;
     unsigned long long bits = 0;
     RAD_OK(hipMemcpyAsync(&bits, r->dtbits, sizeof bits, hipMemcpyDeviceToHost, r->stream));
     RAD_OK(hipStreamSynchronize(r->stream));
@@ -695,48 +810,7 @@ int torchgpu_radiation_timestep(torchgpu_radiation* r, double dt_max, double* dt
 
 int torchgpu_radiation_substep(torchgpu_radiation* r, double dt, int has_heat) {
     if (!r || !(dt > 0.0)) return TORCHGPU_ERR_INVALID_ARG;
-    tg::CtxView v = tg::ctx_view(r->ctx);
-    const long long nc = v.g->ncell;
-    int rc;
-    if (!has_heat || !r->q_valid) {
-        // Torch::subStep: checkValues on the current U and Q, then Q and
-        // preTimeStepCalculations anew
-        if ((rc = rad_gather(r, 1)) || (rc = rad_heat(r))) return rc;
-    } else {
-        hipLaunchKernelGGL(tg::rad_check_kernel, dim3(grid_blocks(nc)), dim3(256), 0, r->stream, *v.g, v.U, r->q4, v.st);
-        RAD_OK(hipGetLastError());
-    }
-    // Radiation::integrate (Radiation.cpp:58-63)
-    const bool impl = r->rc.scheme == TORCHGPU_RT_IMPLICIT;
-    const int kmax = tg::shell_max(r->star);
-    tg::RadXFaces xf;
-    xf.buf = r->xface;
-    xf.K = kmax;
-    xf.w = 2 * kmax + 1;
-    tg::ShellFaces f;
-    for (int k = 0; k <= kmax; ++k) {
-        tg::shell_faces(r->star, k, f);
-        const long long cells = f.start[f.nf];
-        if (cells == 0) continue;
-        const unsigned nb = (unsigned)((cells + 255) / 256);
-        if (impl)
-            hipLaunchKernelGGL(tg::rad_sweep_kernel<true>, dim3(nb), dim3(256), 0, r->stream, r->star, r->rc, r->tables,
-                               f, k, xf, dt, r->q4, r->R, v.st);
-        else
-            hipLaunchKernelGGL(tg::rad_sweep_kernel<false>, dim3(nb), dim3(256), 0, r->stream, r->star, r->rc,
-                               r->tables, f, k, xf, dt, r->q4, r->R, v.st);
-        RAD_OK(hipGetLastError());
-    }
-    if (!impl) {
-        hipLaunchKernelGGL(tg::rad_hii_kernel, dim3(grid_blocks(nc)), dim3(256), 0, r->stream, r->star, r->rc, r->tables,
-                           dt, r->q4, r->R, v.st);
-        RAD_OK(hipGetLastError());
-    }
-    hipLaunchKernelGGL(tg::rad_source_kernel, dim3(grid_blocks(nc)), dim3(256), 0, r->stream, *v.g, r->rc, r->q4, r->R,
-                       v.U, dt, v.st);
-    RAD_OK(hipGetLastError());
-    tg::ctx_state_changed(r->ctx);
-    return TORCHGPU_OK;
+    return tg::rad_substep_dev(r, dt, has_heat);
 }
 
 int torchgpu_radiation_get_state(torchgpu_radiation* r, double* R, double* q_hii_adv, int on_device) {

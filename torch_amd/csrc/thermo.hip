@@ -1089,38 +1089,72 @@ int torchgpu_thermo_grid_inputs(torchgpu_ctx* ctx, torchgpu_cooling* c, const do
 
 int torchgpu_run_split(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* T_min, const double* col_den,
                        const double* F, int64_t excluded, int nsteps, int surrogate) {
-    if (!ctx || !c || nsteps < 0) return TORCHGPU_ERR_INVALID_ARG;
-    if (surrogate && !c->has_surrogate) return TORCHGPU_ERR_NOT_TRAINED;
+    if (!c) return TORCHGPU_ERR_INVALID_ARG;
+    return torchgpu_run_full(ctx, c, nullptr, T_min, col_den, F, excluded, nsteps, surrogate);
+}
+
+int torchgpu_run_full(torchgpu_ctx* ctx, torchgpu_cooling* c, torchgpu_radiation* r, const double* T_min,
+                      const double* col_den, const double* F, int64_t excluded, int nsteps, int surrogate) {
+    if (!ctx || nsteps < 0) return TORCHGPU_ERR_INVALID_ARG;
+    if (!c && !r) return torchgpu_run(ctx, nsteps);  // one component: Torch::hydroStep
+    if (c && surrogate && !c->has_surrogate) return TORCHGPU_ERR_NOT_TRAINED;
     tg::CtxView v = tg::ctx_view(ctx);
-    if (v.stream != c->stream) return TORCHGPU_ERR_INVALID_ARG;  // one stream orders the two handles' work
-    tg::ctx_set_print_mode(ctx, 2);
+    // one stream orders the handles' work
+    if ((c && v.stream != c->stream) || (r && v.stream != tg::rad_stream(r))) return TORCHGPU_ERR_INVALID_ARG;
+    tg::ctx_set_print_mode(ctx, r ? 3 : 2);
     double* print_state = tg::ctx_print_buffer(ctx);
+    // activeComponents (Torch.cpp:256-261): hydro, thermodynamics, radiation
+    enum { HYDRO, THERMO, RAD };
+    int comps[3], ncomps = 0;
+    comps[ncomps++] = HYDRO;
+    if (c) comps[ncomps++] = THERMO;
+    if (r) comps[ncomps++] = RAD;
+    // Torch::fullStep's sub-steps (Torch.cpp:397-415): h dt (h = 1/2, ...,
+    // 1 for the last) over the components rotated by stepCounter, then dt/2
+    // back down; the first reuses the pre-step calculations
+    int order[5];
+    double code[5];
+    const int nsub = 2 * ncomps - 1;
     int rc;
     for (int s = 0; s < nsteps; ++s) {
         v = tg::ctx_view(ctx);  // U and U* swap roles after every hydro sub-step
-        // Torch::calculateTimeStep (Torch.cpp:304-339) on the device: the
-        // hydro CFL maximum, the thermo minimum, then dt in step_begin
+        // Torch::fullStep's preTimeStepCalculations and calculateTimeStep
+        // (Torch.cpp:304-339,388-397) on the device: the hydro CFL maximum,
+        // the thermo and radiation minima, then dt in step_begin
         HIP_OK(tg::launch_cfl(*v.g, v.U, v.st, v.stream));
-        if ((rc = grid_timestep_dev(ctx, c, T_min, col_den, F, excluded))) return rc;
-        HIP_OK(tg::launch_step_begin(v.st, 1, 0.0, v.stream, c->dtbits));
-        // Torch::fullStep's Strang alternation over (hydro, thermo),
-        // Torch.cpp:406-415: T(dt/2) H(dt) T(dt/2), then H(dt/2) T(dt) H(dt/2)
+        if (c && (rc = grid_timestep_dev(ctx, c, T_min, col_den, F, excluded))) return rc;
+        if (r && ((rc = tg::rad_pre_step_dev(r, 1)) || (rc = tg::rad_timestep_dev(r, 0.0, 1)))) return rc;
+        HIP_OK(tg::launch_step_begin(v.st, 1, 0.0, v.stream, c ? c->dtbits : nullptr,
+                                     r ? tg::rad_dt_bits(r) : nullptr));
+        if (r && (rc = tg::rad_once_commit(r, 1))) return rc;
         int& counter = tg::ctx_step_counter(ctx);
-        counter = (counter + 1) % 2;
-        for (int k = 0; k < 3; ++k) {
-            const bool hydro = ((k == 1 ? 1 : 0) + counter) % 2 == 0;
-            const double code = (k == 1) ? tg::DT_FULL : tg::DT_HALF;
-            if (k == 2 && print_state) {
+        counter = (counter + 1) % ncomps;
+        for (int i = 0; i < ncomps; ++i) {
+            order[i] = comps[(i + counter) % ncomps];
+            code[i] = (i == ncomps - 1) ? tg::DT_FULL : tg::DT_HALF;
+        }
+        for (int i = ncomps - 2, j = ncomps; i >= 0; --i, ++j) {
+            order[j] = comps[(i + counter) % ncomps];
+            code[j] = tg::DT_HALF;
+        }
+        for (int k = 0; k < nsub; ++k) {
+            const bool last = k == nsub - 1;
+            if (last && print_state) {
                 // the reference's Fluid::Q after the step is fixPrimitives(Q)
-                // of the last sub-step's input (Torch.cpp:352-358): kept for
+                // of the last sub-step's input (Torch.cpp:352-358), kept for
                 // the snapshots printed at checkpoints
                 tg::CtxView w = tg::ctx_view(ctx);
-                HIP_OK(tg::launch_copy_if_running(w.st, w.U, print_state, 7 * w.g->ncell, w.stream));
+                if (r) HIP_OK(tg::launch_q_from_u_if_running(*w.g, w.st, w.U, print_state, w.stream));
+                else HIP_OK(tg::launch_copy_if_running(w.st, w.U, print_state, 7 * w.g->ncell, w.stream));
             }
-            if (hydro) {
-                if ((rc = tg::ctx_run_stage(ctx, 0, code))) return rc;
-            } else if ((rc = grid_substep_dev(ctx, c, T_min, col_den, F, excluded, code, surrogate))) {
-                return rc;
+            if (order[k] == HYDRO) {
+                if ((rc = tg::ctx_run_stage(ctx, 0, code[k]))) return rc;
+            } else if (order[k] == THERMO) {
+                if ((rc = grid_substep_dev(ctx, c, T_min, col_den, F, excluded, code[k], surrogate))) return rc;
+            } else {
+                if ((rc = tg::rad_substep_dev(r, code[k], k == 0))) return rc;
+                // Radiation::integrate leaves its HII fraction in Fluid::Q
+                if (last && print_state && (rc = tg::rad_print_hii(r, print_state))) return rc;
             }
         }
         HIP_OK(tg::launch_step_end(tg::ctx_view(ctx).st, v.stream));

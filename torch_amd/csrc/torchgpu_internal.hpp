@@ -117,6 +117,7 @@ struct CtxView {
 
 }  // namespace tg
 typedef struct torchgpu_ctx torchgpu_ctx;
+typedef struct torchgpu_radiation torchgpu_radiation;
 namespace tg {
 CtxView ctx_view(torchgpu_ctx* c);
 
@@ -147,14 +148,18 @@ int ctx_run_stage(torchgpu_ctx* c, int mode, double dtf);
 int& ctx_step_counter(torchgpu_ctx* c);
 // Bookkeeping of the state Torch::run prints (see torchgpu_get_print_state):
 // the mode of the last run, and (when pausing at checkpoints) the buffer
-// that keeps the input of an operator-split step's last sub-step.
+// that keeps the input U of an operator-split step's last sub-step (mode
+// 2), or the printed Q itself (mode 3: runs with radiation, whose sub-step
+// leaves its HII fraction in Fluid::Q).
 void ctx_set_print_mode(torchgpu_ctx* c, int mode);
 double* ctx_print_buffer(torchgpu_ctx* c);
 hipError_t launch_cfl(const HydroGeom& g, const double* U, StepState* st, hipStream_t stream);
 // thermo_dt_bits: the bit pattern of the cooling operator's minimum
-// |E/RATE| (a Torch::fullStep with thermodynamics), or null (hydro only).
+// |E/RATE| (a Torch::fullStep with thermodynamics), or null; rad_dt_bits:
+// that of Radiation::calculateTimeStep's minimum, or null.
 hipError_t launch_step_begin(StepState* st, int use_cfl, double dt_override, hipStream_t stream,
-                             const unsigned long long* thermo_dt_bits = nullptr);
+                             const unsigned long long* thermo_dt_bits = nullptr,
+                             const unsigned long long* rad_dt_bits = nullptr);
 hipError_t launch_step_end(StepState* st, hipStream_t stream);
 // Clock control for Torch::run's checkpoint printing: set the time (and the
 // checkpoint count Checkpointer::update gives for it), resume a paused run.
@@ -178,6 +183,18 @@ hipError_t launch_copy_if_running(const StepState* st, const double* src, double
                                   hipStream_t stream);
 hipError_t launch_resume(StepState* st, hipStream_t stream);
 hipError_t launch_q_from_u(const HydroGeom& g, const double* U, double* Q, int fix, hipStream_t stream);
+// Q = fixPrimitives(QfromU(U)) unless the run has halted.
+hipError_t launch_q_from_u_if_running(const HydroGeom& g, const StepState* st, const double* U, double* Q,
+                                      hipStream_t stream);
 hipError_t launch_u_from_q(const HydroGeom& g, double* Q, double* U, hipStream_t stream);
+// The radiation operator's pieces of Torch::fullStep for the device loop
+// (radiation.hip).  dt of rad_substep_dev: explicit, or a DT_* code.
+hipStream_t rad_stream(const torchgpu_radiation* r);
+const unsigned long long* rad_dt_bits(const torchgpu_radiation* r);
+int rad_pre_step_dev(torchgpu_radiation* r, int gated);
+int rad_timestep_dev(torchgpu_radiation* r, double dt_max, int gated);
+int rad_once_commit(torchgpu_radiation* r, int gated);
+int rad_substep_dev(torchgpu_radiation* r, double dt, int has_heat);
+int rad_print_hii(torchgpu_radiation* r, double* P);
 
 }  // namespace tg

@@ -575,13 +575,18 @@ void Torch::print2D(int count) const {
         throw std::runtime_error("DataPrinter::print2D: unable to open" + path);
 }
 
-void Torch::run(int max_steps, bool print) {  // Torch.cpp:240-302, capped at max_steps
-    if (print && params.radiation_on)
-        throw std::runtime_error("--snapshots: checkpoint printing runs on the device loops (hydro, hydro + cooling); "
-                                 "radiation_on runs the host-driven loop");
+void Torch::run(int max_steps, bool print, bool host_loop) {  // Torch.cpp:240-302, capped at max_steps
+    torchgpu_ctx* c = fluid.ctx();
+    // the whole Torch::run loop on the device: dt, the pre-step
+    // calculations and the sub-steps of every active component
+    auto device_run = [&](int n) {
+        check_rc(torchgpu_run_full(c, params.cooling_on ? thermodynamics.handle() : nullptr,
+                                   params.radiation_on ? radiation.handle() : nullptr, nullptr, nullptr, nullptr, -1,
+                                   n, 0),
+                 "run", c);
+    };
     if (print) {
         // the device loop pauses at every checkpoint for the print
-        torchgpu_ctx* c = fluid.ctx();
         int count = 0, due = 0;
         check_rc(torchgpu_checkpoint(c, &count, &due), "checkpoint", c);
         print2D(count);
@@ -592,11 +597,7 @@ void Torch::run(int max_steps, bool print) {  // Torch.cpp:240-302, capped at ma
         int n0 = 0, n1 = 0;
         while (done < max_steps) {
             check_rc(torchgpu_get_clock(c, &t, &dt, &n0), "clock", c);
-            if (params.cooling_on)
-                check_rc(torchgpu_run_split(c, thermodynamics.handle(), nullptr, nullptr, nullptr, -1,
-                                            max_steps - done, 0), "run", c);
-            else
-                check_rc(torchgpu_run(c, max_steps - done), "run", c);
+            device_run(max_steps - done);
             fluid.check();
             check_rc(torchgpu_get_clock(c, &t, &dt, &n1), "clock", c);
             done += n1 - n0;
@@ -612,17 +613,17 @@ void Torch::run(int max_steps, bool print) {  // Torch.cpp:240-302, capped at ma
         steps = n1;
         return;
     }
-    if (!params.cooling_on && !params.radiation_on) {
-        // hydro-only: the whole loop stays on the device (dt chosen there)
-        check_rc(torchgpu_run(fluid.ctx(), max_steps), "run", fluid.ctx());
+    if (!host_loop) {
+        device_run(max_steps);
         fluid.check();
         int n = 0;
         double t = 0, dt = 0;
-        check_rc(torchgpu_get_clock(fluid.ctx(), &t, &dt, &n), "clock", fluid.ctx());
+        check_rc(torchgpu_get_clock(c, &t, &dt, &n), "clock", c);
         currentTime = t;
         steps = n;
         return;
     }
+    // host-driven: Torch::fullStep through the library's operator calls
     int count = 0;
     const double delta = 100.0 / params.ncheckpoints;
     auto checkpoint = [&](double t) {  // Checkpointer::update, Checkpointer.cpp:14-26

@@ -213,8 +213,9 @@ public:
     void setPrimitivesCgs(const double* Q);   // Torch::toCodeUnits + fixPrimitives + globalUfromQ
     void setUpLua(const std::string& filename);  // Torch::setUpLua + the tail of Torch::initialise
     // Torch::run, capped; with `print`, the data2D snapshots of
-    // Integration.output_directory at the start, every checkpoint and the end
-    void run(int max_steps, bool print = false);
+    // Integration.output_directory at the start, every checkpoint and the end;
+    // `host_loop` (without `print`): fullStep driven from the host
+    void run(int max_steps, bool print = false, bool host_loop = false);
     void print2D(int count) const;
     double fullStep(double dt_nextCheckpoint);
     void hydroStep(double dt, bool hasCalculatedHeatFlux);

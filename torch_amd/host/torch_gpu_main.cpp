@@ -16,7 +16,9 @@
 // file's initialise(x, y, z, xs, ys, zs) for every cell, as the reference's
 // --setupfile (main.cpp:36-50, Torch::setUpLua); --dump-setup=FILE.bin
 // writes the primitives it returns (cgs, float64 [7][nz][ny][nx]) and exits
-// without touching the GPU.
+// without touching the GPU.  Torch::run keeps its whole loop on the device
+// (torchgpu_run_full); --host-loop drives Torch::fullStep from the host
+// through the operator calls instead (no checkpoint printing).
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -126,7 +128,7 @@ int main(int argc, char** argv) {
         if (torch.params.initialConditions.empty() && setupfile.empty()) torch.setPrimitivesCgs(Q.data());
         const int steps = std::stoi(arg(argc, argv, "--steps", "100"));
         auto t0 = std::chrono::steady_clock::now();
-        torch.run(steps, arg(argc, argv, "--snapshots", "") == "1");
+        torch.run(steps, arg(argc, argv, "--snapshots", "") == "1", arg(argc, argv, "--host-loop", "") == "1");
         auto t1 = std::chrono::steady_clock::now();
         const double secs = std::chrono::duration<double>(t1 - t0).count();
         std::printf("{\"steps\": %d, \"time\": %.17g, \"seconds\": %.6f, \"mcell_updates_per_s\": %.3f}\n", torch.steps,
