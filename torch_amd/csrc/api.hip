@@ -23,6 +23,7 @@ struct torchgpu_ctx {
     int qcur = 0;
     bool qvalid = false, qstar_valid = false;
     int step_counter = 0;  // Torch::stepCounter of the operator-split loop
+    int counter_step = 0;  // ... as of this device step count
     // the primitive state the reference's Fluid holds after the last step,
     // which Torch::run prints: 1 = QfromU(U*) (torchgpu_run, hydro_step),
     // 2 = fixPrimitives(QfromU(U*)) with U* the input of the last sub-step
@@ -104,6 +105,7 @@ int reset_clock(torchgpu_ctx* c) {
     st.first = 1;
     st.ncheckpoints = c->p.ncheckpoints > 0 ? c->p.ncheckpoints : 1;
     c->step_counter = 0;
+    c->counter_step = 0;
     HIP_TRY(c, hipMemcpyAsync(c->st, &st, sizeof st, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return TORCHGPU_OK;
@@ -160,6 +162,7 @@ int run_stage(torchgpu_ctx* c, int mode, double dtf) {
 
 int tg::ctx_run_stage(torchgpu_ctx* c, int mode, double dtf) { return run_stage(c, mode, dtf); }
 int& tg::ctx_step_counter(torchgpu_ctx* c) { return c->step_counter; }
+int& tg::ctx_counter_step(torchgpu_ctx* c) { return c->counter_step; }
 void tg::ctx_set_print_mode(torchgpu_ctx* c, int mode) { c->last_mode = mode; }
 double* tg::ctx_print_buffer(torchgpu_ctx* c) {
     if (!c->pause_ckpt) return nullptr;

@@ -1116,6 +1116,17 @@ int torchgpu_run_full(torchgpu_ctx* ctx, torchgpu_cooling* c, torchgpu_radiation
     double code[5];
     const int nsub = 2 * ncomps - 1;
     int rc;
+    // Torch::stepCounter turns once per step that ran: bring it up to date
+    // with the steps the device completed since the last call (the host
+    // launched more when a pause or tmax halted the run)
+    tg::StepState st_now;
+    HIP_OK(hipMemcpyAsync(&st_now, v.st, sizeof st_now, hipMemcpyDeviceToHost, v.stream));
+    HIP_OK(hipStreamSynchronize(v.stream));
+    int& counter = tg::ctx_step_counter(ctx);
+    int& counter_step = tg::ctx_counter_step(ctx);
+    counter = (counter + (st_now.step - counter_step)) % ncomps;
+    counter_step = st_now.step;
+    int launched = counter;
     for (int s = 0; s < nsteps; ++s) {
         v = tg::ctx_view(ctx);  // U and U* swap roles after every hydro sub-step
         // Torch::fullStep's preTimeStepCalculations and calculateTimeStep
@@ -1127,14 +1138,13 @@ int torchgpu_run_full(torchgpu_ctx* ctx, torchgpu_cooling* c, torchgpu_radiation
         HIP_OK(tg::launch_step_begin(v.st, 1, 0.0, v.stream, c ? c->dtbits : nullptr,
                                      r ? tg::rad_dt_bits(r) : nullptr));
         if (r && (rc = tg::rad_once_commit(r, 1))) return rc;
-        int& counter = tg::ctx_step_counter(ctx);
-        counter = (counter + 1) % ncomps;
+        launched = (launched + 1) % ncomps;
         for (int i = 0; i < ncomps; ++i) {
-            order[i] = comps[(i + counter) % ncomps];
+            order[i] = comps[(i + launched) % ncomps];
             code[i] = (i == ncomps - 1) ? tg::DT_FULL : tg::DT_HALF;
         }
         for (int i = ncomps - 2, j = ncomps; i >= 0; --i, ++j) {
-            order[j] = comps[(i + counter) % ncomps];
+            order[j] = comps[(i + launched) % ncomps];
             code[j] = tg::DT_HALF;
         }
         for (int k = 0; k < nsub; ++k) {

@@ -146,6 +146,10 @@ void ctx_state_changed(torchgpu_ctx* c);
 int ctx_run_stage(torchgpu_ctx* c, int mode, double dtf);
 // The context's Strang alternation counter (Torch::stepCounter).
 int& ctx_step_counter(torchgpu_ctx* c);
+// The device step count at which ctx_step_counter was last brought up to
+// date (the device loop launches steps that may not run: a checkpoint pause
+// or tmax halts them, and only the steps that ran turn the counter).
+int& ctx_counter_step(torchgpu_ctx* c);
 // Bookkeeping of the state Torch::run prints (see torchgpu_get_print_state):
 // the mode of the last run, and (when pausing at checkpoints) the buffer
 // that keeps the input U of an operator-split step's last sub-step (mode
