@@ -84,6 +84,18 @@ def test_ray_trace_matches_oracle(gpu, shape, star, wind_r):
     in9b, dcolb = cool.grid_inputs(g)
     np.testing.assert_array_equal(in9b, in9)
     np.testing.assert_array_equal(dcolb, dcol)
+    # new densities set between calls (the x-face buffers hold the previous
+    # call's columns), each call against the oracle
+    rng = np.random.default_rng(7)
+    for rep in range(3):
+        Q2 = Q.copy()
+        Q2[0] = Q[0] * rng.uniform(0.5, 2.0, size=shape)
+        g.set_primitive(Q2)
+        g.synchronize()
+        in9c, dcolc = cool.grid_inputs(g)
+        col_o, dcol_o = orc.thermo_ray_trace(shape, star, geom, cool.params.hydrogen_mass, in9c[0].reshape(shape))
+        np.testing.assert_array_equal(in9c[6], col_o.ravel(), err_msg=f"replay {rep}")
+        np.testing.assert_array_equal(dcolc, dcol_o.ravel(), err_msg=f"replay {rep}")
 
 
 def test_star_outside_grid_rejected(gpu):
