@@ -325,6 +325,11 @@ typedef struct torchgpu_star {
  * (Grid.cpp:922-951) and Radiation::cellPathLength (Radiation.cpp:102-117).
  * NULL switches the star off.  The star cell must lie inside the grid. */
 int torchgpu_cooling_set_star(torchgpu_cooling* c, torchgpu_ctx* ctx, const torchgpu_star* star);
+/* Thermodynamics::initialiseMinTempField (Thermodynamics.cpp:102-111): the
+ * per-cell T_min [n] the grid calls take when they are passed no T_min
+ * (with min_temp_initial_state; NULL restores the constant m_T_min = 100).
+ * Host array unless `on_device`.  Synchronises. */
+int torchgpu_cooling_set_min_temp(torchgpu_cooling* c, const double* T_min, int64_t n, int on_device);
 /* The star's per-cell geometry, as Radiation::initField sets it
  * (Radiation.cpp:78-96): the four upstream neighbours (flat indices, -1
  * none) and weights (int64 / double [n][4]), the path length ds and the

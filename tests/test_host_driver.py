@@ -201,9 +201,9 @@ Parameters = {
 
 @pytest.mark.gpu
 def test_driver_cooling_matches_device_loop(tmp_path, gpu):
-    """A 16^3 run with cooling_on through the driver's Torch::run (the
-    host-driven Torch::fullStep sequence of the integrators) and, with
-    --snapshots, through the device loop, against the library's device loop
+    """A 16^3 run with cooling_on through the driver's Torch::run with
+    --host-loop (the host-driven Torch::fullStep sequence of the integrators)
+    and, with --snapshots, through the device loop, against the library's device loop
     (torchgpu_run_split) driven from Python with the parameters the driver
     reports: identical conserved states.  Set-up: a warm (3000 K), partly
     ionised, turbulent medium of 1e-22 g/cm^3 with ADV = 1, so that both
@@ -232,8 +232,7 @@ def test_driver_cooling_matches_device_loop(tmp_path, gpu):
     for mode in ("host", "device"):
         out = tmp_path / f"U_{mode}.bin"
         cmd = [EXE, f"--paramfile={f}", f"--setup={setup}", f"--steps={steps}", f"--output={out}"]
-        if mode == "device":
-            cmd.append("--snapshots")
+        cmd.append("--snapshots" if mode == "device" else "--host-loop")
         r = subprocess.run(cmd, capture_output=True, text=True, timeout=120, cwd=tmp_path)
         assert r.returncode == 0, r.stderr
         res[mode] = np.fromfile(out, dtype=np.float64).reshape(7, n, n, n)
@@ -295,6 +294,75 @@ Parameters = {
     },
 }
 """
+
+
+@pytest.mark.gpu
+def test_driver_min_temp_initial_state(tmp_path, gpu):
+    """Thermodynamics.min_temp_initial_state = true (the reference's shipped
+    config): each cell's T_min is Fluid::calcTemperature of the set-up's cgs
+    primitives with the code-unit gas constant, computed before
+    Torch::toCodeUnits (Torch.cpp:113-117, Thermodynamics.cpp:102-111), so
+    for these scales it sits ~1e-8 of the gas temperature.  The driver's run
+    against the library's device loop given that field, and against the
+    constant T_min = 100 K that `false` gives (cool 150-250 K gas: the soft
+    landing below T_min + 200 K separates the two)."""
+    _exe_or_skip()
+    import torch
+
+    n, steps = 16, 6
+    rng = np.random.default_rng(11)
+    rho = 1.0e-22 * 10.0 ** rng.uniform(-0.5, 0.5, (n, n, n))
+    hii = rng.uniform(0.0, 0.05, rho.shape)
+    T = rng.uniform(150.0, 250.0, rho.shape)
+    Q = np.zeros((7, n, n, n))
+    Q[0] = rho
+    Q[1] = rho * 8.314462e7 * (1.0 + hii) * T
+    Q[2] = hii
+    Q[3] = 1.0
+    setup = tmp_path / "Q.bin"
+    Q.tofile(setup)
+    res = {}
+    for on in ("true", "false"):
+        f = tmp_path / f"cool_{on}.lua"
+        f.write_text(COOL_LUA.replace("heating_amplification = 1.0 }",
+                                      f"heating_amplification = 1.0, min_temp_initial_state = {on} }}"))
+        out = tmp_path / f"U_{on}.bin"
+        r = subprocess.run([EXE, f"--paramfile={f}", f"--setup={setup}", f"--steps={steps}", f"--output={out}"],
+                           capture_output=True, text=True, timeout=120, cwd=tmp_path)
+        assert r.returncode == 0, r.stderr
+        res[on] = np.fromfile(out, dtype=np.float64).reshape(7, n, n, n)
+    d = json.loads(subprocess.run([EXE, f"--paramfile={f}", "--dump-params"], capture_output=True, text=True,
+                                  timeout=60).stdout)
+    U0out = tmp_path / "U0.bin"
+    r = subprocess.run([EXE, f"--paramfile={f}", f"--setup={setup}", "--steps=0", f"--output={U0out}"],
+                       capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    U0 = np.fromfile(U0out, dtype=np.float64).reshape(7, n, n, n)
+    cu = d["code_units"]
+    X = d["mass_fraction_h"]
+    mu_inv = X * (Q[2] + 1.0) + (1.0 - X) * 0.25
+    Tmin = (Q[1] / Q[0]) * (1.0 / mu_inv) / cu["spec_gas_const"]
+    assert Tmin.max() < 1e-3 * T.min()
+    ups = {}
+    for key, field in (("true", Tmin), ("false", None)):
+        p = gpu.make_params((n, n, n), side_length=cu["side_length"], gamma=d["gamma"], dfloor=cu["dfloor"],
+                            pfloor=cu["pfloor"], tfloor=d["tfloor"], spec_gas_const=cu["spec_gas_const"],
+                            tmax=cu["tmax"], ncheckpoints=d["ncheckpoints"],
+                            bc=["outflow", "outflow", "periodic", "periodic", "reflecting", "reflecting"])
+        g = gpu.Grid(p)
+        g.set_conserved(U0)
+        tp = gpu.thermo_params(scales=tuple(d["scales"]), thermo_hii_switch=d["thermo_hii_switch"],
+                               mass_fraction_h=X, gamma=d["gamma"], density_floor=d["dfloor"],
+                               temperature_floor=d["tfloor"])
+        tp.pfloor = cu["pfloor"]
+        cool = gpu.Cooling(tp)
+        tm = None if field is None else torch.from_numpy(np.ascontiguousarray(field).ravel()).cuda()
+        cool.run_split(g, steps, T_min=tm)
+        g.synchronize()
+        ups[key] = g.get_conserved()
+    np.testing.assert_array_equal(res["true"], ups["true"])
+    np.testing.assert_array_equal(res["false"], ups["false"])
+    assert not np.array_equal(res["true"][1], res["false"][1])
 
 
 @pytest.mark.gpu

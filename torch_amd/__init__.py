@@ -284,6 +284,7 @@ def _bind_cooling(L):
     L.torchgpu_thermo_grid_substep.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int64, d, i]
     L.torchgpu_run_split.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int64, i, i]
     L.torchgpu_cooling_set_star.argtypes = [vp, vp, ctypes.POINTER(Star)]
+    L.torchgpu_cooling_set_min_temp.argtypes = [vp, vp, ctypes.c_int64, i]
     L.torchgpu_star_geometry.argtypes = [vp, vp, vp, vp, vp, i]
     L.torchgpu_thermo_grid_inputs.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int64, vp, vp, i]
 
@@ -497,6 +498,22 @@ class Cooling:
         the Strang alternation on the device (no host round trip)."""
         self._check(lib().torchgpu_run_split(grid._h, self._h, self._dptr(T_min), self._dptr(col_den),
                                              self._dptr(F_FUV), excluded, nsteps, int(surrogate)))
+
+    def set_min_temp(self, T_min=None):
+        """Thermodynamics::initialiseMinTempField: the per-cell T_min (host
+        or device array [n]) the grid calls use when given none; None
+        restores the constant 100 K."""
+        if T_min is None:
+            self._check(lib().torchgpu_cooling_set_min_temp(self._h, None, 0, 0))
+            return
+        if hasattr(T_min, "data_ptr"):
+            self._check(lib().torchgpu_cooling_set_min_temp(self._h, ctypes.c_void_p(T_min.data_ptr()),
+                                                            T_min.numel(), int(T_min.is_cuda)))
+        else:
+            import numpy as np
+
+            a = np.ascontiguousarray(T_min, dtype=np.float64).ravel()
+            self._check(lib().torchgpu_cooling_set_min_temp(self._h, a.ctypes.data_as(ctypes.c_void_p), a.size, 0))
 
     def set_star(self, grid: "Grid", star: "Star | None"):
         """Star-on thermodynamics on `grid` (torchgpu_cooling_set_star): ray

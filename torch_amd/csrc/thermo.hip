@@ -629,6 +629,10 @@ struct torchgpu_cooling {
     // star (torchgpu_cooling_set_star): geometry and the DCOL_DEN of the
     // ray trace (COL_DEN goes straight into in9)
     tg::StarGeom star{};
+    // Thermodynamics::initialiseMinTempField's per-cell T_min, taken when a
+    // grid call passes no T_min (else the constant m_T_min = 100 K)
+    double* tmin = nullptr;
+    long long tmin_n = 0;
     double* dcol = nullptr;
     long long dcol_cap = 0;
     double* xface = nullptr;   // the ray trace's x-face buffers
@@ -791,7 +795,7 @@ void torchgpu_cooling_destroy(torchgpu_cooling* c) {
     for (void* ptr : {(void*)c->tables, (void*)c->counters, (void*)c->err, (void*)c->mlp_list, (void*)c->fb_list,
                       (void*)c->feat, (void*)c->y, (void*)c->sdata, (void*)c->in9, (void*)c->out5, (void*)c->dtbits,
                       (void*)c->lpt_order, (void*)c->lpt_bucket, (void*)c->lpt_hist, (void*)c->dcol,
-                      (void*)c->xface})
+                      (void*)c->xface, (void*)c->tmin})
         if (ptr) (void)hipFree(ptr);
     delete c;
 }
@@ -932,6 +936,10 @@ static int gather(torchgpu_cooling* c, torchgpu_ctx* ctx, const double* T_min, c
     if (nb > 65536) nb = 65536;
     if (c->star.on && (c->star.n[0] != v.g->nx || c->star.n[1] != v.g->ny || c->star.n[2] != v.g->nz))
         return TORCHGPU_ERR_INVALID_ARG;
+    if (!T_min && c->tmin) {
+        if (c->tmin_n != n) return TORCHGPU_ERR_INVALID_ARG;
+        T_min = c->tmin;
+    }
     hipLaunchKernelGGL(tg::thermo_gather_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, *v.g, v.U, T_min,
                        100.0, col_den, F, dt, v.st, excluded, c->star, c->in9);
     HIP_OK(hipGetLastError());
@@ -1044,6 +1052,21 @@ int torchgpu_cooling_set_star(torchgpu_cooling* c, torchgpu_ctx* ctx, const torc
         c->xface_cap = nx;
     }
     c->star = s;
+    return TORCHGPU_OK;
+}
+
+int torchgpu_cooling_set_min_temp(torchgpu_cooling* c, const double* T_min, int64_t n, int on_device) {
+    if (!c || (T_min && n < 1)) return TORCHGPU_ERR_INVALID_ARG;
+    HIP_OK(hipStreamSynchronize(c->stream));
+    if (c->tmin) (void)hipFree(c->tmin);
+    c->tmin = nullptr;
+    c->tmin_n = 0;
+    if (!T_min) return TORCHGPU_OK;
+    HIP_OK(hipMalloc(&c->tmin, n * sizeof(double)));
+    c->tmin_n = n;
+    HIP_OK(hipMemcpyAsync(c->tmin, T_min, n * sizeof(double),
+                          on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
     return TORCHGPU_OK;
 }
 

@@ -286,6 +286,25 @@ void Thermodynamics::initialise(const TorchParameters& p, void* stream) {
     check_rc(torchgpu_cooling_create(&t, stream, &cool_), "cooling create");
 }
 
+void Thermodynamics::initialiseMinTempField(const TorchParameters& p, const double* Qcgs, size_t n) {
+    // Thermodynamics.cpp:102-111, called by Torch::initialise before
+    // toCodeUnits (Torch.cpp:113-117): with min_temp_initial_state each
+    // cell's T_min is Fluid::calcTemperature of its cgs primitives with the
+    // code-unit gas constant, as the reference computes it; else the
+    // constant m_T_min = 100 the library uses by default
+    if (!p.minTempInitialState) {
+        check_rc(torchgpu_cooling_set_min_temp(cool_, nullptr, 0, 0), "min temp field");
+        return;
+    }
+    std::vector<double> T(n);
+    for (size_t c = 0; c < n; ++c) {
+        const double hii = Qcgs[2 * n + c], pre = Qcgs[1 * n + c], den = Qcgs[0 * n + c];
+        const double mu_inv = p.massFractionH * (hii + 1.0) + (1.0 - p.massFractionH) * 0.25;
+        T[c] = (pre / den) * (1.0 / mu_inv) / p.specificGasConstant;
+    }
+    check_rc(torchgpu_cooling_set_min_temp(cool_, T.data(), (int64_t)n, 0), "min temp field");
+}
+
 void Thermodynamics::preTimeStepCalculations(Fluid&) const {}  // rates are evaluated with integrate
 
 double Thermodynamics::calculateTimeStep(double dt_max, Fluid& fluid) const {
@@ -474,6 +493,7 @@ void Torch::setPrimitivesCgs(const double* Qcgs) {
                                     params.patchoffset[0], params.patchoffset[1], params.patchoffset[2]) != TORCHGPU_OK)
             throw std::runtime_error("DataReader::patchGrid: cannot patch the grid with " + params.patchfilename);
     }
+    if (params.cooling_on) thermodynamics.initialiseMinTempField(params, Q.data(), n);
     const double kd = params.toCodeUnits(1.0, 1, -3, 0), kp = params.toCodeUnits(1.0, 1, -1, -2),
                  kv = params.toCodeUnits(1.0, 0, 1, -1);
     const bool unit = (kd == 1.0 && kp == 1.0 && kv == 1.0);

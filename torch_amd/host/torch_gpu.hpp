@@ -171,6 +171,9 @@ public:
     double calculateTimeStep(double dt_max, Fluid& fluid) const override;
     void integrate(double dt, Fluid& fluid) const override;
     void updateSourceTerms(double dt, Fluid& fluid) const override;
+    // Thermodynamics::initialiseMinTempField on the set-up's primitives
+    // (cgs, [7][n], before Torch::toCodeUnits)
+    void initialiseMinTempField(const TorchParameters& p, const double* Qcgs, size_t n);
     torchgpu_cooling* handle() const { return cool_; }
 
 private:
