@@ -300,8 +300,10 @@ int torchgpu_thermo_grid_substep(torchgpu_ctx* ctx, torchgpu_cooling* c, const d
  * thermo min |E/RATE|) (Torch.cpp:304-339) -- chosen on the device, and the
  * Strang alternation of single-stage hydro sub-steps (Torch::subStep) and
  * thermo sub-steps (as torchgpu_thermo_grid_substep).  Both handles must use
- * the same stream.  Asynchronous: no host round trip inside the loop; errors
- * end the run and surface at the next synchronizing call. */
+ * the same stream.  One host round trip at the start of the call (the
+ * step count, so that Torch::stepCounter turns only with steps that ran),
+ * none inside the loop; errors end the run and surface at the next
+ * synchronizing call. */
 int torchgpu_run_split(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* T_min, const double* col_den,
                        const double* F_FUV, int64_t excluded, int nsteps, int surrogate);
 
