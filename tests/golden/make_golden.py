@@ -340,6 +340,46 @@ def star_fullstep():
     print("star_fullstep", info, dts)
 
 
+MINTEMP = dict(n=12, nsteps=6, side=3.0857e18, seed=43, kv={"cooling_on": 1, "thermo_hii_switch": 1e-2})
+
+
+def mintemp_q():
+    """The cgs primitives of the min_temp_initial_state case: cool (150-250 K),
+    nearly neutral gas of ~1e-22 g/cm^3 with ADV = 1, at rest, where the
+    soft landing below T_min + 200 K acts when T_min is 100 K."""
+    cs = MINTEMP
+    n = cs["n"]
+    rng = np.random.default_rng(cs["seed"])
+    rho = 1.0e-22 * 10.0 ** rng.uniform(-0.5, 0.5, (n, n, n))
+    hii = rng.uniform(0.0, 0.05, rho.shape)
+    T = rng.uniform(150.0, 250.0, rho.shape)
+    Q = np.zeros((7, n, n, n))
+    Q[0] = rho
+    Q[1] = rho * 8.314462e7 * (1.0 + hii) * T
+    Q[2] = hii
+    Q[3] = 1.0
+    return Q
+
+
+def mintemp():
+    """Thermodynamics with min_temp_initial_state on and off through the
+    reference harness's driver loop (hydro + thermo): U after the last step
+    and every dt of both runs."""
+    cs = MINTEMP
+    Q = mintemp_q()
+    res = {}
+    for on in (1, 0):
+        extra = {"density_scale": 1e-20, "pressure_scale": 1e-28, "time_scale": 1e11, "side_length": cs["side"],
+                 "temperature_floor": 0.1, "simulation_time": 1e13, "ncheckpoints": 10,
+                 "min_temp_initial_state": on, **cs["kv"]}
+        out, dts, info = run_hydro_case(Q, cs["nsteps"], [cs["nsteps"]], tfloor=0.1, extra=extra)
+        res[on] = (out[cs["nsteps"]], dts)
+        print("mintemp", on, info, dts)
+    assert not np.array_equal(res[1][0], res[0][0])
+    np.savez_compressed(f"{HERE}/mintemp.npz", Q=Q, U_on=res[1][0], dts_on=res[1][1], U_off=res[0][0],
+                        dts_off=res[0][1])
+
+
 if __name__ == "__main__":
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle ref")

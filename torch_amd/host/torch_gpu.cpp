@@ -244,7 +244,8 @@ void Fluid::advSolution(double dt) {
         check_rc(torchgpu_hydro_stage_dt(ctx_, stage, dt), "hydro stage", ctx_);
         if (stage == 2) w_saved_ = u_is_w_ = false;
     } else if (pending == Pending::THERMO) {
-        check_rc(torchgpu_thermo_grid_substep(ctx_, cooling, nullptr, nullptr, nullptr, -1, dt, 0), "thermo substep",
+        check_rc(torchgpu_thermo_grid_substep(ctx_, cooling, nullptr, nullptr, nullptr, cooling_excluded, dt, 0),
+                 "thermo substep",
                  ctx_);
     } else if (pending == Pending::RAD) {
         // Torch::subStep has made preTimeStepCalculations on this state
@@ -284,6 +285,22 @@ void Thermodynamics::initialise(const TorchParameters& p, void* stream) {
              "thermo params");
     t.pfloor = p.pfloor;
     check_rc(torchgpu_cooling_create(&t, stream, &cool_), "cooling create");
+    // Fluid::initialiseGrid (Fluid.cpp:41-54) puts the star's cell, and the
+    // cells within the wind radius, on the CausalWind list whether or not the
+    // star is on, and Thermodynamics only walks CausalNonWind: with the star
+    // off that is the one cell at Star.cell_position (wind radius 0)
+    if (!p.star_on) {
+        if (p.windCellRadius > 0)
+            throw std::runtime_error("Star.wind_radius_in_cells > 0 with Star.on = false: the thermo operator skips "
+                                     "one excluded cell only");
+        const torchgpu_params g = p.gridParams();
+        for (int a = 0; a < 3; ++a) {
+            const int na = a == 0 ? g.nx : (a == 1 ? g.ny : g.nz);
+            if (p.star_position[a] < 0 || p.star_position[a] >= na)
+                throw std::runtime_error("Star.cell_position outside the grid");
+        }
+        excluded_ = p.star_position[0] + (int64_t)g.nx * (p.star_position[1] + (int64_t)g.ny * p.star_position[2]);
+    }
 }
 
 void Thermodynamics::initialiseMinTempField(const TorchParameters& p, const double* Qcgs, size_t n) {
@@ -309,13 +326,14 @@ void Thermodynamics::preTimeStepCalculations(Fluid&) const {}  // rates are eval
 
 double Thermodynamics::calculateTimeStep(double dt_max, Fluid& fluid) const {
     double dt = 0;
-    check_rc(torchgpu_thermo_grid_timestep(fluid.ctx(), cool_, nullptr, nullptr, nullptr, -1, dt_max, &dt),
+    check_rc(torchgpu_thermo_grid_timestep(fluid.ctx(), cool_, nullptr, nullptr, nullptr, excluded_, dt_max, &dt),
              "thermo timestep", fluid.ctx());
     return dt;
 }
 
 void Thermodynamics::integrate(double, Fluid& fluid) const {
     fluid.cooling = cool_;
+    fluid.cooling_excluded = excluded_;
     fluid.pending = Fluid::Pending::THERMO;
 }
 void Thermodynamics::updateSourceTerms(double, Fluid&) const {}
@@ -601,8 +619,8 @@ void Torch::run(int max_steps, bool print, bool host_loop) {  // Torch.cpp:240-3
     // calculations and the sub-steps of every active component
     auto device_run = [&](int n) {
         check_rc(torchgpu_run_full(c, params.cooling_on ? thermodynamics.handle() : nullptr,
-                                   params.radiation_on ? radiation.handle() : nullptr, nullptr, nullptr, nullptr, -1,
-                                   n, 0),
+                                   params.radiation_on ? radiation.handle() : nullptr, nullptr, nullptr, nullptr,
+                                   params.cooling_on ? thermodynamics.excluded() : -1, n, 0),
                  "run", c);
     };
     if (print) {

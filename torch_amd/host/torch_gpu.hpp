@@ -139,6 +139,7 @@ public:
     enum class Pending { NONE, HYDRO, THERMO, RAD };
     Pending pending = Pending::NONE;
     torchgpu_cooling* cooling = nullptr;      // set by Thermodynamics
+    int64_t cooling_excluded = -1;            // ... with its excluded cell
     torchgpu_radiation* radiation = nullptr;  // set by Radiation
 
     void check() const;  // reports device errors as std::runtime_error
@@ -175,9 +176,13 @@ public:
     // (cgs, [7][n], before Torch::toCodeUnits)
     void initialiseMinTempField(const TorchParameters& p, const double* Qcgs, size_t n);
     torchgpu_cooling* handle() const { return cool_; }
+    // the CausalWind cell of a star-off run (-1 with the star on: the
+    // cooling handle skips the wind cells itself)
+    int64_t excluded() const { return excluded_; }
 
 private:
     torchgpu_cooling* cool_ = nullptr;
+    int64_t excluded_ = -1;
 };
 
 // Radiation (src/Integrators/Radiation.cpp) for a star-on run.
