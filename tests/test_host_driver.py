@@ -434,6 +434,14 @@ def test_driver_star_radiation_cooling_matches_python_loop(tmp_path, gpu):
     assert np.isfinite(U).all() and dts[1] > dts[0]
     assert not np.array_equal(U[2], outs[0][2])  # the star ionised gas
     np.testing.assert_array_equal(outs[steps], U)
+    # the driver's host-driven Torch::fullStep (--host-loop) and its device
+    # loop with checkpoint printing (--snapshots) give the same state
+    for flag in ("--host-loop", "--snapshots"):
+        out = tmp_path / f"U_{flag[2:]}.bin"
+        r = subprocess.run([EXE, f"--paramfile={f}", f"--setup={setup}", f"--steps={steps}", f"--output={out}", flag],
+                           capture_output=True, text=True, timeout=120, cwd=tmp_path)
+        assert r.returncode == 0, (flag, r.stderr)
+        np.testing.assert_array_equal(np.fromfile(out, dtype=np.float64).reshape(7, n, n, n), U, err_msg=flag)
 
 
 @pytest.mark.gpu
