@@ -5,7 +5,7 @@ updateSourceTerms, implicit scheme, neq coupling), on an n^3 clumpy neutral
 medium (n_H ~ 1e2..1e4 cm^-3, 100 K) with the star at the centre, wind
 radius 2, side 1 pc, 4.9e48 photons/s.
 
-    python -m torch_amd.star_bench [--n 256] [--reps 5]
+    python -m torch_amd.star_bench [--n 256] [--reps 5] [--fullstep N]
 
 Prints one JSON line: GPU seconds per call and Mcell/s of each operator
 (HIP events around `reps` calls after one warm-up call; inputs resident in
@@ -83,11 +83,59 @@ def gpu_rates(n: int = 256, reps: int = 5) -> dict:
             "radiation_dt": dt, "radiation_substep_s": t_rad, "radiation_mcells_per_s": n ** 3 / t_rad / 1e6}
 
 
+def fullstep_rates(n: int = 128, steps: int = 5, warm: int = 2) -> dict:
+    """ms per Torch::fullStep (hydro FP64, star-on cooling, radiation) on the
+    n^3 star workload: the host-driven FullStep (one host round trip per
+    time-step reduction) and the device loop (torchgpu_run_full), each after
+    `warm` steps, on two grids that follow the same trajectory."""
+    import time
+
+    import torch
+
+    import torch_amd as ta
+    from torch_amd import fullstep
+
+    units = lambda v, m, l, t: ta.units(SCALES, v, m, l, t)  # noqa: E731
+    Q, side = medium(n, units)
+    R = units(8.314462e7, 0, 2, -2)
+    dfloor = units(1e-30, 1, -3, 0)
+    out = {"n": n, "steps": steps, "warm": warm}
+    for mode in ("host", "device"):
+        p = ta.make_params((n, n, n), side_length=units(side, 0, 1, 0), gamma=1.67, dfloor=dfloor,
+                           pfloor=0.1 * R * dfloor * 0.1, tfloor=0.1, spec_gas_const=R, tmax=units(1e13, 0, 0, 1))
+        g = ta.Grid(p)
+        g.set_primitive(Q)
+        star = ta.make_star((n // 2, n // 2, n // 2), 2, units(4.9e48, 0, 0, -1))
+        tp = ta.thermo_params(scales=SCALES)
+        tp.pfloor = p.pfloor
+        cool = ta.Cooling(tp)
+        cool.set_star(g, star)
+        rad = ta.Radiation(ta.rad_params(SCALES, K=(0.2, 0, 0, 0), coupling="neq"), g, star)
+        rad.init_field(Q)
+        loop = fullstep.FullStep(g, cooling=cool, radiation=rad)
+        run = (lambda k: [loop.step() for _ in range(k)]) if mode == "host" else \
+            (lambda k: rad.run_full(k, cooling=cool))
+        run(warm)
+        g.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run(steps)
+        g.synchronize()
+        out[f"{mode}_ms_per_step"] = (time.perf_counter() - t0) * 1e3 / steps
+        out[f"{mode}_time"] = g.clock()[0] if mode == "device" else loop.time
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--fullstep", type=int, default=0, metavar="STEPS",
+                    help="time STEPS full star steps, host-driven and device loop, instead")
     a = ap.parse_args()
+    if a.fullstep:
+        print(json.dumps(fullstep_rates(a.n, a.fullstep)), flush=True)
+        return
     print(json.dumps(gpu_rates(a.n, a.reps)), flush=True)
 
 
