@@ -65,7 +65,7 @@ def cpu_baseline(n_sample: int, steps: int, threads: int) -> dict:
                 with open(f"{d}/case.txt", "w") as f:
                     f.write(f"nx {n_sample}\nny {n_sample}\nnz {n_sample}\nside_length 1.0\ngamma {GAMMA}\n"
                             f"density_floor 1e-30\ntemperature_floor {TFLOOR}\nsimulation_time 1000\n"
-                            f"ncheckpoints 1\nnsteps {steps}\ninput_q {d}/q.bin\n")
+                            f"ncheckpoints 1\ndebug 0\nnsteps {steps}\ninput_q {d}/q.bin\n")
                 nproc = min(threads, n_sample)
                 r = subprocess.run([mpirun, "-np", str(nproc), harness, "hydro", f"{d}/case.txt"],
                                    capture_output=True, text=True, timeout=300)
@@ -84,7 +84,7 @@ def cpu_baseline(n_sample: int, steps: int, threads: int) -> dict:
                            nthreads=threads)
     U0 = oracle.UfromQ_grid(oracle.fix_primitives(p, Q, np.zeros_like(Q)), GAMMA)
     t0 = time.perf_counter()
-    oracle.run_hydro(p, U0, steps)
+    oracle.run_hydro(p, U0, steps, debug=False)
     dt = time.perf_counter() - t0
     return {"value": round(n_sample ** 3 * steps / dt / 1e6, 4), "unit": "Mcell-updates/s", "cores": threads,
             "kind": "port", "sample": f"{n_sample}^3 turbulent box, {steps} steps, FP64 oracle, {threads} OpenMP threads"}
@@ -116,7 +116,7 @@ def hydro_rates(n: int, steps: int, warmup: int, err_steps: int):
     log(f"IC {n}^3 in {time.perf_counter() - t0:.1f}s")
     res = {}
     for name, prec in (("fp64", ta.FP64), ("mixed", ta.MIXED)):
-        g = ta.Grid(ta.make_params((n, n, n), gamma=GAMMA, tfloor=TFLOOR, pfloor=PFLOOR, precision=prec),
+        g = ta.Grid(ta.make_params((n, n, n), gamma=GAMMA, tfloor=TFLOOR, pfloor=PFLOOR, precision=prec, debug=False),
                     stream=stream.cuda_stream)
         g.set_primitive(Q)
         g.run(warmup)
@@ -149,9 +149,9 @@ def hydro_rates(n: int, steps: int, warmup: int, err_steps: int):
         g.close()
 
     # accuracy of the mixed path: same dt sequence as the FP64 run
-    g64 = ta.Grid(ta.make_params((n, n, n), gamma=GAMMA, tfloor=TFLOOR, pfloor=PFLOOR, precision=ta.FP64),
+    g64 = ta.Grid(ta.make_params((n, n, n), gamma=GAMMA, tfloor=TFLOOR, pfloor=PFLOOR, precision=ta.FP64, debug=False),
                   stream=stream.cuda_stream)
-    gmx = ta.Grid(ta.make_params((n, n, n), gamma=GAMMA, tfloor=TFLOOR, pfloor=PFLOOR, precision=ta.MIXED),
+    gmx = ta.Grid(ta.make_params((n, n, n), gamma=GAMMA, tfloor=TFLOOR, pfloor=PFLOOR, precision=ta.MIXED, debug=False),
                   stream=stream.cuda_stream)
     g64.set_primitive(Q)
     gmx.set_primitive(Q)

@@ -11,7 +11,7 @@
  *     Fluid::globalUfromQ   (src/Torch/Torch.cpp:117-119)
  *   Fluid::globalQfromU / fixPrimitives                   torchgpu_get_primitive
  *     (src/Fluid/Fluid.cpp:138-150,162-165)
- *   GridCell::U state read/write (src/Fluid/GridCell.hpp:265)  torchgpu_set_conserved /
+ *   GridCell::U state read/write (src/Fluid/GridCell.hpp:43-105) torchgpu_set_conserved /
  *                                                              torchgpu_get_conserved
  *   Hydrodynamics::calculateTimeStep                      torchgpu_hydro_timestep
  *     (src/Integrators/Hydro.cpp:77-89)
@@ -30,7 +30,7 @@
  *     hydrodynamics and thermodynamics and/or radiation     torchgpu_run_full
  *     (Torch.cpp:240-418)
  *   Torch::run's Checkpointer test and the state it       torchgpu_pause_at_checkpoints /
- *     prints (Torch.cpp:268-277, Checkpointer.cpp:14-40)     torchgpu_checkpoint /
+ *     prints (Torch.cpp:268-277, Checkpointer.cpp:14-30)     torchgpu_checkpoint /
  *                                                            torchgpu_get_print_state
  *   Torch::initialise start time from initial conditions  torchgpu_set_time
  *     (Torch.cpp:100-103)
@@ -74,7 +74,7 @@ enum {
     TORCHGPU_ERR_CHECK_VALUES = 5,   /* Torch::checkValues throw (Torch.cpp:420-443) */
     TORCHGPU_ERR_INVALID_ORDER = 6,  /* Hydrodynamics::calcFluxes invalid order (Hydro.cpp:110-112) */
     TORCHGPU_ERR_UNKNOWN_SOLVER = 7, /* RiemannSolverFactory::create unknown type (Riemann.cpp:256) */
-    TORCHGPU_ERR_UNKNOWN_LIMITER = 8,/* SlopeLimiterFactory::create unknown type (SlopeLimiter.cpp:331) */
+    TORCHGPU_ERR_UNKNOWN_LIMITER = 8,/* SlopeLimiterFactory::create unknown type (SlopeLimiter.cpp:73-74) */
     TORCHGPU_ERR_NSTEPS_RANGE = 9,   /* subcycle count beyond int range (Thermodynamics.cpp:307) */
     TORCHGPU_ERR_NOT_TRAINED = 10,   /* surrogate used before weights were set */
     TORCHGPU_ERR_RADIATION = 11      /* Radiation throws: NaN heating rate, invalid timestep, implicit
@@ -90,7 +90,8 @@ enum { TORCHGPU_BC_FREE = 0, TORCHGPU_BC_REFLECTING = 1, TORCHGPU_BC_OUTFLOW = 2
 enum { TORCHGPU_SOLVER_HLL = 0, TORCHGPU_SOLVER_HLLC = 1, TORCHGPU_SOLVER_ROTATED_HLLC = 2 };
 /* Slope limiters, SlopeLimiterFactory::create strings
  * "monotonised_central", "superbee", "minmod", "maxmod", "leer", "ospre",
- * "albada" ("default"), src/Integrators/SlopeLimiter.cpp:313-332. */
+ * "albada" ("default"), src/Integrators/SlopeLimiter.cpp:56-75; the limiters
+ * themselves are SlopeLimiter.cpp:7-54. */
 enum { TORCHGPU_LIM_MC = 0, TORCHGPU_LIM_SUPERBEE = 1, TORCHGPU_LIM_MINMOD = 2, TORCHGPU_LIM_MAXMOD = 3,
        TORCHGPU_LIM_LEER = 4, TORCHGPU_LIM_OSPRE = 5, TORCHGPU_LIM_ALBADA = 6 };
 /* Precision of the hydro flux path. */
@@ -108,6 +109,9 @@ typedef struct torchgpu_params {
     int slope_limiter;        /* TORCHGPU_LIM_* */
     int precision;            /* TORCHGPU_FP64 or TORCHGPU_MIXED */
     int ncheckpoints;         /* Integration.ncheckpoints */
+    int debug;                /* Integration.debug (Parameters.hpp:50, default true): the device loops
+                                 end after a step whose hydro dt has 100 dt_hydro / tmax <= 1e-6
+                                 (Torch::calculateTimeStep, Torch.cpp:321-332) */
     double side_length;       /* Grid.side_length */
     double gamma;             /* Hydrodynamics.gamma */
     double dfloor;            /* density floor */
@@ -177,12 +181,19 @@ int torchgpu_reset_clock(torchgpu_ctx* ctx);
 int torchgpu_synchronize(torchgpu_ctx* ctx);
 /* Simulation time, dt of the last step and number of completed steps. */
 int torchgpu_get_clock(torchgpu_ctx* ctx, double* time, double* last_dt, int* steps);
+/* Torch::isQuitting (src/Torch/Torch.hpp:47): set, with params.debug, by the
+ * step whose hydro time step gave 100 dt_hydro / tmax <= 1e-6
+ * ("Integration deltas are too small.", Torch.cpp:321-332); that step runs,
+ * later steps of torchgpu_run / run_split / run_full do nothing (the
+ * reference's Torch::run loop ends).  Cleared by torchgpu_reset_clock.
+ * Synchronises. */
+int torchgpu_is_quitting(torchgpu_ctx* ctx, int* quitting);
 /* Torch::initialise's start time from an initial-conditions snapshot
  * (src/Torch/Torch.cpp:100-103, code units; t >= 0).  Synchronises. */
 int torchgpu_set_time(torchgpu_ctx* ctx, double t);
 /* With `on`, torchgpu_run, torchgpu_run_split and torchgpu_run_full stop at
  * the end of a step that reaches a checkpoint (the Checkpointer test of
- * Torch::run, src/Torch/Torch.cpp:268-277, src/IO/Checkpointer.cpp:14-40): later steps
+ * Torch::run, src/Torch/Torch.cpp:268-277, src/IO/Checkpointer.cpp:14-30): later steps
  * of the same call do nothing until torchgpu_checkpoint resumes the run.
  * Also keeps the state Torch::run prints at that point.  Synchronises. */
 int torchgpu_pause_at_checkpoints(torchgpu_ctx* ctx, int on);

@@ -87,11 +87,11 @@ def lib() -> ctypes.CDLL:
         L.orc_hydro_rhs.restype = ctypes.c_int
         L.orc_hydro_step.argtypes = [P(HydroParams), dp, dp, dp, dp, d]
         L.orc_hydro_step.restype = ctypes.c_int
-        L.orc_run_hydro.argtypes = [P(HydroParams), dp, ctypes.c_int, d, ctypes.c_int, dp, dp]
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.orc_run_hydro.argtypes = [P(HydroParams), dp, ctypes.c_int, d, ctypes.c_int, dp, dp, ctypes.c_int, ip, ip]
         L.orc_run_hydro.restype = ctypes.c_int
         L.orc_run_hydro_print.argtypes = [P(HydroParams), dp, ctypes.c_int, d, d, ctypes.c_int, dp, dp, dp,
-                                          ctypes.c_int,
-                                          ctypes.POINTER(ctypes.c_int), dp, ctypes.POINTER(ctypes.c_int)]
+                                          ctypes.c_int, ip, dp, ip, ctypes.c_int, ip, ip]
         L.orc_run_hydro_print.restype = ctypes.c_int
         L.orc_UfromQ.argtypes = [dp, dp, d]
         L.orc_QfromU.argtypes = [dp, dp, d]
@@ -203,18 +203,28 @@ def hydro_step(p: HydroParams, U: np.ndarray, dt: float) -> np.ndarray:
     return U
 
 
-def run_hydro(p: HydroParams, U: np.ndarray, nsteps: int, tmax: float = 1e3, ncheckpoints: int = 1):
-    """Torch::run for a hydro-only problem, capped at nsteps.  Returns (U, dts, time)."""
+def run_hydro(p: HydroParams, U: np.ndarray, nsteps: int, tmax: float = 1e3, ncheckpoints: int = 1,
+              debug: bool = True, info: bool = False):
+    """Torch::run for a hydro-only problem, capped at nsteps.  Returns (U,
+    dts, time); dts holds zeros after the last step run.  `debug` is
+    Integration.debug (default true, Parameters.hpp:50): the loop ends after
+    a step whose hydro dt is at most 1e-8 tmax (Torch.cpp:321-332).  With
+    `info`, returns (U, dts of the steps run, time, quitting)."""
     U = np.ascontiguousarray(U, dtype=np.float64).copy()
     dts = np.zeros(nsteps)
     t = ctypes.c_double(0)
-    err = lib().orc_run_hydro(ctypes.byref(p), _dp(U), nsteps, tmax, ncheckpoints, _dp(dts), ctypes.byref(t))
+    ns, quit_ = ctypes.c_int(0), ctypes.c_int(0)
+    err = lib().orc_run_hydro(ctypes.byref(p), _dp(U), nsteps, tmax, ncheckpoints, _dp(dts), ctypes.byref(t),
+                              1 if debug else 0, ctypes.byref(ns), ctypes.byref(quit_))
     if err:
         raise RuntimeError(f"hydro run failed (code {err})")
+    if info:
+        return U, dts[:ns.value], t.value, bool(quit_.value)
     return U, dts, t.value
 
 
-def run_hydro_print(p: HydroParams, U: np.ndarray, nsteps: int, tmax: float, ncheckpoints: int, t0: float = 0.0):
+def run_hydro_print(p: HydroParams, U: np.ndarray, nsteps: int, tmax: float, ncheckpoints: int, t0: float = 0.0,
+                    debug: bool = True):
     """run_hydro from time t0 with the snapshots Torch::run prints
     (Torch.cpp:240-302): returns (U, dts, time, [(count, time, Q), ...])."""
     U = np.ascontiguousarray(U, dtype=np.float64).copy()
@@ -226,7 +236,7 @@ def run_hydro_print(p: HydroParams, U: np.ndarray, nsteps: int, tmax: float, nch
     pt = np.zeros(maxp)
     npr = ctypes.c_int(0)
     err = lib().orc_run_hydro_print(ctypes.byref(p), _dp(U), nsteps, t0, tmax, ncheckpoints, _dp(dts), ctypes.byref(t),
-                                    _dp(qp), maxp, pc, _dp(pt), ctypes.byref(npr))
+                                    _dp(qp), maxp, pc, _dp(pt), ctypes.byref(npr), 1 if debug else 0, None, None)
     if err:
         raise RuntimeError(f"hydro run failed (code {err})")
     prints = [(pc[i], pt[i], qp[i]) for i in range(npr.value)]

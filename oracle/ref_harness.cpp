@@ -101,6 +101,7 @@ static void init_sim(Sim& s, const KV& kv) {
     p.spatialOrder = geti(kv, "spatial_order", 1);
     p.tmax = getd(kv, "simulation_time", 1e3);
     p.ncheckpoints = geti(kv, "ncheckpoints", 1);
+    p.debug = geti(kv, "debug", 1) != 0;  // TorchParameters default (Parameters.hpp:50)
     p.cooling_on = geti(kv, "cooling_on", 0) != 0;
     p.radiation_on = geti(kv, "radiation_on", 0) != 0;
     p.nd = 3;
@@ -297,9 +298,10 @@ static int run_hydro(const KV& kv) {
     if (s.p.cooling_on) comps.push_back(&s.thermo);
     if (s.p.radiation_on) comps.push_back(&s.rad);
 
+    bool quitting = false;  // Torch::m_isQuitting
     mpi.barrier();
     auto t0 = std::chrono::steady_clock::now();
-    for (int step = 1; step <= nsteps && t < s.p.tmax; ++step) {
+    for (int step = 1; step <= nsteps && t < s.p.tmax && !quitting; ++step) {
         double dt_next = ckpt(t);
         // Torch::fullStep, Torch.cpp:389-418.
         s.fluid.globalQfromU();
@@ -316,6 +318,11 @@ static int run_hydro(const KV& kv) {
             if (s.p.radiation_on) dt_rad = s.rad.calculateTimeStep(s.p.dt_max, s.fluid);
             if (s.p.cooling_on) dt_thermo = s.thermo.calculateTimeStep(s.p.dt_max, s.fluid);
             dt = std::min(std::min(dt_hydro, dt_rad), dt_thermo);
+            if (s.p.debug) {  // Torch.cpp:321-332, its condition as written
+                double thyd = 100.0 * dt_hydro / s.p.tmax;
+                thyd = mpi.minimum(thyd);
+                if (thyd <= 1.0e-6) quitting = true;
+            }
         }
         dt = mpi.minimum(dt);
         dt = std::min(dt_next, dt);
@@ -348,8 +355,8 @@ static int run_hydro(const KV& kv) {
         fclose(f);
     }
     if (mpi.getRank() == 0) {
-        printf("{\"cells\": %lld, \"steps\": %zu, \"seconds\": %.6f, \"mcell_updates_per_s\": %.6f, \"nproc\": %d, \"time\": %.17g}\n",
-               NG, dts.size(), secs, (double)NG * dts.size() / secs / 1e6, mpi.nProcessors(), t);
+        printf("{\"cells\": %lld, \"steps\": %zu, \"seconds\": %.6f, \"mcell_updates_per_s\": %.6f, \"nproc\": %d, \"time\": %.17g, \"quitting\": %d}\n",
+               NG, dts.size(), secs, (double)NG * dts.size() / secs / 1e6, mpi.nProcessors(), t, quitting ? 1 : 0);
     }
     return 0;
 }

@@ -38,7 +38,7 @@ static inline double smin(double a, double b) { return (b < a) ? b : a; }
 static inline double smax(double a, double b) { return (a < b) ? b : a; }
 
 /* ------------------------------------------------------------------------ */
-/* Slope limiters: reference src/Integrators/SlopeLimiter.cpp:263-311        */
+/* Slope limiters: reference src/Integrators/SlopeLimiter.cpp:7-54            */
 /* ------------------------------------------------------------------------ */
 static inline double minmod(double a, double b) {
     if (a * b > 0) return fabs(a) < fabs(b) ? a : b;
@@ -61,7 +61,7 @@ double orc_limiter(int limiter, double a, double b) {
     case ORC_LIM_OSPRE:
         if (a * b <= 1.0e-30) return 0;
         return 1.5 * b * (a * a + a * b) / (a * a + a * b + b * b);
-    default: /* ORC_LIM_ALBADA, SlopeLimiter.cpp:309-311 */
+    default: /* ORC_LIM_ALBADA, SlopeLimiter.cpp:52-54 */
         return a * b <= 1.0e-30 ? 0 : a * b * (a + b) / (a * a + b * b);
     }
 }
@@ -565,13 +565,13 @@ static double checkpoint_dt(double t, double tmax, int ncheck, int* count) {
  * Each print stores Q (7n), the checkpoint count and the time. */
 static int run_hydro_impl(const orc_hydro_params* p, double* U, int nsteps, double t0, double tmax,
                           int ncheckpoints, double* dts, double* time_out, double* qprint, int max_prints,
-                          int* pcount, double* ptime, int* nprint) {
+                          int* pcount, double* ptime, int* nprint, int debug, int* steps_out, int* quit_out) {
     const int64_t n = ncells(p);
     double* Q = malloc(sizeof(double) * 7 * n);
     double* W = malloc(sizeof(double) * 7 * n);
     double* UDOT = calloc(7 * n, sizeof(double));
     double t = t0;
-    int count = 0, err = 0, np = 0, final_print = 0;
+    int count = 0, err = 0, np = 0, final_print = 0, quitting = 0, steps = 0;
     (void)checkpoint_dt(t, tmax, ncheckpoints, &count);
 #define PRINT_Q(src, cnt)                                                       \
     do {                                                                      \
@@ -587,7 +587,7 @@ static int run_hydro_impl(const orc_hydro_params* p, double* U, int nsteps, doub
         orc_fixPrimitives(p, Q, U);
         PRINT_Q(Q, count);
     }
-    for (int s = 0; s < nsteps && t < tmax; ++s) {
+    for (int s = 0; s < nsteps && t < tmax && !quitting; ++s) {
         const int before = count;
         double dt_next = checkpoint_dt(t, tmax, ncheckpoints, &count);
         if (count != before) {
@@ -604,14 +604,26 @@ static int run_hydro_impl(const orc_hydro_params* p, double* U, int nsteps, doub
                 if (U[v * n + c] != U[v * n + c] || isinf(U[v * n + c])) bad = 1;
         }
         if (bad) { err = 3; break; }
-        double dt = (s == 0) ? p->dt_max * 1.0e-20 : orc_hydro_timestep(p, Q);
+        double dt;
+        if (s == 0) {
+            dt = p->dt_max * 1.0e-20;
+        } else {
+            dt = orc_hydro_timestep(p, Q);
+            /* Torch::calculateTimeStep, Torch.cpp:321-332: the condition
+             * tests the hydro percentage (all three of its terms) */
+            if (debug && 100.0 * dt / tmax <= 1.0e-6) quitting = 1;
+        }
         dt = smin(dt_next, dt);
         err = orc_hydro_step(p, U, Q, W, UDOT, dt);
         if (dts) dts[s] = dt;
         t += dt;
+        ++steps;
         if (err) break;
     }
-    if (final_print && !(t < tmax)) PRINT_Q(Q, ncheckpoints);
+    /* the print after Torch::run's loop, Torch.cpp:295-297 */
+    if (final_print && (!(t < tmax) || quitting)) PRINT_Q(Q, ncheckpoints);
+    if (steps_out) *steps_out = steps;
+    if (quit_out) *quit_out = quitting;
 #undef PRINT_Q
     if (time_out) *time_out = t;
     if (nprint) *nprint = np;
@@ -620,15 +632,16 @@ static int run_hydro_impl(const orc_hydro_params* p, double* U, int nsteps, doub
 }
 
 int orc_run_hydro(const orc_hydro_params* p, double* U, int nsteps, double tmax, int ncheckpoints,
-                  double* dts, double* time_out) {
-    return run_hydro_impl(p, U, nsteps, 0.0, tmax, ncheckpoints, dts, time_out, NULL, 0, NULL, NULL, NULL);
+                  double* dts, double* time_out, int debug, int* steps_out, int* quit_out) {
+    return run_hydro_impl(p, U, nsteps, 0.0, tmax, ncheckpoints, dts, time_out, NULL, 0, NULL, NULL, NULL, debug,
+                          steps_out, quit_out);
 }
 
 int orc_run_hydro_print(const orc_hydro_params* p, double* U, int nsteps, double t0, double tmax,
                         int ncheckpoints, double* dts, double* time_out, double* qprint, int max_prints, int* pcount,
-                        double* ptime, int* nprint) {
+                        double* ptime, int* nprint, int debug, int* steps_out, int* quit_out) {
     return run_hydro_impl(p, U, nsteps, t0, tmax, ncheckpoints, dts, time_out, qprint, max_prints, pcount, ptime,
-                          nprint);
+                          nprint, debug, steps_out, quit_out);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -40,7 +40,7 @@ enum { ORC_DEN = 0, ORC_PRE = 1, ORC_HII = 2, ORC_ADV = 3, ORC_VEL = 4, ORC_NV =
 enum { ORC_BC_FREE = 0, ORC_BC_REFLECTING = 1, ORC_BC_OUTFLOW = 2, ORC_BC_INFLOW = 3, ORC_BC_PERIODIC = 4 };
 /* Riemann solvers (reference Riemann.cpp:246-257). */
 enum { ORC_HLL = 0, ORC_HLLC = 1, ORC_ROTATED_HLLC = 2 };
-/* Slope limiters (reference SlopeLimiter.cpp:313-332). */
+/* Slope limiters (reference SlopeLimiter.cpp:56-75). */
 enum { ORC_LIM_MC = 0, ORC_LIM_SUPERBEE = 1, ORC_LIM_MINMOD = 2, ORC_LIM_MAXMOD = 3,
        ORC_LIM_LEER = 4, ORC_LIM_OSPRE = 5, ORC_LIM_ALBADA = 6 };
 
@@ -87,17 +87,21 @@ int orc_hydro_step(const orc_hydro_params* p, double* U, double* Q, double* W,
                    double* UDOT, double dt);
 
 /* Torch::run for a hydro-only problem, capped at nsteps.  tmax is code time,
- * ncheckpoints as in the reference.  dts (nsteps) receives each dt.
+ * ncheckpoints as in the reference.  dts (nsteps) receives each dt.  With
+ * debug (Integration.debug), Torch::calculateTimeStep's test ends the loop
+ * after a step whose hydro dt has 100 dt / tmax <= 1e-6 (Torch.cpp:321-332);
+ * *steps_out / *quit_out (may be NULL) receive the steps run and whether the
+ * loop quit that way.
  * Returns 0 or an error code (1 NaN flux, 2 fixSolution failure, 3 checkValues). */
 int orc_run_hydro(const orc_hydro_params* p, double* U, int nsteps, double tmax,
-                  int ncheckpoints, double* dts, double* time_out);
+                  int ncheckpoints, double* dts, double* time_out, int debug, int* steps_out, int* quit_out);
 /* The same from the start time t0 (the time of an initial-conditions
  * snapshot, Torch.cpp:58), with the snapshots Torch::run prints
  * (Torch.cpp:254,270-281,295-297): up to max_prints states Q (7n each) with
  * their checkpoint count and time; *nprint receives how many. */
 int orc_run_hydro_print(const orc_hydro_params* p, double* U, int nsteps, double t0, double tmax,
                         int ncheckpoints, double* dts, double* time_out, double* qprint, int max_prints,
-                        int* pcount, double* ptime, int* nprint);
+                        int* pcount, double* ptime, int* nprint, int debug, int* steps_out, int* quit_out);
 
 /* ---- units (reference Converter.cpp / Constants.cpp) ------------------- */
 typedef struct { double M, L, T; } orc_converter;

@@ -380,6 +380,93 @@ def mintemp():
                         dts_off=res[0][1])
 
 
+DEBUG_STOP = dict(n=16, nsteps=14)
+
+
+def converging_q(n: int) -> np.ndarray:
+    """A converging flow on the unit cube: rho = 1, p = 0.1, v = -2 (x - 1/2):
+    the gas heats up in the centre, so the CFL dt rises and then falls."""
+    x = (np.arange(n) + 0.5) / n - 0.5
+    z, y, xx = np.meshgrid(x, x, x, indexing="ij")
+    Q = np.zeros((7, n, n, n))
+    Q[0] = 1.0
+    Q[1] = 0.1
+    Q[4], Q[5], Q[6] = -2.0 * xx, -2.0 * y, -2.0 * z
+    return Q
+
+
+def debug_stop():
+    """Torch::calculateTimeStep's debug test (Torch.cpp:321-332: the loop ends
+    after the step whose hydro dt has 100 dt / tmax <= 1e-6; Integration.debug
+    defaults to true, Parameters.hpp:50) on a 16^3 converging flow:
+    - "early": simulation_time 1e6, the stop after step 2 (the first CFL step);
+    - "late": simulation_time set between 1e8 times the dt of a later step and
+      that of every step before it, so the stop comes after that step;
+    - "nodebug": the early case with debug = 0, which runs all its steps;
+    - "run": simulation_time 1e3 with debug on, which never stops.
+    U after the last step run, every dt, the time and the quitting flag."""
+    cs = DEBUG_STOP
+    Q = converging_q(cs["n"])
+    res = {"Q": Q, "tfloor": 1e-12, "nsteps": cs["nsteps"]}
+
+    def run(tmax, debug, dumps=()):
+        return run_hydro_case(Q, cs["nsteps"], list(dumps), tfloor=1e-12,
+                              extra={"simulation_time": repr(float(tmax)), "debug": debug})
+
+    # the dts of a run without the test find a step k > 2 whose dt is below
+    # every earlier CFL dt
+    _, dts0, _ = run(1e6, 0)
+    k = next(i for i in range(3, len(dts0)) if dts0[i] < dts0[1:i].min())
+    cases = {"early": (1e6, 1), "late": (float(1e8 * 0.5 * (dts0[k] + dts0[1:k].min())), 1), "nodebug": (1e6, 0),
+             "run": (1e3, 1)}
+    for name, (tmax, debug) in cases.items():
+        _, dts, info = run(tmax, debug)
+        steps = info["steps"]
+        out, dts2, _ = run(tmax, debug, [steps])
+        assert np.array_equal(dts, dts2)
+        res[f"{name}_tmax"] = tmax
+        res[f"{name}_debug"] = debug
+        res[f"{name}_dts"] = dts
+        res[f"{name}_time"] = info["time"]
+        res[f"{name}_steps"] = steps
+        res[f"{name}_quitting"] = info["quitting"]
+        res[f"{name}_U"] = out[steps]
+        print("debug_stop", name, tmax, info)
+    assert res["early_steps"] == 2 and res["early_quitting"] == 1
+    assert res["late_steps"] == k + 1 and res["late_quitting"] == 1
+    assert res["nodebug_steps"] == cs["nsteps"] and res["nodebug_quitting"] == 0
+    assert res["run_steps"] == cs["nsteps"] and res["run_quitting"] == 0
+    np.savez_compressed(f"{HERE}/debug_stop.npz", **res)
+
+
+CLOUD_SPLIT = dict(n=16, nsteps=6, seed=None)
+
+
+def cloud_split():
+    """Torch::fullStep with hydrodynamics and thermodynamics, star off, on a
+    16^3 version of the operator-split cloud (torch_amd.cloud.setup in cgs:
+    side 0.5 pc, n(R) = n0 / (1 + R^2 / rc^2) about a centre 0.35 pc from
+    the grid centre, T = 300 K, HII fraction 1 within 0.1 pc of the centre,
+    ADV = 1), scales 1e-20 / 1e-28 / 1e11, temperature_floor 0.1, the star
+    cell (the CausalWind cell the reference keeps with the star off) at the
+    grid centre.  With the star off the reference's cooling sees no FUV flux
+    and zero column density.  U after every step and every dt."""
+    from torch_amd import cloud
+
+    cs = CLOUD_SPLIT
+    n = cs["n"]
+    Q, _, _, _, side = cloud.setup(n, lambda v, m, l, t: v)
+    c = n // 2
+    extra = {"density_scale": 1e-20, "pressure_scale": 1e-28, "time_scale": 1e11, "side_length": repr(side),
+             "temperature_floor": 0.1, "simulation_time": 1e13, "ncheckpoints": 10, "cooling_on": 1,
+             "thermo_hii_switch": 1e-2, "star_x": c, "star_y": c, "star_z": c}
+    dumps = list(range(1, cs["nsteps"] + 1))
+    out, dts, info = run_hydro_case(Q, cs["nsteps"], dumps, tfloor=0.1, extra=extra)
+    np.savez_compressed(f"{HERE}/cloud_split.npz", Q=Q, side=side, dts=dts, time=info["time"],
+                        **{f"U_{k}": out[k] for k in dumps})
+    print("cloud_split", info, dts)
+
+
 if __name__ == "__main__":
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle ref")

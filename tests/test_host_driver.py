@@ -59,6 +59,7 @@ def test_lua_parameters_parsed(tmp_path):
     assert d["bc"] == ["reflecting", "outflow", "periodic", "periodic", "free", "inflow"]
     assert d["gamma"] == 5.0 / 3.0 and d["mass_fraction_h"] == 0.7 and d["thermo_hii_switch"] == 0.5
     assert d["scales"] == [1e-20, 1e-28, 1e11] and d["ncheckpoints"] == 4 and d["cooling_on"] is False
+    assert d["debug"] is True  # TorchParameters default (Parameters.hpp:50)
     assert d["side_length"] == 3.0857e18 and d["tmax"] == 2.0e12 and d["tfloor"] == 10.0
     # after TorchParameters::initialise: L = sqrt(p/rho) t = 1e7 cm, T = 1e11 s,
     # M = L^3 rho = 1e1 g (Converter.cpp:22-24); dt_max = tmax / 10, pfloor =
@@ -469,3 +470,41 @@ def test_driver_lua_setupfile_run(tmp_path):
         outs.append(np.fromfile(out))
     assert np.isfinite(outs[0]).all()
     np.testing.assert_array_equal(outs[0], outs[1])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("host_loop", [False, True])
+@pytest.mark.parametrize("debug", [None, False])
+def test_driver_debug_stop(tmp_path, orc, host_loop, debug):
+    """Integration.debug through the driver: a 16^3 Sod run with
+    simulation_time 1e6 has 100 dt_hydro / tmax below 1e-6 at its first CFL
+    step, so Torch::run ends after step 2 (Torch.cpp:321-332) -- both the
+    device loop and the host-driven fullStep; with debug = false it runs all
+    its steps.  The state equals the oracle's run with the same rule."""
+    _exe_or_skip()
+    from torch_amd import setups
+
+    cfg = open(os.path.join(ROOT, "torch_amd", "configs", "sod64.lua")).read()
+    for k in ("x", "y", "z"):
+        cfg = cfg.replace(f"no_cells_{k} = 64", f"no_cells_{k} = 16")
+    cfg = cfg.replace("simulation_time = 1.0e3", "simulation_time = 1.0e6")
+    if debug is not None:
+        cfg = cfg.replace("ncheckpoints =    1,", f"ncheckpoints =    1, debug = {str(debug).lower()},")
+    f = tmp_path / "sod16.lua"
+    f.write_text(cfg)
+    out = tmp_path / "U.bin"
+    cmd = [EXE, f"--paramfile={f}", "--setup=sod", "--steps=6", f"--output={out}"]
+    if host_loop:
+        cmd.append("--host-loop")
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    stop = debug is None
+    assert info["steps"] == (2 if stop else 6) and info["quitting"] is stop
+    assert ("Integration deltas are too small." in r.stderr) is stop
+    Q = setups.sod(16)
+    p = orc.make_params((16, 16, 16), 1.0 / 16, dt_max=1e5, nthreads=8)
+    U0 = orc.UfromQ_grid(orc.fix_primitives(p, Q, np.zeros_like(Q)), p.gamma)
+    Uo, dts, t, quit_ = orc.run_hydro(p, U0, 6, tmax=1e6, debug=stop, info=True)
+    assert quit_ is stop and len(dts) == info["steps"]
+    np.testing.assert_array_equal(np.fromfile(out, dtype=np.float64).reshape(7, 16, 16, 16), Uo)

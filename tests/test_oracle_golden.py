@@ -121,3 +121,39 @@ def test_thermo_cells_match_reference():
     # gate: ADV below thermo_hii_switch -> RATE exactly 0 (Thermodynamics.cpp:227-230,275-280)
     gated = f["inputs"][:, 3] < 1e-2
     assert gated.sum() == 100 and (out[gated, 2] == 0).all()
+
+
+@pytest.mark.parametrize("case", ["early", "late", "nodebug", "run"])
+def test_debug_stop_bitwise(case):
+    """Torch::calculateTimeStep's debug test (Torch.cpp:321-332) against the
+    reference harness's loop: the step after which the loop quits, every
+    dt, the time and the state, bitwise; without debug, or with the hydro dt
+    above 1e-8 tmax, every step runs."""
+    f = _load("debug_stop.npz")
+    Q = f["Q"]
+    tfloor = float(f["tfloor"])
+    tmax = float(f[f"{case}_tmax"])
+    p = oracle.make_params(Q.shape[1:], 1.0 / Q.shape[3], tfloor=tfloor, pfloor=0.1 * 8.314462e7 * 1e-30 * tfloor,
+                           dt_max=tmax / 10.0, nthreads=4)
+    U, dts, t, quit_ = oracle.run_hydro(p, _u0(p, Q), int(f["nsteps"]), tmax=tmax,
+                                        debug=bool(f[f"{case}_debug"]), info=True)
+    assert len(dts) == int(f[f"{case}_steps"])
+    assert quit_ == bool(f[f"{case}_quitting"])
+    np.testing.assert_array_equal(dts, f[f"{case}_dts"])
+    assert t == float(f[f"{case}_time"])
+    np.testing.assert_array_equal(U, f[f"{case}_U"])
+
+
+def test_debug_stop_planted_mistake():
+    """A stop that tested dt rather than 100 dt / tmax, or tested the
+    clipped dt, would quit in the 'run' case as well; one that missed the
+    test would run on in 'early'.  Both are told apart by the fixture: the
+    early case quits after exactly two steps while 'run' (tmax 1e3) does
+    not."""
+    f = _load("debug_stop.npz")
+    assert int(f["early_steps"]) == 2 and int(f["run_steps"]) == int(f["nsteps"])
+    # the threshold sits between the late case's stopping dt and every
+    # earlier CFL dt of its run
+    late = f["late_dts"]
+    thr = 1e-8 * float(f["late_tmax"])
+    assert late[-1] <= thr and (late[1:-1] > thr).all()

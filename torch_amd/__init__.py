@@ -41,7 +41,7 @@ class Params(ctypes.Structure):
         ("spatial_order", ctypes.c_int),
         ("bc_left", ctypes.c_int * 3), ("bc_right", ctypes.c_int * 3),
         ("riemann_solver", ctypes.c_int), ("slope_limiter", ctypes.c_int),
-        ("precision", ctypes.c_int), ("ncheckpoints", ctypes.c_int),
+        ("precision", ctypes.c_int), ("ncheckpoints", ctypes.c_int), ("debug", ctypes.c_int),
         ("side_length", ctypes.c_double), ("gamma", ctypes.c_double),
         ("dfloor", ctypes.c_double), ("pfloor", ctypes.c_double), ("tfloor", ctypes.c_double),
         ("mass_fraction_h", ctypes.c_double), ("spec_gas_const", ctypes.c_double),
@@ -83,8 +83,15 @@ def lib() -> ctypes.CDLL:
         L.torchgpu_reset_clock.argtypes = [vp]
         L.torchgpu_synchronize.argtypes = [vp]
         L.torchgpu_get_clock.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(i)]
+        L.torchgpu_is_quitting.argtypes = [vp, ctypes.POINTER(i)]
         _lib = L
     return _lib
+
+
+def _current_device() -> int:
+    import torch
+
+    return torch.cuda.current_device()
 
 
 def default_stream(stream: int | None = None) -> int:
@@ -102,8 +109,11 @@ def default_stream(stream: int | None = None) -> int:
 def make_params(shape, side_length=1.0, gamma=1.67, dfloor=1e-30, pfloor=0.0, tfloor=0.0,
                 mass_fraction_h=1.0, spec_gas_const=8.314462e7, tmax=1e3, dt_max=None,
                 ncheckpoints=1, solver="RotatedHLLC", limiter="albada", spatial_order=1,
-                bc="outflow", precision=FP64) -> Params:
-    """Parameters in code units.  shape = (nz, ny, nx)."""
+                bc="outflow", precision=FP64, debug=True) -> Params:
+    """Parameters in code units.  shape = (nz, ny, nx).  `debug` is
+    Integration.debug, true by default as in TorchParameters
+    (Parameters.hpp:50): the time loop stops after a step whose hydro dt is
+    at most 1e-8 tmax (Torch.cpp:321-332)."""
     p = Params()
     p.nz, p.ny, p.nx = shape
     p.spatial_order = spatial_order
@@ -115,6 +125,7 @@ def make_params(shape, side_length=1.0, gamma=1.67, dfloor=1e-30, pfloor=0.0, tf
     p.slope_limiter = LIMITER[limiter] if isinstance(limiter, str) else limiter
     p.precision = precision
     p.ncheckpoints = ncheckpoints
+    p.debug = 1 if debug else 0
     p.side_length, p.gamma = side_length, gamma
     p.dfloor, p.pfloor, p.tfloor = dfloor, pfloor, tfloor
     p.mass_fraction_h, p.spec_gas_const = mass_fraction_h, spec_gas_const
@@ -242,6 +253,13 @@ class Grid:
         t, dt, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
         self._check(lib().torchgpu_get_clock(self._h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n)))
         return t.value, dt.value, n.value
+
+    def quitting(self) -> bool:
+        """Torch::isQuitting: the debug test of Torch::calculateTimeStep
+        ended the time loop (Torch.cpp:321-332)."""
+        q = ctypes.c_int()
+        self._check(lib().torchgpu_is_quitting(self._h, ctypes.byref(q)))
+        return bool(q.value)
 
 
 # -- cooling operator -------------------------------------------------------
@@ -381,7 +399,8 @@ def run_printing(grid: "Grid", nsteps: int, outdir: str, scales=(1.0, 1.0, 1.0),
     """Torch::run with its snapshots (Torch.cpp:240-302): prints
     data2D_<count>.txt.gz (6-digit count) into `outdir` for the initial
     state, at every checkpoint (the run pauses on the device there) and at
-    the end when tmax is reached; at most `nsteps` steps (hydro only, or
+    the end when the loop ends (tmax reached, or the debug stop of
+    Torch::calculateTimeStep); at most `nsteps` steps (hydro only, or
     operator-split with `cooling` and its `thermo` keyword arguments and/or
     `radiation`).
     Returns the list of files written."""
@@ -418,7 +437,9 @@ def run_printing(grid: "Grid", nsteps: int, outdir: str, scales=(1.0, 1.0, 1.0),
                 break
             emit(count)
             final_print = count != p.ncheckpoints
-        if final_print and not grid.clock()[0] < p.tmax:
+        # Torch::run's last print once its loop has ended: tmax reached or
+        # quitting (Torch.cpp:266,295-297)
+        if final_print and (not grid.clock()[0] < p.tmax or grid.quitting()):
             emit(p.ncheckpoints)
     finally:
         grid.pause_at_checkpoints(False)
@@ -474,30 +495,45 @@ class Cooling:
             pass
 
     @staticmethod
-    def _dptr(a):
-        return ctypes.c_void_p(0 if a is None else a.data_ptr())
+    def _dptr(a, grid: "Grid"):
+        """Device address of an optional per-cell array of `grid` (T_min,
+        COL_DEN, F_FUV): a contiguous float64 CUDA tensor of one value per
+        cell, or None.  Anything else is refused here, before a kernel would
+        read it out of bounds."""
+        if a is None:
+            return ctypes.c_void_p(0)
+        p = grid.params
+        n = p.nx * p.ny * p.nz
+        if not hasattr(a, "data_ptr") or not getattr(a, "is_cuda", False):
+            raise ValueError("per-cell thermo arrays must be CUDA tensors")
+        if str(a.dtype) != "torch.float64" or not a.is_contiguous() or a.numel() != n:
+            raise ValueError(f"per-cell thermo arrays must be contiguous float64 with {n} values "
+                             f"(got {a.dtype}, {tuple(a.shape)}, contiguous={a.is_contiguous()})")
+        if a.device.index != _current_device():
+            raise ValueError(f"per-cell thermo array on {a.device}, the grid is on cuda:{_current_device()}")
+        return ctypes.c_void_p(a.data_ptr())
 
     def grid_timestep(self, grid: "Grid", dt_max: float, T_min=None, col_den=None, F_FUV=None,
                       excluded: int = -1) -> float:
         """Thermodynamics::calculateTimeStep on the grid state (device
         arrays [n] or None for T_min, COL_DEN, F_FUV)."""
         dt = ctypes.c_double()
-        self._check(lib().torchgpu_thermo_grid_timestep(grid._h, self._h, self._dptr(T_min), self._dptr(col_den),
-                                                        self._dptr(F_FUV), excluded, dt_max, ctypes.byref(dt)))
+        self._check(lib().torchgpu_thermo_grid_timestep(grid._h, self._h, self._dptr(T_min, grid), self._dptr(col_den, grid),
+                                                        self._dptr(F_FUV, grid), excluded, dt_max, ctypes.byref(dt)))
         return dt.value
 
     def grid_substep(self, grid: "Grid", dt: float, T_min=None, col_den=None, F_FUV=None, excluded: int = -1,
                      surrogate: bool = False):
         """The thermo sub-step of Torch::subStep on the grid state."""
-        self._check(lib().torchgpu_thermo_grid_substep(grid._h, self._h, self._dptr(T_min), self._dptr(col_den),
-                                                       self._dptr(F_FUV), excluded, dt, int(surrogate)))
+        self._check(lib().torchgpu_thermo_grid_substep(grid._h, self._h, self._dptr(T_min, grid), self._dptr(col_den, grid),
+                                                       self._dptr(F_FUV, grid), excluded, dt, int(surrogate)))
 
     def run_split(self, grid: "Grid", nsteps: int, T_min=None, col_den=None, F_FUV=None, excluded: int = -1,
                   surrogate: bool = False):
         """`nsteps` Torch::fullStep iterations of hydro + thermo with dt and
         the Strang alternation on the device (no host round trip)."""
-        self._check(lib().torchgpu_run_split(grid._h, self._h, self._dptr(T_min), self._dptr(col_den),
-                                             self._dptr(F_FUV), excluded, nsteps, int(surrogate)))
+        self._check(lib().torchgpu_run_split(grid._h, self._h, self._dptr(T_min, grid), self._dptr(col_den, grid),
+                                             self._dptr(F_FUV, grid), excluded, nsteps, int(surrogate)))
 
     def set_min_temp(self, T_min=None):
         """Thermodynamics::initialiseMinTempField: the per-cell T_min (host
@@ -507,6 +543,10 @@ class Cooling:
             self._check(lib().torchgpu_cooling_set_min_temp(self._h, None, 0, 0))
             return
         if hasattr(T_min, "data_ptr"):
+            if str(T_min.dtype) != "torch.float64" or not T_min.is_contiguous():
+                raise ValueError(f"T_min must be a contiguous float64 tensor (got {T_min.dtype})")
+            if T_min.is_cuda and T_min.device.index != _current_device():
+                raise ValueError(f"T_min on {T_min.device}, the handle is on cuda:{_current_device()}")
             self._check(lib().torchgpu_cooling_set_min_temp(self._h, ctypes.c_void_p(T_min.data_ptr()),
                                                             T_min.numel(), int(T_min.is_cuda)))
         else:
@@ -547,8 +587,8 @@ class Cooling:
         n = grid.params.nx * grid.params.ny * grid.params.nz
         in9 = np.zeros((9, n))
         dcol = np.zeros(n)
-        self._check(lib().torchgpu_thermo_grid_inputs(grid._h, self._h, self._dptr(T_min), self._dptr(col_den),
-                                                      self._dptr(F_FUV), excluded, in9.ctypes.data, dcol.ctypes.data,
+        self._check(lib().torchgpu_thermo_grid_inputs(grid._h, self._h, self._dptr(T_min, grid), self._dptr(col_den, grid),
+                                                      self._dptr(F_FUV, grid), excluded, in9.ctypes.data, dcol.ctypes.data,
                                                       0))
         return in9, dcol
 
@@ -557,9 +597,14 @@ class Cooling:
         the total number of subcycles (if count)."""
         import torch
 
+        if not inp.is_cuda or inp.dtype != torch.float64 or not inp.is_contiguous() or inp.dim() != 2 \
+                or inp.shape[0] != 9:
+            raise ValueError("inp must be a contiguous float64 CUDA tensor [9][n]")
         n = inp.shape[1]
         if out is None:
             out = torch.empty((5, n), dtype=torch.float64, device=inp.device)
+        elif not out.is_cuda or out.dtype != torch.float64 or not out.is_contiguous() or tuple(out.shape) != (5, n):
+            raise ValueError(f"out must be a contiguous float64 CUDA tensor [5][{n}]")
         tot = ctypes.c_int64()
         self._check(lib().torchgpu_thermo_cells(self._h, n, ctypes.c_void_p(inp.data_ptr()),
                                                 ctypes.c_void_p(out.data_ptr()),
@@ -677,7 +722,10 @@ class Radiation:
         """`nsteps` Torch::fullStep iterations of hydro + (thermo) + radiation
         with dt, the pre-step calculations and the Strang alternation on the
         device (torchgpu_run_full, no host round trip)."""
-        dptr = cooling._dptr if cooling is not None else (lambda a: None)
+        if cooling is None:
+            dptr = lambda a: None  # noqa: E731
+        else:
+            dptr = lambda a: cooling._dptr(a, self.grid)  # noqa: E731
         self._check(lib().torchgpu_run_full(self.grid._h, None if cooling is None else cooling._h, self._h,
                                             dptr(T_min), dptr(col_den), dptr(F_FUV), excluded, nsteps,
                                             int(surrogate)))

@@ -103,6 +103,7 @@ int reset_clock(torchgpu_ctx* c) {
     st.dt_max = c->p.dt_max;
     st.tmax = c->p.tmax;
     st.first = 1;
+    st.debug = c->p.debug ? 1 : 0;
     st.ncheckpoints = c->p.ncheckpoints > 0 ? c->p.ncheckpoints : 1;
     c->step_counter = 0;
     c->counter_step = 0;
@@ -443,6 +444,15 @@ int torchgpu_get_clock(torchgpu_ctx* c, double* time, double* last_dt, int* step
     if (time) *time = st.time;
     if (last_dt) *last_dt = st.dt;
     if (steps) *steps = st.step;
+    return TORCHGPU_OK;
+}
+
+int torchgpu_is_quitting(torchgpu_ctx* c, int* quitting) {
+    if (!c || !quitting) return TORCHGPU_ERR_INVALID_ARG;
+    tg::StepState st;
+    HIP_TRY(c, hipMemcpyAsync(&st, c->st, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    *quitting = st.quitting;
     return TORCHGPU_OK;
 }
 

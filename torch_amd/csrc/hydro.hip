@@ -281,8 +281,9 @@ __global__ void step_begin_kernel(StepState* st, int use_cfl, double dt_override
     }
     if (st->halted) return;
     // a step that raised an error ends the run as the reference's exception
-    // does; the error is reported by the next synchronising call
-    if (!(st->time < st->tmax) || st->err) {
+    // does; the error is reported by the next synchronising call; the loop
+    // condition of Torch::run (Torch.cpp:266) also ends it once quitting
+    if (!(st->time < st->tmax) || st->err || st->quitting) {
         st->halted = 1;
         return;
     }
@@ -307,6 +308,10 @@ __global__ void step_begin_kernel(StepState* st, int use_cfl, double dt_override
         double tmin = st->dt_max;
         if (maxinv > 0.0) tmin = smin(tmin, 0.5 / maxinv);
         dt = smin(tmin, st->dt_max);
+        // Torch::calculateTimeStep's debug test (Torch.cpp:321-332): its
+        // condition reads the hydro percentage only; the step still runs and
+        // Torch::run's loop ends after it
+        if (st->debug && 100.0 * dt / st->tmax <= 1.0e-6) st->quitting = 1;
         if (thermo_dt_bits) {
             // Thermodynamics::calculateTimeStep: dt_max lowered by every
             // smaller |f E/RATE| (Thermodynamics.cpp:465-474)
@@ -363,9 +368,20 @@ __global__ void u_from_q_kernel(HydroGeom g, double* __restrict__ Q, double* __r
 // Thermodynamics::updateSourceTerms + Fluid::advSolution + fixSolution on
 // the grid: UDOT[PRE] = rate (other UDOT zero), U += dtf * UDOT, then
 // fixSolution (Thermodynamics.cpp:478-485, Fluid.cpp:80-136).
+// cerr: the cooling operator's error word; a subcycle count beyond the int
+// range raised by the integrate that produced `rate` is moved into the step
+// state here (reported at this step, like the fixSolution errors) and the
+// word cleared.
 __global__ void apply_rate_fix_kernel(HydroGeom g, double* __restrict__ U, const double* __restrict__ rate, double dtf,
-                                      StepState* st) {
+                                      StepState* st, unsigned int* cerr) {
     if (dtf < 0.0 && st->halted) return;
+    if (cerr && blockIdx.x == 0 && threadIdx.x == 0) {
+        const unsigned int e = *cerr;
+        if (e & ERR_THERMO_NSTEPS) {
+            set_error(st, ERR_THERMO_NSTEPS);
+            *cerr = 0u;
+        }
+    }
     dtf = stage_dtf(dtf, st, 0);
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < g.ncell; c += (long long)gridDim.x * blockDim.x) {
         double u[7];
@@ -409,10 +425,10 @@ hipError_t launch_halted_copy(const HydroGeom& g, const StepState* st, const dou
 }
 
 hipError_t launch_fix_solution(const HydroGeom& g, double* U, const double* rate, double dtf, StepState* st,
-                               hipStream_t stream) {
+                               hipStream_t stream, unsigned int* cerr) {
     long long nb = (g.ncell + 255) / 256;
     if (nb > 65536) nb = 65536;
-    hipLaunchKernelGGL(apply_rate_fix_kernel, dim3((unsigned)nb), dim3(256), 0, stream, g, U, rate, dtf, st);
+    hipLaunchKernelGGL(apply_rate_fix_kernel, dim3((unsigned)nb), dim3(256), 0, stream, g, U, rate, dtf, st, cerr);
     return hipGetLastError();
 }
 

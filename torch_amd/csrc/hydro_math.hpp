@@ -5,7 +5,7 @@
 // For R = double every expression keeps the operand order of the reference
 // (hgsteggles/TORCH) so that, compiled with -ffp-contract=off, results are
 // bitwise equal to its x86-64 FP64 build:
-//   slope limiters         src/Integrators/SlopeLimiter.cpp:263-311
+//   slope limiters         src/Integrators/SlopeLimiter.cpp:7-54
 //   UfromQ/QfromU/FfromQ   src/Fluid/GridCell.cpp:163-213
 //   wave speeds            src/Integrators/Riemann.cpp:27-42
 //   HLLC / HLL             src/Integrators/Riemann.cpp:52-104, 108-140

@@ -749,10 +749,16 @@ int ensure_queues(torchgpu_cooling* c, long long n) {
     return TORCHGPU_OK;
 }
 
+// Reports (and clears) the error word the cooling kernels raise.
 int check_err(torchgpu_cooling* c) {
     unsigned int e = 0;
     HIP_OK(hipMemcpyAsync(&e, c->err, sizeof e, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
+    if (e) {
+        static const unsigned int zero = 0;
+        HIP_OK(hipMemcpyAsync(c->err, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+    }
     if (e & tg::ERR_THERMO_NSTEPS) {
         snprintf(c->msg, sizeof c->msg, "Thermodynamics::integrate: subcycle count out of int range");
         return TORCHGPU_ERR_NSTEPS_RANGE;
@@ -892,19 +898,28 @@ int torchgpu_surrogate_training_data(torchgpu_cooling* c, int64_t n, const doubl
     // untouched
     double* scratch = nullptr;
     HIP_OK(hipMalloc(&scratch, 5 * n * sizeof(double)));
+    // every path below frees scratch: the first HIP error is kept and
+    // reported after the free
+    hipError_t e = hipSuccess;
+    auto keep = [&e](hipError_t r) {
+        if (e == hipSuccess) e = r;
+    };
     long long nb = (n + 255) / 256;
     if (nb > (long long)c->nsm * 16) nb = (long long)c->nsm * 16;
     hipLaunchKernelGGL(tg::surrogate_prepass_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t, c->tables, sc,
                        (long long)n, in, scratch, (unsigned char*)nullptr, (long long*)list, feat, c->sdata, raw,
                        c->fb_list, c->counters, c->err);
-    HIP_OK(hipGetLastError());
-    hipLaunchKernelGGL(tg::surrogate_targets_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, c->counters,
-                       (long long)n, (const long long*)list, c->sdata, out, target);
-    HIP_OK(hipGetLastError());
-    unsigned long long h[4];
-    HIP_OK(hipMemcpyAsync(h, c->counters, sizeof h, hipMemcpyDeviceToHost, c->stream));
-    HIP_OK(hipStreamSynchronize(c->stream));
-    (void)hipFree(scratch);
+    keep(hipGetLastError());
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(tg::surrogate_targets_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, c->counters,
+                           (long long)n, (const long long*)list, c->sdata, out, target);
+        keep(hipGetLastError());
+    }
+    unsigned long long h[4] = {0, 0, 0, 0};
+    if (e == hipSuccess) keep(hipMemcpyAsync(h, c->counters, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    keep(hipStreamSynchronize(c->stream));
+    keep(hipFree(scratch));
+    HIP_OK(e);
     *count = (int64_t)h[2];
     return check_err(c);
 }
@@ -978,7 +993,7 @@ static int grid_substep_dev(torchgpu_ctx* ctx, torchgpu_cooling* c, const double
     else rc = torchgpu_thermo_cells(c, n, c->in9, c->out5, nullptr);
     if (rc) return rc;
     // Thermodynamics::updateSourceTerms + advSolution(dt) + fixSolution
-    HIP_OK(tg::launch_fix_solution(*v.g, v.U, c->out5 + 2 * n, dt, v.st, c->stream));
+    HIP_OK(tg::launch_fix_solution(*v.g, v.U, c->out5 + 2 * n, dt, v.st, c->stream, c->err));
     tg::ctx_state_changed(ctx);
     return TORCHGPU_OK;
 }

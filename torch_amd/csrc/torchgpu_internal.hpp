@@ -68,7 +68,7 @@ struct StepState {
     unsigned int err;       // error bits (ERR_*) raised during step err_step
     int err_step1;          // 1 + step index of the first error (0: none)
     int step;               // steps completed
-    int pad;
+    int debug;              // Integration.debug: quit when the hydro dt is too small (Torch.cpp:321-332)
     // mixed path: float bit patterns of the largest density and pressure
     // the state scale is taken from, and of those the running predictor
     // records for the next one (hydro_mixed.hip)
@@ -79,6 +79,10 @@ struct StepState {
     // step (paused, halted) until the host has printed and resumes it
     int pause_ckpt;
     int paused;
+    // Torch::m_isQuitting: set by the step whose hydro time step had
+    // 100 dt_hydro / tmax <= 1e-6 with debug on; the steps after it halt
+    int quitting;
+    int pad;
 };
 
 // Negative dt arguments of the stage and cooling kernels take the dt of the
@@ -122,8 +126,10 @@ namespace tg {
 CtxView ctx_view(torchgpu_ctx* c);
 
 // Launchers (hydro.hip).  All enqueue on `stream` and return hipError_t.
+// cerr (may be null): a cooling operator's error word, whose subcycle-count
+// overflow is moved into st (and the word cleared).
 hipError_t launch_fix_solution(const HydroGeom& g, double* U, const double* dpre, double dtf, StepState* st,
-                               hipStream_t stream);
+                               hipStream_t stream, unsigned int* cerr = nullptr);
 // FP64 stage kernel (hydro.hip).  dtf < 0: the stage takes its dt from st
 // (dt/2 predictor, dt corrector).
 hipError_t launch_hydro_stage(const HydroGeom& g, int solver, int stage, const double* Usrc, const double* Ubase,

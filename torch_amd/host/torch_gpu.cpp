@@ -41,7 +41,7 @@ int riemannSolverId(const std::string& type) {  // RiemannSolverFactory::create,
     throw std::runtime_error("RiemannSolverFactory::create: unknown type: " + type + "\n");
 }
 
-int slopeLimiterId(const std::string& type) {  // SlopeLimiterFactory::create, SlopeLimiter.cpp:313-332
+int slopeLimiterId(const std::string& type) {  // SlopeLimiterFactory::create, SlopeLimiter.cpp:56-75
     if (type == "monotonised_central") return TORCHGPU_LIM_MC;
     if (type == "superbee") return TORCHGPU_LIM_SUPERBEE;
     if (type == "minmod") return TORCHGPU_LIM_MINMOD;
@@ -96,6 +96,7 @@ torchgpu_params TorchParameters::gridParams() const {
     try { g.slope_limiter = slopeLimiterId(slopeLimiter); } catch (std::exception&) { g.slope_limiter = TORCHGPU_LIM_ALBADA; }
     g.precision = (precision == "mixed") ? TORCHGPU_MIXED : TORCHGPU_FP64;
     g.ncheckpoints = ncheckpoints;
+    g.debug = debug ? 1 : 0;
     g.side_length = sideLength;
     g.gamma = heatCapacityRatio;
     g.dfloor = dfloor;
@@ -487,6 +488,7 @@ void Torch::initialise(const TorchParameters& p_in, void* stream) {
     currentTime = 0;
     steps = 0;
     first_time_ = true;
+    quitting_ = false;
     stepCounter_ = 0;
     if (!ic.empty()) {
         // DataReader::readGrid, then the tail of Torch::initialise
@@ -536,6 +538,15 @@ double Torch::calculateTimeStep() {  // Torch.cpp:304-339 (single process: the M
         if (params.radiation_on) dt_rad = radiation.calculateTimeStep(params.dt_max, fluid);
         if (params.cooling_on) dt_thermo = thermodynamics.calculateTimeStep(params.dt_max, fluid);
         dt = std::min(std::min(dt_hydro, dt_rad), dt_thermo);
+        if (params.debug) {
+            // the reference's test reads the hydro percentage in all three
+            // terms of its condition (Torch.cpp:321-332)
+            const double thyd = 100.0 * dt_hydro / params.tmax;
+            if (thyd <= 1.0e-6) {
+                std::fprintf(stderr, "Integration deltas are too small.\n");
+                quitting_ = true;
+            }
+        }
     }
     return dt;
 }
@@ -645,7 +656,12 @@ void Torch::run(int max_steps, bool print, bool host_loop) {  // Torch.cpp:240-3
             final_print = count != params.ncheckpoints;
         }
         check_rc(torchgpu_get_clock(c, &t, &dt, &n1), "clock", c);
-        if (final_print && !(t < params.tmax)) print2D(params.ncheckpoints);
+        int quit = 0;
+        check_rc(torchgpu_is_quitting(c, &quit), "quitting", c);
+        quitting_ = quit != 0;
+        if (quitting_) std::fprintf(stderr, "Integration deltas are too small.\n");
+        // the print after Torch::run's loop (Torch.cpp:295-297)
+        if (final_print && (!(t < params.tmax) || quitting_)) print2D(params.ncheckpoints);
         check_rc(torchgpu_pause_at_checkpoints(c, 0), "pause at checkpoints", c);
         currentTime = t;
         steps = n1;
@@ -657,6 +673,10 @@ void Torch::run(int max_steps, bool print, bool host_loop) {  // Torch.cpp:240-3
         int n = 0;
         double t = 0, dt = 0;
         check_rc(torchgpu_get_clock(c, &t, &dt, &n), "clock", c);
+        int quit = 0;
+        check_rc(torchgpu_is_quitting(c, &quit), "quitting", c);
+        quitting_ = quit != 0;
+        if (quitting_) std::fprintf(stderr, "Integration deltas are too small.\n");
         currentTime = t;
         steps = n;
         return;
@@ -670,7 +690,7 @@ void Torch::run(int max_steps, bool print, bool host_loop) {  // Torch.cpp:240-3
         return ((count + 1) * delta - pct) * params.tmax / 100.0;
     };
     checkpoint(currentTime);
-    while (currentTime < params.tmax && steps < max_steps) {
+    while (currentTime < params.tmax && !quitting_ && steps < max_steps) {
         double dt_next = checkpoint(currentTime);
         double dt = fullStep(dt_next);
         currentTime += dt;

@@ -237,9 +237,11 @@ public:
     TorchParameters params;
     double currentTime = 0;
     int steps = 0;
+    bool isQuitting() const { return quitting_; }  // Torch::isQuitting (Torch.hpp:47)
 
 private:
     bool first_time_ = true;
+    bool quitting_ = false;  // Torch::m_isQuitting
     int stepCounter_ = 0;
 };
 

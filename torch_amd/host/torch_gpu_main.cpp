@@ -50,8 +50,8 @@ void dump(const tg::TorchParameters& p) {
     std::printf(" \"bc\": [\"%s\", \"%s\", \"%s\", \"%s\", \"%s\", \"%s\"], \"cooling_on\": %s, \"ncheckpoints\": %d,\n",
                 p.leftBC[0].c_str(), p.rightBC[0].c_str(), p.leftBC[1].c_str(), p.rightBC[1].c_str(),
                 p.leftBC[2].c_str(), p.rightBC[2].c_str(), p.cooling_on ? "true" : "false", p.ncheckpoints);
-    std::printf(" \"mass_fraction_h\": %.17g, \"thermo_hii_switch\": %.17g, \"precision\": \"%s\",\n", p.massFractionH,
-                p.thermoHII_Switch, p.precision.c_str());
+    std::printf(" \"mass_fraction_h\": %.17g, \"thermo_hii_switch\": %.17g, \"precision\": \"%s\", \"debug\": %s,\n",
+                p.massFractionH, p.thermoHII_Switch, p.precision.c_str(), p.debug ? "true" : "false");
     std::printf(" \"star_on\": %s, \"star_cell\": [%d, %d, %d], \"wind_radius\": %d, \"photon_rate\": %.17g, "
                 "\"photon_energy\": %.17g, \"radiation_on\": %s,\n", p.star_on ? "true" : "false", p.star_position[0],
                 p.star_position[1], p.star_position[2], p.windCellRadius, p.photonRate, p.photonEnergy,
@@ -131,8 +131,9 @@ int main(int argc, char** argv) {
         torch.run(steps, arg(argc, argv, "--snapshots", "") == "1", arg(argc, argv, "--host-loop", "") == "1");
         auto t1 = std::chrono::steady_clock::now();
         const double secs = std::chrono::duration<double>(t1 - t0).count();
-        std::printf("{\"steps\": %d, \"time\": %.17g, \"seconds\": %.6f, \"mcell_updates_per_s\": %.3f}\n", torch.steps,
-                    torch.currentTime, secs, (double)n * torch.steps / secs / 1e6);
+        std::printf("{\"steps\": %d, \"time\": %.17g, \"seconds\": %.6f, \"mcell_updates_per_s\": %.3f, "
+                    "\"quitting\": %s}\n", torch.steps, torch.currentTime, secs, (double)n * torch.steps / secs / 1e6,
+                    torch.isQuitting() ? "true" : "false");
         const std::string output = arg(argc, argv, "--output", "");
         if (!output.empty()) {
             std::vector<double> U(7 * n);

@@ -53,7 +53,8 @@ def main():
     Q = torch.from_numpy(setups.turbulent(n, a.sigma, a.mach, gamma=a.gamma, device="cuda")).cuda()
     print(json.dumps({"setup": f"{n}^3 turbulent box sigma_ln_rho={a.sigma} rms Mach {a.mach}",
                       "ic_seconds": round(time.perf_counter() - t0, 1)}), flush=True)
-    kw = dict(gamma=a.gamma, tfloor=a.tfloor, pfloor=pfloor)
+    # debug = false as the reference's shipped configuration (config/torch-config.lua)
+    kw = dict(gamma=a.gamma, tfloor=a.tfloor, pfloor=pfloor, debug=False)
     g64 = ta.Grid(ta.make_params((n, n, n), precision=ta.FP64, **kw))
     gmx = ta.Grid(ta.make_params((n, n, n), precision=ta.MIXED, **kw))
     g64.set_primitive(Q)
