@@ -289,6 +289,18 @@ int torchgpu_cooling_last_error(const torchgpu_cooling* c, char* buf, size_t len
  * total_substeps (host, may be NULL) receives the sum of subcycle counts
  * (this synchronizes). */
 int torchgpu_thermo_cells(torchgpu_cooling* c, int64_t n, const double* in, double* out, int64_t* total_substeps);
+/* torchgpu_thermo_cells without allocating device memory.  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * longest-first work order that torchgpu_thermo_cells keeps in buffers the
+ * handle grows lives here in one fixed device array of the library (9 MiB per
+ * GPU, held until the process ends), so n is limited to 1048576 cells (more:
+ * TORCHGPU_ERR_INVALID_ARG, nothing launched).  Results equal
+ * torchgpu_thermo_cells'.  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (other streams or threads) race.  The handle's device
+ * must be current. */
+int torchgpu_thermo_cells_noalloc(torchgpu_cooling* c, int64_t n, const double* in, double* out,
+                                  int64_t* total_substeps);
 
 /* Thermodynamics on the grid state of ctx (operator-split step).  Per-cell
  * inputs come from U (Q = fixPrimitives(QfromU(U)), E = U[PRE]); T_min,

@@ -33,6 +33,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 
 #include "../../include/torchgpu.h"
 #include "hydro_math.hpp"
@@ -637,7 +638,24 @@ struct torchgpu_cooling {
     long long dcol_cap = 0;
     double* xface = nullptr;   // the ray trace's x-face buffers
     long long xface_cap = 0;
+    int device = 0;            // the HIP device current at create
 };
+
+// ---------------------------------------------------------------------------
+// Fixed scratch of torchgpu_thermo_cells_noalloc: the longest-first work
+// order of at most NOALLOC_MAX_CELLS cells (histogram and cursors, per-cell
+// buckets, order), in one device array per GPU that lives until the process
+// ends.  Host code never passes `scratch` itself to a kernel (on the host the
+// name is a stand-in, not a device address): each device's address comes
+// from hipGetSymbolAddress, taken once per device at set-up.
+// ---------------------------------------------------------------------------
+constexpr long long NOALLOC_MAX_CELLS = 1LL << 20;
+constexpr size_t NOALLOC_HIST_BYTES = 2 * tg::LPT_NB * sizeof(unsigned long long);   // 1 KiB
+constexpr size_t NOALLOC_ORDER_BYTES = NOALLOC_MAX_CELLS * sizeof(long long);         // 8 MiB
+constexpr size_t NOALLOC_BYTES = NOALLOC_HIST_BYTES + NOALLOC_ORDER_BYTES + NOALLOC_MAX_CELLS;
+static_assert(NOALLOC_HIST_BYTES % 256 == 0 && NOALLOC_ORDER_BYTES % 256 == 0, "scratch sections stay aligned");
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
 
 // the MLP handle is defined in mlp_api.cpp; the cooling operator only needs
 // its device weights
@@ -694,7 +712,30 @@ int launch_fp64(torchgpu_cooling* c, long long stride, long long count, const un
 }
 
 // Work order (longest cells first) of `count` items (count_dev: on the
-// device) that are cells list[i] (list == NULL: i); returns c->lpt_order.
+// device) that are cells list[i] (list == NULL: i), into `order` [count],
+// with the per-item buckets in `bucket` [count] and the histogram and
+// cursors in `hist` [2 LPT_NB] (cleared here by a kernel).
+int lpt_order_into(torchgpu_cooling* c, long long stride, long long count, const unsigned long long* count_dev,
+                   const long long* list, const double* in, long long* order, unsigned char* bucket,
+                   unsigned long long* hist) {
+    hipLaunchKernelGGL(tg::zero_u64_kernel, dim3(1), dim3(2 * tg::LPT_NB), 0, c->stream, hist, 2 * tg::LPT_NB);
+    HIP_OK(hipGetLastError());
+    long long blocks = (count + 255) / 256;
+    if (blocks > (long long)c->nsm * 16) blocks = (long long)c->nsm * 16;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(tg::lpt_bucket_kernel, dim3((unsigned)blocks), dim3(256), 0, c->stream, c->t, c->tables,
+                       stride, count, count_dev, list, in, bucket, hist, c->err);
+    HIP_OK(hipGetLastError());
+    hipLaunchKernelGGL(tg::lpt_offsets_kernel, dim3(1), dim3(64), 0, c->stream, hist, hist + tg::LPT_NB);
+    HIP_OK(hipGetLastError());
+    hipLaunchKernelGGL(tg::lpt_scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, c->stream, count, count_dev,
+                       list, bucket, hist + tg::LPT_NB, order);
+    HIP_OK(hipGetLastError());
+    return TORCHGPU_OK;
+}
+
+// lpt_order_into on the handle's own buffers (grown as needed); returns
+// c->lpt_order.
 int lpt_order(torchgpu_cooling* c, long long stride, long long count, const unsigned long long* count_dev,
               const long long* list, const double* in, const long long** order) {
     if (count > c->lpt_cap) {
@@ -709,22 +750,34 @@ int lpt_order(torchgpu_cooling* c, long long stride, long long count, const unsi
         c->lpt_cap = count;
     }
     if (!c->lpt_hist) HIP_OK(hipMalloc(&c->lpt_hist, 2 * tg::LPT_NB * sizeof(unsigned long long)));
-    hipLaunchKernelGGL(tg::zero_u64_kernel, dim3(1), dim3(2 * tg::LPT_NB), 0, c->stream, c->lpt_hist,
-                       2 * tg::LPT_NB);
-    HIP_OK(hipGetLastError());
-    long long blocks = (count + 255) / 256;
-    if (blocks > (long long)c->nsm * 16) blocks = (long long)c->nsm * 16;
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(tg::lpt_bucket_kernel, dim3((unsigned)blocks), dim3(256), 0, c->stream, c->t, c->tables,
-                       stride, count, count_dev, list, in, c->lpt_bucket, c->lpt_hist, c->err);
-    HIP_OK(hipGetLastError());
-    hipLaunchKernelGGL(tg::lpt_offsets_kernel, dim3(1), dim3(64), 0, c->stream, c->lpt_hist,
-                       c->lpt_hist + tg::LPT_NB);
-    HIP_OK(hipGetLastError());
-    hipLaunchKernelGGL(tg::lpt_scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, c->stream, count, count_dev,
-                       list, c->lpt_bucket, c->lpt_hist + tg::LPT_NB, c->lpt_order);
-    HIP_OK(hipGetLastError());
+    int rc = lpt_order_into(c, stride, count, count_dev, list, in, c->lpt_order, c->lpt_bucket, c->lpt_hist);
+    if (rc) return rc;
     *order = c->lpt_order;
+    return TORCHGPU_OK;
+}
+
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex noalloc_mu;
+unsigned char* noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The device address of `scratch` on device `dev`, which must be current;
+// taken once per device.
+int noalloc_setup(torchgpu_cooling* c, int dev) {
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+        if (c) snprintf(c->msg, sizeof c->msg, "device %d beyond the %d devices of the fixed scratch", dev,
+                        NOALLOC_MAX_DEVICES);
+        return TORCHGPU_ERR_INVALID_ARG;
+    }
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if (noalloc_ptr[dev]) return TORCHGPU_OK;
+    void* ptr = nullptr;
+    HIP_OK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) {
+        if (c) snprintf(c->msg, sizeof c->msg, "hipGetSymbolAddress gave no address for the fixed scratch");
+        return TORCHGPU_ERR_HIP;
+    }
+    noalloc_ptr[dev] = static_cast<unsigned char*>(ptr);
     return TORCHGPU_OK;
 }
 
@@ -783,6 +836,12 @@ int torchgpu_cooling_create(const torchgpu_thermo_params* p, void* stream, torch
     int dev = 0;
     if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
         c->nsm = prop.multiProcessorCount;
+    c->device = dev;
+    // the fixed scratch of torchgpu_thermo_cells_noalloc on this device
+    if (noalloc_setup(c, dev) != TORCHGPU_OK) {
+        torchgpu_cooling_destroy(c);
+        return TORCHGPU_ERR_HIP;
+    }
     unsigned int zero = 0;
     if (hipMalloc(&c->tables, sizeof tab) != hipSuccess || hipMalloc(&c->counters, 4 * sizeof(unsigned long long)) != hipSuccess ||
         hipMalloc(&c->err, sizeof(unsigned int)) != hipSuccess ||
@@ -824,6 +883,44 @@ int torchgpu_thermo_cells(torchgpu_cooling* c, int64_t n, const double* in, doub
     if ((rc = lpt_order(c, n, n, nullptr, nullptr, in, &order))) return rc;
     rc = launch_fp64(c, n, n, nullptr, order, in, out, nullptr);
     if (rc) return rc;
+    if (total_substeps) {
+        unsigned long long tot = 0;
+        HIP_OK(hipMemcpyAsync(&tot, c->counters + 1, sizeof tot, hipMemcpyDeviceToHost, c->stream));
+        rc = check_err(c);
+        *total_substeps = (int64_t)tot;
+        if (rc) return rc;
+    }
+    return TORCHGPU_OK;
+}
+
+int torchgpu_thermo_cells_noalloc(torchgpu_cooling* c, int64_t n, const double* in, double* out,
+                                  int64_t* total_substeps) {
+    if (!c || n < 0 || (n > 0 && (!in || !out))) return TORCHGPU_ERR_INVALID_ARG;
+    if (n > NOALLOC_MAX_CELLS) {
+        snprintf(c->msg, sizeof c->msg, "torchgpu_thermo_cells_noalloc: %lld cells, at most %lld", (long long)n,
+                 NOALLOC_MAX_CELLS);
+        return TORCHGPU_ERR_INVALID_ARG;
+    }
+    if (n == 0) {
+        if (total_substeps) *total_substeps = 0;
+        return TORCHGPU_OK;
+    }
+    int dev = -1;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev != c->device) {
+        snprintf(c->msg, sizeof c->msg, "torchgpu_thermo_cells_noalloc: device %d is current, the handle is on %d",
+                 dev, c->device);
+        return TORCHGPU_ERR_INVALID_ARG;
+    }
+    int rc = noalloc_setup(c, dev);
+    if (rc) return rc;
+    unsigned char* base = noalloc_ptr[dev];
+    unsigned long long* hist = reinterpret_cast<unsigned long long*>(base);
+    long long* order = reinterpret_cast<long long*>(base + NOALLOC_HIST_BYTES);
+    unsigned char* bucket = base + NOALLOC_HIST_BYTES + NOALLOC_ORDER_BYTES;
+    if ((rc = zero_counters(c))) return rc;
+    if ((rc = lpt_order_into(c, n, n, nullptr, nullptr, in, order, bucket, hist))) return rc;
+    if ((rc = launch_fp64(c, n, n, nullptr, order, in, out, nullptr))) return rc;
     if (total_substeps) {
         unsigned long long tot = 0;
         HIP_OK(hipMemcpyAsync(&tot, c->counters + 1, sizeof tot, hipMemcpyDeviceToHost, c->stream));
