@@ -1,21 +1,28 @@
 #!/usr/bin/env python3
 """bench.py -- hydro step throughput on one MI355X.
 
-Workload (the reference's 256^3 turbulent box): 256^3 cells on the unit cube,
-gamma = 1.67, outflow boundaries, density log-normal (sigma_ln rho = 1, mean
-1) from a seeded Gaussian random field with P(k) ~ k^-11/3 on 2 <= |k| <= 32,
-uniform pressure 1, velocities from three more such fields scaled to rms Mach
-2; 2nd-order (linear reconstruction, van Albada) and the rotated HLLC-HLL
-solver; Torch::run step sequence (first step dt_max*1e-20, then CFL 0.5).
+Headline workload (the 512^3 mixed-precision case): 512^3 cells on the unit
+cube, gamma = 1.67, outflow boundaries, density log-normal (sigma_ln rho =
+1.5, mean 1) from a seeded Gaussian random field with P(k) ~ k^-11/3 on
+2 <= |k| <= 32, uniform pressure 1, velocities from three more such fields
+scaled to rms Mach 4; 2nd-order (linear reconstruction, van Albada) and the
+rotated HLLC-HLL solver; Torch::run step sequence (first step dt_max*1e-20,
+then CFL 0.5); Integration.debug off, as the reference's shipped
+configuration has it.
 
 One "step" is one Torch::hydroStep (predictor + corrector) of the whole grid
 with the CFL reduction for the next step; a cell-update is one cell advanced
 by one step.  `value` is the mixed-precision path (FP32 reconstruction and
 Riemann fluxes, FP64 accumulation, pressure recovery, floors and CFL); the
-FP64 path's rate, the speed-up and the mixed path's error against FP64 are
-reported alongside.  Inputs are resident in HBM before the timed region.
+FP64 path's rate, the speed-up and the mixed path's density, pressure and
+velocity error against FP64 after 200 steps are reported alongside (both
+runs take the smaller of their two CFL limits each step).  Inputs are
+resident in HBM before the timed region.  Extra keys: the 256^3 box
+(sigma 1, Mach 2), the cooling step on 2^24 independent cells (FP64
+subcycled integrator vs the surrogate step), the operator-split cloud
+(384^3, 100 steps, surrogate cooling) and the star operators.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512]
 """
 from __future__ import annotations
 
@@ -32,7 +39,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-SIGMA, MACH, GAMMA = 1.0, 2.0, 1.67
+SIGMA, MACH, GAMMA = 1.5, 4.0, 1.67     # the 512^3 mixed-precision workload
+SIGMA2, MACH2 = 1.0, 2.0                # the 256^3 box
 TFLOOR = 1e-12  # with the C++ default 0 the reference's step aborts on a zero pressure
 R_CGS = 8.314462e7
 PFLOOR = 0.1 * R_CGS * 1e-30 * TFLOOR
@@ -73,7 +81,7 @@ def cpu_baseline(n_sample: int, steps: int, threads: int) -> dict:
                     info = json.loads(r.stdout.strip().splitlines()[-1])
                     return {"value": round(info["mcell_updates_per_s"], 4), "unit": "Mcell-updates/s",
                             "cores": nproc, "kind": "reference",
-                            "sample": f"{n_sample}^3 turbulent box, {info['steps']} steps, FP64, "
+                            "sample": f"{n_sample}^3 turbulent box (sigma_ln_rho {SIGMA}, Mach {MACH}), {info['steps']} steps, FP64, "
                                       f"mpirun -np {nproc} (x-slab decomposition)"}
                 log("reference harness failed:", r.stderr[-500:])
         except Exception as e:  # noqa: BLE001
@@ -87,7 +95,7 @@ def cpu_baseline(n_sample: int, steps: int, threads: int) -> dict:
     oracle.run_hydro(p, U0, steps, debug=False)
     dt = time.perf_counter() - t0
     return {"value": round(n_sample ** 3 * steps / dt / 1e6, 4), "unit": "Mcell-updates/s", "cores": threads,
-            "kind": "port", "sample": f"{n_sample}^3 turbulent box, {steps} steps, FP64 oracle, {threads} OpenMP threads"}
+            "kind": "port", "sample": f"{n_sample}^3 turbulent box (sigma_ln_rho {SIGMA}, Mach {MACH}), {steps} steps, FP64 oracle, {threads} OpenMP threads"}
 
 
 def measured_traffic(n: int, kernel: str):
@@ -103,17 +111,22 @@ def measured_traffic(n: int, kernel: str):
     return None if k is None else round(k["read_bytes"] + k["write_bytes"])
 
 
-def hydro_rates(n: int, steps: int, warmup: int, err_steps: int):
-    import numpy as np
+def _event_pair():
+    import torch
+
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def hydro_rates(Q, n: int, steps: int, warmup: int):
+    """FP64 and mixed-precision Torch::run steps on the field Q (device
+    tensor): Mcell-updates/s over `steps` device-loop steps after `warmup`
+    untimed ones, and the two stage kernels' launch times (HIP events on the
+    stream the library enqueues on)."""
     import torch
 
     import torch_amd as ta
-    from torch_amd import setups
 
     stream = torch.cuda.current_stream()
-    t0 = time.perf_counter()
-    Q = torch.from_numpy(setups.turbulent(n, SIGMA, MACH, gamma=GAMMA, device="cuda")).cuda().contiguous()
-    log(f"IC {n}^3 in {time.perf_counter() - t0:.1f}s")
     res = {}
     for name, prec in (("fp64", ta.FP64), ("mixed", ta.MIXED)):
         g = ta.Grid(ta.make_params((n, n, n), gamma=GAMMA, tfloor=TFLOOR, pfloor=PFLOOR, precision=prec, debug=False),
@@ -122,7 +135,7 @@ def hydro_rates(n: int, steps: int, warmup: int, err_steps: int):
         g.run(warmup)
         g.synchronize()
         torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0, e1 = _event_pair()
         e0.record(stream)
         g.run(steps)
         e1.record(stream)
@@ -145,45 +158,79 @@ def hydro_rates(n: int, steps: int, warmup: int, err_steps: int):
         s1 = sum(e[0].elapsed_time(e[1]) for e in ev) / reps
         s2 = sum(e[1].elapsed_time(e[2]) for e in ev) / reps
         res[name] = {"ms_per_step": ms, "mcells": n ** 3 / (ms * 1e-3) / 1e6, "stage1_ms": s1, "stage2_ms": s2}
-        log(f"{name}: {ms:.3f} ms/step  {res[name]['mcells']:.1f} Mcell-updates/s  stage1 {s1:.3f} ms  stage2 {s2:.3f} ms")
+        log(f"{n}^3 {name}: {ms:.3f} ms/step  {res[name]['mcells']:.1f} Mcell-updates/s  stage1 {s1:.3f} ms  "
+            f"stage2 {s2:.3f} ms")
         g.close()
+    torch.cuda.empty_cache()
+    return res
 
-    # accuracy of the mixed path: same dt sequence as the FP64 run
-    g64 = ta.Grid(ta.make_params((n, n, n), gamma=GAMMA, tfloor=TFLOOR, pfloor=PFLOOR, precision=ta.FP64, debug=False),
-                  stream=stream.cuda_stream)
-    gmx = ta.Grid(ta.make_params((n, n, n), gamma=GAMMA, tfloor=TFLOOR, pfloor=PFLOOR, precision=ta.MIXED, debug=False),
-                  stream=stream.cuda_stream)
+
+def mixed_accuracy(Q, n: int, steps: int, every: int = 50):
+    """Density, pressure and velocity L1 / L-inf error of the mixed path
+    against the FP64 path after `steps` steps from the same field.  Both runs
+    take dt = min(their two CFL limits) each step (the first step
+    dt_max*1e-20): with the FP64 run's dt alone the mixed run can exceed its
+    own CFL limit in the near-vacuum rarefactions this field develops."""
+    import torch
+
+    import torch_amd as ta
+
+    stream = torch.cuda.current_stream()
+    kw = dict(gamma=GAMMA, tfloor=TFLOOR, pfloor=PFLOOR, debug=False)
+    g64 = ta.Grid(ta.make_params((n, n, n), precision=ta.FP64, **kw), stream=stream.cuda_stream)
+    gmx = ta.Grid(ta.make_params((n, n, n), precision=ta.MIXED, **kw), stream=stream.cuda_stream)
     g64.set_primitive(Q)
     gmx.set_primitive(Q)
-    for _ in range(err_steps):
-        g64.run(1)
-        _, dt, _ = g64.clock()
+    dt_max = g64.params.dt_max
+    t0 = time.perf_counter()
+    sim_t = 0.0
+    for s in range(1, steps + 1):
+        dt = dt_max * 1.0e-20 if s == 1 else min(g64.hydro_timestep(dt_max), gmx.hydro_timestep(dt_max))
+        g64.hydro_step(dt)
         gmx.hydro_step(dt)
+        sim_t += dt
+        if s % every == 0:
+            g64.synchronize()
+            gmx.synchronize()
+            log(f"accuracy run: step {s}, t {sim_t:.4g}, dt {dt:.4g} ({time.perf_counter() - t0:.1f}s)")
     g64.synchronize()
     gmx.synchronize()
     q64 = torch.empty((7, n, n, n), dtype=torch.float64, device="cuda")
     qmx = torch.empty_like(q64)
     g64.get_primitive(fix=False, out=q64)
     gmx.get_primitive(fix=False, out=qmx)
+    g64.close()
+    gmx.close()
     errs = {}
     for key, idx in (("density", [0]), ("pressure", [1]), ("velocity", [4, 5, 6])):
         a, b = q64[idx], qmx[idx]
         d = (b - a).abs()
         errs[key] = {"rel_l1": float(d.sum() / a.abs().sum()), "rel_linf": float(d.max() / a.abs().max())}
-    g64.close()
-    gmx.close()
-    del Q, q64, qmx
+    del q64, qmx
     torch.cuda.empty_cache()
-    return res, {"steps": err_steps, **errs}
+    return {"steps": steps, "time": sim_t, "dt_rule": "min of both CFL limits", **errs}
 
 
-def cooling_rates(log2_cells: int, weights: str, gate: float) -> dict:
+def turbulent_field(n: int, sigma: float, mach: float):
+    import torch
+
+    from torch_amd import setups
+
+    t0 = time.perf_counter()
+    Q = torch.from_numpy(setups.turbulent(n, sigma, mach, gamma=GAMMA, device="cuda")).cuda().contiguous()
+    log(f"IC {n}^3 (sigma {sigma}, Mach {mach}) in {time.perf_counter() - t0:.1f}s")
+    return Q
+
+
+def cooling_rates(log2_cells: int, weights: str, gate: float, tail_frac: float = 1e-3) -> dict:
     """The cooling source step on 2^log2_cells independent cells of the
     cooling workload (evaluation seed 11, disjoint from the training seeds):
     the FP64 subcycled integrator and the surrogate step (FP64 prepass, BF16
     MLP, FP64 fallback) on the same inputs, each event-timed on the stream
     it runs on, with the surrogate's relative temperature error against the
-    FP64 result."""
+    FP64 result.  The FP64 time is set by the longest cells (one cell needs
+    ~1e8 serial subcycles), so both steps are also timed on the same cells
+    without the `tail_frac` longest: a throughput-bound comparison."""
     import torch
 
     import torch_amd as ta
@@ -202,7 +249,7 @@ def cooling_rates(log2_cells: int, weights: str, gate: float) -> dict:
     inp = workload_inputs(n, seed=11)
     if not have:
         s = torch.cuda.current_stream()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0, e1 = _event_pair()
         e0.record(s)
         _, nsub = cool.thermo_cells(inp)
         e1.record(s)
@@ -210,19 +257,78 @@ def cooling_rates(log2_cells: int, weights: str, gate: float) -> dict:
         ms = e0.elapsed_time(e1)
         return {"cells": n, "fp64_subcycled": {"ms": round(ms, 2), "Mcells_per_s": round(n / ms / 1e3, 4),
                                                "subcycles": int(nsub)}, "surrogate": None}
-    r = sg.compare(cool, inp)
+    r, ref = sg.compare(cool, inp, return_fp64=True)
+    # the same cells without the longest tail_frac of them
+    nsub = ref[3]
+    k = int(n * (1.0 - tail_frac))
+    keep = torch.argsort(nsub)[:k]
+    cut = float(nsub[keep[-1]])
+    sub = inp[:, keep].contiguous()
+    del ref, nsub, keep
+    rt = sg.compare(cool, sub)
+    del sub
+    torch.cuda.empty_cache()
     del mlp
     return {
         "cells": n,
         "fp64_subcycled": {"ms": round(r["fp64_ms"], 2), "Mcells_per_s": round(n / r["fp64_ms"] / 1e3, 4),
-                           "subcycles": r["fp64_subcycles"]},
+                           "subcycles": r["fp64_subcycles"],
+                           "subcycles_per_s": round(r["fp64_subcycles"] / (r["fp64_ms"] * 1e-3), 1)},
         "surrogate": {"ms": round(r["surrogate_ms"], 2), "Mcells_per_s": round(n / r["surrogate_ms"] / 1e3, 4),
                       "gate": gate, "cells_exact": r["exact"], "cells_mlp": r["mlp_accepted"],
                       "cells_fallback": r["fallback"]},
         "speedup": round(r["fp64_ms"] / r["surrogate_ms"], 3),
         "rel_T_err_all_cells": r["rel_T_err_all"],
         "rel_T_err_mlp_cells": r["rel_T_err_mlp_cells"],
+        "without_longest": {
+            "cells": k, "excluded_fraction": tail_frac, "max_subcycles_kept": cut,
+            "fp64_ms": round(rt["fp64_ms"], 2), "fp64_subcycles": rt["fp64_subcycles"],
+            "fp64_subcycles_per_s": round(rt["fp64_subcycles"] / (rt["fp64_ms"] * 1e-3), 1),
+            "surrogate_ms": round(rt["surrogate_ms"], 2), "speedup": round(rt["fp64_ms"] / rt["surrogate_ms"], 3),
+            "cells_exact": rt["exact"], "cells_mlp": rt["mlp_accepted"], "cells_fallback": rt["fallback"],
+            "rel_T_err_mlp_cells": rt["rel_T_err_mlp_cells"]},
     }
+
+
+def cloud_rates(n: int, steps: int, weights: str, gate: float) -> dict:
+    """The operator-split cloud (torch_amd.cloud: 0.5 pc box, the softened
+    cloud profile, reference unit scales): `steps` Torch::fullStep
+    iterations of mixed-precision hydro sub-steps and the surrogate cooling
+    step in the device loop (torchgpu_run_split), event-timed, then the
+    cooling step on the grid's final cells compared with the FP64
+    integrator, with the cells on each path."""
+    import torch
+
+    from torch_amd import cloud
+    from torch_amd import surrogate as sg
+
+    t0 = time.perf_counter()
+    p, g, cool, thermo = cloud.build(n, "mixed")
+    mlp = sg.attach(cool, weights, gate=gate)
+    cool.run_split(g, 2, surrogate=True, **thermo)
+    g.synchronize()
+    log(f"cloud {n}^3 set up in {time.perf_counter() - t0:.1f}s")
+    s = torch.cuda.current_stream()
+    e0, e1 = _event_pair()
+    e0.record(s)
+    cool.run_split(g, steps, surrogate=True, **thermo)
+    e1.record(s)
+    torch.cuda.synchronize()
+    g.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    t, dt, nst = g.clock()
+    inp = cloud.cooling_inputs(g, p, thermo["col_den"], thermo["F_FUV"], thermo["excluded"], dt)
+    r = sg.compare(cool, inp)
+    del inp, mlp
+    g.close()
+    torch.cuda.empty_cache()
+    return {"n": n, "steps": steps, "ms_per_step": round(ms, 3),
+            "Mcell_updates_per_s": round(n ** 3 / (ms * 1e-3) / 1e6, 2), "time": t, "dt": dt,
+            "cooling_check": {"fp64_ms": round(r["fp64_ms"], 2), "surrogate_ms": round(r["surrogate_ms"], 2),
+                              "cells_exact": r["exact"], "cells_mlp": r["mlp_accepted"],
+                              "cells_fallback": r["fallback"],
+                              "cells_out_of_envelope": r["cells"] - r["exact"] - r["mlp_queued"],
+                              "rel_T_err_all": r["rel_T_err_all"], "rel_T_err_mlp_cells": r["rel_T_err_mlp_cells"]}}
 
 
 def star_cpu_baseline(m: int) -> dict:
@@ -299,8 +405,10 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--accuracy-steps", type=int, default=200, help="steps of the mixed-vs-FP64 error run (0: none)")
+    ap.add_argument("--n2", type=int, default=256, help="the sigma 1 / Mach 2 box (0: none)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--cpu-sample", type=int, default=96)
     ap.add_argument("--no-cpu", action="store_true")
@@ -308,6 +416,8 @@ def main():
     ap.add_argument("--no-cooling", action="store_true")
     ap.add_argument("--surrogate", default=os.path.join(ROOT, "torch_amd", "data", "surrogate_cooling.npz"))
     ap.add_argument("--gate", type=float, default=0.01)
+    ap.add_argument("--cloud-n", type=int, default=384, help="operator-split cloud grid (0: none)")
+    ap.add_argument("--cloud-steps", type=int, default=100)
     ap.add_argument("--star-n", type=int, default=256, help="star workload (ray trace, radiation) grid")
     ap.add_argument("--no-star", action="store_true")
     args = ap.parse_args()
@@ -322,13 +432,40 @@ def main():
         cpu = cpu_baseline(args.cpu_sample, 3, args.cpu_threads)
         log(f"cpu baseline {cpu} ({time.perf_counter() - t0:.1f}s)")
 
-    res, errs = hydro_rates(args.n, args.steps, args.warmup, args.warmup + args.steps)
+    import torch
+
+    n = args.n
+    Q = turbulent_field(n, SIGMA, MACH)
+    res = hydro_rates(Q, n, args.steps, args.warmup)
+    errs = None
+    if args.accuracy_steps > 0:
+        t0 = time.perf_counter()
+        errs = mixed_accuracy(Q, n, args.accuracy_steps)
+        log(f"mixed vs FP64 after {args.accuracy_steps} steps: {errs} ({time.perf_counter() - t0:.1f}s)")
+    del Q
+    torch.cuda.empty_cache()
     mx, f64 = res["mixed"], res["fp64"]
+    box2 = None
+    if args.n2 > 0:
+        Q2 = turbulent_field(args.n2, SIGMA2, MACH2)
+        r2 = hydro_rates(Q2, args.n2, args.steps, args.warmup)
+        del Q2
+        torch.cuda.empty_cache()
+        box2 = {"workload": f"hydro step, {args.n2}^3 turbulent box (sigma_ln_rho={SIGMA2}, rms Mach {MACH2})",
+                "mixed": {"value": round(r2["mixed"]["mcells"], 2), "ms_per_step": round(r2["mixed"]["ms_per_step"], 4),
+                          "stage1_ms": round(r2["mixed"]["stage1_ms"], 4), "stage2_ms": round(r2["mixed"]["stage2_ms"], 4)},
+                "fp64": {"value": round(r2["fp64"]["mcells"], 2), "ms_per_step": round(r2["fp64"]["ms_per_step"], 4)},
+                "mixed_over_fp64": round(r2["mixed"]["mcells"] / r2["fp64"]["mcells"], 3)}
     cool = None
     if not args.no_cooling:
         t0 = time.perf_counter()
         cool = cooling_rates(args.cooling_log2, args.surrogate, args.gate)
         log(f"cooling {cool} ({time.perf_counter() - t0:.1f}s)")
+    cl = None
+    if args.cloud_n > 0 and os.path.exists(args.surrogate):
+        t0 = time.perf_counter()
+        cl = cloud_rates(args.cloud_n, args.cloud_steps, args.surrogate, args.gate)
+        log(f"cloud {cl} ({time.perf_counter() - t0:.1f}s)")
     star = None
     if not args.no_star:
         from torch_amd import star_bench
@@ -338,9 +475,9 @@ def main():
         if not args.no_cpu:
             star["cpu_baseline"] = star_cpu_baseline(48)
         log(f"star {star} ({time.perf_counter() - t0:.1f}s)")
-    ncell = args.n ** 3
+    ncell = n ** 3
     achieved = BYTES_STAGE2 * ncell / (mx["stage2_ms"] * 1e-3) / 1e9
-    traffic = measured_traffic(args.n, "void tg::hydro_stage_mixed_kernel<2, true, 2>")
+    traffic = measured_traffic(n, "void tg::hydro_stage_mixed_kernel<2, true, 2>")
     out = {
         "metric": "Mcell-updates/s: FP64 vs mixed hydro; surrogate vs subcycled cooling",
         "value": round(mx["mcells"], 2),
@@ -354,19 +491,22 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 flux / f64 update",
         "data": "synthetic: seeded log-normal turbulent field (PCG64 seeds 1001-1004, P(k)~k^-11/3, 2<=|k|<=32)",
-        "config": {"workload": f"hydro step, {args.n}^3 turbulent box (sigma_ln_rho={SIGMA}, rms Mach {MACH}), "
-                               "RotatedHLLC + albada, outflow", "grid": args.n, "gamma": GAMMA,
-                   "temperature_floor": TFLOOR},
+        "config": {"workload": f"hydro step, {n}^3 turbulent box (sigma_ln_rho={SIGMA}, rms Mach {MACH}), "
+                               "RotatedHLLC + albada, outflow", "grid": n, "gamma": GAMMA,
+                   "temperature_floor": TFLOOR, "debug": False},
         "fp64": {"value": round(f64["mcells"], 2), "ms_per_step": round(f64["ms_per_step"], 4),
                  "stage1_ms": round(f64["stage1_ms"], 4), "stage2_ms": round(f64["stage2_ms"], 4)},
+        "mixed": {"stage1_ms": round(mx["stage1_ms"], 4), "stage2_ms": round(mx["stage2_ms"], 4)},
         "mixed_over_fp64": round(mx["mcells"] / f64["mcells"], 3),
         "mixed_vs_fp64_error": errs,
-        "roofline": {"bound": "hbm", "kernel": "hydro_stage_mixed_kernel<rotated,corrector>",
+        "roofline": {"bound": "hbm", "kernel": "hydro_stage_mixed_kernel<rotated,albada,corrector>",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_cell": BYTES_STAGE2, "launch_ms": round(mx["stage2_ms"], 4)},
         "cpu_baseline": cpu,
+        "box256": box2,
         "cooling": cool,
+        "cloud": cl,
         "star": star,
     }
     print(json.dumps(out), flush=True)

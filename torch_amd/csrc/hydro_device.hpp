@@ -125,6 +125,43 @@ __device__ __forceinline__ bool fix_solution_mixed(const HydroGeom& g, double u[
     return bad;
 }
 
+// fix_solution_mixed that also returns the primitives it rewrites U from
+// (density and pressure after the floors, fractions, velocities): the same
+// state QfromU would recover from the new U, without a second division and
+// a second energy-minus-kinetic cancellation.
+__device__ __forceinline__ bool fix_solution_mixed_q(const HydroGeom& g, double u[7], double q[7]) {
+    bool bad = !isfinite(u[DEN]) || !isfinite(u[PRE]);
+    const double ir = 1.0 / u[DEN];
+    const double hii = smax(smin(u[HII] * ir, 1.0), 0.0);
+    const double adv = smax(smin(u[ADV] * ir, 1.0), 0.0);
+    const double den = smax(u[DEN], g.dfloor);
+    double v[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) v[d] = u[VEL + d] * ir;
+    double ke = 0.5 * u[DEN] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    double pre = (u[PRE] - ke) * g.gm1;
+    ke *= den * ir;
+    if (pre < g.pfloor) pre = g.pfloor;
+    const double muR = (g.X * (hii + 1.0) + g.mu_c) * g.Rgas;
+    if (pre < g.tfloor * muR * den) pre = muR * den * g.tfloor;
+    u[DEN] = den;
+    u[PRE] = pre * g.igm1 + ke;
+    u[HII] = hii * den;
+    u[ADV] = adv * den;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) u[VEL + d] = den * v[d];
+    q[DEN] = den;
+    q[PRE] = pre;
+    q[HII] = hii;
+    q[ADV] = adv;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) q[VEL + d] = v[d];
+    bad |= (u[DEN] == 0.0 || u[PRE] == 0.0);
+#pragma unroll
+    for (int i = 0; i < 7; ++i) bad |= (u[i] != u[i]) || isinf(u[i]);
+    return bad;
+}
+
 __device__ __forceinline__ double cfl_inv_t_mixed(const HydroGeom& g, const double q[7]) {
     const double ss = sqrt(g.g * q[PRE] / q[DEN]);
     double inv_t = (fabs(q[VEL + 0]) + ss + (fabs(q[VEL + 1]) + ss) + (fabs(q[VEL + 2]) + ss)) * g.idx;

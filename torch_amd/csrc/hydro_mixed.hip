@@ -5,19 +5,23 @@
 // float and the state, the flux accumulation, the primitive recovery, the
 // floors and the CFL reduction in FP64.
 //
-// Layout for throughput: a workgroup (576 threads, 9 waves) owns an 8x8x8
-// tile.  One LDS array q[7][1280] (float, one plane per variable) holds the
-// tile's 512 cells and the 2-deep halo slabs of all three directions
-// (3 x 4 x 64 cells), loaded once at the start by all threads together; a
-// face finds each stencil cell with a select between its tile and halo
-// index (no branches) and reads the 7 variables with one address and
-// immediate plane offsets.  Per direction every thread solves one face into
-// an LDS flux array and the 512 cell threads accumulate their two faces
-// into FP64 registers in the reference's order.  For the y and z sweeps
-// consecutive lanes take consecutive x; for the x sweep consecutive lanes
-// walk along x.  LDS 52 KB, 74 VGPRs (six waves per SIMD: two workgroups
-// per CU; capping the registers at the 72 that three workgroups need
-// spills and measured 6 % slower).
+// Layout for throughput: a workgroup of 256 threads (4 waves) owns a 16 x 16
+// column of cells and marches it along z through MZ planes.  Each thread
+// keeps the float primitives of its own column at planes k-1 .. k+2 in
+// registers, so the z faces need neither LDS nor a reloaded z halo; plane k
+// with its 2-deep x and y halos (256 + 128 cells, float [7][384], one plane
+// per variable) goes through LDS for the 544 x and y faces, which the 256
+// threads take as one flat list (consecutive lanes read consecutive x).  The
+// z flux of face k+1/2 waits in LDS for the update; the thread keeps it in
+// registers as face k+1/2 of the next plane.  Per cell that is 3.125 face
+// solves (plus 1/MZ for the first z face of a column) and 1.5 cell loads, where
+// 8^3 tiles with 2-deep halos in all three directions needed 3.375 and 2.5.
+// After the faces each thread updates its cell of plane k in FP64 (fluxes
+// accumulated in the reference's order x, y, z; += F_L, -= F_R), applies
+// fixSolution, writes U, the float primitives of the output for the next
+// stage and (corrector) its CFL term.  33 KB LDS, 128 VGPRs: four waves per
+// SIMD, four workgroups per CU (neither the registers nor the LDS allow a
+// fifth).
 //
 // FP32 range: the float state is the FP64 state scaled by powers of two,
 // density by 2^kr and pressure by 2^kp with the grid's largest density and
@@ -36,15 +40,6 @@
 #include "hydro_device.hpp"
 
 namespace tg {
-
-constexpr int NLOAD = TCELLS + 3 * THALO;        // tile cells + halo slabs: 1280
-
-struct MixedLDS {
-    float q[7][NLOAD];   // [0, 512): tile cells x + 8 (y + 8 z); [512, 1280): halo slabs
-    float fl[7][TFACES];
-    double red[NTHREADS / 64];
-    unsigned int qmw[NTHREADS / 64][2];  // per-wave largest density / pressure (float bits)
-};
 
 // Power-of-two state scale of a launch (see the header) and the reference's
 // thresholds in the scaled units.  Everything here is uniform over the
@@ -93,132 +88,6 @@ __device__ __forceinline__ unsigned int max_bits(double v) {
     return v > 0.0 ? __float_as_uint((float)fmin(v, 3.0e38)) : 0u;
 }
 
-// LDS slot of along-position pos (-2..9) of pencil (ta, tb) in sweep DIM.
-// Tile cells: x + 8 (y + 8 z); halo cell of direction DIM at layer h
-// (pos -2, -1, 8, 9 -> h 0..3): 512 + 256 DIM + 64 h + 8 tb + ta.
-template <int DIM>
-__device__ __forceinline__ int slot(int pos, int ta, int tb) {
-    int c;
-    if (DIM == 0) c = pos + TILE * (ta + TILE * tb);
-    else if (DIM == 1) c = ta + TILE * (pos + TILE * tb);
-    else c = ta + TILE * (tb + TILE * pos);
-    const int h = pos < 0 ? pos + 2 : pos - (TILE - 2);
-    const int hs = TCELLS + DIM * THALO + h * (TILE * TILE) + tb * TILE + ta;
-    return (unsigned)pos < (unsigned)TILE ? c : hs;
-}
-
-// Face index of along-position f of pencil (ta, tb) in sweep DIM.
-template <int DIM>
-__device__ __forceinline__ int face_id(int f, int ta, int tb) {
-    if (DIM == 0) return (tb * TILE + ta) * (TILE + 1) + f;
-    return f * (TILE * TILE) + tb * TILE + ta;
-}
-
-template <int DIM>
-__device__ __forceinline__ void read_q(const MixedLDS& s, int pos, int ta, int tb, float q[7]) {
-    const int i = slot<DIM>(pos, ta, tb);
-#pragma unroll
-    for (int v = 0; v < 7; ++v) q[v] = s.q[v][i];
-}
-
-template <int SOLVER, bool ALB, int DIM>
-__device__ __forceinline__ bool face_flux_mixed(const HydroGeom& g, const MixedLDS& s, const MixedScale& m, int f,
-                                                int ta, int tb, const GasConsts<float>& gc, float F[7]) {
-    float ql[7], qr[7];
-    if (g.spatial_order == 1) {
-        float a[7], b[7], cc[7], dd[7];
-        read_q<DIM>(s, f - 2, ta, tb, a);
-        read_q<DIM>(s, f - 1, ta, tb, b);
-        read_q<DIM>(s, f, ta, tb, cc);
-        read_q<DIM>(s, f + 1, ta, tb, dd);
-        if constexpr (ALB) {
-#pragma unroll
-            for (int v = 0; v < 7; ++v) {
-                ql[v] = b[v] + 0.5f * albada_f(b[v] - a[v], cc[v] - b[v], thr_of(m, v));
-                qr[v] = cc[v] - 0.5f * albada_f(cc[v] - b[v], dd[v] - cc[v], thr_of(m, v));
-            }
-        } else {
-#pragma unroll
-            for (int v = 0; v < 7; ++v) {
-                ql[v] = b[v] + 0.5f * limiter_f(g.limiter, b[v] - a[v], cc[v] - b[v], thr_of(m, v));
-                qr[v] = cc[v] - 0.5f * limiter_f(g.limiter, cc[v] - b[v], dd[v] - cc[v], thr_of(m, v));
-            }
-        }
-    } else {
-        read_q<DIM>(s, f - 1, ta, tb, ql);
-        read_q<DIM>(s, f, ta, tb, qr);
-    }
-    return riemann_f<SOLVER, DIM>(F, ql, qr, gc, g.igm1f, m.nn_thr);
-}
-
-template <int SOLVER, bool ALB, int STAGE, int DIM>
-__device__ __forceinline__ void sweep_mixed(const HydroGeom& g, MixedLDS& s, const MixedScale& m, const int o[3],
-                                            StepState* st, const GasConsts<float>& gc, double udot[7]) {
-    const int tid = threadIdx.x;
-    const int N[3] = {g.nx, g.ny, g.nz};
-    constexpr int DA = DIM == 0 ? 1 : 0;
-    constexpr int DB = DIM == 2 ? 1 : 2;
-    {
-        int f, ta, tb;
-        if (DIM == 0) {
-            f = tid % (TILE + 1);
-            const int rest = tid / (TILE + 1);
-            ta = rest & 7;
-            tb = rest >> 3;
-        } else {
-            f = tid >> 6;
-            ta = tid & 7;
-            tb = (tid >> 3) & 7;
-        }
-        const bool valid = (o[DIM] + f <= N[DIM]) && (o[DA] + ta < N[DA]) && (o[DB] + tb < N[DB]);
-        float F[7];
-        bool bad = false;
-        if (valid) {
-            bad = face_flux_mixed<SOLVER, ALB, DIM>(g, s, m, f, ta, tb, gc, F);
-        } else {
-#pragma unroll
-            for (int v = 0; v < 7; ++v) F[v] = 0.f;
-        }
-        if (bad) set_error(st, STAGE == 1 ? ERR_RIEMANN_NAN_1 : ERR_RIEMANN_NAN_2);
-        const int id = face_id<DIM>(f, ta, tb);
-#pragma unroll
-        for (int v = 0; v < 7; ++v) s.fl[v][id] = F[v];
-    }
-    __syncthreads();
-    // += (A/V) F_L, -= (A/V) F_R (Hydro.cpp:117-137)
-    if (tid < TCELLS) {
-        const int lc[3] = {tid & 7, (tid >> 3) & 7, tid >> 6};
-        const int fL = face_id<DIM>(lc[DIM], lc[DA], lc[DB]);
-        const int fR = face_id<DIM>(lc[DIM] + 1, lc[DA], lc[DB]);
-#pragma unroll
-        for (int v = 0; v < 7; ++v) {
-            double u = udot[v];
-            u += g.kflux * (double)s.fl[v][fL];
-            u -= g.kflux * (double)s.fl[v][fR];
-            udot[v] = u;
-        }
-    }
-    __syncthreads();
-}
-
-// Tile-relative coordinates of load slot idx: the tile cell x + 8 (y + 8 z),
-// or halo slot 512 + 256 dim + 64 h + 8 tb + ta.
-__device__ __forceinline__ void slot_coords(int idx, int c[3]) {
-    if (idx < TCELLS) {
-        c[0] = idx & 7;
-        c[1] = (idx >> 3) & 7;
-        c[2] = idx >> 6;
-    } else {
-        const int k = idx - TCELLS;
-        const int dim = k >> 8, r = k & 255;
-        const int h = r >> 6, ta = r & 7, tb = (r >> 3) & 7;
-        const int pos = h < 2 ? h - 2 : TILE + h - 2;
-        if (dim == 0) { c[0] = pos; c[1] = ta; c[2] = tb; }
-        else if (dim == 1) { c[0] = ta; c[1] = pos; c[2] = tb; }
-        else { c[0] = ta; c[1] = tb; c[2] = pos; }
-    }
-}
-
 // Ghost-layer velocity rule (see flip_normal) on float primitives.
 __device__ __forceinline__ void flip_normal_f(float& vn, int mode) {
     switch (mode) {
@@ -258,158 +127,339 @@ __device__ __forceinline__ void store_qf(const HydroGeom& g, double q[7], const 
     for (int v = 0; v < 7; ++v) qout[v * g.ncell + c] = (float)q[v];
 }
 
-// The tile cells and the halo slabs of all three directions, three rounds of
-// 576 slots: float Q of the source cell, ghost velocity rule, power-of-two
-// scale, into LDS.  INNER: the tile and its 2-deep halos lie inside the grid
-// (no ghost cells, no clamping).  (Issuing all three rounds' loads before
-// using any measured slower: the extra registers cost more than the overlap
-// gains.)
-template <int STAGE, bool INNER>
-__device__ __forceinline__ void load_tile(const HydroGeom& g, const float* __restrict__ qin, const int o[3],
-                                          MixedLDS& s, const int ks[7], StepState* st, unsigned int& mr,
-                                          unsigned int& mp) {
-    constexpr int NR = (NLOAD + NTHREADS - 1) / NTHREADS;
-    const int tid = threadIdx.x;
-    for (int r = 0; r < NR; ++r) {
-        const int idx = tid + r * NTHREADS;
-        if (idx < NLOAD) {
-            int c[3];
-            slot_coords(idx, c);
-            long long src;
-            int mode = 0, mdim = 0;
-            if (INNER) {
-                src = (long long)(o[0] + c[0]) +
-                      (long long)g.nx * ((long long)(o[1] + c[1]) + (long long)g.ny * (o[2] + c[2]));
-            } else {
-                const CellSrc cs = cell_source(g, o[0] + c[0], o[1] + c[1], o[2] + c[2]);
-                src = cs.idx;
-                mode = cs.mode;
-                mdim = cs.mdim;
-            }
-            float q[7];
+#ifndef TG_MTX
+#define TG_MTX 16
+#endif
+constexpr int MTX = TG_MTX;               // x extent of a column
+constexpr int MTY = 256 / MTX;            // y extent
+constexpr int MCELLS = MTX * MTY;         // 256 cells per plane
+constexpr int PW = MTX + 4, PH = MTY + 4; // plane with its 2-deep halos (corners unused)
+constexpr int MHALO = 4 * (MTX + MTY);    // x and y halo cells per plane
+constexpr int NXF = (MTX + 1) * MTY;      // x faces per plane
+constexpr int NYF = MTX * (MTY + 1);      // y faces per plane
+constexpr int MTHREADS = MCELLS;
+
+struct MarchLDS {
+    float p[7][PW * PH];    // plane k: cell (x, y) at (y + 2) PW + x + 2, x in [-2, MTX + 2), y in [-2, MTY + 2)
+    float fx[7][NXF];       // x faces of plane k: y (MTX + 1) + f
+    float fy[7][NYF];       // y faces of plane k: f MTX + x
+    float fz[7][MCELLS];    // z face k + 1/2 of each column
+    double red[MTHREADS / 64];
+    unsigned int qmw[MTHREADS / 64][2];
+};
+
+__device__ __forceinline__ int mslot(int x, int y) { return (y + 2) * PW + x + 2; }
+
+// Float primitives of global cell (x, y, z) (positions out of range are
+// ghost cells: the first out-of-range dimension decides, as cell_source),
+// scaled by 2^ks.  INNER: (x, y) lies inside the grid.
+template <bool INNER>
+__device__ __forceinline__ void load_cell_f(const HydroGeom& g, const float* __restrict__ qin, int x, int y, int z,
+                                            const int ks[7], float q[7]) {
+    long long src;
+    int mode = 0, mdim = 0;
+    if (INNER && (unsigned)z < (unsigned)g.nz) {
+        src = (long long)x + (long long)g.nx * ((long long)y + (long long)g.ny * z);
+    } else {
+        const CellSrc cs = cell_source(g, x, y, z);
+        src = cs.idx;
+        mode = cs.mode;
+        mdim = cs.mdim;
+    }
 #pragma unroll
-            for (int v = 0; v < 7; ++v) q[v] = qin[v * g.ncell + src];
-            if (mdim) apply_ghost_mode_f(q, mode, mdim - 1);
-            if (STAGE == 1 && r == 0 && idx < TCELLS) {
-                const bool check = INNER || ((o[0] + (idx & 7) < g.nx) && (o[1] + ((idx >> 3) & 7) < g.ny) &&
-                                             (o[2] + (idx >> 6) < g.nz));
-                if (check) {
-                    if (q[DEN] != q[DEN]) set_error(st, ERR_CHECK_VALUES);
-                    mr = max_bits_f(q[DEN]);
-                    mp = max_bits_f(q[PRE]);
-                }
-            }
+    for (int v = 0; v < 7; ++v) q[v] = qin[v * g.ncell + src];
+    if (mdim) apply_ghost_mode_f(q, mode, mdim - 1);
 #pragma unroll
-            for (int v = 0; v < 7; ++v) s.q[v][idx] = __builtin_ldexpf(q[v], ks[v]);
-        }
+    for (int v = 0; v < 7; ++v) q[v] = __builtin_ldexpf(q[v], ks[v]);
+}
+
+// Halo cell h (0 .. MHALO - 1) of the plane: in-plane position relative to
+// the column (x halos first, then y halos).
+__device__ __forceinline__ void mhalo_pos(int h, int& x, int& y) {
+    if (h < 4 * MTY) {
+        const int l = h / MTY;
+        x = l < 2 ? l - 2 : MTX + l - 2;
+        y = h - l * MTY;
+    } else {
+        const int j = h - 4 * MTY;
+        const int l = j / MTX;
+        y = l < 2 ? l - 2 : MTY + l - 2;
+        x = j - l * MTX;
     }
 }
 
-// Six waves per SIMD (at most 80 VGPRs): two 9-wave workgroups per CU.
-// ALB: van Albada limiter (the other limiters in the ALB = false instance).
-// qin: float Q of the input state (fixPrimitives applied for stage 1);
-// qout: float Q of the output state for the next stage, fixed if qout_fix.
+template <int SOLVER, bool ALB, int DIM>
+__device__ __forceinline__ bool recon_flux(const HydroGeom& g, const MixedScale& m, const float a[7], const float b[7],
+                                           const float cc[7], const float dd[7], const GasConsts<float>& gc,
+                                           float F[7]) {
+    float ql[7], qr[7];
+    if (g.spatial_order == 1) {
+        if constexpr (ALB) {
+#pragma unroll
+            for (int v = 0; v < 7; ++v) {
+                ql[v] = b[v] + 0.5f * albada_f(b[v] - a[v], cc[v] - b[v], thr_of(m, v));
+                qr[v] = cc[v] - 0.5f * albada_f(cc[v] - b[v], dd[v] - cc[v], thr_of(m, v));
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < 7; ++v) {
+                ql[v] = b[v] + 0.5f * limiter_f(g.limiter, b[v] - a[v], cc[v] - b[v], thr_of(m, v));
+                qr[v] = cc[v] - 0.5f * limiter_f(g.limiter, cc[v] - b[v], dd[v] - cc[v], thr_of(m, v));
+            }
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < 7; ++v) {
+            ql[v] = b[v];
+            qr[v] = cc[v];
+        }
+    }
+    return riemann_f<SOLVER, DIM>(F, ql, qr, gc, g.igm1f, m.nn_thr);
+}
+
+template <int STRIDE>
+__device__ __forceinline__ void read_stencil(const MarchLDS& s, int base, float a[7], float b[7], float cc[7],
+                                             float dd[7]) {
+#pragma unroll
+    for (int v = 0; v < 7; ++v) {
+        a[v] = s.p[v][base];
+        b[v] = s.p[v][base + STRIDE];
+        cc[v] = s.p[v][base + 2 * STRIDE];
+        dd[v] = s.p[v][base + 3 * STRIDE];
+    }
+}
+
+// One x face (item < NXF: row y = item / (MTX + 1), face f = item % (MTX + 1))
+// or y face (j = item - NXF: face f = j / MTX, column x = j % MTX) of plane
+// k; the four stencil cells are consecutive slots (x) or PW apart (y).
 template <int SOLVER, bool ALB, int STAGE>
-__global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(7))) void hydro_stage_mixed_kernel(
-    HydroGeom g, const float* __restrict__ qin, const double* Ubase, double* Udst, float* __restrict__ qout,
-    int qout_fix, StepState* st, double dtf_host) {
-    __shared__ MixedLDS s;
-    // A halted sub-step's pass-through is launch_halted_copy's: a copy branch
-    // here (never taken while running) made every stage 45% slower at 256^3.
-    if (st->halted) return;
-    const int tile = tile_of_block(blockIdx.x, g.ntiles);
-    const int tx = tile % g.ntx, ty = (tile / g.ntx) % g.nty, tz = tile / (g.ntx * g.nty);
-    const int o[3] = {tx * TILE, ty * TILE, tz * TILE};
+__device__ __forceinline__ void march_xy_face(const HydroGeom& g, MarchLDS& s, const MixedScale& m, int item, int ox,
+                                              int oy, StepState* st, const GasConsts<float>& gc) {
+    float a[7], b[7], cc[7], dd[7], F[7];
+    bool bad = false;
+    if (item < NXF) {
+        const int y = item / (MTX + 1), f = item - y * (MTX + 1);
+        const bool valid = (ox + f <= g.nx) && (oy + y < g.ny);
+        if (valid) {
+            read_stencil<1>(s, mslot(f - 2, y), a, b, cc, dd);
+            bad = recon_flux<SOLVER, ALB, 0>(g, m, a, b, cc, dd, gc, F);
+        } else {
+#pragma unroll
+            for (int v = 0; v < 7; ++v) F[v] = 0.f;
+        }
+#pragma unroll
+        for (int v = 0; v < 7; ++v) s.fx[v][item] = F[v];
+    } else {
+        const int j = item - NXF;
+        const int f = j / MTX, x = j - f * MTX;
+        const bool valid = (oy + f <= g.ny) && (ox + x < g.nx);
+        if (valid) {
+            read_stencil<PW>(s, mslot(x, f - 2), a, b, cc, dd);
+            bad = recon_flux<SOLVER, ALB, 1>(g, m, a, b, cc, dd, gc, F);
+        } else {
+#pragma unroll
+            for (int v = 0; v < 7; ++v) F[v] = 0.f;
+        }
+#pragma unroll
+        for (int v = 0; v < 7; ++v) s.fy[v][j] = F[v];
+    }
+    if (bad) set_error(st, STAGE == 1 ? ERR_RIEMANN_NAN_1 : ERR_RIEMANN_NAN_2);
+}
+
+template <int SOLVER, bool ALB, int STAGE, bool INNER>
+__device__ __forceinline__ void march_column(const HydroGeom& g, MarchLDS& s, const MixedScale& m, const int ks[7],
+                                             int ox, int oy, int z0, int z1, const float* __restrict__ qin,
+                                             const double* Ubase, double* Udst, float* __restrict__ qout,
+                                             int qout_fix, StepState* st, double dtf_host, double& inv_t_max,
+                                             unsigned int& mr, unsigned int& mp) {
     const int tid = threadIdx.x;
+    const int tx = tid % MTX, ty = tid / MTX;
+    const int gx = ox + tx, gy = oy + ty;
+    const bool col = gx < g.nx && gy < g.ny;
     const GasConsts<float> gc = {g.gf, g.gm1f, g.g2f};
+    const int ke[7] = {m.kr + m.kv, m.kp + m.kv, m.kr + m.kv, m.kr + m.kv, m.kp, m.kp, m.kp};
+    const double dtf = stage_dtf(dtf_host, st, STAGE);
+    // halo slot of this thread and its in-plane position
+    int hx = 0, hy = 0;
+    if (tid < MHALO) mhalo_pos(tid, hx, hy);
+
+    // window of the own column: planes k-1, k, k+1, k+2
+    float w0[7], w1[7], w2[7], w3[7], fz[7];
+    {
+        float a[7];
+        load_cell_f<INNER>(g, qin, gx, gy, z0 - 2, ks, a);
+        load_cell_f<INNER>(g, qin, gx, gy, z0 - 1, ks, w0);
+        load_cell_f<INNER>(g, qin, gx, gy, z0, ks, w1);
+        load_cell_f<INNER>(g, qin, gx, gy, z0 + 1, ks, w2);
+        bool bad = false;
+        if (col) bad = recon_flux<SOLVER, ALB, 2>(g, m, a, w0, w1, w2, gc, fz);   // face z0 - 1/2
+        if (bad) set_error(st, STAGE == 1 ? ERR_RIEMANN_NAN_1 : ERR_RIEMANN_NAN_2);
+        load_cell_f<INNER>(g, qin, gx, gy, z0 + 2, ks, w3);
+    }
+    for (int k = z0; k < z1; ++k) {
+        // plane k into LDS
+#pragma unroll
+        for (int v = 0; v < 7; ++v) s.p[v][mslot(tx, ty)] = w1[v];
+        if (tid < MHALO) {
+            float hq[7];
+            load_cell_f<INNER>(g, qin, ox + hx, oy + hy, k, ks, hq);
+            const int hs = mslot(hx, hy);
+#pragma unroll
+            for (int v = 0; v < 7; ++v) s.p[v][hs] = hq[v];
+        }
+        if (STAGE == 1 && col) {
+            // "hydro before" mark and the input's largest density and pressure
+            const float rho = __builtin_ldexpf(w1[DEN], -ks[DEN]);
+            if (rho != rho) set_error(st, ERR_CHECK_VALUES);
+            mr = max(mr, max_bits_f(rho));
+            mp = max(mp, max_bits_f(__builtin_ldexpf(w1[PRE], -ks[PRE])));
+        }
+        __syncthreads();
+        // z face k + 1/2 from registers (kept in LDS until the update), x and
+        // y faces of plane k from LDS
+        {
+            float fzn[7];
+            bool bad = false;
+            if (col) bad = recon_flux<SOLVER, ALB, 2>(g, m, w0, w1, w2, w3, gc, fzn);
+            else {
+#pragma unroll
+                for (int v = 0; v < 7; ++v) fzn[v] = 0.f;
+            }
+            if (bad) set_error(st, STAGE == 1 ? ERR_RIEMANN_NAN_1 : ERR_RIEMANN_NAN_2);
+#pragma unroll
+            for (int v = 0; v < 7; ++v) s.fz[v][tid] = fzn[v];
+        }
+        march_xy_face<SOLVER, ALB, STAGE>(g, s, m, tid, ox, oy, st, gc);
+        march_xy_face<SOLVER, ALB, STAGE>(g, s, m, tid + MTHREADS, ox, oy, st, gc);
+        if (tid < NXF + NYF - 2 * MTHREADS)
+            march_xy_face<SOLVER, ALB, STAGE>(g, s, m, tid + 2 * MTHREADS, ox, oy, st, gc);
+        __syncthreads();
+        if (col) {
+            // += (A/V) F_L, -= (A/V) F_R per dimension (Hydro.cpp:117-137),
+            // then the FP64 update of the cell (gravity is zero)
+            double udot[7];
+            const int fxl = ty * (MTX + 1) + tx, fyl = ty * MTX + tx;
+#pragma unroll
+            for (int v = 0; v < 7; ++v) {
+                double u = 0.0;
+                u += g.kflux * (double)s.fx[v][fxl];
+                u -= g.kflux * (double)s.fx[v][fxl + 1];
+                u += g.kflux * (double)s.fy[v][fyl];
+                u -= g.kflux * (double)s.fy[v][fyl + MTX];
+                u += g.kflux * (double)fz[v];
+                u -= g.kflux * (double)s.fz[v][tid];
+                udot[v] = ldexp(u, -ke[v]);
+            }
+            const long long c = (long long)gx + (long long)g.nx * ((long long)gy + (long long)g.ny * k);
+            double u[7];
+#pragma unroll
+            for (int v = 0; v < 7; ++v) u[v] = Ubase[v * g.ncell + c];
+#pragma unroll
+            for (int v = 0; v < 7; ++v) u[v] += dtf * udot[v];
+            double q[7];
+            if (fix_solution_mixed_q(g, u, q)) set_error(st, STAGE == 1 ? ERR_FIX_SOLUTION_1 : ERR_FIX_SOLUTION_2);
+#pragma unroll
+            for (int v = 0; v < 7; ++v) Udst[v * g.ncell + c] = u[v];
+            if (STAGE == 2) {
+                fix_primitives_mixed(g, q, u[DEN]);
+                inv_t_max = fmax(inv_t_max, cfl_inv_t_mixed(g, q));
+            }
+            store_qf(g, q, u, qout, c, qout_fix, STAGE == 2);
+        }
+#pragma unroll
+        for (int v = 0; v < 7; ++v) {
+            fz[v] = s.fz[v][tid];
+            w0[v] = w1[v];
+            w1[v] = w2[v];
+            w2[v] = w3[v];
+        }
+        if (k + 1 < z1) load_cell_f<INNER>(g, qin, gx, gy, k + 3, ks, w3);
+    }
+}
+
+template <int SOLVER, bool ALB, int STAGE>
+// Four waves per SIMD (128 VGPRs; the 33 KB of LDS allow four workgroups
+// per CU): about 20 registers spill, which measured faster than three waves
+// per SIMD without spills (14.9 against 16.7 ms per 512^3 step).
+__global__ __launch_bounds__(MTHREADS) __attribute__((amdgpu_waves_per_eu(4))) void hydro_stage_mixed_kernel(HydroGeom g, int mtx, int mty, int mtz, int mz,
+                                                                     const float* __restrict__ qin, const double* Ubase,
+                                                                     double* Udst, float* __restrict__ qout,
+                                                                     int qout_fix, StepState* st, double dtf_host) {
+    __shared__ MarchLDS s;
+    if (st->halted) return;
+    const int ntiles = mtx * mty * mtz;
+    const int tile = tile_of_block(blockIdx.x, ntiles);
+    const int bx = tile % mtx, by = (tile / mtx) % mty, bz = tile / (mtx * mty);
+    const int ox = bx * MTX, oy = by * MTY, z0 = bz * mz;
+    const int z1 = min(z0 + mz, g.nz);
     const MixedScale m = mixed_scale(st);
     const int ks[7] = {m.kr, m.kp, 0, 0, m.kv, m.kv, m.kv};
-
-    const int lx = tid & 7, ly = (tid >> 3) & 7, lz = tid >> 6;
-    const int gx = o[0] + lx, gy = o[1] + ly, gz = o[2] + lz;
-    const bool active = tid < TCELLS && gx < g.nx && gy < g.ny && gz < g.nz;
-    const long long cell = (long long)gx + (long long)g.nx * ((long long)gy + (long long)g.ny * gz);
-
+    double inv_t = 0.0;
     unsigned int mr = 0, mp = 0;
-    const bool inner = o[0] >= 2 && o[1] >= 2 && o[2] >= 2 && o[0] + TILE + 2 <= g.nx &&
-                       o[1] + TILE + 2 <= g.ny && o[2] + TILE + 2 <= g.nz;
-    if (inner) load_tile<STAGE, true>(g, qin, o, s, ks, st, mr, mp);
-    else load_tile<STAGE, false>(g, qin, o, s, ks, st, mr, mp);
+    const bool inner = ox >= 2 && oy >= 2 && ox + MTX + 2 <= g.nx && oy + MTY + 2 <= g.ny;
+    if (inner)
+        march_column<SOLVER, ALB, STAGE, true>(g, s, m, ks, ox, oy, z0, z1, qin, Ubase, Udst, qout, qout_fix, st,
+                                               dtf_host, inv_t, mr, mp);
+    else
+        march_column<SOLVER, ALB, STAGE, false>(g, s, m, ks, ox, oy, z0, z1, qin, Ubase, Udst, qout, qout_fix, st,
+                                                dtf_host, inv_t, mr, mp);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (STAGE == 1) {
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
             mr = max(mr, (unsigned int)__shfl_xor((int)mr, off, 64));
             mp = max(mp, (unsigned int)__shfl_xor((int)mp, off, 64));
         }
-        if ((tid & 63) == 0) {
-            s.qmw[tid >> 6][0] = mr;
-            s.qmw[tid >> 6][1] = mp;
+        if (lane == 0) {
+            s.qmw[wave][0] = mr;
+            s.qmw[wave][1] = mp;
         }
+    } else {
+        if (!(inv_t >= 0.0)) inv_t = 0.0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) inv_t = fmax(inv_t, __shfl_xor(inv_t, off, 64));
+        if (lane == 0) s.red[wave] = inv_t;
     }
     __syncthreads();
-    if (STAGE == 1 && tid == 0) {
-        unsigned int a = 0, b = 0;
-        for (int w = 0; w < NTHREADS / 64; ++w) {
-            a = max(a, s.qmw[w][0]);
-            b = max(b, s.qmw[w][1]);
+    if (tid == 0) {
+        if (STAGE == 1) {
+            unsigned int a = 0, b = 0;
+            for (int w = 0; w < MTHREADS / 64; ++w) {
+                a = max(a, s.qmw[w][0]);
+                b = max(b, s.qmw[w][1]);
+            }
+            if (a) atomicMax(&st->qmax_next[0], a);
+            if (b) atomicMax(&st->qmax_next[1], b);
+        } else {
+            double mx = s.red[0];
+            for (int w = 1; w < MTHREADS / 64; ++w) mx = fmax(mx, s.red[w]);
+            atomicMax(&st->maxinv, (unsigned long long)__double_as_longlong(mx));
         }
-        if (a) atomicMax(&st->qmax_next[0], a);
-        if (b) atomicMax(&st->qmax_next[1], b);
     }
-
-    double udot[7] = {0, 0, 0, 0, 0, 0, 0};
-    sweep_mixed<SOLVER, ALB, STAGE, 0>(g, s, m, o, st, gc, udot);
-    sweep_mixed<SOLVER, ALB, STAGE, 1>(g, s, m, o, st, gc, udot);
-    sweep_mixed<SOLVER, ALB, STAGE, 2>(g, s, m, o, st, gc, udot);
-    {
-        // the sums are in flux units scaled by 2^(kr+kv) (mass, HII, ADV),
-        // 2^(kp+kv) (energy) and 2^kp (momentum): unscaled exactly
-        const int ke[7] = {m.kr + m.kv, m.kp + m.kv, m.kr + m.kv, m.kr + m.kv, m.kp, m.kp, m.kp};
-#pragma unroll
-        for (int v = 0; v < 7; ++v) udot[v] = ldexp(udot[v], -ke[v]);
-    }
-
-    double inv_t = 0.0;
-    if (active) {
-        // gravity source with GRAV = 0 (Hydro.cpp:134-137) adds nothing
-        const long long c = cell;
-        const double dtf = stage_dtf(dtf_host, st, STAGE);
-        double u[7];
-#pragma unroll
-        for (int v = 0; v < 7; ++v) u[v] = Ubase[v * g.ncell + c];
-#pragma unroll
-        for (int v = 0; v < 7; ++v) u[v] += dtf * udot[v];
-        if (fix_solution_mixed(g, u)) set_error(st, STAGE == 1 ? ERR_FIX_SOLUTION_1 : ERR_FIX_SOLUTION_2);
-#pragma unroll
-        for (int v = 0; v < 7; ++v) Udst[v * g.ncell + c] = u[v];
-        double q[7];
-        q_from_u_mixed(q, u, g.gm1);
-        if (STAGE == 2) {
-            fix_primitives_mixed(g, q, u[DEN]);
-            inv_t = cfl_inv_t_mixed(g, q);
-        }
-        store_qf(g, q, u, qout, c, qout_fix, STAGE == 2);
-    }
-    if (STAGE == 2) block_max_to(inv_t, &st->maxinv, s.red);
 }
+
+// z planes per workgroup: 16 and 32 measured the same at 512^3, 16 ran 2 %
+// faster at 256^3 (twice the workgroups).
+constexpr int MZ = 16;
 
 template <int SOLVER, bool ALB>
 static hipError_t launch_mixed_t(const HydroGeom& g, int stage, const float* qin, const double* Ubase, double* Udst,
                                  float* qout, int qout_fix, StepState* st, double dtf, hipStream_t stream) {
-    dim3 grid(g.ntiles), block(NTHREADS);
+    const int mtx = (g.nx + MTX - 1) / MTX, mty = (g.ny + MTY - 1) / MTY, mtz = (g.nz + MZ - 1) / MZ;
+    dim3 grid(mtx * mty * mtz), block(MTHREADS);
     if (stage == 1)
-        hipLaunchKernelGGL((hydro_stage_mixed_kernel<SOLVER, ALB, 1>), grid, block, 0, stream, g, qin, Ubase, Udst,
-                           qout, qout_fix, st, dtf);
+        hipLaunchKernelGGL((hydro_stage_mixed_kernel<SOLVER, ALB, 1>), grid, block, 0, stream, g, mtx, mty, mtz, MZ,
+                           qin, Ubase, Udst, qout, qout_fix, st, dtf);
     else
-        hipLaunchKernelGGL((hydro_stage_mixed_kernel<SOLVER, ALB, 2>), grid, block, 0, stream, g, qin, Ubase, Udst,
-                           qout, qout_fix, st, dtf);
+        hipLaunchKernelGGL((hydro_stage_mixed_kernel<SOLVER, ALB, 2>), grid, block, 0, stream, g, mtx, mty, mtz, MZ,
+                           qin, Ubase, Udst, qout, qout_fix, st, dtf);
     return hipGetLastError();
 }
 
 // The van Albada instance carries no code of the other limiters: the
-// generic limiter switch at the 42 reconstruction sites (14 per sweep)
-// triples the kernel's code (57 KB against 19 KB), close to the 64 KB
-// instruction cache two CUs share.
+// generic limiter switch at every reconstruction site (14 per face solve,
+// four inlined face solves) multiplies the kernel's code, and the 64 KB
+// instruction cache is shared by two CUs.
 template <int SOLVER>
 static hipError_t launch_mixed_l(const HydroGeom& g, int stage, const float* qin, const double* Ubase, double* Udst,
                                  float* qout, int qout_fix, StepState* st, double dtf, hipStream_t stream) {

@@ -95,6 +95,8 @@ def main():
     ap.add_argument("--commands", default=None, help="text file with the commands that made the inputs")
     ap.add_argument("--algorithmic", default="{}",
                     help='JSON {kernel: bytes per cell} for the algorithmic-bytes column')
+    ap.add_argument("--workload", default="sigma_ln rho = 1, rms Mach 2",
+                    help="field statistics of the turbulent box the runs used")
     ap.add_argument("--md", required=True)
     ap.add_argument("--json", required=True)
     a = ap.parse_args()
@@ -103,7 +105,7 @@ def main():
     copy_bytes = a.copy_mib * (1 << 20)
 
     lines = [f"# hydro stage kernels, {a.grid}^3 turbulent box: rocprofv3 summaries", ""]
-    lines += ["Workload: `bench.py` (sigma_ln rho = 1, rms Mach 2, RotatedHLLC + van Albada, outflow); "
+    lines += [f"Workload: `bench.py` ({a.workload}, RotatedHLLC + van Albada, outflow); "
               "FP64 parity kernel `hydro_stage_kernel<solver, stage>`, mixed-precision kernel "
               "`hydro_stage_mixed_kernel<solver, albada, stage>` (solver 2 = RotatedHLLC; stage 1 predictor, "
               "2 corrector + CFL).", ""]
