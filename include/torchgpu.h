@@ -368,6 +368,38 @@ int torchgpu_star_geometry(torchgpu_cooling* c, int64_t* nb, double* w, double* 
 int torchgpu_thermo_grid_inputs(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* T_min, const double* col_den,
                                 const double* F_FUV, int64_t excluded, double* in9, double* dcol_den, int on_device);
 
+/* GridCell::H, the heating arrays Torch::run prints at every checkpoint
+ * (Torch.cpp:264,272-276): HID order IMLC, NMLC, RHII, CEHI, CIEC, NMC, EUVH,
+ * FUVH, IRH, CRH, TOT (src/Torch/Common.hpp:32-34). */
+enum { TORCHGPU_HEAT_IMLC = 0, TORCHGPU_HEAT_NMLC, TORCHGPU_HEAT_RHII, TORCHGPU_HEAT_CEHI, TORCHGPU_HEAT_CIEC,
+       TORCHGPU_HEAT_NMC, TORCHGPU_HEAT_EUVH, TORCHGPU_HEAT_FUVH, TORCHGPU_HEAT_IRH, TORCHGPU_HEAT_CRH,
+       TORCHGPU_HEAT_TOT, TORCHGPU_HEAT_N };
+/* With `on`, the grid keeps the heating terms the reference keeps in
+ * GridCell::H between fills (zero at set_primitive, as a new GridCell):
+ * every thermo sub-step's Thermodynamics::integrate sets TOT to the
+ * post-subcycle RATE of each non-wind cell or zeroes H below the HII switch
+ * (Thermodynamics.cpp:275-280,334-335); every Radiation pre-step sets RHII
+ * and EUVH (Radiation.cpp:278-279).  Off frees them. */
+int torchgpu_heating_track(torchgpu_ctx* ctx, int on);
+/* Thermodynamics::fillHeatingArrays (Thermodynamics.cpp:419-461) on the
+ * state Torch::run prints (torchgpu_get_print_state): per non-wind cell the
+ * eight heating and cooling terms from Q (cooling terms negated), RHII and
+ * EUVH as kept, and TOT += RHII + EUVH (kept: it accumulates over fills until
+ * the next integrate resets it); wind cells and cells below the HII switch
+ * give zeros.  COL_DEN is the star's ray trace on Q's density when a star is
+ * set and col_den is NULL (Thermodynamics.cpp:420-421), else col_den or zero;
+ * F_FUV the star's flux, else F_FUV or zero.  Needs heating_track on before
+ * the run.  H [11][n] code units, host unless `on_device`.  Synchronises. */
+int torchgpu_fill_heating(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* col_den, const double* F_FUV,
+                          int64_t excluded, double* H, int on_device);
+/* DataPrinter::printHeating (src/IO/DataPrinter.cpp:325-354): time t (s),
+ * nx, ny, nz, then per cell (x fastest) the nd cell-centre coordinates in
+ * cell units (index + 0.5, as the reference prints GridCell::xc) and the 11
+ * terms of H (host [11][nz][ny][nx], code units) in erg cm^-3 s^-1, tab
+ * separated, "%.10e"; gzip when `gz`. */
+int torchgpu_heating_write(const char* path, int gz, double t, const double* H, int nx, int ny, int nz,
+                           double density_scale, double pressure_scale, double time_scale);
+
 /* ------------------------------------------------------------------------ */
 /* Radiation (photoionisation by the star; src/Integrators/Radiation.cpp)     */
 /* ------------------------------------------------------------------------ */

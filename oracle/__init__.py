@@ -112,6 +112,9 @@ def lib() -> ctypes.CDLL:
         L.orc_star_wind.argtypes = [i3, i3, ctypes.c_int, ctypes.POINTER(ctypes.c_ubyte)]
         L.orc_thermo_ray_trace.argtypes = [i3, i3, i64, dp, dp, d, dp, dp, dp]
         L.orc_star_fuv.argtypes = [i3, i3, d, d, d, dp]
+        L.orc_fill_heating.argtypes = [ctypes.POINTER(ThermoParams), ctypes.c_int64, dp, dp, dp,
+                                       ctypes.POINTER(ctypes.c_ubyte), dp, dp]
+        L.orc_fill_heating.restype = None
         L.orc_rad_init.argtypes = [ctypes.POINTER(RadParams), ctypes.POINTER(Converter), dp, d, d, d, d, d, d,
                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, d, d, d, d]
         L.orc_rad_recombination.argtypes = [ctypes.POINTER(RadParams), d]
@@ -338,6 +341,23 @@ def thermo_ray_trace(shape, star, geom, hydrogen_mass: float, den: np.ndarray):
     lib().orc_thermo_ray_trace(_i3((nx, ny, nz)), _i3(star), nb.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                                _dp(w), _dp(ds), hydrogen_mass, _dp(den), _dp(col), _dp(dcol))
     return col, dcol
+
+
+def fill_heating(t: ThermoParams, Q: np.ndarray, heat: np.ndarray, col_den=None, F=None, wind=None) -> np.ndarray:
+    """Thermodynamics::fillHeatingArrays (Thermodynamics.cpp:419-461): H
+    [11][...] of Fluid::Q (Q [7][...]); heat [3][...] (TOT, RHII, EUVH kept in
+    GridCell::H between calls) is updated in place."""
+    shape = Q.shape[1:]
+    n = int(np.prod(shape))
+    Q = np.ascontiguousarray(Q, dtype=np.float64)
+    assert heat.dtype == np.float64 and heat.flags.c_contiguous and heat.shape == (3,) + shape
+    col = None if col_den is None else np.ascontiguousarray(col_den, dtype=np.float64)
+    f = None if F is None else np.ascontiguousarray(F, dtype=np.float64)
+    w = None if wind is None else np.ascontiguousarray(wind, dtype=np.uint8)
+    H = np.zeros((11,) + shape)
+    lib().orc_fill_heating(ctypes.byref(t), n, _dp(Q), None if col is None else _dp(col), None if f is None else _dp(f),
+                           None if w is None else w.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), _dp(heat), _dp(H))
+    return H
 
 
 def star_fuv(shape, star, dx: float, pi: float, photon_rate: float) -> np.ndarray:

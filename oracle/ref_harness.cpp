@@ -354,6 +354,26 @@ static int run_hydro(const KV& kv) {
         fwrite(dts.data(), 8, dts.size(), f);
         fclose(f);
     }
+    // GridCell::H and Fluid::Q as the loop left them, then after each of
+    // `heating_fills` calls of Thermodynamics::fillHeatingArrays (the calls
+    // Torch::run makes at a checkpoint, Torch.cpp:272-273)
+    const int fills = geti(kv, "heating_fills", 0);
+    if (write && fills > 0) {
+        write_state(s, prefix + "_Q.bin", true);
+        for (int k = 0; k <= fills; ++k) {
+            if (k > 0) s.thermo.fillHeatingArrays(s.fluid);
+            std::vector<double> h((size_t)HID::N * NG);
+            for (long long c = 0; c < NG; ++c) {
+                GridCell& cell = grid.getCell((int)c);
+                for (int i = 0; i < HID::N; ++i) h[(size_t)i * NG + c] = cell.H[i];
+            }
+            char buf[32];
+            snprintf(buf, sizeof buf, "_H_%d.bin", k);
+            FILE* f = fopen((prefix + buf).c_str(), "wb");
+            fwrite(h.data(), 8, h.size(), f);
+            fclose(f);
+        }
+    }
     if (mpi.getRank() == 0) {
         printf("{\"cells\": %lld, \"steps\": %zu, \"seconds\": %.6f, \"mcell_updates_per_s\": %.6f, \"nproc\": %d, \"time\": %.17g, \"quitting\": %d}\n",
                NG, dts.size(), secs, (double)NG * dts.size() / secs / 1e6, mpi.nProcessors(), t, quitting ? 1 : 0);

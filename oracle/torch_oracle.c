@@ -840,6 +840,50 @@ void orc_thermo_rates(const orc_thermo_params* t, double nH, double x, double T,
     out[8] = orc_log_spline(t, log10(T));
 }
 
+/* Thermodynamics::fillHeatingArrays, :419-461 (the ray trace and F_FUV come
+ * in as arrays).  HID order IMLC, NMLC, RHII, CEHI, CIEC, NMC, EUVH, FUVH,
+ * IRH, CRH, TOT (Common.hpp:32-34). */
+void orc_fill_heating(const orc_thermo_params* t, int64_t n, const double* Q, const double* col_den, const double* F,
+                      const unsigned char* wind, double* heat, double* H) {
+    for (int64_t c = 0; c < n; ++c) {
+        double h[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (!wind || !wind[c]) {
+            double* tot = &heat[0 * n + c];
+            double* rhii = &heat[1 * n + c];
+            double* euvh = &heat[2 * n + c];
+            if (Q[3 * n + c] < t->thermo_hii_switch) {
+                *tot = 0;
+                *rhii = 0;
+                *euvh = 0;
+            } else {
+                const double den = Q[0 * n + c], pre = Q[1 * n + c], HIIFRAC = Q[2 * n + c];
+                double nH = t->mass_frac_h * den / t->hydrogen_mass;
+                double ne = HIIFRAC * nH;
+                double nn = (1.0 - HIIFRAC) * nH;
+                /* GridCell::temperature, GridCell.cpp:158-161 */
+                double mu_inv = t->mass_frac_h * (HIIFRAC + 1.0) + (1.0 - t->mass_frac_h) * 0.25;
+                double T = (pre / den) * (1.0 / mu_inv) / t->spec_gas_const;
+                double F_FUV = F ? F[c] : 0.0;
+                double tau = col_den ? col_den[c] : 0.0;
+                double Av_FUV = 1.086 * t->dust_xsec * tau;
+                h[7] = fuvh(t, nH, Av_FUV, F_FUV);
+                h[8] = irh(t, nH, Av_FUV, F_FUV);
+                h[9] = crh(t, nH);
+                h[0] = -imlc(t, ne, T);
+                h[1] = -nmlc(t, ne, nn, T);
+                h[3] = -cehi(t, nH, HIIFRAC, T);
+                h[4] = -ciec(t, ne, T);
+                h[5] = -nmc(t, nH, HIIFRAC, T);
+                *tot += *rhii + *euvh;
+                h[2] = *rhii;
+                h[6] = *euvh;
+                h[10] = *tot;
+            }
+        }
+        for (int i = 0; i < 11; ++i) H[i * n + c] = h[i];
+    }
+}
+
 /* preTimeStepCalculations (:218-264) followed by integrate (:266-337) for one
  * non-wind cell. */
 void orc_thermo_cell(const orc_thermo_params* t, double den, double pre, double hii, double adv,

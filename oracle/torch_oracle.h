@@ -151,6 +151,13 @@ void orc_star_geometry(const int n[3], const int star[3], double dx, double pi, 
                        double* shell_vol);
 void orc_star_wind(const int n[3], const int star[3], int wind_radius, unsigned char* wind);
 void orc_star_fuv(const int n[3], const int star[3], double dx, double pi, double photon_rate, double* F);
+/* Thermodynamics::fillHeatingArrays (Thermodynamics.cpp:419-461) on n cells:
+ * Q [7][n] is Fluid::Q, col_den [n] T[COL_DEN] (NULL: 0), F [n] the F_FUV of
+ * each cell (NULL: 0), wind [n] the CausalWind mask (cells left untouched);
+ * heat [3][n] holds GridCell::H's TOT, RHII and EUVH across calls (in/out);
+ * H [11][n] receives GridCell::H in HID order (zeros for wind cells). */
+void orc_fill_heating(const orc_thermo_params* t, int64_t n, const double* Q, const double* col_den, const double* F,
+                      const unsigned char* wind, double* heat, double* H);
 void orc_thermo_ray_trace(const int n[3], const int star[3], const int64_t* nb, const double* w, const double* ds,
                           double hydrogen_mass, const double* den, double* col, double* dcol);
 

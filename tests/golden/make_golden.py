@@ -467,6 +467,57 @@ def cloud_split():
     print("cloud_split", info, dts)
 
 
+def heating():
+    """Thermodynamics::fillHeatingArrays as Torch::run calls it at a
+    checkpoint, through the reference harness's loop: GridCell::H and Fluid::Q
+    after the last step, then H after each of two fills (TOT accumulates
+    RHII + EUVH until an integrate resets it).  Cases: the composed star run
+    (hydro, cooling, radiation; 4 steps), the star-off cloud (hydro, cooling;
+    6 steps) and the star run without cooling (hydro, radiation; TOT is never
+    reset)."""
+    cases = {}
+    cs = STAR_FULLSTEP
+    base_star = {"density_scale": 1e-20, "pressure_scale": 1e-28, "time_scale": 1e11, "side_length": cs["side"],
+                 "temperature_floor": 0.1, "simulation_time": 1e13, "ncheckpoints": 10, "star_x": cs["star"][0],
+                 "star_y": cs["star"][1], "star_z": cs["star"][2], "wind_radius": cs["wind"], **cs["kv"]}
+    cases["star"] = (star_fullstep_q(), cs["nsteps"], base_star)
+    cases["radonly"] = (star_fullstep_q(), 3, dict(base_star, cooling_on=0))
+    from torch_amd import cloud
+
+    cl = CLOUD_SPLIT
+    Qc, _, _, _, side = cloud.setup(cl["n"], lambda v, m, l, t: v)
+    c = cl["n"] // 2
+    cases["cloud"] = (Qc, cl["nsteps"], {"density_scale": 1e-20, "pressure_scale": 1e-28, "time_scale": 1e11,
+                                         "side_length": repr(side), "temperature_floor": 0.1,
+                                         "simulation_time": 1e13, "ncheckpoints": 10, "cooling_on": 1,
+                                         "thermo_hii_switch": 1e-2, "star_x": c, "star_y": c, "star_z": c})
+    res = {}
+    for name, (Q, nsteps, extra) in cases.items():
+        nz, ny, nx = Q.shape[1:]
+        with tempfile.TemporaryDirectory() as d:
+            Q.astype(np.float64).tofile(f"{d}/q.bin")
+            kv = {"nx": nx, "ny": ny, "nz": nz, "gamma": 1.67, "density_floor": 1e-30, "nsteps": nsteps,
+                  "input_q": f"{d}/q.bin", "dump_steps": "", "output_prefix": f"{d}/ref", "heating_fills": 2}
+            kv.update(extra)
+            with open(f"{d}/case.txt", "w") as f:
+                for k, v in kv.items():
+                    f.write(f"{k} {v}\n")
+            r = subprocess.run([HARNESS, "hydro", f"{d}/case.txt"], capture_output=True, text=True, check=True)
+            info = json.loads(r.stdout.strip().splitlines()[-1])
+            res[f"{name}_Q0"] = Q
+            res[f"{name}_nsteps"] = nsteps
+            res[f"{name}_dts"] = np.fromfile(f"{d}/ref_dt.bin")
+            res[f"{name}_time"] = info["time"]
+            res[f"{name}_Qprint"] = np.fromfile(f"{d}/ref_Q.bin").reshape(7, nz, ny, nx)
+            for k in range(3):
+                res[f"{name}_H{k}"] = np.fromfile(f"{d}/ref_H_{k}.bin").reshape(11, nz, ny, nx)
+        print("heating", name, info)
+    # the accumulation is visible: without cooling, TOT grows by RHII + EUVH
+    tot1, tot2 = res["radonly_H1"][10], res["radonly_H2"][10]
+    assert not np.array_equal(tot1, tot2)
+    np.savez_compressed(f"{HERE}/heating.npz", **res)
+
+
 if __name__ == "__main__":
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle ref")

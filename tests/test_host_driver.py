@@ -146,8 +146,18 @@ def test_driver_snapshots_and_restart(tmp_path, orc, gpu):
     po = orc.make_params(Q.shape[1:], 1.0 / n, dt_max=tmax / 10.0, nthreads=4)
     U0 = orc.UfromQ_grid(orc.fix_primitives(po, Q, np.zeros_like(Q)), po.gamma)
     Uo, dts, t, prints = orc.run_hydro_print(po, U0, 1000, tmax, nck)
-    names = sorted(os.listdir(tmp_path / "out"))
+    names = sorted(x for x in os.listdir(tmp_path / "out") if x.startswith("data2D"))
     assert names == [f"data2D_{c:06d}.txt.gz" for c, _, _ in prints] and len(names) == 5
+    # printHeating before every print inside the loop (Torch.cpp:270-276), not
+    # for the initial snapshot nor the final one after the loop (count
+    # ncheckpoints, Torch.cpp:295-297); Sod's ADV is 0, below the HII
+    # switch: zeros
+    heats = sorted(x for x in os.listdir(tmp_path / "out") if x.startswith("heating"))
+    assert heats == [f"heating_{c:06d}.txt.gz" for c, _, _ in prints[1:] if c != nck] and len(heats) == 3
+    for h in heats:
+        lines = _gunzip(tmp_path / "out" / h).decode().split("\n")
+        assert lines[1:4] == ["16", "16", "16"] and len(lines) == 4 + 16 ** 3 + 1
+        assert all(float(v) == 0.0 for line in lines[4:-1] for v in line.split("\t")[3:])
     for c, tp, Qp in prints:
         ref = str(tmp_path / f"oracle_{c}.txt.gz")
         gpu.snapshot_write(ref, tp, Qp, 1.0)
@@ -166,7 +176,8 @@ def test_driver_snapshots_and_restart(tmp_path, orc, gpu):
     U1 = orc.UfromQ_grid(orc.fix_primitives(po, Qs, np.zeros_like(Qs)), po.gamma)
     Uo2, dts2, t2, prints2 = orc.run_hydro_print(po, U1, 1000, tmax, nck, t0=h["time"])
     assert [c for c, _, _ in prints2] == [2, 3, 4]
-    assert sorted(os.listdir(tmp_path / "out2")) == [f"data2D_{c:06d}.txt.gz" for c, _, _ in prints2]
+    assert sorted(x for x in os.listdir(tmp_path / "out2") if x.startswith("data2D")) == \
+        [f"data2D_{c:06d}.txt.gz" for c, _, _ in prints2]
     for c, tp, Qp in prints2:
         ref = str(tmp_path / f"oracle2_{c}.txt.gz")
         gpu.snapshot_write(ref, tp, Qp, 1.0)

@@ -128,3 +128,40 @@ def test_reference_file_patch():
     assert (Q[3] == 3.0).all() and (Q[6] == 3.0).all()  # ADV and the third velocity untouched in 2-D
     with pytest.raises(ta.TorchGpuError):
         ta.snapshot_patch(FIX, np.zeros((7, 1, 100, 75)), 2 * 1.03e16)
+
+
+def test_heating_text_matches_oracle_format(tmp_path):
+    """torchgpu_heating_write against the oracle's restatement of
+    DataPrinter::printHeating (DataPrinter.cpp:325-354): the header (time
+    in s, nx, ny, nz), then per cell the raw cell-centre coordinates
+    (GridCell::xc, index + 0.5) and the 11 terms in erg cm^-3 s^-1,
+    "%.10e", tab separated, byte for byte.  The fixture's star run holds NaN
+    terms where the radiation pre-step's rate is 0 (RHII and EUVH are
+    x (soft / rate), Radiation.cpp:278-279); they print as the reference's
+    stream prints them."""
+    assert np.isnan(np.load(os.path.join(os.path.dirname(FIX), "heating.npz"))["star_H2"]).any()
+    import oracle as orc
+
+    f = np.load(os.path.join(os.path.dirname(FIX), "heating.npz"))
+    H = f["star_H2"]
+    t = float(f["star_time"])
+    path = str(tmp_path / "heating_000001.txt.gz")
+    ta.heating_write(path, t, H, SCALES)
+    text = gzip.open(path, "rt").read()
+    import math
+
+    def fmt(v):  # printf's "%.10e", whose NaN keeps its sign
+        if math.isnan(v):
+            return "-nan" if math.copysign(1.0, v) < 0 else "nan"
+        return f"{v:.10e}"
+
+    c = orc.converter(*SCALES)
+    nz, ny, nx = H.shape[1:]
+    lines = [f"{orc.from_code(c, t, 0, 0, 1):.10e}", str(nx), str(ny), str(nz)]
+    Hf = H.reshape(11, -1)
+    for cidx in range(nx * ny * nz):
+        ijk = (cidx % nx, (cidx // nx) % ny, cidx // (nx * ny))
+        cols = [f"{a + 0.5:.10e}" for a in ijk]
+        cols += [fmt(orc.from_code(c, Hf[i, cidx], 1, -1, -3)) for i in range(11)]
+        lines.append("\t".join(cols))
+    assert text == "\n".join(lines) + "\n"

@@ -84,6 +84,7 @@ def lib() -> ctypes.CDLL:
         L.torchgpu_synchronize.argtypes = [vp]
         L.torchgpu_get_clock.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(i)]
         L.torchgpu_is_quitting.argtypes = [vp, ctypes.POINTER(i)]
+        L.torchgpu_heating_track.argtypes = [vp, i]
         _lib = L
     return _lib
 
@@ -254,6 +255,11 @@ class Grid:
         self._check(lib().torchgpu_get_clock(self._h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n)))
         return t.value, dt.value, n.value
 
+    def heating_track(self, on: bool = True):
+        """Keep the GridCell::H terms that persist between heating fills
+        (torchgpu_heating_track)."""
+        self._check(lib().torchgpu_heating_track(self._h, int(on)))
+
     def quitting(self) -> bool:
         """Torch::isQuitting: the debug test of Torch::calculateTimeStep
         ended the time loop (Torch.cpp:321-332)."""
@@ -305,6 +311,7 @@ def _bind_cooling(L):
     L.torchgpu_cooling_set_min_temp.argtypes = [vp, vp, ctypes.c_int64, i]
     L.torchgpu_star_geometry.argtypes = [vp, vp, vp, vp, vp, i]
     L.torchgpu_thermo_grid_inputs.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int64, vp, vp, i]
+    L.torchgpu_fill_heating.argtypes = [vp, vp, vp, vp, ctypes.c_int64, vp, i]
 
 
 def units(scales, value, m, l, t, to_code=True) -> float:
@@ -329,7 +336,24 @@ def _bind_snapshot(L):
     L.torchgpu_snapshot_header.argtypes = [cp] + [ctypes.POINTER(d)] + [ctypes.POINTER(i)] * 4 + [ctypes.POINTER(d)]
     L.torchgpu_snapshot_read.argtypes = [cp, vp, i, i, i]
     L.torchgpu_snapshot_patch.argtypes = [cp, vp, i, i, i, d, i, i, i]
+    L.torchgpu_heating_write.argtypes = [cp, i, d, vp, i, i, i, d, d, d]
     L._snapshot_bound = True
+
+
+def heating_write(path: str, t: float, H, scales=(1.0, 1.0, 1.0), gz=None):
+    """DataPrinter::printHeating: time t and the heating arrays H (host
+    float64 [11][nz][ny][nx], code units) as text (gzip when `gz`, by default
+    when `path` ends in .gz)."""
+    import numpy as np
+
+    L = lib()
+    _bind_snapshot(L)
+    H = np.ascontiguousarray(H, dtype=np.float64)
+    nz, ny, nx = H.shape[1:]
+    gz = path.endswith(".gz") if gz is None else gz
+    rc = L.torchgpu_heating_write(path.encode(), int(gz), t, H.ctypes.data, nx, ny, nz, *scales)
+    if rc:
+        raise TorchGpuError(rc, f"heating_write {path}")
 
 
 def snapshot_write(path: str, t: float, Q, side_length: float, scales=(1.0, 1.0, 1.0), gz=None):
@@ -395,7 +419,7 @@ def snapshot_read(path: str, shape=None):
 
 
 def run_printing(grid: "Grid", nsteps: int, outdir: str, scales=(1.0, 1.0, 1.0), cooling=None, thermo=None,
-                 surrogate: bool = False, radiation=None):
+                 surrogate: bool = False, radiation=None, heating=None):
     """Torch::run with its snapshots (Torch.cpp:240-302): prints
     data2D_<count>.txt.gz (6-digit count) into `outdir` for the initial
     state, at every checkpoint (the run pauses on the device there) and at
@@ -403,12 +427,17 @@ def run_printing(grid: "Grid", nsteps: int, outdir: str, scales=(1.0, 1.0, 1.0),
     Torch::calculateTimeStep); at most `nsteps` steps (hydro only, or
     operator-split with `cooling` and its `thermo` keyword arguments and/or
     `radiation`).
+    With `heating` (a Cooling handle: its thermo constants, star and the
+    `thermo` arrays), the heating arrays are tracked from the start and
+    printed before each checkpoint's snapshot as heating_<count>.txt.gz
+    (Torch::run's fillHeatingArrays + printHeating, Torch.cpp:264,272-276).
     Returns the list of files written."""
     import os
 
     p = grid.params
     os.makedirs(outdir, exist_ok=True)
     written = []
+    hkw = {k: v for k, v in (thermo or {}).items() if k in ("col_den", "F_FUV", "excluded")}
 
     def emit(count):
         path = os.path.join(outdir, f"data2D_{count:06d}.txt.gz")
@@ -416,9 +445,19 @@ def run_printing(grid: "Grid", nsteps: int, outdir: str, scales=(1.0, 1.0, 1.0),
         snapshot_write(path, t, grid.print_state(), p.side_length, scales)
         written.append(path)
 
+    def emit_heating(count):
+        path = os.path.join(outdir, f"heating_{count:06d}.txt.gz")
+        t, _, _ = grid.clock()
+        heating_write(path, t, heating.fill_heating(grid, **hkw), scales)
+        written.append(path)
+
+    if heating is not None:
+        grid.heating_track(True)
     count, _ = grid.checkpoint()
     emit(count)
     grid.pause_at_checkpoints(True)
+    if heating is not None:
+        heating.fill_heating(grid, **hkw)  # the fill before Torch::run's loop (Torch.cpp:264)
     final_print = False
     done = 0
     try:
@@ -435,6 +474,8 @@ def run_printing(grid: "Grid", nsteps: int, outdir: str, scales=(1.0, 1.0, 1.0),
             count, due = grid.checkpoint()
             if not due:
                 break
+            if heating is not None:
+                emit_heating(count)
             emit(count)
             final_print = count != p.ncheckpoints
         # Torch::run's last print once its loop has ended: tmax reached or
@@ -578,6 +619,19 @@ class Cooling:
         self._check(lib().torchgpu_star_geometry(self._h, nb.ctypes.data, w.ctypes.data, ds.ctypes.data,
                                                  sv.ctypes.data, 0))
         return nb, w, ds, sv
+
+    def fill_heating(self, grid: "Grid", col_den=None, F_FUV=None, excluded: int = -1):
+        """Thermodynamics::fillHeatingArrays on the state Torch::run prints:
+        H [11][nz][ny][nx] (host, code units, HID order IMLC, NMLC, RHII,
+        CEHI, CIEC, NMC, EUVH, FUVH, IRH, CRH, TOT).  The grid must track its
+        heating terms (Grid.heating_track) from before the run."""
+        import numpy as np
+
+        p = grid.params
+        H = np.zeros((11, p.nz, p.ny, p.nx))
+        self._check(lib().torchgpu_fill_heating(grid._h, self._h, self._dptr(col_den, grid), self._dptr(F_FUV, grid),
+                                                excluded, H.ctypes.data, 0))
+        return H
 
     def grid_inputs(self, grid: "Grid", T_min=None, col_den=None, F_FUV=None, excluded: int = -1):
         """(in9 [9, n], DCOL_DEN [n]) the grid thermo operator forms for the

@@ -33,6 +33,7 @@ struct torchgpu_ctx {
     bool pause_ckpt = false;
     double* qprint = nullptr;
     double* pstate = nullptr;  // operator-split: input of the last sub-step
+    double* heat = nullptr;    // GridCell::H terms kept across steps while tracked ([3][n])
     std::string err;
 };
 
@@ -114,7 +115,29 @@ int reset_clock(torchgpu_ctx* c) {
 
 }  // namespace
 
-tg::CtxView tg::ctx_view(torchgpu_ctx* c) { return tg::CtxView{c->U, c->Ustar, c->st, c->stream, &c->g}; }
+namespace {
+
+// Fills c->qprint with Fluid::Q as Torch::run prints it (see
+// torchgpu_get_print_state).
+int print_state(torchgpu_ctx* c) {
+    tg::StepState st;
+    HIP_TRY(c, hipMemcpyAsync(&st, c->st, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const int mode = st.step > c->state_step ? c->last_mode : 0;
+    if (!c->qprint) HIP_TRY(c, hipMalloc(&c->qprint, state_bytes(c)));
+    if (mode >= 2 && !c->pstate) return fail(c, TORCHGPU_ERR_INVALID_ARG, "print state: pause at checkpoints first");
+    if (mode == 0) HIP_TRY(c, tg::launch_q_from_u(c->g, c->U, c->qprint, 1, c->stream));
+    else if (mode == 1) HIP_TRY(c, tg::launch_q_from_u(c->g, c->Ustar, c->qprint, 0, c->stream));
+    else if (mode == 2) HIP_TRY(c, tg::launch_q_from_u(c->g, c->pstate, c->qprint, 1, c->stream));
+    else HIP_TRY(c, hipMemcpyAsync(c->qprint, c->pstate, state_bytes(c), hipMemcpyDeviceToDevice, c->stream));
+    return TORCHGPU_OK;
+}
+
+}  // namespace
+
+const double* tg::ctx_print_state_dev(torchgpu_ctx* c) { return print_state(c) == TORCHGPU_OK ? c->qprint : nullptr; }
+
+tg::CtxView tg::ctx_view(torchgpu_ctx* c) { return tg::CtxView{c->U, c->Ustar, c->st, c->stream, &c->g, c->heat}; }
 
 void tg::ctx_state_changed(torchgpu_ctx* c) { c->qvalid = false; }
 
@@ -247,6 +270,7 @@ void torchgpu_destroy(torchgpu_ctx* c) {
     if (c->pstate) (void)hipFree(c->pstate);
     if (c->qf[0]) (void)hipFree(c->qf[0]);
     if (c->qf[1]) (void)hipFree(c->qf[1]);
+    if (c->heat) (void)hipFree(c->heat);
     delete c;
 }
 
@@ -269,6 +293,8 @@ int torchgpu_set_primitive(torchgpu_ctx* c, const double* Q, int on_device) {
                               c->stream));
     HIP_TRY(c, tg::launch_u_from_q(c->g, c->Ustar, c->U, c->stream));
     c->qvalid = c->qstar_valid = false;
+    // a new set-up: GridCell::H starts at zero (GridCell.cpp:30-39)
+    if (c->heat) HIP_TRY(c, tg::launch_zero(c->heat, 3 * c->g.ncell, c->stream));
     return seed_scale(c);
 }
 
@@ -415,19 +441,28 @@ int torchgpu_checkpoint(torchgpu_ctx* c, int* count, int* due) {
 
 int torchgpu_get_print_state(torchgpu_ctx* c, double* Q, int on_device) {
     if (!c || !Q) return TORCHGPU_ERR_INVALID_ARG;
-    tg::StepState st;
-    HIP_TRY(c, hipMemcpyAsync(&st, c->st, sizeof st, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    const int mode = st.step > c->state_step ? c->last_mode : 0;
-    if (!c->qprint) HIP_TRY(c, hipMalloc(&c->qprint, state_bytes(c)));
-    if (mode >= 2 && !c->pstate) return fail(c, TORCHGPU_ERR_INVALID_ARG, "print state: pause at checkpoints first");
-    if (mode == 0) HIP_TRY(c, tg::launch_q_from_u(c->g, c->U, c->qprint, 1, c->stream));
-    else if (mode == 1) HIP_TRY(c, tg::launch_q_from_u(c->g, c->Ustar, c->qprint, 0, c->stream));
-    else if (mode == 2) HIP_TRY(c, tg::launch_q_from_u(c->g, c->pstate, c->qprint, 1, c->stream));
-    else HIP_TRY(c, hipMemcpyAsync(c->qprint, c->pstate, state_bytes(c), hipMemcpyDeviceToDevice, c->stream));
+    int rc = print_state(c);
+    if (rc) return rc;
     HIP_TRY(c, hipMemcpyAsync(Q, c->qprint, state_bytes(c), on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                               c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return TORCHGPU_OK;
+}
+
+int torchgpu_heating_track(torchgpu_ctx* c, int on) {
+    if (!c) return TORCHGPU_ERR_INVALID_ARG;
+    if (!on) {
+        if (c->heat) {
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+            (void)hipFree(c->heat);
+            c->heat = nullptr;
+        }
+        return TORCHGPU_OK;
+    }
+    if (!c->heat) {
+        HIP_TRY(c, hipMalloc(&c->heat, 3 * sizeof(double) * (size_t)c->g.ncell));
+        HIP_TRY(c, tg::launch_zero(c->heat, 3 * c->g.ncell, c->stream));
+    }
     return TORCHGPU_OK;
 }
 

@@ -416,6 +416,19 @@ __global__ void halted_copy_kernel(HydroGeom g, const StepState* st, const doubl
     }
 }
 
+__global__ void zero_kernel(double* __restrict__ p, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+
+hipError_t launch_zero(double* p, long long n, hipStream_t stream) {
+    long long nb = (n + 255) / 256;
+    if (nb > 8192) nb = 8192;
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL(zero_kernel, dim3((unsigned)nb), dim3(256), 0, stream, p, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_halted_copy(const HydroGeom& g, const StepState* st, const double* Usrc, double* Udst,
                               const float* qin, float* qout, hipStream_t stream) {
     long long nb = (g.ncell + 255) / 256;

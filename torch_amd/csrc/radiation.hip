@@ -249,8 +249,10 @@ __global__ void rad_init_field_kernel(HydroGeom g, StarGeom s, RadConsts r, cons
     }
 }
 
+// heat (may be null): the grid's kept GridCell::H terms, whose RHII and
+// EUVH this pre-step sets (Radiation.cpp:278-279).
 __global__ void rad_heat_kernel(StarGeom s, RadConsts r, const double* __restrict__ tab, const double* __restrict__ q4,
-                                double* __restrict__ R, StepState* st, int gated) {
+                                double* __restrict__ R, StepState* st, int gated, double* __restrict__ heat) {
     if (gated && st->halted) return;
     const long long n = (long long)s.n[0] * s.n[1] * s.n[2];
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
@@ -271,6 +273,10 @@ __global__ void rad_heat_kernel(StarGeom s, RadConsts r, const double* __restric
         if (T < r.T_min + 200 && rate < 0.0) soft = smin(0.0, soft * (T - r.T_min) / 200.0);
         soft *= r.heat_amp;
         R[R_HEAT * n + c] = soft;
+        if (heat) {
+            heat[HEAT_EUVH * n + c] = photoion * (soft / rate);
+            heat[HEAT_RHII * n + c] = -recomb * (soft / rate);
+        }
         if (rate != rate) set_error(st, ERR_RAD_NAN);
     }
 }
@@ -570,7 +576,7 @@ int rad_gather(torchgpu_radiation* r, int check, int gated) {
 int rad_heat(torchgpu_radiation* r, int gated) {
     tg::CtxView v = tg::ctx_view(r->ctx);
     hipLaunchKernelGGL(tg::rad_heat_kernel, dim3(grid_blocks(v.g->ncell)), dim3(256), 0, r->stream, r->star, r->rc,
-                       r->tables, r->q4, r->R, v.st, gated);
+                       r->tables, r->q4, r->R, v.st, gated, v.heat);
     RAD_OK(hipGetLastError());
     return TORCHGPU_OK;
 }

@@ -4,6 +4,7 @@
 //   torchgpu_snapshot_header  DataReader::readDataParameters (src/IO/DataReader.cpp:17-68)
 //   torchgpu_snapshot_read    DataReader::readGrid   (src/IO/DataReader.cpp:70-125)
 //   torchgpu_snapshot_patch   DataReader::patchGrid  (src/IO/DataReader.cpp:126-231)
+//   torchgpu_heating_write    DataPrinter::printHeating (src/IO/DataPrinter.cpp:325-354)
 //
 // Format: the time (s), then nx, ny and nz, one per line; then one line per
 // cell, x fastest, with the nd cell-centre coordinates (cm), density
@@ -133,6 +134,32 @@ extern "C" int torchgpu_snapshot_write(const char* path, int gz, double t, const
         k += std::snprintf(line + k, sizeof line - k, "\t%.10e", Q[2 * n + c]);
         for (int d = 0; d < nd; ++d)
             k += std::snprintf(line + k, sizeof line - k, "\t%.10e", from_code(sc, Q[(4 + d) * n + c], 0, 1, -1));
+        line[k++] = '\n';
+        ok = out.write(line, (size_t)k);
+    }
+    ok = out.close() && ok;
+    return ok ? TORCHGPU_OK : TORCHGPU_ERR_INVALID_ARG;
+}
+
+extern "C" int torchgpu_heating_write(const char* path, int gz, double t, const double* H, int nx, int ny, int nz,
+                                      double density_scale, double pressure_scale, double time_scale) {
+    if (!path || !H || nx < 1 || ny < 1 || nz < 1) return TORCHGPU_ERR_INVALID_ARG;
+    double sc[3] = {density_scale, pressure_scale, time_scale};
+    const int nd = dims_of(ny, nz);
+    const long long n = (long long)nx * ny * nz;
+    TextOut out;
+    if (!out.open(path, gz != 0)) return TORCHGPU_ERR_INVALID_ARG;
+    char line[640];
+    int k = std::snprintf(line, sizeof line, "%.10e\n%d\n%d\n%d\n", from_code(sc, t, 0, 0, 1), nx, ny, nz);
+    bool ok = out.write(line, (size_t)k);
+    for (long long c = 0; c < n && ok; ++c) {
+        const long long idx[3] = {c % nx, (c / nx) % ny, c / ((long long)nx * ny)};
+        k = 0;
+        // GridCell::xc as it is, in cell units (the reference does not scale it here)
+        for (int d = 0; d < nd; ++d) k += std::snprintf(line + k, sizeof line - k, "%.10e\t", (double)idx[d] + 0.5);
+        k += std::snprintf(line + k, sizeof line - k, "%.10e", from_code(sc, H[c], 1, -1, -3));
+        for (int i = 1; i < TORCHGPU_HEAT_N; ++i)
+            k += std::snprintf(line + k, sizeof line - k, "\t%.10e", from_code(sc, H[i * n + c], 1, -1, -3));
         line[k++] = '\n';
         ok = out.write(line, (size_t)k);
     }

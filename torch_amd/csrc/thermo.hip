@@ -521,6 +521,80 @@ __global__ void zero_f64_kernel(double* p, long long n) {
 }
 
 // ---------------------------------------------------------------------------
+// GridCell::H, the heating arrays Torch::run prints at checkpoints.  A grid
+// that tracks them keeps TOT, RHII and EUVH between calls (heat [3][n]):
+// Thermodynamics::integrate sets TOT to the post-subcycle RATE of each
+// non-wind cell, or zeroes all of H below the HII switch
+// (Thermodynamics.cpp:275-280,334-335); Radiation::preTimeStepCalculations
+// sets RHII and EUVH (Radiation.cpp:278-279); fillHeatingArrays recomputes
+// the other eight from Q and adds RHII + EUVH to TOT (:419-461).
+// ---------------------------------------------------------------------------
+__global__ void heat_integrate_kernel(long long n, const double* __restrict__ in9, const double* __restrict__ out5,
+                                      double dt, const StepState* st, double hii_switch, double* __restrict__ heat) {
+    if (dt < 0.0 && st->halted) return;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+        const double adv = in9[3 * n + c];
+        if (adv == -1.0) continue;  // a wind cell (the gather's mark): never integrated
+        if (adv < hii_switch) {
+            heat[HEAT_TOT * n + c] = 0.0;
+            heat[HEAT_RHII * n + c] = 0.0;
+            heat[HEAT_EUVH * n + c] = 0.0;
+        } else {
+            heat[HEAT_TOT * n + c] = out5[2 * n + c];
+        }
+    }
+}
+
+// Thermodynamics::fillHeatingArrays on the primitives Q [7][n] of the
+// printed state; col_den: COL_DEN per cell (the star's ray trace or the
+// caller's array), or null for zero; F: the caller's F_FUV or null (the
+// star's flux when a star is on, else zero).  H [11][n] in HID order.
+__global__ __launch_bounds__(256) void fill_heating_kernel(HydroGeom g, ThermoConsts t, const double* __restrict__ tables,
+                                                           const double* __restrict__ Q,
+                                                           const double* __restrict__ col_den,
+                                                           const double* __restrict__ F, StarGeom star,
+                                                           long long excluded, double* __restrict__ heat,
+                                                           double* __restrict__ H) {
+    __shared__ double sx[26], sy[26], sy2[26];
+    load_tables(tables, sx, sy, sy2);
+    const long long n = g.ncell;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+        const int ijk[3] = {(int)(c % g.nx), (int)((c / g.nx) % g.ny), (int)(c / ((long long)g.nx * g.ny))};
+        bool wind = c == excluded;
+        if (star.on) wind = wind || star_wind_cell(star, ijk);
+        double h[11];
+#pragma unroll
+        for (int i = 0; i < 11; ++i) h[i] = 0.0;
+        if (!wind) {
+            if (Q[ADV * n + c] < t.thermo_hii_switch) {
+                heat[HEAT_TOT * n + c] = 0.0;
+                heat[HEAT_RHII * n + c] = 0.0;
+                heat[HEAT_EUVH * n + c] = 0.0;
+            } else {
+                const double den = Q[DEN * n + c], pre = Q[PRE * n + c], x = Q[HII * n + c];
+                const double nH = t.X * den / t.hydrogen_mass;
+                // GridCell::temperature (GridCell.cpp:158-161)
+                const double mu_inv = t.X * (x + 1.0) + (1.0 - t.X) * 0.25;
+                const double T = (pre / den) * (1.0 / mu_inv) / t.spec_gas_const;
+                double f = F ? F[c] : 0.0;
+                if (star.on && !F) f = star_fuv(star, ijk);
+                const double tau = col_den ? col_den[c] : 0.0;
+                const double Av = 1.086 * t.dust_xsec * tau;
+                heating_terms(t, sx, sy, sy2, nH, x, T, Av, f, h);
+                const double rhii = heat[HEAT_RHII * n + c], euvh = heat[HEAT_EUVH * n + c];
+                const double tot = heat[HEAT_TOT * n + c] + (rhii + euvh);
+                heat[HEAT_TOT * n + c] = tot;
+                h[2] = rhii;
+                h[6] = euvh;
+                h[10] = tot;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 11; ++i) H[i * n + c] = h[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Longest-first work order for the persistent FP64 kernel.  A cell's cost is
 // its subcycle count, known from the pre-step rate (thermo_init), and spans
 // 1 .. ~1e8; handed out in index order, a long cell that starts late ends
@@ -1089,6 +1163,14 @@ static int grid_substep_dev(torchgpu_ctx* ctx, torchgpu_cooling* c, const double
     if (surrogate) rc = torchgpu_thermo_cells_surrogate(c, n, c->in9, c->out5, nullptr, nullptr);
     else rc = torchgpu_thermo_cells(c, n, c->in9, c->out5, nullptr);
     if (rc) return rc;
+    if (v.heat && c->t.subcycling) {
+        // GridCell::H[TOT] of Thermodynamics::integrate (zeroed below the switch)
+        long long nb = (n + 255) / 256;
+        if (nb > 65536) nb = 65536;
+        hipLaunchKernelGGL(tg::heat_integrate_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, n, c->in9, c->out5,
+                           dt, v.st, c->t.thermo_hii_switch, v.heat);
+        HIP_OK(hipGetLastError());
+    }
     // Thermodynamics::updateSourceTerms + advSolution(dt) + fixSolution
     HIP_OK(tg::launch_fix_solution(*v.g, v.U, c->out5 + 2 * n, dt, v.st, c->stream, c->err));
     tg::ctx_state_changed(ctx);
@@ -1115,6 +1197,56 @@ int torchgpu_thermo_grid_substep(torchgpu_ctx* ctx, torchgpu_cooling* c, const d
     if (!ctx || !c || !(dt >= 0.0)) return TORCHGPU_ERR_INVALID_ARG;
     if (surrogate && !c->has_surrogate) return TORCHGPU_ERR_NOT_TRAINED;
     return grid_substep_dev(ctx, c, T_min, col_den, F, excluded, dt, surrogate);
+}
+
+int torchgpu_fill_heating(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* col_den, const double* F,
+                          int64_t excluded, double* H, int on_device) {
+    if (!ctx || !c || !H) return TORCHGPU_ERR_INVALID_ARG;
+    tg::CtxView v = tg::ctx_view(ctx);
+    if (!v.heat) {
+        snprintf(c->msg, sizeof c->msg, "fill_heating: the grid does not track its heating terms "
+                                        "(torchgpu_heating_track before the run)");
+        return TORCHGPU_ERR_INVALID_ARG;
+    }
+    const long long n = v.g->ncell;
+    if (c->star.on && (c->star.n[0] != v.g->nx || c->star.n[1] != v.g->ny || c->star.n[2] != v.g->nz))
+        return TORCHGPU_ERR_INVALID_ARG;
+    const double* Q = tg::ctx_print_state_dev(ctx);
+    if (!Q) {
+        snprintf(c->msg, sizeof c->msg, "fill_heating: the printed state is not available");
+        return TORCHGPU_ERR_INVALID_ARG;
+    }
+    int rc = ensure_grid(c, n);
+    if (rc) return rc;
+    if (c->star.on && !col_den) {
+        // fillHeatingArrays starts with rayTrace on Fluid::Q (Thermodynamics.cpp:420-421)
+        HIP_OK(tg::launch_thermo_ray_trace(c->star, Q, c->in9 + 6 * n, c->dcol, c->xface, c->stream));
+        col_den = c->in9 + 6 * n;
+    }
+    long long nb = (n + 255) / 256;
+    if (nb > 65536) nb = 65536;
+    if (on_device) {
+        hipLaunchKernelGGL(tg::fill_heating_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, *v.g, c->t, c->tables,
+                           Q, col_den, F, c->star, (long long)excluded, v.heat, H);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipStreamSynchronize(c->stream));
+        return TORCHGPU_OK;
+    }
+    // host output: the 11 planes through a temporary device array, freed on
+    // every path
+    double* tmp = nullptr;
+    HIP_OK(hipMalloc(&tmp, 11 * n * sizeof(double)));
+    hipError_t e = hipSuccess;
+    hipLaunchKernelGGL(tg::fill_heating_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, *v.g, c->t, c->tables, Q,
+                       col_den, F, c->star, (long long)excluded, v.heat, tmp);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(H, tmp, 11 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    const hipError_t es = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = es;
+    const hipError_t ef = hipFree(tmp);
+    if (e == hipSuccess) e = ef;
+    HIP_OK(e);
+    return TORCHGPU_OK;
 }
 
 int torchgpu_cooling_set_star(torchgpu_cooling* c, torchgpu_ctx* ctx, const torchgpu_star* star) {

@@ -124,4 +124,40 @@ __device__ __forceinline__ double heating(const ThermoConsts& t, double nH, doub
     return rate;
 }
 
+// The heating and cooling terms one by one, as Thermodynamics::
+// fillHeatingArrays writes them into GridCell::H (Thermodynamics.cpp:445-457,
+// the term functions :121-191): ionised and neutral metal lines, HI
+// collisional excitation, collisional ionisation equilibrium and neutral /
+// molecular cooling (negated), FUV, IR and cosmic-ray heating.  Written with
+// the reference's own operations (one libm call per exponential), not the
+// shared-log form of subtract_cooling.  Order of H: IMLC, NMLC, RHII, CEHI,
+// CIEC, NMC, EUVH, FUVH, IRH, CRH, TOT (Common.hpp:32-34); RHII, EUVH and
+// TOT are left to the caller.
+__device__ __forceinline__ void heating_terms(const ThermoConsts& t, const double* sx, const double* sy,
+                                              const double* sy2, double nH, double x, double T, double Av,
+                                              double F, double H[11]) {
+    const double ne = x * nH, nn = (1.0 - x) * nH;
+    H[7] = t.fuvh_a * nH * F * exp(-1.9 * Av) / (t.fuvh_b + t.fuvh_c * F * exp(-1.9 * Av) / nH);
+    H[8] = t.irh_a * nH * F * exp(-0.05 * Av) * exp(-2.0 * log(1.0 + t.irh_b / nH));
+    H[9] = t.crh * nH;
+    H[0] = -(t.imlc * t.z0 * ne * ne * exp(-t.T1 / T - (t.T2 / T) * (t.T2 / T)));
+    H[1] = -(t.nmlc * t.z0 * ne * nn * exp(-t.T3 / T - (t.T4 / T) * (t.T4 / T)));
+    {
+        const double r = log_spline(t, sx, sy, sy2, log10(T));
+        H[3] = -(x * (1.0 - x) * nH * nH * exp((2.302585093 * r) - ((T / t.cxhi_damp) * (T / t.cxhi_damp))));
+    }
+    if (T > t.ciec_minT) {
+        const double cie = t.ciec * ne * ne * t.z0 * exp(-0.63 * log(T)) * (1.0 - exp(-pow(1.0e-5 * T, 1.63)));
+        double smoothing = (T - 5.0e4) / (2.0e4);
+        smoothing = (smoothing < 1.0) ? smoothing : 1.0;
+        H[4] = -(cie * smoothing);
+    } else {
+        H[4] = -0.0;
+    }
+    {
+        const double T0 = 70.0 + 220.0 * pow(nH / t.n0, 0.2);
+        H[5] = -(t.nmc * (1.0 - x) * (1.0 - x) * pow(nH, 1.6) * sqrt(T) * exp(-T0 / T));
+    }
+}
+
 }  // namespace tg

@@ -117,7 +117,16 @@ struct CtxView {
     StepState* st;
     hipStream_t stream;
     const HydroGeom* g;
+    // GridCell::H kept across steps while heating output is tracked
+    // (torchgpu_heating_track), else null: [3][n] TOT, RHII, EUVH
+    double* heat;
 };
+
+// The three heating terms a grid keeps between Thermodynamics::
+// fillHeatingArrays calls (HEAT_TOT set by the thermo integrate, HEAT_RHII
+// and HEAT_EUVH by the radiation pre-step); the other eight are recomputed
+// from Q by each fill.
+enum { HEAT_TOT = 0, HEAT_RHII = 1, HEAT_EUVH = 2 };
 
 }  // namespace tg
 typedef struct torchgpu_ctx torchgpu_ctx;
@@ -164,6 +173,11 @@ int& ctx_counter_step(torchgpu_ctx* c);
 void ctx_set_print_mode(torchgpu_ctx* c, int mode);
 double* ctx_print_buffer(torchgpu_ctx* c);
 hipError_t launch_cfl(const HydroGeom& g, const double* U, StepState* st, hipStream_t stream);
+// p[0 .. n) = 0 with a kernel (no memset node if a caller captures a graph).
+hipError_t launch_zero(double* p, long long n, hipStream_t stream);
+// Fluid::Q as Torch::run prints it (see torchgpu_get_print_state), in a
+// device buffer of the context; null on error (message in the context).
+const double* ctx_print_state_dev(torchgpu_ctx* c);
 // thermo_dt_bits: the bit pattern of the cooling operator's minimum
 // |E/RATE| (a Torch::fullStep with thermodynamics), or null; rad_dt_bits:
 // that of Radiation::calculateTimeStep's minimum, or null.

@@ -291,7 +291,7 @@ void Thermodynamics::initialise(const TorchParameters& p, void* stream) {
     // star is on, and Thermodynamics only walks CausalNonWind: with the star
     // off that is the one cell at Star.cell_position (wind radius 0)
     if (!p.star_on) {
-        if (p.windCellRadius > 0)
+        if (p.windCellRadius > 0 && p.cooling_on)
             throw std::runtime_error("Star.wind_radius_in_cells > 0 with Star.on = false: the thermo operator skips "
                                      "one excluded cell only");
         const torchgpu_params g = p.gridParams();
@@ -470,14 +470,13 @@ void Torch::initialise(const TorchParameters& p_in, void* stream) {
     }
     params.initialise();
     fluid.initialise(params, stream);
-    if (params.cooling_on) {
-        thermodynamics.initialise(params, stream);
-        if (params.star_on) {
-            // Thermodynamics::preTimeStepCalculations with Star.on
-            const torchgpu_star star = starOf(params);
-            check_rc(torchgpu_cooling_set_star(thermodynamics.handle(), fluid.ctx(), &star), "cooling star",
-                     fluid.ctx());
-        }
+    // Torch::initialise sets up Thermodynamics for every run (Torch.cpp:80-81):
+    // Torch::run fills and prints its heating arrays whether or not cooling is on
+    thermodynamics.initialise(params, stream);
+    if (params.star_on) {
+        // Thermodynamics::preTimeStepCalculations (and fillHeatingArrays) with Star.on
+        const torchgpu_star star = starOf(params);
+        check_rc(torchgpu_cooling_set_star(thermodynamics.handle(), fluid.ctx(), &star), "cooling star", fluid.ctx());
     }
     if (params.radiation_on) {
         if (!params.star_on)
@@ -624,6 +623,37 @@ void Torch::print2D(int count) const {
         throw std::runtime_error("DataPrinter::print2D: unable to open" + path);
 }
 
+std::vector<double> Torch::fillHeating() {
+    // Thermodynamics::fillHeatingArrays (Thermodynamics.cpp:419-461) on the
+    // printed state; the star's ray trace and FUV flux come from the star
+    // set on the cooling handle
+    const torchgpu_params g = params.gridParams();
+    std::vector<double> H((size_t)TORCHGPU_HEAT_N * g.nx * g.ny * g.nz);
+    const int rc = torchgpu_fill_heating(fluid.ctx(), thermodynamics.handle(), nullptr, nullptr,
+                                         params.star_on ? -1 : thermodynamics.excluded(), H.data(), 0);
+    if (rc != TORCHGPU_OK) {
+        char msg[256];
+        torchgpu_cooling_last_error(thermodynamics.handle(), msg, sizeof msg);
+        throw std::runtime_error(std::string("fillHeatingArrays: ") + msg);
+    }
+    return H;
+}
+
+void Torch::printHeating(int count, const std::vector<double>& H) const {
+    // DataPrinter::printHeating (DataPrinter.cpp:325-354)
+    const torchgpu_params g = params.gridParams();
+    double t = 0, dt = 0;
+    int n = 0;
+    check_rc(torchgpu_get_clock(fluid.ctx(), &t, &dt, &n), "clock", fluid.ctx());
+    std::filesystem::create_directories(params.outputDirectory);
+    char name[64];
+    std::snprintf(name, sizeof name, "heating_%06d.txt.gz", count);
+    const std::string path = (std::filesystem::path(params.outputDirectory) / name).string();
+    if (torchgpu_heating_write(path.c_str(), 1, t, H.data(), g.nx, g.ny, g.nz, params.dscale, params.pscale,
+                               params.tscale) != TORCHGPU_OK)
+        throw std::runtime_error("DataPrinter::printHeating: unable to open" + path);
+}
+
 void Torch::run(int max_steps, bool print, bool host_loop) {  // Torch.cpp:240-302, capped at max_steps
     torchgpu_ctx* c = fluid.ctx();
     // the whole Torch::run loop on the device: dt, the pre-step
@@ -636,10 +666,15 @@ void Torch::run(int max_steps, bool print, bool host_loop) {  // Torch.cpp:240-3
     };
     if (print) {
         // the device loop pauses at every checkpoint for the print
+        if (!params.star_on && params.windCellRadius > 0)
+            throw std::runtime_error("heating output with Star.on = false and Star.wind_radius_in_cells > 0: only "
+                                     "the star's own cell is kept out of the heating arrays");
+        check_rc(torchgpu_heating_track(c, 1), "heating track", c);
         int count = 0, due = 0;
         check_rc(torchgpu_checkpoint(c, &count, &due), "checkpoint", c);
         print2D(count);
         check_rc(torchgpu_pause_at_checkpoints(c, 1), "pause at checkpoints", c);
+        fillHeating();  // Torch.cpp:264
         bool final_print = false;
         int done = 0;
         double t = 0, dt = 0;
@@ -652,6 +687,8 @@ void Torch::run(int max_steps, bool print, bool host_loop) {  // Torch.cpp:240-3
             done += n1 - n0;
             check_rc(torchgpu_checkpoint(c, &count, &due), "checkpoint", c);
             if (!due) break;
+            // Torch.cpp:271-276: fillHeatingArrays, printHeating, print2D
+            printHeating(count, fillHeating());
             print2D(count);
             final_print = count != params.ncheckpoints;
         }

@@ -225,6 +225,10 @@ public:
     // `host_loop` (without `print`): fullStep driven from the host
     void run(int max_steps, bool print = false, bool host_loop = false);
     void print2D(int count) const;
+    // Thermodynamics::fillHeatingArrays on the printed state (H [11][n]) and
+    // DataPrinter::printHeating of it
+    std::vector<double> fillHeating();
+    void printHeating(int count, const std::vector<double>& H) const;
     double fullStep(double dt_nextCheckpoint);
     void hydroStep(double dt, bool hasCalculatedHeatFlux);
     void subStep(double dt, bool hasCalculatedHeatFlux, Integrator& comp);
