@@ -177,6 +177,30 @@ def test_mixed_kernel_variants_ragged_grid(gpu, orc, solver, limiter, order):
     assert n == 6 and abs(tg - t) / t < 1e-5
 
 
+def test_mixed_kernel_interior_and_ring_columns(gpu, orc):
+    """A 64 x 48 x 40 grid, large enough that the mixed kernel has columns
+    whose x and y halos lie inside the grid (their loads skip the ghost
+    mapping; 16 x 16 columns: 2 x 1 of them) beside boundary columns, with
+    periodic, reflecting, outflow and inflow boundaries and 40 planes (three
+    z blocks, the last one partial): relative L1 of density, energy and
+    momenta against the FP64 oracle below 1e-5 after 4 steps."""
+    from torch_amd import setups
+
+    Q = setups.turbulent(64, 1.0, 1.5)[:, :40, :48, :64].copy()
+    bcs = ["periodic", "periodic", "reflecting", "outflow", "inflow", "outflow"]
+    kw = dict(tfloor=1e-12, pfloor=0.1 * 8.314462e7 * 1e-30 * 1e-12, bc=bcs)
+    g = _grid(gpu, Q.shape[1:], precision=gpu.MIXED, **kw)
+    g.set_primitive(np.ascontiguousarray(Q))
+    p = _oparams(orc, Q.shape[1:], dx=1.0 / Q.shape[3], **kw)
+    Uo, dts, t = orc.run_hydro(p, _u0(orc, p, Q), 4)
+    g.run(4)
+    g.synchronize()
+    Ug = g.get_conserved()
+    for v in (0, 1, 4, 6):
+        rel = np.abs(Ug[v] - Uo[v]).sum() / np.abs(Uo[v]).sum()
+        assert rel < 1e-5, (v, rel)
+
+
 @pytest.mark.parametrize("solver,limiter", [("RotatedHLLC", "albada"), ("HLL", "ospre"), ("HLLC", "leer")])
 def test_mixed_kernel_large_code_values(gpu, orc, solver, limiter):
     """Code units with pressures of 1e24 and velocities of 1e12 (the

@@ -150,14 +150,27 @@ def test_driver_snapshots_and_restart(tmp_path, orc, gpu):
     assert names == [f"data2D_{c:06d}.txt.gz" for c, _, _ in prints] and len(names) == 5
     # printHeating before every print inside the loop (Torch.cpp:270-276), not
     # for the initial snapshot nor the final one after the loop (count
-    # ncheckpoints, Torch.cpp:295-297); Sod's ADV is 0, below the HII
-    # switch: zeros
+    # ncheckpoints, Torch.cpp:295-297): the oracle's fillHeatingArrays on the
+    # oracle's printed state (the default HII switch is 0, so Sod's cells
+    # count; cosmic-ray heating is their one nonzero term; the star-off
+    # CausalWind cell (0, 0, 0) stays zero)
     heats = sorted(x for x in os.listdir(tmp_path / "out") if x.startswith("heating"))
     assert heats == [f"heating_{c:06d}.txt.gz" for c, _, _ in prints[1:] if c != nck] and len(heats) == 3
-    for h in heats:
-        lines = _gunzip(tmp_path / "out" / h).decode().split("\n")
-        assert lines[1:4] == ["16", "16", "16"] and len(lines) == 4 + 16 ** 3 + 1
-        assert all(float(v) == 0.0 for line in lines[4:-1] for v in line.split("\t")[3:])
+    tpar = orc.thermo_params(orc.converter(1.0, 1.0, 1.0), thermo_hii_switch=0.0)
+    wind = np.zeros((n, n, n), dtype=bool)
+    wind[0, 0, 0] = True
+    for c, tp, Qp in prints:
+        if c == 0 or c == nck:
+            continue
+        H = orc.fill_heating(tpar, Qp, np.zeros((3, n, n, n)), wind=wind)
+        lines = _gunzip(tmp_path / "out" / f"heating_{c:06d}.txt.gz").decode().split("\n")
+        assert lines[1:4] == ["16", "16", "16"] and len(lines) == 4 + n ** 3 + 1
+        vals = np.array([[float(v) for v in line.split("\t")] for line in lines[4:-1]])
+        np.testing.assert_array_equal(vals[:, :3], np.stack(np.meshgrid(np.arange(n) + 0.5, np.arange(n) + 0.5,
+                                                                        np.arange(n) + 0.5, indexing="ij")[::-1],
+                                                            -1).reshape(-1, 3))
+        np.testing.assert_allclose(vals[:, 3:], H.reshape(11, -1).T, rtol=1e-9, atol=0)
+        assert (vals[1:, 3 + 9] > 0).all() and (vals[0, 3:] == 0).all()
     for c, tp, Qp in prints:
         ref = str(tmp_path / f"oracle_{c}.txt.gz")
         gpu.snapshot_write(ref, tp, Qp, 1.0)
