@@ -296,7 +296,8 @@ def cloud_rates(n: int, steps: int, weights: str, gate: float) -> dict:
     iterations of mixed-precision hydro sub-steps and the surrogate cooling
     step in the device loop (torchgpu_run_split), event-timed, then the
     cooling step on the grid's final cells compared with the FP64
-    integrator, with the cells on each path."""
+    integrator at the loop's dt and at 10x and 100x it, with the cells on
+    each path."""
     import torch
 
     from torch_amd import cloud
@@ -317,18 +318,26 @@ def cloud_rates(n: int, steps: int, weights: str, gate: float) -> dict:
     g.synchronize()
     ms = e0.elapsed_time(e1) / steps
     t, dt, nst = g.clock()
-    inp = cloud.cooling_inputs(g, p, thermo["col_den"], thermo["F_FUV"], thermo["excluded"], dt)
-    r = sg.compare(cool, inp)
-    del inp, mlp
+    out = {"n": n, "steps": steps, "ms_per_step": round(ms, 3),
+           "Mcell_updates_per_s": round(n ** 3 / (ms * 1e-3) / 1e6, 2), "time": t, "dt": dt}
+    # the cooling step on the final cells at the loop's dt, and at 10x and
+    # 100x it (beyond what Thermodynamics::calculateTimeStep lets the loop
+    # take), with the cells on each path
+    for mult in (1, 10, 100):
+        inp = cloud.cooling_inputs(g, p, thermo["col_den"], thermo["F_FUV"], thermo["excluded"], dt * mult)
+        r = sg.compare(cool, inp)
+        del inp
+        torch.cuda.empty_cache()
+        out["cooling_check" if mult == 1 else f"cooling_check_dt_x{mult}"] = {
+            "fp64_ms": round(r["fp64_ms"], 2), "surrogate_ms": round(r["surrogate_ms"], 2),
+            "fp64_subcycles": r["fp64_subcycles"],
+            "cells_exact": r["exact"], "cells_mlp": r["mlp_accepted"], "cells_fallback": r["fallback"],
+            "cells_out_of_envelope": r["cells"] - r["exact"] - r["mlp_queued"],
+            "rel_T_err_all": r["rel_T_err_all"], "rel_T_err_mlp_cells": r["rel_T_err_mlp_cells"]}
+    del mlp
     g.close()
     torch.cuda.empty_cache()
-    return {"n": n, "steps": steps, "ms_per_step": round(ms, 3),
-            "Mcell_updates_per_s": round(n ** 3 / (ms * 1e-3) / 1e6, 2), "time": t, "dt": dt,
-            "cooling_check": {"fp64_ms": round(r["fp64_ms"], 2), "surrogate_ms": round(r["surrogate_ms"], 2),
-                              "cells_exact": r["exact"], "cells_mlp": r["mlp_accepted"],
-                              "cells_fallback": r["fallback"],
-                              "cells_out_of_envelope": r["cells"] - r["exact"] - r["mlp_queued"],
-                              "rel_T_err_all": r["rel_T_err_all"], "rel_T_err_mlp_cells": r["rel_T_err_mlp_cells"]}}
+    return out
 
 
 def star_cpu_baseline(m: int) -> dict:

@@ -89,6 +89,8 @@ def main():
     ap.add_argument("--fetch-copy", required=True)
     ap.add_argument("--write-copy", required=True)
     ap.add_argument("--sq", default=None)
+    ap.add_argument("--lds", default=None,
+                    help="PMC CSV with SQ_LDS_BANK_CONFLICT, SQ_LDS_IDX_ACTIVE and GRBM_GUI_ACTIVE")
     ap.add_argument("--copy-mib", type=float, default=1024.0)
     ap.add_argument("--cus", type=int, default=256)
     ap.add_argument("--xcds", type=int, default=8)
@@ -97,6 +99,7 @@ def main():
                     help='JSON {kernel: bytes per cell} for the algorithmic-bytes column')
     ap.add_argument("--workload", default="sigma_ln rho = 1, rms Mach 2",
                     help="field statistics of the turbulent box the runs used")
+    ap.add_argument("--driver", default="bench.py", help="the program the runs profiled")
     ap.add_argument("--md", required=True)
     ap.add_argument("--json", required=True)
     a = ap.parse_args()
@@ -105,7 +108,7 @@ def main():
     copy_bytes = a.copy_mib * (1 << 20)
 
     lines = [f"# hydro stage kernels, {a.grid}^3 turbulent box: rocprofv3 summaries", ""]
-    lines += [f"Workload: `bench.py` ({a.workload}, RotatedHLLC + van Albada, outflow); "
+    lines += [f"Workload: `{a.driver}` ({a.workload}, RotatedHLLC + van Albada, outflow); "
               "FP64 parity kernel `hydro_stage_kernel<solver, stage>`, mixed-precision kernel "
               "`hydro_stage_mixed_kernel<solver, albada, stage>` (solver 2 = RotatedHLLC; stage 1 predictor, "
               "2 corrector + CFL).", ""]
@@ -159,6 +162,23 @@ def main():
             w = c["SQ_WAVES"]
             lines.append(f"| `{k}` | {w:.0f} | {c['SQ_INSTS_VALU'] / w:.0f} | {c['SQ_INSTS_SALU'] / w:.0f} | "
                          f"{c['SQ_INSTS_LDS'] / w:.0f} | {cyc:.3g} | {util:.2f} |")
+        lines.append("")
+    if a.lds:
+        ld = counters(a.lds)
+        lines += ["## LDS counters (PMC), per dispatch", "",
+                  "Bank-conflict cycles per conflict-free LDS cycle (SQ_LDS_BANK_CONFLICT / (SQ_LDS_IDX_ACTIVE - "
+                  "SQ_LDS_BANK_CONFLICT)), and LDS-active and bank-conflict cycles as a share of the kernel's CU "
+                  "cycles (CUs x GRBM_GUI_ACTIVE / XCDs).", "",
+                  "| kernel | conflict / conflict-free | LDS active share | bank-conflict share |",
+                  "|---|---|---|---|"]
+        for k in sorted(ld):
+            c = ld[k]
+            if "hydro_stage" not in k or "SQ_LDS_BANK_CONFLICT" not in c:
+                continue
+            cyc = a.cus * c["GRBM_GUI_ACTIVE"] / a.xcds
+            act, bc = c["SQ_LDS_IDX_ACTIVE"], c["SQ_LDS_BANK_CONFLICT"]
+            ratio = bc / (act - bc) if act > bc else float("nan")
+            lines.append(f"| `{k}` | {ratio:.3f} | {act / cyc:.3f} | {bc / cyc:.4f} |")
         lines.append("")
     with open(a.md, "w") as fh:
         fh.write("\n".join(lines))
