@@ -20,7 +20,8 @@ runs take the smaller of their two CFL limits each step).  Inputs are
 resident in HBM before the timed region.  Extra keys: the 256^3 box
 (sigma 1, Mach 2), the cooling step on 2^24 independent cells (FP64
 subcycled integrator vs the surrogate step), the operator-split cloud
-(384^3, 100 steps, surrogate cooling) and the star operators.
+(384^3, 100 steps, surrogate cooling with the network trained on the
+cloud's conditions) and the star operators.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512]
 """
@@ -296,8 +297,9 @@ def cloud_rates(n: int, steps: int, weights: str, gate: float) -> dict:
     iterations of mixed-precision hydro sub-steps and the surrogate cooling
     step in the device loop (torchgpu_run_split), event-timed, then the
     cooling step on the grid's final cells compared with the FP64
-    integrator at the loop's dt and at 10x and 100x it, with the cells on
-    each path."""
+    integrator at the loop's dt and at 10x, 100x and 1000x it, with the
+    cells on each path.  `weights` is the network trained on the cloud's
+    conditions (train_surrogate --workload cloud)."""
     import torch
 
     from torch_amd import cloud
@@ -319,11 +321,12 @@ def cloud_rates(n: int, steps: int, weights: str, gate: float) -> dict:
     ms = e0.elapsed_time(e1) / steps
     t, dt, nst = g.clock()
     out = {"n": n, "steps": steps, "ms_per_step": round(ms, 3),
-           "Mcell_updates_per_s": round(n ** 3 / (ms * 1e-3) / 1e6, 2), "time": t, "dt": dt}
-    # the cooling step on the final cells at the loop's dt, and at 10x and
-    # 100x it (beyond what Thermodynamics::calculateTimeStep lets the loop
-    # take), with the cells on each path
-    for mult in (1, 10, 100):
+           "Mcell_updates_per_s": round(n ** 3 / (ms * 1e-3) / 1e6, 2), "time": t, "dt": dt,
+           "network": os.path.basename(weights)}
+    # the cooling step on the final cells at the loop's dt, and at 10x, 100x
+    # and 1000x it (beyond what Thermodynamics::calculateTimeStep lets the
+    # loop take), with the cells on each path
+    for mult in (1, 10, 100, 1000):
         inp = cloud.cooling_inputs(g, p, thermo["col_den"], thermo["F_FUV"], thermo["excluded"], dt * mult)
         r = sg.compare(cool, inp)
         del inp
@@ -424,6 +427,8 @@ def main():
     ap.add_argument("--cooling-log2", type=int, default=24, help="cooling workload size, 2^k cells")
     ap.add_argument("--no-cooling", action="store_true")
     ap.add_argument("--surrogate", default=os.path.join(ROOT, "torch_amd", "data", "surrogate_cooling.npz"))
+    ap.add_argument("--cloud-surrogate", default=os.path.join(ROOT, "torch_amd", "data", "surrogate_cloud.npz"),
+                    help="network trained on the cloud's conditions (train_surrogate --workload cloud)")
     ap.add_argument("--gate", type=float, default=0.01)
     ap.add_argument("--cloud-n", type=int, default=384, help="operator-split cloud grid (0: none)")
     ap.add_argument("--cloud-steps", type=int, default=100)
@@ -471,9 +476,9 @@ def main():
         cool = cooling_rates(args.cooling_log2, args.surrogate, args.gate)
         log(f"cooling {cool} ({time.perf_counter() - t0:.1f}s)")
     cl = None
-    if args.cloud_n > 0 and os.path.exists(args.surrogate):
+    if args.cloud_n > 0:
         t0 = time.perf_counter()
-        cl = cloud_rates(args.cloud_n, args.cloud_steps, args.surrogate, args.gate)
+        cl = cloud_rates(args.cloud_n, args.cloud_steps, args.cloud_surrogate, args.gate)
         log(f"cloud {cl} ({time.perf_counter() - t0:.1f}s)")
     star = None
     if not args.no_star:

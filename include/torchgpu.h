@@ -513,14 +513,15 @@ int torchgpu_mlp_train_step(torchgpu_mlp* m, const float* feat, const float* tar
 
 /* Surrogate configuration of a cooling operator: network, feature
  * normalisation f = (u - mu) * isd of the raw features u = (log10 rho,
- * log10 T, x, log10 F_FUV, Av, log10 dt, log10 subcycle count, sign of the
- * pre-step rate) in code units (the last two from the FP64 prepass), the
- * training envelope [lo, hi] of the first six, the error gate on the
- * estimated relative T error and the subcycle count up to which cells are
- * integrated exactly. */
+ * log10 T, x, log10 max(F_FUV, 1), Av, log10 dt, log10(subcycle count + 1),
+ * log10(Tc / T) of the coarse end-temperature estimate Tc) in code units
+ * (the last two from the FP64 prepass), the training envelope [lo, hi] of
+ * the first seven (a cell outside it goes to the FP64 fallback; +-HUGE_VAL
+ * leaves a feature unbounded), the error gate on the estimated relative T
+ * error and the subcycle count up to which cells are integrated exactly. */
 typedef struct torchgpu_surrogate_cfg {
     float mu[8], isd[8];
-    double lo[6], hi[6];
+    double lo[7], hi[7];
     double gate;
     int nexact;
     int pad;
@@ -532,6 +533,14 @@ int torchgpu_cooling_set_surrogate(torchgpu_cooling* c, torchgpu_mlp* m, const t
  * paths. */
 int torchgpu_thermo_cells_surrogate(torchgpu_cooling* c, int64_t n, const double* in, double* out, uint8_t* path,
                                     int64_t* counts);
+/* Subcycle count of each cell (count, device int64 [n]) that
+ * torchgpu_thermo_cells would integrate: the explicit steps after the first
+ * (Thermodynamics::integrate, Thermodynamics.cpp:303-312, fixed by the
+ * pre-step rate), 0 for a cell that is not subcycled, -1 where dt / dti is
+ * beyond the int range; nothing is integrated and no error is raised (no
+ * reference counterpart: it prices a call, or filters training samples,
+ * before the integration).  Asynchronous. */
+int torchgpu_thermo_subcycles(torchgpu_cooling* c, int64_t n, const double* in, int64_t* count);
 /* Training data: for the cells with more than nexact subcycles, writes their
  * indices (list, device int64 [n]), features (feat [n][8], normalised with
  * the configured mu/isd), raw features (raw [n][8], may be NULL) and, from

@@ -83,3 +83,36 @@ def test_thermo_gate_and_no_subcycling(gpu, orc):
     out = out.cpu().numpy()
     assert (out[2][:256] == 0).all() and (out[3] == 0).all()
     np.testing.assert_array_equal(out[2][256:], out[1][256:])  # RATE unchanged by integrate
+
+
+def test_subcycle_counts_match_oracle(gpu, orc):
+    """torchgpu_thermo_subcycles gives, without integrating, the number of
+    subcycles Thermodynamics::integrate takes (Thermodynamics.cpp:303-312):
+    equal to the oracle's count on cooling-workload cells and on cells of the
+    cloud's conditions (dense, unlit, dt to 1e13 s), -1 where dt / dti is
+    past the int range, and no error is left on the handle."""
+    import torch
+
+    from torch_amd import surrogate as sg
+
+    c = orc.converter(1e-20, 1e-28, 1e11)
+    tc = lambda v, m, l, t: orc.to_code(c, v, m, l, t)  # noqa: E731
+    pfloor = 0.1 * tc(8.314462e7, 0, 2, -2) * tc(1e-30, 1, -3, 0) * 0.1
+    t = orc.thermo_params(c, pfloor=pfloor)
+    cool = _cooling(gpu)
+    inp = cooling_inputs.to_code_inputs(cooling_inputs.sample_cgs(4096, seed=99), tc)
+    cnt = sg.subcycles(cool, torch.from_numpy(np.ascontiguousarray(inp.T)).cuda()).cpu().numpy()
+    ref = orc.thermo_batch(t, inp, nthreads=8)
+    np.testing.assert_array_equal(cnt, ref[:, 3])
+    # the cloud's conditions; one cell with dt / dti past the int range
+    cl = cooling_inputs.to_code_inputs(cooling_inputs.sample_cgs_cloud(4096, seed=7), tc)
+    cl[0, 8] = 1e300
+    cnt = sg.subcycles(cool, torch.from_numpy(np.ascontiguousarray(cl.T)).cuda()).cpu().numpy()
+    assert cnt[0] == -1 and (cnt >= -1).all()
+    keep = (cnt >= 0) & (cnt <= 100000)
+    assert keep.sum() > 2500 and (cnt[keep] > 4).sum() > 100, (keep.sum(), (cnt[keep] > 4).sum())
+    ref = orc.thermo_batch(t, np.ascontiguousarray(cl[keep]), nthreads=8)
+    np.testing.assert_array_equal(cnt[keep], ref[:, 3])
+    # the counting call raised nothing: the integrator runs on the kept cells
+    out, tot = cool.thermo_cells(torch.from_numpy(np.ascontiguousarray(cl[keep].T)).cuda())
+    np.testing.assert_array_equal(out[3].cpu().numpy(), ref[:, 3])

@@ -162,7 +162,8 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--cooling", choices=["surrogate", "fp64"], default="surrogate")
     ap.add_argument("--surrogate", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
-                                                        "surrogate_cooling.npz"))
+                                                        "surrogate_cloud.npz"),
+                    help="network weights (default: the one trained on the cloud's conditions)")
     ap.add_argument("--gate", type=float, default=0.01)
     ap.add_argument("--tmax", type=float, default=1.0e13, help="simulation time, s (dt_max = tmax / 10)")
     ap.add_argument("--every", type=int, default=10, help="report (and check for step errors) every N steps")

@@ -41,6 +41,39 @@ def sample_cgs(n: int, seed: int = 2024) -> dict:
     }
 
 
+def sample_cgs_cloud(n: int, seed: int = 3030) -> dict:
+    """Cells of the operator-split cloud's conditions (the 0.5 pc box of
+    torch_amd.cloud): hydrogen densities from the cloud's rim to past its
+    core (log-uniform 1e-22..3e-16 g/cm^3), T log-uniform 10..1e5 K, x
+    exactly 0 (neutral) for a quarter, exactly 1 (the ionised sphere) for a
+    quarter and uniform in [0, 1] for the rest, an unlit half (F_ion = 0: the
+    ionising flux is absorbed within a cell of the source) beside a lit half
+    log-uniform 1e6..1e14 cm^-2 s^-1, optical depths from none to behind the
+    core (tau = 0 for 5 %, uniform in [0, 10] for 45 %, log-uniform
+    10..3e3 for the rest), and dt log-uniform 1e8..1e13 s (the loop's dt is
+    ~2e9 s; 1e13 s is ~5000 times it).  The exact end values put the
+    cloud's own x = 0, x = 1, F = 0 and tau = 0 cells inside the training
+    envelope."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lu = lambda lo, hi, m: np.exp(rng.uniform(np.log(lo), np.log(hi), m))  # noqa: E731
+    ux = rng.uniform(0.0, 1.0, n)
+    x = np.where(ux < 0.25, 0.0, np.where(ux < 0.5, 1.0, rng.uniform(0.0, 1.0, n)))
+    lit = rng.uniform(0.0, 1.0, n) < 0.5
+    ut = rng.uniform(0.0, 1.0, n)
+    tau = np.where(ut < 0.05, 0.0, np.where(ut < 0.5, rng.uniform(0.0, 10.0, n), lu(10.0, 3e3, n)))
+    return {
+        "rho": lu(1e-22, 3e-16, n),
+        "T": lu(10.0, 1e5, n),
+        "x": x,
+        "F_ion": np.where(lit, lu(1e6, 1e14, n), 0.0),
+        "tau": tau,
+        "dt": lu(1e8, 1e13, n),
+    }
+
+
+SAMPLERS = {"c3": sample_cgs, "cloud": sample_cgs_cloud}
+
+
 def to_code_inputs(s: dict, to_code, gamma: float = 1.67, X: float = 1.0, T_min: float = 100.0) -> np.ndarray:
     """(n, 9) array: den, pre, hii, adv, E, T_min, col_den, F_FUV, dt in code
     units, the column order of the cooling kernels and of the oracle."""

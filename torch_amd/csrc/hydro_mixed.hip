@@ -7,8 +7,10 @@
 //
 // Layout for throughput: a workgroup of 256 threads (4 waves) owns a 16 x 16
 // column of cells and marches it along z through MZ planes.  Each thread
-// keeps the float primitives of its own column at planes k-1 .. k+2 in
-// registers, so the z faces need neither LDS nor a reloaded z halo; plane k
+// keeps the float primitives of its own column at planes k .. k+2 and the
+// limited z slope of plane k in registers, so the z faces need neither LDS
+// nor a reloaded z halo, and each z slope serves both z faces of its cell;
+// plane k
 // with its 2-deep x and y halos (256 + 128 cells, float [7][384], one plane
 // per variable) goes through LDS for the 544 x and y faces, which the 256
 // threads take as one flat list (consecutive lanes read consecutive x).  The
@@ -217,6 +219,47 @@ __device__ __forceinline__ bool recon_flux(const HydroGeom& g, const MixedScale&
     return riemann_f<SOLVER, DIM>(F, ql, qr, gc, g.igm1f, m.nn_thr);
 }
 
+// Limited slope of cell b between its neighbours a and c (zero at first
+// order): the value recon_flux adds to b (times 1/2) on b's right face and
+// subtracts on its left face, so one evaluation serves both faces of b.
+template <bool ALB>
+__device__ __forceinline__ void cell_slope(const HydroGeom& g, const MixedScale& m, const float a[7], const float b[7],
+                                           const float c[7], float s[7]) {
+    if (g.spatial_order == 1) {
+#pragma unroll
+        for (int v = 0; v < 7; ++v) {
+            if constexpr (ALB) s[v] = albada_f(b[v] - a[v], c[v] - b[v], thr_of(m, v));
+            else s[v] = limiter_f(g.limiter, b[v] - a[v], c[v] - b[v], thr_of(m, v));
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < 7; ++v) s[v] = 0.f;
+    }
+}
+
+// Face flux between cells b and c given their limited slopes sb and sc
+// (cell_slope): the same face states as recon_flux on the stencil a b c d.
+template <int SOLVER, int DIM>
+__device__ __forceinline__ bool slope_flux(const HydroGeom& g, const MixedScale& m, const float b[7], const float sb[7],
+                                           const float c[7], const float sc[7], const GasConsts<float>& gc,
+                                           float F[7]) {
+    float ql[7], qr[7];
+    if (g.spatial_order == 1) {
+#pragma unroll
+        for (int v = 0; v < 7; ++v) {
+            ql[v] = b[v] + 0.5f * sb[v];
+            qr[v] = c[v] - 0.5f * sc[v];
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < 7; ++v) {
+            ql[v] = b[v];
+            qr[v] = c[v];
+        }
+    }
+    return riemann_f<SOLVER, DIM>(F, ql, qr, gc, g.igm1f, m.nn_thr);
+}
+
 template <int STRIDE>
 __device__ __forceinline__ void read_stencil(const MarchLDS& s, int base, float a[7], float b[7], float cc[7],
                                              float dd[7]) {
@@ -283,16 +326,20 @@ __device__ __forceinline__ void march_column(const HydroGeom& g, MarchLDS& s, co
     int hx = 0, hy = 0;
     if (tid < MHALO) mhalo_pos(tid, hx, hy);
 
-    // window of the own column: planes k-1, k, k+1, k+2
-    float w0[7], w1[7], w2[7], w3[7], fz[7];
+    // window of the own column: the limited z slope of plane k and the
+    // primitives of planes k, k+1, k+2 (face k+1/2 needs the slopes of k and
+    // k+1, and each slope serves the two z faces of its cell)
+    float sk[7], w1[7], w2[7], w3[7], fz[7];
     {
-        float a[7];
+        float a[7], w0[7], s0[7];
         load_cell_f<INNER>(g, qin, gx, gy, z0 - 2, ks, a);
         load_cell_f<INNER>(g, qin, gx, gy, z0 - 1, ks, w0);
         load_cell_f<INNER>(g, qin, gx, gy, z0, ks, w1);
         load_cell_f<INNER>(g, qin, gx, gy, z0 + 1, ks, w2);
+        cell_slope<ALB>(g, m, a, w0, w1, s0);
+        cell_slope<ALB>(g, m, w0, w1, w2, sk);
         bool bad = false;
-        if (col) bad = recon_flux<SOLVER, ALB, 2>(g, m, a, w0, w1, w2, gc, fz);   // face z0 - 1/2
+        if (col) bad = slope_flux<SOLVER, 2>(g, m, w0, s0, w1, sk, gc, fz);   // face z0 - 1/2
         if (bad) set_error(st, STAGE == 1 ? ERR_RIEMANN_NAN_1 : ERR_RIEMANN_NAN_2);
         load_cell_f<INNER>(g, qin, gx, gy, z0 + 2, ks, w3);
     }
@@ -318,16 +365,20 @@ __device__ __forceinline__ void march_column(const HydroGeom& g, MarchLDS& s, co
         // z face k + 1/2 from registers (kept in LDS until the update), x and
         // y faces of plane k from LDS
         {
-            float fzn[7];
+            float fzn[7], sn[7];
+            cell_slope<ALB>(g, m, w1, w2, w3, sn);
             bool bad = false;
-            if (col) bad = recon_flux<SOLVER, ALB, 2>(g, m, w0, w1, w2, w3, gc, fzn);
+            if (col) bad = slope_flux<SOLVER, 2>(g, m, w1, sk, w2, sn, gc, fzn);
             else {
 #pragma unroll
                 for (int v = 0; v < 7; ++v) fzn[v] = 0.f;
             }
             if (bad) set_error(st, STAGE == 1 ? ERR_RIEMANN_NAN_1 : ERR_RIEMANN_NAN_2);
 #pragma unroll
-            for (int v = 0; v < 7; ++v) s.fz[v][tid] = fzn[v];
+            for (int v = 0; v < 7; ++v) {
+                s.fz[v][tid] = fzn[v];
+                sk[v] = sn[v];
+            }
         }
         march_xy_face<SOLVER, ALB, STAGE>(g, s, m, tid, ox, oy, st, gc);
         march_xy_face<SOLVER, ALB, STAGE>(g, s, m, tid + MTHREADS, ox, oy, st, gc);
@@ -369,7 +420,6 @@ __device__ __forceinline__ void march_column(const HydroGeom& g, MarchLDS& s, co
 #pragma unroll
         for (int v = 0; v < 7; ++v) {
             fz[v] = s.fz[v][tid];
-            w0[v] = w1[v];
             w1[v] = w2[v];
             w2[v] = w3[v];
         }

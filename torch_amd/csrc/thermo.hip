@@ -106,7 +106,8 @@ __device__ __forceinline__ bool thermo_init(const ThermoConsts& t, const double*
         const double dtdti = dt / dti;
         if (!(dtdti < 2147483647.0)) {
             // the reference's int conversion (:307) is undefined here
-            atomicOr(err, ERR_THERMO_NSTEPS);
+            if (err) atomicOr(err, ERR_THERMO_NSTEPS);
+            s.remaining = -1;
             return false;
         }
         int nsteps = dtdti - (int)dtdti > 0 ? (int)(dtdti + 1.0) : (int)(dtdti + 0.5);
@@ -233,7 +234,7 @@ __global__ void zero_u64_kernel(unsigned long long* p, int n) {
 // ---------------------------------------------------------------------------
 struct SurrogateConsts {
     float mu[8], isd[8];
-    double lo[6], hi[6];
+    double lo[7], hi[7];
     double gate;
     int nexact;
     int train;  // 1: queue every cell past nexact (no envelope), for training data
@@ -291,16 +292,18 @@ __device__ __forceinline__ double coarse_end_temperature(const ThermoConsts& t, 
     return p * s.pre2temp;
 }
 
-// Raw features u (code units): log10 rho, log10 T, x, log10 F_FUV, Av,
-// log10 dt, and from the FP64 prepass log10 of the subcycle count and
-// log10(Tc / T0) of the coarse end-temperature estimate.  The training
-// envelope applies to the first six.
+// Raw features u (code units): log10 rho, log10 T, x, log10 F_FUV (F_FUV
+// floored at 1 code unit, so that an unlit cell, F_FUV = 0, has the finite
+// feature 0; with the reference's unit scales 1 code unit is 1e-25
+// photons cm^-2 s^-1), Av, log10 dt, and from the FP64 prepass log10 of the
+// subcycle count and log10(Tc / T0) of the coarse end-temperature estimate.
+// The training envelope applies to the first seven.
 __device__ __forceinline__ void raw_features(const ThermoConsts& t, const double* in, long long stride, long long cell,
                                              const CellState& s, double Tc, double u[8]) {
     u[0] = log10(in[0 * stride + cell]);
     u[1] = log10(s.T0);
     u[2] = in[2 * stride + cell];
-    u[3] = log10(in[7 * stride + cell]);
+    u[3] = log10(fmax(in[7 * stride + cell], 1.0));
     u[4] = 1.086 * t.dust_xsec * in[6 * stride + cell];
     u[5] = log10(in[8 * stride + cell]);
     u[6] = log10((double)(s.remaining + 1));
@@ -339,9 +342,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
             } else {
                 Tc = coarse_end_temperature(t, sx, sy, sy2, s);
                 raw_features(t, in, n, cell, s, Tc, u);
+                // the bounds are [min, max] of the training set's features,
+                // which were stored as floats: compare the float-rounded
+                // features, so that a cell equal to a training sample is in
                 bool inside = true;
 #pragma unroll
-                for (int i = 0; i < 6; ++i) inside &= (u[i] >= sc.lo[i]) && (u[i] <= sc.hi[i]);
+                for (int i = 0; i < 7; ++i) {
+                    const float uf = (float)u[i];
+                    inside &= (uf >= (float)sc.lo[i]) && (uf <= (float)sc.hi[i]);
+                }
                 to_mlp = inside || sc.train;
                 to_fb = !to_mlp;
             }
@@ -630,6 +639,23 @@ __global__ __launch_bounds__(256) void lpt_bucket_kernel(ThermoConsts t, const d
     }
     __syncthreads();
     if (threadIdx.x < LPT_NB && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], (unsigned long long)h[threadIdx.x]);
+}
+
+// Subcycle count of each cell (the explicit steps integrate() takes after
+// the first, fixed by the pre-step rate): 0 for a cell that is not
+// subcycled, -1 where dt / dti is beyond the int range.  Nothing is
+// integrated and no error is raised.
+__global__ __launch_bounds__(256) void subcycle_count_kernel(ThermoConsts t, const double* __restrict__ tables,
+                                                            long long n, const double* __restrict__ in,
+                                                            long long* __restrict__ count) {
+    __shared__ double sx[26], sy[26], sy2[26];
+    load_tables(tables, sx, sy, sy2);
+    for (long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x; cell < n;
+         cell += (long long)gridDim.x * blockDim.x) {
+        CellState s;
+        thermo_init(t, sx, sy, sy2, in, n, cell, s, nullptr);
+        count[cell] = s.remaining;
+    }
 }
 
 // cursor[b] = number of items in buckets above b (longest first)
@@ -1005,11 +1031,22 @@ int torchgpu_thermo_cells_noalloc(torchgpu_cooling* c, int64_t n, const double* 
     return TORCHGPU_OK;
 }
 
+int torchgpu_thermo_subcycles(torchgpu_cooling* c, int64_t n, const double* in, int64_t* count) {
+    if (!c || n < 0 || (n > 0 && (!in || !count))) return TORCHGPU_ERR_INVALID_ARG;
+    if (n == 0) return TORCHGPU_OK;
+    long long nb = (n + 255) / 256;
+    if (nb > 65536) nb = 65536;
+    hipLaunchKernelGGL(tg::subcycle_count_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t, c->tables, n, in,
+                       (long long*)count);
+    HIP_OK(hipGetLastError());
+    return TORCHGPU_OK;
+}
+
 int torchgpu_cooling_set_surrogate(torchgpu_cooling* c, torchgpu_mlp* m, const torchgpu_surrogate_cfg* cfg) {
     if (!c || !cfg) return TORCHGPU_ERR_INVALID_ARG;
     c->mlp = m;
     for (int i = 0; i < 8; ++i) { c->sc.mu[i] = cfg->mu[i]; c->sc.isd[i] = cfg->isd[i]; }
-    for (int i = 0; i < 6; ++i) { c->sc.lo[i] = cfg->lo[i]; c->sc.hi[i] = cfg->hi[i]; }
+    for (int i = 0; i < 7; ++i) { c->sc.lo[i] = cfg->lo[i]; c->sc.hi[i] = cfg->hi[i]; }
     c->sc.gate = cfg->gate;
     c->sc.nexact = cfg->nexact;
     c->sc.train = 0;

@@ -23,8 +23,8 @@ NB = 4 * NH + NOUT
 
 
 class SurrogateCfg(ctypes.Structure):
-    _fields_ = [("mu", ctypes.c_float * 8), ("isd", ctypes.c_float * 8), ("lo", ctypes.c_double * 6),
-                ("hi", ctypes.c_double * 6), ("gate", ctypes.c_double), ("nexact", ctypes.c_int), ("pad", ctypes.c_int)]
+    _fields_ = [("mu", ctypes.c_float * 8), ("isd", ctypes.c_float * 8), ("lo", ctypes.c_double * 7),
+                ("hi", ctypes.c_double * 7), ("gate", ctypes.c_double), ("nexact", ctypes.c_int), ("pad", ctypes.c_int)]
 
 
 _bound = False
@@ -47,6 +47,7 @@ def _bind():
     L.torchgpu_cooling_set_surrogate.argtypes = [vp, vp, ctypes.POINTER(SurrogateCfg)]
     L.torchgpu_thermo_cells_surrogate.argtypes = [vp, i64, vp, vp, vp, ctypes.POINTER(i64)]
     L.torchgpu_surrogate_training_data.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, ctypes.POINTER(i64)]
+    L.torchgpu_thermo_subcycles.argtypes = [vp, i64, vp, vp]
     _bound = True
 
 
@@ -150,13 +151,15 @@ def set_surrogate(cool: Cooling, mlp: MLP, cfg: SurrogateCfg):
 
 
 def make_cfg(mu, isd, lo, hi, gate=0.01, nexact=4) -> SurrogateCfg:
+    """Configuration with the training envelope [lo, hi] over the first seven
+    raw features; bounds past the given ones (a file with six) are open."""
     c = SurrogateCfg()
     for i in range(8):
         c.mu[i] = float(mu[i]) if i < len(mu) else 0.0
         c.isd[i] = float(isd[i]) if i < len(isd) else 0.0
-    for i in range(6):
-        c.lo[i] = float(lo[i])
-        c.hi[i] = float(hi[i])
+    for i in range(7):
+        c.lo[i] = float(lo[i]) if i < len(lo) else -math.inf
+        c.hi[i] = float(hi[i]) if i < len(hi) else math.inf
     c.gate = gate
     c.nexact = nexact
     return c
@@ -177,6 +180,21 @@ def thermo_cells_surrogate(cool: Cooling, inp, out=None, path=None):
     _check(lib().torchgpu_thermo_cells_surrogate(cool._h, n, inp.data_ptr(), out.data_ptr(), path.data_ptr(), counts),
            "thermo_cells_surrogate")
     return out, path, (counts[0], counts[1])
+
+
+def subcycles(cool: Cooling, inp):
+    """Per-cell subcycle count (device int64 [n]) of [9][n] device inputs:
+    the explicit steps the FP64 integrator would take after the first, 0 for
+    a cell that is not subcycled, -1 past the int range
+    (torchgpu_thermo_subcycles; nothing is integrated)."""
+    import torch
+
+    _bind()
+    n = inp.shape[1]
+    assert inp.dtype == torch.float64 and inp.is_cuda and inp.is_contiguous() and inp.shape[0] == 9
+    cnt = torch.empty(n, dtype=torch.int64, device=inp.device)
+    _check(lib().torchgpu_thermo_subcycles(cool._h, n, inp.data_ptr(), cnt.data_ptr()), "thermo_subcycles")
+    return cnt
 
 
 def training_data(cool: Cooling, inp, out_fp64):
