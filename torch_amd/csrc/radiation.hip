@@ -4,7 +4,7 @@
 //   rad_gather_kernel        Q = fixPrimitives(QfromU(U)) of den, pre, hii, adv
 //                            (+ Torch::checkValues before a sub-step)
 //   rad_init_field_kernel    Radiation::initField          (Radiation.cpp:78-96)
-//   rad_heat_kernel          preTimeStepCalculations        (:257-296)
+//   rad_gather_heat_kernel   the gather + preTimeStepCalculations (:257-296) in one pass
 //   rad_dt_kernel            calculateTimeStep              (:298-351), block min + one atomic
 //   rad_sweep_kernel<IMPL>   transferRadiation's causal loop (:569-651) for the implicit
 //                            scheme (IMPL) / Radiation::rayTrace (:480-508) for implicit2 and
@@ -129,18 +129,20 @@ __device__ __forceinline__ void rad_doric(double dt, double& x_avg, double& x, d
     }
 }
 
-// Radiation::update_HIIfrac (Radiation.cpp:375-478) for a non-wind cell;
-// returns false when the implicit iteration fails.
-__device__ __forceinline__ bool rad_update_hii(const RadConsts& r, const double* tab, long long n, long long c,
-                                               double ds, double sv, double dt, double* q4, double* R) {
-    const double den = q4[c], pre = q4[n + c], q_hii = q4[2 * n + c];
+// Radiation::update_HIIfrac (Radiation.cpp:375-478) for a non-wind cell on
+// values in registers: the cell's density, pressure and HII fraction, its
+// TAU (explicit scheme: also DTAU) and the time-averaged optical depth of
+// the implicit schemes; returns the new HII fraction x and (implicit
+// schemes) the time-averaged x_avg, false when the implicit iteration fails.
+__device__ __forceinline__ bool rad_update_hii_v(const RadConsts& r, const double* tab, double den, double pre,
+                                                 double q_hii, double tau, double dtau, double tau_avg, double ds,
+                                                 double sv, double dt, double& x_out, double& x_avg_out) {
     const double n_H = r.X * den / r.mh;
     double x = q_hii, x_avg = x;
     const double T = rad_temperature(r, q_hii, pre, den);
     const double alphaB = rad_alpha(r, tab, T);
     const double A_ci = rad_collisional(r, T);
     if (r.scheme != TORCHGPU_RT_EXPLICIT) {
-        const double tau_avg = r.scheme == TORCHGPU_RT_IMPLICIT2 ? R[R_TAU * n + c] : R[R_TAU_A * n + c];
         const double em_tau_avg = exp(-tau_avg);
         int niter = 0, miter = 0;
         bool converged = false;
@@ -162,24 +164,68 @@ __device__ __forceinline__ bool rad_update_hii(const RadConsts& r, const double*
             }
             if (miter > 5 || x != x) return false;
         }
-        R[R_HII_A * n + c] = x_avg;
     } else {
         const double nHI = (1.0 - x) * (r.X * den / r.mh);
-        const double A_pi = rad_photoion(nHI, R[R_TAU * n + c], R[R_DTAU * n + c], sv, r.photon_rate);
+        const double A_pi = rad_photoion(nHI, tau, dtau, sv, r.photon_rate);
         const double nHII_aB = x_avg * n_H * alphaB;
         const double nHII_Aci = x_avg * n_H * A_ci;
         double dummy = x;
         rad_doric(dt, x, dummy, A_pi, nHII_aB, nHII_Aci);
     }
+    x_out = x;
+    x_avg_out = x_avg;
+    return true;
+}
+
+// The same on the cell's stored state: reads q4 and R, writes HII_A
+// (implicit schemes), the HII fraction and the ADV flag.
+__device__ __forceinline__ bool rad_update_hii(const RadConsts& r, const double* tab, long long n, long long c,
+                                               double ds, double sv, double dt, double* q4, double* R) {
+    const bool expl = r.scheme == TORCHGPU_RT_EXPLICIT;
+    const double tau_avg = expl ? 0.0 : (r.scheme == TORCHGPU_RT_IMPLICIT2 ? R[R_TAU * n + c] : R[R_TAU_A * n + c]);
+    const double tau = expl ? R[R_TAU * n + c] : 0.0, dtau = expl ? R[R_DTAU * n + c] : 0.0;
+    double x, x_avg;
+    if (!rad_update_hii_v(r, tab, q4[c], q4[n + c], q4[2 * n + c], tau, dtau, tau_avg, ds, sv, dt, x, x_avg))
+        return false;
+    if (!expl) R[R_HII_A * n + c] = x_avg;
     q4[2 * n + c] = x;
     if (x > 0.001) q4[3 * n + c] = 1;
     return true;
 }
 
-__device__ __forceinline__ void cell_ijk(const StarGeom& s, long long c, int ijk[3]) {
-    ijk[0] = (int)(c % s.n[0]);
-    ijk[1] = (int)((c / s.n[0]) % s.n[1]);
-    ijk[2] = (int)(c / ((long long)s.n[0] * s.n[1]));
+__device__ __forceinline__ void cell_ijk(const StarGeom& s, long long c, int ijk[3]) { cell_ijk32(s, c, ijk); }
+
+// The spline tables into LDS for the block (every thread must call it): the
+// bisection of lin_spline is a chain of dependent table reads.
+__device__ __forceinline__ const double* stage_tables(const double* __restrict__ tab, double* lds) {
+    for (int i = threadIdx.x; i < TAB_N; i += blockDim.x) lds[i] = tab[i];
+    __syncthreads();
+    return lds;
+}
+
+// Q = fixPrimitives(QfromU(U)) of cell c into q4 (and q), after
+// Torch::checkValues when `check`.
+__device__ __forceinline__ void rad_gather_cell(const HydroGeom& g, const double* __restrict__ U,
+                                                double* __restrict__ q4, StepState* st, int check, int check_old,
+                                                long long n, long long c, double q[7]) {
+    double u[7];
+#pragma unroll
+    for (int v = 0; v < 7; ++v) u[v] = U[v * n + c];
+    if (check) {
+        // Torch::checkValues: U finite, Fluid's current Q (the previous
+        // gather's) with nonzero density and pressure
+        bool bad = false;
+#pragma unroll
+        for (int v = 0; v < 7; ++v) bad = bad || u[v] != u[v] || isinf(u[v]);
+        if (check_old) bad = bad || q4[c] == 0 || q4[n + c] == 0;
+        if (bad) set_error(st, ERR_RAD_CHECK);
+    }
+    QfromU(q, u, g.gm1);
+    fix_primitives(g, q, u[DEN]);
+    q4[c] = q[DEN];
+    q4[n + c] = q[PRE];
+    q4[2 * n + c] = q[HII];
+    q4[3 * n + c] = q[ADV];
 }
 
 __global__ void rad_gather_kernel(HydroGeom g, const double* __restrict__ U, double* __restrict__ q4, StepState* st,
@@ -187,24 +233,8 @@ __global__ void rad_gather_kernel(HydroGeom g, const double* __restrict__ U, dou
     if (gated && st->halted) return;
     const long long n = g.ncell;
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
-        double u[7], q[7];
-#pragma unroll
-        for (int v = 0; v < 7; ++v) u[v] = U[v * n + c];
-        if (check) {
-            // Torch::checkValues: U finite, Fluid's current Q (the previous
-            // gather's) with nonzero density and pressure
-            bool bad = false;
-#pragma unroll
-            for (int v = 0; v < 7; ++v) bad = bad || u[v] != u[v] || isinf(u[v]);
-            if (check_old) bad = bad || q4[c] == 0 || q4[n + c] == 0;
-            if (bad) set_error(st, ERR_RAD_CHECK);
-        }
-        QfromU(q, u, g.gm1);
-        fix_primitives(g, q, u[DEN]);
-        q4[c] = q[DEN];
-        q4[n + c] = q[PRE];
-        q4[2 * n + c] = q[HII];
-        q4[3 * n + c] = q[ADV];
+        double q[7];
+        rad_gather_cell(g, U, q4, st, check, check_old, n, c, q);
     }
 }
 
@@ -251,16 +281,16 @@ __global__ void rad_init_field_kernel(HydroGeom g, StarGeom s, RadConsts r, cons
 
 // heat (may be null): the grid's kept GridCell::H terms, whose RHII and
 // EUVH this pre-step sets (Radiation.cpp:278-279).
-__global__ void rad_heat_kernel(StarGeom s, RadConsts r, const double* __restrict__ tab, const double* __restrict__ q4,
-                                double* __restrict__ R, StepState* st, int gated, double* __restrict__ heat) {
-    if (gated && st->halted) return;
-    const long long n = (long long)s.n[0] * s.n[1] * s.n[2];
-    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
-        int ijk[3];
-        cell_ijk(s, c, ijk);
-        if (star_wind_cell(s, ijk)) continue;
+// preTimeStepCalculations (Radiation.cpp:257-296) of cell c from its
+// density, pressure and HII fraction.
+__device__ __forceinline__ void rad_heat_cell(const StarGeom& s, const RadConsts& r, const double* tab, long long n,
+                                              long long c, double den, double pre, double hii, double* __restrict__ R,
+                                              StepState* st, double* __restrict__ heat) {
+    int ijk[3];
+    cell_ijk(s, c, ijk);
+    if (star_wind_cell(s, ijk)) return;
+    {
         const double sv = star_shell_volume(s, ijk, star_path_length(s, ijk));
-        const double den = q4[c], pre = q4[n + c], hii = q4[2 * n + c];
         const double n_H = (r.X * den / r.mh);
         const double T = rad_temperature(r, hii, pre, den);
         const double nHI = (1.0 - hii) * n_H;
@@ -281,7 +311,23 @@ __global__ void rad_heat_kernel(StarGeom s, RadConsts r, const double* __restric
     }
 }
 
-__global__ __launch_bounds__(256) void rad_dt_kernel(StarGeom s, RadConsts r, const double* __restrict__ tab,
+// The gather and preTimeStepCalculations in one pass (the heating reads the
+// primitives the gather just formed, from registers).
+__global__ void rad_gather_heat_kernel(HydroGeom g, StarGeom s, RadConsts r, const double* __restrict__ gtab,
+                                       const double* __restrict__ U, double* __restrict__ q4, double* __restrict__ R,
+                                       StepState* st, int check, int check_old, int gated, double* __restrict__ heat) {
+    if (gated && st->halted) return;
+    __shared__ double stab[TAB_N];
+    const double* tab = stage_tables(gtab, stab);
+    const long long n = g.ncell;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+        double q[7];
+        rad_gather_cell(g, U, q4, st, check, check_old, n, c, q);
+        rad_heat_cell(s, r, tab, n, c, q[DEN], q[PRE], q[HII], R, st, heat);
+    }
+}
+
+__global__ __launch_bounds__(256) void rad_dt_kernel(StarGeom s, RadConsts r, const double* __restrict__ gtab,
                                                      const double* __restrict__ q4, const double* __restrict__ U,
                                                      const double* __restrict__ R, double dt_max,
                                                      unsigned long long* dtbits, StepState* st, unsigned long long* once,
@@ -290,6 +336,8 @@ __global__ __launch_bounds__(256) void rad_dt_kernel(StarGeom s, RadConsts r, co
     // in the device loop: no time step is chosen on the first step or after
     // the run halted, and dt_max is the run's
     if (gated && (st->halted || st->first)) return;
+    __shared__ double stab[TAB_N];
+    const double* tab = stage_tables(gtab, stab);
     if (gated) dt_max = st->dt_max;
     const long long n = (long long)s.n[0] * s.n[1] * s.n[2];
     double m = dt_max;
@@ -360,12 +408,14 @@ __device__ __forceinline__ long long rad_xface_index(const StarGeom& s, const Ra
 // finite tau), so it is not read, and x-face cells read their upstream
 // neighbours from the previous shell's x-face copies (`fast`).
 template <bool IMPL>
-__global__ __launch_bounds__(256) void rad_sweep_kernel(StarGeom s, RadConsts r, const double* __restrict__ tab,
+__global__ __launch_bounds__(256) void rad_sweep_kernel(StarGeom s, RadConsts r, const double* __restrict__ gtab,
                                                         ShellFaces f, int k, RadXFaces xf, double dt, double* q4,
                                                         double* R, StepState* st) {
+    if (dt < 0.0 && st->halted) return;  // a DT_* code: the device loop's step
+    __shared__ double stab[IMPL ? TAB_N : 1];
+    const double* tab = IMPL ? stage_tables(gtab, stab) : gtab;
     const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
     if (t >= f.start[f.nf]) return;
-    if (dt < 0.0 && st->halted) return;  // a DT_* code: the device loop's step
     dt = stage_dtf(dt, st, 0);
     const long long n = (long long)s.n[0] * s.n[1] * s.n[2];
     int ijk[3];
@@ -384,6 +434,12 @@ __global__ __launch_bounds__(256) void rad_sweep_kernel(StarGeom s, RadConsts r,
         R[R_HII_A * n + c] = 1;
         out[0] = out[1] = out[2] = out[3] = 0.0;
     } else {
+        // the cell's own state, read once and first, so that its (for
+        // x-face cells strided) loads overlap the neighbour reads below
+        const double den = q4[c];
+        const double pre = IMPL ? q4[n + c] : 0.0;
+        const double q_hii = q4[2 * n + c];
+        const double hii_a_old = R[R_HII_A * n + c];
         double dist2 = 0;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -425,13 +481,26 @@ __global__ __launch_bounds__(256) void rad_sweep_kernel(StarGeom s, RadConsts r,
             out[2 * pass] = v;
         }
         const double ds = star_path_length(s, ijk);
+        // the new HII values stay in registers
+        double hii = q_hii, hii_a = hii_a_old;
         if (IMPL) {
+            // the implicit scheme: update_HIIfrac with tau_avg = TAU_A (just
+            // computed above); a failed iteration leaves the state as it was
             const double sv = star_shell_volume(s, ijk, ds);
-            if (!rad_update_hii(r, tab, n, c, ds, sv, dt, q4, R)) set_error(st, ERR_RAD_ITER);
+            double x, x_avg;
+            if (rad_update_hii_v(r, tab, den, pre, q_hii, 0.0, 0.0, out[2], ds, sv, dt, x, x_avg)) {
+                hii = x;
+                hii_a = x_avg;
+                R[R_HII_A * n + c] = hii_a;
+                q4[2 * n + c] = hii;
+                if (hii > 0.001) q4[3 * n + c] = 1;
+            } else {
+                set_error(st, ERR_RAD_ITER);
+            }
         }
-        const double nH = r.X * q4[c] / r.mh;
-        out[1] = ((1.0 - q4[2 * n + c]) * nH) * r.sigma * ds;
-        out[3] = ((1.0 - R[R_HII_A * n + c]) * nH) * r.sigma * ds;
+        const double nH = r.X * den / r.mh;
+        out[1] = ((1.0 - hii) * nH) * r.sigma * ds;
+        out[3] = ((1.0 - hii_a) * nH) * r.sigma * ds;
         R[R_DTAU * n + c] = out[1];
         R[R_DTAU_A * n + c] = out[3];
     }
@@ -444,9 +513,11 @@ __global__ __launch_bounds__(256) void rad_sweep_kernel(StarGeom s, RadConsts r,
     }
 }
 
-__global__ void rad_hii_kernel(StarGeom s, RadConsts r, const double* __restrict__ tab, double dt, double* q4,
+__global__ void rad_hii_kernel(StarGeom s, RadConsts r, const double* __restrict__ gtab, double dt, double* q4,
                                double* R, StepState* st) {
     if (dt < 0.0 && st->halted) return;
+    __shared__ double stab[TAB_N];
+    const double* tab = stage_tables(gtab, stab);
     dt = stage_dtf(dt, st, 0);
     const long long n = (long long)s.n[0] * s.n[1] * s.n[2];
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
@@ -573,11 +644,13 @@ int rad_gather(torchgpu_radiation* r, int check, int gated) {
     return TORCHGPU_OK;
 }
 
-int rad_heat(torchgpu_radiation* r, int gated) {
+// rad_gather then rad_heat, fused.
+int rad_gather_heat(torchgpu_radiation* r, int check, int gated) {
     tg::CtxView v = tg::ctx_view(r->ctx);
-    hipLaunchKernelGGL(tg::rad_heat_kernel, dim3(grid_blocks(v.g->ncell)), dim3(256), 0, r->stream, r->star, r->rc,
-                       r->tables, r->q4, r->R, v.st, gated, v.heat);
+    hipLaunchKernelGGL(tg::rad_gather_heat_kernel, dim3(grid_blocks(v.g->ncell)), dim3(256), 0, r->stream, *v.g,
+                       r->star, r->rc, r->tables, v.U, r->q4, r->R, v.st, check, (int)r->q_valid, gated, v.heat);
     RAD_OK(hipGetLastError());
+    r->q_valid = true;
     return TORCHGPU_OK;
 }
 
@@ -590,9 +663,7 @@ const unsigned long long* rad_dt_bits(const torchgpu_radiation* r) { return r->d
 
 // Radiation::preTimeStepCalculations on the fixed primitives of U.
 int rad_pre_step_dev(torchgpu_radiation* r, int gated) {
-    int rc = rad_gather(r, 0, gated);
-    if (!rc) rc = rad_heat(r, gated);
-    return rc;
+    return rad_gather_heat(r, 0, gated);
 }
 
 // Radiation::calculateTimeStep's minimum into dtbits[0] (DBL_MAX when no
@@ -625,7 +696,7 @@ int rad_substep_dev(torchgpu_radiation* r, double dt, int has_heat) {
     if (!has_heat || !r->q_valid) {
         // checkValues on the current U and Q, then Q and
         // preTimeStepCalculations anew
-        if ((rc = rad_gather(r, 1, gated)) || (rc = rad_heat(r, gated))) return rc;
+        if ((rc = rad_gather_heat(r, 1, gated))) return rc;
     } else {
         hipLaunchKernelGGL(rad_check_kernel, dim3(grid_blocks(nc)), dim3(256), 0, r->stream, *v.g, v.U, r->q4, v.st,
                            gated);
@@ -688,6 +759,7 @@ int torchgpu_radiation_create(const torchgpu_rad_params* p, torchgpu_ctx* ctx, c
     for (int a = 0; a < 3; ++a)
         if (star->cell[a] < 0 || star->cell[a] >= n[a]) return TORCHGPU_ERR_INVALID_ARG;
     if (star->wind_cell_radius < 0 || !(star->photon_rate >= 0.0)) return TORCHGPU_ERR_INVALID_ARG;
+    if (g.ncell >= (1LL << 32)) return TORCHGPU_ERR_INVALID_ARG;  // 32-bit cell index math (cell_ijk32)
     torchgpu_radiation* r = new torchgpu_radiation();
     r->ctx = ctx;
     r->stream = v.stream;

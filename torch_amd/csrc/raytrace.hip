@@ -53,6 +53,9 @@ __global__ __launch_bounds__(256) void coldens_shell_kernel(StarGeom s, ShellFac
     const long long c = star_flat(s, ijk[0], ijk[1], ijk[2]);
     const bool xface = ijk[0] - s.s[0] == k || s.s[0] - ijk[0] == k;
     const int side = ijk[0] > s.s[0] ? 1 : 0;
+    // the cell's density first: for x-face cells a strided load, whose
+    // latency then overlaps the neighbour reads
+    const double den_c = den[c];
     // distance to the star in cell units
     double dist2 = 0;
 #pragma unroll
@@ -86,7 +89,7 @@ __global__ __launch_bounds__(256) void coldens_shell_kernel(StarGeom s, ShellFac
             for (int q = 0; q < 4; ++q) v += (wr[q] / sum_w) * cd[q];
         }
     }
-    const double dc = (den[c] / s.hydrogen_mass) * star_path_length(s, ijk);
+    const double dc = (den_c / s.hydrogen_mass) * star_path_length(s, ijk);
     col[c] = v;
     dcol[c] = dc;
     if (xface) {

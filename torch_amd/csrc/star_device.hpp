@@ -164,14 +164,28 @@ __device__ __forceinline__ bool star_wind_cell(const StarGeom& s, const int ijk[
 
 // Cell t of a shell's enumeration (face by face, the face's first axis
 // fastest).
+// A face holds at most n^2 < 2^32 cells, so the position within it is split
+// with 32-bit unsigned division (the 64-bit one is a long call chain).
 __device__ __forceinline__ void shell_cell(const ShellFaces& f, long long t, int ijk[3]) {
     int q = 0;
     while (q + 1 < f.nf && t >= f.start[q + 1]) ++q;
-    const long long l = t - f.start[q];
+    const unsigned int l = (unsigned int)(t - f.start[q]);
+    const unsigned int w = (unsigned int)f.w0[q];
+    const unsigned int l1 = l / w;
     const int a = f.axis[q];
     ijk[a] = f.fixed[q];
-    ijk[a == 0 ? 1 : 0] = f.lo0[q] + (int)(l % f.w0[q]);
-    ijk[a == 2 ? 1 : 2] = f.lo1[q] + (int)(l / f.w0[q]);
+    ijk[a == 0 ? 1 : 0] = f.lo0[q] + (int)(l - l1 * w);
+    ijk[a == 2 ? 1 : 2] = f.lo1[q] + (int)l1;
+}
+
+// Indices of flat cell c of a grid with fewer than 2^32 cells (32-bit
+// unsigned division; the handles that call it check the grid size).
+__device__ __forceinline__ void cell_ijk32(const StarGeom& s, long long c, int ijk[3]) {
+    const unsigned int u = (unsigned int)c, n0 = (unsigned int)s.n[0], n1 = (unsigned int)s.n[1];
+    const unsigned int r = u / n0, k = r / n1;
+    ijk[0] = (int)(u - r * n0);
+    ijk[1] = (int)(r - k * n1);
+    ijk[2] = (int)k;
 }
 
 __device__ __forceinline__ double star_fuv(const StarGeom& s, const int ijk[3]) {

@@ -1301,6 +1301,10 @@ int torchgpu_cooling_set_star(torchgpu_cooling* c, torchgpu_ctx* ctx, const torc
             return TORCHGPU_ERR_INVALID_ARG;
         }
     if (star->wind_cell_radius < 0 || !(star->photon_rate >= 0.0)) return TORCHGPU_ERR_INVALID_ARG;
+    if (g.ncell >= (1LL << 32)) {  // 32-bit index math of the shell sweeps
+        snprintf(c->msg, sizeof c->msg, "star operators need fewer than 2^32 cells");
+        return TORCHGPU_ERR_INVALID_ARG;
+    }
     const long long ncell = g.ncell;
     if (ncell > c->dcol_cap) {
         (void)hipStreamSynchronize(c->stream);
