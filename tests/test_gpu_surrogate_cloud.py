@@ -76,3 +76,38 @@ def test_cloud_grid_at_100x_dt_goes_to_the_network(gpu):
         assert r["rel_T_err_mlp_cells"]["median"] <= 0.01 and r["rel_T_err_mlp_cells"]["p99"] <= 0.05, r
     g.close()
     del mlp
+
+
+@pytest.mark.skipif(not os.path.exists(WEIGHTS), reason="cloud surrogate weights not trained yet")
+def test_grid_substep_forms_inputs_in_the_prepass(gpu):
+    """A surrogate thermo sub-step on the grid forms each cell's inputs in the
+    prepass from U (writing the gathered array only for the cells it queues);
+    with the heating record on it gathers the whole array first.  Both give
+    the same state bit for bit (mixed and FP64 grids), at the loop's dt and
+    at 100x it, where cells go to the network and the FP64 fallback, and the
+    thermo time step agrees too."""
+    import torch
+
+    from torch_amd import cloud
+    from torch_amd import surrogate as sg
+
+    for precision in ("mixed", "fp64"):
+        states, dts = [], []
+        for track in (False, True):
+            p, g, cool, thermo = cloud.build(48, precision)
+            if track:
+                g.heating_track(True)
+            mlp = sg.attach(cool, WEIGHTS, gate=0.01)
+            cool.run_split(g, 6, surrogate=True, **thermo)
+            g.synchronize()
+            t, dt, _ = g.clock()
+            dts.append(cool.grid_timestep(g, p.dt_max, **thermo))
+            for mult in (1.0, 100.0):
+                cool.grid_substep(g, mult * dt, surrogate=True, **thermo)
+            g.stage_dt(0, 0.5 * dt)   # a hydro sub-step on the float Q the cooling wrote
+            g.synchronize()
+            states.append(g.get_conserved())
+            g.close()
+            del mlp
+        assert dts[0] == dts[1], (precision, dts)
+        assert np.array_equal(states[0], states[1]), precision

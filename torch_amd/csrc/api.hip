@@ -140,6 +140,10 @@ const double* tg::ctx_print_state_dev(torchgpu_ctx* c) { return print_state(c) =
 tg::CtxView tg::ctx_view(torchgpu_ctx* c) { return tg::CtxView{c->U, c->Ustar, c->st, c->stream, &c->g, c->heat}; }
 
 void tg::ctx_state_changed(torchgpu_ctx* c) { c->qvalid = false; }
+float* tg::ctx_qf_for_state(torchgpu_ctx* c) {
+    return c->p.precision == TORCHGPU_FP64 ? nullptr : c->qf[c->qcur];
+}
+void tg::ctx_qf_written(torchgpu_ctx* c) { c->qvalid = true; }
 
 namespace {
 

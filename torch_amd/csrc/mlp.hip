@@ -304,6 +304,9 @@ __global__ __launch_bounds__(INFER_THREADS) void mlp_infer_kernel(const bf16_t* 
                                                        float* __restrict__ y) {
     __shared__ InferLDS s;
     if (n_dev) n = (long long)*n_dev;
+    // a block with no cells (the grid is sized for the largest queue) leaves
+    // before staging the weights
+    if ((long long)blockIdx.x * (INFER_THREADS / 64) * 32 >= n) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // weight blob layout (bf16): W1[128][8] | W2 | W3 | W4 [128][128] | W5[NOUT][128]
     for (int i = tid; i < MLP_H * 8; i += INFER_THREADS) s.w1[i / 8][i % 8] = wts[i];

@@ -181,8 +181,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void t
                                                           unsigned long long* __restrict__ counters,
                                                           unsigned int* __restrict__ err, unsigned char* path) {
     __shared__ double sx[26], sy[26], sy2[26];
-    load_tables(tables, sx, sy, sy2);
     if (count_dev) count = (long long)*count_dev;
+    if (count <= 0) return;  // an empty queue (the grid is sized for a full one)
+    load_tables(tables, sx, sy, sy2);
     const int lane = threadIdx.x & 63;
     bool active = false, exhausted = false;
     long long cell = 0;
@@ -310,18 +311,79 @@ __device__ __forceinline__ void raw_features(const ThermoConsts& t, const double
     u[7] = log10(Tc / s.T0);
 }
 
-// counters: [2] surrogate queue length, [3] fallback queue length
+// The cooling operator's nine inputs of a grid cell (the column order of the
+// cooling kernels: den, pre, hii, adv, E, T_min, col_den, F_FUV, dt) from the
+// grid state: Q = QfromU(U) with fixPrimitives' clamps and floors, ADV = -1
+// for the excluded cell and, with a star, its CausalWind cells, F_FUV from
+// the star when no field is given.
+struct GridSrc {
+    HydroGeom g;
+    const double* U;
+    const double* T_min;     // NULL: T_min_default
+    double T_min_default;
+    const double* col_den;   // NULL: 0
+    const double* F;         // NULL: the star's F_FUV, or 0
+    double dt;               // >= 0, or DT_HALF / DT_FULL of the step in flight
+    const StepState* st;
+    long long excluded;
+    StarGeom star;
+};
+
+__device__ __forceinline__ double grid_dt(const GridSrc& gs) {
+    // DT_HALF / DT_FULL: the sub-step of the step in flight
+    return gs.dt < 0.0 ? ((gs.dt == DT_HALF) ? 0.5 * gs.st->dt : gs.st->dt) : gs.dt;
+}
+
+__device__ __forceinline__ void gather_cell(const GridSrc& gs, long long c, double dt, double in[9]) {
+    const HydroGeom& g = gs.g;
+    const long long n = g.ncell;
+    double u[7], q[7];
+#pragma unroll
+    for (int v = 0; v < 7; ++v) u[v] = gs.U[v * n + c];
+    QfromU(q, u, g.gm1);
+    q[HII] = smax_d(smin_d(q[HII], 1.0), 0.0);
+    q[ADV] = smax_d(smin_d(q[ADV], 1.0), 0.0);
+    q[DEN] = smax_d(q[DEN], g.dfloor);
+    q[PRE] = smax_d(q[PRE], g.pfloor);
+    const double mu_inv = g.X * (q[HII] + 1.0) + g.mu_c;
+    const double temperature = q[PRE] / (mu_inv * g.Rgas * q[DEN]);
+    if (temperature < g.tfloor) q[PRE] = mu_inv * g.Rgas * u[DEN] * g.tfloor;
+    bool skip = c == gs.excluded;
+    double f = gs.F ? gs.F[c] : 0.0;
+    if (gs.star.on) {
+        int ijk[3];
+        cell_ijk32(gs.star, c, ijk);
+        skip = skip || star_wind_cell(gs.star, ijk);
+        if (!gs.F) f = star_fuv(gs.star, ijk);
+    }
+    in[0] = q[DEN];
+    in[1] = q[PRE];
+    in[2] = q[HII];
+    in[3] = skip ? -1.0 : q[ADV];
+    in[4] = u[PRE];
+    in[5] = gs.T_min ? gs.T_min[c] : gs.T_min_default;
+    in[6] = gs.col_den ? gs.col_den[c] : 0.0;
+    in[7] = f;
+    in[8] = dt;
+}
+
+// counters: [2] surrogate queue length, [3] fallback queue length.
+// GRID: the inputs of each cell are formed from the grid state (gs, as
+// thermo_gather_kernel) in registers, and written to `in` only for the
+// cells queued for the network or the fallback, which read them later.
+template <bool GRID>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void surrogate_prepass_kernel(ThermoConsts t, const double* __restrict__ tables,
                                                                SurrogateConsts sc, long long n,
-                                                               const double* __restrict__ in, double* __restrict__ out,
+                                                               double* __restrict__ in, double* __restrict__ out,
                                                                unsigned char* path, long long* __restrict__ mlp_list,
                                                                float* __restrict__ feat, SurrCell* __restrict__ sdata,
                                                                float* __restrict__ raw_out,
                                                                long long* __restrict__ fb_list,
                                                                unsigned long long* __restrict__ counters,
-                                                               unsigned int* __restrict__ err) {
+                                                               unsigned int* __restrict__ err, GridSrc gs) {
     __shared__ double sx[26], sy[26], sy2[26];
     load_tables(tables, sx, sy, sy2);
+    const double gdt = GRID ? grid_dt(gs) : 0.0;
     const long long nloop = ((n + blockDim.x - 1) / blockDim.x) * blockDim.x;  // uniform trip count per wave
     for (long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x; cell - threadIdx.x < nloop;
          cell += (long long)gridDim.x * blockDim.x) {
@@ -330,8 +392,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
         CellState s;
         double u[8];
         double Tc = 0.0;
+        double lin[9];
+        const double* cin = in;  // this cell's inputs: cin[k * cstride + ccell]
+        long long cstride = n, ccell = cell;
+        if (GRID && valid) {
+            gather_cell(gs, cell, gdt, lin);
+            cin = lin;
+            cstride = 1;
+            ccell = 0;
+        }
         if (valid) {
-            bool active = thermo_init(t, sx, sy, sy2, in, n, cell, s, err);
+            bool active = thermo_init(t, sx, sy, sy2, cin, cstride, ccell, s, err);
             if (active && s.remaining < sc.nexact) {
                 while (s.remaining > 0) thermo_substep(t, sx, sy, sy2, s);
                 active = false;
@@ -341,7 +412,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
                 if (path) path[cell] = 0;
             } else {
                 Tc = coarse_end_temperature(t, sx, sy, sy2, s);
-                raw_features(t, in, n, cell, s, Tc, u);
+                raw_features(t, cin, cstride, ccell, s, Tc, u);
                 // the bounds are [min, max] of the training set's features,
                 // which were stored as floats: compare the float-rounded
                 // features, so that a cell equal to a training sample is in
@@ -353,6 +424,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
                 }
                 to_mlp = inside || sc.train;
                 to_fb = !to_mlp;
+                if (GRID) {
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) in[k * n + cell] = lin[k];
+                }
             }
         }
         const long long slot = wave_append(to_mlp, &counters[2]);
@@ -452,51 +527,26 @@ __global__ void surrogate_targets_kernel(const unsigned long long* __restrict__ 
 // (Fluid::initialiseGrid, Fluid.cpp:41-54), which the reference's
 // preTimeStepCalculations skips, and F_FUV is the star's when no array is
 // given (Thermodynamics.cpp:237-243).
-__global__ void thermo_gather_kernel(HydroGeom g, const double* __restrict__ U, const double* __restrict__ T_min,
-                                     double T_min_default, const double* __restrict__ col_den,
-                                     const double* __restrict__ F, double dt, const StepState* st,
-                                     long long excluded, StarGeom star, double* __restrict__ in9) {
-    const long long n = g.ncell;
-    // DT_HALF / DT_FULL: the sub-step of the step in flight
-    if (dt < 0.0) dt = (dt == DT_HALF) ? 0.5 * st->dt : st->dt;
+__global__ void thermo_gather_kernel(GridSrc gs, double* __restrict__ in9) {
+    const long long n = gs.g.ncell;
+    const double dt = grid_dt(gs);
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
-        double u[7], q[7];
+        double in[9];
+        gather_cell(gs, c, dt, in);
 #pragma unroll
-        for (int v = 0; v < 7; ++v) u[v] = U[v * n + c];
-        QfromU(q, u, g.gm1);
-        q[HII] = smax_d(smin_d(q[HII], 1.0), 0.0);
-        q[ADV] = smax_d(smin_d(q[ADV], 1.0), 0.0);
-        q[DEN] = smax_d(q[DEN], g.dfloor);
-        q[PRE] = smax_d(q[PRE], g.pfloor);
-        const double mu_inv = g.X * (q[HII] + 1.0) + g.mu_c;
-        const double temperature = q[PRE] / (mu_inv * g.Rgas * q[DEN]);
-        if (temperature < g.tfloor) q[PRE] = mu_inv * g.Rgas * u[DEN] * g.tfloor;
-        in9[0 * n + c] = q[DEN];
-        in9[1 * n + c] = q[PRE];
-        in9[2 * n + c] = q[HII];
-        bool skip = c == excluded;
-        double f = F ? F[c] : 0.0;
-        if (star.on) {
-            const int ijk[3] = {(int)(c % g.nx), (int)((c / g.nx) % g.ny), (int)(c / ((long long)g.nx * g.ny))};
-            skip = skip || star_wind_cell(star, ijk);
-            if (!F) f = star_fuv(star, ijk);
-        }
-        in9[3 * n + c] = skip ? -1.0 : q[ADV];
-        in9[4 * n + c] = u[PRE];
-        in9[5 * n + c] = T_min ? T_min[c] : T_min_default;
-        in9[6 * n + c] = col_den ? col_den[c] : 0.0;
-        in9[7 * n + c] = f;
-        in9[8 * n + c] = dt;
+        for (int k = 0; k < 9; ++k) in9[k * n + c] = in[k];
     }
 }
 
 // Thermodynamics::calculateTimeStep (Thermodynamics.cpp:463-476): min over
 // cells with RATE != 0 of |f U[PRE] / RATE|, f = 1 when subcycling else 0.1,
 // from the pre-step RATE; reduced as the bit pattern of a non-negative double.
+// GRID: the inputs from the grid state (gs) in registers, no gathered array.
+template <bool GRID>
 __global__ __launch_bounds__(256) void thermo_timestep_kernel(ThermoConsts t, const double* __restrict__ tables,
                                                              long long n, const double* __restrict__ in9,
                                                              unsigned long long* __restrict__ dtmin_bits,
-                                                             unsigned int* __restrict__ err) {
+                                                             unsigned int* __restrict__ err, GridSrc gs) {
     __shared__ double sx[26], sy[26], sy2[26];
     __shared__ double red[4];
     load_tables(tables, sx, sy, sy2);
@@ -505,10 +555,19 @@ __global__ __launch_bounds__(256) void thermo_timestep_kernel(ThermoConsts t, co
         CellState s;
         ThermoConsts t1 = t;
         t1.subcycling = 0;  // pre-step rate only
-        (void)thermo_init(t1, sx, sy, sy2, in9, n, c, s, err);
+        double lin[9];
+        const double* cin = in9;
+        long long cstride = n, ccell = c;
+        if (GRID) {
+            gather_cell(gs, c, 0.0, lin);
+            cin = lin;
+            cstride = 1;
+            ccell = 0;
+        }
+        (void)thermo_init(t1, sx, sy, sy2, cin, cstride, ccell, s, err);
         if (s.rate_pre != 0) {
             const double frac = t.subcycling ? 1.0 : 0.1;
-            const double dti = fabs(frac * in9[4 * n + c] / s.rate_pre);
+            const double dti = fabs(frac * cin[4 * cstride + ccell] / s.rate_pre);
             if (dti < m) m = dti;
         }
     }
@@ -1054,9 +1113,12 @@ int torchgpu_cooling_set_surrogate(torchgpu_cooling* c, torchgpu_mlp* m, const t
     return TORCHGPU_OK;
 }
 
-int torchgpu_thermo_cells_surrogate(torchgpu_cooling* c, int64_t n, const double* in, double* out, uint8_t* path,
-                                    int64_t* counts) {
-    if (!c || n < 0 || (n > 0 && (!in || !out))) return TORCHGPU_ERR_INVALID_ARG;
+}  // extern "C"
+
+// The surrogate step on `in` (gs NULL), or on the grid state (gs: the inputs
+// are formed in the prepass and `in` receives those of the queued cells).
+static int surrogate_step(torchgpu_cooling* c, int64_t n, double* in, double* out, uint8_t* path, int64_t* counts,
+                          const tg::GridSrc* gs) {
     if (!c->has_surrogate) return TORCHGPU_ERR_NOT_TRAINED;
     if (n == 0) return TORCHGPU_OK;
     int rc = ensure_queues(c, n);
@@ -1069,9 +1131,14 @@ int torchgpu_thermo_cells_surrogate(torchgpu_cooling* c, int64_t n, const double
     sc.train = 0;
     long long nb = (n + 255) / 256;
     if (nb > (long long)c->nsm * 16) nb = (long long)c->nsm * 16;
-    hipLaunchKernelGGL(tg::surrogate_prepass_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t, c->tables, sc,
-                       (long long)n, in, out, path, c->mlp_list, c->feat, c->sdata, (float*)nullptr, c->fb_list,
-                       c->counters, c->err);
+    if (gs)
+        hipLaunchKernelGGL(tg::surrogate_prepass_kernel<true>, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t,
+                           c->tables, sc, (long long)n, in, out, path, c->mlp_list, c->feat, c->sdata, (float*)nullptr,
+                           c->fb_list, c->counters, c->err, *gs);
+    else
+        hipLaunchKernelGGL(tg::surrogate_prepass_kernel<false>, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t,
+                           c->tables, sc, (long long)n, in, out, path, c->mlp_list, c->feat, c->sdata, (float*)nullptr,
+                           c->fb_list, c->counters, c->err, tg::GridSrc{});
     HIP_OK(hipGetLastError());
     HIP_OK(tg::launch_mlp_infer_dev(w16, b, c->feat, c->counters + 2, n, c->y, c->nsm, c->stream));
     hipLaunchKernelGGL(tg::surrogate_finish_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t, sc,
@@ -1091,6 +1158,15 @@ int torchgpu_thermo_cells_surrogate(torchgpu_cooling* c, int64_t n, const double
         counts[1] = (int64_t)h[3];
     }
     return TORCHGPU_OK;
+}
+
+extern "C" {
+
+int torchgpu_thermo_cells_surrogate(torchgpu_cooling* c, int64_t n, const double* in, double* out, uint8_t* path,
+                                    int64_t* counts) {
+    if (!c || n < 0 || (n > 0 && (!in || !out))) return TORCHGPU_ERR_INVALID_ARG;
+    // the prepass only reads `in` when it is not forming the inputs itself
+    return surrogate_step(c, n, const_cast<double*>(in), out, path, counts, nullptr);
 }
 
 int torchgpu_surrogate_training_data(torchgpu_cooling* c, int64_t n, const double* in, const double* out,
@@ -1114,9 +1190,9 @@ int torchgpu_surrogate_training_data(torchgpu_cooling* c, int64_t n, const doubl
     };
     long long nb = (n + 255) / 256;
     if (nb > (long long)c->nsm * 16) nb = (long long)c->nsm * 16;
-    hipLaunchKernelGGL(tg::surrogate_prepass_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t, c->tables, sc,
-                       (long long)n, in, scratch, (unsigned char*)nullptr, (long long*)list, feat, c->sdata, raw,
-                       c->fb_list, c->counters, c->err);
+    hipLaunchKernelGGL(tg::surrogate_prepass_kernel<false>, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t,
+                       c->tables, sc, (long long)n, const_cast<double*>(in), scratch, (unsigned char*)nullptr,
+                       (long long*)list, feat, c->sdata, raw, c->fb_list, c->counters, c->err, tg::GridSrc{});
     keep(hipGetLastError());
     if (e == hipSuccess) {
         hipLaunchKernelGGL(tg::surrogate_targets_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, c->counters,
@@ -1149,28 +1225,49 @@ static int ensure_grid(torchgpu_cooling* c, long long n) {
     return TORCHGPU_OK;
 }
 
-static int gather(torchgpu_cooling* c, torchgpu_ctx* ctx, const double* T_min, const double* col_den, const double* F,
-                  long long excluded, double dt) {
+// The grid source of the cooling inputs (see tg::GridSrc) for the
+// operator's current set-up.
+static int grid_src(torchgpu_cooling* c, torchgpu_ctx* ctx, const double* T_min, const double* col_den, const double* F,
+                    long long excluded, double dt, tg::GridSrc& gs) {
     tg::CtxView v = tg::ctx_view(ctx);
     const long long n = v.g->ncell;
-    int rc = ensure_grid(c, n);
-    if (rc) return rc;
-    long long nb = (n + 255) / 256;
-    if (nb > 65536) nb = 65536;
     if (c->star.on && (c->star.n[0] != v.g->nx || c->star.n[1] != v.g->ny || c->star.n[2] != v.g->nz))
         return TORCHGPU_ERR_INVALID_ARG;
     if (!T_min && c->tmin) {
         if (c->tmin_n != n) return TORCHGPU_ERR_INVALID_ARG;
         T_min = c->tmin;
     }
-    hipLaunchKernelGGL(tg::thermo_gather_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, *v.g, v.U, T_min,
-                       100.0, col_den, F, dt, v.st, excluded, c->star, c->in9);
+    gs = tg::GridSrc{};
+    gs.g = *v.g;
+    gs.U = v.U;
+    gs.T_min = T_min;
+    gs.T_min_default = 100.0;
+    gs.col_den = col_den;
+    gs.F = F;
+    gs.dt = dt;
+    gs.st = v.st;
+    gs.excluded = excluded;
+    gs.star = c->star;
+    return ensure_grid(c, n);
+}
+
+// The star's column densities come from the ray trace over the gathered
+// densities (Thermodynamics::preTimeStepCalculations starts with rayTrace,
+// Thermodynamics.cpp:218-220): such a set-up needs the gathered array.
+static bool needs_gather(const torchgpu_cooling* c, const double* col_den) { return c->star.on && !col_den; }
+
+static int gather(torchgpu_cooling* c, torchgpu_ctx* ctx, const double* T_min, const double* col_den, const double* F,
+                  long long excluded, double dt) {
+    tg::GridSrc gs;
+    int rc = grid_src(c, ctx, T_min, col_den, F, excluded, dt, gs);
+    if (rc) return rc;
+    const long long n = gs.g.ncell;
+    long long nb = (n + 255) / 256;
+    if (nb > 65536) nb = 65536;
+    hipLaunchKernelGGL(tg::thermo_gather_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, gs, c->in9);
     HIP_OK(hipGetLastError());
-    if (c->star.on && !col_den) {
-        // Thermodynamics::preTimeStepCalculations starts with rayTrace on
-        // the fixed primitives (Thermodynamics.cpp:218-220)
+    if (needs_gather(c, col_den))
         HIP_OK(tg::launch_thermo_ray_trace(c->star, c->in9, c->in9 + 6 * n, c->dcol, c->xface, c->stream));
-    }
     return TORCHGPU_OK;
 }
 
@@ -1178,14 +1275,22 @@ static int gather(torchgpu_cooling* c, torchgpu_ctx* ctx, const double* T_min, c
 // of min |f E/RATE| over the gated cells, 0x7fef... when none), no host sync.
 static int grid_timestep_dev(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* T_min, const double* col_den,
                              const double* F, int64_t excluded) {
-    int rc = gather(c, ctx, T_min, col_den, F, excluded, 0.0);
+    // without the star's ray trace the inputs are formed in the kernel
+    const bool direct = !needs_gather(c, col_den);
+    tg::GridSrc gs;
+    int rc = direct ? grid_src(c, ctx, T_min, col_den, F, excluded, 0.0, gs)
+                    : gather(c, ctx, T_min, col_den, F, excluded, 0.0);
     if (rc) return rc;
     const long long n = tg::ctx_view(ctx).g->ncell;
     hipLaunchKernelGGL(tg::set_u64_kernel, dim3(1), dim3(1), 0, c->stream, c->dtbits, 0x7fefffffffffffffull);
     long long nb = (n + 255) / 256;
     if (nb > (long long)c->nsm * 16) nb = (long long)c->nsm * 16;
-    hipLaunchKernelGGL(tg::thermo_timestep_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t, c->tables, n,
-                       c->in9, c->dtbits, c->err);
+    if (direct)
+        hipLaunchKernelGGL(tg::thermo_timestep_kernel<true>, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t,
+                           c->tables, n, (const double*)nullptr, c->dtbits, c->err, gs);
+    else
+        hipLaunchKernelGGL(tg::thermo_timestep_kernel<false>, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t,
+                           c->tables, n, c->in9, c->dtbits, c->err, tg::GridSrc{});
     HIP_OK(hipGetLastError());
     return TORCHGPU_OK;
 }
@@ -1193,12 +1298,21 @@ static int grid_timestep_dev(torchgpu_ctx* ctx, torchgpu_cooling* c, const doubl
 // One thermo sub-step on the grid (dt explicit or a DT_* code), no host sync.
 static int grid_substep_dev(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* T_min, const double* col_den,
                             const double* F, int64_t excluded, double dt, int surrogate) {
-    int rc = gather(c, ctx, T_min, col_den, F, excluded, dt);
-    if (rc) return rc;
     tg::CtxView v = tg::ctx_view(ctx);
     const long long n = v.g->ncell;
-    if (surrogate) rc = torchgpu_thermo_cells_surrogate(c, n, c->in9, c->out5, nullptr, nullptr);
-    else rc = torchgpu_thermo_cells(c, n, c->in9, c->out5, nullptr);
+    // the surrogate step forms the inputs in its prepass unless the star's
+    // ray trace or the heating record needs the gathered array
+    const bool direct = surrogate && !needs_gather(c, col_den) && !(v.heat && c->t.subcycling);
+    int rc;
+    if (direct) {
+        tg::GridSrc gs;
+        if ((rc = grid_src(c, ctx, T_min, col_den, F, excluded, dt, gs))) return rc;
+        rc = surrogate_step(c, n, c->in9, c->out5, nullptr, nullptr, &gs);
+    } else {
+        if ((rc = gather(c, ctx, T_min, col_den, F, excluded, dt))) return rc;
+        if (surrogate) rc = surrogate_step(c, n, c->in9, c->out5, nullptr, nullptr, nullptr);
+        else rc = torchgpu_thermo_cells(c, n, c->in9, c->out5, nullptr);
+    }
     if (rc) return rc;
     if (v.heat && c->t.subcycling) {
         // GridCell::H[TOT] of Thermodynamics::integrate (zeroed below the switch)
@@ -1208,9 +1322,16 @@ static int grid_substep_dev(torchgpu_ctx* ctx, torchgpu_cooling* c, const double
                            dt, v.st, c->t.thermo_hii_switch, v.heat);
         HIP_OK(hipGetLastError());
     }
-    // Thermodynamics::updateSourceTerms + advSolution(dt) + fixSolution
-    HIP_OK(tg::launch_fix_solution(*v.g, v.U, c->out5 + 2 * n, dt, v.st, c->stream, c->err));
-    tg::ctx_state_changed(ctx);
+    // Thermodynamics::updateSourceTerms + advSolution(dt) + fixSolution; a
+    // mixed-precision grid also gets the float Q of the result here
+    float* qf = tg::ctx_qf_for_state(ctx);
+    if (qf) {
+        HIP_OK(tg::launch_fix_solution_qf(*v.g, v.U, c->out5 + 2 * n, dt, v.st, c->stream, c->err, qf));
+        tg::ctx_qf_written(ctx);
+    } else {
+        HIP_OK(tg::launch_fix_solution(*v.g, v.U, c->out5 + 2 * n, dt, v.st, c->stream, c->err));
+        tg::ctx_state_changed(ctx);
+    }
     return TORCHGPU_OK;
 }
 

@@ -150,12 +150,19 @@ hipError_t launch_hydro_stage(const HydroGeom& g, int solver, int stage, const d
 // next stage; launch_q_float produces them for any other state.
 hipError_t launch_mixed_scale_seed(const HydroGeom& g, const double* U, StepState* st, hipStream_t stream);
 hipError_t launch_q_float(const HydroGeom& g, const double* U, float* q, int fix, hipStream_t stream);
+// launch_fix_solution fused with launch_q_float(U, qout, fix 1) on the result.
+hipError_t launch_fix_solution_qf(const HydroGeom& g, double* U, const double* rate, double dtf, StepState* st,
+                                  hipStream_t stream, unsigned int* cerr, float* qout);
 hipError_t launch_hydro_stage_mixed(const HydroGeom& g, int solver, int stage, const float* qin, const double* Ubase,
                                     double* Udst, float* qout, int qout_fix, StepState* st, hipStream_t stream,
                                     double dtf = -1.0);
 // The grid state was changed by another operator (the float primitives of
 // the mixed path are recomputed before the next stage).
 void ctx_state_changed(torchgpu_ctx* c);
+// Mixed precision: the float Q buffer that must hold Q(U) for the next stage
+// (NULL for FP64 contexts), and the mark that it now does.
+float* ctx_qf_for_state(torchgpu_ctx* c);
+void ctx_qf_written(torchgpu_ctx* c);
 // One hydro stage on the context's buffers (api.hip): mode 1 predictor,
 // 2 corrector, 0 single-stage sub-step; dtf as above.
 int ctx_run_stage(torchgpu_ctx* c, int mode, double dtf);
