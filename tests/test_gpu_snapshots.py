@@ -38,3 +38,48 @@ def test_checkpoint_snapshots_match_oracle(gpu, orc, tmp_path):
         Qr, hr = gpu.snapshot_read(ref)
         assert hg["time"] == hr["time"], c
         np.testing.assert_array_equal(Qg, Qr, err_msg=f"checkpoint {c}")
+
+
+def test_mixed_run_prints_the_predictor_state_it_skips_storing(gpu, tmp_path):
+    """The mixed predictor stores its FP64 U* only where something reads it:
+    the last step of a run call, and a step after which the device loop can
+    print (it reaches a new checkpoint or tmax).  A run that pauses at
+    checkpoints in one call must print exactly what a run of one-step calls
+    prints (every one of those steps stores U*): the same files, the same
+    final state.  Turbulent 24^3, 4 checkpoints, mixed path."""
+    n, tmax, nck = 24, 0.05, 4
+    Q0 = setups.turbulent(n, 1.0, 2.0)
+    kw = dict(side_length=1.0, tmax=tmax, ncheckpoints=nck, tfloor=1e-12, pfloor=0.1 * 8.314462e7 * 1e-30 * 1e-12,
+              precision=gpu.MIXED)
+    ga = gpu.Grid(gpu.make_params((n, n, n), **kw))
+    ga.set_primitive(Q0)
+    files_a = gpu.run_printing(ga, 100000, str(tmp_path / "a"))
+    ga.synchronize()
+
+    gb = gpu.Grid(gpu.make_params((n, n, n), **kw))
+    gb.set_primitive(Q0)
+    prints_b = [(0, gb.clock()[0], gb.print_state())]
+    gb.pause_at_checkpoints(True)
+    final = False
+    for _ in range(100000):
+        gb.run(1)
+        gb.synchronize()
+        count, due = gb.checkpoint()
+        if due:
+            prints_b.append((count, gb.clock()[0], gb.print_state()))
+            final = count != nck
+        elif gb.clock()[0] >= tmax:
+            break
+    if final:
+        prints_b.append((nck, gb.clock()[0], gb.print_state()))
+    gb.pause_at_checkpoints(False)
+    assert len(files_a) == len(prints_b) == nck + 1, (files_a, [c for c, _, _ in prints_b])
+    for f, (c, t, Qb) in zip(files_a, prints_b):
+        Qa, h = gpu.snapshot_read(f)
+        ref = str(tmp_path / f"b_{c}.txt.gz")
+        gpu.snapshot_write(ref, t, Qb, 1.0)
+        Qr, hr = gpu.snapshot_read(ref)
+        assert h["time"] == hr["time"], c
+        np.testing.assert_array_equal(Qa, Qr, err_msg=f"checkpoint {c}")
+    np.testing.assert_array_equal(ga.get_conserved(), gb.get_conserved())
+    assert ga.clock() == gb.clock()

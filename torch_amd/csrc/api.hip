@@ -150,7 +150,9 @@ namespace {
 // One hydro stage on the context's buffers.  mode 1: predictor U* from U;
 // 2: corrector U from U* and U; 0: single-stage sub-step, U* from U, then
 // the two buffers swap roles.  dtf < 0: dt from the step state.
-int run_stage(torchgpu_ctx* c, int mode, double dtf) {
+// last: the predictor of this step is the last of the caller's request, so
+// its U* is stored whether or not the device loop foresees a print after it.
+int run_stage(torchgpu_ctx* c, int mode, double dtf, bool last = true) {
     const int solver = c->p.riemann_solver;
     if (c->p.precision == TORCHGPU_FP64) {
         if (mode == 2) HIP_TRY(c, tg::launch_hydro_stage(c->g, solver, 2, c->Ustar, c->U, c->U, c->st, c->stream, dtf));
@@ -171,7 +173,7 @@ int run_stage(torchgpu_ctx* c, int mode, double dtf) {
     }
     if (!c->qvalid) HIP_TRY(c, tg::launch_q_float(c->g, c->U, q0, 1, c->stream));
     HIP_TRY(c, tg::launch_hydro_stage_mixed(c->g, solver, 1, q0, c->U, c->Ustar, q1, mode == 0, c->st, c->stream,
-                                            dtf));
+                                            dtf, (mode == 0 || last) ? 1 : 0));
     if (mode == 0 && (dtf == tg::DT_HALF || dtf == tg::DT_FULL))
         HIP_TRY(c, tg::launch_halted_copy(c->g, c->st, c->U, c->Ustar, q0, q1, c->stream));
     if (mode == 1) {
@@ -353,10 +355,10 @@ int torchgpu_hydro_timestep(torchgpu_ctx* c, double dt_max, double* dt) {
     return TORCHGPU_OK;
 }
 
-static int hydro_step_device(torchgpu_ctx* c) {
+static int hydro_step_device(torchgpu_ctx* c, bool last = true) {
     // Torch::hydroStep: predictor U* = fix(U + dt/2 L(Q(U))), corrector
     // U = fix(U + dt L(Q(U*))); U is updated in place by the corrector.
-    int rc = run_stage(c, 1, -1.0);
+    int rc = run_stage(c, 1, -1.0, last);
     if (rc) return rc;
     return run_stage(c, 2, -1.0);
 }
@@ -396,7 +398,7 @@ int torchgpu_run(torchgpu_ctx* c, int nsteps) {
     c->last_mode = 1;
     for (int s = 0; s < nsteps; ++s) {
         HIP_TRY(c, tg::launch_step_begin(c->st, 1, 0.0, c->stream));
-        int rc = hydro_step_device(c);
+        int rc = hydro_step_device(c, s == nsteps - 1);
         if (rc) return rc;
         HIP_TRY(c, tg::launch_step_end(c->st, c->stream));
     }

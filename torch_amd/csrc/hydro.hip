@@ -277,6 +277,7 @@ __global__ void step_begin_kernel(StepState* st, int use_cfl, double dt_override
     if (!use_cfl) {  // an explicit Torch::hydroStep(dt): no clock rules apply
         st->halted = 0;
         st->dt = dt_override;
+        st->keep_ustar = 1;
         return;
     }
     if (st->halted) return;
@@ -326,6 +327,10 @@ __global__ void step_begin_kernel(StepState* st, int use_cfl, double dt_override
     }
     st->dt = smin(dt_next, dt);
     st->maxinv = 0ull;
+    // the next step_begin's tests on this step's end time (step_end adds dt
+    // to time the same way): a pause for a new checkpoint, tmax, quitting
+    const double t_end = st->time + st->dt;
+    st->keep_ustar = checkpoint_count(st, t_end) > count || !(t_end < st->tmax) || st->quitting;
 }
 
 __global__ void step_end_kernel(StepState* st) {

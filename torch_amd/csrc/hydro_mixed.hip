@@ -313,8 +313,8 @@ template <int SOLVER, bool ALB, int STAGE, bool INNER>
 __device__ __forceinline__ void march_column(const HydroGeom& g, MarchLDS& s, const MixedScale& m, const int ks[7],
                                              int ox, int oy, int z0, int z1, const float* __restrict__ qin,
                                              const double* Ubase, double* Udst, float* __restrict__ qout,
-                                             int qout_fix, StepState* st, double dtf_host, double& inv_t_max,
-                                             unsigned int& mr, unsigned int& mp) {
+                                             int qout_fix, StepState* st, double dtf_host, bool store_u,
+                                             double& inv_t_max, unsigned int& mr, unsigned int& mp) {
     const int tid = threadIdx.x;
     const int tx = tid % MTX, ty = tid / MTX;
     const int gx = ox + tx, gy = oy + ty;
@@ -409,8 +409,10 @@ __device__ __forceinline__ void march_column(const HydroGeom& g, MarchLDS& s, co
             for (int v = 0; v < 7; ++v) u[v] += dtf * udot[v];
             double q[7];
             if (fix_solution_mixed_q(g, u, q)) set_error(st, STAGE == 1 ? ERR_FIX_SOLUTION_1 : ERR_FIX_SOLUTION_2);
+            if (store_u) {
 #pragma unroll
-            for (int v = 0; v < 7; ++v) Udst[v * g.ncell + c] = u[v];
+                for (int v = 0; v < 7; ++v) Udst[v * g.ncell + c] = u[v];
+            }
             if (STAGE == 2) {
                 fix_primitives_mixed(g, q, u[DEN]);
                 inv_t_max = fmax(inv_t_max, cfl_inv_t_mixed(g, q));
@@ -434,9 +436,15 @@ template <int SOLVER, bool ALB, int STAGE>
 __global__ __launch_bounds__(MTHREADS) __attribute__((amdgpu_waves_per_eu(4))) void hydro_stage_mixed_kernel(HydroGeom g, int mtx, int mty, int mtz, int mz,
                                                                      const float* __restrict__ qin, const double* Ubase,
                                                                      double* Udst, float* __restrict__ qout,
-                                                                     int qout_fix, StepState* st, double dtf_host) {
+                                                                     int qout_fix, StepState* st, double dtf_host,
+                                                                     int ustar_store) {
     __shared__ MarchLDS s;
     if (st->halted) return;
+    // a two-stage corrector reads the predictor's float Q* only: the FP64 U*
+    // is stored where something reads it (a single-stage sub-step's output,
+    // a caller's stage, the last step of a run call, a step a print can
+    // follow)
+    const bool store_u = STAGE == 2 || ustar_store || st->keep_ustar;
     const int ntiles = mtx * mty * mtz;
     const int tile = tile_of_block(blockIdx.x, ntiles);
     const int bx = tile % mtx, by = (tile / mtx) % mty, bz = tile / (mtx * mty);
@@ -449,10 +457,10 @@ __global__ __launch_bounds__(MTHREADS) __attribute__((amdgpu_waves_per_eu(4))) v
     const bool inner = ox >= 2 && oy >= 2 && ox + MTX + 2 <= g.nx && oy + MTY + 2 <= g.ny;
     if (inner)
         march_column<SOLVER, ALB, STAGE, true>(g, s, m, ks, ox, oy, z0, z1, qin, Ubase, Udst, qout, qout_fix, st,
-                                               dtf_host, inv_t, mr, mp);
+                                               dtf_host, store_u, inv_t, mr, mp);
     else
         march_column<SOLVER, ALB, STAGE, false>(g, s, m, ks, ox, oy, z0, z1, qin, Ubase, Udst, qout, qout_fix, st,
-                                                dtf_host, inv_t, mr, mp);
+                                                dtf_host, store_u, inv_t, mr, mp);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (STAGE == 1) {
 #pragma unroll
@@ -494,15 +502,15 @@ constexpr int MZ = 16;
 
 template <int SOLVER, bool ALB>
 static hipError_t launch_mixed_t(const HydroGeom& g, int stage, const float* qin, const double* Ubase, double* Udst,
-                                 float* qout, int qout_fix, StepState* st, double dtf, hipStream_t stream) {
+                                 float* qout, int qout_fix, StepState* st, double dtf, int ustar, hipStream_t stream) {
     const int mtx = (g.nx + MTX - 1) / MTX, mty = (g.ny + MTY - 1) / MTY, mtz = (g.nz + MZ - 1) / MZ;
     dim3 grid(mtx * mty * mtz), block(MTHREADS);
     if (stage == 1)
         hipLaunchKernelGGL((hydro_stage_mixed_kernel<SOLVER, ALB, 1>), grid, block, 0, stream, g, mtx, mty, mtz, MZ,
-                           qin, Ubase, Udst, qout, qout_fix, st, dtf);
+                           qin, Ubase, Udst, qout, qout_fix, st, dtf, ustar);
     else
         hipLaunchKernelGGL((hydro_stage_mixed_kernel<SOLVER, ALB, 2>), grid, block, 0, stream, g, mtx, mty, mtz, MZ,
-                           qin, Ubase, Udst, qout, qout_fix, st, dtf);
+                           qin, Ubase, Udst, qout, qout_fix, st, dtf, 1);
     return hipGetLastError();
 }
 
@@ -512,10 +520,10 @@ static hipError_t launch_mixed_t(const HydroGeom& g, int stage, const float* qin
 // instruction cache is shared by two CUs.
 template <int SOLVER>
 static hipError_t launch_mixed_l(const HydroGeom& g, int stage, const float* qin, const double* Ubase, double* Udst,
-                                 float* qout, int qout_fix, StepState* st, double dtf, hipStream_t stream) {
+                                 float* qout, int qout_fix, StepState* st, double dtf, int ustar, hipStream_t stream) {
     if (g.limiter == LIM_ALBADA)
-        return launch_mixed_t<SOLVER, true>(g, stage, qin, Ubase, Udst, qout, qout_fix, st, dtf, stream);
-    return launch_mixed_t<SOLVER, false>(g, stage, qin, Ubase, Udst, qout, qout_fix, st, dtf, stream);
+        return launch_mixed_t<SOLVER, true>(g, stage, qin, Ubase, Udst, qout, qout_fix, st, dtf, ustar, stream);
+    return launch_mixed_t<SOLVER, false>(g, stage, qin, Ubase, Udst, qout, qout_fix, st, dtf, ustar, stream);
 }
 
 // Float Q of a state for the next mixed stage (see store_qf), for a state
@@ -632,13 +640,15 @@ hipError_t launch_mixed_scale_seed(const HydroGeom& g, const double* U, StepStat
 
 hipError_t launch_hydro_stage_mixed(const HydroGeom& g, int solver, int stage, const float* qin, const double* Ubase,
                                     double* Udst, float* qout, int qout_fix, StepState* st, hipStream_t stream,
-                                    double dtf) {
+                                    double dtf, int ustar) {
     if (stage == 1) hipLaunchKernelGGL(mixed_scale_rotate_kernel, dim3(1), dim3(1), 0, stream, st);
     switch (solver) {
-    case SOLVER_HLL:  return launch_mixed_l<SOLVER_HLL>(g, stage, qin, Ubase, Udst, qout, qout_fix, st, dtf, stream);
-    case SOLVER_HLLC: return launch_mixed_l<SOLVER_HLLC>(g, stage, qin, Ubase, Udst, qout, qout_fix, st, dtf, stream);
+    case SOLVER_HLL:
+        return launch_mixed_l<SOLVER_HLL>(g, stage, qin, Ubase, Udst, qout, qout_fix, st, dtf, ustar, stream);
+    case SOLVER_HLLC:
+        return launch_mixed_l<SOLVER_HLLC>(g, stage, qin, Ubase, Udst, qout, qout_fix, st, dtf, ustar, stream);
     default:
-        return launch_mixed_l<SOLVER_ROTATED>(g, stage, qin, Ubase, Udst, qout, qout_fix, st, dtf, stream);
+        return launch_mixed_l<SOLVER_ROTATED>(g, stage, qin, Ubase, Udst, qout, qout_fix, st, dtf, ustar, stream);
     }
 }
 

@@ -82,7 +82,10 @@ struct StepState {
     // Torch::m_isQuitting: set by the step whose hydro time step had
     // 100 dt_hydro / tmax <= 1e-6 with debug on; the steps after it halt
     int quitting;
-    int pad;
+    // the step in flight is the last before a print can happen (it reaches
+    // a new checkpoint or tmax, or quits): the mixed predictor then stores
+    // U*, whose QfromU Torch::run prints after a hydro step
+    int keep_ustar;
 };
 
 // Negative dt arguments of the stage and cooling kernels take the dt of the
@@ -155,7 +158,7 @@ hipError_t launch_fix_solution_qf(const HydroGeom& g, double* U, const double* r
                                   hipStream_t stream, unsigned int* cerr, float* qout);
 hipError_t launch_hydro_stage_mixed(const HydroGeom& g, int solver, int stage, const float* qin, const double* Ubase,
                                     double* Udst, float* qout, int qout_fix, StepState* st, hipStream_t stream,
-                                    double dtf = -1.0);
+                                    double dtf = -1.0, int ustar = 1);
 // The grid state was changed by another operator (the float primitives of
 // the mixed path are recomputed before the next stage).
 void ctx_state_changed(torchgpu_ctx* c);
