@@ -44,8 +44,8 @@ __device__ __forceinline__ long long xface_index(const StarGeom& s, const XFaces
 }
 
 __global__ __launch_bounds__(256) void coldens_shell_kernel(StarGeom s, ShellFaces f, int k, XFaces xf,
-                                                            const double* __restrict__ den, double* col,
-                                                            double* dcol) {
+                                                            const double* __restrict__ den, double den_floor,
+                                                            double* col, double* dcol) {
     const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
     if (t >= f.start[f.nf]) return;
     int ijk[3];
@@ -54,8 +54,10 @@ __global__ __launch_bounds__(256) void coldens_shell_kernel(StarGeom s, ShellFac
     const bool xface = ijk[0] - s.s[0] == k || s.s[0] - ijk[0] == k;
     const int side = ijk[0] > s.s[0] ? 1 : 0;
     // the cell's density first: for x-face cells a strided load, whose
-    // latency then overlaps the neighbour reads
-    const double den_c = den[c];
+    // latency then overlaps the neighbour reads; floored as fixPrimitives
+    // floors it (den_floor -HUGE_VAL: as given)
+    const double den_raw = den[c];
+    const double den_c = den_raw < den_floor ? den_floor : den_raw;
     // distance to the star in cell units
     double dist2 = 0;
 #pragma unroll
@@ -165,8 +167,8 @@ long long xface_doubles(const StarGeom& s) {
     return 2 * 2 * 2 * w * w;  // col and dcol, 2 parities, 2 sides
 }
 
-hipError_t launch_thermo_ray_trace(const StarGeom& s, const double* den, double* col, double* dcol, double* xface,
-                                   hipStream_t stream) {
+hipError_t launch_thermo_ray_trace(const StarGeom& s, const double* den, double den_floor, double* col, double* dcol,
+                                   double* xface, hipStream_t stream) {
     const int kmax = shell_max(s);
     XFaces xf;
     xf.K = kmax;
@@ -179,8 +181,8 @@ hipError_t launch_thermo_ray_trace(const StarGeom& s, const double* den, double*
         const long long cells = f.start[f.nf];
         if (cells == 0) continue;
         const long long nb = (cells + 255) / 256;
-        hipLaunchKernelGGL(coldens_shell_kernel, dim3((unsigned)nb), dim3(256), 0, stream, s, f, k, xf, den, col,
-                           dcol);
+        hipLaunchKernelGGL(coldens_shell_kernel, dim3((unsigned)nb), dim3(256), 0, stream, s, f, k, xf, den,
+                           den_floor, col, dcol);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

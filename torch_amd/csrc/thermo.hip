@@ -794,6 +794,7 @@ struct torchgpu_cooling {
     double* tmin = nullptr;
     long long tmin_n = 0;
     double* dcol = nullptr;
+    double* col = nullptr;   // COL_DEN of the last ray trace
     long long dcol_cap = 0;
     double* xface = nullptr;   // the ray trace's x-face buffers
     long long xface_cap = 0;
@@ -1019,7 +1020,7 @@ void torchgpu_cooling_destroy(torchgpu_cooling* c) {
     for (void* ptr : {(void*)c->tables, (void*)c->counters, (void*)c->err, (void*)c->mlp_list, (void*)c->fb_list,
                       (void*)c->feat, (void*)c->y, (void*)c->sdata, (void*)c->in9, (void*)c->out5, (void*)c->dtbits,
                       (void*)c->lpt_order, (void*)c->lpt_bucket, (void*)c->lpt_hist, (void*)c->dcol,
-                      (void*)c->xface, (void*)c->tmin})
+                      (void*)c->col, (void*)c->xface, (void*)c->tmin})
         if (ptr) (void)hipFree(ptr);
     delete c;
 }
@@ -1237,6 +1238,15 @@ static int grid_src(torchgpu_cooling* c, torchgpu_ctx* ctx, const double* T_min,
         if (c->tmin_n != n) return TORCHGPU_ERR_INVALID_ARG;
         T_min = c->tmin;
     }
+    int rc = ensure_grid(c, n);
+    if (rc) return rc;
+    if (c->star.on && !col_den) {
+        // Thermodynamics::preTimeStepCalculations starts with rayTrace on
+        // the fixed primitives (Thermodynamics.cpp:218-220): the density of
+        // U with fixPrimitives' floor
+        HIP_OK(tg::launch_thermo_ray_trace(c->star, v.U, v.g->dfloor, c->col, c->dcol, c->xface, c->stream));
+        col_den = c->col;
+    }
     gs = tg::GridSrc{};
     gs.g = *v.g;
     gs.U = v.U;
@@ -1248,13 +1258,8 @@ static int grid_src(torchgpu_cooling* c, torchgpu_ctx* ctx, const double* T_min,
     gs.st = v.st;
     gs.excluded = excluded;
     gs.star = c->star;
-    return ensure_grid(c, n);
+    return TORCHGPU_OK;
 }
-
-// The star's column densities come from the ray trace over the gathered
-// densities (Thermodynamics::preTimeStepCalculations starts with rayTrace,
-// Thermodynamics.cpp:218-220): such a set-up needs the gathered array.
-static bool needs_gather(const torchgpu_cooling* c, const double* col_den) { return c->star.on && !col_den; }
 
 static int gather(torchgpu_cooling* c, torchgpu_ctx* ctx, const double* T_min, const double* col_den, const double* F,
                   long long excluded, double dt) {
@@ -1266,8 +1271,6 @@ static int gather(torchgpu_cooling* c, torchgpu_ctx* ctx, const double* T_min, c
     if (nb > 65536) nb = 65536;
     hipLaunchKernelGGL(tg::thermo_gather_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, gs, c->in9);
     HIP_OK(hipGetLastError());
-    if (needs_gather(c, col_den))
-        HIP_OK(tg::launch_thermo_ray_trace(c->star, c->in9, c->in9 + 6 * n, c->dcol, c->xface, c->stream));
     return TORCHGPU_OK;
 }
 
@@ -1275,22 +1278,16 @@ static int gather(torchgpu_cooling* c, torchgpu_ctx* ctx, const double* T_min, c
 // of min |f E/RATE| over the gated cells, 0x7fef... when none), no host sync.
 static int grid_timestep_dev(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* T_min, const double* col_den,
                              const double* F, int64_t excluded) {
-    // without the star's ray trace the inputs are formed in the kernel
-    const bool direct = !needs_gather(c, col_den);
+    // the inputs are formed in the kernel
     tg::GridSrc gs;
-    int rc = direct ? grid_src(c, ctx, T_min, col_den, F, excluded, 0.0, gs)
-                    : gather(c, ctx, T_min, col_den, F, excluded, 0.0);
+    int rc = grid_src(c, ctx, T_min, col_den, F, excluded, 0.0, gs);
     if (rc) return rc;
     const long long n = tg::ctx_view(ctx).g->ncell;
     hipLaunchKernelGGL(tg::set_u64_kernel, dim3(1), dim3(1), 0, c->stream, c->dtbits, 0x7fefffffffffffffull);
     long long nb = (n + 255) / 256;
     if (nb > (long long)c->nsm * 16) nb = (long long)c->nsm * 16;
-    if (direct)
-        hipLaunchKernelGGL(tg::thermo_timestep_kernel<true>, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t,
-                           c->tables, n, (const double*)nullptr, c->dtbits, c->err, gs);
-    else
-        hipLaunchKernelGGL(tg::thermo_timestep_kernel<false>, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t,
-                           c->tables, n, c->in9, c->dtbits, c->err, tg::GridSrc{});
+    hipLaunchKernelGGL(tg::thermo_timestep_kernel<true>, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t,
+                       c->tables, n, (const double*)nullptr, c->dtbits, c->err, gs);
     HIP_OK(hipGetLastError());
     return TORCHGPU_OK;
 }
@@ -1300,9 +1297,9 @@ static int grid_substep_dev(torchgpu_ctx* ctx, torchgpu_cooling* c, const double
                             const double* F, int64_t excluded, double dt, int surrogate) {
     tg::CtxView v = tg::ctx_view(ctx);
     const long long n = v.g->ncell;
-    // the surrogate step forms the inputs in its prepass unless the star's
-    // ray trace or the heating record needs the gathered array
-    const bool direct = surrogate && !needs_gather(c, col_den) && !(v.heat && c->t.subcycling);
+    // the surrogate step forms the inputs in its prepass unless the heating
+    // record needs the gathered array
+    const bool direct = surrogate && !(v.heat && c->t.subcycling);
     int rc;
     if (direct) {
         tg::GridSrc gs;
@@ -1378,8 +1375,8 @@ int torchgpu_fill_heating(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* 
     if (rc) return rc;
     if (c->star.on && !col_den) {
         // fillHeatingArrays starts with rayTrace on Fluid::Q (Thermodynamics.cpp:420-421)
-        HIP_OK(tg::launch_thermo_ray_trace(c->star, Q, c->in9 + 6 * n, c->dcol, c->xface, c->stream));
-        col_den = c->in9 + 6 * n;
+        HIP_OK(tg::launch_thermo_ray_trace(c->star, Q, -HUGE_VAL, c->col, c->dcol, c->xface, c->stream));
+        col_den = c->col;
     }
     long long nb = (n + 255) / 256;
     if (nb > 65536) nb = 65536;
@@ -1430,12 +1427,16 @@ int torchgpu_cooling_set_star(torchgpu_cooling* c, torchgpu_ctx* ctx, const torc
     if (ncell > c->dcol_cap) {
         (void)hipStreamSynchronize(c->stream);
         if (c->dcol) (void)hipFree(c->dcol);
-        c->dcol = nullptr;
+        if (c->col) (void)hipFree(c->col);
+        c->dcol = c->col = nullptr;
         c->dcol_cap = 0;
         HIP_OK(hipMalloc(&c->dcol, ncell * sizeof(double)));
+        HIP_OK(hipMalloc(&c->col, ncell * sizeof(double)));
         c->dcol_cap = ncell;
     }
     hipLaunchKernelGGL(tg::zero_f64_kernel, dim3(1024), dim3(256), 0, c->stream, c->dcol, ncell);
+    HIP_OK(hipGetLastError());
+    hipLaunchKernelGGL(tg::zero_f64_kernel, dim3(1024), dim3(256), 0, c->stream, c->col, ncell);
     HIP_OK(hipGetLastError());
     tg::StarGeom s{};
     s.on = 1;

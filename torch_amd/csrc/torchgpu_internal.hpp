@@ -199,9 +199,10 @@ hipError_t launch_step_end(StepState* st, hipStream_t stream);
 // checkpoint count Checkpointer::update gives for it), resume a paused run.
 hipError_t launch_set_time(StepState* st, double t, hipStream_t stream);
 // The column-density ray trace (Thermodynamics::rayTrace): one launch per
-// shell, den/col/dcol [n] device arrays; the star's geometry per cell.
-hipError_t launch_thermo_ray_trace(const StarGeom& s, const double* den, double* col, double* dcol, double* xface,
-                                   hipStream_t stream);
+// shell, den/col/dcol [n] device arrays (den floored at den_floor, as
+// fixPrimitives does; -HUGE_VAL: as given); the star's geometry per cell.
+hipError_t launch_thermo_ray_trace(const StarGeom& s, const double* den, double den_floor, double* col, double* dcol,
+                                   double* xface, hipStream_t stream);
 // doubles of the x-face buffers of launch_thermo_ray_trace
 long long xface_doubles(const StarGeom& s);
 // The cells of Chebyshev shell k around the star, and the largest k.

@@ -68,7 +68,7 @@ def gpu_rates(n: int = 256, reps: int = 5) -> dict:
         return e0.elapsed_time(e1) / 1e3 / reps
 
     # the thermo timestep with and without the star: the difference is the
-    # ray trace (the rest is the same gather and rate kernels)
+    # ray trace (both form their inputs from the grid in the rate kernel)
     t_with = timed(lambda: cool.grid_timestep(g, p.dt_max))
     cool.set_star(g, None)
     t_without = timed(lambda: cool.grid_timestep(g, p.dt_max))
