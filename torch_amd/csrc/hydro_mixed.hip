@@ -110,12 +110,23 @@ __device__ __forceinline__ void apply_ghost_mode_f(float q[7], int mode, int dim
 
 __device__ __forceinline__ unsigned int max_bits_f(float v) { return v > 0.f ? __float_as_uint(v) : 0u; }
 
+// Element (v, c) of a [7][ncell] state array.  A32: the variable's base
+// (uniform, in scalar registers) plus a 32-bit byte offset, which the global
+// loads and stores take as they are -- no 64-bit address arithmetic per
+// access in vector registers; needs ncell * sizeof(T) < 2^32.
+template <bool A32, typename T>
+__device__ __forceinline__ T* soa_at(T* base, long long ncell, int v, long long c) {
+    if constexpr (A32) return (T*)((char*)(base + v * ncell) + (unsigned int)((unsigned int)c * (unsigned int)sizeof(T)));
+    else return base + v * ncell + c;
+}
+
 // The float primitives a stage reads, written per cell by the stage (or
 // pass) that produced U: Q = QfromU(U) in FP64, rounded to float (not
 // scaled).  In fix mode (the input of a predictor or single-stage sub-step)
 // fixPrimitives is applied and Torch::checkValues("hydro before")
 // (Torch.cpp:420-443) is evaluated on U: a failing cell is stored with a NaN
 // density, which the consuming predictor reports at its step.
+template <bool A32 = false>
 __device__ __forceinline__ void store_qf(const HydroGeom& g, double q[7], const double u[7], float* __restrict__ qout,
                                          long long c, int fix, bool fixed_already) {
     if (fix) {
@@ -126,7 +137,7 @@ __device__ __forceinline__ void store_qf(const HydroGeom& g, double q[7], const 
         if (bad) q[DEN] = __builtin_nan("");
     }
 #pragma unroll
-    for (int v = 0; v < 7; ++v) qout[v * g.ncell + c] = (float)q[v];
+    for (int v = 0; v < 7; ++v) *soa_at<A32>(qout, g.ncell, v, c) = (float)q[v];
 }
 
 #ifndef TG_MTX
@@ -155,7 +166,7 @@ __device__ __forceinline__ int mslot(int x, int y) { return (y + 2) * PW + x + 2
 // Float primitives of global cell (x, y, z) (positions out of range are
 // ghost cells: the first out-of-range dimension decides, as cell_source),
 // scaled by 2^ks.  INNER: (x, y) lies inside the grid.
-template <bool INNER>
+template <bool INNER, bool A32>
 __device__ __forceinline__ void load_cell_f(const HydroGeom& g, const float* __restrict__ qin, int x, int y, int z,
                                             const int ks[7], float q[7]) {
     long long src;
@@ -169,7 +180,7 @@ __device__ __forceinline__ void load_cell_f(const HydroGeom& g, const float* __r
         mdim = cs.mdim;
     }
 #pragma unroll
-    for (int v = 0; v < 7; ++v) q[v] = qin[v * g.ncell + src];
+    for (int v = 0; v < 7; ++v) q[v] = *soa_at<A32>(qin, g.ncell, v, src);
     if (mdim) apply_ghost_mode_f(q, mode, mdim - 1);
 #pragma unroll
     for (int v = 0; v < 7; ++v) q[v] = __builtin_ldexpf(q[v], ks[v]);
@@ -309,7 +320,7 @@ __device__ __forceinline__ void march_xy_face(const HydroGeom& g, MarchLDS& s, c
     if (bad) set_error(st, STAGE == 1 ? ERR_RIEMANN_NAN_1 : ERR_RIEMANN_NAN_2);
 }
 
-template <int SOLVER, bool ALB, int STAGE, bool INNER>
+template <int SOLVER, bool ALB, int STAGE, bool INNER, bool A32>
 __device__ __forceinline__ void march_column(const HydroGeom& g, MarchLDS& s, const MixedScale& m, const int ks[7],
                                              int ox, int oy, int z0, int z1, const float* __restrict__ qin,
                                              const double* Ubase, double* Udst, float* __restrict__ qout,
@@ -332,16 +343,16 @@ __device__ __forceinline__ void march_column(const HydroGeom& g, MarchLDS& s, co
     float sk[7], w1[7], w2[7], w3[7], fz[7];
     {
         float a[7], w0[7], s0[7];
-        load_cell_f<INNER>(g, qin, gx, gy, z0 - 2, ks, a);
-        load_cell_f<INNER>(g, qin, gx, gy, z0 - 1, ks, w0);
-        load_cell_f<INNER>(g, qin, gx, gy, z0, ks, w1);
-        load_cell_f<INNER>(g, qin, gx, gy, z0 + 1, ks, w2);
+        load_cell_f<INNER, A32>(g, qin, gx, gy, z0 - 2, ks, a);
+        load_cell_f<INNER, A32>(g, qin, gx, gy, z0 - 1, ks, w0);
+        load_cell_f<INNER, A32>(g, qin, gx, gy, z0, ks, w1);
+        load_cell_f<INNER, A32>(g, qin, gx, gy, z0 + 1, ks, w2);
         cell_slope<ALB>(g, m, a, w0, w1, s0);
         cell_slope<ALB>(g, m, w0, w1, w2, sk);
         bool bad = false;
         if (col) bad = slope_flux<SOLVER, 2>(g, m, w0, s0, w1, sk, gc, fz);   // face z0 - 1/2
         if (bad) set_error(st, STAGE == 1 ? ERR_RIEMANN_NAN_1 : ERR_RIEMANN_NAN_2);
-        load_cell_f<INNER>(g, qin, gx, gy, z0 + 2, ks, w3);
+        load_cell_f<INNER, A32>(g, qin, gx, gy, z0 + 2, ks, w3);
     }
     for (int k = z0; k < z1; ++k) {
         // plane k into LDS
@@ -349,7 +360,7 @@ __device__ __forceinline__ void march_column(const HydroGeom& g, MarchLDS& s, co
         for (int v = 0; v < 7; ++v) s.p[v][mslot(tx, ty)] = w1[v];
         if (tid < MHALO) {
             float hq[7];
-            load_cell_f<INNER>(g, qin, ox + hx, oy + hy, k, ks, hq);
+            load_cell_f<INNER, A32>(g, qin, ox + hx, oy + hy, k, ks, hq);
             const int hs = mslot(hx, hy);
 #pragma unroll
             for (int v = 0; v < 7; ++v) s.p[v][hs] = hq[v];
@@ -404,20 +415,20 @@ __device__ __forceinline__ void march_column(const HydroGeom& g, MarchLDS& s, co
             const long long c = (long long)gx + (long long)g.nx * ((long long)gy + (long long)g.ny * k);
             double u[7];
 #pragma unroll
-            for (int v = 0; v < 7; ++v) u[v] = Ubase[v * g.ncell + c];
+            for (int v = 0; v < 7; ++v) u[v] = *soa_at<A32>(Ubase, g.ncell, v, c);
 #pragma unroll
             for (int v = 0; v < 7; ++v) u[v] += dtf * udot[v];
             double q[7];
             if (fix_solution_mixed_q(g, u, q)) set_error(st, STAGE == 1 ? ERR_FIX_SOLUTION_1 : ERR_FIX_SOLUTION_2);
             if (store_u) {
 #pragma unroll
-                for (int v = 0; v < 7; ++v) Udst[v * g.ncell + c] = u[v];
+                for (int v = 0; v < 7; ++v) *soa_at<A32>(Udst, g.ncell, v, c) = u[v];
             }
             if (STAGE == 2) {
                 fix_primitives_mixed(g, q, u[DEN]);
                 inv_t_max = fmax(inv_t_max, cfl_inv_t_mixed(g, q));
             }
-            store_qf(g, q, u, qout, c, qout_fix, STAGE == 2);
+            store_qf<A32>(g, q, u, qout, c, qout_fix, STAGE == 2);
         }
 #pragma unroll
         for (int v = 0; v < 7; ++v) {
@@ -425,11 +436,11 @@ __device__ __forceinline__ void march_column(const HydroGeom& g, MarchLDS& s, co
             w1[v] = w2[v];
             w2[v] = w3[v];
         }
-        if (k + 1 < z1) load_cell_f<INNER>(g, qin, gx, gy, k + 3, ks, w3);
+        if (k + 1 < z1) load_cell_f<INNER, A32>(g, qin, gx, gy, k + 3, ks, w3);
     }
 }
 
-template <int SOLVER, bool ALB, int STAGE>
+template <int SOLVER, bool ALB, int STAGE, bool A32>
 // Four waves per SIMD (128 VGPRs; the 33 KB of LDS allow four workgroups
 // per CU): about 20 registers spill, which measured faster than three waves
 // per SIMD without spills (14.9 against 16.7 ms per 512^3 step).
@@ -456,10 +467,10 @@ __global__ __launch_bounds__(MTHREADS) __attribute__((amdgpu_waves_per_eu(4))) v
     unsigned int mr = 0, mp = 0;
     const bool inner = ox >= 2 && oy >= 2 && ox + MTX + 2 <= g.nx && oy + MTY + 2 <= g.ny;
     if (inner)
-        march_column<SOLVER, ALB, STAGE, true>(g, s, m, ks, ox, oy, z0, z1, qin, Ubase, Udst, qout, qout_fix, st,
+        march_column<SOLVER, ALB, STAGE, true, A32>(g, s, m, ks, ox, oy, z0, z1, qin, Ubase, Udst, qout, qout_fix, st,
                                                dtf_host, store_u, inv_t, mr, mp);
     else
-        march_column<SOLVER, ALB, STAGE, false>(g, s, m, ks, ox, oy, z0, z1, qin, Ubase, Udst, qout, qout_fix, st,
+        march_column<SOLVER, ALB, STAGE, false, A32>(g, s, m, ks, ox, oy, z0, z1, qin, Ubase, Udst, qout, qout_fix, st,
                                                 dtf_host, store_u, inv_t, mr, mp);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (STAGE == 1) {
@@ -505,12 +516,24 @@ static hipError_t launch_mixed_t(const HydroGeom& g, int stage, const float* qin
                                  float* qout, int qout_fix, StepState* st, double dtf, int ustar, hipStream_t stream) {
     const int mtx = (g.nx + MTX - 1) / MTX, mty = (g.ny + MTY - 1) / MTY, mtz = (g.nz + MZ - 1) / MZ;
     dim3 grid(mtx * mty * mtz), block(MTHREADS);
-    if (stage == 1)
-        hipLaunchKernelGGL((hydro_stage_mixed_kernel<SOLVER, ALB, 1>), grid, block, 0, stream, g, mtx, mty, mtz, MZ,
-                           qin, Ubase, Udst, qout, qout_fix, st, dtf, ustar);
-    else
-        hipLaunchKernelGGL((hydro_stage_mixed_kernel<SOLVER, ALB, 2>), grid, block, 0, stream, g, mtx, mty, mtz, MZ,
-                           qin, Ubase, Udst, qout, qout_fix, st, dtf, 1);
+    // 32-bit byte offsets within a variable while the FP64 arrays allow it
+    // (ncell < 2^29, i.e. up to ~812^3 cells)
+    const bool a32 = g.ncell * (long long)sizeof(double) < (1LL << 32);
+    if (stage == 1) {
+        if (a32)
+            hipLaunchKernelGGL((hydro_stage_mixed_kernel<SOLVER, ALB, 1, true>), grid, block, 0, stream, g, mtx, mty,
+                               mtz, MZ, qin, Ubase, Udst, qout, qout_fix, st, dtf, ustar);
+        else
+            hipLaunchKernelGGL((hydro_stage_mixed_kernel<SOLVER, ALB, 1, false>), grid, block, 0, stream, g, mtx, mty,
+                               mtz, MZ, qin, Ubase, Udst, qout, qout_fix, st, dtf, ustar);
+    } else {
+        if (a32)
+            hipLaunchKernelGGL((hydro_stage_mixed_kernel<SOLVER, ALB, 2, true>), grid, block, 0, stream, g, mtx, mty,
+                               mtz, MZ, qin, Ubase, Udst, qout, qout_fix, st, dtf, 1);
+        else
+            hipLaunchKernelGGL((hydro_stage_mixed_kernel<SOLVER, ALB, 2, false>), grid, block, 0, stream, g, mtx, mty,
+                               mtz, MZ, qin, Ubase, Udst, qout, qout_fix, st, dtf, 1);
+    }
     return hipGetLastError();
 }
 
