@@ -509,7 +509,10 @@ __global__ __launch_bounds__(MTHREADS) __attribute__((amdgpu_waves_per_eu(4))) v
 
 // z planes per workgroup: 16 and 32 measured the same at 512^3, 16 ran 2 %
 // faster at 256^3 (twice the workgroups).
-constexpr int MZ = 16;
+#ifndef TG_MZ
+#define TG_MZ 16
+#endif
+constexpr int MZ = TG_MZ;
 
 template <int SOLVER, bool ALB>
 static hipError_t launch_mixed_t(const HydroGeom& g, int stage, const float* qin, const double* Ubase, double* Udst,
