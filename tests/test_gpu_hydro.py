@@ -249,3 +249,29 @@ def test_sod64_200_steps_bitwise_vs_reference_fixture(gpu):
             np.testing.assert_array_equal(U[:, 0, 0, :], f[f"U_{s}"], err_msg=f"step {s}")
             assert (U == U[:, :1, :1, :]).all(), s
     np.testing.assert_array_equal(np.array(dts), f["dts"])
+
+
+def test_mixed_kernel_64bit_addressing_path(gpu):
+    """The mixed kernel addresses each state variable by a 32-bit byte offset
+    below 2^29 cells and with 64-bit addresses above; TORCHGPU_ADDR64=1 (read
+    when a grid is created) takes the 64-bit path on a small grid.  Only the
+    addressing differs, so both give the same state bit for bit (ragged grid
+    with every boundary condition, 6 steps)."""
+    import os
+
+    Q = np.ascontiguousarray(setups.turbulent(32, 1.0, 2.0)[:, :20, :27, :31])
+    bcs = ["outflow", "inflow", "periodic", "periodic", "reflecting", "free"]
+    kw = dict(tfloor=1e-12, pfloor=0.1 * 8.314462e7 * 1e-30 * 1e-12, bc=bcs)
+    out = []
+    for a64 in ("0", "1"):
+        os.environ["TORCHGPU_ADDR64"] = a64
+        try:
+            g = _grid(gpu, Q.shape[1:], precision=gpu.MIXED, **kw)
+        finally:
+            os.environ.pop("TORCHGPU_ADDR64", None)
+        g.set_primitive(Q)
+        g.run(6)
+        g.synchronize()
+        out.append((g.get_conserved(), g.clock()))
+        g.close()
+    assert np.array_equal(out[0][0], out[1][0]) and out[0][1] == out[1][1]

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <utility>
@@ -224,6 +225,10 @@ int torchgpu_create(const torchgpu_params* p, void* stream, torchgpu_ctx** out) 
     for (int d = 0; d < 3; ++d) { g.bc_l[d] = p->bc_left[d]; g.bc_r[d] = p->bc_right[d]; }
     g.limiter = p->slope_limiter;
     g.spatial_order = p->spatial_order;
+    // TORCHGPU_ADDR64=1: the mixed kernel's 64-bit addressing (the path of
+    // grids of 2^29 cells and more) on any grid, so that tests reach it
+    const char* a64 = std::getenv("TORCHGPU_ADDR64");
+    g.addr64 = (a64 && a64[0] == '1') ? 1 : 0;
     g.ncell = (long long)p->nx * p->ny * p->nz;
     // Grid::initialise: dx[i] = sideLength / ncells[0] (Grid.cpp:471-472)
     g.dx = p->side_length / (double)p->nx;

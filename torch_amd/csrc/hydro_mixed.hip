@@ -521,7 +521,7 @@ static hipError_t launch_mixed_t(const HydroGeom& g, int stage, const float* qin
     dim3 grid(mtx * mty * mtz), block(MTHREADS);
     // 32-bit byte offsets within a variable while the FP64 arrays allow it
     // (ncell < 2^29, i.e. up to ~812^3 cells)
-    const bool a32 = g.ncell * (long long)sizeof(double) < (1LL << 32);
+    const bool a32 = g.ncell * (long long)sizeof(double) < (1LL << 32) && !g.addr64;
     if (stage == 1) {
         if (a32)
             hipLaunchKernelGGL((hydro_stage_mixed_kernel<SOLVER, ALB, 1, true>), grid, block, 0, stream, g, mtx, mty,

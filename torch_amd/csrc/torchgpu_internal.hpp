@@ -16,6 +16,7 @@ struct HydroGeom {
     int bc_l[3], bc_r[3];
     int limiter;
     int spatial_order;
+    int addr64;             // mixed kernel: 64-bit addressing even where 32-bit offsets fit
     long long ncell;        // nx*ny*nz, SoA stride
     double dx;
     double kflux;           // (area / vol) of a Cartesian face, Hydro.cpp:129-130
