@@ -402,6 +402,52 @@ __global__ void apply_rate_fix_kernel(HydroGeom g, double* __restrict__ U, const
     }
 }
 
+// The cooling sub-step's update of U (as apply_rate_fix_kernel:
+// Thermodynamics::updateSourceTerms + advSolution + fixSolution) fused with
+// the float Q the next mixed stage reads (as q_float_kernel in fix mode), so
+// the updated state is not read back.  A halted run leaves U as it is but
+// still refreshes the float Q from it.  Compiled without FMA contraction,
+// as the FP64 grids' update: the same arithmetic whichever kernel applies
+// a cooling sub-step.
+__global__ void apply_rate_fix_qf_kernel(HydroGeom g, double* __restrict__ U, const double* __restrict__ rate,
+                                         double dtf, StepState* st, unsigned int* cerr, float* __restrict__ qout) {
+    const bool halted = dtf < 0.0 && st->halted;
+    if (!halted && cerr && blockIdx.x == 0 && threadIdx.x == 0) {
+        const unsigned int e = *cerr;
+        if (e & ERR_THERMO_NSTEPS) {
+            set_error(st, ERR_THERMO_NSTEPS);
+            *cerr = 0u;
+        }
+    }
+    if (!halted) dtf = stage_dtf(dtf, st, 0);
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < g.ncell; c += (long long)gridDim.x * blockDim.x) {
+        double u[7];
+#pragma unroll
+        for (int v = 0; v < 7; ++v) u[v] = U[v * g.ncell + c];
+        if (!halted) {
+            double udot[7] = {0, 0, 0, 0, 0, 0, 0};
+            udot[PRE] += rate ? rate[c] : 0.0;
+#pragma unroll
+            for (int v = 0; v < 7; ++v) u[v] += dtf * udot[v];
+            if (fix_solution(g, u)) set_error(st, ERR_THERMO_FIX);
+#pragma unroll
+            for (int v = 0; v < 7; ++v) U[v * g.ncell + c] = u[v];
+        }
+        double q[7];
+        q_from_u_mixed(q, u, g.gm1);
+        store_qf(g, q, u, qout, c, 1, false);
+    }
+}
+
+hipError_t launch_fix_solution_qf(const HydroGeom& g, double* U, const double* rate, double dtf, StepState* st,
+                                  hipStream_t stream, unsigned int* cerr, float* qout) {
+    long long nb = (g.ncell + 255) / 256;
+    if (nb > 65536) nb = 65536;
+    hipLaunchKernelGGL(apply_rate_fix_qf_kernel, dim3((unsigned)nb), dim3(256), 0, stream, g, U, rate, dtf, st, cerr,
+                       qout);
+    return hipGetLastError();
+}
+
 // A halted run's single-stage sub-step (dt code DT_HALF / DT_FULL) still
 // passes its input through to its output, float primitives included: the
 // host swaps the two state buffers after every such sub-step without knowing

@@ -318,4 +318,44 @@ __device__ __forceinline__ void load_q(const HydroGeom& g, const double* __restr
     q_of_cell<STAGE, MIXED>(g, u, src.mode, src.mdim, q, st, check);
 }
 
+// Element (v, c) of a [7][ncell] state array.  A32: the variable's base
+// (uniform, in scalar registers) plus a 32-bit byte offset, which the global
+// loads and stores take as they are -- no 64-bit address arithmetic per
+// access in vector registers; needs ncell * sizeof(T) < 2^32.
+template <bool A32, typename T>
+__device__ __forceinline__ T* soa_at(T* base, long long ncell, int v, long long c) {
+    if constexpr (A32) return (T*)((char*)(base + v * ncell) + (unsigned int)((unsigned int)c * (unsigned int)sizeof(T)));
+    else return base + v * ncell + c;
+}
+
+
+// The float primitives a stage reads, written per cell by the stage (or
+// pass) that produced U: Q = QfromU(U) in FP64, rounded to float (not
+// scaled).  In fix mode (the input of a predictor or single-stage sub-step)
+// fixPrimitives is applied and Torch::checkValues("hydro before")
+// (Torch.cpp:420-443) is evaluated on U: a failing cell is stored with a NaN
+// density, which the consuming predictor reports at its step.
+template <bool A32 = false>
+__device__ __forceinline__ void store_qf(const HydroGeom& g, double q[7], const double u[7], float* __restrict__ qout,
+                                         long long c, int fix, bool fixed_already) {
+    if (fix) {
+        if (!fixed_already) fix_primitives_mixed(g, q, u[DEN]);
+        bool bad = (q[DEN] == 0.0) || (q[PRE] == 0.0);
+#pragma unroll
+        for (int v = 0; v < 7; ++v) bad |= (u[v] != u[v]) || isinf(u[v]);
+        if (bad) q[DEN] = __builtin_nan("");
+    }
+#pragma unroll
+    for (int v = 0; v < 7; ++v) *soa_at<A32>(qout, g.ncell, v, c) = (float)q[v];
+}
+
+// The float Q of a mixed-precision grid's cell for the next stage from its
+// FP64 state (q_float_kernel in fix mode).
+__device__ __forceinline__ void store_q_float(const HydroGeom& g, const double u[7], float* __restrict__ qout,
+                                              long long c) {
+    double q[7];
+    q_from_u_mixed(q, u, g.gm1);
+    store_qf(g, q, u, qout, c, 1, false);
+}
+
 }  // namespace tg

@@ -110,35 +110,6 @@ __device__ __forceinline__ void apply_ghost_mode_f(float q[7], int mode, int dim
 
 __device__ __forceinline__ unsigned int max_bits_f(float v) { return v > 0.f ? __float_as_uint(v) : 0u; }
 
-// Element (v, c) of a [7][ncell] state array.  A32: the variable's base
-// (uniform, in scalar registers) plus a 32-bit byte offset, which the global
-// loads and stores take as they are -- no 64-bit address arithmetic per
-// access in vector registers; needs ncell * sizeof(T) < 2^32.
-template <bool A32, typename T>
-__device__ __forceinline__ T* soa_at(T* base, long long ncell, int v, long long c) {
-    if constexpr (A32) return (T*)((char*)(base + v * ncell) + (unsigned int)((unsigned int)c * (unsigned int)sizeof(T)));
-    else return base + v * ncell + c;
-}
-
-// The float primitives a stage reads, written per cell by the stage (or
-// pass) that produced U: Q = QfromU(U) in FP64, rounded to float (not
-// scaled).  In fix mode (the input of a predictor or single-stage sub-step)
-// fixPrimitives is applied and Torch::checkValues("hydro before")
-// (Torch.cpp:420-443) is evaluated on U: a failing cell is stored with a NaN
-// density, which the consuming predictor reports at its step.
-template <bool A32 = false>
-__device__ __forceinline__ void store_qf(const HydroGeom& g, double q[7], const double u[7], float* __restrict__ qout,
-                                         long long c, int fix, bool fixed_already) {
-    if (fix) {
-        if (!fixed_already) fix_primitives_mixed(g, q, u[DEN]);
-        bool bad = (q[DEN] == 0.0) || (q[PRE] == 0.0);
-#pragma unroll
-        for (int v = 0; v < 7; ++v) bad |= (u[v] != u[v]) || isinf(u[v]);
-        if (bad) q[DEN] = __builtin_nan("");
-    }
-#pragma unroll
-    for (int v = 0; v < 7; ++v) *soa_at<A32>(qout, g.ncell, v, c) = (float)q[v];
-}
 
 #ifndef TG_MTX
 #define TG_MTX 16
@@ -564,50 +535,6 @@ __global__ __launch_bounds__(256) void q_float_kernel(HydroGeom g, const double*
         q_from_u_mixed(qd, u, g.gm1);
         store_qf(g, qd, u, q, c, fix, false);
     }
-}
-
-// The cooling sub-step's update of U (as apply_rate_fix_kernel:
-// Thermodynamics::updateSourceTerms + advSolution + fixSolution) fused with
-// the float Q the next mixed stage reads (as q_float_kernel in fix mode), so
-// the updated state is not read back.  A halted run leaves U as it is but
-// still refreshes the float Q from it.
-__global__ void apply_rate_fix_qf_kernel(HydroGeom g, double* __restrict__ U, const double* __restrict__ rate,
-                                         double dtf, StepState* st, unsigned int* cerr, float* __restrict__ qout) {
-    const bool halted = dtf < 0.0 && st->halted;
-    if (!halted && cerr && blockIdx.x == 0 && threadIdx.x == 0) {
-        const unsigned int e = *cerr;
-        if (e & ERR_THERMO_NSTEPS) {
-            set_error(st, ERR_THERMO_NSTEPS);
-            *cerr = 0u;
-        }
-    }
-    if (!halted) dtf = stage_dtf(dtf, st, 0);
-    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < g.ncell; c += (long long)gridDim.x * blockDim.x) {
-        double u[7];
-#pragma unroll
-        for (int v = 0; v < 7; ++v) u[v] = U[v * g.ncell + c];
-        if (!halted) {
-            double udot[7] = {0, 0, 0, 0, 0, 0, 0};
-            udot[PRE] += rate ? rate[c] : 0.0;
-#pragma unroll
-            for (int v = 0; v < 7; ++v) u[v] += dtf * udot[v];
-            if (fix_solution(g, u)) set_error(st, ERR_THERMO_FIX);
-#pragma unroll
-            for (int v = 0; v < 7; ++v) U[v * g.ncell + c] = u[v];
-        }
-        double q[7];
-        q_from_u_mixed(q, u, g.gm1);
-        store_qf(g, q, u, qout, c, 1, false);
-    }
-}
-
-hipError_t launch_fix_solution_qf(const HydroGeom& g, double* U, const double* rate, double dtf, StepState* st,
-                                  hipStream_t stream, unsigned int* cerr, float* qout) {
-    long long nb = (g.ncell + 255) / 256;
-    if (nb > 65536) nb = 65536;
-    hipLaunchKernelGGL(apply_rate_fix_qf_kernel, dim3((unsigned)nb), dim3(256), 0, stream, g, U, rate, dtf, st, cerr,
-                       qout);
-    return hipGetLastError();
 }
 
 hipError_t launch_q_float(const HydroGeom& g, const double* U, float* q, int fix, hipStream_t stream) {

@@ -36,6 +36,7 @@
 #include <mutex>
 
 #include "../../include/torchgpu.h"
+#include "hydro_device.hpp"
 #include "hydro_math.hpp"
 #include "mlp.hpp"
 #include "star_device.hpp"
@@ -334,10 +335,11 @@ __device__ __forceinline__ double grid_dt(const GridSrc& gs) {
     return gs.dt < 0.0 ? ((gs.dt == DT_HALF) ? 0.5 * gs.st->dt : gs.st->dt) : gs.dt;
 }
 
-__device__ __forceinline__ void gather_cell(const GridSrc& gs, long long c, double dt, double in[9]) {
+__device__ __forceinline__ void gather_cell(const GridSrc& gs, long long c, double dt, double in[9],
+                                            double u[7]) {
     const HydroGeom& g = gs.g;
     const long long n = g.ncell;
-    double u[7], q[7];
+    double q[7];
 #pragma unroll
     for (int v = 0; v < 7; ++v) u[v] = gs.U[v * n + c];
     QfromU(q, u, g.gm1);
@@ -367,12 +369,37 @@ __device__ __forceinline__ void gather_cell(const GridSrc& gs, long long c, doub
     in[8] = dt;
 }
 
+// Thermodynamics::updateSourceTerms + Fluid::advSolution + fixSolution of
+// one cell (apply_rate_fix_kernel's arithmetic): U[PRE] += dtf RATE, then
+// fixSolution; with qf (a mixed-precision grid) the float Q of the result
+// for the next hydro stage.  A halted run's cell keeps U but still gets
+// its float Q.  U is not restrict: in the prepass it is the memory the
+// cell's inputs were read from (GridSrc::U).
+__device__ __forceinline__ void apply_cell(const HydroGeom& g, double u[7], double rate, double dtf, bool halted,
+                                           double* U, float* __restrict__ qf, StepState* st,
+                                           long long c) {
+    if (!halted) {
+        double udot[7] = {0, 0, 0, 0, 0, 0, 0};
+        udot[PRE] += rate;
+#pragma unroll
+        for (int v = 0; v < 7; ++v) u[v] += dtf * udot[v];
+        if (fix_solution(g, u)) set_error(st, ERR_THERMO_FIX);
+#pragma unroll
+        for (int v = 0; v < 7; ++v) U[v * g.ncell + c] = u[v];
+    }
+    if (qf) store_q_float(g, u, qf, c);
+}
+
 // counters: [2] surrogate queue length, [3] fallback queue length.
 // GRID: the inputs of each cell are formed from the grid state (gs, as
 // thermo_gather_kernel) in registers, and written to `in` only for the
 // cells queued for the network or the fallback, which read them later.
-template <bool GRID>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void surrogate_prepass_kernel(ThermoConsts t, const double* __restrict__ tables,
+// APPLY (with GRID): a cell finished here is also updated in the grid state
+// (apply_cell into Uw / qf, dt from gs), and its `out` row is not written;
+// the queued cells are updated by queue_apply_kernel once their rate is
+// known (path marks them).
+template <bool GRID, bool APPLY = false>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void surrogate_prepass_kernel(ThermoConsts t, const double* __restrict__ tables,
                                                                SurrogateConsts sc, long long n,
                                                                double* __restrict__ in, double* __restrict__ out,
                                                                unsigned char* path, long long* __restrict__ mlp_list,
@@ -380,10 +407,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
                                                                float* __restrict__ raw_out,
                                                                long long* __restrict__ fb_list,
                                                                unsigned long long* __restrict__ counters,
-                                                               unsigned int* __restrict__ err, GridSrc gs) {
+                                                               unsigned int* __restrict__ err, GridSrc gs,
+                                                               double* Uw, float* __restrict__ qf,
+                                                               StepState* stw) {
     __shared__ double sx[26], sy[26], sy2[26];
     load_tables(tables, sx, sy, sy2);
     const double gdt = GRID ? grid_dt(gs) : 0.0;
+    const bool halted = APPLY && gs.dt < 0.0 && gs.st->halted;
     const long long nloop = ((n + blockDim.x - 1) / blockDim.x) * blockDim.x;  // uniform trip count per wave
     for (long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x; cell - threadIdx.x < nloop;
          cell += (long long)gridDim.x * blockDim.x) {
@@ -392,11 +422,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
         CellState s;
         double u[8];
         double Tc = 0.0;
-        double lin[9];
+        double lin[9], lu[7];
         const double* cin = in;  // this cell's inputs: cin[k * cstride + ccell]
         long long cstride = n, ccell = cell;
         if (GRID && valid) {
-            gather_cell(gs, cell, gdt, lin);
+            gather_cell(gs, cell, gdt, lin, lu);
             cin = lin;
             cstride = 1;
             ccell = 0;
@@ -408,7 +438,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
                 active = false;
             }
             if (!active) {
-                thermo_finish(s, out, n, cell);
+                if (APPLY) {
+                    // the rate thermo_finish would store (Thermodynamics.cpp:334)
+                    const double rate = s.dpre2rate == 0 ? s.rate_pre : (s.pressure - s.pre) * s.dpre2rate;
+                    apply_cell(gs.g, lu, rate, gdt, halted, Uw, qf, stw, cell);
+                } else {
+                    thermo_finish(s, out, n, cell);
+                }
                 if (path) path[cell] = 0;
             } else {
                 Tc = coarse_end_temperature(t, sx, sy, sy2, s);
@@ -458,6 +494,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
             fb_list[fslot] = cell;
             if (path) path[cell] = 2;
         }
+    }
+}
+
+// The grid update (apply_cell) of the cells the surrogate step queued
+// (path 1: the network's rate; path 2: the FP64 fallback's), after both
+// have run; the prepass updated the rest.  cerr: the cooling operator's
+// error word; a subcycle count beyond the int range is moved into the step
+// state (reported at this step, as the fixSolution errors) and cleared.
+__global__ void queue_apply_kernel(GridSrc gs, const double* __restrict__ out, const unsigned char* __restrict__ path,
+                                   double* __restrict__ U, float* __restrict__ qf, StepState* st, unsigned int* cerr) {
+    const bool halted = gs.dt < 0.0 && gs.st->halted;
+    if (!halted && cerr && blockIdx.x == 0 && threadIdx.x == 0) {
+        const unsigned int e = *cerr;
+        if (e & ERR_THERMO_NSTEPS) {
+            set_error(st, ERR_THERMO_NSTEPS);
+            *cerr = 0u;
+        }
+    }
+    const double dtf = grid_dt(gs);
+    const long long n = gs.g.ncell;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+        if (path[c] == 0) continue;
+        double u[7];
+#pragma unroll
+        for (int v = 0; v < 7; ++v) u[v] = U[v * n + c];
+        apply_cell(gs.g, u, out[2 * n + c], dtf, halted, U, qf, st, c);
     }
 }
 
@@ -531,8 +593,8 @@ __global__ void thermo_gather_kernel(GridSrc gs, double* __restrict__ in9) {
     const long long n = gs.g.ncell;
     const double dt = grid_dt(gs);
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
-        double in[9];
-        gather_cell(gs, c, dt, in);
+        double in[9], u[7];
+        gather_cell(gs, c, dt, in, u);
 #pragma unroll
         for (int k = 0; k < 9; ++k) in9[k * n + c] = in[k];
     }
@@ -555,11 +617,11 @@ __global__ __launch_bounds__(256) void thermo_timestep_kernel(ThermoConsts t, co
         CellState s;
         ThermoConsts t1 = t;
         t1.subcycling = 0;  // pre-step rate only
-        double lin[9];
+        double lin[9], lu[7];
         const double* cin = in9;
         long long cstride = n, ccell = c;
         if (GRID) {
-            gather_cell(gs, c, 0.0, lin);
+            gather_cell(gs, c, 0.0, lin, lu);
             cin = lin;
             cstride = 1;
             ccell = 0;
@@ -785,6 +847,7 @@ struct torchgpu_cooling {
     long long grid_cap = 0;
     double* in9 = nullptr;
     double* out5 = nullptr;
+    unsigned char* gpath = nullptr;  // per grid cell: the surrogate step's path (0 exact, 1 network, 2 fallback)
     unsigned long long* dtbits = nullptr;
     // star (torchgpu_cooling_set_star): geometry and the DCOL_DEN of the
     // ray trace (COL_DEN goes straight into in9)
@@ -1018,7 +1081,8 @@ void torchgpu_cooling_destroy(torchgpu_cooling* c) {
     if (!c) return;
     (void)hipStreamSynchronize(c->stream);
     for (void* ptr : {(void*)c->tables, (void*)c->counters, (void*)c->err, (void*)c->mlp_list, (void*)c->fb_list,
-                      (void*)c->feat, (void*)c->y, (void*)c->sdata, (void*)c->in9, (void*)c->out5, (void*)c->dtbits,
+                      (void*)c->feat, (void*)c->y, (void*)c->sdata, (void*)c->in9, (void*)c->out5, (void*)c->gpath,
+                      (void*)c->dtbits,
                       (void*)c->lpt_order, (void*)c->lpt_bucket, (void*)c->lpt_hist, (void*)c->dcol,
                       (void*)c->col, (void*)c->xface, (void*)c->tmin})
         if (ptr) (void)hipFree(ptr);
@@ -1118,8 +1182,17 @@ int torchgpu_cooling_set_surrogate(torchgpu_cooling* c, torchgpu_mlp* m, const t
 
 // The surrogate step on `in` (gs NULL), or on the grid state (gs: the inputs
 // are formed in the prepass and `in` receives those of the queued cells).
+// The grid update fused into a grid surrogate step (see
+// surrogate_prepass_kernel's APPLY): the state, its float Q (NULL for FP64
+// grids) and the step state.
+struct ApplyDst {
+    double* U;
+    float* qf;
+    tg::StepState* st;
+};
+
 static int surrogate_step(torchgpu_cooling* c, int64_t n, double* in, double* out, uint8_t* path, int64_t* counts,
-                          const tg::GridSrc* gs) {
+                          const tg::GridSrc* gs, const ApplyDst* apply = nullptr) {
     if (!c->has_surrogate) return TORCHGPU_ERR_NOT_TRAINED;
     if (n == 0) return TORCHGPU_OK;
     int rc = ensure_queues(c, n);
@@ -1132,14 +1205,20 @@ static int surrogate_step(torchgpu_cooling* c, int64_t n, double* in, double* ou
     sc.train = 0;
     long long nb = (n + 255) / 256;
     if (nb > (long long)c->nsm * 16) nb = (long long)c->nsm * 16;
-    if (gs)
-        hipLaunchKernelGGL(tg::surrogate_prepass_kernel<true>, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t,
-                           c->tables, sc, (long long)n, in, out, path, c->mlp_list, c->feat, c->sdata, (float*)nullptr,
-                           c->fb_list, c->counters, c->err, *gs);
+    if (gs && apply)
+        hipLaunchKernelGGL((tg::surrogate_prepass_kernel<true, true>), dim3((unsigned)nb), dim3(256), 0, c->stream,
+                           c->t, c->tables, sc, (long long)n, in, out, path, c->mlp_list, c->feat, c->sdata,
+                           (float*)nullptr, c->fb_list, c->counters, c->err, *gs, apply->U, apply->qf, apply->st);
+    else if (gs)
+        hipLaunchKernelGGL((tg::surrogate_prepass_kernel<true, false>), dim3((unsigned)nb), dim3(256), 0, c->stream,
+                           c->t, c->tables, sc, (long long)n, in, out, path, c->mlp_list, c->feat, c->sdata,
+                           (float*)nullptr, c->fb_list, c->counters, c->err, *gs, (double*)nullptr, (float*)nullptr,
+                           (tg::StepState*)nullptr);
     else
-        hipLaunchKernelGGL(tg::surrogate_prepass_kernel<false>, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t,
-                           c->tables, sc, (long long)n, in, out, path, c->mlp_list, c->feat, c->sdata, (float*)nullptr,
-                           c->fb_list, c->counters, c->err, tg::GridSrc{});
+        hipLaunchKernelGGL((tg::surrogate_prepass_kernel<false, false>), dim3((unsigned)nb), dim3(256), 0, c->stream,
+                           c->t, c->tables, sc, (long long)n, in, out, path, c->mlp_list, c->feat, c->sdata,
+                           (float*)nullptr, c->fb_list, c->counters, c->err, tg::GridSrc{}, (double*)nullptr,
+                           (float*)nullptr, (tg::StepState*)nullptr);
     HIP_OK(hipGetLastError());
     HIP_OK(tg::launch_mlp_infer_dev(w16, b, c->feat, c->counters + 2, n, c->y, c->nsm, c->stream));
     hipLaunchKernelGGL(tg::surrogate_finish_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t, sc,
@@ -1191,9 +1270,10 @@ int torchgpu_surrogate_training_data(torchgpu_cooling* c, int64_t n, const doubl
     };
     long long nb = (n + 255) / 256;
     if (nb > (long long)c->nsm * 16) nb = (long long)c->nsm * 16;
-    hipLaunchKernelGGL(tg::surrogate_prepass_kernel<false>, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t,
-                       c->tables, sc, (long long)n, const_cast<double*>(in), scratch, (unsigned char*)nullptr,
-                       (long long*)list, feat, c->sdata, raw, c->fb_list, c->counters, c->err, tg::GridSrc{});
+    hipLaunchKernelGGL((tg::surrogate_prepass_kernel<false, false>), dim3((unsigned)nb), dim3(256), 0, c->stream,
+                       c->t, c->tables, sc, (long long)n, const_cast<double*>(in), scratch, (unsigned char*)nullptr,
+                       (long long*)list, feat, c->sdata, raw, c->fb_list, c->counters, c->err, tg::GridSrc{},
+                       (double*)nullptr, (float*)nullptr, (tg::StepState*)nullptr);
     keep(hipGetLastError());
     if (e == hipSuccess) {
         hipLaunchKernelGGL(tg::surrogate_targets_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, c->counters,
@@ -1214,10 +1294,13 @@ static int ensure_grid(torchgpu_cooling* c, long long n) {
     (void)hipStreamSynchronize(c->stream);
     if (c->in9) (void)hipFree(c->in9);
     if (c->out5) (void)hipFree(c->out5);
+    if (c->gpath) (void)hipFree(c->gpath);
     c->in9 = c->out5 = nullptr;
+    c->gpath = nullptr;
     c->grid_cap = 0;
     HIP_OK(hipMalloc(&c->in9, 9 * n * sizeof(double)));
     HIP_OK(hipMalloc(&c->out5, 5 * n * sizeof(double)));
+    HIP_OK(hipMalloc(&c->gpath, n));
     // finite contents everywhere: the ray trace reads columns of zero weight
     hipLaunchKernelGGL(tg::zero_f64_kernel, dim3(1024), dim3(256), 0, c->stream, c->in9, 9 * n);
     HIP_OK(hipGetLastError());
@@ -1302,9 +1385,23 @@ static int grid_substep_dev(torchgpu_ctx* ctx, torchgpu_cooling* c, const double
     const bool direct = surrogate && !(v.heat && c->t.subcycling);
     int rc;
     if (direct) {
+        // the inputs formed in the prepass, which also applies the update
+        // (Thermodynamics::updateSourceTerms + advSolution(dt) +
+        // fixSolution) to the cells it finishes; the queued cells get it
+        // once the network and the fallback have run
         tg::GridSrc gs;
         if ((rc = grid_src(c, ctx, T_min, col_den, F, excluded, dt, gs))) return rc;
-        rc = surrogate_step(c, n, c->in9, c->out5, nullptr, nullptr, &gs);
+        float* qf = tg::ctx_qf_for_state(ctx);
+        const ApplyDst ad{v.U, qf, v.st};
+        if ((rc = surrogate_step(c, n, c->in9, c->out5, c->gpath, nullptr, &gs, &ad))) return rc;
+        long long nb = (n + 255) / 256;
+        if (nb > 65536) nb = 65536;
+        hipLaunchKernelGGL(tg::queue_apply_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, gs, c->out5, c->gpath,
+                           v.U, qf, v.st, c->err);
+        HIP_OK(hipGetLastError());
+        if (qf) tg::ctx_qf_written(ctx);
+        else tg::ctx_state_changed(ctx);
+        return TORCHGPU_OK;
     } else {
         if ((rc = gather(c, ctx, T_min, col_den, F, excluded, dt))) return rc;
         if (surrogate) rc = surrogate_step(c, n, c->in9, c->out5, nullptr, nullptr, nullptr);
