@@ -504,7 +504,9 @@ def main():
         log(f"star {star} ({time.perf_counter() - t0:.1f}s)")
     ncell = n ** 3
     achieved = BYTES_STAGE2 * ncell / (mx["stage2_ms"] * 1e-3) / 1e9
-    traffic = measured_traffic(n, "void tg::hydro_stage_mixed_kernel<2, true, 2>")
+    # the kernel instance with 32-bit byte offsets runs on grids of fewer than 2^29 cells
+    a32 = "true" if ncell * 8 < (1 << 32) else "false"
+    traffic = measured_traffic(n, f"void tg::hydro_stage_mixed_kernel<2, true, 2, {a32}>")
     out = {
         "metric": "Mcell-updates/s: FP64 vs mixed hydro; surrogate vs subcycled cooling",
         "value": round(mx["mcells"], 2),

@@ -394,12 +394,13 @@ __device__ __forceinline__ void apply_cell(const HydroGeom& g, double u[7], doub
 // GRID: the inputs of each cell are formed from the grid state (gs, as
 // thermo_gather_kernel) in registers, and written to `in` only for the
 // cells queued for the network or the fallback, which read them later.
-// APPLY (with GRID): a cell finished here is also updated in the grid state
-// (apply_cell into Uw / qf, dt from gs), and its `out` row is not written;
-// the queued cells are updated by queue_apply_kernel once their rate is
-// known (path marks them).
-template <bool GRID, bool APPLY = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void surrogate_prepass_kernel(ThermoConsts t, const double* __restrict__ tables,
+// A GRID cell finished here is also updated in the grid state (apply_cell
+// into Uw / qf, dt from gs), and its `out` row is not written; the queued
+// cells are updated by queue_apply_kernel once their rate is known (path
+// marks them).  The GRID instance runs at two waves per SIMD: at four it
+// spills ~150 registers to scratch and is slower; the other at four.
+template <bool GRID>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GRID ? 2 : 4))) void surrogate_prepass_kernel(ThermoConsts t, const double* __restrict__ tables,
                                                                SurrogateConsts sc, long long n,
                                                                double* __restrict__ in, double* __restrict__ out,
                                                                unsigned char* path, long long* __restrict__ mlp_list,
@@ -413,7 +414,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void s
     __shared__ double sx[26], sy[26], sy2[26];
     load_tables(tables, sx, sy, sy2);
     const double gdt = GRID ? grid_dt(gs) : 0.0;
-    const bool halted = APPLY && gs.dt < 0.0 && gs.st->halted;
+    const bool halted = GRID && gs.dt < 0.0 && gs.st->halted;
     const long long nloop = ((n + blockDim.x - 1) / blockDim.x) * blockDim.x;  // uniform trip count per wave
     for (long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x; cell - threadIdx.x < nloop;
          cell += (long long)gridDim.x * blockDim.x) {
@@ -438,7 +439,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void s
                 active = false;
             }
             if (!active) {
-                if (APPLY) {
+                if (GRID) {
                     // the rate thermo_finish would store (Thermodynamics.cpp:334)
                     const double rate = s.dpre2rate == 0 ? s.rate_pre : (s.pressure - s.pre) * s.dpre2rate;
                     apply_cell(gs.g, lu, rate, gdt, halted, Uw, qf, stw, cell);
@@ -1180,19 +1181,22 @@ int torchgpu_cooling_set_surrogate(torchgpu_cooling* c, torchgpu_mlp* m, const t
 
 }  // extern "C"
 
-// The surrogate step on `in` (gs NULL), or on the grid state (gs: the inputs
-// are formed in the prepass and `in` receives those of the queued cells).
-// The grid update fused into a grid surrogate step (see
-// surrogate_prepass_kernel's APPLY): the state, its float Q (NULL for FP64
-// grids) and the step state.
+// Where a grid surrogate step writes the cooling update of the cells its
+// prepass finishes (surrogate_prepass_kernel<true>): the state, its float Q
+// (NULL for FP64 grids) and the step state.
 struct ApplyDst {
     double* U;
     float* qf;
     tg::StepState* st;
 };
 
+// The surrogate step on `in` (gs and apply NULL), or on the grid state (gs
+// and apply both given: the inputs are formed in the prepass, `in` receives
+// those of the queued cells, and the prepass applies the update of the cells
+// it finishes; the caller applies the rest, marked in `path`).
 static int surrogate_step(torchgpu_cooling* c, int64_t n, double* in, double* out, uint8_t* path, int64_t* counts,
                           const tg::GridSrc* gs, const ApplyDst* apply = nullptr) {
+    if ((gs == nullptr) != (apply == nullptr)) return TORCHGPU_ERR_INVALID_ARG;
     if (!c->has_surrogate) return TORCHGPU_ERR_NOT_TRAINED;
     if (n == 0) return TORCHGPU_OK;
     int rc = ensure_queues(c, n);
@@ -1205,17 +1209,12 @@ static int surrogate_step(torchgpu_cooling* c, int64_t n, double* in, double* ou
     sc.train = 0;
     long long nb = (n + 255) / 256;
     if (nb > (long long)c->nsm * 16) nb = (long long)c->nsm * 16;
-    if (gs && apply)
-        hipLaunchKernelGGL((tg::surrogate_prepass_kernel<true, true>), dim3((unsigned)nb), dim3(256), 0, c->stream,
-                           c->t, c->tables, sc, (long long)n, in, out, path, c->mlp_list, c->feat, c->sdata,
-                           (float*)nullptr, c->fb_list, c->counters, c->err, *gs, apply->U, apply->qf, apply->st);
-    else if (gs)
-        hipLaunchKernelGGL((tg::surrogate_prepass_kernel<true, false>), dim3((unsigned)nb), dim3(256), 0, c->stream,
-                           c->t, c->tables, sc, (long long)n, in, out, path, c->mlp_list, c->feat, c->sdata,
-                           (float*)nullptr, c->fb_list, c->counters, c->err, *gs, (double*)nullptr, (float*)nullptr,
-                           (tg::StepState*)nullptr);
+    if (gs)
+        hipLaunchKernelGGL(tg::surrogate_prepass_kernel<true>, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t,
+                           c->tables, sc, (long long)n, in, out, path, c->mlp_list, c->feat, c->sdata, (float*)nullptr,
+                           c->fb_list, c->counters, c->err, *gs, apply->U, apply->qf, apply->st);
     else
-        hipLaunchKernelGGL((tg::surrogate_prepass_kernel<false, false>), dim3((unsigned)nb), dim3(256), 0, c->stream,
+        hipLaunchKernelGGL(tg::surrogate_prepass_kernel<false>, dim3((unsigned)nb), dim3(256), 0, c->stream,
                            c->t, c->tables, sc, (long long)n, in, out, path, c->mlp_list, c->feat, c->sdata,
                            (float*)nullptr, c->fb_list, c->counters, c->err, tg::GridSrc{}, (double*)nullptr,
                            (float*)nullptr, (tg::StepState*)nullptr);
@@ -1270,7 +1269,7 @@ int torchgpu_surrogate_training_data(torchgpu_cooling* c, int64_t n, const doubl
     };
     long long nb = (n + 255) / 256;
     if (nb > (long long)c->nsm * 16) nb = (long long)c->nsm * 16;
-    hipLaunchKernelGGL((tg::surrogate_prepass_kernel<false, false>), dim3((unsigned)nb), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL(tg::surrogate_prepass_kernel<false>, dim3((unsigned)nb), dim3(256), 0, c->stream,
                        c->t, c->tables, sc, (long long)n, const_cast<double*>(in), scratch, (unsigned char*)nullptr,
                        (long long*)list, feat, c->sdata, raw, c->fb_list, c->counters, c->err, tg::GridSrc{},
                        (double*)nullptr, (float*)nullptr, (tg::StepState*)nullptr);

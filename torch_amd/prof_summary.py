@@ -7,6 +7,7 @@
         --md profiles/hydro_stage_turb256.md --json profiles/hydro_stage_turb256_traffic.json
 
     python -m torch_amd.prof_summary db RESULTS.db [REGEX]   # kernel table of a rocprofv3 database
+    python -m torch_amd.prof_summary stats KERNEL_STATS.csv [REGEX]   # the same from a --stats CSV
 
 Inputs are the CSV files rocprofv3 writes for `--kernel-trace --stats` and
 for `--pmc` runs of bench.py and of torch_amd.hbm_probe (a device copy of
@@ -76,10 +77,27 @@ def db_kernel_table(path: str, keep: str = ".") -> list[str]:
     return out
 
 
+def stats_kernel_table(path: str, keep: str = ".") -> list[str]:
+    """The same rows from the `*_kernel_stats.csv` of a `--kernel-trace
+    --stats --output-format csv` run (rocprofv3 sorts it by total time)."""
+    out = ["| kernel | calls | total µs | avg µs | min µs | max µs |", "|---|---|---|---|---|---|"]
+    pat = re.compile(keep)
+    with open(path) as fh:
+        for r in csv.DictReader(fh):
+            k = short(r["Name"])
+            if pat.search(k):
+                out.append(f"| `{k}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e3:.1f} | "
+                           f"{float(r['AverageNs']) / 1e3:.2f} | {float(r['MinNs']) / 1e3:.2f} | "
+                           f"{float(r['MaxNs']) / 1e3:.2f} |")
+    return out
+
+
 def main():
-    if len(sys.argv) > 1 and sys.argv[1] == "db":
+    if len(sys.argv) > 1 and sys.argv[1] in ("db", "stats"):
         # python -m torch_amd.prof_summary db RESULTS.db [REGEX]
-        print("\n".join(db_kernel_table(sys.argv[2], sys.argv[3] if len(sys.argv) > 3 else ".")))
+        # python -m torch_amd.prof_summary stats KERNEL_STATS.csv [REGEX]
+        table = db_kernel_table if sys.argv[1] == "db" else stats_kernel_table
+        print("\n".join(table(sys.argv[2], sys.argv[3] if len(sys.argv) > 3 else ".")))
         return
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--grid", type=int, required=True)
@@ -110,8 +128,8 @@ def main():
     lines = [f"# hydro stage kernels, {a.grid}^3 turbulent box: rocprofv3 summaries", ""]
     lines += [f"Workload: `{a.driver}` ({a.workload}, RotatedHLLC + van Albada, outflow); "
               "FP64 parity kernel `hydro_stage_kernel<solver, stage>`, mixed-precision kernel "
-              "`hydro_stage_mixed_kernel<solver, albada, stage>` (solver 2 = RotatedHLLC; stage 1 predictor, "
-              "2 corrector + CFL).", ""]
+              "`hydro_stage_mixed_kernel<solver, albada, stage, a32>` (solver 2 = RotatedHLLC; stage 1 predictor, "
+              "2 corrector + CFL; a32: 32-bit byte offsets into the state, grids below 2^29 cells).", ""]
     if a.commands:
         with open(a.commands) as fh:
             cmds = fh.read().strip()
