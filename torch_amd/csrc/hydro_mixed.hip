@@ -10,18 +10,21 @@
 // keeps the float primitives of its own column at planes k .. k+2 and the
 // limited z slope of plane k in registers, so the z faces need neither LDS
 // nor a reloaded z halo, and each z slope serves both z faces of its cell;
-// plane k
-// with its 2-deep x and y halos (256 + 128 cells, float [7][384], one plane
-// per variable) goes through LDS for the 544 x and y faces, which the 256
-// threads take as one flat list (consecutive lanes read consecutive x).  The
-// z flux of face k+1/2 waits in LDS for the update; the thread keeps it in
-// registers as face k+1/2 of the next plane.  Per cell that is 3.125 face
+// plane k with its 2-deep x and y halos (256 + 128 cells, float [7][384],
+// one plane per variable) goes through LDS.  Each thread limits the x and y
+// slopes of its own cell there once (and 64 threads those of the 1-deep
+// halo cells), so every slope serves both faces of its cell as in z; then
+// the 544 x and y faces, which the 256 threads take as one flat list
+// (consecutive lanes read consecutive x), are solved from the primitives
+// and slopes of their two cells, and their fluxes replace the slopes in LDS
+// once every face has read them.  The thread keeps the z flux of face
+// k+1/2 in registers for the update and as face k-1/2 of the next plane.  Per cell that is 3.125 face
 // solves (plus 1/MZ for the first z face of a column) and 1.5 cell loads, where
 // 8^3 tiles with 2-deep halos in all three directions needed 3.375 and 2.5.
 // After the faces each thread updates its cell of plane k in FP64 (fluxes
 // accumulated in the reference's order x, y, z; += F_L, -= F_R), applies
 // fixSolution, writes U, the float primitives of the output for the next
-// stage and (corrector) its CFL term.  33 KB LDS, 128 VGPRs: four waves per
+// stage and (corrector) its CFL term.  34 KB LDS, 128 VGPRs: four waves per
 // SIMD, four workgroups per CU (neither the registers nor the LDS allow a
 // fifth).
 //
@@ -123,11 +126,28 @@ constexpr int NXF = (MTX + 1) * MTY;      // x faces per plane
 constexpr int NYF = MTX * (MTY + 1);      // y faces per plane
 constexpr int MTHREADS = MCELLS;
 
+constexpr int NSX = (MTX + 2) * MTY;      // cells with an x slope per plane (x in [-1, MTX])
+constexpr int NSY = MTX * (MTY + 2);      // cells with a y slope per plane (y in [-1, MTY])
+static_assert(NXF == NYF && NSX == NSY, "x and y lists of one length (square columns)");
+
 struct MarchLDS {
     float p[7][PW * PH];    // plane k: cell (x, y) at (y + 2) PW + x + 2, x in [-2, MTX + 2), y in [-2, MTY + 2)
-    float fx[7][NXF];       // x faces of plane k: y (MTX + 1) + f
-    float fy[7][NYF];       // y faces of plane k: f MTX + x
-    float fz[7][MCELLS];    // z face k + 1/2 of each column
+    // the limited slopes of plane k, read by the face solves; then (after a
+    // barrier) the face fluxes, read by the update
+    union {
+        struct {
+            float x[7][NSX];  // x slope of cell (x, y): y (MTX + 2) + x + 1
+            float y[7][NSY];  // y slope of cell (x, y): (y + 1) MTX + x
+        } s;
+        struct {
+            float x[7][NXF];  // x faces of plane k: y (MTX + 1) + f
+            float y[7][NYF];  // y faces of plane k: f MTX + x
+        } f;
+    } sf;
+    // a thread's flux of its first face, and of its third (the threads below
+    // NXF + NYF - 2 MTHREADS), from the solve until the slopes are dead
+    float park0[7][MTHREADS];
+    float park2[7][NXF + NYF - 2 * MTHREADS];
     double red[MTHREADS / 64];
     unsigned int qmw[MTHREADS / 64][2];
 };
@@ -172,38 +192,9 @@ __device__ __forceinline__ void mhalo_pos(int h, int& x, int& y) {
     }
 }
 
-template <int SOLVER, bool ALB, int DIM>
-__device__ __forceinline__ bool recon_flux(const HydroGeom& g, const MixedScale& m, const float a[7], const float b[7],
-                                           const float cc[7], const float dd[7], const GasConsts<float>& gc,
-                                           float F[7]) {
-    float ql[7], qr[7];
-    if (g.spatial_order == 1) {
-        if constexpr (ALB) {
-#pragma unroll
-            for (int v = 0; v < 7; ++v) {
-                ql[v] = b[v] + 0.5f * albada_f(b[v] - a[v], cc[v] - b[v], thr_of(m, v));
-                qr[v] = cc[v] - 0.5f * albada_f(cc[v] - b[v], dd[v] - cc[v], thr_of(m, v));
-            }
-        } else {
-#pragma unroll
-            for (int v = 0; v < 7; ++v) {
-                ql[v] = b[v] + 0.5f * limiter_f(g.limiter, b[v] - a[v], cc[v] - b[v], thr_of(m, v));
-                qr[v] = cc[v] - 0.5f * limiter_f(g.limiter, cc[v] - b[v], dd[v] - cc[v], thr_of(m, v));
-            }
-        }
-    } else {
-#pragma unroll
-        for (int v = 0; v < 7; ++v) {
-            ql[v] = b[v];
-            qr[v] = cc[v];
-        }
-    }
-    return riemann_f<SOLVER, DIM>(F, ql, qr, gc, g.igm1f, m.nn_thr);
-}
-
 // Limited slope of cell b between its neighbours a and c (zero at first
-// order): the value recon_flux adds to b (times 1/2) on b's right face and
-// subtracts on its left face, so one evaluation serves both faces of b.
+// order): slope_flux adds it to b (times 1/2) on b's right face and
+// subtracts it on its left face, so one evaluation serves both faces of b.
 template <bool ALB>
 __device__ __forceinline__ void cell_slope(const HydroGeom& g, const MixedScale& m, const float a[7], const float b[7],
                                            const float c[7], float s[7]) {
@@ -219,12 +210,17 @@ __device__ __forceinline__ void cell_slope(const HydroGeom& g, const MixedScale&
     }
 }
 
-// Face flux between cells b and c given their limited slopes sb and sc
-// (cell_slope): the same face states as recon_flux on the stencil a b c d.
+// Face flux along axis DIM between cells b and c given their limited slopes
+// sb and sc (cell_slope): face states b + sb/2 and c - sc/2.  With yswap
+// (DIM 0 only) the face is a y face solved as an x face: the x and y
+// velocity components of both face states are exchanged before the solve
+// and of the flux after it (the Euler flux is covariant under that
+// reflection, and the limiter threshold is the same for all velocity
+// components).
 template <int SOLVER, int DIM>
 __device__ __forceinline__ bool slope_flux(const HydroGeom& g, const MixedScale& m, const float b[7], const float sb[7],
                                            const float c[7], const float sc[7], const GasConsts<float>& gc,
-                                           float F[7]) {
+                                           float F[7], bool yswap = false) {
     float ql[7], qr[7];
     if (g.spatial_order == 1) {
 #pragma unroll
@@ -239,56 +235,120 @@ __device__ __forceinline__ bool slope_flux(const HydroGeom& g, const MixedScale&
             qr[v] = c[v];
         }
     }
-    return riemann_f<SOLVER, DIM>(F, ql, qr, gc, g.igm1f, m.nn_thr);
+    if (DIM == 0 && yswap) {
+        const float lx = ql[VEL], rx = qr[VEL];
+        ql[VEL] = ql[VEL + 1];
+        qr[VEL] = qr[VEL + 1];
+        ql[VEL + 1] = lx;
+        qr[VEL + 1] = rx;
+    }
+    const bool bad = riemann_f<SOLVER, DIM>(F, ql, qr, gc, g.igm1f, m.nn_thr);
+    if (DIM == 0 && yswap) {
+        const float fx = F[VEL];
+        F[VEL] = F[VEL + 1];
+        F[VEL + 1] = fx;
+    }
+    return bad;
 }
 
-template <int STRIDE>
-__device__ __forceinline__ void read_stencil(const MarchLDS& s, int base, float a[7], float b[7], float cc[7],
-                                             float dd[7]) {
+// The x and y slopes of plane k (cell_slope on the plane in LDS): each
+// thread those of its own cell; threads below 4 MTX (MTX == MTY) the x slope
+// of an x-halo cell (x = -1 or MTX) or the y slope of a y-halo cell (y = -1
+// or MTY), which the boundary faces of the column read.
+template <bool ALB>
+__device__ __forceinline__ void march_slopes(const HydroGeom& g, MarchLDS& s, const MixedScale& m, int tid, int tx,
+                                             int ty, const float w[7]) {
+    float a[7], c[7], sl[7];
+    const int o = mslot(tx, ty);
 #pragma unroll
     for (int v = 0; v < 7; ++v) {
-        a[v] = s.p[v][base];
-        b[v] = s.p[v][base + STRIDE];
-        cc[v] = s.p[v][base + 2 * STRIDE];
-        dd[v] = s.p[v][base + 3 * STRIDE];
+        a[v] = s.p[v][o - 1];
+        c[v] = s.p[v][o + 1];
+    }
+    cell_slope<ALB>(g, m, a, w, c, sl);
+#pragma unroll
+    for (int v = 0; v < 7; ++v) s.sf.s.x[v][ty * (MTX + 2) + tx + 1] = sl[v];
+#pragma unroll
+    for (int v = 0; v < 7; ++v) {
+        a[v] = s.p[v][o - PW];
+        c[v] = s.p[v][o + PW];
+    }
+    cell_slope<ALB>(g, m, a, w, c, sl);
+#pragma unroll
+    for (int v = 0; v < 7; ++v) s.sf.s.y[v][(ty + 1) * MTX + tx] = sl[v];
+    if (tid < 4 * MTX) {
+        // halo cell h: x halos (x = -1, MTX; row h % MTY) then y halos
+        const bool yh = tid >= 2 * MTY;
+        const int h = yh ? tid - 2 * MTY : tid;
+        const int side = h / MTY, r = h - side * MTY;
+        const int hx = yh ? r : (side ? MTX : -1), hy = yh ? (side ? MTY : -1) : r;
+        const int ho = mslot(hx, hy), st = yh ? PW : 1;
+        float b[7];
+#pragma unroll
+        for (int v = 0; v < 7; ++v) {
+            a[v] = s.p[v][ho - st];
+            b[v] = s.p[v][ho];
+            c[v] = s.p[v][ho + st];
+        }
+        cell_slope<ALB>(g, m, a, b, c, sl);
+        float* dst = yh ? &s.sf.s.y[0][(hy + 1) * MTX + hx] : &s.sf.s.x[0][hy * (MTX + 2) + hx + 1];
+#pragma unroll
+        for (int v = 0; v < 7; ++v) dst[v * NSX] = sl[v];
     }
 }
 
 // One x face (item < NXF: row y = item / (MTX + 1), face f = item % (MTX + 1))
 // or y face (j = item - NXF: face f = j / MTX, column x = j % MTX) of plane
-// k; the four stencil cells are consecutive slots (x) or PW apart (y).
-template <int SOLVER, bool ALB, int STAGE>
-__device__ __forceinline__ void march_xy_face(const HydroGeom& g, MarchLDS& s, const MixedScale& m, int item, int ox,
-                                              int oy, StepState* st, const GasConsts<float>& gc) {
-    float a[7], b[7], cc[7], dd[7], F[7];
-    bool bad = false;
-    if (item < NXF) {
+// k from the primitives and limited slopes of its two cells: flux in F
+// (zero for a face outside the grid).  Both kinds go through one code path
+// (slope_flux's yswap), so the wave whose items straddle NXF does not run
+// the solver once per axis.
+template <int SOLVER, int STAGE>
+__device__ __forceinline__ void march_xy_face(const HydroGeom& g, const MarchLDS& s, const MixedScale& m, int item,
+                                              int ox, int oy, StepState* st, const GasConsts<float>& gc,
+                                              float F[7]) {
+    const bool yface = item >= NXF;
+    int qb, qs, sb, ss;
+    bool valid;
+    if (!yface) {
         const int y = item / (MTX + 1), f = item - y * (MTX + 1);
-        const bool valid = (ox + f <= g.nx) && (oy + y < g.ny);
-        if (valid) {
-            read_stencil<1>(s, mslot(f - 2, y), a, b, cc, dd);
-            bad = recon_flux<SOLVER, ALB, 0>(g, m, a, b, cc, dd, gc, F);
-        } else {
-#pragma unroll
-            for (int v = 0; v < 7; ++v) F[v] = 0.f;
-        }
-#pragma unroll
-        for (int v = 0; v < 7; ++v) s.fx[v][item] = F[v];
+        valid = (ox + f <= g.nx) && (oy + y < g.ny);
+        qb = mslot(f - 1, y);
+        qs = 1;
+        sb = y * (MTX + 2) + f;
+        ss = 1;
     } else {
         const int j = item - NXF;
         const int f = j / MTX, x = j - f * MTX;
-        const bool valid = (oy + f <= g.ny) && (ox + x < g.nx);
-        if (valid) {
-            read_stencil<PW>(s, mslot(x, f - 2), a, b, cc, dd);
-            bad = recon_flux<SOLVER, ALB, 1>(g, m, a, b, cc, dd, gc, F);
-        } else {
+        valid = (oy + f <= g.ny) && (ox + x < g.nx);
+        qb = mslot(x, f - 1);
+        qs = PW;
+        sb = NSX * 7 + f * MTX + x;  // into the y slopes, which follow the x slopes
+        ss = MTX;
+    }
+    bool bad = false;
+    if (valid) {
+        float b[7], c[7], sbv[7], scv[7];
+        const float* sl = &s.sf.s.x[0][0];
 #pragma unroll
-            for (int v = 0; v < 7; ++v) F[v] = 0.f;
+        for (int v = 0; v < 7; ++v) {
+            b[v] = s.p[v][qb];
+            c[v] = s.p[v][qb + qs];
+            sbv[v] = sl[v * NSX + sb];
+            scv[v] = sl[v * NSX + sb + ss];
         }
+        bad = slope_flux<SOLVER, 0>(g, m, b, sbv, c, scv, gc, F, yface);
+    } else {
 #pragma unroll
-        for (int v = 0; v < 7; ++v) s.fy[v][j] = F[v];
+        for (int v = 0; v < 7; ++v) F[v] = 0.f;
     }
     if (bad) set_error(st, STAGE == 1 ? ERR_RIEMANN_NAN_1 : ERR_RIEMANN_NAN_2);
+}
+
+__device__ __forceinline__ void store_xy_flux(MarchLDS& s, int item, const float F[7]) {
+    float* dst = item >= NXF ? &s.sf.f.y[0][item - NXF] : &s.sf.f.x[0][item];
+#pragma unroll
+    for (int v = 0; v < 7; ++v) dst[v * NXF] = F[v];
 }
 
 template <int SOLVER, bool ALB, int STAGE, bool INNER, bool A32>
@@ -344,10 +404,40 @@ __device__ __forceinline__ void march_column(const HydroGeom& g, MarchLDS& s, co
             mp = max(mp, max_bits_f(__builtin_ldexpf(w1[PRE], -ks[PRE])));
         }
         __syncthreads();
-        // z face k + 1/2 from registers (kept in LDS until the update), x and
-        // y faces of plane k from LDS
+        // x and y slopes of plane k, then its x and y faces; each flux waits
+        // (in LDS or, the last one solved, in registers) until every face has
+        // read the slopes, whose LDS the fluxes then take over
+        march_slopes<ALB>(g, s, m, tid, tx, ty, w1);
+        __syncthreads();
         {
-            float fzn[7], sn[7];
+            constexpr int N3 = NXF + NYF - 2 * MTHREADS;
+            const bool third = tid < N3;
+            float F[7];
+            if (third) {
+                march_xy_face<SOLVER, STAGE>(g, s, m, tid + 2 * MTHREADS, ox, oy, st, gc, F);
+#pragma unroll
+                for (int v = 0; v < 7; ++v) s.park2[v][tid] = F[v];
+            }
+            march_xy_face<SOLVER, STAGE>(g, s, m, tid, ox, oy, st, gc, F);
+#pragma unroll
+            for (int v = 0; v < 7; ++v) s.park0[v][tid] = F[v];
+            march_xy_face<SOLVER, STAGE>(g, s, m, tid + MTHREADS, ox, oy, st, gc, F);
+            __syncthreads();
+            store_xy_flux(s, tid + MTHREADS, F);
+#pragma unroll
+            for (int v = 0; v < 7; ++v) F[v] = s.park0[v][tid];
+            store_xy_flux(s, tid, F);
+            if (third) {
+#pragma unroll
+                for (int v = 0; v < 7; ++v) F[v] = s.park2[v][tid];
+                store_xy_flux(s, tid + 2 * MTHREADS, F);
+            }
+        }
+        // z face k + 1/2 from registers; the thread keeps its flux in registers
+        // for the update (parked in LDS instead, it measured slower)
+        float fzn[7];
+        {
+            float sn[7];
             cell_slope<ALB>(g, m, w1, w2, w3, sn);
             bool bad = false;
             if (col) bad = slope_flux<SOLVER, 2>(g, m, w1, sk, w2, sn, gc, fzn);
@@ -357,15 +447,8 @@ __device__ __forceinline__ void march_column(const HydroGeom& g, MarchLDS& s, co
             }
             if (bad) set_error(st, STAGE == 1 ? ERR_RIEMANN_NAN_1 : ERR_RIEMANN_NAN_2);
 #pragma unroll
-            for (int v = 0; v < 7; ++v) {
-                s.fz[v][tid] = fzn[v];
-                sk[v] = sn[v];
-            }
+            for (int v = 0; v < 7; ++v) sk[v] = sn[v];
         }
-        march_xy_face<SOLVER, ALB, STAGE>(g, s, m, tid, ox, oy, st, gc);
-        march_xy_face<SOLVER, ALB, STAGE>(g, s, m, tid + MTHREADS, ox, oy, st, gc);
-        if (tid < NXF + NYF - 2 * MTHREADS)
-            march_xy_face<SOLVER, ALB, STAGE>(g, s, m, tid + 2 * MTHREADS, ox, oy, st, gc);
         __syncthreads();
         if (col) {
             // += (A/V) F_L, -= (A/V) F_R per dimension (Hydro.cpp:117-137),
@@ -375,12 +458,12 @@ __device__ __forceinline__ void march_column(const HydroGeom& g, MarchLDS& s, co
 #pragma unroll
             for (int v = 0; v < 7; ++v) {
                 double u = 0.0;
-                u += g.kflux * (double)s.fx[v][fxl];
-                u -= g.kflux * (double)s.fx[v][fxl + 1];
-                u += g.kflux * (double)s.fy[v][fyl];
-                u -= g.kflux * (double)s.fy[v][fyl + MTX];
+                u += g.kflux * (double)s.sf.f.x[v][fxl];
+                u -= g.kflux * (double)s.sf.f.x[v][fxl + 1];
+                u += g.kflux * (double)s.sf.f.y[v][fyl];
+                u -= g.kflux * (double)s.sf.f.y[v][fyl + MTX];
                 u += g.kflux * (double)fz[v];
-                u -= g.kflux * (double)s.fz[v][tid];
+                u -= g.kflux * (double)fzn[v];
                 udot[v] = ldexp(u, -ke[v]);
             }
             const long long c = (long long)gx + (long long)g.nx * ((long long)gy + (long long)g.ny * k);
@@ -403,7 +486,7 @@ __device__ __forceinline__ void march_column(const HydroGeom& g, MarchLDS& s, co
         }
 #pragma unroll
         for (int v = 0; v < 7; ++v) {
-            fz[v] = s.fz[v][tid];
+            fz[v] = fzn[v];
             w1[v] = w2[v];
             w2[v] = w3[v];
         }
