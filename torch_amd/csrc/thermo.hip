@@ -336,20 +336,13 @@ __device__ __forceinline__ double grid_dt(const GridSrc& gs) {
 }
 
 __device__ __forceinline__ void gather_cell(const GridSrc& gs, long long c, double dt, double in[9],
-                                            double u[7]) {
+                                            double u[7], double q[7]) {
     const HydroGeom& g = gs.g;
     const long long n = g.ncell;
-    double q[7];
 #pragma unroll
     for (int v = 0; v < 7; ++v) u[v] = gs.U[v * n + c];
     QfromU(q, u, g.gm1);
-    q[HII] = smax_d(smin_d(q[HII], 1.0), 0.0);
-    q[ADV] = smax_d(smin_d(q[ADV], 1.0), 0.0);
-    q[DEN] = smax_d(q[DEN], g.dfloor);
-    q[PRE] = smax_d(q[PRE], g.pfloor);
-    const double mu_inv = g.X * (q[HII] + 1.0) + g.mu_c;
-    const double temperature = q[PRE] / (mu_inv * g.Rgas * q[DEN]);
-    if (temperature < g.tfloor) q[PRE] = mu_inv * g.Rgas * u[DEN] * g.tfloor;
+    fix_primitives(g, q, u[DEN]);
     bool skip = c == gs.excluded;
     double f = gs.F ? gs.F[c] : 0.0;
     if (gs.star.on) {
@@ -423,11 +416,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GRID ? 2 : 
         CellState s;
         double u[8];
         double Tc = 0.0;
-        double lin[9], lu[7];
+        double lin[9], lu[7], lq[7];
         const double* cin = in;  // this cell's inputs: cin[k * cstride + ccell]
         long long cstride = n, ccell = cell;
         if (GRID && valid) {
-            gather_cell(gs, cell, gdt, lin, lu);
+            gather_cell(gs, cell, gdt, lin, lu, lq);
             cin = lin;
             cstride = 1;
             ccell = 0;
@@ -594,8 +587,8 @@ __global__ void thermo_gather_kernel(GridSrc gs, double* __restrict__ in9) {
     const long long n = gs.g.ncell;
     const double dt = grid_dt(gs);
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
-        double in[9], u[7];
-        gather_cell(gs, c, dt, in, u);
+        double in[9], u[7], q[7];
+        gather_cell(gs, c, dt, in, u, q);
 #pragma unroll
         for (int k = 0; k < 9; ++k) in9[k * n + c] = in[k];
     }
@@ -604,28 +597,37 @@ __global__ void thermo_gather_kernel(GridSrc gs, double* __restrict__ in9) {
 // Thermodynamics::calculateTimeStep (Thermodynamics.cpp:463-476): min over
 // cells with RATE != 0 of |f U[PRE] / RATE|, f = 1 when subcycling else 0.1,
 // from the pre-step RATE; reduced as the bit pattern of a non-negative double.
-// GRID: the inputs from the grid state (gs) in registers, no gathered array.
+// GRID: the inputs from the grid state (gs) in registers, no gathered array;
+// with cfl_st, also the hydro CFL maximum of the same state into
+// cfl_st->maxinv (cfl_kernel's: the same fixPrimitives(QfromU(U)) per cell,
+// and a maximum, which any order of reduction gives bitwise).
 template <bool GRID>
 __global__ __launch_bounds__(256) void thermo_timestep_kernel(ThermoConsts t, const double* __restrict__ tables,
                                                              long long n, const double* __restrict__ in9,
                                                              unsigned long long* __restrict__ dtmin_bits,
-                                                             unsigned int* __restrict__ err, GridSrc gs) {
+                                                             unsigned int* __restrict__ err, GridSrc gs,
+                                                             StepState* cfl_st) {
     __shared__ double sx[26], sy[26], sy2[26];
-    __shared__ double red[4];
+    __shared__ double red[4], redc[4];
     load_tables(tables, sx, sy, sy2);
     double m = __longlong_as_double(0x7fefffffffffffffll);
+    double mc = 0.0;
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
         CellState s;
         ThermoConsts t1 = t;
         t1.subcycling = 0;  // pre-step rate only
-        double lin[9], lu[7];
+        double lin[9], lu[7], lq[7];
         const double* cin = in9;
         long long cstride = n, ccell = c;
         if (GRID) {
-            gather_cell(gs, c, 0.0, lin, lu);
+            gather_cell(gs, c, 0.0, lin, lu, lq);
             cin = lin;
             cstride = 1;
             ccell = 0;
+            if (cfl_st) {
+                const double it = cfl_inv_t(gs.g, lq);
+                if (it >= 0.0) mc = fmax(mc, it);
+            }
         }
         (void)thermo_init(t1, sx, sy, sy2, cin, cstride, ccell, s, err);
         if (s.rate_pre != 0) {
@@ -635,12 +637,22 @@ __global__ __launch_bounds__(256) void thermo_timestep_kernel(ThermoConsts t, co
         }
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off, 64));
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    for (int off = 32; off > 0; off >>= 1) {
+        m = fmin(m, __shfl_xor(m, off, 64));
+        mc = fmax(mc, __shfl_xor(mc, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6] = m;
+        redc[threadIdx.x >> 6] = mc;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         double b = fmin(fmin(red[0], red[1]), fmin(red[2], red[3]));
         if (b >= 0.0) atomicMin(dtmin_bits, (unsigned long long)__double_as_longlong(b));
+        if (GRID && cfl_st) {
+            const double bc = fmax(fmax(redc[0], redc[1]), fmax(redc[2], redc[3]));
+            atomicMax(&cfl_st->maxinv, (unsigned long long)__double_as_longlong(bc));
+        }
     }
 }
 
@@ -1358,8 +1370,10 @@ static int gather(torchgpu_cooling* c, torchgpu_ctx* ctx, const double* T_min, c
 
 // Thermodynamics::calculateTimeStep on the grid into c->dtbits (bit pattern
 // of min |f E/RATE| over the gated cells, 0x7fef... when none), no host sync.
+// The thermo time step of the grid state; with cfl_st, the kernel also
+// reduces the hydro CFL maximum of the same state (launch_cfl's).
 static int grid_timestep_dev(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* T_min, const double* col_den,
-                             const double* F, int64_t excluded) {
+                             const double* F, int64_t excluded, tg::StepState* cfl_st = nullptr) {
     // the inputs are formed in the kernel
     tg::GridSrc gs;
     int rc = grid_src(c, ctx, T_min, col_den, F, excluded, 0.0, gs);
@@ -1369,7 +1383,7 @@ static int grid_timestep_dev(torchgpu_ctx* ctx, torchgpu_cooling* c, const doubl
     long long nb = (n + 255) / 256;
     if (nb > (long long)c->nsm * 16) nb = (long long)c->nsm * 16;
     hipLaunchKernelGGL(tg::thermo_timestep_kernel<true>, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t,
-                       c->tables, n, (const double*)nullptr, c->dtbits, c->err, gs);
+                       c->tables, n, (const double*)nullptr, c->dtbits, c->err, gs, cfl_st);
     HIP_OK(hipGetLastError());
     return TORCHGPU_OK;
 }
@@ -1657,9 +1671,11 @@ int torchgpu_run_full(torchgpu_ctx* ctx, torchgpu_cooling* c, torchgpu_radiation
         v = tg::ctx_view(ctx);  // U and U* swap roles after every hydro sub-step
         // Torch::fullStep's preTimeStepCalculations and calculateTimeStep
         // (Torch.cpp:304-339,388-397) on the device: the hydro CFL maximum,
-        // the thermo and radiation minima, then dt in step_begin
-        HIP_OK(tg::launch_cfl(*v.g, v.U, v.st, v.stream));
-        if (c && (rc = grid_timestep_dev(ctx, c, T_min, col_den, F, excluded))) return rc;
+        // the thermo and radiation minima, then dt in step_begin; with the
+        // thermo component the CFL maximum comes from the thermo time step
+        // kernel, which reads the same state
+        if (!c) HIP_OK(tg::launch_cfl(*v.g, v.U, v.st, v.stream));
+        if (c && (rc = grid_timestep_dev(ctx, c, T_min, col_den, F, excluded, v.st))) return rc;
         if (r && ((rc = tg::rad_pre_step_dev(r, 1)) || (rc = tg::rad_timestep_dev(r, 0.0, 1)))) return rc;
         HIP_OK(tg::launch_step_begin(v.st, 1, 0.0, v.stream, c ? c->dtbits : nullptr,
                                      r ? tg::rad_dt_bits(r) : nullptr));
