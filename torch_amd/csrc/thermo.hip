@@ -496,8 +496,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GRID ? 2 : 
 // have run; the prepass updated the rest.  cerr: the cooling operator's
 // error word; a subcycle count beyond the int range is moved into the step
 // state (reported at this step, as the fixSolution errors) and cleared.
+// counters: the surrogate step's, [2] network and [3] fallback queue
+// lengths; with both queues empty there is no cell to update.
 __global__ void queue_apply_kernel(GridSrc gs, const double* __restrict__ out, const unsigned char* __restrict__ path,
-                                   double* __restrict__ U, float* __restrict__ qf, StepState* st, unsigned int* cerr) {
+                                   double* __restrict__ U, float* __restrict__ qf, StepState* st, unsigned int* cerr,
+                                   const unsigned long long* __restrict__ counters) {
     const bool halted = gs.dt < 0.0 && gs.st->halted;
     if (!halted && cerr && blockIdx.x == 0 && threadIdx.x == 0) {
         const unsigned int e = *cerr;
@@ -506,6 +509,7 @@ __global__ void queue_apply_kernel(GridSrc gs, const double* __restrict__ out, c
             *cerr = 0u;
         }
     }
+    if (counters[2] == 0 && counters[3] == 0) return;
     const double dtf = grid_dt(gs);
     const long long n = gs.g.ncell;
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
@@ -1410,7 +1414,7 @@ static int grid_substep_dev(torchgpu_ctx* ctx, torchgpu_cooling* c, const double
         long long nb = (n + 255) / 256;
         if (nb > 65536) nb = 65536;
         hipLaunchKernelGGL(tg::queue_apply_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, gs, c->out5, c->gpath,
-                           v.U, qf, v.st, c->err);
+                           v.U, qf, v.st, c->err, (const unsigned long long*)c->counters);
         HIP_OK(hipGetLastError());
         if (qf) tg::ctx_qf_written(ctx);
         else tg::ctx_state_changed(ctx);
