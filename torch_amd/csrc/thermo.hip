@@ -383,6 +383,73 @@ __device__ __forceinline__ void apply_cell(const HydroGeom& g, double u[7], doub
     if (qf) store_q_float(g, u, qf, c);
 }
 
+// A cell whose subcycle is longer than the exact limit: the coarse end
+// temperature, its raw features and the training-envelope test, which
+// route it to the network (to_mlp) or the FP64 fallback (to_fb).
+__device__ __forceinline__ void route_active(const ThermoConsts& t, const double* sx, const double* sy,
+                                             const double* sy2, const SurrogateConsts& sc, const double* cin,
+                                             long long cstride, long long ccell, const CellState& s, double& Tc,
+                                             double u[8], bool& to_mlp, bool& to_fb) {
+    Tc = coarse_end_temperature(t, sx, sy, sy2, s);
+    raw_features(t, cin, cstride, ccell, s, Tc, u);
+    // the bounds are [min, max] of the training set's features, which were
+    // stored as floats: compare the float-rounded features, so that a cell
+    // equal to a training sample is in
+    bool inside = true;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+        const float uf = (float)u[i];
+        inside &= (uf >= (float)sc.lo[i]) && (uf <= (float)sc.hi[i]);
+    }
+    to_mlp = inside || sc.train;
+    to_fb = !to_mlp;
+}
+
+// Queues a routed cell: network features and the state its finish needs,
+// or the fallback list; path 1 / 2.  Every lane of the wave calls it (the
+// appends are wave-aggregated).
+__device__ __forceinline__ void queue_cell(bool to_mlp, bool to_fb, long long cell, const double u[8],
+                                           const CellState& s, double Tc, const SurrogateConsts& sc,
+                                           unsigned char* path, long long* __restrict__ mlp_list,
+                                           float* __restrict__ feat, SurrCell* __restrict__ sdata,
+                                           float* __restrict__ raw_out, long long* __restrict__ fb_list,
+                                           unsigned long long* __restrict__ counters) {
+    const long long slot = wave_append(to_mlp, &counters[2]);
+    const long long fslot = wave_append(to_fb, &counters[3]);
+    if (to_mlp) {
+        mlp_list[slot] = cell;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) feat[slot * 8 + i] = (float)(u[i] - (double)sc.mu[i]) * sc.isd[i];
+        if (raw_out) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) raw_out[slot * 8 + i] = (float)u[i];
+        }
+        SurrCell d;
+        d.T0 = s.T0;
+        d.pre = s.pre;
+        d.temp2pre = s.temp2pre;
+        d.pf = s.pf;
+        d.dpre2rate = s.dpre2rate;
+        d.T_min = s.T_min;
+        d.heat = s.heat;
+        d.rate_pre = s.rate_pre;
+        d.Tc = Tc;
+        d.sigma = residual_scale(s.remaining + 1);
+        sdata[slot] = d;
+        if (path) path[cell] = 1;
+    }
+    if (to_fb) {
+        fb_list[fslot] = cell;
+        if (path) path[cell] = 2;
+    }
+}
+
+// Cells of more remaining subcycles than this have their coarse estimate
+// (up to COARSE_STEPS iterations of two rate evaluations) deferred from the
+// prepass to surrogate_long_kernel, where every lane of a wave holds such a
+// cell: inline, one of them in a wave keeps its other 63 lanes idle.
+constexpr int COARSE_DEFER = 64;
+
 // counters: [2] surrogate queue length, [3] fallback queue length.
 // GRID: the inputs of each cell are formed from the grid state (gs, as
 // thermo_gather_kernel) in registers, and written to `in` only for the
@@ -403,7 +470,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GRID ? 2 : 
                                                                unsigned long long* __restrict__ counters,
                                                                unsigned int* __restrict__ err, GridSrc gs,
                                                                double* Uw, float* __restrict__ qf,
-                                                               StepState* stw) {
+                                                               StepState* stw, long long* __restrict__ long_list) {
     __shared__ double sx[26], sy[26], sy2[26];
     load_tables(tables, sx, sy, sy2);
     const double gdt = GRID ? grid_dt(gs) : 0.0;
@@ -412,7 +479,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GRID ? 2 : 
     for (long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x; cell - threadIdx.x < nloop;
          cell += (long long)gridDim.x * blockDim.x) {
         const bool valid = cell < n;
-        bool to_mlp = false, to_fb = false;
+        bool to_mlp = false, to_fb = false, defer = false;
         CellState s;
         double u[8];
         double Tc = 0.0;
@@ -441,53 +508,51 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GRID ? 2 : 
                 }
                 if (path) path[cell] = 0;
             } else {
-                Tc = coarse_end_temperature(t, sx, sy, sy2, s);
-                raw_features(t, cin, cstride, ccell, s, Tc, u);
-                // the bounds are [min, max] of the training set's features,
-                // which were stored as floats: compare the float-rounded
-                // features, so that a cell equal to a training sample is in
-                bool inside = true;
-#pragma unroll
-                for (int i = 0; i < 7; ++i) {
-                    const float uf = (float)u[i];
-                    inside &= (uf >= (float)sc.lo[i]) && (uf <= (float)sc.hi[i]);
-                }
-                to_mlp = inside || sc.train;
-                to_fb = !to_mlp;
+                if (long_list && s.remaining > COARSE_DEFER) defer = true;
+                else route_active(t, sx, sy, sy2, sc, cin, cstride, ccell, s, Tc, u, to_mlp, to_fb);
                 if (GRID) {
 #pragma unroll
                     for (int k = 0; k < 9; ++k) in[k * n + cell] = lin[k];
                 }
             }
         }
-        const long long slot = wave_append(to_mlp, &counters[2]);
-        const long long fslot = wave_append(to_fb, &counters[3]);
-        if (to_mlp) {
-            mlp_list[slot] = cell;
-#pragma unroll
-            for (int i = 0; i < 8; ++i) feat[slot * 8 + i] = (float)(u[i] - (double)sc.mu[i]) * sc.isd[i];
-            if (raw_out) {
-#pragma unroll
-                for (int i = 0; i < 8; ++i) raw_out[slot * 8 + i] = (float)u[i];
-            }
-            SurrCell d;
-            d.T0 = s.T0;
-            d.pre = s.pre;
-            d.temp2pre = s.temp2pre;
-            d.pf = s.pf;
-            d.dpre2rate = s.dpre2rate;
-            d.T_min = s.T_min;
-            d.heat = s.heat;
-            d.rate_pre = s.rate_pre;
-            d.Tc = Tc;
-            d.sigma = residual_scale(s.remaining + 1);
-            sdata[slot] = d;
-            if (path) path[cell] = 1;
+        if (long_list) {
+            const long long lslot = wave_append(defer, &counters[4]);
+            if (defer) long_list[lslot] = cell;
         }
-        if (to_fb) {
-            fb_list[fslot] = cell;
-            if (path) path[cell] = 2;
+        queue_cell(to_mlp, to_fb, cell, u, s, Tc, sc, path, mlp_list, feat, sdata, raw_out, fb_list, counters);
+    }
+}
+
+// The deferred cells of the prepass (long_list, counters[4] of them): the
+// same routing from their inputs in `in` (column k of cell c at k n + c;
+// the grid prepass writes them there), after thermo_init again.  One wave
+// per workgroup, so that a few hundred waves of such cells spread over all
+// CUs, and two waves per SIMD, which leave the coarse loop's registers
+// unspilled.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void surrogate_long_kernel(
+    ThermoConsts t, const double* __restrict__ tables, SurrogateConsts sc, long long n, const double* __restrict__ in,
+    unsigned char* path, long long* __restrict__ mlp_list, float* __restrict__ feat, SurrCell* __restrict__ sdata,
+    long long* __restrict__ fb_list, unsigned long long* __restrict__ counters,
+    const long long* __restrict__ long_list, unsigned int* __restrict__ err) {
+    __shared__ double sx[26], sy[26], sy2[26];
+    const long long count = (long long)counters[4];
+    if ((long long)blockIdx.x * blockDim.x >= count) return;  // uniform per block, before the table load's barrier
+    load_tables(tables, sx, sy, sy2);
+    const long long nloop = ((count + blockDim.x - 1) / blockDim.x) * blockDim.x;  // uniform trip count per wave
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i - threadIdx.x < nloop;
+         i += (long long)gridDim.x * blockDim.x) {
+        bool to_mlp = false, to_fb = false;
+        CellState s;
+        double u[8];
+        double Tc = 0.0;
+        long long cell = 0;
+        if (i < count) {
+            cell = long_list[i];
+            (void)thermo_init(t, sx, sy, sy2, in, n, cell, s, err);
+            route_active(t, sx, sy, sy2, sc, in, n, cell, s, Tc, u, to_mlp, to_fb);
         }
+        queue_cell(to_mlp, to_fb, cell, u, s, Tc, sc, path, mlp_list, feat, sdata, nullptr, fb_list, counters);
     }
 }
 
@@ -841,7 +906,8 @@ struct torchgpu_cooling {
     tg::ThermoConsts t;
     hipStream_t stream = nullptr;
     double* tables = nullptr;                 // spline x, y, y2 (78 doubles)
-    unsigned long long* counters = nullptr;   // [0] work, [1] subcycles, [2] surrogate queue, [3] fallback queue
+    unsigned long long* counters = nullptr;   // [0] work, [1] subcycles, [2] surrogate queue, [3] fallback queue,
+                                              // [4] cells the surrogate prepass deferred
     unsigned int* err = nullptr;
     char msg[256] = {0};
     int nsm = 256;
@@ -852,6 +918,7 @@ struct torchgpu_cooling {
     long long cap = 0;                        // capacity of the queues below
     long long* mlp_list = nullptr;
     long long* fb_list = nullptr;
+    long long* long_list = nullptr;           // the surrogate prepass's deferred cells
     float* feat = nullptr;
     float* y = nullptr;
     tg::SurrCell* sdata = nullptr;
@@ -935,7 +1002,7 @@ tg::ThermoConsts to_consts(const torchgpu_thermo_params& p) {
 }
 
 int zero_counters(torchgpu_cooling* c) {
-    hipLaunchKernelGGL(tg::zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, c->counters, 4);
+    hipLaunchKernelGGL(tg::zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, c->counters, 5);
     HIP_OK(hipGetLastError());
     return TORCHGPU_OK;
 }
@@ -1026,15 +1093,17 @@ int ensure_queues(torchgpu_cooling* c, long long n) {
     (void)hipStreamSynchronize(c->stream);
     if (c->mlp_list) (void)hipFree(c->mlp_list);
     if (c->fb_list) (void)hipFree(c->fb_list);
+    if (c->long_list) (void)hipFree(c->long_list);
     if (c->feat) (void)hipFree(c->feat);
     if (c->y) (void)hipFree(c->y);
     if (c->sdata) (void)hipFree(c->sdata);
-    c->mlp_list = c->fb_list = nullptr;
+    c->mlp_list = c->fb_list = c->long_list = nullptr;
     c->feat = c->y = nullptr;
     c->sdata = nullptr;
     c->cap = 0;
     HIP_OK(hipMalloc(&c->mlp_list, n * sizeof(long long)));
     HIP_OK(hipMalloc(&c->fb_list, n * sizeof(long long)));
+    HIP_OK(hipMalloc(&c->long_list, n * sizeof(long long)));
     HIP_OK(hipMalloc(&c->feat, n * 8 * sizeof(float)));
     HIP_OK(hipMalloc(&c->y, n * 3 * sizeof(float)));
     HIP_OK(hipMalloc(&c->sdata, n * sizeof(tg::SurrCell)));
@@ -1083,7 +1152,7 @@ int torchgpu_cooling_create(const torchgpu_thermo_params* p, void* stream, torch
         return TORCHGPU_ERR_HIP;
     }
     unsigned int zero = 0;
-    if (hipMalloc(&c->tables, sizeof tab) != hipSuccess || hipMalloc(&c->counters, 4 * sizeof(unsigned long long)) != hipSuccess ||
+    if (hipMalloc(&c->tables, sizeof tab) != hipSuccess || hipMalloc(&c->counters, 5 * sizeof(unsigned long long)) != hipSuccess ||
         hipMalloc(&c->err, sizeof(unsigned int)) != hipSuccess ||
         hipMemcpy(c->tables, tab, sizeof tab, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->err, &zero, sizeof zero, hipMemcpyHostToDevice) != hipSuccess) {
@@ -1097,7 +1166,7 @@ int torchgpu_cooling_create(const torchgpu_thermo_params* p, void* stream, torch
 void torchgpu_cooling_destroy(torchgpu_cooling* c) {
     if (!c) return;
     (void)hipStreamSynchronize(c->stream);
-    for (void* ptr : {(void*)c->tables, (void*)c->counters, (void*)c->err, (void*)c->mlp_list, (void*)c->fb_list,
+    for (void* ptr : {(void*)c->tables, (void*)c->counters, (void*)c->err, (void*)c->mlp_list, (void*)c->fb_list, (void*)c->long_list,
                       (void*)c->feat, (void*)c->y, (void*)c->sdata, (void*)c->in9, (void*)c->out5, (void*)c->gpath,
                       (void*)c->dtbits,
                       (void*)c->lpt_order, (void*)c->lpt_bucket, (void*)c->lpt_hist, (void*)c->dcol,
@@ -1228,12 +1297,19 @@ static int surrogate_step(torchgpu_cooling* c, int64_t n, double* in, double* ou
     if (gs)
         hipLaunchKernelGGL(tg::surrogate_prepass_kernel<true>, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t,
                            c->tables, sc, (long long)n, in, out, path, c->mlp_list, c->feat, c->sdata, (float*)nullptr,
-                           c->fb_list, c->counters, c->err, *gs, apply->U, apply->qf, apply->st);
+                           c->fb_list, c->counters, c->err, *gs, apply->U, apply->qf, apply->st, c->long_list);
     else
         hipLaunchKernelGGL(tg::surrogate_prepass_kernel<false>, dim3((unsigned)nb), dim3(256), 0, c->stream,
                            c->t, c->tables, sc, (long long)n, in, out, path, c->mlp_list, c->feat, c->sdata,
                            (float*)nullptr, c->fb_list, c->counters, c->err, tg::GridSrc{}, (double*)nullptr,
-                           (float*)nullptr, (tg::StepState*)nullptr);
+                           (float*)nullptr, (tg::StepState*)nullptr, c->long_list);
+    HIP_OK(hipGetLastError());
+    // the cells the prepass deferred (many remaining subcycles), densely
+    long long nb64 = (n + 63) / 64;
+    if (nb64 > (long long)c->nsm * 64) nb64 = (long long)c->nsm * 64;
+    hipLaunchKernelGGL(tg::surrogate_long_kernel, dim3((unsigned)nb64), dim3(64), 0, c->stream, c->t, c->tables, sc,
+                       (long long)n, (const double*)in, path, c->mlp_list, c->feat, c->sdata, c->fb_list, c->counters,
+                       (const long long*)c->long_list, c->err);
     HIP_OK(hipGetLastError());
     HIP_OK(tg::launch_mlp_infer_dev(w16, b, c->feat, c->counters + 2, n, c->y, c->nsm, c->stream));
     hipLaunchKernelGGL(tg::surrogate_finish_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t, sc,
@@ -1288,7 +1364,7 @@ int torchgpu_surrogate_training_data(torchgpu_cooling* c, int64_t n, const doubl
     hipLaunchKernelGGL(tg::surrogate_prepass_kernel<false>, dim3((unsigned)nb), dim3(256), 0, c->stream,
                        c->t, c->tables, sc, (long long)n, const_cast<double*>(in), scratch, (unsigned char*)nullptr,
                        (long long*)list, feat, c->sdata, raw, c->fb_list, c->counters, c->err, tg::GridSrc{},
-                       (double*)nullptr, (float*)nullptr, (tg::StepState*)nullptr);
+                       (double*)nullptr, (float*)nullptr, (tg::StepState*)nullptr, (long long*)nullptr);
     keep(hipGetLastError());
     if (e == hipSuccess) {
         hipLaunchKernelGGL(tg::surrogate_targets_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, c->counters,
