@@ -46,6 +46,16 @@ __device__ __forceinline__ bf16_t to_bf16(float x) {
     return (bf16_t)(u >> 16);
 }
 __device__ __forceinline__ float from_bf16(bf16_t x) { return __uint_as_float((unsigned int)x << 16); }
+// Inference activation: SiLU z / (1 + e^-z) from the hardware base-2
+// exponential and reciprocal (about 1 ulp each, where the training kernels
+// use the correctly rounded expf and division), rounded to bfloat16 by the
+// hardware conversion (round to nearest even, as to_bf16).  The result
+// keeps 8 significant bits, so the two forms differ at most in its last bit.
+__device__ __forceinline__ bf16_t silu_bf16_fast(float z) {
+    const float e = __builtin_amdgcn_exp2f(-1.4426950408889634f * z);
+    const float a = z * __builtin_amdgcn_rcpf(1.0f + e);
+    return __builtin_bit_cast(bf16_t, (__bf16)a);
+}
 __device__ __forceinline__ bf16x8 as_bf16x8(u16x8 v) { return __builtin_bit_cast(bf16x8, v); }
 
 // ---------------------------------------------------------------------------
@@ -286,8 +296,8 @@ __device__ __forceinline__ void hidden_layer(const bf16_t (*w)[MLP_H], const flo
             u16x8 o;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                o[e] = to_bf16(silu_f(acc[0][nt][e] + bias[(2 * s) * 16 + 4 * q + e]));
-                o[4 + e] = to_bf16(silu_f(acc[1][nt][e] + bias[(2 * s + 1) * 16 + 4 * q + e]));
+                o[e] = silu_bf16_fast(acc[0][nt][e] + bias[(2 * s) * 16 + 4 * q + e]);
+                o[4 + e] = silu_bf16_fast(acc[1][nt][e] + bias[(2 * s + 1) * 16 + 4 * q + e]);
             }
             hout[s][nt] = as_bf16x8(o);
         }
@@ -361,8 +371,8 @@ __global__ __launch_bounds__(INFER_THREADS) void mlp_infer_kernel(const bf16_t* 
                 u16x8 o;
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    o[e] = to_bf16(silu_f(acc[0][nt][e] + s.b[0][(2 * s2) * 16 + 4 * q + e]));
-                    o[4 + e] = to_bf16(silu_f(acc[1][nt][e] + s.b[0][(2 * s2 + 1) * 16 + 4 * q + e]));
+                    o[e] = silu_bf16_fast(acc[0][nt][e] + s.b[0][(2 * s2) * 16 + 4 * q + e]);
+                    o[4 + e] = silu_bf16_fast(acc[1][nt][e] + s.b[0][(2 * s2 + 1) * 16 + 4 * q + e]);
                 }
                 ha[s2][nt] = as_bf16x8(o);
             }
