@@ -444,12 +444,6 @@ __device__ __forceinline__ void queue_cell(bool to_mlp, bool to_fb, long long ce
     }
 }
 
-// Cells of more remaining subcycles than this have their coarse estimate
-// (up to COARSE_STEPS iterations of two rate evaluations) deferred from the
-// prepass to surrogate_long_kernel, where every lane of a wave holds such a
-// cell: inline, one of them in a wave keeps its other 63 lanes idle.
-constexpr int COARSE_DEFER = 64;
-
 // counters: [2] surrogate queue length, [3] fallback queue length.
 // GRID: the inputs of each cell are formed from the grid state (gs, as
 // thermo_gather_kernel) in registers, and written to `in` only for the
@@ -457,10 +451,18 @@ constexpr int COARSE_DEFER = 64;
 // A GRID cell finished here is also updated in the grid state (apply_cell
 // into Uw / qf, dt from gs), and its `out` row is not written; the queued
 // cells are updated by queue_apply_kernel once their rate is known (path
-// marks them).  The GRID instance runs at two waves per SIMD: at four it
-// spills ~150 registers to scratch and is slower; the other at four.
-template <bool GRID>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GRID ? 2 : 4))) void surrogate_prepass_kernel(ThermoConsts t, const double* __restrict__ tables,
+// marks them).
+// DEFER (long_list given): every cell routed to the network or the fallback
+// is queued in long_list instead, for surrogate_long_kernel, which estimates
+// its coarse end temperature (up to COARSE_STEPS iterations of two rate
+// evaluations) with every lane of a wave on such a cell; inline, one long
+// cell in a wave keeps its other 63 lanes idle, and the estimate's registers
+// would bound this kernel's occupancy.  Without DEFER (the training data)
+// the routing is done here.  The GRID instance (which also forms the inputs
+// and applies the update) runs at three waves per SIMD with a few spilled
+// registers: measured faster than two without spills and four with ~50.
+template <bool GRID, bool DEFER>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GRID ? 3 : 4))) void surrogate_prepass_kernel(ThermoConsts t, const double* __restrict__ tables,
                                                                SurrogateConsts sc, long long n,
                                                                double* __restrict__ in, double* __restrict__ out,
                                                                unsigned char* path, long long* __restrict__ mlp_list,
@@ -508,7 +510,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GRID ? 2 : 
                 }
                 if (path) path[cell] = 0;
             } else {
-                if (long_list && s.remaining > COARSE_DEFER) defer = true;
+                if (DEFER) defer = true;
                 else route_active(t, sx, sy, sy2, sc, cin, cstride, ccell, s, Tc, u, to_mlp, to_fb);
                 if (GRID) {
 #pragma unroll
@@ -516,11 +518,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GRID ? 2 : 
                 }
             }
         }
-        if (long_list) {
+        if (DEFER) {
             const long long lslot = wave_append(defer, &counters[4]);
             if (defer) long_list[lslot] = cell;
+        } else {
+            queue_cell(to_mlp, to_fb, cell, u, s, Tc, sc, path, mlp_list, feat, sdata, raw_out, fb_list, counters);
         }
-        queue_cell(to_mlp, to_fb, cell, u, s, Tc, sc, path, mlp_list, feat, sdata, raw_out, fb_list, counters);
     }
 }
 
@@ -1267,7 +1270,7 @@ int torchgpu_cooling_set_surrogate(torchgpu_cooling* c, torchgpu_mlp* m, const t
 }  // extern "C"
 
 // Where a grid surrogate step writes the cooling update of the cells its
-// prepass finishes (surrogate_prepass_kernel<true>): the state, its float Q
+// prepass finishes (surrogate_prepass_kernel<true, true>): the state, its float Q
 // (NULL for FP64 grids) and the step state.
 struct ApplyDst {
     double* U;
@@ -1295,11 +1298,11 @@ static int surrogate_step(torchgpu_cooling* c, int64_t n, double* in, double* ou
     long long nb = (n + 255) / 256;
     if (nb > (long long)c->nsm * 16) nb = (long long)c->nsm * 16;
     if (gs)
-        hipLaunchKernelGGL(tg::surrogate_prepass_kernel<true>, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t,
+        hipLaunchKernelGGL((tg::surrogate_prepass_kernel<true, true>), dim3((unsigned)nb), dim3(256), 0, c->stream, c->t,
                            c->tables, sc, (long long)n, in, out, path, c->mlp_list, c->feat, c->sdata, (float*)nullptr,
                            c->fb_list, c->counters, c->err, *gs, apply->U, apply->qf, apply->st, c->long_list);
     else
-        hipLaunchKernelGGL(tg::surrogate_prepass_kernel<false>, dim3((unsigned)nb), dim3(256), 0, c->stream,
+        hipLaunchKernelGGL((tg::surrogate_prepass_kernel<false, true>), dim3((unsigned)nb), dim3(256), 0, c->stream,
                            c->t, c->tables, sc, (long long)n, in, out, path, c->mlp_list, c->feat, c->sdata,
                            (float*)nullptr, c->fb_list, c->counters, c->err, tg::GridSrc{}, (double*)nullptr,
                            (float*)nullptr, (tg::StepState*)nullptr, c->long_list);
@@ -1361,7 +1364,7 @@ int torchgpu_surrogate_training_data(torchgpu_cooling* c, int64_t n, const doubl
     };
     long long nb = (n + 255) / 256;
     if (nb > (long long)c->nsm * 16) nb = (long long)c->nsm * 16;
-    hipLaunchKernelGGL(tg::surrogate_prepass_kernel<false>, dim3((unsigned)nb), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL((tg::surrogate_prepass_kernel<false, false>), dim3((unsigned)nb), dim3(256), 0, c->stream,
                        c->t, c->tables, sc, (long long)n, const_cast<double*>(in), scratch, (unsigned char*)nullptr,
                        (long long*)list, feat, c->sdata, raw, c->fb_list, c->counters, c->err, tg::GridSrc{},
                        (double*)nullptr, (float*)nullptr, (tg::StepState*)nullptr, (long long*)nullptr);
