@@ -452,6 +452,10 @@ __device__ __forceinline__ void queue_cell(bool to_mlp, bool to_fb, long long ce
 // into Uw / qf, dt from gs), and its `out` row is not written; the queued
 // cells are updated by queue_apply_kernel once their rate is known (path
 // marks them).
+// Deferred cells of more remaining subcycles than this go to the front of
+// the deferred list, the others to its back.
+constexpr int COARSE_LONG = 64;
+
 // DEFER (long_list given): every cell routed to the network or the fallback
 // is queued in long_list instead, for surrogate_long_kernel, which estimates
 // its coarse end temperature (up to COARSE_STEPS iterations of two rate
@@ -519,17 +523,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GRID ? 3 : 
             }
         }
         if (DEFER) {
-            const long long lslot = wave_append(defer, &counters[4]);
-            if (defer) long_list[lslot] = cell;
+            // cells of many remaining subcycles from the front of long_list,
+            // the others from its back, so that the former share waves
+            const bool lng = defer && s.remaining > COARSE_LONG;
+            const long long fslot = wave_append(lng, &counters[4]);
+            const long long bslot = wave_append(defer && !lng, &counters[5]);
+            if (lng) long_list[fslot] = cell;
+            else if (defer) long_list[n - 1 - bslot] = cell;
         } else {
             queue_cell(to_mlp, to_fb, cell, u, s, Tc, sc, path, mlp_list, feat, sdata, raw_out, fb_list, counters);
         }
     }
 }
 
-// The deferred cells of the prepass (long_list, counters[4] of them): the
-// same routing from their inputs in `in` (column k of cell c at k n + c;
-// the grid prepass writes them there), after thermo_init again.  One wave
+// The deferred cells of the prepass (long_list: counters[4] from its front,
+// counters[5] from its back, of n slots): the same routing from their
+// inputs in `in` (column k of cell c at k n + c; the grid prepass writes
+// them there), after thermo_init again.  One wave
 // per workgroup, so that a few hundred waves of such cells spread over all
 // CUs, and two waves per SIMD, which leave the coarse loop's registers
 // unspilled.
@@ -539,7 +549,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void su
     long long* __restrict__ fb_list, unsigned long long* __restrict__ counters,
     const long long* __restrict__ long_list, unsigned int* __restrict__ err) {
     __shared__ double sx[26], sy[26], sy2[26];
-    const long long count = (long long)counters[4];
+    const long long nfront = (long long)counters[4];
+    const long long count = nfront + (long long)counters[5];
     if ((long long)blockIdx.x * blockDim.x >= count) return;  // uniform per block, before the table load's barrier
     load_tables(tables, sx, sy, sy2);
     const long long nloop = ((count + blockDim.x - 1) / blockDim.x) * blockDim.x;  // uniform trip count per wave
@@ -551,7 +562,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void su
         double Tc = 0.0;
         long long cell = 0;
         if (i < count) {
-            cell = long_list[i];
+            cell = i < nfront ? long_list[i] : long_list[n - 1 - (i - nfront)];
             (void)thermo_init(t, sx, sy, sy2, in, n, cell, s, err);
             route_active(t, sx, sy, sy2, sc, in, n, cell, s, Tc, u, to_mlp, to_fb);
         }
@@ -910,7 +921,7 @@ struct torchgpu_cooling {
     hipStream_t stream = nullptr;
     double* tables = nullptr;                 // spline x, y, y2 (78 doubles)
     unsigned long long* counters = nullptr;   // [0] work, [1] subcycles, [2] surrogate queue, [3] fallback queue,
-                                              // [4] cells the surrogate prepass deferred
+                                              // [4], [5] cells the surrogate prepass deferred (long, short)
     unsigned int* err = nullptr;
     char msg[256] = {0};
     int nsm = 256;
@@ -1005,7 +1016,7 @@ tg::ThermoConsts to_consts(const torchgpu_thermo_params& p) {
 }
 
 int zero_counters(torchgpu_cooling* c) {
-    hipLaunchKernelGGL(tg::zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, c->counters, 5);
+    hipLaunchKernelGGL(tg::zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, c->counters, 6);
     HIP_OK(hipGetLastError());
     return TORCHGPU_OK;
 }
@@ -1155,7 +1166,7 @@ int torchgpu_cooling_create(const torchgpu_thermo_params* p, void* stream, torch
         return TORCHGPU_ERR_HIP;
     }
     unsigned int zero = 0;
-    if (hipMalloc(&c->tables, sizeof tab) != hipSuccess || hipMalloc(&c->counters, 5 * sizeof(unsigned long long)) != hipSuccess ||
+    if (hipMalloc(&c->tables, sizeof tab) != hipSuccess || hipMalloc(&c->counters, 6 * sizeof(unsigned long long)) != hipSuccess ||
         hipMalloc(&c->err, sizeof(unsigned int)) != hipSuccess ||
         hipMemcpy(c->tables, tab, sizeof tab, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->err, &zero, sizeof zero, hipMemcpyHostToDevice) != hipSuccess) {
