@@ -57,10 +57,14 @@ struct CellState {
 
 // Thermodynamics::preTimeStepCalculations for one non-wind cell followed by
 // the set-up of integrate (:266-311).  Returns false when the cell is done
-// (gate, no subcycling, or a single step).
+// (gate, no subcycling, or a single step).  pre_rate (optional): the
+// cell's heating rate and pre-step RATE as this function computed them on
+// the same state (pre_rate[pidx], pre_rate[pn + pidx]; thermo_timestep_kernel
+// stores them), taken instead of evaluating them again.
 __device__ __forceinline__ bool thermo_init(const ThermoConsts& t, const double* sx, const double* sy,
                                             const double* sy2, const double* in, long long stride, long long cell,
-                                            CellState& s, unsigned int* err) {
+                                            CellState& s, unsigned int* err, const double* pre_rate = nullptr,
+                                            long long pidx = 0, long long pn = 0) {
     const double den = in[0 * stride + cell], pre = in[1 * stride + cell], hii = in[2 * stride + cell];
     const double adv = in[3 * stride + cell], E = in[4 * stride + cell], T_min = in[5 * stride + cell];
     const double col_den = in[6 * stride + cell], F = in[7 * stride + cell], dt = in[8 * stride + cell];
@@ -81,11 +85,17 @@ __device__ __forceinline__ bool thermo_init(const ThermoConsts& t, const double*
     const double Av = 1.086 * t.dust_xsec * col_den;
     s.T0 = T;
     s.cc = cooling_cell(t, nH, x);
-    double rate = heating(t, nH, Av, F);
-    s.heat = rate;
-    rate = subtract_cooling(t, s.cc, sx, sy, sy2, rate, T);
-    rate = soft_landing(rate, T, T_min);
-    const double RATE = t.heating_amplification * rate;
+    double RATE;
+    if (pre_rate) {
+        s.heat = pre_rate[pidx];
+        RATE = pre_rate[pn + pidx];
+    } else {
+        double rate = heating(t, nH, Av, F);
+        s.heat = rate;
+        rate = subtract_cooling(t, s.cc, sx, sy, sy2, rate, T);
+        rate = soft_landing(rate, T, T_min);
+        RATE = t.heating_amplification * rate;
+    }
     s.rate_pre = RATE;
     if (!t.subcycling) return false;
     double dti = fabs(0.10 * E / RATE);
@@ -476,7 +486,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GRID ? 3 : 
                                                                unsigned long long* __restrict__ counters,
                                                                unsigned int* __restrict__ err, GridSrc gs,
                                                                double* Uw, float* __restrict__ qf,
-                                                               StepState* stw, long long* __restrict__ long_list) {
+                                                               StepState* stw, long long* __restrict__ long_list,
+                                                               const double* __restrict__ pre_rate) {
     __shared__ double sx[26], sy[26], sy2[26];
     load_tables(tables, sx, sy, sy2);
     const double gdt = GRID ? grid_dt(gs) : 0.0;
@@ -499,7 +510,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GRID ? 3 : 
             ccell = 0;
         }
         if (valid) {
-            bool active = thermo_init(t, sx, sy, sy2, cin, cstride, ccell, s, err);
+            bool active = thermo_init(t, sx, sy, sy2, cin, cstride, ccell, s, err, GRID ? pre_rate : nullptr, cell, n);
             if (active && s.remaining < sc.nexact) {
                 while (s.remaining > 0) thermo_substep(t, sx, sy, sy2, s);
                 active = false;
@@ -683,13 +694,15 @@ __global__ void thermo_gather_kernel(GridSrc gs, double* __restrict__ in9) {
 // GRID: the inputs from the grid state (gs) in registers, no gathered array;
 // with cfl_st, also the hydro CFL maximum of the same state into
 // cfl_st->maxinv (cfl_kernel's: the same fixPrimitives(QfromU(U)) per cell,
-// and a maximum, which any order of reduction gives bitwise).
+// and a maximum, which any order of reduction gives bitwise); with rate_out,
+// each cell's heating rate and pre-step RATE ([2][n]) for a cooling sub-step
+// on the same state (thermo_init's pre_rate).
 template <bool GRID>
 __global__ __launch_bounds__(256) void thermo_timestep_kernel(ThermoConsts t, const double* __restrict__ tables,
                                                              long long n, const double* __restrict__ in9,
                                                              unsigned long long* __restrict__ dtmin_bits,
                                                              unsigned int* __restrict__ err, GridSrc gs,
-                                                             StepState* cfl_st) {
+                                                             StepState* cfl_st, double* __restrict__ rate_out) {
     __shared__ double sx[26], sy[26], sy2[26];
     __shared__ double red[4], redc[4];
     load_tables(tables, sx, sy, sy2);
@@ -713,6 +726,10 @@ __global__ __launch_bounds__(256) void thermo_timestep_kernel(ThermoConsts t, co
             }
         }
         (void)thermo_init(t1, sx, sy, sy2, cin, cstride, ccell, s, err);
+        if (GRID && rate_out) {
+            rate_out[c] = s.heat;
+            rate_out[n + c] = s.rate_pre;
+        }
         if (s.rate_pre != 0) {
             const double frac = t.subcycling ? 1.0 : 0.1;
             const double dti = fabs(frac * cin[4 * cstride + ccell] / s.rate_pre);
@@ -946,6 +963,7 @@ struct torchgpu_cooling {
     double* in9 = nullptr;
     double* out5 = nullptr;
     unsigned char* gpath = nullptr;  // per grid cell: the surrogate step's path (0 exact, 1 network, 2 fallback)
+    double* rcache = nullptr;        // [2][n]: heating rate and pre-step RATE of the thermo time step's state
     unsigned long long* dtbits = nullptr;
     // star (torchgpu_cooling_set_star): geometry and the DCOL_DEN of the
     // ray trace (COL_DEN goes straight into in9)
@@ -1181,7 +1199,7 @@ void torchgpu_cooling_destroy(torchgpu_cooling* c) {
     if (!c) return;
     (void)hipStreamSynchronize(c->stream);
     for (void* ptr : {(void*)c->tables, (void*)c->counters, (void*)c->err, (void*)c->mlp_list, (void*)c->fb_list, (void*)c->long_list,
-                      (void*)c->feat, (void*)c->y, (void*)c->sdata, (void*)c->in9, (void*)c->out5, (void*)c->gpath,
+                      (void*)c->feat, (void*)c->y, (void*)c->sdata, (void*)c->in9, (void*)c->out5, (void*)c->gpath, (void*)c->rcache,
                       (void*)c->dtbits,
                       (void*)c->lpt_order, (void*)c->lpt_bucket, (void*)c->lpt_hist, (void*)c->dcol,
                       (void*)c->col, (void*)c->xface, (void*)c->tmin})
@@ -1287,6 +1305,7 @@ struct ApplyDst {
     double* U;
     float* qf;
     tg::StepState* st;
+    const double* pre_rate;  // heating rate and pre-step RATE of each cell on this state, or NULL
 };
 
 // The surrogate step on `in` (gs and apply NULL), or on the grid state (gs
@@ -1311,12 +1330,13 @@ static int surrogate_step(torchgpu_cooling* c, int64_t n, double* in, double* ou
     if (gs)
         hipLaunchKernelGGL((tg::surrogate_prepass_kernel<true, true>), dim3((unsigned)nb), dim3(256), 0, c->stream, c->t,
                            c->tables, sc, (long long)n, in, out, path, c->mlp_list, c->feat, c->sdata, (float*)nullptr,
-                           c->fb_list, c->counters, c->err, *gs, apply->U, apply->qf, apply->st, c->long_list);
+                           c->fb_list, c->counters, c->err, *gs, apply->U, apply->qf, apply->st, c->long_list,
+                           apply->pre_rate);
     else
         hipLaunchKernelGGL((tg::surrogate_prepass_kernel<false, true>), dim3((unsigned)nb), dim3(256), 0, c->stream,
                            c->t, c->tables, sc, (long long)n, in, out, path, c->mlp_list, c->feat, c->sdata,
                            (float*)nullptr, c->fb_list, c->counters, c->err, tg::GridSrc{}, (double*)nullptr,
-                           (float*)nullptr, (tg::StepState*)nullptr, c->long_list);
+                           (float*)nullptr, (tg::StepState*)nullptr, c->long_list, (const double*)nullptr);
     HIP_OK(hipGetLastError());
     // the cells the prepass deferred (many remaining subcycles), densely
     long long nb64 = (n + 63) / 64;
@@ -1378,7 +1398,8 @@ int torchgpu_surrogate_training_data(torchgpu_cooling* c, int64_t n, const doubl
     hipLaunchKernelGGL((tg::surrogate_prepass_kernel<false, false>), dim3((unsigned)nb), dim3(256), 0, c->stream,
                        c->t, c->tables, sc, (long long)n, const_cast<double*>(in), scratch, (unsigned char*)nullptr,
                        (long long*)list, feat, c->sdata, raw, c->fb_list, c->counters, c->err, tg::GridSrc{},
-                       (double*)nullptr, (float*)nullptr, (tg::StepState*)nullptr, (long long*)nullptr);
+                       (double*)nullptr, (float*)nullptr, (tg::StepState*)nullptr, (long long*)nullptr,
+                       (const double*)nullptr);
     keep(hipGetLastError());
     if (e == hipSuccess) {
         hipLaunchKernelGGL(tg::surrogate_targets_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, c->counters,
@@ -1400,12 +1421,14 @@ static int ensure_grid(torchgpu_cooling* c, long long n) {
     if (c->in9) (void)hipFree(c->in9);
     if (c->out5) (void)hipFree(c->out5);
     if (c->gpath) (void)hipFree(c->gpath);
-    c->in9 = c->out5 = nullptr;
+    if (c->rcache) (void)hipFree(c->rcache);
+    c->in9 = c->out5 = c->rcache = nullptr;
     c->gpath = nullptr;
     c->grid_cap = 0;
     HIP_OK(hipMalloc(&c->in9, 9 * n * sizeof(double)));
     HIP_OK(hipMalloc(&c->out5, 5 * n * sizeof(double)));
     HIP_OK(hipMalloc(&c->gpath, n));
+    HIP_OK(hipMalloc(&c->rcache, 2 * n * sizeof(double)));
     // finite contents everywhere: the ray trace reads columns of zero weight
     hipLaunchKernelGGL(tg::zero_f64_kernel, dim3(1024), dim3(256), 0, c->stream, c->in9, 9 * n);
     HIP_OK(hipGetLastError());
@@ -1465,9 +1488,12 @@ static int gather(torchgpu_cooling* c, torchgpu_ctx* ctx, const double* T_min, c
 // Thermodynamics::calculateTimeStep on the grid into c->dtbits (bit pattern
 // of min |f E/RATE| over the gated cells, 0x7fef... when none), no host sync.
 // The thermo time step of the grid state; with cfl_st, the kernel also
-// reduces the hydro CFL maximum of the same state (launch_cfl's).
+// reduces the hydro CFL maximum of the same state (launch_cfl's); with
+// keep_rate, it keeps each cell's heating rate and pre-step RATE in
+// c->rcache for a cooling sub-step that follows on the same state.
 static int grid_timestep_dev(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* T_min, const double* col_den,
-                             const double* F, int64_t excluded, tg::StepState* cfl_st = nullptr) {
+                             const double* F, int64_t excluded, tg::StepState* cfl_st = nullptr,
+                             bool keep_rate = false) {
     // the inputs are formed in the kernel
     tg::GridSrc gs;
     int rc = grid_src(c, ctx, T_min, col_den, F, excluded, 0.0, gs);
@@ -1477,14 +1503,17 @@ static int grid_timestep_dev(torchgpu_ctx* ctx, torchgpu_cooling* c, const doubl
     long long nb = (n + 255) / 256;
     if (nb > (long long)c->nsm * 16) nb = (long long)c->nsm * 16;
     hipLaunchKernelGGL(tg::thermo_timestep_kernel<true>, dim3((unsigned)nb), dim3(256), 0, c->stream, c->t,
-                       c->tables, n, (const double*)nullptr, c->dtbits, c->err, gs, cfl_st);
+                       c->tables, n, (const double*)nullptr, c->dtbits, c->err, gs, cfl_st,
+                       keep_rate ? c->rcache : (double*)nullptr);
     HIP_OK(hipGetLastError());
     return TORCHGPU_OK;
 }
 
 // One thermo sub-step on the grid (dt explicit or a DT_* code), no host sync.
+// kept_rate: the state is the one grid_timestep_dev(keep_rate) last saw,
+// so the surrogate prepass takes its heating rates and pre-step RATEs.
 static int grid_substep_dev(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* T_min, const double* col_den,
-                            const double* F, int64_t excluded, double dt, int surrogate) {
+                            const double* F, int64_t excluded, double dt, int surrogate, bool kept_rate = false) {
     tg::CtxView v = tg::ctx_view(ctx);
     const long long n = v.g->ncell;
     // the surrogate step forms the inputs in its prepass unless the heating
@@ -1499,7 +1528,7 @@ static int grid_substep_dev(torchgpu_ctx* ctx, torchgpu_cooling* c, const double
         tg::GridSrc gs;
         if ((rc = grid_src(c, ctx, T_min, col_den, F, excluded, dt, gs))) return rc;
         float* qf = tg::ctx_qf_for_state(ctx);
-        const ApplyDst ad{v.U, qf, v.st};
+        const ApplyDst ad{v.U, qf, v.st, kept_rate ? c->rcache : nullptr};
         if ((rc = surrogate_step(c, n, c->in9, c->out5, c->gpath, nullptr, &gs, &ad))) return rc;
         long long nb = (n + 255) / 256;
         if (nb > 65536) nb = 65536;
@@ -1727,6 +1756,9 @@ int torchgpu_run_split(torchgpu_ctx* ctx, torchgpu_cooling* c, const double* T_m
     return torchgpu_run_full(ctx, c, nullptr, T_min, col_den, F, excluded, nsteps, surrogate);
 }
 
+// The component of a step's first sub-step: comps rotated by `launched`.
+static int order_first(const int* comps, int ncomps, int launched) { return comps[launched % ncomps]; }
+
 int torchgpu_run_full(torchgpu_ctx* ctx, torchgpu_cooling* c, torchgpu_radiation* r, const double* T_min,
                       const double* col_den, const double* F, int64_t excluded, int nsteps, int surrogate) {
     if (!ctx || nsteps < 0) return TORCHGPU_ERR_INVALID_ARG;
@@ -1769,7 +1801,12 @@ int torchgpu_run_full(torchgpu_ctx* ctx, torchgpu_cooling* c, torchgpu_radiation
         // thermo component the CFL maximum comes from the thermo time step
         // kernel, which reads the same state
         if (!c) HIP_OK(tg::launch_cfl(*v.g, v.U, v.st, v.stream));
-        if (c && (rc = grid_timestep_dev(ctx, c, T_min, col_den, F, excluded, v.st))) return rc;
+        // a step that opens with the cooling sub-step (every other one: the
+        // order turns each step) reuses the pre-step rates of the time step,
+        // which sees the same state (not with radiation, whose pre-step
+        // comes in between)
+        const bool share = c && !r && surrogate && order_first(comps, ncomps, (launched + 1) % ncomps) == THERMO;
+        if (c && (rc = grid_timestep_dev(ctx, c, T_min, col_den, F, excluded, v.st, share))) return rc;
         if (r && ((rc = tg::rad_pre_step_dev(r, 1)) || (rc = tg::rad_timestep_dev(r, 0.0, 1)))) return rc;
         HIP_OK(tg::launch_step_begin(v.st, 1, 0.0, v.stream, c ? c->dtbits : nullptr,
                                      r ? tg::rad_dt_bits(r) : nullptr));
@@ -1796,7 +1833,8 @@ int torchgpu_run_full(torchgpu_ctx* ctx, torchgpu_cooling* c, torchgpu_radiation
             if (order[k] == HYDRO) {
                 if ((rc = tg::ctx_run_stage(ctx, 0, code[k]))) return rc;
             } else if (order[k] == THERMO) {
-                if ((rc = grid_substep_dev(ctx, c, T_min, col_den, F, excluded, code[k], surrogate))) return rc;
+                if ((rc = grid_substep_dev(ctx, c, T_min, col_den, F, excluded, code[k], surrogate, share && k == 0)))
+                    return rc;
             } else {
                 if ((rc = tg::rad_substep_dev(r, code[k], k == 0))) return rc;
                 // Radiation::integrate leaves its HII fraction in Fluid::Q
