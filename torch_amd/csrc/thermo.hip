@@ -964,6 +964,7 @@ struct torchgpu_cooling {
     double* out5 = nullptr;
     unsigned char* gpath = nullptr;  // per grid cell: the surrogate step's path (0 exact, 1 network, 2 fallback)
     double* rcache = nullptr;        // [2][n]: heating rate and pre-step RATE of the thermo time step's state
+    long long rcache_cap = 0;        // (allocated by the first time step asked to keep them)
     unsigned long long* dtbits = nullptr;
     // star (torchgpu_cooling_set_star): geometry and the DCOL_DEN of the
     // ray trace (COL_DEN goes straight into in9)
@@ -1421,14 +1422,12 @@ static int ensure_grid(torchgpu_cooling* c, long long n) {
     if (c->in9) (void)hipFree(c->in9);
     if (c->out5) (void)hipFree(c->out5);
     if (c->gpath) (void)hipFree(c->gpath);
-    if (c->rcache) (void)hipFree(c->rcache);
-    c->in9 = c->out5 = c->rcache = nullptr;
+    c->in9 = c->out5 = nullptr;
     c->gpath = nullptr;
     c->grid_cap = 0;
     HIP_OK(hipMalloc(&c->in9, 9 * n * sizeof(double)));
     HIP_OK(hipMalloc(&c->out5, 5 * n * sizeof(double)));
     HIP_OK(hipMalloc(&c->gpath, n));
-    HIP_OK(hipMalloc(&c->rcache, 2 * n * sizeof(double)));
     // finite contents everywhere: the ray trace reads columns of zero weight
     hipLaunchKernelGGL(tg::zero_f64_kernel, dim3(1024), dim3(256), 0, c->stream, c->in9, 9 * n);
     HIP_OK(hipGetLastError());
@@ -1499,6 +1498,14 @@ static int grid_timestep_dev(torchgpu_ctx* ctx, torchgpu_cooling* c, const doubl
     int rc = grid_src(c, ctx, T_min, col_den, F, excluded, 0.0, gs);
     if (rc) return rc;
     const long long n = tg::ctx_view(ctx).g->ncell;
+    if (keep_rate && n > c->rcache_cap) {
+        (void)hipStreamSynchronize(c->stream);
+        if (c->rcache) (void)hipFree(c->rcache);
+        c->rcache = nullptr;
+        c->rcache_cap = 0;
+        HIP_OK(hipMalloc(&c->rcache, 2 * n * sizeof(double)));
+        c->rcache_cap = n;
+    }
     hipLaunchKernelGGL(tg::set_u64_kernel, dim3(1), dim3(1), 0, c->stream, c->dtbits, 0x7fefffffffffffffull);
     long long nb = (n + 255) / 256;
     if (nb > (long long)c->nsm * 16) nb = (long long)c->nsm * 16;
